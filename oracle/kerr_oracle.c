@@ -1,0 +1,796 @@
+/*
+ * kerr_oracle.c -- FP64 CPU oracle for the Kerr per-pixel photon trace.
+ *
+ * TEST INFRASTRUCTURE ONLY (see kerr_oracle.h): the checker for the HIP
+ * library, never part of it.  Build with -ffp-contract=off (oracle/Makefile).
+ */
+#include "kerr_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stddef.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------- */
+/* radii (src/kerr.cpp:157-223) and circular orbits (src/kerr.cpp:296-299)   */
+/* ------------------------------------------------------------------------- */
+
+static int naked(double a) { return fabs(a) > 1.0; }
+
+double ko_horizon_radius(double a) {
+    if (naked(a)) return 0.0;
+    return 1.0 + sqrt(fmax(0.0, 1.0 - a * a));
+}
+
+double ko_inner_horizon_radius(double a) {
+    if (naked(a)) return 0.0;
+    return 1.0 - sqrt(fmax(0.0, 1.0 - a * a));
+}
+
+/* Bardeen, Press & Teukolsky (1972) closed form, kerr.cpp:179-190 (|a| <= 1 only) */
+double ko_isco_radius(double a, int prograde) {
+    double z1 = 1.0 + cbrt(1.0 - a * a) * (cbrt(1.0 + a) + cbrt(1.0 - a));
+    double z2 = sqrt(3.0 * a * a + z1 * z1);
+    double root = sqrt(fmax(0.0, (3.0 - z1) * (3.0 + z1 + 2.0 * z2)));
+    return prograde ? 3.0 + z2 - root : 3.0 + z2 + root;
+}
+
+/* kerr.cpp:192-206 (|a| <= 1) */
+double ko_photon_orbit_radius(double a, int prograde) {
+    double s = prograde ? -1.0 : 1.0;
+    return 2.0 * (1.0 + cos(2.0 / 3.0 * acos(s * a)));
+}
+
+/* kerr.cpp:208-223 (|a| <= 1) */
+double ko_marginally_bound_radius(double a, int prograde) {
+    double s = prograde ? 1.0 : -1.0;
+    return 2.0 - s * a + 2.0 * sqrt(1.0 - s * a);
+}
+
+/* kerr.cpp:296-299 */
+double ko_circular_omega(double a, double r, int prograde) {
+    double s = prograde ? 1.0 : -1.0;
+    return s / (r * sqrt(r) + s * a);
+}
+
+/* ------------------------------------------------------------------------- */
+/* reference-form equations (src/kerr.cpp:323-363, 521-542)                   */
+/* ------------------------------------------------------------------------- */
+
+/* kerr.cpp:323-329 */
+double ko_inflow_factor(double P, double vr, double delta, double K) {
+    if (P * vr < 0.0) return K / (P - vr);
+    double d = fabs(delta) > 1e-12 ? delta : copysign(1e-12, delta);
+    return (P + vr) / d;
+}
+
+static double safe_sin_cubed(double s) {
+    double m = fmax(fabs(s), 1e-9);
+    return (s < 0.0 ? -1.0 : 1.0) * m * m * m;
+}
+
+/* kerr.cpp:331-351: Kerr-Schild t, phi; forward Mino time */
+ko_derivs ko_derivatives(double a, ko_consts c, ko_state s) {
+    const double r = s.r;
+    const double sn = sin(s.theta), cs = cos(s.theta);
+    const double sn2 = fmax(sn * sn, 1e-18);
+    const double a2 = a * a, mu2 = c.mu * c.mu;
+    const double delta = r * r - 2.0 * r + a2;
+    const double rra = r * r + a2;
+    const double P = c.E * rra - a * c.L;
+    const double x = c.L - a * c.E;
+    const double K = mu2 * r * r + x * x + c.Q;
+    const double inflow = ko_inflow_factor(P, s.vr, delta, K);
+    ko_derivs d;
+    d.dr = s.vr;
+    d.dtheta = s.vtheta;
+    d.dvr = 2.0 * c.E * r * P - (r - 1.0) * K - delta * mu2 * r;
+    d.dvtheta = sn * cs * a2 * (mu2 - c.E * c.E) + c.L * c.L * cs / safe_sin_cubed(sn);
+    d.dphi = a * inflow - a * c.E + c.L / sn2;
+    d.dt = rra * inflow - a * (a * c.E * sn * sn - c.L) - s.vr;
+    return d;
+}
+
+/* kerr.cpp:353-358 */
+double ko_radial_potential(double a, ko_consts c, double r) {
+    const double delta = r * r - 2.0 * r + a * a;
+    const double P = c.E * (r * r + a * a) - a * c.L;
+    const double x = c.L - a * c.E;
+    return P * P - delta * (c.mu * c.mu * r * r + x * x + c.Q);
+}
+
+/* kerr.cpp:360-363 */
+double ko_polar_potential(double a, ko_consts c, double theta) {
+    const double sn = sin(theta), cs = cos(theta);
+    return c.Q - cs * cs * (a * a * (c.mu * c.mu - c.E * c.E) + c.L * c.L / fmax(sn * sn, 1e-18));
+}
+
+/* kerr.cpp:521-542 (kTurningZone = 0.05) */
+void ko_enforce_constraints(double a, ko_consts c, ko_state* s) {
+    const double zone = 0.05;
+    const double r = s->r;
+    const double delta = r * r - 2.0 * r + a * a;
+    const double P = c.E * (r * r + a * a) - a * c.L;
+    const double x = c.L - a * c.E;
+    const double K = c.mu * c.mu * r * r + x * x + c.Q;
+    const double R = P * P - delta * K;
+    if (R > zone * (P * P + fabs(delta) * K)) s->vr = copysign(sqrt(R), s->vr);
+    const double sn = sin(s->theta), cs = cos(s->theta);
+    const double cot2 = cs * cs / fmax(sn * sn, 1e-18);
+    const double polarMass = a * a * (c.mu * c.mu - c.E * c.E) * cs * cs;
+    const double Theta = c.Q - polarMass - c.L * c.L * cot2;
+    if (Theta > zone * (fabs(c.Q) + fabs(polarMass) + c.L * c.L * cot2))
+        s->vtheta = copysign(sqrt(Theta), s->vtheta);
+}
+
+/* ------------------------------------------------------------------------- */
+/* metric, frames and photon launch (src/kerr.cpp:238-270, 424-483)          */
+/* ------------------------------------------------------------------------- */
+
+/* covariant Kerr-Schild metric, kerr.cpp:238-254 */
+void ko_metric_tensor(double a, double r, double theta, double g[4][4]) {
+    const double s = sin(theta), c = cos(theta);
+    const double s2 = s * s;
+    const double sigma = r * r + a * a * c * c;
+    const double rra = r * r + a * a;
+    const double A = rra * rra - a * a * (r * r - 2.0 * r + a * a) * s2;
+    const double f = 2.0 * r / sigma;
+    memset(g, 0, 16 * sizeof(double));
+    g[0][0] = -(1.0 - f);
+    g[0][1] = g[1][0] = f;
+    g[0][3] = g[3][0] = -f * a * s2;
+    g[1][1] = 1.0 + f;
+    g[1][3] = g[3][1] = -a * s2 * (1.0 + f);
+    g[2][2] = sigma;
+    g[3][3] = A * s2 / sigma;
+}
+
+static double dot4(const double g[4][4], const double x[4], const double y[4]) {
+    double sum = 0.0;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) sum += g[i][j] * x[i] * y[j];
+    return sum;
+}
+
+static void lower4(const double g[4][4], const double x[4], double out[4]) {
+    for (int i = 0; i < 4; ++i) {
+        out[i] = 0.0;
+        for (int j = 0; j < 4; ++j) out[i] += g[i][j] * x[j];
+    }
+}
+
+/* kerr.cpp:424-430 */
+void ko_zamo_four_velocity(double a, double r, double theta, double u[4]) {
+    const double s = sin(theta), c = cos(theta);
+    const double sigma = r * r + a * a * c * c;
+    const double delta = r * r - 2.0 * r + a * a;
+    const double rra = r * r + a * a;
+    const double A = rra * rra - a * a * delta * s * s;
+    const double omega = 2.0 * a * r / A;
+    const double ut = sqrt(A / (sigma * delta));
+    u[0] = ut;
+    u[1] = 0.0;
+    u[2] = 0.0;
+    u[3] = omega * ut;
+}
+
+/* Gram-Schmidt frame, kerr.cpp:432-456 */
+void ko_observer_frame(double a, double r, double theta, const double u[4], double e[4][4]) {
+    double g[4][4];
+    ko_metric_tensor(a, r, theta, g);
+    static const double cand[3][4] = {{0, 0, 1, 0}, {0, 0, 0, 1}, {0, 1, 0, 0}};
+    static const int slot[3] = {2, 3, 1};
+    for (int i = 0; i < 4; ++i) e[0][i] = u[i];
+    for (int k = 0; k < 3; ++k) {
+        double w[4];
+        for (int i = 0; i < 4; ++i) w[i] = cand[k][i];
+        const double along = dot4(g, w, u);
+        for (int i = 0; i < 4; ++i) w[i] += along * u[i];
+        for (int j = 0; j < k; ++j) {
+            const double* ej = e[slot[j]];
+            const double p = dot4(g, w, ej);
+            for (int i = 0; i < 4; ++i) w[i] -= p * ej[i];
+        }
+        const double n = sqrt(fmax(dot4(g, w, w), 1e-300));
+        for (int i = 0; i < 4; ++i) w[i] /= n;
+        for (int i = 0; i < 4; ++i) e[slot[k]][i] = w[i];
+    }
+}
+
+#define KO_AXIS_SNAP 3e-4 /* kerr.h:115 */
+
+/* kerr.cpp:458-483 (forward photon, Kerr-Schild state) */
+void ko_photon_from_frame(double a, double r, double theta, double phi, const double e[4][4],
+                          const double look[3], ko_consts* c, ko_state* s) {
+    double p[4], low[4], g[4][4];
+    for (int i = 0; i < 4; ++i) p[i] = e[0][i] - look[0] * e[1][i] - look[1] * e[2][i] - look[2] * e[3][i];
+    ko_metric_tensor(a, r, theta, g);
+    lower4(g, p, low);
+    const double sn = sin(theta), cs = cos(theta);
+    const double sigma = r * r + a * a * cs * cs;
+    c->mu = 0.0;
+    c->E = -low[0];
+    c->L = low[3];
+    s->t = 0.0;
+    s->r = r;
+    s->theta = theta;
+    s->phi = phi;
+    s->vr = sigma * p[1];
+    s->vtheta = low[2];
+    c->Q = s->vtheta * s->vtheta + cs * cs * (c->L * c->L / fmax(sn * sn, 1e-18) - a * a * c->E * c->E);
+    if (fabs(c->L) < KO_AXIS_SNAP * sqrt(fmax(c->Q + a * a * c->E * c->E, 0.0))) {
+        c->L = 0.0;
+        c->Q = s->vtheta * s->vtheta - cs * cs * a * a * c->E * c->E;
+    }
+}
+
+/* camera.cpp:6-21 in FP64 */
+void ko_camera_basis(double yaw, double pitch, double roll, double B[9]) {
+    const double f0[3] = {-1.0, 0.0, 0.0}, u0[3] = {0.0, -1.0, 0.0}, r0[3] = {0.0, 0.0, 1.0};
+    const double cy = cos(yaw), sy = sin(yaw), cp = cos(pitch), sp = sin(pitch);
+    const double cr = cos(roll), sr = sin(roll);
+    double yawed[3], right[3], fwd[3], up[3];
+    for (int i = 0; i < 3; ++i) {
+        yawed[i] = cy * f0[i] + sy * r0[i];
+        right[i] = cy * r0[i] - sy * f0[i];
+    }
+    for (int i = 0; i < 3; ++i) fwd[i] = cp * yawed[i] + sp * u0[i];
+    up[0] = right[1] * fwd[2] - right[2] * fwd[1];
+    up[1] = right[2] * fwd[0] - right[0] * fwd[2];
+    up[2] = right[0] * fwd[1] - right[1] * fwd[0];
+    for (int i = 0; i < 3; ++i) {
+        B[0 + i] = cr * right[i] + sr * up[i];
+        B[3 + i] = cr * up[i] - sr * right[i];
+        B[6 + i] = fwd[i];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* the integrated system                                                      */
+/* ------------------------------------------------------------------------- */
+
+void ko_photon_init(ko_photon* p, double a, double E, double L, double Q) {
+    p->a = a;
+    p->a2 = a * a;
+    p->E = E;
+    p->L = L;
+    p->Q = Q;
+    p->aE = a * E;
+    p->aL = a * L;
+    const double x = L - p->aE;
+    p->K = x * x + Q;
+    p->A2E2 = p->a2 * (E * E);
+    p->C1 = (Q + L * L) - p->A2E2;
+}
+
+double ko_R(const ko_photon* p, double r) {
+    const double P = p->E * (r * r + p->a2) - p->aL;
+    const double delta = (r * r - 2.0 * r) + p->a2;
+    return P * P - delta * p->K;
+}
+
+double ko_M(const ko_photon* p, double m) {
+    const double m2 = m * m;
+    return (p->Q - p->C1 * m2) - p->A2E2 * (m2 * m2);
+}
+
+/* Carter's equations in Mino time, backward (s = -tau):
+ *   dr/ds = w_r,  dw_r/ds = R'(r)/2 = 2 E r P - (r - 1) K          (kerr.cpp:346, mu = 0)
+ *   dmu/ds = w_mu, dw_mu/ds = M'(mu)/2 = -(Q + L^2 - a^2 E^2) mu - 2 a^2 E^2 mu^3
+ *   dphi/ds = -(a P / Delta - a E + L / (1 - mu^2))                  (BL; kerr.cpp:348 less a vr/Delta)
+ * M(mu) = sin^2(theta) Theta(theta); on the constraint surface w_mu = -sin(theta) w_theta. */
+void ko_rhs(const ko_photon* p, const double y[5], double dy[5]) {
+    const double r = y[0], m = y[1];
+    const double r2 = r * r;
+    const double delta = (r2 - 2.0 * r) + p->a2;
+    const double P = p->E * (r2 + p->a2) - p->aL;
+    dy[0] = y[3];
+    dy[1] = y[4];
+    dy[2] = -(((p->a * P) / delta - p->aE) + p->L / (1.0 - m * m));
+    dy[3] = 2.0 * p->E * r * P - (r - 1.0) * p->K;
+    dy[4] = -(p->C1 * m) - 2.0 * p->A2E2 * (m * m * m);
+}
+
+/* tail system in u = 1/r for z = (mu, phi, w_mu), outgoing (w_r = +sqrt(R)):
+ *   ds/du = -1/sqrt(U), U(u) = u^4 R(1/u) = Pu^2 - u^2 Du K,
+ *   Pu = E (1 + a^2 u^2) - a L u^2, Du = 1 - 2u + a^2 u^2 */
+static void tail_rhs(const ko_photon* p, double u, const double z[3], double dz[3]) {
+    const double u2 = u * u;
+    const double Pu = p->E * (1.0 + p->a2 * u2) - p->aL * u2;
+    const double Du = (1.0 - 2.0 * u) + p->a2 * u2;
+    const double U = Pu * Pu - u2 * Du * p->K;
+    const double isq = 1.0 / sqrt(fmax(U, 1e-300));
+    const double m = z[0];
+    dz[0] = -z[2] * isq;
+    dz[1] = (((p->a * Pu) / Du - p->aE) + p->L / (1.0 - m * m)) * isq;
+    dz[2] = (p->C1 * m + 2.0 * p->A2E2 * (m * m * m)) * isq;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Dormand-Prince 5(4) (Hairer, Norsett & Wanner I, Table II.5.2)             */
+/* ------------------------------------------------------------------------- */
+
+#define A21 (1.0 / 5.0)
+#define A31 (3.0 / 40.0)
+#define A32 (9.0 / 40.0)
+#define A41 (44.0 / 45.0)
+#define A42 (-56.0 / 15.0)
+#define A43 (32.0 / 9.0)
+#define A51 (19372.0 / 6561.0)
+#define A52 (-25360.0 / 2187.0)
+#define A53 (64448.0 / 6561.0)
+#define A54 (-212.0 / 729.0)
+#define A61 (9017.0 / 3168.0)
+#define A62 (-355.0 / 33.0)
+#define A63 (46732.0 / 5247.0)
+#define A64 (49.0 / 176.0)
+#define A65 (-5103.0 / 18656.0)
+#define A71 (35.0 / 384.0)
+#define A73 (500.0 / 1113.0)
+#define A74 (125.0 / 192.0)
+#define A75 (-2187.0 / 6784.0)
+#define A76 (11.0 / 84.0)
+#define E1 (71.0 / 57600.0)
+#define E3 (-71.0 / 16695.0)
+#define E4 (71.0 / 1920.0)
+#define E5 (-17253.0 / 339200.0)
+#define E6 (22.0 / 525.0)
+#define E7 (-1.0 / 40.0)
+/* dense output (Hairer's DOPRI5 continuous extension) */
+#define D1 (-12715105075.0 / 11282082432.0)
+#define D3 (87487479700.0 / 32700410799.0)
+#define D4 (-10690763975.0 / 1880347072.0)
+#define D5 (701980252875.0 / 199316789632.0)
+#define D6 (-1453857185.0 / 822651844.0)
+#define D7 (69997945.0 / 29380423.0)
+#define C2 (1.0 / 5.0)
+#define C3 (3.0 / 10.0)
+#define C4 (4.0 / 5.0)
+#define C5 (8.0 / 9.0)
+
+#define SAFETY 0.9
+#define FACMIN 0.2
+#define FACMAX 5.0
+
+/* x^(-1/5) for x > 0 by exponent split + three Newton steps on y^-5 = x.
+   Only +, *, frexp, ldexp: the same bits on any IEEE machine. */
+double ko_pow_m15(double x) {
+    static const double T[5] = {1.0, 0.8705505632961241, 0.757858283255199, 0.6597539553864471,
+                                0.5743491774985174};
+    int e;
+    const double m = frexp(x, &e); /* x = m 2^e, m in [0.5, 1) */
+    const int q = (e + 1100) / 5 - 220;
+    const int s = e - 5 * q;
+    double y = 1.1487 - 0.2974 * (m - 0.5);
+    y = ldexp(y * T[s], -q);
+    for (int it = 0; it < 3; ++it) {
+        const double y5 = y * y * y * y * y;
+        y = y * (6.0 - x * y5) * 0.2;
+    }
+    return y;
+}
+
+static double step_factor(double err, int rejected_before) {
+    double fac;
+    if (err != err) return FACMIN;
+    if (err == 0.0) fac = FACMAX;
+    else fac = SAFETY * ko_pow_m15(err);
+    if (fac > FACMAX) fac = FACMAX;
+    if (fac < FACMIN) fac = FACMIN;
+    if (rejected_before && fac > 1.0) fac = 1.0;
+    return fac;
+}
+
+static double amax(double x, double y) { return x > y ? x : y; }
+
+/* one DP5 step of the main system; k1 holds f(y) on entry.  Fills ynew, k[7][5]
+   (k[0] = k1, ..., k[6] = f(ynew)) and returns the max-norm scaled error. */
+static double dp5_main(const ko_photon* p, const ko_opts* o, const double y[5], double h, double k[7][5],
+                       double ynew[5]) {
+    double t[5];
+    int i;
+    for (i = 0; i < 5; ++i) t[i] = y[i] + h * (A21 * k[0][i]);
+    ko_rhs(p, t, k[1]);
+    for (i = 0; i < 5; ++i) t[i] = y[i] + h * (A31 * k[0][i] + A32 * k[1][i]);
+    ko_rhs(p, t, k[2]);
+    for (i = 0; i < 5; ++i) t[i] = y[i] + h * (A41 * k[0][i] + A42 * k[1][i] + A43 * k[2][i]);
+    ko_rhs(p, t, k[3]);
+    for (i = 0; i < 5; ++i)
+        t[i] = y[i] + h * (A51 * k[0][i] + A52 * k[1][i] + A53 * k[2][i] + A54 * k[3][i]);
+    ko_rhs(p, t, k[4]);
+    for (i = 0; i < 5; ++i)
+        t[i] = y[i] + h * (A61 * k[0][i] + A62 * k[1][i] + A63 * k[2][i] + A64 * k[3][i] + A65 * k[4][i]);
+    ko_rhs(p, t, k[5]);
+    for (i = 0; i < 5; ++i)
+        ynew[i] = y[i] + h * (A71 * k[0][i] + A73 * k[2][i] + A74 * k[3][i] + A75 * k[4][i] + A76 * k[5][i]);
+    ko_rhs(p, ynew, k[6]);
+    double err = 0.0;
+    for (i = 0; i < 5; ++i) {
+        const double e = h * (E1 * k[0][i] + E3 * k[2][i] + E4 * k[3][i] + E5 * k[4][i] + E6 * k[5][i] +
+                              E7 * k[6][i]);
+        double sc;
+        const double big = amax(fabs(y[i]), fabs(ynew[i]));
+        if (i == 0) sc = o->rtol * big;
+        else if (i == 3) sc = o->rtol * amax(big, y[0] * y[0]);
+        else sc = o->atol + o->rtol * big;
+        const double q = fabs(e) / sc;
+        if (!(q <= err)) err = q; /* propagates NaN */
+    }
+    return err;
+}
+
+static double dp5_tail(const ko_photon* p, const ko_opts* o, double u, const double z[3], double h,
+                       double k[7][3], double znew[3]) {
+    double t[3];
+    int i;
+    for (i = 0; i < 3; ++i) t[i] = z[i] + h * (A21 * k[0][i]);
+    tail_rhs(p, u + C2 * h, t, k[1]);
+    for (i = 0; i < 3; ++i) t[i] = z[i] + h * (A31 * k[0][i] + A32 * k[1][i]);
+    tail_rhs(p, u + C3 * h, t, k[2]);
+    for (i = 0; i < 3; ++i) t[i] = z[i] + h * (A41 * k[0][i] + A42 * k[1][i] + A43 * k[2][i]);
+    tail_rhs(p, u + C4 * h, t, k[3]);
+    for (i = 0; i < 3; ++i)
+        t[i] = z[i] + h * (A51 * k[0][i] + A52 * k[1][i] + A53 * k[2][i] + A54 * k[3][i]);
+    tail_rhs(p, u + C5 * h, t, k[4]);
+    for (i = 0; i < 3; ++i)
+        t[i] = z[i] + h * (A61 * k[0][i] + A62 * k[1][i] + A63 * k[2][i] + A64 * k[3][i] + A65 * k[4][i]);
+    tail_rhs(p, u + h, t, k[5]);
+    for (i = 0; i < 3; ++i)
+        znew[i] = z[i] + h * (A71 * k[0][i] + A73 * k[2][i] + A74 * k[3][i] + A75 * k[4][i] + A76 * k[5][i]);
+    tail_rhs(p, u + h, znew, k[6]);
+    double err = 0.0;
+    for (i = 0; i < 3; ++i) {
+        const double e = h * (E1 * k[0][i] + E3 * k[2][i] + E4 * k[3][i] + E5 * k[4][i] + E6 * k[5][i] +
+                              E7 * k[6][i]);
+        const double sc = o->atol + o->rtol * amax(fabs(z[i]), fabs(znew[i]));
+        const double q = fabs(e) / sc;
+        if (!(q <= err)) err = q;
+    }
+    return err;
+}
+
+/* dense output of component i at fraction th of the step */
+static double dense(const double y[5], const double ynew[5], double k[7][5], double h, int i, double th) {
+    const double ydiff = ynew[i] - y[i];
+    const double bspl = h * k[0][i] - ydiff;
+    const double c4 = ydiff - h * k[6][i] - bspl;
+    const double c5 = h * (D1 * k[0][i] + D3 * k[2][i] + D4 * k[3][i] + D5 * k[4][i] + D6 * k[5][i] +
+                           D7 * k[6][i]);
+    const double th1 = 1.0 - th;
+    return y[i] + th * (ydiff + th1 * (bspl + th * (c4 + th1 * c5)));
+}
+
+static double wrap_2pi(double x) {
+    const double twopi = 2.0 * KO_PI;
+    double w = x - twopi * floor(x / twopi);
+    if (w >= twopi) w -= twopi;
+    if (w < 0.0) w += twopi;
+    return w;
+}
+
+/* ------------------------------------------------------------------------- */
+/* launch                                                                     */
+/* ------------------------------------------------------------------------- */
+
+/* eht.cpp:262-271: E = 1, L = -alpha sin i, Q = beta^2 + (alpha^2 - a^2) cos^2 i,
+   r = r_cam, theta = clamp(i, 1e-4, pi - 1e-4), inward, heading for smaller theta
+   when beta > 0.  The view's theta_cam is the inclination. */
+void ko_launch_bardeen(const ko_view* v, double alpha, double beta, ko_photon* p, double y0[5]) {
+    const double a = v->spin;
+    const double sinI = sin(v->theta_cam), cosI = cos(v->theta_cam);
+    double th = v->theta_cam;
+    if (th < 1e-4) th = 1e-4;
+    if (th > KO_PI - 1e-4) th = KO_PI - 1e-4;
+    const double m0 = cos(th);
+    const double L = -alpha * sinI;
+    const double Q = beta * beta + (alpha * alpha - a * a) * cosI * cosI;
+    ko_photon_init(p, a, 1.0, L, Q);
+    y0[0] = v->r_cam;
+    y0[1] = m0;
+    y0[2] = v->phi_cam;
+    y0[3] = -sqrt(fmax(ko_R(p, v->r_cam), 0.0));
+    y0[4] = (beta > 0.0 ? 1.0 : -1.0) * sqrt(fmax(ko_M(p, m0), 0.0));
+}
+
+/* Pinhole camera in the ZAMO frame (camera.cpp:6-40 with Hover, no drift), one
+   pixel as trace.frag:1494-1496 + launchPhoton (trace.frag:1470-1488), with the
+   frame vectors lowered once per view (renderer.cpp:345-357). */
+static void launch_pinhole(const ko_view* v, int col, int row, ko_photon* p, double y0[5]) {
+    const double a = v->spin, r = v->r_cam, th = v->theta_cam;
+    double u[4], e[4][4], g[4][4], low[4][4], B[9];
+    ko_zamo_four_velocity(a, r, th, u);
+    ko_observer_frame(a, r, th, u, e);
+    ko_metric_tensor(a, r, th, g);
+    for (int j = 0; j < 4; ++j) lower4(g, e[j], low[j]);
+    ko_camera_basis(v->yaw, v->pitch, v->roll, B);
+    const double tanh_fov = tan(0.5 * v->fov_deg * KO_PI / 180.0);
+    const double sn = sin(th), cs = cos(th);
+    const double sigma = r * r + a * a * cs * cs;
+
+    const double W = (double)v->width, H = (double)v->height;
+    const double nx = (((double)col + 0.5) - 0.5 * W) / (0.5 * H);
+    const double ny = (((double)(v->height - row) - 0.5) - 0.5 * H) / (0.5 * H);
+    const double x = nx * tanh_fov, yv = ny * tanh_fov;
+    double d[3];
+    for (int i = 0; i < 3; ++i) d[i] = (x * B[i] + yv * B[3 + i]) + B[6 + i];
+    const double inv = 1.0 / sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+    const double n0 = d[0] * inv, n1 = d[1] * inv, n2 = d[2] * inv;
+    /* q = frame^T (1, -n) on the lowered (t, phi, theta) components and e^r */
+    const double qt = ((low[0][0] - n0 * low[1][0]) - n1 * low[2][0]) - n2 * low[3][0];
+    const double qp = ((low[0][3] - n0 * low[1][3]) - n1 * low[2][3]) - n2 * low[3][3];
+    const double qh = ((low[0][2] - n0 * low[1][2]) - n1 * low[2][2]) - n2 * low[3][2];
+    const double qr = ((e[0][1] - n0 * e[1][1]) - n1 * e[2][1]) - n2 * e[3][1];
+    const double E = -qt;
+    double L = qp;
+    const double cs2 = cs * cs, sn2 = fmax(sn * sn, 1e-18), a2 = a * a;
+    double Q = qh * qh + cs2 * (L * L / sn2 - a2 * E * E);
+    if (fabs(L) < KO_AXIS_SNAP * sqrt(fmax(Q + a2 * E * E, 0.0))) {
+        L = 0.0;
+        Q = qh * qh - cs2 * a2 * E * E;
+    }
+    ko_photon_init(p, a, E, L, Q);
+    y0[0] = r;
+    y0[1] = cs;
+    y0[2] = v->phi_cam;
+    y0[3] = -(sigma * qr);
+    y0[4] = sn * qh;
+}
+
+void ko_launch_pixel(const ko_view* v, int col, int row, ko_photon* p, double y0[5]) {
+    if (v->camera_kind == KO_CAMERA_PINHOLE) {
+        launch_pinhole(v, col, row, p, y0);
+        return;
+    }
+    const double px = 2.0 * v->half_width / (double)v->width;
+    const double py = 2.0 * v->half_height / (double)v->height;
+    const double alpha = -v->half_width + ((double)col + 0.5) * px;
+    const double beta = v->half_height - ((double)row + 0.5) * py;
+    ko_launch_bardeen(v, alpha, beta, p, y0);
+}
+
+/* ------------------------------------------------------------------------- */
+/* the tracer                                                                 */
+/* ------------------------------------------------------------------------- */
+
+static void finish_escape(const ko_photon* p, const ko_opts* o, const double y[5], int* steps, ko_result* out) {
+    double z[3] = {y[1], y[2], y[4]}, zn[3], k[7][3];
+    double u = 1.0 / y[0];
+    double h = -u;
+    int rejected = 0;
+    tail_rhs(p, u, z, k[0]);
+    while (*steps < o->max_steps) {
+        int last = 0;
+        if (h <= -u) {
+            h = -u;
+            last = 1;
+        }
+        const double err = dp5_tail(p, o, u, z, h, k, zn);
+        ++*steps;
+        if (!(err <= 1.0)) {
+            h *= step_factor(err, 1);
+            rejected = 1;
+            continue;
+        }
+        const double fac = step_factor(err, rejected);
+        rejected = 0;
+        if (p->L == 0.0 && z[0] * z[2] > 0.0 && zn[0] * zn[2] < 0.0) zn[1] += KO_PI;
+        for (int i = 0; i < 3; ++i) z[i] = zn[i];
+        for (int i = 0; i < 3; ++i) k[0][i] = k[6][i];
+        if (last) {
+            u = 0.0;
+            double m = z[0];
+            if (m > 1.0) m = 1.0;
+            if (m < -1.0) m = -1.0;
+            out->cls = KO_ESCAPED;
+            out->theta_inf = acos(m);
+            out->phi_inf = wrap_2pi(z[1]);
+            out->y_end[0] = INFINITY;
+            out->y_end[1] = z[0];
+            out->y_end[2] = z[1];
+            out->y_end[3] = INFINITY;
+            out->y_end[4] = z[2];
+            return;
+        }
+        u = u + h;
+        h *= fac;
+    }
+    out->cls = KO_UNFINISHED;
+}
+
+void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, const double y0[5],
+                     ko_result* out) {
+    const double a = v->spin;
+    const double r_cap = v->capture_factor * ko_horizon_radius(a);
+    const double r_esc = amax(v->escape_radius, 1.5 * v->r_cam);
+    const int disk = v->disk_enabled;
+    const double r_in = v->disk_inner > 0.0 ? v->disk_inner : ko_isco_radius(a, v->disk_prograde);
+    const double r_out = v->disk_outer;
+
+    double y[5], yn[5], k[7][5];
+    for (int i = 0; i < 5; ++i) y[i] = y0[i];
+    out->cls = KO_UNFINISHED;
+    out->flags = 0;
+    out->theta_inf = out->phi_inf = out->r_hit = out->g = NAN;
+    int steps = 0, rejected = 0;
+    double h = 0.01 / v->r_cam;
+    ko_rhs(p, y, k[0]);
+    while (steps < o->max_steps) {
+        const double err = dp5_main(p, o, y, h, k, yn);
+        ++steps;
+        if (!(err <= 1.0)) {
+            h *= step_factor(err, 1);
+            rejected = 1;
+            continue;
+        }
+        const double fac = step_factor(err, rejected);
+        rejected = 0;
+        /* through the pole (L = 0 only): same point, phi + pi */
+        if (p->L == 0.0 && y[1] * y[4] > 0.0 && yn[1] * yn[4] < 0.0) yn[2] += KO_PI;
+
+        /* events in the reference's order: disk, horizon, escape (trajectory.cpp:307-392) */
+        if (disk && y[1] * yn[1] < 0.0) {
+            double lo = 0.0, hi = 1.0;
+            const int pos = y[1] > 0.0;
+            for (int it = 0; it < 48; ++it) {
+                const double mid = 0.5 * (lo + hi);
+                const double f = dense(y, yn, k, h, 1, mid);
+                if ((f > 0.0) == pos) lo = mid;
+                else hi = mid;
+            }
+            const double rc = dense(y, yn, k, h, 0, 0.5 * (lo + hi));
+            if (rc >= r_in && rc <= r_out) {
+                /* trajectory.cpp:311-321 */
+                const double omega = ko_circular_omega(a, rc, v->disk_prograde);
+                const double gtt = -(1.0 - 2.0 / rc), gtp = -2.0 * a / rc;
+                const double gpp = rc * rc + a * a + 2.0 * a * a / rc;
+                const double norm = -(gtt + 2.0 * gtp * omega + gpp * omega * omega);
+                if (norm > 0.0) {
+                    out->cls = KO_DISK;
+                    out->r_hit = rc;
+                    out->g = sqrt(norm) / (p->E - p->L * omega);
+                    for (int i = 0; i < 5; ++i) out->y_end[i] = yn[i];
+                    out->steps = steps;
+                    return;
+                }
+            }
+        }
+        if (yn[0] < r_cap) {
+            out->cls = KO_CAPTURED;
+            for (int i = 0; i < 5; ++i) out->y_end[i] = yn[i];
+            out->steps = steps;
+            return;
+        }
+        if (yn[0] > r_esc && yn[3] > 0.0) {
+            finish_escape(p, o, yn, &steps, out);
+            out->steps = steps;
+            return;
+        }
+        for (int i = 0; i < 5; ++i) y[i] = yn[i];
+        for (int i = 0; i < 5; ++i) k[0][i] = k[6][i];
+        h *= fac;
+    }
+    for (int i = 0; i < 5; ++i) out->y_end[i] = y[i];
+    out->steps = steps;
+}
+
+void ko_trace_pixel(const ko_view* v, const ko_opts* o, int col, int row, ko_result* out) {
+    ko_photon p;
+    double y0[5];
+    ko_launch_pixel(v, col, row, &p, y0);
+    ko_trace_photon(v, o, &p, y0, out);
+}
+
+/* ------------------------------------------------------------------------- */
+/* threaded image driver (eht.cpp:321-347)                                    */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    const ko_view* v;
+    const ko_opts* o;
+    const volatile int* cancel;
+    int stride;
+    int next_row;
+    int cancelled;
+    int8_t* cls;
+    double *theta, *phi, *rhit, *g;
+    int32_t* steps;
+} render_job;
+
+static void* render_worker(void* arg) {
+    render_job* j = (render_job*)arg;
+    const int W = j->v->width, H = j->v->height;
+    for (;;) {
+        const int row = __atomic_fetch_add(&j->next_row, 1, __ATOMIC_RELAXED);
+        if (row >= H) break;
+        if (j->cancel && *j->cancel) {
+            __atomic_store_n(&j->cancelled, 1, __ATOMIC_RELAXED);
+            break;
+        }
+        for (int col = 0; col < W; ++col) {
+            const long idx = (long)row * W + col;
+            if (j->stride > 1 && idx % j->stride != 0) continue;
+            const long o = j->stride > 1 ? idx / j->stride : idx;
+            ko_result res;
+            ko_trace_pixel(j->v, j->o, col, row, &res);
+            if (j->cls) j->cls[o] = (int8_t)(res.cls | res.flags);
+            if (j->theta) j->theta[o] = res.theta_inf;
+            if (j->phi) j->phi[o] = res.phi_inf;
+            if (j->rhit) j->rhit[o] = res.r_hit;
+            if (j->g) j->g[o] = res.g;
+            if (j->steps) j->steps[o] = res.steps;
+        }
+    }
+    return NULL;
+}
+
+long ko_render(const ko_view* v, const ko_opts* o, int threads, const volatile int* cancel, int stride,
+               int8_t* cls, double* theta_inf, double* phi_inf, double* r_hit, double* g, int32_t* steps) {
+    render_job j = {v, o, cancel, stride < 1 ? 1 : stride, 0, 0, cls, theta_inf, phi_inf, r_hit, g, steps};
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t pool[256];
+    int started = 0;
+    for (int t = 1; t < threads; ++t)
+        if (pthread_create(&pool[started], NULL, render_worker, &j) == 0) ++started;
+    render_worker(&j);
+    for (int t = 0; t < started; ++t) pthread_join(pool[t], NULL);
+    if (j.cancelled) return -1;
+    const long n = (long)v->width * v->height;
+    return j.stride > 1 ? (n + j.stride - 1) / j.stride : n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* critical curve (eht.cpp:52-97)                                             */
+/* ------------------------------------------------------------------------- */
+
+int ko_critical_curve(double a, double inclination, int samples, double* alpha, double* beta, int cap) {
+    const double sinI = sin(inclination), cosI = cos(inclination);
+    int n_out = 0;
+    if (fabs(a) < 1e-6) {
+        for (int k = 0; k < samples && n_out < cap; ++k) {
+            const double psi = 2.0 * KO_PI * k / samples;
+            alpha[n_out] = sqrt(27.0) * cos(psi);
+            beta[n_out] = sqrt(27.0) * sin(psi);
+            ++n_out;
+        }
+        return n_out;
+    }
+    const double r1 = ko_photon_orbit_radius(a, 1), r2 = ko_photon_orbit_radius(a, 0);
+    const int n = (samples > 16 ? samples : 16) * 8;
+    int nu = 0;
+    /* upper half first, into the front of the buffers */
+    for (int k = 0; k <= n; ++k) {
+        const double x = 0.5 - 0.5 * cos(KO_PI * k / n);
+        const double r = r1 + (r2 - r1) * x;
+        const double xi = (r * r * (3.0 - r) - a * a * (r + 1.0)) / (a * (r - 1.0));
+        const double eta = r * r * r * (4.0 * a * a - r * (r - 3.0) * (r - 3.0)) / (a * a * (r - 1.0) * (r - 1.0));
+        const double b2 = eta + a * a * cosI * cosI - xi * xi * cosI * cosI / (sinI * sinI);
+        if (b2 < 0.0) continue;
+        if (nu >= cap) break;
+        alpha[nu] = -xi / sinI;
+        beta[nu] = sqrt(b2);
+        ++nu;
+    }
+    n_out = nu;
+    for (int k = nu; k-- > 0 && n_out < cap;) {
+        alpha[n_out] = alpha[k];
+        beta[n_out] = -beta[k];
+        ++n_out;
+    }
+    return n_out;
+}
+
+double ko_critical_radius(const double* alpha, const double* beta, int n, double psi) {
+    const double dx = cos(psi), dy = sin(psi);
+    double best = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const int q = (k + 1) % n;
+        const double ex = alpha[q] - alpha[k], ey = beta[q] - beta[k];
+        const double det = ex * dy - ey * dx;
+        if (fabs(det) < 1e-15) continue;
+        const double t = (beta[k] * dx - alpha[k] * dy) / det;
+        const double s = (ex * beta[k] - ey * alpha[k]) / det;
+        if (t >= 0.0 && t <= 1.0 && s > 0.0 && s > best) best = s;
+    }
+    return best;
+}

@@ -1,0 +1,146 @@
+/*
+ * kerr_oracle.h -- FP64 CPU oracle for the Kerr per-pixel photon trace.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker, never the product: only the
+ * test suite (tests/), the smoke check in __graft_entry__.py and the
+ * cpu_baseline leg of bench.py may load it.  The HIP library under
+ * black_simulation_amd/ never links or calls anything in oracle/.
+ *
+ * What it restates (all citations are files of Dhruv-Deku/Black_simulation):
+ *   - characteristic radii                  src/kerr.cpp:157-223
+ *   - circular-orbit angular velocity       src/kerr.cpp:296-299
+ *   - Mino-time geodesic right-hand side    src/kerr.cpp:323-351 (ko_derivatives, reference form)
+ *   - radial / polar potentials             src/kerr.cpp:353-363
+ *   - metric tensor, lower, observer frame  src/kerr.cpp:238-270, 424-456
+ *   - photon launch from a frame            src/kerr.cpp:458-483 (axis snap kerr.h:115)
+ *   - constraint projection                 src/kerr.cpp:521-542
+ *   - Bardeen image-plane launch            src/eht.cpp:262-271, pixel centres eht.cpp:332-334
+ *   - threaded row-claiming image driver    src/eht.cpp:321-347
+ *   - critical curve / critical radius      src/eht.cpp:52-97
+ *   - disk crossing + Keplerian redshift    src/trajectory.cpp:307-324
+ *   - capture / escape tests                src/trajectory.cpp:377-392, src/eht.cpp:285-289
+ *   - camera basis (FP64 here, FP32 glm in the reference)  src/camera.cpp:6-21
+ *
+ * New (no reference counterpart, pinned by closed forms and self-convergence):
+ *   - the integrated system: Carter's equations in Mino time in (r, mu = cos theta),
+ *     traced backward (s = -tau), with Boyer-Lindquist phi;
+ *   - Dormand-Prince 5(4) with dense output (Hairer, Norsett & Wanner, Solving
+ *     ODEs I, sec. II.5 and II.6), a max-norm error controller, a dense-output root
+ *     for the disk crossing and an exact tail integration in u = 1/r to u = 0
+ *     for the asymptotic sky angles.
+ *
+ * Arithmetic is plain IEEE binary64 with FP contraction disabled; the step loop
+ * uses only +, -, *, /, sqrt and fma, each correctly rounded, so an
+ * implementation that performs the same operations in the same order gets the
+ * same bits.
+ */
+#ifndef KERR_ORACLE_H
+#define KERR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KO_PI 3.14159265358979323846
+
+/* ray classes (kt_class in include/kerr_trace.h uses the same codes) */
+enum { KO_ESCAPED = 0, KO_CAPTURED = 1, KO_DISK = 2, KO_UNFINISHED = 3 };
+enum { KO_FLAG_NONFINITE = 0x40 };
+
+enum { KO_CAMERA_BARDEEN = 0, KO_CAMERA_PINHOLE = 1 };
+
+/* ---- radii and circular orbits (kerr.cpp:157-223, 296-299) ---- */
+double ko_horizon_radius(double a);
+double ko_inner_horizon_radius(double a);
+double ko_isco_radius(double a, int prograde);
+double ko_photon_orbit_radius(double a, int prograde);
+double ko_marginally_bound_radius(double a, int prograde);
+double ko_circular_omega(double a, double r, int prograde);
+
+/* ---- reference-form geodesic equations (kerr.cpp:323-363) ---- */
+typedef struct { double E, L, Q, mu; } ko_consts;
+typedef struct { double t, r, theta, phi, vr, vtheta; } ko_state;
+typedef struct { double dt, dr, dtheta, dphi, dvr, dvtheta; } ko_derivs;
+
+double ko_inflow_factor(double P, double vr, double delta, double K);
+ko_derivs ko_derivatives(double a, ko_consts c, ko_state s);
+double ko_radial_potential(double a, ko_consts c, double r);
+double ko_polar_potential(double a, ko_consts c, double theta);
+void ko_enforce_constraints(double a, ko_consts c, ko_state* s);
+
+/* ---- metric, frames, launch (kerr.cpp:227-270, 424-483) ---- */
+void ko_metric_tensor(double a, double r, double theta, double g[4][4]);
+void ko_zamo_four_velocity(double a, double r, double theta, double u[4]);
+void ko_observer_frame(double a, double r, double theta, const double u[4], double e[4][4]);
+void ko_photon_from_frame(double a, double r, double theta, double phi, const double e[4][4],
+                          const double look[3], ko_consts* c, ko_state* s);
+/* camera basis of src/camera.cpp:6-21 as a column-major 3x3 (right, up, forward),
+   components along the frame's (r, theta, phi) axes */
+void ko_camera_basis(double yaw, double pitch, double roll, double basis[9]);
+
+/* ---- the integrated system: backward Mino time, (r, mu, phi, w_r, w_mu) ---- */
+typedef struct {
+    double a, a2, E, L, Q;
+    double aE, aL, K, C1, A2E2;
+} ko_photon;
+
+void ko_photon_init(ko_photon* p, double a, double E, double L, double Q);
+/* dy/ds for y = (r, mu, phi_BL, w_r = dr/ds, w_mu = dmu/ds), s = -tau */
+void ko_rhs(const ko_photon* p, const double y[5], double dy[5]);
+/* R(r) and M(mu) = sin^2(theta) Theta(theta) for the photon */
+double ko_R(const ko_photon* p, double r);
+double ko_M(const ko_photon* p, double m);
+
+/* ---- views and the tracer ---- */
+typedef struct {
+    int camera_kind;           /* KO_CAMERA_BARDEEN or KO_CAMERA_PINHOLE */
+    int width, height;
+    double spin;
+    double r_cam, theta_cam, phi_cam;
+    double half_width, half_height;          /* Bardeen image plane (M)           */
+    double yaw, pitch, roll, fov_deg;        /* pinhole camera (ZAMO frame)        */
+    double disk_inner, disk_outer;           /* disk_inner <= 0 means r_ISCO       */
+    int disk_prograde, disk_enabled;
+    double escape_radius;                    /* effective: max(this, 1.5 r_cam)    */
+    double capture_factor;                   /* captured when r < this * r_+       */
+} ko_view;
+
+typedef struct {
+    double rtol, atol;
+    int max_steps;
+} ko_opts;
+
+typedef struct {
+    int cls, flags, steps;
+    double theta_inf, phi_inf, r_hit, g;
+    double y_end[5];            /* state at termination (tail: u = 0 -> y_end[0] = inf) */
+} ko_result;
+
+/* launch helpers: initial photon + state for pixel (col, row) of a view */
+void ko_launch_pixel(const ko_view* v, int col, int row, ko_photon* p, double y0[5]);
+void ko_launch_bardeen(const ko_view* v, double alpha, double beta, ko_photon* p, double y0[5]);
+
+void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, const double y0[5],
+                     ko_result* out);
+void ko_trace_pixel(const ko_view* v, const ko_opts* o, int col, int row, ko_result* out);
+
+/* whole image, `threads` workers claiming rows from an atomic counter (eht.cpp:321-347).
+   Outputs are row-major from the top row; any pointer may be NULL. `stride` > 1 traces
+   only pixels with (row * width + col) % stride == 0 (a sparse timing sample) and writes
+   them densely.  Returns the number of rays traced, or -1 if cancelled. */
+long ko_render(const ko_view* v, const ko_opts* o, int threads, const volatile int* cancel, int stride,
+               int8_t* cls, double* theta_inf, double* phi_inf, double* r_hit, double* g, int32_t* steps);
+
+/* ---- critical curve (eht.cpp:52-97) ---- */
+int ko_critical_curve(double a, double inclination, int samples, double* alpha, double* beta, int cap);
+double ko_critical_radius(const double* alpha, const double* beta, int n, double psi);
+
+/* the controller's deterministic x^(-1/5) */
+double ko_pow_m15(double x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,195 @@
+"""ctypes front end of the FP64 CPU oracle (oracle/kerr_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+bench.py's cpu_baseline leg, as the checker.  The product (black_simulation_amd)
+never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_LIB_PATH = os.path.join(HERE, "_ref", "libkerr_ref.so")
+
+ESCAPED, CAPTURED, DISK, UNFINISHED = 0, 1, 2, 3
+BARDEEN, PINHOLE = 0, 1
+
+
+class View(C.Structure):
+    """Mirror of ko_view (kerr_oracle.h); same fields as kt_view in include/kerr_trace.h."""
+
+    _fields_ = [
+        ("camera_kind", C.c_int),
+        ("width", C.c_int),
+        ("height", C.c_int),
+        ("spin", C.c_double),
+        ("r_cam", C.c_double),
+        ("theta_cam", C.c_double),
+        ("phi_cam", C.c_double),
+        ("half_width", C.c_double),
+        ("half_height", C.c_double),
+        ("yaw", C.c_double),
+        ("pitch", C.c_double),
+        ("roll", C.c_double),
+        ("fov_deg", C.c_double),
+        ("disk_inner", C.c_double),
+        ("disk_outer", C.c_double),
+        ("disk_prograde", C.c_int),
+        ("disk_enabled", C.c_int),
+        ("escape_radius", C.c_double),
+        ("capture_factor", C.c_double),
+    ]
+
+
+class Opts(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("max_steps", C.c_int)]
+
+
+class Photon(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("a", "a2", "E", "L", "Q", "aE", "aL", "K", "C1", "A2E2")]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("cls", C.c_int),
+        ("flags", C.c_int),
+        ("steps", C.c_int),
+        ("theta_inf", C.c_double),
+        ("phi_inf", C.c_double),
+        ("r_hit", C.c_double),
+        ("g", C.c_double),
+        ("y_end", C.c_double * 5),
+    ]
+
+
+class Consts(C.Structure):
+    _fields_ = [("E", C.c_double), ("L", C.c_double), ("Q", C.c_double), ("mu", C.c_double)]
+
+
+class State(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("t", "r", "theta", "phi", "vr", "vtheta")]
+
+
+class Derivs(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("dt", "dr", "dtheta", "dphi", "dvr", "dvtheta")]
+
+
+def build() -> None:
+    """Compile liboracle.so (and oracle/_ref when the reference sources exist)."""
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    if os.path.isdir("/root/reference/src"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        d, i, P = C.c_double, C.c_int, C.POINTER
+        for name, res, args in [
+            ("ko_horizon_radius", d, [d]),
+            ("ko_inner_horizon_radius", d, [d]),
+            ("ko_isco_radius", d, [d, i]),
+            ("ko_photon_orbit_radius", d, [d, i]),
+            ("ko_marginally_bound_radius", d, [d, i]),
+            ("ko_circular_omega", d, [d, d, i]),
+            ("ko_derivatives", Derivs, [d, Consts, State]),
+            ("ko_radial_potential", d, [d, Consts, d]),
+            ("ko_polar_potential", d, [d, Consts, d]),
+            ("ko_enforce_constraints", None, [d, Consts, P(State)]),
+            ("ko_photon_from_frame", None, [d, d, d, d, P(d), P(d), P(Consts), P(State)]),
+            ("ko_zamo_four_velocity", None, [d, d, d, P(d)]),
+            ("ko_observer_frame", None, [d, d, d, P(d), P(d)]),
+            ("ko_metric_tensor", None, [d, d, d, P(d)]),
+            ("ko_camera_basis", None, [d, d, d, P(d)]),
+            ("ko_photon_init", None, [P(Photon), d, d, d, d]),
+            ("ko_rhs", None, [P(Photon), P(d), P(d)]),
+            ("ko_R", d, [P(Photon), d]),
+            ("ko_M", d, [P(Photon), d]),
+            ("ko_launch_pixel", None, [P(View), i, i, P(Photon), P(d)]),
+            ("ko_launch_bardeen", None, [P(View), d, d, P(Photon), P(d)]),
+            ("ko_trace_photon", None, [P(View), P(Opts), P(Photon), P(d), P(Result)]),
+            ("ko_trace_pixel", None, [P(View), P(Opts), i, i, P(Result)]),
+            ("ko_render", C.c_long, [P(View), P(Opts), i, P(C.c_int), i, P(C.c_int8), P(d), P(d), P(d), P(d),
+                                     P(C.c_int32)]),
+            ("ko_critical_curve", i, [d, d, i, P(d), P(d), i]),
+            ("ko_critical_radius", d, [P(d), P(d), i, d]),
+            ("ko_pow_m15", d, [d]),
+        ]:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_view(**kw) -> View:
+    v = View()
+    defaults = dict(camera_kind=BARDEEN, width=256, height=256, spin=0.9, r_cam=30.0,
+                    theta_cam=np.deg2rad(80.0), phi_cam=0.0, half_width=12.0, half_height=12.0,
+                    yaw=0.0, pitch=0.0, roll=0.0, fov_deg=60.0, disk_inner=0.0, disk_outer=20.0,
+                    disk_prograde=1, disk_enabled=1, escape_radius=1000.0, capture_factor=1.01)
+    defaults.update(kw)
+    for k, val in defaults.items():
+        setattr(v, k, val)
+    return v
+
+
+def make_opts(rtol=1e-10, atol=1e-12, max_steps=200000) -> Opts:
+    return Opts(rtol, atol, max_steps)
+
+
+def trace_pixel(view: View, opts: Opts, col: int, row: int) -> Result:
+    res = Result()
+    lib().ko_trace_pixel(C.byref(view), C.byref(opts), col, row, C.byref(res))
+    return res
+
+
+def render(view: View, opts: Opts, threads: int = 0, stride: int = 1) -> dict:
+    """Whole image on `threads` CPU threads (0 = all cores); stride > 1 = sparse sample."""
+    if threads <= 0:
+        threads = os.cpu_count() or 1
+    n = view.width * view.height
+    m = (n + stride - 1) // stride if stride > 1 else n
+    out = {
+        "cls": np.zeros(m, np.int8),
+        "theta": np.zeros(m),
+        "phi": np.zeros(m),
+        "r_hit": np.zeros(m),
+        "g": np.zeros(m),
+        "steps": np.zeros(m, np.int32),
+    }
+    got = lib().ko_render(C.byref(view), C.byref(opts), threads, None, stride,
+                          out["cls"].ctypes.data_as(C.POINTER(C.c_int8)), _dp(out["theta"]),
+                          _dp(out["phi"]), _dp(out["r_hit"]), _dp(out["g"]),
+                          out["steps"].ctypes.data_as(C.POINTER(C.c_int32)))
+    assert got == m, (got, m)
+    return out
+
+
+def critical_curve(a: float, inclination: float, samples: int):
+    cap = 8 * max(samples, 16) * 2 + 4 + samples
+    al = np.zeros(cap)
+    be = np.zeros(cap)
+    n = lib().ko_critical_curve(a, inclination, samples, _dp(al), _dp(be), cap)
+    return al[:n].copy(), be[:n].copy()
+
+
+def critical_radius(alpha: np.ndarray, beta: np.ndarray, psi: float) -> float:
+    al = np.ascontiguousarray(alpha, dtype=np.float64)
+    be = np.ascontiguousarray(beta, dtype=np.float64)
+    return lib().ko_critical_radius(_dp(al), _dp(be), len(al), psi)
