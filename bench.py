@@ -1,0 +1,142 @@
+"""Headline benchmark: Mrays/s tracing 1920x1080 frames of Kerr a=0.998.
+
+One step = one full 1080p frame (2,073,600 rays, camera r=50 M, i=85 deg, 30 M-wide
+Bardeen plane, disk r_ISCO..40 M) traced by the FP64 persistent integrator (and, with
+--path surrogate, by the surrogate + integrator fallback).  Inputs are the view
+constants; outputs stay in HBM.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--path plain|mixed|surrogate] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# FP64 operations of one DP5 attempt of the integrator (kt_device.hpp), counting each
+# +, -, *, / as one (no FMA: contraction is off):
+#   6 right-hand sides x 28                                   = 168
+#   stage combinations (5 comps: 3, 5, 7, 9, 11, 11 ops)      = 230
+#   error estimate (5 comps x 12) + norm (5 x 3 + 1)          =  76
+#   controller (x^-1/5: 28, safety and h update: 2)           =  30
+OPS_PER_ATTEMPT = 504
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+
+
+def cpu_baseline(view, seconds_target: float = 15.0) -> dict:
+    """The FP64 oracle (same DP5, rtol 1e-10) on this host's cores over a sparse sample
+    of the same frame: every `stride`-th pixel, the stride sized for ~10-30 s."""
+    from oracle import oracle as O
+
+    cores = os.cpu_count() or 1
+    ov = O.make_view(**{k: getattr(view, k) for k in (
+        "camera_kind", "width", "height", "spin", "r_cam", "theta_cam", "phi_cam", "half_width", "half_height",
+        "yaw", "pitch", "roll", "fov_deg", "disk_inner", "disk_outer", "disk_prograde", "disk_enabled",
+        "escape_radius", "capture_factor")})
+    opts = O.make_opts()
+    # calibrate on a tiny sample first
+    stride = 4099
+    t0 = time.perf_counter()
+    out = O.render(ov, opts, threads=cores, stride=stride)
+    dt = time.perf_counter() - t0
+    rate = len(out["cls"]) / max(dt, 1e-6)
+    want = max(1, int(rate * seconds_target))
+    stride = max(1, (view.width * view.height) // want)
+    t0 = time.perf_counter()
+    out = O.render(ov, opts, threads=cores, stride=stride)
+    dt = time.perf_counter() - t0
+    n = len(out["cls"])
+    return {"value": n / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"every {stride}th pixel of the same 1920x1080 a=0.998 frame ({n} rays, {dt:.1f} s, "
+                      f"FP64 DP5 rtol 1e-10, {cores} threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--path", choices=["plain", "mixed", "surrogate"], default="plain")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: rays are independent and this build runs on a single MI355X; "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import black_simulation_amd as bsa
+    from black_simulation_amd.configs import c2_view
+
+    view = c2_view(args.width, args.height)
+    prec = {"plain": bsa.PREC_FP64, "mixed": bsa.PREC_MIXED, "surrogate": bsa.PREC_FP64}[args.path]
+    opts = bsa.TraceOpts(precision=prec, use_surrogate=1 if args.path == "surrogate" else 0)
+    tr = bsa.Tracer(0)
+    out = tr.alloc_outputs(view.rays)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        tr.trace_views([view], opts, out=out, stream=stream)
+    torch.cuda.synchronize()
+    integ_ms, kern_ms, steps = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tr.trace_views([view], opts, out=out, stream=stream)
+        st = tr.stats()  # waits on the HIP events of this step's kernels
+        integ_ms.append(st["integrator_ms"])
+        kern_ms.append(st["kernel_ms"])
+        steps.append(st["steps"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = view.rays * args.steps / elapsed / 1e6
+
+    avg_integ = sum(integ_ms) / len(integ_ms)
+    attempts = sum(steps) / len(steps)
+    achieved = attempts * OPS_PER_ATTEMPT / (avg_integ * 1e-3) / 1e12
+    peak = FP64_VECTOR_PEAK_TFLOPS if prec == bsa.PREC_FP64 else FP32_VECTOR_PEAK_TFLOPS
+    cls = out["cls"].cpu()
+    counts = torch.bincount((cls & 7).to(torch.int64), minlength=4).tolist()
+    rec = {
+        "metric": "Mrays/s 1080p a=0.998: surrogate+fallback vs plain integrator @<=1e-4 rad",
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == bsa.PREC_FP64 else "f32+f64",
+        "data": "synthetic: analytic Bardeen image plane, no input files",
+        "config": {"workload": f"{args.path} integrator, Kerr a=0.998, {view.width}x{view.height} Bardeen plane "
+                               f"30 M wide, camera r=50 M i=85 deg, disk r_ISCO..40 M, DP5 rtol 1e-10",
+                   "rays_per_step": view.rays, "path": args.path,
+                   "dp5_attempts_per_step": int(attempts), "class_counts": counts,
+                   "launch": {"block": st["block"], "grid": st["grid"], "waves_per_cu": st["waves_per_cu"],
+                              "steps_per_dequeue": st["steps_per_dequeue"]}},
+        "roofline": {"bound": "valu_fp64" if prec == bsa.PREC_FP64 else "valu_fp32",
+                     "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "kernel": "k_trace_persistent", "kernel_ms": round(avg_integ, 3),
+                     "ops_per_attempt": OPS_PER_ATTEMPT},
+        "cpu_baseline": None,
+    }
+    tr.close()
+    if not args.no_cpu:
+        rec["cpu_baseline"] = cpu_baseline(view)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,133 @@
+"""ctypes binding of libkerr_trace.so (the C ABI declared in include/kerr_trace.h).
+
+The shared library is the product; this module only loads it.  It raises if the
+library is missing -- there is no CPU fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libkerr_trace.so")
+
+KT_OK, KT_EINVAL, KT_EHIP, KT_ECANCELLED, KT_ENOMEM, KT_EUNSUPPORTED, KT_ENODEVICE = range(7)
+ESCAPED, CAPTURED, DISK, UNFINISHED = 0, 1, 2, 3
+FLAG_FP64_RESTEP, FLAG_SURROGATE = 0x10, 0x20
+CAMERA_BARDEEN, CAMERA_PINHOLE = 0, 1
+PREC_FP64, PREC_MIXED, PREC_FP32 = 0, 1, 2
+MODE_PERSISTENT, MODE_ONESHOT = 0, 1
+
+
+class KtView(C.Structure):
+    _fields_ = [
+        ("camera_kind", C.c_int),
+        ("width", C.c_int),
+        ("height", C.c_int),
+        ("spin", C.c_double),
+        ("r_cam", C.c_double),
+        ("theta_cam", C.c_double),
+        ("phi_cam", C.c_double),
+        ("half_width", C.c_double),
+        ("half_height", C.c_double),
+        ("yaw", C.c_double),
+        ("pitch", C.c_double),
+        ("roll", C.c_double),
+        ("fov_deg", C.c_double),
+        ("disk_inner", C.c_double),
+        ("disk_outer", C.c_double),
+        ("disk_prograde", C.c_int),
+        ("disk_enabled", C.c_int),
+        ("escape_radius", C.c_double),
+        ("capture_factor", C.c_double),
+    ]
+
+
+class KtTraceOpts(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int),
+        ("mode", C.c_int),
+        ("use_surrogate", C.c_int),
+        ("max_steps", C.c_int),
+        ("rtol", C.c_double),
+        ("atol", C.c_double),
+        ("rtol32", C.c_double),
+        ("atol32", C.c_double),
+        ("restep_drift", C.c_double),
+        ("block", C.c_int),
+        ("waves_per_cu", C.c_int),
+        ("steps_per_dequeue", C.c_int),
+        ("gate_prob", C.c_double),
+        ("gate_ring", C.c_double),
+    ]
+
+
+class KtRaysOut(C.Structure):
+    _fields_ = [
+        ("cls", C.c_void_p),
+        ("theta_inf", C.c_void_p),
+        ("phi_inf", C.c_void_p),
+        ("r_hit", C.c_void_p),
+        ("g", C.c_void_p),
+        ("steps", C.c_void_p),
+        ("on_device", C.c_int),
+    ]
+
+
+class KtDeviceOpts(C.Structure):
+    _fields_ = [("device", C.c_int)]
+
+
+class KtStats(C.Structure):
+    _fields_ = [
+        ("kernel_ms", C.c_double),
+        ("integrator_ms", C.c_double),
+        ("surrogate_ms", C.c_double),
+        ("rays", C.c_longlong),
+        ("steps", C.c_longlong),
+        ("fallback_rays", C.c_longlong),
+        ("restep_rays", C.c_longlong),
+        ("block", C.c_int),
+        ("waves_per_cu", C.c_int),
+        ("steps_per_dequeue", C.c_int),
+        ("grid", C.c_int),
+    ]
+
+
+# every entry point include/kerr_trace.h declares: name -> (restype, argtypes)
+_P = C.POINTER
+SIGNATURES = {
+    "kt_trace_opts_default": (None, [_P(KtTraceOpts)]),
+    "kt_view_default": (None, [_P(KtView)]),
+    "kt_create": (C.c_int, [_P(KtDeviceOpts), _P(C.c_void_p)]),
+    "kt_destroy": (C.c_int, [C.c_void_p]),
+    "kt_last_error_string": (C.c_char_p, [C.c_void_p]),
+    "kt_last_stats": (C.c_int, [C.c_void_p, _P(KtStats)]),
+    "kt_trace_views": (C.c_int, [C.c_void_p, _P(KtView), C.c_int, _P(KtTraceOpts), _P(KtRaysOut), C.c_void_p]),
+    "kt_trace_rays": (C.c_int, [C.c_void_p, _P(KtView), C.c_void_p, C.c_longlong, _P(KtTraceOpts), _P(KtRaysOut),
+                                C.c_void_p]),
+    "kt_horizon_radius": (C.c_double, [C.c_double]),
+    "kt_isco_radius": (C.c_double, [C.c_double, C.c_int]),
+    "kt_photon_orbit_radius": (C.c_double, [C.c_double, C.c_int]),
+    "kt_circular_omega": (C.c_double, [C.c_double, C.c_double, C.c_int]),
+    "kt_critical_curve": (C.c_int, [C.c_double, C.c_double, C.c_int, _P(C.c_double), _P(C.c_double), C.c_int]),
+    "kt_critical_radius": (C.c_double, [_P(C.c_double), _P(C.c_double), C.c_int, C.c_double]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libkerr_trace.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; "
+                               "g.build()')")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib

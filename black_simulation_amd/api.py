@@ -1,0 +1,258 @@
+"""Python face of the MI355X Kerr tracer, mirroring the reference's own entry points.
+
+    render(view)            ~ eht::render(View, Flow, threads, cancel)   (src/eht.h:140)
+    trace_ray(view, a, b)   ~ eht::traceRay(View, Flow, alpha, beta)     (src/eht.h:130)
+    trace_looks(view, looks)~ trajectory::traceRay(a, r, th, ph, frame, look, scene)
+                                                                        (src/trajectory.h:118)
+    horizon_radius, isco_radius, photon_orbit_radius, circular_omega      (src/kerr.h:39-45, 71)
+    critical_curve, critical_radius                                       (src/eht.h:66-68)
+
+All work goes through libkerr_trace.so (include/kerr_trace.h); torch is used only for
+device buffers and streams.  There is no CPU fallback: without the library and a GPU
+these functions raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import math
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import _lib as K
+
+OUTPUT_FIELDS = ("cls", "theta_inf", "phi_inf", "r_hit", "g", "steps")
+
+
+@dataclasses.dataclass
+class View:
+    """One camera view (kt_view).  Bardeen: the image plane of eht::View; pinhole: the
+    simulator's Camera (src/camera.h:20-29) in the ZAMO frame."""
+
+    camera_kind: int = K.CAMERA_BARDEEN
+    width: int = 256
+    height: int = 256
+    spin: float = 0.9
+    r_cam: float = 30.0
+    theta_cam: float = math.radians(80.0)
+    phi_cam: float = 0.0
+    half_width: float = 12.0
+    half_height: float = 12.0
+    yaw: float = 0.0
+    pitch: float = 0.0
+    roll: float = 0.0
+    fov_deg: float = 60.0
+    disk_inner: float = 0.0
+    disk_outer: float = 20.0
+    disk_prograde: int = 1
+    disk_enabled: int = 1
+    escape_radius: float = 1000.0
+    capture_factor: float = 1.01
+
+    def to_c(self) -> K.KtView:
+        v = K.KtView()
+        for f in dataclasses.fields(self):
+            setattr(v, f.name, getattr(self, f.name))
+        return v
+
+    @property
+    def rays(self) -> int:
+        return self.width * self.height
+
+
+@dataclasses.dataclass
+class TraceOpts:
+    precision: int = K.PREC_FP64
+    mode: int = K.MODE_PERSISTENT
+    use_surrogate: int = 0
+    max_steps: int = 200000
+    rtol: float = 1e-10
+    atol: float = 1e-12
+    rtol32: float = 1e-6
+    atol32: float = 1e-7
+    restep_drift: float = 1e-6
+    block: int = 0
+    waves_per_cu: int = 0
+    steps_per_dequeue: int = 0
+    gate_prob: float = 0.999
+    gate_ring: float = 0.03
+
+    def to_c(self) -> K.KtTraceOpts:
+        o = K.KtTraceOpts()
+        for f in dataclasses.fields(self):
+            setattr(o, f.name, getattr(self, f.name))
+        return o
+
+
+def _check(ctx, rc: int, what: str):
+    if rc != K.KT_OK:
+        msg = K.lib().kt_last_error_string(ctx)
+        raise RuntimeError(f"{what} failed (status {rc}): {msg.decode() if msg else ''}")
+
+
+class Tracer:
+    """A library context (kt_ctx) on one GPU."""
+
+    def __init__(self, device: int = 0):
+        import torch  # device memory and streams only
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("no GPU: the Kerr tracer runs only on the HIP device")
+        self.torch = torch
+        self.device = device
+        self._ctx = C.c_void_p()
+        opts = K.KtDeviceOpts(device)
+        rc = K.lib().kt_create(C.byref(opts), C.byref(self._ctx))
+        if rc != K.KT_OK:
+            raise RuntimeError(f"kt_create failed with status {rc}")
+
+    def close(self):
+        if self._ctx:
+            K.lib().kt_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- device outputs -------------------------------------------------------------
+    def alloc_outputs(self, n: int) -> dict:
+        t = self.torch
+        dev = t.device("cuda", self.device)
+        return {
+            "cls": t.empty(n, dtype=t.int8, device=dev),
+            "theta_inf": t.empty(n, dtype=t.float64, device=dev),
+            "phi_inf": t.empty(n, dtype=t.float64, device=dev),
+            "r_hit": t.empty(n, dtype=t.float64, device=dev),
+            "g": t.empty(n, dtype=t.float64, device=dev),
+            "steps": t.empty(n, dtype=t.int32, device=dev),
+        }
+
+    @staticmethod
+    def _out_struct(out: dict) -> K.KtRaysOut:
+        s = K.KtRaysOut()
+        for name in OUTPUT_FIELDS:
+            tens = out.get(name)
+            setattr(s, name, tens.data_ptr() if tens is not None else None)
+        s.on_device = 1
+        return s
+
+    def trace_views(self, views: Sequence[View], opts: TraceOpts | None = None, out: dict | None = None,
+                    stream=None) -> dict:
+        """Trace every pixel of `views` in one launch; returns device tensors (views concatenated)."""
+        n = sum(v.rays for v in views)
+        if out is None:
+            out = self.alloc_outputs(n)
+        arr = (K.KtView * len(views))(*[v.to_c() for v in views])
+        o = (opts or TraceOpts()).to_c()
+        s = self._out_struct(out)
+        st = stream if stream is not None else self.torch.cuda.current_stream(self.device)
+        rc = K.lib().kt_trace_views(self._ctx, arr, len(views), C.byref(o), C.byref(s), C.c_void_p(st.cuda_stream))
+        _check(self._ctx, rc, "kt_trace_views")
+        return out
+
+    def trace_params(self, view: View, params, opts: TraceOpts | None = None, out: dict | None = None,
+                     stream=None) -> dict:
+        """Explicit rays: (alpha, beta) rows for a Bardeen view, unit looks for a pinhole view."""
+        t = self.torch
+        per = 2 if view.camera_kind == K.CAMERA_BARDEEN else 3
+        p = t.as_tensor(np.asarray(params, dtype=np.float64).reshape(-1, per)) if not isinstance(params, t.Tensor) \
+            else params.to(t.float64).reshape(-1, per)
+        p = p.to(f"cuda:{self.device}").contiguous()
+        n = p.shape[0]
+        if out is None:
+            out = self.alloc_outputs(n)
+        o = (opts or TraceOpts()).to_c()
+        s = self._out_struct(out)
+        v = view.to_c()
+        st = stream if stream is not None else t.cuda.current_stream(self.device)
+        rc = K.lib().kt_trace_rays(self._ctx, C.byref(v), C.c_void_p(p.data_ptr()), n, C.byref(o), C.byref(s),
+                                   C.c_void_p(st.cuda_stream))
+        _check(self._ctx, rc, "kt_trace_rays")
+        self._keep = p  # alive until the stream has used it
+        return out
+
+    def stats(self) -> dict:
+        s = K.KtStats()
+        _check(self._ctx, K.lib().kt_last_stats(self._ctx, C.byref(s)), "kt_last_stats")
+        return {f: getattr(s, f) for f, _ in K.KtStats._fields_}
+
+
+def to_numpy(out: dict) -> dict:
+    return {k: v.cpu().numpy() for k, v in out.items()}
+
+
+_default_tracer: Tracer | None = None
+
+
+def _tracer() -> Tracer:
+    global _default_tracer
+    if _default_tracer is None:
+        _default_tracer = Tracer()
+    return _default_tracer
+
+
+def render(view: View, opts: TraceOpts | None = None) -> dict:
+    """eht::render counterpart: every pixel of one view, as host numpy arrays reshaped to
+    (height, width), row-major from the top."""
+    out = to_numpy(_tracer().trace_views([view], opts))
+    return {k: v.reshape(view.height, view.width) for k, v in out.items()}
+
+
+def trace_ray(view: View, alpha: float, beta: float, opts: TraceOpts | None = None) -> dict:
+    """eht::traceRay counterpart for one (alpha, beta) of a Bardeen view."""
+    out = to_numpy(_tracer().trace_params(view, [[alpha, beta]], opts))
+    return {k: v[0].item() for k, v in out.items()}
+
+
+def trace_looks(view: View, looks: Iterable[Sequence[float]], opts: TraceOpts | None = None) -> dict:
+    """trajectory::traceRay counterpart: unit look vectors of a pinhole view's ZAMO frame."""
+    return to_numpy(_tracer().trace_params(view, np.asarray(list(looks), dtype=np.float64), opts))
+
+
+def horizon_radius(a: float) -> float:
+    return K.lib().kt_horizon_radius(a)
+
+
+def isco_radius(a: float, prograde: bool = True) -> float:
+    return K.lib().kt_isco_radius(a, int(prograde))
+
+
+def photon_orbit_radius(a: float, prograde: bool = True) -> float:
+    return K.lib().kt_photon_orbit_radius(a, int(prograde))
+
+
+def circular_omega(a: float, r: float, prograde: bool = True) -> float:
+    return K.lib().kt_circular_omega(a, r, int(prograde))
+
+
+def critical_curve(a: float, inclination: float, samples: int = 400):
+    cap = 16 * max(samples, 16) + 8 + samples
+    al = np.zeros(cap)
+    be = np.zeros(cap)
+    dp = C.POINTER(C.c_double)
+    n = K.lib().kt_critical_curve(a, inclination, samples, al.ctypes.data_as(dp), be.ctypes.data_as(dp), cap)
+    return al[:n].copy(), be[:n].copy()
+
+
+def critical_radius(alpha, beta, psi: float) -> float:
+    al = np.ascontiguousarray(alpha, dtype=np.float64)
+    be = np.ascontiguousarray(beta, dtype=np.float64)
+    dp = C.POINTER(C.c_double)
+    return K.lib().kt_critical_radius(al.ctypes.data_as(dp), be.ctypes.data_as(dp), len(al), psi)
+
+
+def shadow_radius(a: float, inclination: float, samples: int = 400, directions: int = 72) -> float:
+    """Mean critical radius over `directions` angles about the image origin
+    (as src/eht_view.cpp:101 sizes its field)."""
+    al, be = critical_curve(a, inclination, samples)
+    return float(np.mean([critical_radius(al, be, 2 * math.pi * k / directions) for k in range(directions)]))

@@ -1,0 +1,618 @@
+// kt_host.cpp -- host side of libkerr_trace.so: the C ABI of include/kerr_trace.h.
+//
+// Prepares per-view constants in FP64 (trig, ZAMO frame, camera basis), owns the
+// device scratch (work queue head, counters, view table, staging for host
+// outputs), launches the trace kernels on the caller's stream and times them
+// with HIP events on that same stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/kerr_trace.h"
+#include "kt_internal.hpp"
+#include "kt_surrogate.hpp"
+
+using ktd::ViewDev;
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// ---- closed forms (src/kerr.cpp:157-206, 296-299) ----
+
+double horizon(double a) {
+    if (std::fabs(a) > 1.0) return 0.0;
+    return 1.0 + std::sqrt(std::fmax(0.0, 1.0 - a * a));
+}
+
+double isco(double a, int prograde) {
+    double z1 = 1.0 + std::cbrt(1.0 - a * a) * (std::cbrt(1.0 + a) + std::cbrt(1.0 - a));
+    double z2 = std::sqrt(3.0 * a * a + z1 * z1);
+    double root = std::sqrt(std::fmax(0.0, (3.0 - z1) * (3.0 + z1 + 2.0 * z2)));
+    return prograde ? 3.0 + z2 - root : 3.0 + z2 + root;
+}
+
+double photon_orbit(double a, int prograde) {
+    double s = prograde ? -1.0 : 1.0;
+    return 2.0 * (1.0 + std::cos(2.0 / 3.0 * std::acos(s * a)));
+}
+
+// ---- Kerr-Schild metric and the ZAMO frame (src/kerr.cpp:238-270, 424-456) ----
+
+void metric_tensor(double a, double r, double theta, double g[4][4]) {
+    const double s = std::sin(theta), c = std::cos(theta);
+    const double s2 = s * s;
+    const double sigma = r * r + a * a * c * c;
+    const double rra = r * r + a * a;
+    const double A = rra * rra - a * a * (r * r - 2.0 * r + a * a) * s2;
+    const double f = 2.0 * r / sigma;
+    std::memset(g, 0, 16 * sizeof(double));
+    g[0][0] = -(1.0 - f);
+    g[0][1] = g[1][0] = f;
+    g[0][3] = g[3][0] = -f * a * s2;
+    g[1][1] = 1.0 + f;
+    g[1][3] = g[3][1] = -a * s2 * (1.0 + f);
+    g[2][2] = sigma;
+    g[3][3] = A * s2 / sigma;
+}
+
+double dot4(const double g[4][4], const double* x, const double* y) {
+    double sum = 0.0;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) sum += g[i][j] * x[i] * y[j];
+    return sum;
+}
+
+void lower4(const double g[4][4], const double* x, double* out) {
+    for (int i = 0; i < 4; ++i) {
+        out[i] = 0.0;
+        for (int j = 0; j < 4; ++j) out[i] += g[i][j] * x[j];
+    }
+}
+
+void zamo_u(double a, double r, double theta, double u[4]) {
+    const double s = std::sin(theta), c = std::cos(theta);
+    const double sigma = r * r + a * a * c * c;
+    const double delta = r * r - 2.0 * r + a * a;
+    const double rra = r * r + a * a;
+    const double A = rra * rra - a * a * delta * s * s;
+    const double omega = 2.0 * a * r / A;
+    const double ut = std::sqrt(A / (sigma * delta));
+    u[0] = ut;
+    u[1] = 0.0;
+    u[2] = 0.0;
+    u[3] = omega * ut;
+}
+
+void observer_frame(double a, double r, double theta, const double u[4], double e[4][4]) {
+    double g[4][4];
+    metric_tensor(a, r, theta, g);
+    static const double cand[3][4] = {{0, 0, 1, 0}, {0, 0, 0, 1}, {0, 1, 0, 0}};
+    static const int slot[3] = {2, 3, 1};
+    for (int i = 0; i < 4; ++i) e[0][i] = u[i];
+    for (int k = 0; k < 3; ++k) {
+        double w[4];
+        for (int i = 0; i < 4; ++i) w[i] = cand[k][i];
+        const double along = dot4(g, w, u);
+        for (int i = 0; i < 4; ++i) w[i] += along * u[i];
+        for (int j = 0; j < k; ++j) {
+            const double* ej = e[slot[j]];
+            const double p = dot4(g, w, ej);
+            for (int i = 0; i < 4; ++i) w[i] -= p * ej[i];
+        }
+        const double n = std::sqrt(std::fmax(dot4(g, w, w), 1e-300));
+        for (int i = 0; i < 4; ++i) w[i] /= n;
+        for (int i = 0; i < 4; ++i) e[slot[k]][i] = w[i];
+    }
+}
+
+// camera basis, src/camera.cpp:6-21 (FP64): columns right, up, forward on (r, theta, phi)
+void camera_basis(double yaw, double pitch, double roll, double B[9]) {
+    const double f0[3] = {-1.0, 0.0, 0.0}, u0[3] = {0.0, -1.0, 0.0}, r0[3] = {0.0, 0.0, 1.0};
+    const double cy = std::cos(yaw), sy = std::sin(yaw), cp = std::cos(pitch), sp = std::sin(pitch);
+    const double cr = std::cos(roll), sr = std::sin(roll);
+    double yawed[3], right[3], fwd[3], up[3];
+    for (int i = 0; i < 3; ++i) {
+        yawed[i] = cy * f0[i] + sy * r0[i];
+        right[i] = cy * r0[i] - sy * f0[i];
+    }
+    for (int i = 0; i < 3; ++i) fwd[i] = cp * yawed[i] + sp * u0[i];
+    up[0] = right[1] * fwd[2] - right[2] * fwd[1];
+    up[1] = right[2] * fwd[0] - right[0] * fwd[2];
+    up[2] = right[0] * fwd[1] - right[1] * fwd[0];
+    for (int i = 0; i < 3; ++i) {
+        B[0 + i] = cr * right[i] + sr * up[i];
+        B[3 + i] = cr * up[i] - sr * right[i];
+        B[6 + i] = fwd[i];
+    }
+}
+
+const char* check_view(const kt_view& v) {
+    if (v.camera_kind != KT_CAMERA_BARDEEN && v.camera_kind != KT_CAMERA_PINHOLE) return "unknown camera_kind";
+    if (v.width <= 0 || v.height <= 0) return "image size must be positive";
+    if ((long long)v.width * v.height > (1ll << 31)) return "view larger than 2^31 pixels";
+    if (!(std::fabs(v.spin) <= 1.0)) return "spin must lie in [-1, 1] (naked singularities are not traced)";
+    if (!(v.r_cam > horizon(v.spin))) return "camera must be outside the horizon";
+    if (!(v.theta_cam >= 0.0 && v.theta_cam <= kPi)) return "theta_cam must lie in [0, pi]";
+    if (!(v.capture_factor >= 1.0)) return "capture_factor must be >= 1";
+    if (v.camera_kind == KT_CAMERA_BARDEEN && !(v.half_width > 0.0 && v.half_height > 0.0))
+        return "Bardeen half widths must be positive";
+    if (v.camera_kind == KT_CAMERA_PINHOLE && !(v.fov_deg > 0.0 && v.fov_deg < 180.0)) return "fov out of range";
+    if (v.disk_enabled && !(v.disk_outer > 0.0)) return "disk_outer must be positive";
+    return nullptr;
+}
+
+ViewDev make_view_dev(const kt_view& v, long long offset) {
+    ViewDev d;
+    std::memset(&d, 0, sizeof d);
+    const double a = v.spin;
+    d.offset = offset;
+    d.kind = v.camera_kind;
+    d.width = v.width;
+    d.height = v.height;
+    d.disk = v.disk_enabled ? 1 : 0;
+    d.prograde = v.disk_prograde ? 1 : 0;
+    d.a = a;
+    d.r_cam = v.r_cam;
+    d.phi_cam = v.phi_cam;
+    d.half_width = v.half_width;
+    d.half_height = v.half_height;
+    d.px = 2.0 * v.half_width / (double)v.width;
+    d.py = 2.0 * v.half_height / (double)v.height;
+    d.sinI = std::sin(v.theta_cam);
+    d.cosI = std::cos(v.theta_cam);
+    double th = v.theta_cam;
+    if (th < 1e-4) th = 1e-4;
+    if (th > kPi - 1e-4) th = kPi - 1e-4;
+    d.m0 = std::cos(th);
+    d.r_cap = v.capture_factor * horizon(a);
+    d.r_esc = v.escape_radius > 1.5 * v.r_cam ? v.escape_radius : 1.5 * v.r_cam;
+    d.r_in = v.disk_inner > 0.0 ? v.disk_inner : isco(a, v.disk_prograde);
+    d.r_out = v.disk_outer;
+    d.h0 = 0.01 / v.r_cam;
+    if (v.camera_kind == KT_CAMERA_PINHOLE) {
+        const double r = v.r_cam, t = v.theta_cam;
+        double u[4], e[4][4], g[4][4], low[4][4];
+        zamo_u(a, r, t, u);
+        observer_frame(a, r, t, u, e);
+        metric_tensor(a, r, t, g);
+        for (int j = 0; j < 4; ++j) lower4(g, e[j], low[j]);
+        for (int j = 0; j < 4; ++j) {
+            d.lt[j] = low[j][0];
+            d.lp[j] = low[j][3];
+            d.lh[j] = low[j][2];
+            d.er[j] = e[j][1];
+        }
+        camera_basis(v.yaw, v.pitch, v.roll, d.B);
+        d.tanh_fov = std::tan(0.5 * v.fov_deg * kPi / 180.0);
+        d.sn = std::sin(t);
+        d.cs = std::cos(t);
+        d.sigma = r * r + a * a * d.cs * d.cs;
+    }
+    d.params = nullptr;
+    return d;
+}
+
+}  // namespace
+
+struct kt_ctx {
+    int device = 0;
+    int cu_count = 0;
+    std::string err;
+    kt_stats stats{};
+    unsigned long long* d_counter = nullptr;  // [0] queue head, [1..] stats
+    ViewDev* d_views = nullptr;
+    int views_cap = 0;
+    void* d_stage = nullptr;  // staging for host outputs
+    size_t stage_cap = 0;
+    void* d_params = nullptr;  // staging for host explicit-ray parameters
+    size_t params_cap = 0;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool timed = false;
+    kts::Surrogate* surrogate = nullptr;
+};
+
+namespace {
+
+int fail(kt_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(kt_ctx* c, hipError_t e, const char* where) {
+    return fail(c, e == hipErrorOutOfMemory ? KT_ENOMEM : KT_EHIP,
+                std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define KT_HIP(call)                                    \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+    } while (0)
+
+void fill_defaults(const kt_trace_opts* in, kt_trace_opts& o) {
+    kt_trace_opts_default(&o);
+    if (!in) return;
+    o = *in;
+    kt_trace_opts d;
+    kt_trace_opts_default(&d);
+    if (o.max_steps <= 0) o.max_steps = d.max_steps;
+    if (!(o.rtol > 0.0)) o.rtol = d.rtol;
+    if (!(o.atol > 0.0)) o.atol = d.atol;
+    if (!(o.rtol32 > 0.0)) o.rtol32 = d.rtol32;
+    if (!(o.atol32 > 0.0)) o.atol32 = d.atol32;
+    if (!(o.restep_drift > 0.0)) o.restep_drift = d.restep_drift;
+    if (!(o.gate_prob > 0.0)) o.gate_prob = d.gate_prob;
+    if (!(o.gate_ring > 0.0)) o.gate_ring = d.gate_ring;
+}
+
+int ensure_stage(kt_ctx* ctx, size_t bytes) {
+    if (bytes <= ctx->stage_cap) return KT_OK;
+    if (ctx->d_stage) (void)hipFree(ctx->d_stage);
+    ctx->d_stage = nullptr;
+    ctx->stage_cap = 0;
+    size_t cap = std::max(bytes, (size_t)1 << 20);
+    KT_HIP(hipMalloc(&ctx->d_stage, cap));
+    ctx->stage_cap = cap;
+    return KT_OK;
+}
+
+int ensure_views(kt_ctx* ctx, int n) {
+    if (n <= ctx->views_cap) return KT_OK;
+    if (ctx->d_views) (void)hipFree(ctx->d_views);
+    ctx->d_views = nullptr;
+    ctx->views_cap = 0;
+    int cap = std::max(n, 64);
+    KT_HIP(hipMalloc(&ctx->d_views, sizeof(ViewDev) * (size_t)cap));
+    ctx->views_cap = cap;
+    return KT_OK;
+}
+
+// tuned defaults for one MI355X (see the autotuner in black_simulation_amd/autotune.py)
+void launch_shape(const kt_ctx* ctx, const kt_trace_opts& o, int& block, int& grid, int& spd) {
+    block = o.block > 0 ? o.block : 256;
+    const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : (o.precision == KT_PREC_FP64 ? 8 : 16);
+    spd = o.steps_per_dequeue > 0 ? o.steps_per_dequeue : 4;
+    const int waves = std::max(1, ctx->cu_count) * wpc;
+    grid = std::max(1, (waves * 64 + block - 1) / block);
+}
+
+int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const kt_trace_opts& o,
+                 const kt_rays_out* out, hipStream_t stream) {
+    KT_HIP(hipSetDevice(ctx->device));
+    if (ensure_views(ctx, (int)vd.size()) != KT_OK) return KT_ENOMEM;
+
+    // outputs: device pointers as given, or staging for host pointers
+    ktd::Outputs dout{};
+    size_t off[6] = {0, 0, 0, 0, 0, 0};
+    if (out->on_device) {
+        dout.cls = out->cls;
+        dout.theta = out->theta_inf;
+        dout.phi = out->phi_inf;
+        dout.r_hit = out->r_hit;
+        dout.g = out->g;
+        dout.steps = out->steps;
+    } else {
+        size_t bytes = 0;
+        const size_t n = (size_t)total;
+        auto take = [&](size_t each) {
+            size_t at = (bytes + 255) & ~(size_t)255;
+            bytes = at + each * n;
+            return at;
+        };
+        off[0] = out->cls ? take(1) : 0;
+        off[1] = out->theta_inf ? take(8) : 0;
+        off[2] = out->phi_inf ? take(8) : 0;
+        off[3] = out->r_hit ? take(8) : 0;
+        off[4] = out->g ? take(8) : 0;
+        off[5] = out->steps ? take(4) : 0;
+        if (ensure_stage(ctx, bytes + 256) != KT_OK) return KT_ENOMEM;
+        char* s = static_cast<char*>(ctx->d_stage);
+        if (out->cls) dout.cls = reinterpret_cast<int8_t*>(s + off[0]);
+        if (out->theta_inf) dout.theta = reinterpret_cast<double*>(s + off[1]);
+        if (out->phi_inf) dout.phi = reinterpret_cast<double*>(s + off[2]);
+        if (out->r_hit) dout.r_hit = reinterpret_cast<double*>(s + off[3]);
+        if (out->g) dout.g = reinterpret_cast<double*>(s + off[4]);
+        if (out->steps) dout.steps = reinterpret_cast<int32_t*>(s + off[5]);
+    }
+
+    KT_HIP(hipMemcpyAsync(ctx->d_views, vd.data(), sizeof(ViewDev) * vd.size(), hipMemcpyHostToDevice, stream));
+
+    kti::TraceLaunch L{};
+    L.views = ctx->d_views;
+    L.n_views = (int)vd.size();
+    L.total = total;
+    L.counter = ctx->d_counter;
+    L.stats = ctx->d_counter + 1;
+    L.out = dout;
+    L.ray_list = nullptr;
+    L.rtol = o.rtol;
+    L.atol = o.atol;
+    L.rtol32 = o.rtol32;
+    L.atol32 = o.atol32;
+    L.restep_drift = o.restep_drift;
+    L.max_steps = o.max_steps;
+    L.precision = o.precision;
+    L.mode = o.mode;
+    launch_shape(ctx, o, L.block, L.grid, L.steps_per_dequeue);
+
+    ctx->stats = kt_stats{};
+    ctx->stats.rays = total;
+    ctx->stats.block = L.block;
+    ctx->stats.grid = L.grid;
+    ctx->stats.steps_per_dequeue = L.steps_per_dequeue;
+    ctx->stats.waves_per_cu = (L.grid * L.block / 64) / std::max(1, ctx->cu_count);
+
+    if (o.use_surrogate) {
+        if (!ctx->surrogate) return fail(ctx, KT_EUNSUPPORTED, "use_surrogate set but no surrogate is loaded");
+        KT_HIP(hipEventRecord(ctx->ev[0], stream));
+        hipError_t e = kts::trace_with_surrogate(ctx->surrogate, L, o.gate_prob, o.gate_ring, stream, ctx->ev);
+        if (e != hipSuccess) return hip_fail(ctx, e, "surrogate trace");
+        KT_HIP(hipEventRecord(ctx->ev[3], stream));
+    } else {
+        KT_HIP(hipEventRecord(ctx->ev[0], stream));
+        KT_HIP(hipEventRecord(ctx->ev[1], stream));
+        hipError_t e = kti::launch_trace(L, stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "trace kernel launch");
+        KT_HIP(hipEventRecord(ctx->ev[2], stream));
+        KT_HIP(hipEventRecord(ctx->ev[3], stream));
+    }
+    ctx->timed = true;
+
+    if (!out->on_device) {
+        const size_t n = (size_t)total;
+        if (out->cls) KT_HIP(hipMemcpyAsync(out->cls, dout.cls, n, hipMemcpyDeviceToHost, stream));
+        if (out->theta_inf) KT_HIP(hipMemcpyAsync(out->theta_inf, dout.theta, 8 * n, hipMemcpyDeviceToHost, stream));
+        if (out->phi_inf) KT_HIP(hipMemcpyAsync(out->phi_inf, dout.phi, 8 * n, hipMemcpyDeviceToHost, stream));
+        if (out->r_hit) KT_HIP(hipMemcpyAsync(out->r_hit, dout.r_hit, 8 * n, hipMemcpyDeviceToHost, stream));
+        if (out->g) KT_HIP(hipMemcpyAsync(out->g, dout.g, 8 * n, hipMemcpyDeviceToHost, stream));
+        if (out->steps) KT_HIP(hipMemcpyAsync(out->steps, dout.steps, 4 * n, hipMemcpyDeviceToHost, stream));
+        KT_HIP(hipStreamSynchronize(stream));
+    }
+    return KT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void kt_trace_opts_default(kt_trace_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->precision = KT_PREC_FP64;
+    o->mode = KT_MODE_PERSISTENT;
+    o->use_surrogate = 0;
+    o->max_steps = 200000;
+    o->rtol = 1e-10;
+    o->atol = 1e-12;
+    o->rtol32 = 1e-6;
+    o->atol32 = 1e-7;
+    o->restep_drift = 1e-6;
+    o->gate_prob = 0.999;
+    o->gate_ring = 0.03;
+}
+
+void kt_view_default(kt_view* v) {
+    if (!v) return;
+    std::memset(v, 0, sizeof *v);
+    v->camera_kind = KT_CAMERA_BARDEEN;
+    v->width = v->height = 256;
+    v->spin = 0.9;
+    v->r_cam = 30.0;
+    v->theta_cam = 80.0 * kPi / 180.0;
+    v->half_width = v->half_height = 12.0;
+    v->fov_deg = 60.0;
+    v->disk_inner = 0.0;
+    v->disk_outer = 20.0;
+    v->disk_prograde = 1;
+    v->disk_enabled = 1;
+    v->escape_radius = 1000.0;
+    v->capture_factor = 1.01;
+}
+
+int kt_create(const kt_device_opts* opts, kt_ctx** out) {
+    if (!out) return KT_EINVAL;
+    *out = nullptr;
+    kt_ctx* ctx = new (std::nothrow) kt_ctx();
+    if (!ctx) return KT_ENOMEM;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        delete ctx;
+        return KT_ENODEVICE;
+    }
+    ctx->device = opts ? opts->device : 0;
+    if (ctx->device < 0 || ctx->device >= n) {
+        delete ctx;
+        return KT_EINVAL;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        delete ctx;
+        return KT_EHIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) ctx->cu_count = prop.multiProcessorCount;
+    if (hipMalloc(&ctx->d_counter, sizeof(unsigned long long) * (1 + kti::kStatCount)) != hipSuccess) {
+        delete ctx;
+        return KT_ENOMEM;
+    }
+    for (auto& e : ctx->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            kt_destroy(ctx);
+            return KT_EHIP;
+        }
+    }
+    *out = ctx;
+    return KT_OK;
+}
+
+int kt_destroy(kt_ctx* ctx) {
+    if (!ctx) return KT_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    if (ctx->surrogate) kts::destroy(ctx->surrogate);
+    if (ctx->d_counter) (void)hipFree(ctx->d_counter);
+    if (ctx->d_views) (void)hipFree(ctx->d_views);
+    if (ctx->d_stage) (void)hipFree(ctx->d_stage);
+    if (ctx->d_params) (void)hipFree(ctx->d_params);
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete ctx;
+    return KT_OK;
+}
+
+const char* kt_last_error_string(const kt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int kt_last_stats(const kt_ctx* ctx_c, kt_stats* s) {
+    kt_ctx* ctx = const_cast<kt_ctx*>(ctx_c);
+    if (!ctx || !s) return KT_EINVAL;
+    if (ctx->timed) {
+        KT_HIP(hipEventSynchronize(ctx->ev[3]));
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]) == hipSuccess) ctx->stats.kernel_ms = ms;
+        if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) == hipSuccess) ctx->stats.integrator_ms = ms;
+        ctx->stats.surrogate_ms = ctx->stats.kernel_ms - ctx->stats.integrator_ms;
+        unsigned long long h[1 + kti::kStatCount];
+        KT_HIP(hipMemcpy(h, ctx->d_counter, sizeof h, hipMemcpyDeviceToHost));
+        ctx->stats.steps = (long long)h[1 + kti::kStatSteps];
+        ctx->stats.restep_rays = (long long)h[1 + kti::kStatRestepRays];
+        ctx->stats.fallback_rays = (long long)h[1 + kti::kStatFallback];
+        ctx->timed = false;
+    }
+    *s = ctx->stats;
+    return KT_OK;
+}
+
+int kt_trace_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trace_opts* opts,
+                   const kt_rays_out* out, void* hip_stream) {
+    if (!ctx) return KT_EINVAL;
+    try {
+        if (!views || n_views <= 0 || !out) return fail(ctx, KT_EINVAL, "views/out missing");
+        kt_trace_opts o;
+        fill_defaults(opts, o);
+        if (o.precision < 0 || o.precision > 2) return fail(ctx, KT_EINVAL, "unknown precision");
+        std::vector<ViewDev> vd;
+        vd.reserve(n_views);
+        long long total = 0;
+        for (int i = 0; i < n_views; ++i) {
+            if (const char* why = check_view(views[i]))
+                return fail(ctx, KT_EINVAL, "view " + std::to_string(i) + ": " + why);
+            vd.push_back(make_view_dev(views[i], total));
+            total += (long long)views[i].width * views[i].height;
+        }
+        return trace_common(ctx, vd, total, o, out, static_cast<hipStream_t>(hip_stream));
+    } catch (const std::exception& e) {
+        return fail(ctx, KT_EINVAL, e.what());
+    } catch (...) {
+        return fail(ctx, KT_EINVAL, "unknown error");
+    }
+}
+
+int kt_trace_rays(kt_ctx* ctx, const kt_view* view, const double* params, long long n, const kt_trace_opts* opts,
+                  const kt_rays_out* out, void* hip_stream) {
+    if (!ctx) return KT_EINVAL;
+    try {
+        if (!view || !params || n <= 0 || !out) return fail(ctx, KT_EINVAL, "view/params/out missing");
+        kt_view v = *view;
+        // explicit rays: the image size only has to be valid
+        v.width = 1;
+        v.height = 1;
+        if (const char* why = check_view(v)) return fail(ctx, KT_EINVAL, why);
+        kt_trace_opts o;
+        fill_defaults(opts, o);
+        hipStream_t stream = static_cast<hipStream_t>(hip_stream);
+        std::vector<ViewDev> vd(1, make_view_dev(v, 0));
+        const int per = view->camera_kind == KT_CAMERA_BARDEEN ? 2 : 3;
+        if (out->on_device) {
+            vd[0].params = params;
+        } else {
+            KT_HIP(hipSetDevice(ctx->device));
+            const size_t bytes = (size_t)n * per * 8;
+            if (bytes > ctx->params_cap) {
+                if (ctx->d_params) (void)hipFree(ctx->d_params);
+                ctx->d_params = nullptr;
+                ctx->params_cap = 0;
+                KT_HIP(hipMalloc(&ctx->d_params, bytes));
+                ctx->params_cap = bytes;
+            }
+            KT_HIP(hipMemcpyAsync(ctx->d_params, params, bytes, hipMemcpyHostToDevice, stream));
+            vd[0].params = static_cast<const double*>(ctx->d_params);
+        }
+        vd[0].n_params = (int)std::min<long long>(n, 0x7fffffff);
+        vd[0].width = (int)std::min<long long>(n, 0x7fffffff);
+        vd[0].height = 1;
+        return trace_common(ctx, vd, n, o, out, stream);
+    } catch (const std::exception& e) {
+        return fail(ctx, KT_EINVAL, e.what());
+    } catch (...) {
+        return fail(ctx, KT_EINVAL, "unknown error");
+    }
+}
+
+double kt_horizon_radius(double a) { return horizon(a); }
+double kt_isco_radius(double a, int prograde) { return isco(a, prograde); }
+double kt_photon_orbit_radius(double a, int prograde) { return photon_orbit(a, prograde); }
+double kt_circular_omega(double a, double r, int prograde) {
+    double s = prograde ? 1.0 : -1.0;
+    return s / (r * std::sqrt(r) + s * a);
+}
+
+int kt_critical_curve(double a, double inclination, int samples, double* alpha, double* beta, int cap) {
+    if (!alpha || !beta || cap <= 0) return 0;
+    const double sinI = std::sin(inclination), cosI = std::cos(inclination);
+    if (std::fabs(a) < 1e-6) {
+        int n = 0;
+        for (int k = 0; k < samples && n < cap; ++k, ++n) {
+            const double psi = 2.0 * kPi * k / samples;
+            alpha[n] = std::sqrt(27.0) * std::cos(psi);
+            beta[n] = std::sqrt(27.0) * std::sin(psi);
+        }
+        return n;
+    }
+    const double r1 = photon_orbit(a, 1), r2 = photon_orbit(a, 0);
+    const int steps = std::max(samples, 16) * 8;
+    std::vector<double> ua, ub;
+    for (int k = 0; k <= steps; ++k) {
+        const double x = 0.5 - 0.5 * std::cos(kPi * k / steps);
+        const double r = r1 + (r2 - r1) * x;
+        const double xi = (r * r * (3.0 - r) - a * a * (r + 1.0)) / (a * (r - 1.0));
+        const double eta = r * r * r * (4.0 * a * a - r * (r - 3.0) * (r - 3.0)) / (a * a * (r - 1.0) * (r - 1.0));
+        const double b2 = eta + a * a * cosI * cosI - xi * xi * cosI * cosI / (sinI * sinI);
+        if (b2 < 0.0) continue;
+        ua.push_back(-xi / sinI);
+        ub.push_back(std::sqrt(b2));
+    }
+    int n = 0;
+    for (size_t k = 0; k < ua.size() && n < cap; ++k, ++n) {
+        alpha[n] = ua[k];
+        beta[n] = ub[k];
+    }
+    for (size_t k = ua.size(); k-- > 0 && n < cap; ++n) {
+        alpha[n] = ua[k];
+        beta[n] = -ub[k];
+    }
+    return n;
+}
+
+double kt_critical_radius(const double* alpha, const double* beta, int n, double psi) {
+    if (!alpha || !beta || n <= 0) return 0.0;
+    const double dx = std::cos(psi), dy = std::sin(psi);
+    double best = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const int q = (k + 1) % n;
+        const double ex = alpha[q] - alpha[k], ey = beta[q] - beta[k];
+        const double det = ex * dy - ey * dx;
+        if (std::fabs(det) < 1e-15) continue;
+        const double t = (beta[k] * dx - alpha[k] * dy) / det;
+        const double s = (ex * beta[k] - ey * alpha[k]) / det;
+        if (t >= 0.0 && t <= 1.0 && s > 0.0 && s > best) best = s;
+    }
+    return best;
+}
+
+}  // extern "C"

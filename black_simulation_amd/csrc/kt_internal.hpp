@@ -1,0 +1,36 @@
+// kt_internal.hpp -- what the host side (kt_host.cpp) and the kernels (kt_trace.hip) share.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kt_device.hpp"
+
+namespace kti {
+
+constexpr int kMaxLdsViews = 64;
+
+// device-side counters written by the trace kernels
+enum : int { kStatSteps = 0, kStatRestepRays = 1, kStatFallback = 2, kStatCount = 4 };
+
+struct TraceLaunch {
+    const ktd::ViewDev* views;  // device array
+    int n_views;
+    long long total;            // rays in the batch
+    unsigned long long* counter;  // work-queue head (zeroed by the launch)
+    unsigned long long* stats;    // kStatCount counters (zeroed by the launch)
+    ktd::Outputs out;
+    const long long* ray_list;  // optional: trace only these ray indices (fallback re-queue)
+    double rtol, atol, rtol32, atol32, restep_drift;
+    int max_steps;
+    int precision;              // KT_PREC_*
+    int mode;                   // KT_MODE_*
+    int block, grid, steps_per_dequeue;
+};
+
+// launches the counter reset and the trace kernel on `stream`; returns a hipError_t
+hipError_t launch_trace(const TraceLaunch& L, hipStream_t stream);
+
+// registers per lane etc. of the trace kernels, for the autotuner (0 if unknown)
+int trace_kernel_vgprs(int precision);
+
+}  // namespace kti

@@ -1,0 +1,165 @@
+/*
+ * kerr_trace.h -- C ABI of the MI355X Kerr photon tracer (libkerr_trace.so).
+ *
+ * A drop-in for the Black_simulation per-pixel photon trace: camera pose, spin,
+ * disk and escape radii and image size go in; per-ray termination class,
+ * asymptotic sky angles, disk-hit radius and redshift factor g come out.
+ *
+ * The reference has no plugin or FFI layer; the trace is reached through three
+ * in-process C++ calls, which these entry points replace:
+ *   kt_trace_views  <- eht::render            (src/eht.h:140, src/eht.cpp:321-347)
+ *                      and the trace pass of Renderer::render (src/renderer.cpp:283-379,
+ *                      shaders/trace.frag:1490-1800)
+ *   kt_trace_rays   <- eht::traceRay          (src/eht.h:130, src/eht.cpp:260-319)
+ *                      trajectory::traceRay   (src/trajectory.h:118-119) for a look list
+ *   kt_horizon_radius, kt_isco_radius, kt_photon_orbit_radius, kt_circular_omega
+ *                   <- kerr::horizonRadius, iscoRadius, photonOrbitRadius,
+ *                      circularOmega          (src/kerr.h:39-45, 71)
+ *   kt_critical_curve, kt_critical_radius
+ *                   <- eht::criticalCurve, eht::criticalRadius (src/eht.h:66-68)
+ *
+ * Conventions
+ *   - Units G = c = M = 1.  Angles in radians except fov_deg.
+ *   - Sky angles are Boyer-Lindquist (theta, phi) of the ray's asymptotic direction
+ *     at r -> infinity, traced backward from the camera (phi in [0, 2 pi)).
+ *   - Outputs are structure-of-arrays, views concatenated in order, each view
+ *     row-major from the top row (beta = +half_height), as eht::Image.
+ *   - Every call returns a kt_status; per-ray anomalies are classes, never errors.
+ *     No C++ exception crosses this boundary.
+ *   - A context is externally synchronised: one call at a time per context, on the
+ *     caller's HIP stream (NULL = the default stream).  Several contexts may coexist.
+ */
+#ifndef KERR_TRACE_H
+#define KERR_TRACE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum kt_status {
+    KT_OK = 0,
+    KT_EINVAL = 1,        /* bad argument (message in kt_last_error_string) */
+    KT_EHIP = 2,          /* a HIP runtime call failed                      */
+    KT_ECANCELLED = 3,    /* the cancel flag was set                        */
+    KT_ENOMEM = 4,        /* device allocation failed                       */
+    KT_EUNSUPPORTED = 5,  /* feature not available (e.g. no surrogate)      */
+    KT_ENODEVICE = 6      /* no usable GPU                                  */
+} kt_status;
+
+/* per-ray class codes (low 3 bits of cls) -- trajectory::RayEnd (src/trajectory.h:85) */
+enum {
+    KT_ESCAPED = 0,    /* RayEnd::Sky                                   */
+    KT_CAPTURED = 1,   /* RayEnd::Horizon: r < capture_factor * r_+       */
+    KT_DISK = 2,       /* RayEnd::Disk                                  */
+    KT_UNFINISHED = 3  /* RayEnd::Unfinished: step budget exhausted      */
+};
+/* flag bits OR-ed into cls */
+enum {
+    KT_FLAG_FP64_RESTEP = 0x10,  /* mixed path: at least one FP64 re-step          */
+    KT_FLAG_SURROGATE = 0x20     /* answered by the learned surrogate              */
+};
+
+enum { KT_CAMERA_BARDEEN = 0, KT_CAMERA_PINHOLE = 1 };
+enum { KT_PREC_FP64 = 0, KT_PREC_MIXED = 1, KT_PREC_FP32 = 2 };
+enum { KT_MODE_PERSISTENT = 0, KT_MODE_ONESHOT = 1 };
+
+/* One camera view.  Bardeen views (eht::View, src/eht.h:114-121) trace the image
+ * plane of a camera at (r_cam, theta_cam = inclination, phi_cam): pixel centres
+ * alpha = -half_width + (col + 0.5) 2 half_width / width, beta likewise from the top
+ * (src/eht.cpp:332-334).  Pinhole views are the simulator's camera
+ * (src/camera.h:20-29: pose, yaw/pitch/roll, vertical fov) in the ZAMO frame.
+ * Disk fields follow trajectory::RayScene (src/trajectory.h:74-83). */
+typedef struct kt_view {
+    int camera_kind;
+    int width, height;
+    double spin;
+    double r_cam, theta_cam, phi_cam;
+    double half_width, half_height;
+    double yaw, pitch, roll, fov_deg;
+    double disk_inner, disk_outer;  /* disk_inner <= 0: prograde/retrograde ISCO */
+    int disk_prograde, disk_enabled;
+    double escape_radius;           /* escape when r > max(escape_radius, 1.5 r_cam), outgoing */
+    double capture_factor;          /* captured when r < capture_factor * r_+ (e.g. 1.01) */
+} kt_view;
+
+typedef struct kt_trace_opts {
+    int precision;          /* KT_PREC_*                                            */
+    int mode;               /* KT_MODE_*                                            */
+    int use_surrogate;      /* 1: surrogate answers confident rays, others re-queued */
+    int max_steps;          /* per ray, accepted + rejected (default 200000)        */
+    double rtol, atol;      /* FP64 DP5 tolerances (default 1e-10, 1e-12)           */
+    double rtol32, atol32;  /* FP32 bulk tolerances (default 1e-6, 1e-7)            */
+    double restep_drift;    /* mixed: FP64 re-step when |H| drift > this (1e-6)     */
+    int block;              /* threads per block (0 = tuned)                        */
+    int waves_per_cu;       /* persistent grid size (0 = tuned)                     */
+    int steps_per_dequeue;  /* persistent refill period (0 = tuned)                 */
+    double gate_prob;       /* surrogate: min class probability (0.999)             */
+    double gate_ring;       /* surrogate: min |rho/rho_ring - 1| (0.03)             */
+} kt_trace_opts;
+
+/* SoA outputs; any pointer may be NULL.  on_device: 1 = device pointers
+ * (written on the call's stream), 0 = host pointers (the call synchronises). */
+typedef struct kt_rays_out {
+    int8_t* cls;
+    double* theta_inf;
+    double* phi_inf;
+    double* r_hit;
+    double* g;
+    int32_t* steps;
+    int on_device;
+} kt_rays_out;
+
+typedef struct kt_device_opts {
+    int device;  /* HIP device ordinal */
+} kt_device_opts;
+
+typedef struct kt_stats {
+    double kernel_ms;          /* device time of the last trace call's kernels (HIP events) */
+    double integrator_ms;      /* ... of its integrator kernel alone                        */
+    double surrogate_ms;       /* ... of its surrogate inference kernels                    */
+    long long rays;            /* rays traced                                               */
+    long long steps;           /* DP5 attempts, summed over rays                            */
+    long long fallback_rays;   /* surrogate path: rays re-queued to the integrator          */
+    long long restep_rays;     /* mixed path: rays with at least one FP64 re-step           */
+    int block, waves_per_cu, steps_per_dequeue, grid;
+} kt_stats;
+
+typedef struct kt_ctx kt_ctx;
+
+void kt_trace_opts_default(kt_trace_opts* opts);
+void kt_view_default(kt_view* view);
+
+int kt_create(const kt_device_opts* opts, kt_ctx** ctx);
+int kt_destroy(kt_ctx* ctx);
+const char* kt_last_error_string(const kt_ctx* ctx);
+int kt_last_stats(const kt_ctx* ctx, kt_stats* stats);
+
+/* Rays of all views in one launch (one device work queue). */
+int kt_trace_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trace_opts* opts,
+                   const kt_rays_out* out, void* hip_stream);
+
+/* Explicit rays of one view: for a Bardeen view `params` holds n (alpha, beta) pairs;
+ * for a pinhole view n unit look vectors (x, y, z) along the frame's (r, theta, phi)
+ * axes (the `look` of kerr::photonFromFrame, src/kerr.cpp:458).  params is a host
+ * pointer unless out->on_device, then a device pointer. */
+int kt_trace_rays(kt_ctx* ctx, const kt_view* view, const double* params, long long n,
+                  const kt_trace_opts* opts, const kt_rays_out* out, void* hip_stream);
+
+/* Host-side closed forms (src/kerr.cpp:157-206, 296-299). */
+double kt_horizon_radius(double a);
+double kt_isco_radius(double a, int prograde);
+double kt_photon_orbit_radius(double a, int prograde);
+double kt_circular_omega(double a, double r, int prograde);
+
+/* Critical curve of spin a at inclination i: writes up to cap points, returns the
+ * count (src/eht.cpp:52-80); radius of a closed curve along direction psi (:82-97). */
+int kt_critical_curve(double a, double inclination, int samples, double* alpha, double* beta, int cap);
+double kt_critical_radius(const double* alpha, const double* beta, int n, double psi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

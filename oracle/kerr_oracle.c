@@ -358,6 +358,7 @@ double ko_pow_m15(double x) {
     static const double T[5] = {1.0, 0.8705505632961241, 0.757858283255199, 0.6597539553864471,
                                 0.5743491774985174};
     int e;
+    if (!(x > 1e-200 && x < 1e200)) return x > 1.0 ? 0.0 : 1e40; /* inf/NaN/0: clamped by the caller */
     const double m = frexp(x, &e); /* x = m 2^e, m in [0.5, 1) */
     const int q = (e + 1100) / 5 - 220;
     const int s = e - 5 * q;

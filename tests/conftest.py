@@ -1,0 +1,21 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and libkerr_trace.so")
+
+
+@pytest.fixture(scope="session")
+def tracer():
+    import black_simulation_amd as bsa
+
+    tr = bsa.Tracer(0)
+    yield tr
+    tr.close()
