@@ -228,6 +228,25 @@ __device__ __forceinline__ R dense(const R y[5], const R yn[5], const R k[7][5],
     return y[i] + th * (ydiff + th1 * (bspl + th * (c4 + th1 * c5)));
 }
 
+// Constraint projection, src/kerr.cpp:521-542 (kTurningZone = 0.05) in (r, mu) form:
+// away from a turning point w_r goes back onto w_r^2 = R(r) and w_mu onto
+// w_mu^2 = M(mu) (the polar zone test is the reference's times sin^2 theta).  The
+// second-order equations conserve w_r^2 - R with its error, and R ~ r^4.
+template <class R>
+__device__ __forceinline__ void project(const Photon<R>& p, R y[5]) {
+    const R r = y[0], m = y[1];
+    const R r2 = r * r;
+    const R P = p.E * (r2 + p.a2) - p.aL;
+    const R delta = (r2 - R(2) * r) + p.a2;
+    const R PP = P * P;
+    const R Rr = PP - delta * p.K;
+    if (Rr > R(0.05) * (PP + fabs(delta) * p.K)) y[3] = copysign(sqrt(Rr), y[3]);
+    const R m2 = m * m, s2 = R(1) - m2;
+    const R M = (p.Q - p.C1 * m2) - p.A2E2 * (m2 * m2);
+    const R scale = (fabs(p.Q) * s2 + p.A2E2 * (m2 * s2)) + p.L * p.L * m2;
+    if (M > R(0.05) * scale) y[4] = copysign(sqrt(M), y[4]);
+}
+
 __device__ __forceinline__ double wrap_2pi(double x) {
     const double twopi = 2.0 * kPi;
     double w = x - twopi * floor(x / twopi);
@@ -399,6 +418,7 @@ __device__ __forceinline__ bool lane_step(Lane<R>& s, R rtol, R atol, int max_st
         return false;
     }
 
+    project(s.p, yn);
     if (s.disk && s.y[1] * yn[1] < R(0)) {
         R lo = R(0), hi = R(1);
         const bool pos = s.y[1] > R(0);
@@ -442,6 +462,8 @@ __device__ __forceinline__ bool lane_step(Lane<R>& s, R rtol, R atol, int max_st
     for (int i = 0; i < 5; ++i) s.y[i] = yn[i];
 #pragma unroll
     for (int i = 0; i < 5; ++i) s.k1[i] = k[6][i];
+    s.k1[0] = s.y[3];  // FSAL after the projection: only dr/ds = w_r, dmu/ds = w_mu move
+    s.k1[1] = s.y[4];
     s.h = h * fac;
     return false;
 }

@@ -24,6 +24,11 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
+// sin and cos each behind its own call: compilers otherwise fuse sin(x), cos(x) into
+// sincos(x), whose glibc result can differ from sin(x) in the last bit.
+__attribute__((noinline)) double hsin(double x) { return std::sin(x); }
+__attribute__((noinline)) double hcos(double x) { return std::cos(x); }
+
 // ---- closed forms (src/kerr.cpp:157-206, 296-299) ----
 
 double horizon(double a) {
@@ -40,13 +45,13 @@ double isco(double a, int prograde) {
 
 double photon_orbit(double a, int prograde) {
     double s = prograde ? -1.0 : 1.0;
-    return 2.0 * (1.0 + std::cos(2.0 / 3.0 * std::acos(s * a)));
+    return 2.0 * (1.0 + hcos(2.0 / 3.0 * std::acos(s * a)));
 }
 
 // ---- Kerr-Schild metric and the ZAMO frame (src/kerr.cpp:238-270, 424-456) ----
 
 void metric_tensor(double a, double r, double theta, double g[4][4]) {
-    const double s = std::sin(theta), c = std::cos(theta);
+    const double s = hsin(theta), c = hcos(theta);
     const double s2 = s * s;
     const double sigma = r * r + a * a * c * c;
     const double rra = r * r + a * a;
@@ -77,7 +82,7 @@ void lower4(const double g[4][4], const double* x, double* out) {
 }
 
 void zamo_u(double a, double r, double theta, double u[4]) {
-    const double s = std::sin(theta), c = std::cos(theta);
+    const double s = hsin(theta), c = hcos(theta);
     const double sigma = r * r + a * a * c * c;
     const double delta = r * r - 2.0 * r + a * a;
     const double rra = r * r + a * a;
@@ -115,8 +120,8 @@ void observer_frame(double a, double r, double theta, const double u[4], double 
 // camera basis, src/camera.cpp:6-21 (FP64): columns right, up, forward on (r, theta, phi)
 void camera_basis(double yaw, double pitch, double roll, double B[9]) {
     const double f0[3] = {-1.0, 0.0, 0.0}, u0[3] = {0.0, -1.0, 0.0}, r0[3] = {0.0, 0.0, 1.0};
-    const double cy = std::cos(yaw), sy = std::sin(yaw), cp = std::cos(pitch), sp = std::sin(pitch);
-    const double cr = std::cos(roll), sr = std::sin(roll);
+    const double cy = hcos(yaw), sy = hsin(yaw), cp = hcos(pitch), sp = hsin(pitch);
+    const double cr = hcos(roll), sr = hsin(roll);
     double yawed[3], right[3], fwd[3], up[3];
     for (int i = 0; i < 3; ++i) {
         yawed[i] = cy * f0[i] + sy * r0[i];
@@ -165,12 +170,12 @@ ViewDev make_view_dev(const kt_view& v, long long offset) {
     d.half_height = v.half_height;
     d.px = 2.0 * v.half_width / (double)v.width;
     d.py = 2.0 * v.half_height / (double)v.height;
-    d.sinI = std::sin(v.theta_cam);
-    d.cosI = std::cos(v.theta_cam);
+    d.sinI = hsin(v.theta_cam);
+    d.cosI = hcos(v.theta_cam);
     double th = v.theta_cam;
     if (th < 1e-4) th = 1e-4;
     if (th > kPi - 1e-4) th = kPi - 1e-4;
-    d.m0 = std::cos(th);
+    d.m0 = hcos(th);
     d.r_cap = v.capture_factor * horizon(a);
     d.r_esc = v.escape_radius > 1.5 * v.r_cam ? v.escape_radius : 1.5 * v.r_cam;
     d.r_in = v.disk_inner > 0.0 ? v.disk_inner : isco(a, v.disk_prograde);
@@ -191,8 +196,8 @@ ViewDev make_view_dev(const kt_view& v, long long offset) {
         }
         camera_basis(v.yaw, v.pitch, v.roll, d.B);
         d.tanh_fov = std::tan(0.5 * v.fov_deg * kPi / 180.0);
-        d.sn = std::sin(t);
-        d.cs = std::cos(t);
+        d.sn = hsin(t);
+        d.cs = hcos(t);
         d.sigma = r * r + a * a * d.cs * d.cs;
     }
     d.params = nullptr;
@@ -564,13 +569,13 @@ double kt_circular_omega(double a, double r, int prograde) {
 
 int kt_critical_curve(double a, double inclination, int samples, double* alpha, double* beta, int cap) {
     if (!alpha || !beta || cap <= 0) return 0;
-    const double sinI = std::sin(inclination), cosI = std::cos(inclination);
+    const double sinI = hsin(inclination), cosI = hcos(inclination);
     if (std::fabs(a) < 1e-6) {
         int n = 0;
         for (int k = 0; k < samples && n < cap; ++k, ++n) {
             const double psi = 2.0 * kPi * k / samples;
-            alpha[n] = std::sqrt(27.0) * std::cos(psi);
-            beta[n] = std::sqrt(27.0) * std::sin(psi);
+            alpha[n] = std::sqrt(27.0) * hcos(psi);
+            beta[n] = std::sqrt(27.0) * hsin(psi);
         }
         return n;
     }
@@ -578,7 +583,7 @@ int kt_critical_curve(double a, double inclination, int samples, double* alpha, 
     const int steps = std::max(samples, 16) * 8;
     std::vector<double> ua, ub;
     for (int k = 0; k <= steps; ++k) {
-        const double x = 0.5 - 0.5 * std::cos(kPi * k / steps);
+        const double x = 0.5 - 0.5 * hcos(kPi * k / steps);
         const double r = r1 + (r2 - r1) * x;
         const double xi = (r * r * (3.0 - r) - a * a * (r + 1.0)) / (a * (r - 1.0));
         const double eta = r * r * r * (4.0 * a * a - r * (r - 3.0) * (r - 3.0)) / (a * a * (r - 1.0) * (r - 1.0));
@@ -601,7 +606,7 @@ int kt_critical_curve(double a, double inclination, int samples, double* alpha, 
 
 double kt_critical_radius(const double* alpha, const double* beta, int n, double psi) {
     if (!alpha || !beta || n <= 0) return 0.0;
-    const double dx = std::cos(psi), dy = std::sin(psi);
+    const double dx = hcos(psi), dy = hsin(psi);
     double best = 0.0;
     for (int k = 0; k < n; ++k) {
         const int q = (k + 1) % n;

@@ -17,6 +17,11 @@
 
 static int naked(double a) { return fabs(a) > 1.0; }
 
+/* sin and cos each behind its own call: compilers otherwise fuse sin(x), cos(x) into
+   sincos(x), whose glibc result can differ from sin(x) in the last bit. */
+static __attribute__((noinline)) double ko_sin(double x) { return sin(x); }
+static __attribute__((noinline)) double ko_cos(double x) { return cos(x); }
+
 double ko_horizon_radius(double a) {
     if (naked(a)) return 0.0;
     return 1.0 + sqrt(fmax(0.0, 1.0 - a * a));
@@ -38,7 +43,7 @@ double ko_isco_radius(double a, int prograde) {
 /* kerr.cpp:192-206 (|a| <= 1) */
 double ko_photon_orbit_radius(double a, int prograde) {
     double s = prograde ? -1.0 : 1.0;
-    return 2.0 * (1.0 + cos(2.0 / 3.0 * acos(s * a)));
+    return 2.0 * (1.0 + ko_cos(2.0 / 3.0 * acos(s * a)));
 }
 
 /* kerr.cpp:208-223 (|a| <= 1) */
@@ -72,7 +77,7 @@ static double safe_sin_cubed(double s) {
 /* kerr.cpp:331-351: Kerr-Schild t, phi; forward Mino time */
 ko_derivs ko_derivatives(double a, ko_consts c, ko_state s) {
     const double r = s.r;
-    const double sn = sin(s.theta), cs = cos(s.theta);
+    const double sn = ko_sin(s.theta), cs = ko_cos(s.theta);
     const double sn2 = fmax(sn * sn, 1e-18);
     const double a2 = a * a, mu2 = c.mu * c.mu;
     const double delta = r * r - 2.0 * r + a2;
@@ -101,7 +106,7 @@ double ko_radial_potential(double a, ko_consts c, double r) {
 
 /* kerr.cpp:360-363 */
 double ko_polar_potential(double a, ko_consts c, double theta) {
-    const double sn = sin(theta), cs = cos(theta);
+    const double sn = ko_sin(theta), cs = ko_cos(theta);
     return c.Q - cs * cs * (a * a * (c.mu * c.mu - c.E * c.E) + c.L * c.L / fmax(sn * sn, 1e-18));
 }
 
@@ -115,7 +120,7 @@ void ko_enforce_constraints(double a, ko_consts c, ko_state* s) {
     const double K = c.mu * c.mu * r * r + x * x + c.Q;
     const double R = P * P - delta * K;
     if (R > zone * (P * P + fabs(delta) * K)) s->vr = copysign(sqrt(R), s->vr);
-    const double sn = sin(s->theta), cs = cos(s->theta);
+    const double sn = ko_sin(s->theta), cs = ko_cos(s->theta);
     const double cot2 = cs * cs / fmax(sn * sn, 1e-18);
     const double polarMass = a * a * (c.mu * c.mu - c.E * c.E) * cs * cs;
     const double Theta = c.Q - polarMass - c.L * c.L * cot2;
@@ -129,7 +134,7 @@ void ko_enforce_constraints(double a, ko_consts c, ko_state* s) {
 
 /* covariant Kerr-Schild metric, kerr.cpp:238-254 */
 void ko_metric_tensor(double a, double r, double theta, double g[4][4]) {
-    const double s = sin(theta), c = cos(theta);
+    const double s = ko_sin(theta), c = ko_cos(theta);
     const double s2 = s * s;
     const double sigma = r * r + a * a * c * c;
     const double rra = r * r + a * a;
@@ -161,7 +166,7 @@ static void lower4(const double g[4][4], const double x[4], double out[4]) {
 
 /* kerr.cpp:424-430 */
 void ko_zamo_four_velocity(double a, double r, double theta, double u[4]) {
-    const double s = sin(theta), c = cos(theta);
+    const double s = ko_sin(theta), c = ko_cos(theta);
     const double sigma = r * r + a * a * c * c;
     const double delta = r * r - 2.0 * r + a * a;
     const double rra = r * r + a * a;
@@ -206,7 +211,7 @@ void ko_photon_from_frame(double a, double r, double theta, double phi, const do
     for (int i = 0; i < 4; ++i) p[i] = e[0][i] - look[0] * e[1][i] - look[1] * e[2][i] - look[2] * e[3][i];
     ko_metric_tensor(a, r, theta, g);
     lower4(g, p, low);
-    const double sn = sin(theta), cs = cos(theta);
+    const double sn = ko_sin(theta), cs = ko_cos(theta);
     const double sigma = r * r + a * a * cs * cs;
     c->mu = 0.0;
     c->E = -low[0];
@@ -227,8 +232,8 @@ void ko_photon_from_frame(double a, double r, double theta, double phi, const do
 /* camera.cpp:6-21 in FP64 */
 void ko_camera_basis(double yaw, double pitch, double roll, double B[9]) {
     const double f0[3] = {-1.0, 0.0, 0.0}, u0[3] = {0.0, -1.0, 0.0}, r0[3] = {0.0, 0.0, 1.0};
-    const double cy = cos(yaw), sy = sin(yaw), cp = cos(pitch), sp = sin(pitch);
-    const double cr = cos(roll), sr = sin(roll);
+    const double cy = ko_cos(yaw), sy = ko_sin(yaw), cp = ko_cos(pitch), sp = ko_sin(pitch);
+    const double cr = ko_cos(roll), sr = ko_sin(roll);
     double yawed[3], right[3], fwd[3], up[3];
     for (int i = 0; i < 3; ++i) {
         yawed[i] = cy * f0[i] + sy * r0[i];
@@ -278,7 +283,7 @@ double ko_M(const ko_photon* p, double m) {
  *   dr/ds = w_r,  dw_r/ds = R'(r)/2 = 2 E r P - (r - 1) K          (kerr.cpp:346, mu = 0)
  *   dmu/ds = w_mu, dw_mu/ds = M'(mu)/2 = -(Q + L^2 - a^2 E^2) mu - 2 a^2 E^2 mu^3
  *   dphi/ds = -(a P / Delta - a E + L / (1 - mu^2))                  (BL; kerr.cpp:348 less a vr/Delta)
- * M(mu) = sin^2(theta) Theta(theta); on the constraint surface w_mu = -sin(theta) w_theta. */
+ * M(mu) = sin^2(theta) Theta(theta); on the constraint surface w_mu = -ko_sin(theta) w_theta. */
 void ko_rhs(const ko_photon* p, const double y[5], double dy[5]) {
     const double r = y[0], m = y[1];
     const double r2 = r * r;
@@ -461,6 +466,25 @@ static double dense(const double y[5], const double ynew[5], double k[7][5], dou
     return y[i] + th * (ydiff + th1 * (bspl + th * (c4 + th1 * c5)));
 }
 
+/* Constraint projection, kerr.cpp:521-542 (kTurningZone = 0.05) in (r, mu) form: away
+   from a turning point put w_r back on w_r^2 = R(r) and w_mu on w_mu^2 = M(mu), where
+   M = sin^2(theta) Theta and the polar zone test is the reference's multiplied by
+   sin^2(theta).  The second-order equations conserve w_r^2 - R including its error,
+   and R ~ r^4: without this, a rounding-level error at r = 1000 M is O(1) near the hole. */
+static void project(const ko_photon* p, double y[5]) {
+    const double r = y[0], m = y[1];
+    const double r2 = r * r;
+    const double P = p->E * (r2 + p->a2) - p->aL;
+    const double delta = (r2 - 2.0 * r) + p->a2;
+    const double PP = P * P;
+    const double R = PP - delta * p->K;
+    if (R > 0.05 * (PP + fabs(delta) * p->K)) y[3] = copysign(sqrt(R), y[3]);
+    const double m2 = m * m, s2 = 1.0 - m2;
+    const double M = (p->Q - p->C1 * m2) - p->A2E2 * (m2 * m2);
+    const double scale = (fabs(p->Q) * s2 + p->A2E2 * (m2 * s2)) + p->L * p->L * m2;
+    if (M > 0.05 * scale) y[4] = copysign(sqrt(M), y[4]);
+}
+
 static double wrap_2pi(double x) {
     const double twopi = 2.0 * KO_PI;
     double w = x - twopi * floor(x / twopi);
@@ -478,11 +502,11 @@ static double wrap_2pi(double x) {
    when beta > 0.  The view's theta_cam is the inclination. */
 void ko_launch_bardeen(const ko_view* v, double alpha, double beta, ko_photon* p, double y0[5]) {
     const double a = v->spin;
-    const double sinI = sin(v->theta_cam), cosI = cos(v->theta_cam);
+    const double sinI = ko_sin(v->theta_cam), cosI = ko_cos(v->theta_cam);
     double th = v->theta_cam;
     if (th < 1e-4) th = 1e-4;
     if (th > KO_PI - 1e-4) th = KO_PI - 1e-4;
-    const double m0 = cos(th);
+    const double m0 = ko_cos(th);
     const double L = -alpha * sinI;
     const double Q = beta * beta + (alpha * alpha - a * a) * cosI * cosI;
     ko_photon_init(p, a, 1.0, L, Q);
@@ -496,18 +520,12 @@ void ko_launch_bardeen(const ko_view* v, double alpha, double beta, ko_photon* p
 /* Pinhole camera in the ZAMO frame (camera.cpp:6-40 with Hover, no drift), one
    pixel as trace.frag:1494-1496 + launchPhoton (trace.frag:1470-1488), with the
    frame vectors lowered once per view (renderer.cpp:345-357). */
+static void pinhole_photon(const ko_view* v, double n0, double n1, double n2, ko_photon* p, double y0[5]);
+
 static void launch_pinhole(const ko_view* v, int col, int row, ko_photon* p, double y0[5]) {
-    const double a = v->spin, r = v->r_cam, th = v->theta_cam;
-    double u[4], e[4][4], g[4][4], low[4][4], B[9];
-    ko_zamo_four_velocity(a, r, th, u);
-    ko_observer_frame(a, r, th, u, e);
-    ko_metric_tensor(a, r, th, g);
-    for (int j = 0; j < 4; ++j) lower4(g, e[j], low[j]);
+    double B[9];
     ko_camera_basis(v->yaw, v->pitch, v->roll, B);
     const double tanh_fov = tan(0.5 * v->fov_deg * KO_PI / 180.0);
-    const double sn = sin(th), cs = cos(th);
-    const double sigma = r * r + a * a * cs * cs;
-
     const double W = (double)v->width, H = (double)v->height;
     const double nx = (((double)col + 0.5) - 0.5 * W) / (0.5 * H);
     const double ny = (((double)(v->height - row) - 0.5) - 0.5 * H) / (0.5 * H);
@@ -515,7 +533,23 @@ static void launch_pinhole(const ko_view* v, int col, int row, ko_photon* p, dou
     double d[3];
     for (int i = 0; i < 3; ++i) d[i] = (x * B[i] + yv * B[3 + i]) + B[6 + i];
     const double inv = 1.0 / sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
-    const double n0 = d[0] * inv, n1 = d[1] * inv, n2 = d[2] * inv;
+    pinhole_photon(v, d[0] * inv, d[1] * inv, d[2] * inv, p, y0);
+}
+
+void ko_launch_look(const ko_view* v, const double look[3], ko_photon* p, double y0[5]) {
+    pinhole_photon(v, look[0], look[1], look[2], p, y0);
+}
+
+/* the photon arriving from unit direction n of the camera's ZAMO frame */
+static void pinhole_photon(const ko_view* v, double n0, double n1, double n2, ko_photon* p, double y0[5]) {
+    const double a = v->spin, r = v->r_cam, th = v->theta_cam;
+    double u[4], e[4][4], g[4][4], low[4][4];
+    ko_zamo_four_velocity(a, r, th, u);
+    ko_observer_frame(a, r, th, u, e);
+    ko_metric_tensor(a, r, th, g);
+    for (int j = 0; j < 4; ++j) lower4(g, e[j], low[j]);
+    const double sn = ko_sin(th), cs = ko_cos(th);
+    const double sigma = r * r + a * a * cs * cs;
     /* q = frame^T (1, -n) on the lowered (t, phi, theta) components and e^r */
     const double qt = ((low[0][0] - n0 * low[1][0]) - n1 * low[2][0]) - n2 * low[3][0];
     const double qp = ((low[0][3] - n0 * low[1][3]) - n1 * low[2][3]) - n2 * low[3][3];
@@ -627,6 +661,7 @@ void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, con
         rejected = 0;
         /* through the pole (L = 0 only): same point, phi + pi */
         if (p->L == 0.0 && y[1] * y[4] > 0.0 && yn[1] * yn[4] < 0.0) yn[2] += KO_PI;
+        project(p, yn);
 
         /* events in the reference's order: disk, horizon, escape (trajectory.cpp:307-392) */
         if (disk && y[1] * yn[1] < 0.0) {
@@ -668,6 +703,8 @@ void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, con
         }
         for (int i = 0; i < 5; ++i) y[i] = yn[i];
         for (int i = 0; i < 5; ++i) k[0][i] = k[6][i];
+        k[0][0] = y[3]; /* FSAL after the projection: only dr/ds = w_r, dmu/ds = w_mu move */
+        k[0][1] = y[4];
         h *= fac;
     }
     for (int i = 0; i < 5; ++i) out->y_end[i] = y[i];
@@ -745,13 +782,13 @@ long ko_render(const ko_view* v, const ko_opts* o, int threads, const volatile i
 /* ------------------------------------------------------------------------- */
 
 int ko_critical_curve(double a, double inclination, int samples, double* alpha, double* beta, int cap) {
-    const double sinI = sin(inclination), cosI = cos(inclination);
+    const double sinI = ko_sin(inclination), cosI = ko_cos(inclination);
     int n_out = 0;
     if (fabs(a) < 1e-6) {
         for (int k = 0; k < samples && n_out < cap; ++k) {
             const double psi = 2.0 * KO_PI * k / samples;
-            alpha[n_out] = sqrt(27.0) * cos(psi);
-            beta[n_out] = sqrt(27.0) * sin(psi);
+            alpha[n_out] = sqrt(27.0) * ko_cos(psi);
+            beta[n_out] = sqrt(27.0) * ko_sin(psi);
             ++n_out;
         }
         return n_out;
@@ -761,7 +798,7 @@ int ko_critical_curve(double a, double inclination, int samples, double* alpha, 
     int nu = 0;
     /* upper half first, into the front of the buffers */
     for (int k = 0; k <= n; ++k) {
-        const double x = 0.5 - 0.5 * cos(KO_PI * k / n);
+        const double x = 0.5 - 0.5 * ko_cos(KO_PI * k / n);
         const double r = r1 + (r2 - r1) * x;
         const double xi = (r * r * (3.0 - r) - a * a * (r + 1.0)) / (a * (r - 1.0));
         const double eta = r * r * r * (4.0 * a * a - r * (r - 3.0) * (r - 3.0)) / (a * a * (r - 1.0) * (r - 1.0));
@@ -782,7 +819,7 @@ int ko_critical_curve(double a, double inclination, int samples, double* alpha, 
 }
 
 double ko_critical_radius(const double* alpha, const double* beta, int n, double psi) {
-    const double dx = cos(psi), dy = sin(psi);
+    const double dx = ko_cos(psi), dy = ko_sin(psi);
     double best = 0.0;
     for (int k = 0; k < n; ++k) {
         const int q = (k + 1) % n;

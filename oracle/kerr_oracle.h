@@ -121,6 +121,8 @@ typedef struct {
 /* launch helpers: initial photon + state for pixel (col, row) of a view */
 void ko_launch_pixel(const ko_view* v, int col, int row, ko_photon* p, double y0[5]);
 void ko_launch_bardeen(const ko_view* v, double alpha, double beta, ko_photon* p, double y0[5]);
+/* pinhole view, explicit unit look vector on the ZAMO frame's (r, theta, phi) axes */
+void ko_launch_look(const ko_view* v, const double look[3], ko_photon* p, double y0[5]);
 
 void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, const double y0[5],
                      ko_result* out);

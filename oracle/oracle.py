@@ -118,6 +118,7 @@ def lib():
             ("ko_M", d, [P(Photon), d]),
             ("ko_launch_pixel", None, [P(View), i, i, P(Photon), P(d)]),
             ("ko_launch_bardeen", None, [P(View), d, d, P(Photon), P(d)]),
+            ("ko_launch_look", None, [P(View), P(d), P(Photon), P(d)]),
             ("ko_trace_photon", None, [P(View), P(Opts), P(Photon), P(d), P(Result)]),
             ("ko_trace_pixel", None, [P(View), P(Opts), i, i, P(Result)]),
             ("ko_render", C.c_long, [P(View), P(Opts), i, P(C.c_int), i, P(C.c_int8), P(d), P(d), P(d), P(d),
@@ -193,3 +194,36 @@ def critical_radius(alpha: np.ndarray, beta: np.ndarray, psi: float) -> float:
     al = np.ascontiguousarray(alpha, dtype=np.float64)
     be = np.ascontiguousarray(beta, dtype=np.float64)
     return lib().ko_critical_radius(_dp(al), _dp(be), len(al), psi)
+
+
+def trace_params(view: View, opts: Opts, params) -> dict:
+    """Explicit rays: (alpha, beta) rows for a Bardeen view, unit looks for a pinhole view."""
+    params = np.asarray(params, dtype=np.float64)
+    per = 2 if view.camera_kind == BARDEEN else 3
+    params = params.reshape(-1, per)
+    n = len(params)
+    out = {"cls": np.zeros(n, np.int8), "theta": np.full(n, np.nan), "phi": np.full(n, np.nan),
+           "r_hit": np.full(n, np.nan), "g": np.full(n, np.nan), "steps": np.zeros(n, np.int32),
+           "y_end": np.zeros((n, 5)), "consts": np.zeros((n, 3))}
+    L = lib()
+    for i, row in enumerate(params):
+        p = Photon()
+        y0 = (C.c_double * 5)()
+        if per == 2:
+            L.ko_launch_bardeen(C.byref(view), row[0], row[1], C.byref(p), y0)
+        else:
+            lk = (C.c_double * 3)(*row)
+            L.ko_launch_look(C.byref(view), lk, C.byref(p), y0)
+        res = Result()
+        L.ko_trace_photon(C.byref(view), C.byref(opts), C.byref(p), y0, C.byref(res))
+        out["cls"][i] = res.cls
+        out["steps"][i] = res.steps
+        out["y_end"][i] = list(res.y_end)
+        out["consts"][i] = (p.E, p.L, p.Q)
+        if res.cls == ESCAPED:
+            out["theta"][i] = res.theta_inf
+            out["phi"][i] = res.phi_inf
+        if res.cls == DISK:
+            out["r_hit"][i] = res.r_hit
+            out["g"][i] = res.g
+    return out

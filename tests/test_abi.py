@@ -1,0 +1,102 @@
+"""The C ABI without a GPU: the library loads, exports every entry point the headers
+declare, its host-side closed forms equal the oracle's, and its error paths are codes."""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from black_simulation_amd import _lib as K
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:[A-Za-z_][\w\s\*]*?)\b(kt_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = K.lib()
+    names = declared_symbols()
+    assert {"kt_create", "kt_trace_views", "kt_trace_rays", "kt_destroy"} <= names
+    for n in sorted(names):
+        assert hasattr(lib, n), f"{n} declared in include/ but not exported"
+    assert names <= set(K.SIGNATURES) | {n for n in names if n.startswith("kt_surrogate")}
+
+
+def test_host_closed_forms_equal_oracle():
+    lib, ol = K.lib(), O.lib()
+    for a in (0.0, 0.3, 0.9, 0.998, 1.0):
+        assert lib.kt_horizon_radius(a) == ol.ko_horizon_radius(a)
+        for pro in (0, 1):
+            assert lib.kt_isco_radius(a, pro) == ol.ko_isco_radius(a, pro)
+            assert lib.kt_photon_orbit_radius(a, pro) == ol.ko_photon_orbit_radius(a, pro)
+            for r in (3.0, 7.5, 40.0):
+                assert lib.kt_circular_omega(a, r, pro) == ol.ko_circular_omega(a, r, pro)
+
+
+def test_critical_curve_equals_oracle():
+    from black_simulation_amd import api
+
+    for a, inc in ((0.9, math.radians(17)), (0.0, 0.7), (0.998, math.radians(85))):
+        al, be = api.critical_curve(a, inc, 100)
+        oal, obe = O.critical_curve(a, inc, 100)
+        np.testing.assert_array_equal(al, oal)
+        np.testing.assert_array_equal(be, obe)
+        for psi in np.linspace(0, 2 * math.pi, 13):
+            assert api.critical_radius(al, be, psi) == O.critical_radius(oal, obe, psi)
+
+
+def test_defaults():
+    lib = K.lib()
+    o = K.KtTraceOpts()
+    lib.kt_trace_opts_default(C.byref(o))
+    assert (o.rtol, o.atol, o.max_steps, o.precision) == (1e-10, 1e-12, 200000, K.PREC_FP64)
+    assert (o.gate_prob, o.gate_ring, o.restep_drift) == (0.999, 0.03, 1e-6)
+    v = K.KtView()
+    lib.kt_view_default(C.byref(v))
+    assert (v.width, v.spin, v.r_cam, v.capture_factor, v.escape_radius) == (256, 0.9, 30.0, 1.01, 1000.0)
+
+
+def test_error_paths_are_status_codes():
+    lib = K.lib()
+    v = K.KtView()
+    lib.kt_view_default(C.byref(v))
+    out = K.KtRaysOut()
+    assert lib.kt_trace_views(None, C.byref(v), 1, None, C.byref(out), None) == K.KT_EINVAL
+    assert lib.kt_trace_rays(None, C.byref(v), None, 1, None, C.byref(out), None) == K.KT_EINVAL
+    assert lib.kt_destroy(None) == K.KT_OK
+    assert lib.kt_last_error_string(None) == b"null context"
+    assert lib.kt_create(None, None) == K.KT_EINVAL
+
+
+def test_create_without_gpu_reports_no_device():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    ctx = C.c_void_p()
+    assert K.lib().kt_create(None, C.byref(ctx)) == K.KT_ENODEVICE
+    assert not ctx.value
+
+
+def test_product_never_touches_the_oracle():
+    """Nothing under black_simulation_amd/ imports, links or names the oracle."""
+    pkg = os.path.join(ROOT, "black_simulation_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for fn in files:
+            if fn.endswith((".py", ".cpp", ".hip", ".hpp", ".h")):
+                text = open(os.path.join(dirpath, fn)).read()
+                assert "oracle" not in text.lower(), f"{fn} mentions the oracle"
