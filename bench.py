@@ -18,13 +18,14 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# FP64 operations of one DP5 attempt of the integrator (kt_device.hpp), counting each
-# +, -, *, / as one (no FMA: contraction is off):
-#   6 right-hand sides x 28                                   = 168
-#   stage combinations (5 comps: 3, 5, 7, 9, 11, 11 ops)      = 230
-#   error estimate (5 comps x 12) + norm (5 x 3 + 1)          =  76
-#   controller (x^-1/5: 28, safety and h update: 2)           =  30
-OPS_PER_ATTEMPT = 504
+# Floating-point operations of one DP5 attempt of the integrator (kt_device.hpp), each
+# +, -, *, /, sqrt counted once and an FMA as two:
+#   6 right-hand sides x 25 (one division each)                  = 150
+#   stage sums (per component 3 + 5 + 7 + 9 + 11 + 11, x 5)       = 230
+#   error estimate (5 x 12) and max-norm (5 x 4 + 1 division)     =  81
+#   controller (x^-1/5 by three Newton steps, safety, h update)   =  30
+#   constraint projection (R, M, two zone tests, two sqrt)         =  31
+OPS_PER_ATTEMPT = 522
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 BF16_MFMA_PEAK_TFLOPS = 2500.0
@@ -83,6 +84,8 @@ def main():
     opts = bsa.TraceOpts(precision=prec, use_surrogate=1 if args.path == "surrogate" else 0)
     tr = bsa.Tracer(0)
     out = tr.alloc_outputs(view.rays)
+    # launch shape from the autotuner (untimed), as a user would run it once per device
+    opts = tr.autotune([view], opts)
     stream = torch.cuda.current_stream()
     for _ in range(args.warmup):
         tr.trace_views([view], opts, out=out, stream=stream)
@@ -123,12 +126,13 @@ def main():
                                f"30 M wide, camera r=50 M i=85 deg, disk r_ISCO..40 M, DP5 rtol 1e-10",
                    "rays_per_step": view.rays, "path": args.path,
                    "dp5_attempts_per_step": int(attempts), "class_counts": counts,
-                   "launch": {"block": st["block"], "grid": st["grid"], "waves_per_cu": st["waves_per_cu"],
-                              "steps_per_dequeue": st["steps_per_dequeue"]}},
+                   "launch": {"mode": ["persistent", "one-shot"][st["mode"]], "block": st["block"],
+                              "grid": st["grid"], "waves_per_cu": st["waves_per_cu"],
+                              "steps_per_dequeue": st["steps_per_dequeue"], "waves_per_simd": st["waves_per_simd"]}},
         "roofline": {"bound": "valu_fp64" if prec == bsa.PREC_FP64 else "valu_fp32",
                      "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": None,
-                     "kernel": "k_trace_persistent", "kernel_ms": round(avg_integ, 3),
+                     "kernel": ["k_trace_persistent", "k_trace_oneshot"][st["mode"]], "kernel_ms": round(avg_integ, 3),
                      "ops_per_attempt": OPS_PER_ATTEMPT},
         "cpu_baseline": None,
     }

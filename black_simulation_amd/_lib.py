@@ -16,7 +16,7 @@ ESCAPED, CAPTURED, DISK, UNFINISHED = 0, 1, 2, 3
 FLAG_FP64_RESTEP, FLAG_SURROGATE = 0x10, 0x20
 CAMERA_BARDEEN, CAMERA_PINHOLE = 0, 1
 PREC_FP64, PREC_MIXED, PREC_FP32 = 0, 1, 2
-MODE_PERSISTENT, MODE_ONESHOT = 0, 1
+MODE_PERSISTENT, MODE_ONESHOT, MODE_AUTO = 0, 1, 2
 
 
 class KtView(C.Structure):
@@ -59,6 +59,7 @@ class KtTraceOpts(C.Structure):
         ("steps_per_dequeue", C.c_int),
         ("gate_prob", C.c_double),
         ("gate_ring", C.c_double),
+        ("waves_per_simd", C.c_int),
     ]
 
 
@@ -87,10 +88,13 @@ class KtStats(C.Structure):
         ("steps", C.c_longlong),
         ("fallback_rays", C.c_longlong),
         ("restep_rays", C.c_longlong),
+        ("fp64_steps", C.c_longlong),
         ("block", C.c_int),
         ("waves_per_cu", C.c_int),
         ("steps_per_dequeue", C.c_int),
         ("grid", C.c_int),
+        ("waves_per_simd", C.c_int),
+        ("mode", C.c_int),
     ]
 
 
@@ -106,6 +110,10 @@ SIGNATURES = {
     "kt_trace_views": (C.c_int, [C.c_void_p, _P(KtView), C.c_int, _P(KtTraceOpts), _P(KtRaysOut), C.c_void_p]),
     "kt_trace_rays": (C.c_int, [C.c_void_p, _P(KtView), C.c_void_p, C.c_longlong, _P(KtTraceOpts), _P(KtRaysOut),
                                 C.c_void_p]),
+    "kt_autotune": (C.c_int, [C.c_void_p, _P(KtView), C.c_int, _P(KtTraceOpts)]),
+    "kt_plan_create": (C.c_int, [C.c_void_p, _P(KtView), C.c_int, _P(KtTraceOpts), _P(KtRaysOut), _P(C.c_void_p)]),
+    "kt_plan_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kt_plan_destroy": (C.c_int, [C.c_void_p]),
     "kt_horizon_radius": (C.c_double, [C.c_double]),
     "kt_isco_radius": (C.c_double, [C.c_double, C.c_int]),
     "kt_photon_orbit_radius": (C.c_double, [C.c_double, C.c_int]),

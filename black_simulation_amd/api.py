@@ -64,7 +64,7 @@ class View:
 @dataclasses.dataclass
 class TraceOpts:
     precision: int = K.PREC_FP64
-    mode: int = K.MODE_PERSISTENT
+    mode: int = K.MODE_AUTO
     use_surrogate: int = 0
     max_steps: int = 200000
     rtol: float = 1e-10
@@ -77,6 +77,7 @@ class TraceOpts:
     steps_per_dequeue: int = 0
     gate_prob: float = 0.999
     gate_ring: float = 0.03
+    waves_per_simd: int = 0
 
     def to_c(self) -> K.KtTraceOpts:
         o = K.KtTraceOpts()
@@ -181,10 +182,57 @@ class Tracer:
         self._keep = p  # alive until the stream has used it
         return out
 
+    def autotune(self, views: Sequence[View], opts: TraceOpts | None = None) -> TraceOpts:
+        """kt_autotune: time launch shapes on `views`, return the fastest (also cached in
+        the context for later calls whose shape fields are 0)."""
+        o = (opts or TraceOpts()).to_c()
+        arr = (K.KtView * len(views))(*[v.to_c() for v in views])
+        _check(self._ctx, K.lib().kt_autotune(self._ctx, arr, len(views), C.byref(o)), "kt_autotune")
+        res = dataclasses.replace(opts or TraceOpts())
+        for f in ("mode", "block", "waves_per_cu", "steps_per_dequeue", "waves_per_simd"):
+            setattr(res, f, getattr(o, f))
+        return res
+
+    def plan(self, views: Sequence[View], opts: TraceOpts | None = None, out: dict | None = None) -> "Plan":
+        """A batch captured once into a hipGraph (kt_plan_create), replayed by Plan.launch()."""
+        n = sum(v.rays for v in views)
+        if out is None:
+            out = self.alloc_outputs(n)
+        arr = (K.KtView * len(views))(*[v.to_c() for v in views])
+        o = (opts or TraceOpts()).to_c()
+        s = self._out_struct(out)
+        handle = C.c_void_p()
+        _check(self._ctx, K.lib().kt_plan_create(self._ctx, arr, len(views), C.byref(o), C.byref(s),
+                                                 C.byref(handle)), "kt_plan_create")
+        return Plan(self, handle, out)
+
     def stats(self) -> dict:
         s = K.KtStats()
         _check(self._ctx, K.lib().kt_last_stats(self._ctx, C.byref(s)), "kt_last_stats")
         return {f: getattr(s, f) for f, _ in K.KtStats._fields_}
+
+
+class Plan:
+    """A captured batch (kt_plan); launch() replays its hipGraph on a stream."""
+
+    def __init__(self, tracer: Tracer, handle, out: dict):
+        self.tracer, self._h, self.out = tracer, handle, out
+
+    def launch(self, stream=None) -> dict:
+        st = stream if stream is not None else self.tracer.torch.cuda.current_stream(self.tracer.device)
+        _check(self.tracer._ctx, K.lib().kt_plan_launch(self._h, C.c_void_p(st.cuda_stream)), "kt_plan_launch")
+        return self.out
+
+    def close(self):
+        if self._h:
+            K.lib().kt_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def to_numpy(out: dict) -> dict:

@@ -31,6 +31,15 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write("\n".join(noise) + "\n")
         raise RuntimeError("hipcc failed building libkerr_trace.so")
     os.replace(tmp, out)
+    # the C++20 inspector, linked against the library through include/kerr_trace.hpp
+    exe = os.path.join(PKG_DIR, "kt_inspect")
+    cmd = ["g++", "-std=c++20", "-O2", "-Wall", "-o", exe + ".tmp", os.path.join(CSRC, "kt_inspect.cpp"),
+           "-L" + PKG_DIR, "-lkerr_trace", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stderr)
+        raise RuntimeError("g++ failed building kt_inspect")
+    os.replace(exe + ".tmp", exe)
     return out
 
 

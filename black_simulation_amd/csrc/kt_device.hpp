@@ -44,6 +44,7 @@ constexpr double kSafety = 0.9, kFacMin = 0.2, kFacMax = 5.0;
 constexpr double kAxisSnap = 3e-4;  // src/kerr.h:115
 
 enum : int { kEscaped = 0, kCaptured = 1, kDisk = 2, kUnfinished = 3 };
+enum : int { kFlagRestep = 0x10, kFlagSurrogate = 0x20, kFlagNonfinite = 0x40 };
 
 // Per-view constants, prepared on the host (trig and frames in FP64 there).
 struct ViewDev {
@@ -52,7 +53,8 @@ struct ViewDev {
     double a, r_cam, phi_cam;
     double half_width, half_height, px, py;  // Bardeen plane, pixel pitch
     double sinI, cosI, m0;                   // sin/cos of inclination, cos of the clamped theta
-    double r_cap, r_esc, r_in, r_out, h0;
+    double r_cap, r_esc, r_in, r_out, h0, r_fp64;  // r_fp64 = 1.2 r+ (mixed precision)
+    double escape_radius_;                         // kind 2: the view's escape_radius
     double lt[4], lp[4], lh[4], er[4];       // pinhole: lowered frame (t, phi, theta) and e^r
     double B[9], tanh_fov, sn, cs, sigma;    // pinhole: camera basis (columns), trig at the camera
     const double* params;                    // explicit rays (alpha, beta) or look vectors, or null
@@ -62,7 +64,7 @@ static_assert(sizeof(ViewDev) % sizeof(unsigned long long) == 0, "ViewDev is sta
 
 template <class R>
 struct Photon {
-    R a, a2, E, L, Q, aE, aL, K, C1, A2E2;
+    R a, a2, E, L, Q, aE, aL, K, C1, A2E2, twoE, twoA2E2;
 };
 
 template <class R>
@@ -81,32 +83,8 @@ __device__ __forceinline__ void photon_init(Photon<R>& p, R a, R E, R L, R Q) {
     p.K = x * x + Q;
     p.A2E2 = p.a2 * (E * E);
     p.C1 = (Q + L * L) - p.A2E2;
-}
-
-template <class R>
-__device__ __forceinline__ R pot_R(const Photon<R>& p, R r) {
-    const R P = p.E * (r * r + p.a2) - p.aL;
-    const R delta = (r * r - R(2) * r) + p.a2;
-    return P * P - delta * p.K;
-}
-
-template <class R>
-__device__ __forceinline__ R pot_M(const Photon<R>& p, R m) {
-    const R m2 = m * m;
-    return (p.Q - p.C1 * m2) - p.A2E2 * (m2 * m2);
-}
-
-template <class R>
-__device__ __forceinline__ void rhs_main(const Photon<R>& p, const R y[5], R dy[5]) {
-    const R r = y[0], m = y[1];
-    const R r2 = r * r;
-    const R delta = (r2 - R(2) * r) + p.a2;
-    const R P = p.E * (r2 + p.a2) - p.aL;
-    dy[0] = y[3];
-    dy[1] = y[4];
-    dy[2] = -(((p.a * P) / delta - p.aE) + p.L / (R(1) - m * m));
-    dy[3] = R(2) * p.E * r * P - (r - R(1)) * p.K;
-    dy[4] = -(p.C1 * m) - R(2) * p.A2E2 * (m * m * m);
+    p.twoE = R(2) * E;
+    p.twoA2E2 = R(2) * p.A2E2;
 }
 
 template <class R>
@@ -116,21 +94,85 @@ __device__ __forceinline__ double tiny_pos<double>() { return 1e-300; }
 template <>
 __device__ __forceinline__ float tiny_pos<float>() { return 1e-30f; }
 
+// finite-range limits: 1e300 and 1e-200..1e200 in FP64, 1e30 and 1e-30..1e30 in FP32
+template <class R>
+__device__ __forceinline__ R huge_val();
+template <>
+__device__ __forceinline__ double huge_val<double>() { return 1e300; }
+template <>
+__device__ __forceinline__ float huge_val<float>() { return 1e30f; }
+template <class R>
+__device__ __forceinline__ R pow_lo();
+template <>
+__device__ __forceinline__ double pow_lo<double>() { return 1e-200; }
+template <>
+__device__ __forceinline__ float pow_lo<float>() { return 1e-30f; }
+template <class R>
+__device__ __forceinline__ R pow_hi();
+template <>
+__device__ __forceinline__ double pow_hi<double>() { return 1e200; }
+template <>
+__device__ __forceinline__ float pow_hi<float>() { return 1e30f; }
+
+// Division and square root: correctly rounded in FP64 (the step loop must round as
+// written); the hardware reciprocal / square root in FP32, whose bulk steps are judged
+// against FP64 anyway.
+__device__ __forceinline__ double qdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double qsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float qsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// P(r) = E (r^2 + a^2) - a L,  Delta(r) = r^2 - 2r + a^2
+template <class R>
+__device__ __forceinline__ R P_of(const Photon<R>& p, R r) { return fma(p.E, fma(r, r, p.a2), -p.aL); }
+template <class R>
+__device__ __forceinline__ R delta_of(const Photon<R>& p, R r) { return fma(r, r - R(2), p.a2); }
+
+template <class R>
+__device__ __forceinline__ R pot_R(const Photon<R>& p, R r) {
+    const R P = P_of(p, r);
+    return fma(P, P, -(delta_of(p, r) * p.K));
+}
+
+template <class R>
+__device__ __forceinline__ R pot_M(const Photon<R>& p, R m) {
+    const R m2 = m * m;
+    return fma(-p.A2E2, m2 * m2, fma(-p.C1, m2, p.Q));
+}
+
+// dphi/ds = aE - [a P (1 - mu^2) + L Delta] / [Delta (1 - mu^2)]: one division; the floor
+// on 1 - mu^2 keeps L = 0 rays finite at the pole (the ratio tends to aP/Delta).
+template <class R>
+__device__ __forceinline__ void rhs_main(const Photon<R>& p, const R y[5], R dy[5]) {
+    const R r = y[0], m = y[1];
+    const R delta = delta_of(p, r);
+    const R P = P_of(p, r);
+    const R s2 = fmax(fma(-m, m, R(1)), tiny_pos<R>());
+    const R num = fma(p.a * P, s2, p.L * delta);
+    dy[0] = y[3];
+    dy[1] = y[4];
+    dy[2] = p.aE - qdiv(num, delta * s2);
+    dy[3] = fma(p.twoE * r, P, -((r - R(1)) * p.K));
+    dy[4] = -(m * fma(p.twoA2E2, m * m, p.C1));
+}
+
 // tail in u = 1/r on slots (1, 2, 4) = (mu, phi, w_mu); ds/du = -1/sqrt(U),
 // U = u^4 R(1/u) = Pu^2 - u^2 Du K
 template <class R>
 __device__ __forceinline__ void rhs_tail(const Photon<R>& p, R u, const R y[5], R dy[5]) {
     const R u2 = u * u;
-    const R Pu = p.E * (R(1) + p.a2 * u2) - p.aL * u2;
-    const R Du = (R(1) - R(2) * u) + p.a2 * u2;
-    const R U = Pu * Pu - u2 * Du * p.K;
-    const R isq = R(1) / sqrt(fmax(U, tiny_pos<R>()));
+    const R Pu = fma(p.E, fma(p.a2, u2, R(1)), -(p.aL * u2));
+    const R Du = fma(p.a2, u2, fma(R(-2), u, R(1)));
+    const R U = fma(Pu, Pu, -(u2 * Du * p.K));
+    const R isq = qdiv(R(1), qsqrt(fmax(U, tiny_pos<R>())));
     const R m = y[1];
+    const R s2 = fmax(fma(-m, m, R(1)), tiny_pos<R>());
+    const R num = fma(p.a * Pu, s2, p.L * Du);
     dy[0] = R(0);
     dy[3] = R(0);
-    dy[1] = -y[4] * isq;
-    dy[2] = (((p.a * Pu) / Du - p.aE) + p.L / (R(1) - m * m)) * isq;
-    dy[4] = (p.C1 * m + R(2) * p.A2E2 * (m * m * m)) * isq;
+    dy[1] = -(y[4] * isq);
+    dy[2] = (qdiv(num, Du * s2) - p.aE) * isq;
+    dy[4] = (m * fma(p.twoA2E2, m * m, p.C1)) * isq;
 }
 
 template <class R>
@@ -145,17 +187,18 @@ __device__ __forceinline__ R pow_m15(R x) {
     const R T0 = R(1.0), T1 = R(0.8705505632961241), T2 = R(0.757858283255199), T3 = R(0.6597539553864471),
             T4 = R(0.5743491774985174);
     int e;
-    if (!(x > R(1e-30) && x < R(1e30))) return x > R(1) ? R(0) : R(1e20);  // inf/NaN/0: clamped by the caller
+    if (!(x > pow_lo<R>() && x < pow_hi<R>())) return x > R(1) ? R(0) : R(1e20);  // clamped by the caller
     const R m = frexp(x, &e);
     const int q = (e + 1100) / 5 - 220;
     const int s = e - 5 * q;
     const R t = s == 0 ? T0 : s == 1 ? T1 : s == 2 ? T2 : s == 3 ? T3 : T4;
-    R y = R(1.1487) - R(0.2974) * (m - R(0.5));
+    R y = fma(R(-0.2974), m - R(0.5), R(1.1487));
     y = ldexp(y * t, -q);
 #pragma unroll
     for (int it = 0; it < 3; ++it) {
-        const R y5 = y * y * y * y * y;
-        y = y * (R(6) - x * y5) * R(0.2);
+        const R y2 = y * y;
+        const R y5 = (y2 * y2) * y;
+        y = (y * fma(-x, y5, R(6))) * R(0.2);
     }
     return y;
 }
@@ -172,60 +215,104 @@ __device__ __forceinline__ R step_factor(R err, bool rejected_before) {
     return fac;
 }
 
+// Stage sums as FMA chains: y + h (c1 k1 + ... + cn kn).
+template <class R>
+__device__ __forceinline__ R sum2(R c1, R k1, R c2, R k2) { return fma(c2, k2, c1 * k1); }
+template <class R>
+__device__ __forceinline__ R sum3(R c1, R k1, R c2, R k2, R c3, R k3) { return fma(c3, k3, sum2(c1, k1, c2, k2)); }
+template <class R>
+__device__ __forceinline__ R sum4(R c1, R k1, R c2, R k2, R c3, R k3, R c4, R k4) {
+    return fma(c4, k4, sum3(c1, k1, c2, k2, c3, k3));
+}
+template <class R>
+__device__ __forceinline__ R sum5(R c1, R k1, R c2, R k2, R c3, R k3, R c4, R k4, R c5, R k5) {
+    return fma(c5, k5, sum4(c1, k1, c2, k2, c3, k3, c4, k4));
+}
+template <class R>
+__device__ __forceinline__ R sum6(R c1, R k1, R c2, R k2, R c3, R k3, R c4, R k4, R c5, R k5, R c6, R k6) {
+    return fma(c6, k6, sum5(c1, k1, c2, k2, c3, k3, c4, k4, c5, k5));
+}
+
+// Max-norm error: the component with the largest |e_i| / sc_i, picked by comparing cross
+// products, then one division.  NaN anywhere in e makes the error NaN.
+template <class R>
+struct ErrPick {
+    R be, bs, guard;
+    __device__ __forceinline__ ErrPick() : be(R(-1)), bs(R(1)), guard(R(0)) {}
+    __device__ __forceinline__ void add(R e, R sc) {
+        const R ae = fabs(e);
+        guard += ae;
+        if (be < R(0) || ae * bs > be * sc) {
+            be = ae;
+            bs = sc;
+        }
+    }
+    __device__ __forceinline__ R value() const { return guard != guard ? guard : qdiv(be, bs); }
+};
+
+// Dense-output coefficients of one component (Hairer's DOPRI5 continuous extension).
+template <class R>
+struct Dense {
+    R y, ydiff, bspl, c4, c5;
+    __device__ __forceinline__ R at(R th) const {
+        const R th1 = R(1) - th;
+        return fma(th, fma(th1, fma(th, fma(th1, c5, c4), bspl), ydiff), y);
+    }
+};
+
 // One DP5 attempt; k[0] = f(y) on entry; fills k[1..6] and yn; returns the scaled error.
 template <class R>
 __device__ __forceinline__ R dp5_attempt(const Photon<R>& p, bool tail, R u, const R y[5], R h, R k[7][5],
                                          R yn[5], R rtol, R atol) {
     R t[5];
 #pragma unroll
-    for (int i = 0; i < 5; ++i) t[i] = y[i] + h * (R(A21) * k[0][i]);
+    for (int i = 0; i < 5; ++i) t[i] = fma(h, R(A21) * k[0][i], y[i]);
     eval(p, tail, u + R(C2) * h, t, k[1]);
 #pragma unroll
-    for (int i = 0; i < 5; ++i) t[i] = y[i] + h * (R(A31) * k[0][i] + R(A32) * k[1][i]);
+    for (int i = 0; i < 5; ++i) t[i] = fma(h, sum2(R(A31), k[0][i], R(A32), k[1][i]), y[i]);
     eval(p, tail, u + R(C3) * h, t, k[2]);
 #pragma unroll
-    for (int i = 0; i < 5; ++i) t[i] = y[i] + h * (R(A41) * k[0][i] + R(A42) * k[1][i] + R(A43) * k[2][i]);
+    for (int i = 0; i < 5; ++i) t[i] = fma(h, sum3(R(A41), k[0][i], R(A42), k[1][i], R(A43), k[2][i]), y[i]);
     eval(p, tail, u + R(C4) * h, t, k[3]);
 #pragma unroll
     for (int i = 0; i < 5; ++i)
-        t[i] = y[i] + h * (R(A51) * k[0][i] + R(A52) * k[1][i] + R(A53) * k[2][i] + R(A54) * k[3][i]);
+        t[i] = fma(h, sum4(R(A51), k[0][i], R(A52), k[1][i], R(A53), k[2][i], R(A54), k[3][i]), y[i]);
     eval(p, tail, u + R(C5) * h, t, k[4]);
 #pragma unroll
     for (int i = 0; i < 5; ++i)
-        t[i] = y[i] + h * (R(A61) * k[0][i] + R(A62) * k[1][i] + R(A63) * k[2][i] + R(A64) * k[3][i] +
-                           R(A65) * k[4][i]);
+        t[i] = fma(h, sum5(R(A61), k[0][i], R(A62), k[1][i], R(A63), k[2][i], R(A64), k[3][i], R(A65), k[4][i]),
+                   y[i]);
     eval(p, tail, u + h, t, k[5]);
 #pragma unroll
     for (int i = 0; i < 5; ++i)
-        yn[i] = y[i] + h * (R(A71) * k[0][i] + R(A73) * k[2][i] + R(A74) * k[3][i] + R(A75) * k[4][i] +
-                            R(A76) * k[5][i]);
+        yn[i] = fma(h, sum5(R(A71), k[0][i], R(A73), k[2][i], R(A74), k[3][i], R(A75), k[4][i], R(A76), k[5][i]),
+                    y[i]);
     eval(p, tail, u + h, yn, k[6]);
-    R err = R(0);
+    ErrPick<R> ep;
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
         if (tail && (i == 0 || i == 3)) continue;
-        const R e = h * (R(E1) * k[0][i] + R(E3) * k[2][i] + R(E4) * k[3][i] + R(E5) * k[4][i] +
-                         R(E6) * k[5][i] + R(E7) * k[6][i]);
+        const R e = h * sum6(R(E1), k[0][i], R(E3), k[2][i], R(E4), k[3][i], R(E5), k[4][i], R(E6), k[5][i],
+                             R(E7), k[6][i]);
         const R big = amax(fabs(y[i]), fabs(yn[i]));
         R sc;
         if (!tail && i == 0) sc = rtol * big;
         else if (!tail && i == 3) sc = rtol * amax(big, y[0] * y[0]);
-        else sc = atol + rtol * big;
-        const R q = fabs(e) / sc;
-        if (!(q <= err)) err = q;
+        else sc = fma(rtol, big, atol);
+        ep.add(e, sc);
     }
-    return err;
+    return ep.value();
 }
 
 template <class R>
-__device__ __forceinline__ R dense(const R y[5], const R yn[5], const R k[7][5], R h, int i, R th) {
-    const R ydiff = yn[i] - y[i];
-    const R bspl = h * k[0][i] - ydiff;
-    const R c4 = ydiff - h * k[6][i] - bspl;
-    const R c5 = h * (R(D1) * k[0][i] + R(D3) * k[2][i] + R(D4) * k[3][i] + R(D5) * k[4][i] + R(D6) * k[5][i] +
-                      R(D7) * k[6][i]);
-    const R th1 = R(1) - th;
-    return y[i] + th * (ydiff + th1 * (bspl + th * (c4 + th1 * c5)));
+__device__ __forceinline__ Dense<R> dense_coeffs(const R y[5], const R yn[5], const R k[7][5], R h, int i) {
+    Dense<R> d;
+    d.y = y[i];
+    d.ydiff = yn[i] - y[i];
+    d.bspl = fma(h, k[0][i], -d.ydiff);
+    d.c4 = fma(-h, k[6][i], d.ydiff - d.bspl);
+    d.c5 = h * sum6(R(D1), k[0][i], R(D3), k[2][i], R(D4), k[3][i], R(D5), k[4][i], R(D6), k[5][i], R(D7), k[6][i]);
+    return d;
 }
 
 // Constraint projection, src/kerr.cpp:521-542 (kTurningZone = 0.05) in (r, mu) form:
@@ -235,15 +322,13 @@ __device__ __forceinline__ R dense(const R y[5], const R yn[5], const R k[7][5],
 template <class R>
 __device__ __forceinline__ void project(const Photon<R>& p, R y[5]) {
     const R r = y[0], m = y[1];
-    const R r2 = r * r;
-    const R P = p.E * (r2 + p.a2) - p.aL;
-    const R delta = (r2 - R(2) * r) + p.a2;
-    const R PP = P * P;
-    const R Rr = PP - delta * p.K;
-    if (Rr > R(0.05) * (PP + fabs(delta) * p.K)) y[3] = copysign(sqrt(Rr), y[3]);
+    const R P = P_of(p, r);
+    const R delta = delta_of(p, r);
+    const R Rr = fma(P, P, -(delta * p.K));
+    if (Rr > R(0.05) * fma(P, P, fabs(delta) * p.K)) y[3] = copysign(sqrt(Rr), y[3]);
     const R m2 = m * m, s2 = R(1) - m2;
-    const R M = (p.Q - p.C1 * m2) - p.A2E2 * (m2 * m2);
-    const R scale = (fabs(p.Q) * s2 + p.A2E2 * (m2 * s2)) + p.L * p.L * m2;
+    const R M = fma(-p.A2E2, m2 * m2, fma(-p.C1, m2, p.Q));
+    const R scale = fma(p.L * p.L, m2, fma(fabs(p.Q), s2, p.A2E2 * (m2 * s2)));
     if (M > R(0.05) * scale) y[4] = copysign(sqrt(M), y[4]);
 }
 
@@ -276,13 +361,71 @@ struct Outputs {
 };
 
 struct Result {
-    int cls;
+    int cls, flags;
     double theta, phi, r_hit, g;
 };
 
-// Launch: per-pixel photon of a view (Bardeen plane or pinhole camera).
-__device__ __forceinline__ void launch_photon(const ViewDev& v, long long local, Photon<double>& p, double y0[5]) {
+// A launched ray: its photon, initial state and the radii its events test.
+struct Launch {
+    Photon<double> p;
+    double y0[5];
+    double r_cap, r_esc, r_in, r_out, h0, r_fp64;
+    int disk, prograde;
+};
+
+// Bardeen launch (src/eht.cpp:262-271): E = 1, L = -alpha sin i,
+// Q = beta^2 + (alpha^2 - a^2) cos^2 i, inward, toward smaller theta when beta > 0.
+__device__ __forceinline__ void bardeen_photon(double a, double r_cam, double phi_cam, double sinI, double cosI,
+                                               double m0, double alpha, double beta, Photon<double>& p, double y0[5]) {
+    const double L = -alpha * sinI;
+    const double Q = beta * beta + (alpha * alpha - a * a) * cosI * cosI;
+    photon_init(p, a, 1.0, L, Q);
+    y0[0] = r_cam;
+    y0[1] = m0;
+    y0[2] = phi_cam;
+    y0[3] = -sqrt(fmax(pot_R(p, r_cam), 0.0));
+    y0[4] = (beta > 0.0 ? 1.0 : -1.0) * sqrt(fmax(pot_M(p, m0), 0.0));
+}
+
+// BPT ISCO radius (src/kerr.cpp:179-190)
+__device__ __forceinline__ double isco_dev(double a, int prograde) {
+    const double z1 = 1.0 + cbrt(1.0 - a * a) * (cbrt(1.0 + a) + cbrt(1.0 - a));
+    const double z2 = sqrt(3.0 * a * a + z1 * z1);
+    const double root = sqrt(fmax(0.0, (3.0 - z1) * (3.0 + z1 + 2.0 * z2)));
+    return prograde ? 3.0 + z2 - root : 3.0 + z2 + root;
+}
+
+// Launch: the photon of ray `local` of a view -- a pixel of a Bardeen plane (kind 0) or of
+// the pinhole camera (kind 1), an explicit (alpha, beta) or look, or (kind 2) a ray with
+// its own camera: params hold (a, r_cam, inclination, alpha, beta) per ray and the
+// per-ray constants are derived here with device libm.
+__device__ __forceinline__ void launch_photon(const ViewDev& v, long long local, Launch& L) {
+    L.r_cap = v.r_cap;
+    L.r_esc = v.r_esc;
+    L.r_in = v.r_in;
+    L.r_out = v.r_out;
+    L.h0 = v.h0;
+    L.r_fp64 = v.r_fp64;
+    L.disk = v.disk;
+    L.prograde = v.prograde;
+    Photon<double>& p = L.p;
+    double* y0 = L.y0;
     const double a = v.a;
+    if (v.kind == 2) {
+        const double* q = v.params + 5 * local;
+        const double ar = q[0], rc = q[1], inc = q[2];
+        const double rp = 1.0 + sqrt(fmax(0.0, 1.0 - ar * ar));
+        double th = inc;
+        if (th < 1e-4) th = 1e-4;
+        if (th > kPi - 1e-4) th = kPi - 1e-4;
+        L.r_cap = v.r_cap * rp;  // kind 2 stores the capture factor in r_cap
+        L.r_esc = v.escape_radius_ > 1.5 * rc ? v.escape_radius_ : 1.5 * rc;
+        L.r_in = isco_dev(ar, v.prograde);
+        L.h0 = 0.01 / rc;
+        L.r_fp64 = 1.2 * rp;
+        bardeen_photon(ar, rc, 0.0, sin(inc), cos(inc), cos(th), q[3], q[4], p, y0);
+        return;
+    }
     if (v.kind == 0) {  // Bardeen image plane, src/eht.cpp:262-271, 332-334
         double alpha, beta;
         if (v.params) {
@@ -293,14 +436,7 @@ __device__ __forceinline__ void launch_photon(const ViewDev& v, long long local,
             alpha = -v.half_width + ((double)col + 0.5) * v.px;
             beta = v.half_height - ((double)row + 0.5) * v.py;
         }
-        const double L = -alpha * v.sinI;
-        const double Q = beta * beta + (alpha * alpha - a * a) * v.cosI * v.cosI;
-        photon_init(p, a, 1.0, L, Q);
-        y0[0] = v.r_cam;
-        y0[1] = v.m0;
-        y0[2] = v.phi_cam;
-        y0[3] = -sqrt(fmax(pot_R(p, v.r_cam), 0.0));
-        y0[4] = (beta > 0.0 ? 1.0 : -1.0) * sqrt(fmax(pot_M(p, v.m0), 0.0));
+        bardeen_photon(a, v.r_cam, v.phi_cam, v.sinI, v.cosI, v.m0, alpha, beta, p, y0);
         return;
     }
     // pinhole camera: trace.frag:1494-1496 + launchPhoton (trace.frag:1470-1488)
@@ -328,14 +464,14 @@ __device__ __forceinline__ void launch_photon(const ViewDev& v, long long local,
     const double qh = ((v.lh[0] - n0 * v.lh[1]) - n1 * v.lh[2]) - n2 * v.lh[3];
     const double qr = ((v.er[0] - n0 * v.er[1]) - n1 * v.er[2]) - n2 * v.er[3];
     const double E = -qt;
-    double L = qp;
+    double Lz = qp;
     const double cs2 = v.cs * v.cs, sn2 = fmax(v.sn * v.sn, 1e-18), a2 = a * a;
-    double Q = qh * qh + cs2 * (L * L / sn2 - a2 * E * E);
-    if (fabs(L) < kAxisSnap * sqrt(fmax(Q + a2 * E * E, 0.0))) {
-        L = 0.0;
+    double Q = qh * qh + cs2 * (Lz * Lz / sn2 - a2 * E * E);
+    if (fabs(Lz) < kAxisSnap * sqrt(fmax(Q + a2 * E * E, 0.0))) {
+        Lz = 0.0;
         Q = qh * qh - cs2 * a2 * E * E;
     }
-    photon_init(p, a, E, L, Q);
+    photon_init(p, a, E, Lz, Q);
     y0[0] = v.r_cam;
     y0[1] = v.cs;
     y0[2] = v.phi_cam;
@@ -344,7 +480,9 @@ __device__ __forceinline__ void launch_photon(const ViewDev& v, long long local,
 }
 
 template <class R>
-__device__ __forceinline__ void lane_start(Lane<R>& s, const ViewDev& v, const Photon<double>& p, const double y0[5]) {
+__device__ __forceinline__ void lane_start(Lane<R>& s, const Launch& la) {
+    const Photon<double>& p = la.p;
+    const double* y0 = la.y0;
     s.p.a = R(p.a);
     s.p.a2 = R(p.a2);
     s.p.E = R(p.E);
@@ -355,40 +493,30 @@ __device__ __forceinline__ void lane_start(Lane<R>& s, const ViewDev& v, const P
     s.p.K = R(p.K);
     s.p.C1 = R(p.C1);
     s.p.A2E2 = R(p.A2E2);
+    s.p.twoE = R(p.twoE);
+    s.p.twoA2E2 = R(p.twoA2E2);
 #pragma unroll
     for (int i = 0; i < 5; ++i) s.y[i] = R(y0[i]);
-    s.h = R(v.h0);
+    s.h = R(la.h0);
     s.u = R(0);
-    s.r_cap = R(v.r_cap);
-    s.r_esc = R(v.r_esc);
-    s.r_in = R(v.r_in);
-    s.r_out = R(v.r_out);
+    s.r_cap = R(la.r_cap);
+    s.r_esc = R(la.r_esc);
+    s.r_in = R(la.r_in);
+    s.r_out = R(la.r_out);
     s.steps = 0;
     s.rejected = false;
     s.tail = false;
-    s.disk = v.disk != 0;
-    s.prograde = v.prograde != 0;
+    s.disk = la.disk != 0;
+    s.prograde = la.prograde != 0;
     rhs_main(s.p, s.y, s.k1);
 }
 
-// One DP5 attempt plus events.  Returns true when the ray has ended (res filled).
+// After an attempt: the controller, the pole, the projection and the events.  k7 is the
+// attempt's f(yn); dr, dm its dense output of r and mu when the step crossed the plane.
+// Returns true when the ray has ended (res filled).
 template <class R>
-__device__ __forceinline__ bool lane_step(Lane<R>& s, R rtol, R atol, int max_steps, Result& res) {
-    if (s.steps >= max_steps) {
-        res.cls = kUnfinished;
-        return true;
-    }
-    R k[7][5], yn[5];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) k[0][i] = s.k1[i];
-    bool last = false;
-    if (s.tail && s.h <= -s.u) {
-        s.h = -s.u;
-        last = true;
-    }
-    const R h = s.h;
-    const R err = dp5_attempt(s.p, s.tail, s.u, s.y, h, k, yn, rtol, atol);
-    ++s.steps;
+__device__ __forceinline__ bool lane_finish(Lane<R>& s, R err, R h, bool last, R yn[5], const R k7[5], bool crossing,
+                                            const Dense<R>& dr, const Dense<R>& dm, Result& res) {
     if (!(err <= R(1))) {
         s.h = h * step_factor(err, true);
         s.rejected = true;
@@ -403,7 +531,7 @@ __device__ __forceinline__ bool lane_step(Lane<R>& s, R rtol, R atol, int max_st
         s.y[2] = yn[2];
         s.y[4] = yn[4];
 #pragma unroll
-        for (int i = 0; i < 5; ++i) s.k1[i] = k[6][i];
+        for (int i = 0; i < 5; ++i) s.k1[i] = k7[i];
         if (last) {
             R m = s.y[1];
             if (m > R(1)) m = R(1);
@@ -419,16 +547,21 @@ __device__ __forceinline__ bool lane_step(Lane<R>& s, R rtol, R atol, int max_st
     }
 
     project(s.p, yn);
-    if (s.disk && s.y[1] * yn[1] < R(0)) {
+    if (!(fabs(yn[0]) < huge_val<R>()) || !(fabs(yn[1]) < huge_val<R>())) {  // non-finite: captured (eht.cpp:285)
+        res.cls = kCaptured;
+        res.flags |= kFlagNonfinite;
+        return true;
+    }
+    if (crossing) {
         R lo = R(0), hi = R(1);
         const bool pos = s.y[1] > R(0);
         for (int it = 0; it < 48; ++it) {
             const R mid = R(0.5) * (lo + hi);
-            const R f = dense(s.y, yn, k, h, 1, mid);
+            const R f = dm.at(mid);
             if ((f > R(0)) == pos) lo = mid;
             else hi = mid;
         }
-        const R rc = dense(s.y, yn, k, h, 0, R(0.5) * (lo + hi));
+        const R rc = dr.at(R(0.5) * (lo + hi));
         if (rc >= s.r_in && rc <= s.r_out) {
             // trajectory.cpp:311-321, in FP64
             const double a = (double)s.p.a, r = (double)rc;
@@ -461,16 +594,157 @@ __device__ __forceinline__ bool lane_step(Lane<R>& s, R rtol, R atol, int max_st
 #pragma unroll
     for (int i = 0; i < 5; ++i) s.y[i] = yn[i];
 #pragma unroll
-    for (int i = 0; i < 5; ++i) s.k1[i] = k[6][i];
+    for (int i = 0; i < 5; ++i) s.k1[i] = k7[i];
     s.k1[0] = s.y[3];  // FSAL after the projection: only dr/ds = w_r, dmu/ds = w_mu move
     s.k1[1] = s.y[4];
     s.h = h * fac;
     return false;
 }
 
+// One DP5 attempt plus events, all in precision R.
+template <class R>
+__device__ __forceinline__ bool lane_step(Lane<R>& s, R rtol, R atol, int max_steps, Result& res) {
+    res.flags = 0;
+    if (s.steps >= max_steps) {
+        res.cls = kUnfinished;
+        return true;
+    }
+    R k[7][5], yn[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) k[0][i] = s.k1[i];
+    bool last = false;
+    if (s.tail && s.h <= -s.u) {
+        s.h = -s.u;
+        last = true;
+    }
+    const R h = s.h;
+    const R err = dp5_attempt(s.p, s.tail, s.u, s.y, h, k, yn, rtol, atol);
+    ++s.steps;
+    // dense output for the disk crossing, taken now so the stage vectors die here
+    const bool crossing = !s.tail && s.disk && s.y[1] * yn[1] < R(0);
+    Dense<R> dr, dm;
+    if (crossing) {
+        dr = dense_coeffs(s.y, yn, k, h, 0);
+        dm = dense_coeffs(s.y, yn, k, h, 1);
+    }
+    return lane_finish(s, err, h, last, yn, k[6], crossing, dr, dm, res);
+}
+
+// ---- mixed precision --------------------------------------------------------------
+//
+// The state and the controller live in FP64; a step is attempted in FP32 unless the ray
+// is in an FP64 zone: r < 1.2 r+, or inside the turning-point zones of kerr.cpp:521-542
+// (R or M below 0.05 of their scale), or in the tail.  An FP32 step that passes the
+// error test but moves H = 0 by more than `drift` (|w_r^2 - R| or |w_mu^2 - M| relative
+// to their scales; E, L and Q are exact constants here) is thrown away and re-taken in
+// FP64 with the same h.  Both precisions use the mixed tolerances.
+struct MixedLane {
+    Lane<double> d;
+    Photon<float> pf;
+    float r_fp64;     // 1.2 r+
+    int fp64_steps;
+};
+
+__device__ __forceinline__ bool fp64_zone(const MixedLane& s) {
+    if (s.d.tail) return true;
+    const float r = (float)s.d.y[0], m = (float)s.d.y[1];
+    if (r < s.r_fp64) return true;
+    const Photon<float>& p = s.pf;
+    const float P = P_of(p, r), delta = delta_of(p, r);
+    const float Rr = fmaf(P, P, -(delta * p.K));
+    if (!(Rr > 0.05f * fmaf(P, P, fabsf(delta) * p.K))) return true;
+    const float m2 = m * m, s2 = 1.0f - m2;
+    const float M = fmaf(-p.A2E2, m2 * m2, fmaf(-p.C1, m2, p.Q));
+    const float scale = fmaf(p.L * p.L, m2, fmaf(fabsf(p.Q), s2, p.A2E2 * (m2 * s2)));
+    return !(M > 0.05f * scale);
+}
+
+// relative violation of H = 0 by an FP32 state
+__device__ __forceinline__ float h_drift(const Photon<float>& p, const float y[5]) {
+    const float r = y[0], m = y[1];
+    const float P = P_of(p, r), delta = delta_of(p, r);
+    const float dr = qdiv(fabsf(fmaf(y[3], y[3], -fmaf(P, P, -(delta * p.K)))), fmaf(P, P, fabsf(delta) * p.K));
+    const float m2 = m * m, s2 = 1.0f - m2;
+    const float M = fmaf(-p.A2E2, m2 * m2, fmaf(-p.C1, m2, p.Q));
+    const float scale = fmaf(p.L * p.L, m2, fmaf(fabsf(p.Q), s2, p.A2E2 * (m2 * s2))) + 1e-30f;
+    const float dm = qdiv(fabsf(fmaf(y[4], y[4], -M)), scale);
+    return dr > dm ? dr : dm;
+}
+
+__device__ __forceinline__ Dense<double> widen(const Dense<float>& f) {
+    Dense<double> d;
+    d.y = f.y;
+    d.ydiff = f.ydiff;
+    d.bspl = f.bspl;
+    d.c4 = f.c4;
+    d.c5 = f.c5;
+    return d;
+}
+
+__device__ __forceinline__ bool lane_step_mixed(MixedLane& s, double rtol, double atol, float drift, int max_steps,
+                                                Result& res) {
+    Lane<double>& d = s.d;
+    res.flags = 0;
+    if (d.steps >= max_steps) {
+        res.cls = kUnfinished;
+        return true;
+    }
+    bool last = false;
+    if (d.tail && d.h <= -d.u) {
+        d.h = -d.u;
+        last = true;
+    }
+    const double h = d.h;
+    double yn[5], k7[5], err = 0.0;
+    bool crossing = false;
+    Dense<double> dr, dm;
+    bool use64 = fp64_zone(s);
+    if (!use64) {
+        float y[5], k[7][5], ynf[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            y[i] = (float)d.y[i];
+            k[0][i] = (float)d.k1[i];
+        }
+        const float hf = (float)h;
+        const float errf = dp5_attempt(s.pf, false, 0.0f, y, hf, k, ynf, (float)rtol, (float)atol);
+        if (errf <= 1.0f && h_drift(s.pf, ynf) > drift) {
+            use64 = true;  // the re-step
+        } else {
+            err = errf;
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                yn[i] = ynf[i];
+                k7[i] = k[6][i];
+            }
+            crossing = d.disk && y[1] * ynf[1] < 0.0f;
+            if (crossing) {
+                dr = widen(dense_coeffs(y, ynf, k, hf, 0));
+                dm = widen(dense_coeffs(y, ynf, k, hf, 1));
+            }
+        }
+    }
+    if (use64) {
+        double k[7][5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) k[0][i] = d.k1[i];
+        err = dp5_attempt(d.p, d.tail, d.u, d.y, h, k, yn, rtol, atol);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) k7[i] = k[6][i];
+        crossing = !d.tail && d.disk && d.y[1] * yn[1] < 0.0;
+        if (crossing) {
+            dr = dense_coeffs(d.y, yn, k, h, 0);
+            dm = dense_coeffs(d.y, yn, k, h, 1);
+        }
+        ++s.fp64_steps;
+    }
+    ++d.steps;
+    return lane_finish(d, err, h, last, yn, k7, crossing, dr, dm, res);
+}
+
 __device__ __forceinline__ void write_result(const Outputs& o, long long idx, const Result& r, int steps, int flags) {
     const double nan = __builtin_nan("");
-    if (o.cls) o.cls[idx] = (int8_t)(r.cls | flags);
+    if (o.cls) o.cls[idx] = (int8_t)(r.cls | r.flags | flags);
     if (o.theta) o.theta[idx] = r.cls == kEscaped ? r.theta : nan;
     if (o.phi) o.phi[idx] = r.cls == kEscaped ? r.phi : nan;
     if (o.r_hit) o.r_hit[idx] = r.cls == kDisk ? r.r_hit : nan;

@@ -181,6 +181,7 @@ ViewDev make_view_dev(const kt_view& v, long long offset) {
     d.r_in = v.disk_inner > 0.0 ? v.disk_inner : isco(a, v.disk_prograde);
     d.r_out = v.disk_outer;
     d.h0 = 0.01 / v.r_cam;
+    d.r_fp64 = 1.2 * horizon(a);
     if (v.camera_kind == KT_CAMERA_PINHOLE) {
         const double r = v.r_cam, t = v.theta_cam;
         double u[4], e[4][4], g[4][4], low[4][4];
@@ -221,6 +222,20 @@ struct kt_ctx {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timed = false;
     kts::Surrogate* surrogate = nullptr;
+    // autotuned launch shape per precision (0 = not tuned)
+    unsigned long long* stats_src = nullptr;  // counters of the last launch (a plan's or d_counter)
+    int tuned_mode[3] = {0, 0, 0};  // 1 + KT_MODE_* once tuned
+    int tuned_block[3] = {0, 0, 0}, tuned_wpc[3] = {0, 0, 0}, tuned_spd[3] = {0, 0, 0}, tuned_wps[3] = {0, 0, 0};
+};
+
+struct kt_plan {
+    kt_ctx* ctx = nullptr;
+    ViewDev* d_views = nullptr;
+    unsigned long long* d_counter = nullptr;
+    hipStream_t capture = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    kti::TraceLaunch L{};
 };
 
 namespace {
@@ -280,12 +295,53 @@ int ensure_views(kt_ctx* ctx, int n) {
 }
 
 // tuned defaults for one MI355X (see the autotuner in black_simulation_amd/autotune.py)
-void launch_shape(const kt_ctx* ctx, const kt_trace_opts& o, int& block, int& grid, int& spd) {
-    block = o.block > 0 ? o.block : 256;
-    const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : (o.precision == KT_PREC_FP64 ? 8 : 16);
-    spd = o.steps_per_dequeue > 0 ? o.steps_per_dequeue : 4;
+void launch_shape(const kt_ctx* ctx, const kt_trace_opts& o, int& block, int& grid, int& spd, int& wps) {
+    const int pr = std::clamp(o.precision, 0, 2);
+    block = o.block > 0 ? o.block : (ctx->tuned_block[pr] > 0 ? ctx->tuned_block[pr] : 256);
+    const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu
+                                       : (ctx->tuned_wpc[pr] > 0 ? ctx->tuned_wpc[pr] : (pr == KT_PREC_FP64 ? 8 : 16));
+    spd = o.steps_per_dequeue > 0 ? o.steps_per_dequeue : (ctx->tuned_spd[pr] > 0 ? ctx->tuned_spd[pr] : 4);
+    wps = o.waves_per_simd > 0 ? std::min(o.waves_per_simd, 4) : (ctx->tuned_wps[pr] > 0 ? ctx->tuned_wps[pr] : 1);
     const int waves = std::max(1, ctx->cu_count) * wpc;
     grid = std::max(1, (waves * 64 + block - 1) / block);
+}
+
+// validates views and builds their device table; returns KT_OK or a status
+int prepare_views(kt_ctx* ctx, const kt_view* views, int n_views, std::vector<ViewDev>& vd, long long& total) {
+    if (!views || n_views <= 0) return fail(ctx, KT_EINVAL, "no views");
+    vd.clear();
+    vd.reserve(n_views);
+    total = 0;
+    for (int i = 0; i < n_views; ++i) {
+        if (const char* why = check_view(views[i]))
+            return fail(ctx, KT_EINVAL, "view " + std::to_string(i) + ": " + why);
+        vd.push_back(make_view_dev(views[i], total));
+        total += (long long)views[i].width * views[i].height;
+    }
+    return KT_OK;
+}
+
+kti::TraceLaunch make_launch(kt_ctx* ctx, const ViewDev* d_views, int n_views, long long total,
+                             unsigned long long* counter, const ktd::Outputs& dout, const kt_trace_opts& o) {
+    kti::TraceLaunch L{};
+    L.views = d_views;
+    L.n_views = n_views;
+    L.total = total;
+    L.counter = counter;
+    L.stats = counter + 1;
+    L.out = dout;
+    L.ray_list = nullptr;
+    L.rtol = o.rtol;
+    L.atol = o.atol;
+    L.rtol32 = o.rtol32;
+    L.atol32 = o.atol32;
+    L.restep_drift = o.restep_drift;
+    L.max_steps = o.max_steps;
+    L.precision = o.precision;
+    const int pr = std::clamp(o.precision, 0, 2);
+    L.mode = o.mode == KT_MODE_AUTO ? (ctx->tuned_mode[pr] ? ctx->tuned_mode[pr] - 1 : KT_MODE_PERSISTENT) : o.mode;
+    launch_shape(ctx, o, L.block, L.grid, L.steps_per_dequeue, L.waves_per_simd);
+    return L;
 }
 
 int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const kt_trace_opts& o,
@@ -329,29 +385,15 @@ int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const k
 
     KT_HIP(hipMemcpyAsync(ctx->d_views, vd.data(), sizeof(ViewDev) * vd.size(), hipMemcpyHostToDevice, stream));
 
-    kti::TraceLaunch L{};
-    L.views = ctx->d_views;
-    L.n_views = (int)vd.size();
-    L.total = total;
-    L.counter = ctx->d_counter;
-    L.stats = ctx->d_counter + 1;
-    L.out = dout;
-    L.ray_list = nullptr;
-    L.rtol = o.rtol;
-    L.atol = o.atol;
-    L.rtol32 = o.rtol32;
-    L.atol32 = o.atol32;
-    L.restep_drift = o.restep_drift;
-    L.max_steps = o.max_steps;
-    L.precision = o.precision;
-    L.mode = o.mode;
-    launch_shape(ctx, o, L.block, L.grid, L.steps_per_dequeue);
+    kti::TraceLaunch L = make_launch(ctx, ctx->d_views, (int)vd.size(), total, ctx->d_counter, dout, o);
 
     ctx->stats = kt_stats{};
     ctx->stats.rays = total;
     ctx->stats.block = L.block;
     ctx->stats.grid = L.grid;
     ctx->stats.steps_per_dequeue = L.steps_per_dequeue;
+    ctx->stats.waves_per_simd = L.waves_per_simd;
+    ctx->stats.mode = L.mode;
     ctx->stats.waves_per_cu = (L.grid * L.block / 64) / std::max(1, ctx->cu_count);
 
     if (o.use_surrogate) {
@@ -369,6 +411,7 @@ int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const k
         KT_HIP(hipEventRecord(ctx->ev[3], stream));
     }
     ctx->timed = true;
+    ctx->stats_src = ctx->d_counter;
 
     if (!out->on_device) {
         const size_t n = (size_t)total;
@@ -391,7 +434,7 @@ void kt_trace_opts_default(kt_trace_opts* o) {
     if (!o) return;
     std::memset(o, 0, sizeof *o);
     o->precision = KT_PREC_FP64;
-    o->mode = KT_MODE_PERSISTENT;
+    o->mode = KT_MODE_AUTO;
     o->use_surrogate = 0;
     o->max_steps = 200000;
     o->rtol = 1e-10;
@@ -483,10 +526,11 @@ int kt_last_stats(const kt_ctx* ctx_c, kt_stats* s) {
         if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) == hipSuccess) ctx->stats.integrator_ms = ms;
         ctx->stats.surrogate_ms = ctx->stats.kernel_ms - ctx->stats.integrator_ms;
         unsigned long long h[1 + kti::kStatCount];
-        KT_HIP(hipMemcpy(h, ctx->d_counter, sizeof h, hipMemcpyDeviceToHost));
+        KT_HIP(hipMemcpy(h, ctx->stats_src ? ctx->stats_src : ctx->d_counter, sizeof h, hipMemcpyDeviceToHost));
         ctx->stats.steps = (long long)h[1 + kti::kStatSteps];
         ctx->stats.restep_rays = (long long)h[1 + kti::kStatRestepRays];
         ctx->stats.fallback_rays = (long long)h[1 + kti::kStatFallback];
+        ctx->stats.fp64_steps = (long long)h[1 + kti::kStatFp64Steps];
         ctx->timed = false;
     }
     *s = ctx->stats;
@@ -502,14 +546,8 @@ int kt_trace_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
         fill_defaults(opts, o);
         if (o.precision < 0 || o.precision > 2) return fail(ctx, KT_EINVAL, "unknown precision");
         std::vector<ViewDev> vd;
-        vd.reserve(n_views);
         long long total = 0;
-        for (int i = 0; i < n_views; ++i) {
-            if (const char* why = check_view(views[i]))
-                return fail(ctx, KT_EINVAL, "view " + std::to_string(i) + ": " + why);
-            vd.push_back(make_view_dev(views[i], total));
-            total += (long long)views[i].width * views[i].height;
-        }
+        if (int st = prepare_views(ctx, views, n_views, vd, total)) return st;
         return trace_common(ctx, vd, total, o, out, static_cast<hipStream_t>(hip_stream));
     } catch (const std::exception& e) {
         return fail(ctx, KT_EINVAL, e.what());
@@ -557,6 +595,179 @@ int kt_trace_rays(kt_ctx* ctx, const kt_view* view, const double* params, long l
     } catch (...) {
         return fail(ctx, KT_EINVAL, "unknown error");
     }
+}
+
+int kt_autotune(kt_ctx* ctx, const kt_view* views, int n_views, kt_trace_opts* opts) {
+    if (!ctx || !opts) return KT_EINVAL;
+    try {
+        kt_trace_opts o;
+        fill_defaults(opts, o);
+        if (o.precision < 0 || o.precision > 2) return fail(ctx, KT_EINVAL, "unknown precision");
+        o.mode = KT_MODE_PERSISTENT;
+        o.use_surrogate = 0;
+        std::vector<ViewDev> vd;
+        long long total = 0;
+        if (int st = prepare_views(ctx, views, n_views, vd, total)) return st;
+        KT_HIP(hipSetDevice(ctx->device));
+        if (ensure_views(ctx, (int)vd.size()) != KT_OK) return KT_ENOMEM;
+        if (ensure_stage(ctx, (size_t)total + 256) != KT_OK) return KT_ENOMEM;
+        KT_HIP(hipMemcpy(ctx->d_views, vd.data(), sizeof(ViewDev) * vd.size(), hipMemcpyHostToDevice));
+        ktd::Outputs dout{};
+        dout.cls = static_cast<int8_t*>(ctx->d_stage);  // classes only: the timing is the same
+        auto time_of = [&](const kt_trace_opts& c) -> double {
+            kti::TraceLaunch L = make_launch(ctx, ctx->d_views, (int)vd.size(), total, ctx->d_counter, dout, c);
+            double best = 1e30;
+            for (int rep = 0; rep < 2; ++rep) {
+                if (hipEventRecord(ctx->ev[1], nullptr) != hipSuccess) return 1e30;
+                if (kti::launch_trace(L, nullptr) != hipSuccess) return 1e30;
+                if (hipEventRecord(ctx->ev[2], nullptr) != hipSuccess) return 1e30;
+                if (hipEventSynchronize(ctx->ev[2]) != hipSuccess) return 1e30;
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) != hipSuccess) return 1e30;
+                best = std::min(best, (double)ms);
+            }
+            return best;
+        };
+        kt_trace_opts cur = o;
+        int b, g, sp, w;
+        launch_shape(ctx, cur, b, g, sp, w);
+        cur.block = b;
+        cur.waves_per_cu = std::max(1, (g * b / 64) / std::max(1, ctx->cu_count));
+        cur.steps_per_dequeue = sp;
+        cur.waves_per_simd = w;
+        // the persistent work queue, tuned by coordinate descent over its four knobs
+        double best = time_of(cur);
+        const std::vector<int> wps_c = {1, 2, 3, 4}, wpc_c = {4, 8, 12, 16, 24, 32}, spd_c = {1, 2, 4, 8, 16},
+                               blk_c = {64, 128, 256};
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int knob = 0; knob < 4; ++knob) {
+                const std::vector<int>& vals = knob == 0 ? wps_c : knob == 1 ? wpc_c : knob == 2 ? spd_c : blk_c;
+                for (int v : vals) {
+                    kt_trace_opts c = cur;
+                    int& f = knob == 0 ? c.waves_per_simd : knob == 1 ? c.waves_per_cu
+                                                          : knob == 2 ? c.steps_per_dequeue : c.block;
+                    if (f == v) continue;
+                    f = v;
+                    const double t = time_of(c);
+                    if (t < best) {
+                        best = t;
+                        cur = c;
+                    }
+                }
+            }
+        }
+        // against one lane per ray without refill (the hardware's block scheduler balances)
+        for (int blk : blk_c) {
+            kt_trace_opts c = cur;
+            c.mode = KT_MODE_ONESHOT;
+            c.block = blk;
+            const double t = time_of(c);
+            if (t < best) {
+                best = t;
+                cur = c;
+            }
+        }
+        const int pr = o.precision;
+        ctx->tuned_mode[pr] = cur.mode + 1;
+        ctx->tuned_block[pr] = cur.block;
+        ctx->tuned_wpc[pr] = cur.waves_per_cu;
+        ctx->tuned_spd[pr] = cur.steps_per_dequeue;
+        ctx->tuned_wps[pr] = cur.waves_per_simd;
+        opts->mode = cur.mode;
+        opts->block = cur.block;
+        opts->waves_per_cu = cur.waves_per_cu;
+        opts->steps_per_dequeue = cur.steps_per_dequeue;
+        opts->waves_per_simd = cur.waves_per_simd;
+        ctx->stats = kt_stats{};
+        ctx->stats.integrator_ms = ctx->stats.kernel_ms = best;
+        ctx->stats.rays = total;
+        ctx->timed = false;
+        return KT_OK;
+    } catch (const std::exception& e) {
+        return fail(ctx, KT_EINVAL, e.what());
+    }
+}
+
+int kt_plan_create(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trace_opts* opts, const kt_rays_out* out,
+                   kt_plan** plan_out) {
+    if (!ctx || !plan_out) return KT_EINVAL;
+    *plan_out = nullptr;
+    try {
+        if (!out || !out->on_device) return fail(ctx, KT_EINVAL, "a plan needs device output pointers");
+        kt_trace_opts o;
+        fill_defaults(opts, o);
+        if (o.use_surrogate) return fail(ctx, KT_EUNSUPPORTED, "surrogate plans are created by kt_surrogate_plan");
+        std::vector<ViewDev> vd;
+        long long total = 0;
+        if (int st = prepare_views(ctx, views, n_views, vd, total)) return st;
+        KT_HIP(hipSetDevice(ctx->device));
+        kt_plan* p = new kt_plan();
+        p->ctx = ctx;
+        auto cleanup = [&](int code, const char* where, hipError_t e) {
+            hip_fail(ctx, e, where);
+            kt_plan_destroy(p);
+            return code;
+        };
+        hipError_t e;
+        if ((e = hipMalloc(&p->d_views, sizeof(ViewDev) * vd.size())) != hipSuccess)
+            return cleanup(KT_ENOMEM, "plan views", e);
+        if ((e = hipMalloc(&p->d_counter, sizeof(unsigned long long) * (1 + kti::kStatCount))) != hipSuccess)
+            return cleanup(KT_ENOMEM, "plan counters", e);
+        if ((e = hipMemcpy(p->d_views, vd.data(), sizeof(ViewDev) * vd.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return cleanup(KT_EHIP, "plan upload", e);
+        ktd::Outputs dout{out->cls, out->theta_inf, out->phi_inf, out->r_hit, out->g, out->steps};
+        p->L = make_launch(ctx, p->d_views, (int)vd.size(), total, p->d_counter, dout, o);
+        if ((e = hipStreamCreateWithFlags(&p->capture, hipStreamNonBlocking)) != hipSuccess)
+            return cleanup(KT_EHIP, "plan stream", e);
+        if ((e = hipStreamBeginCapture(p->capture, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+            return cleanup(KT_EHIP, "begin capture", e);
+        hipError_t le = kti::launch_trace(p->L, p->capture);
+        if ((e = hipStreamEndCapture(p->capture, &p->graph)) != hipSuccess) return cleanup(KT_EHIP, "end capture", e);
+        if (le != hipSuccess) return cleanup(KT_EHIP, "captured launch", le);
+        if ((e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0)) != hipSuccess)
+            return cleanup(KT_EHIP, "graph instantiate", e);
+        *plan_out = p;
+        return KT_OK;
+    } catch (const std::exception& ex) {
+        return fail(ctx, KT_EINVAL, ex.what());
+    }
+}
+
+int kt_plan_launch(kt_plan* p, void* hip_stream) {
+    if (!p) return KT_EINVAL;
+    kt_ctx* ctx = p->ctx;
+    hipStream_t stream = static_cast<hipStream_t>(hip_stream);
+    KT_HIP(hipSetDevice(ctx->device));
+    KT_HIP(hipEventRecord(ctx->ev[0], stream));
+    KT_HIP(hipEventRecord(ctx->ev[1], stream));
+    KT_HIP(hipGraphLaunch(p->exec, stream));
+    KT_HIP(hipEventRecord(ctx->ev[2], stream));
+    KT_HIP(hipEventRecord(ctx->ev[3], stream));
+    ctx->stats = kt_stats{};
+    ctx->stats.rays = p->L.total;
+    ctx->stats.block = p->L.block;
+    ctx->stats.grid = p->L.grid;
+    ctx->stats.steps_per_dequeue = p->L.steps_per_dequeue;
+    ctx->stats.waves_per_simd = p->L.waves_per_simd;
+    ctx->stats.waves_per_cu = (p->L.grid * p->L.block / 64) / std::max(1, ctx->cu_count);
+    ctx->timed = true;
+    ctx->stats_src = p->d_counter;
+    return KT_OK;
+}
+
+int kt_plan_destroy(kt_plan* p) {
+    if (!p) return KT_OK;
+    (void)hipSetDevice(p->ctx->device);
+    if (p->capture) (void)hipStreamSynchronize(p->capture);
+    (void)hipDeviceSynchronize();
+    if (p->ctx->stats_src == p->d_counter) p->ctx->stats_src = nullptr;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->capture) (void)hipStreamDestroy(p->capture);
+    if (p->d_views) (void)hipFree(p->d_views);
+    if (p->d_counter) (void)hipFree(p->d_counter);
+    delete p;
+    return KT_OK;
 }
 
 double kt_horizon_radius(double a) { return horizon(a); }

@@ -10,7 +10,7 @@ namespace kti {
 constexpr int kMaxLdsViews = 64;
 
 // device-side counters written by the trace kernels
-enum : int { kStatSteps = 0, kStatRestepRays = 1, kStatFallback = 2, kStatCount = 4 };
+enum : int { kStatSteps = 0, kStatRestepRays = 1, kStatFallback = 2, kStatFp64Steps = 3, kStatCount = 4 };
 
 struct TraceLaunch {
     const ktd::ViewDev* views;  // device array
@@ -24,7 +24,7 @@ struct TraceLaunch {
     int max_steps;
     int precision;              // KT_PREC_*
     int mode;                   // KT_MODE_*
-    int block, grid, steps_per_dequeue;
+    int block, grid, steps_per_dequeue, waves_per_simd;
 };
 
 // launches the counter reset and the trace kernel on `stream`; returns a hipError_t

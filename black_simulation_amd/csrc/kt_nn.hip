@@ -1,0 +1,290 @@
+// kt_nn.hip -- the surrogate network's GPU kernels: BF16 MFMA GEMMs with fused
+// epilogues, the loss head, reductions and Adam.  Host orchestration lives in
+// kt_surrogate.hip.
+//
+// GEMM: C[M x N] = A[M x K] * B[K x N], bf16 in, fp32 accumulate.  A block of 256 threads
+// (4 waves in 2 x 2) owns a 128 x 128 tile of C; each wave a 64 x 64 sub-tile = 4 x 4
+// MFMA 16x16x32 tiles.  K advances 32 at a time through LDS; tiles whose source layout
+// is not reduction-contiguous are transposed on the way into LDS.
+#include "kt_nn.hpp"
+
+namespace ktn {
+
+using namespace ktm;
+
+namespace {
+
+constexpr int kLdsStride = 40;  // bf16 per LDS row: 32 + 8 pad (80 B, keeps 16-byte alignment)
+
+// 128 rows x 32 k of a tile into LDS [row][k].  RC = false: element (row, k) at P[row * ld + k]
+// (reduction-contiguous); RC = true: at P[k * ld + row].  Rows >= R read as zero.
+template <bool RC>
+__device__ __forceinline__ void load_tile(uint16_t* T, const uint16_t* P, int ld, int r0, int k0, int R, int tid) {
+    if (!RC) {
+        for (int c = tid; c < 512; c += 256) {
+            const int row = c >> 2, kc = (c & 3) * 8;
+            uint4 v = {0u, 0u, 0u, 0u};
+            if (r0 + row < R) v = *reinterpret_cast<const uint4*>(P + (long long)(r0 + row) * ld + k0 + kc);
+            *reinterpret_cast<uint4*>(T + row * kLdsStride + kc) = v;
+        }
+    } else {
+        for (int c = tid; c < 512; c += 256) {
+            const int kr = c >> 4, rc = (c & 15) * 8;
+            uint4 v = {0u, 0u, 0u, 0u};
+            if (r0 + rc < R) v = *reinterpret_cast<const uint4*>(P + (long long)(k0 + kr) * ld + r0 + rc);
+            const uint16_t* e = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) T[(rc + j) * kLdsStride + kr] = e[j];
+        }
+    }
+}
+
+template <bool AT, bool BT, int EPI>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[128 * kLdsStride];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[128 * kLdsStride];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = blockIdx.x * 128, n0 = blockIdx.y * 128;
+    const int kb = blockIdx.z * g.k_split;
+    const int ke = g.K < kb + g.k_split ? g.K : kb + g.k_split;
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    for (int k0 = kb; k0 < ke; k0 += 32) {
+        load_tile<AT>(As, g.A, g.lda, m0, k0, g.M, tid);
+        load_tile<!BT>(Bs, g.B, g.ldb, n0, k0, g.N, tid);
+        __syncthreads();
+        bf16x8 af[4], bfr[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            af[i] = as_bf16x8(*reinterpret_cast<const uint4*>(As + (64 * wm + 16 * i + fr) * kLdsStride + fk));
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            bfr[j] = as_bf16x8(*reinterpret_cast<const uint4*>(Bs + (64 * wn + 16 * j + fr) * kLdsStride + fk));
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = mfma16(af[i], bfr[j], acc[i][j]);
+        __syncthreads();
+    }
+    const int col0 = n0 + 64 * wn + fr, row0 = m0 + 64 * wm + (lane >> 4) * 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = row0 + 16 * i + q, c = col0 + 16 * j;
+                if (r >= g.M || c >= g.N) continue;
+                const float v = acc[i][j][q];
+                const long long o = (long long)r * g.ldc + c;
+                if (EPI == kEpiFwd) {
+                    const float z = v + g.bias[c];
+                    if (g.out32) g.out32[o] = z;
+                    g.outb[o] = f2bf(silu(z));
+                } else if (EPI == kEpiLin) {
+                    g.out32[o] = v + g.bias[c];
+                } else if (EPI == kEpiBwd) {
+                    const float gz = v * dsilu(g.zin[(long long)r * g.ldz + c]);
+                    g.out32[o] = gz;
+                    g.outb[o] = f2bf(gz);
+                } else {  // kEpiWs: split-K partial
+                    g.out32[(long long)blockIdx.z * g.ws_stride + o] = v;
+                }
+            }
+}
+
+// sum over splits: dst[i * N + j] = sum_s ws[s][i][j] for i < rows
+__global__ void k_reduce_ws(const float* ws, int splits, long long stride, int rows, int N, int ld, float* dst) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= rows * N) return;
+    const int i = idx / N, j = idx - i * N;
+    float s = 0.f;
+    for (int k = 0; k < splits; ++k) s += ws[k * stride + (long long)i * ld + j];
+    dst[idx] = s;
+}
+
+// column sums of G (B x N fp32, row-major) in two passes: partial[chunk][j], then dst[j]
+__global__ void k_colsum_partial(const float* G, int B, int N, int rows_per_chunk, float* partial) {
+    const int j = threadIdx.x;
+    if (j >= N) return;
+    const int b0 = blockIdx.x * rows_per_chunk;
+    const int b1 = b0 + rows_per_chunk < B ? b0 + rows_per_chunk : B;
+    float s = 0.f;
+    for (int b = b0; b < b1; ++b) s += G[(long long)b * N + j];
+    partial[(long long)blockIdx.x * N + j] = s;
+}
+
+__global__ void k_colsum_final(const float* partial, int chunks, int N, float* dst) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    float s = 0.f;
+    for (int c = 0; c < chunks; ++c) s += partial[(long long)c * N + j];
+    dst[j] = s;
+}
+
+// loss head: softmax cross-entropy on outputs 0..2, squared error on 3..5 (escaped) and
+// 6..7 (disk); writes dL/dy / B (fp32 B x 8 and bf16 padded B x 32) and accumulates the loss
+__global__ void k_loss_head(const float* Y, const int8_t* cls, const float* tgt, int B, float w_sky, float w_disk,
+                            float* gy32, uint16_t* gyb, double* loss_sum) {
+    __shared__ float red[256];
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    float loss = 0.f;
+    if (b < B) {
+        const float* y = Y + (long long)b * 8;
+        const int c = cls[b];
+        const float m = fmaxf(y[0], fmaxf(y[1], y[2]));
+        const float e0 = __expf(y[0] - m), e1 = __expf(y[1] - m), e2 = __expf(y[2] - m);
+        const float inv = 1.0f / (e0 + e1 + e2);
+        float g[8] = {e0 * inv, e1 * inv, e2 * inv, 0.f, 0.f, 0.f, 0.f, 0.f};
+        const float pc = c == 0 ? g[0] : c == 1 ? g[1] : g[2];
+        loss = -__logf(fmaxf(pc, 1e-30f));
+        g[c] -= 1.0f;
+        const float* t = tgt + (long long)b * 5;
+        if (c == 0) {
+            for (int k = 0; k < 3; ++k) {
+                const float d = y[3 + k] - t[k];
+                loss += w_sky * d * d;
+                g[3 + k] = 2.f * w_sky * d;
+            }
+        } else if (c == 2) {
+            for (int k = 0; k < 2; ++k) {
+                const float d = y[6 + k] - t[3 + k];
+                loss += w_disk * d * d;
+                g[6 + k] = 2.f * w_disk * d;
+            }
+        }
+        const float invB = 1.0f / (float)B;
+        for (int k = 0; k < 8; ++k) {
+            gy32[(long long)b * 8 + k] = g[k] * invB;
+            gyb[(long long)b * 32 + k] = f2bf(g[k] * invB);
+        }
+        for (int k = 8; k < 32; ++k) gyb[(long long)b * 32 + k] = 0;
+    }
+    red[threadIdx.x] = loss;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss_sum, (double)red[0]);
+}
+
+// Adam on the FP32 master parameters, then the bf16 working copies of each weight:
+// Wb (K_pad x N_ld, row-major) and Wt (N x K_pad).
+__global__ void k_adam(float* P, const float* G, float* Mm, float* Vv, long long n, float lr_t, float b1, float b2,
+                       float eps, LayerTable lt) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float g = G[p];
+    const float m = b1 * Mm[p] + (1.f - b1) * g;
+    const float v = b2 * Vv[p] + (1.f - b2) * g * g;
+    Mm[p] = m;
+    Vv[p] = v;
+    const float w = P[p] - lr_t * m / (sqrtf(v) + eps);
+    P[p] = w;
+    // which layer / weight element
+    int l = 0;
+    while (l + 1 < kLayers && p >= lt.off[l + 1]) ++l;
+    const long long q = p - lt.off[l];
+    const long long nw = (long long)lt.kreal[l] * lt.n[l];
+    if (q < nw) {
+        const int i = (int)(q / lt.n[l]), j = (int)(q - (long long)i * lt.n[l]);
+        const uint16_t h = f2bf(w);
+        lt.wb[l][(long long)i * lt.nld[l] + j] = h;
+        lt.wt[l][(long long)j * lt.kpad[l] + i] = h;
+    }
+}
+
+// bf16 copies of the master parameters (after a load), same mapping as k_adam
+__global__ void k_refresh_bf16(const float* P, long long n, LayerTable lt) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int l = 0;
+    while (l + 1 < kLayers && p >= lt.off[l + 1]) ++l;
+    const long long q = p - lt.off[l];
+    const long long nw = (long long)lt.kreal[l] * lt.n[l];
+    if (q < nw) {
+        const int i = (int)(q / lt.n[l]), j = (int)(q - (long long)i * lt.n[l]);
+        const uint16_t h = f2bf(P[p]);
+        lt.wb[l][(long long)i * lt.nld[l] + j] = h;
+        lt.wt[l][(long long)j * lt.kpad[l] + i] = h;
+    }
+}
+
+// gather a batch: X0 (B x 32 bf16, 5 features then zeros), class and targets
+__global__ void k_gather(const float* feat, const int8_t* cls, const float* tgt, const long long* idx, int B,
+                         uint16_t* x0, int8_t* bcls, float* btgt) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const long long s = idx ? idx[b] : b;
+    for (int k = 0; k < 32; ++k) x0[(long long)b * 32 + k] = k < 5 ? f2bf(feat[s * 5 + k]) : (uint16_t)0;
+    if (bcls) bcls[b] = cls[s];
+    if (btgt)
+        for (int k = 0; k < 5; ++k) btgt[(long long)b * 5 + k] = tgt[s * 5 + k];
+}
+
+}  // namespace
+
+hipError_t gemm(const GemmArgs& g, bool at, bool bt, int epi, int splits, hipStream_t stream) {
+    const dim3 grid((unsigned)((g.M + 127) / 128), (unsigned)((g.N + 127) / 128), (unsigned)splits);
+    const dim3 block(256);
+#define KTN_CASE(A_, B_, E_)                                                          \
+    if (at == A_ && bt == B_ && epi == E_) {                                          \
+        hipLaunchKernelGGL((k_gemm<A_, B_, E_>), grid, block, 0, stream, g);          \
+        return hipGetLastError();                                                     \
+    }
+    KTN_CASE(false, true, kEpiFwd)
+    KTN_CASE(false, true, kEpiLin)
+    KTN_CASE(false, true, kEpiBwd)
+    KTN_CASE(true, false, kEpiWs)
+    KTN_CASE(false, false, kEpiFwd)
+    KTN_CASE(false, false, kEpiLin)
+#undef KTN_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t reduce_ws(const float* ws, int splits, long long stride, int rows, int N, int ld, float* dst,
+                     hipStream_t s) {
+    const int n = rows * N;
+    hipLaunchKernelGGL(k_reduce_ws, dim3((n + 255) / 256), dim3(256), 0, s, ws, splits, stride, rows, N, ld, dst);
+    return hipGetLastError();
+}
+
+hipError_t colsum(const float* G, int B, int N, float* partial, int chunks, float* dst, hipStream_t s) {
+    const int rpc = (B + chunks - 1) / chunks;
+    hipLaunchKernelGGL(k_colsum_partial, dim3(chunks), dim3(256), 0, s, G, B, N, rpc, partial);
+    hipLaunchKernelGGL(k_colsum_final, dim3((N + 255) / 256), dim3(256), 0, s, partial, chunks, N, dst);
+    return hipGetLastError();
+}
+
+hipError_t loss_head(const float* Y, const int8_t* cls, const float* tgt, int B, float w_sky, float w_disk,
+                     float* gy32, uint16_t* gyb, double* loss_sum, hipStream_t s) {
+    hipLaunchKernelGGL(k_loss_head, dim3((B + 255) / 256), dim3(256), 0, s, Y, cls, tgt, B, w_sky, w_disk, gy32, gyb,
+                       loss_sum);
+    return hipGetLastError();
+}
+
+hipError_t adam(float* P, const float* G, float* M, float* V, long long n, float lr_t, float b1, float b2, float eps,
+                const LayerTable& lt, hipStream_t s) {
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P, G, M, V, n, lr_t, b1, b2, eps,
+                       lt);
+    return hipGetLastError();
+}
+
+hipError_t refresh_bf16(const float* P, long long n, const LayerTable& lt, hipStream_t s) {
+    hipLaunchKernelGGL(k_refresh_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P, n, lt);
+    return hipGetLastError();
+}
+
+hipError_t gather(const float* feat, const int8_t* cls, const float* tgt, const long long* idx, int B, uint16_t* x0,
+                  int8_t* bcls, float* btgt, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather, dim3((B + 255) / 256), dim3(256), 0, s, feat, cls, tgt, idx, B, x0, bcls, btgt);
+    return hipGetLastError();
+}
+
+}  // namespace ktn

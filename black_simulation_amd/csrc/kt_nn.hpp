@@ -1,0 +1,68 @@
+// kt_nn.hpp -- interface of the surrogate network's kernels (kt_nn.hip).
+//
+// The network: 5 -> 256 -> 256 -> 256 -> 256 -> 8, SiLU after the hidden layers.  Layer l
+// maps x (rows) to x W_l + b_l, W_l stored K_l x N_l row-major in FP32 master copies;
+// the kernels read bf16 working copies W_l (K_pad x N_ld) and W_l^T (N x K_pad), where the
+// input layer's K is padded 5 -> 32 and the output layer's N 8 -> 32 with zeros.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kt_mfma.hpp"
+
+namespace ktn {
+
+constexpr int kLayers = 5;
+constexpr int kHidden = 256;
+constexpr int kInputs = 5, kOutputs = 8;
+constexpr int kKreal[kLayers] = {5, 256, 256, 256, 256};
+constexpr int kKpad[kLayers] = {32, 256, 256, 256, 256};
+constexpr int kN[kLayers] = {256, 256, 256, 256, 8};
+constexpr int kNld[kLayers] = {256, 256, 256, 256, 32};  // row stride of the bf16 W copy
+
+// parameters in FP32, layer by layer: W_l (kKreal x kN row-major) then b_l
+constexpr long long param_offset(int l) {
+    long long o = 0;
+    for (int i = 0; i < l; ++i) o += (long long)kKreal[i] * kN[i] + kN[i];
+    return o;
+}
+constexpr long long kParams = param_offset(kLayers);  // 200968
+static_assert(kParams == 200968, "5-256-256-256-256-8 has 200968 parameters");
+
+enum : int { kEpiFwd = 0, kEpiLin = 1, kEpiBwd = 2, kEpiWs = 3 };
+
+struct GemmArgs {
+    int M, N, K, lda, ldb, ldc;
+    const uint16_t* A;
+    const uint16_t* B;
+    const float* bias;   // Fwd, Lin
+    float* out32;        // Fwd: z (optional); Lin: y; Bwd: g; Ws: split-K partials
+    uint16_t* outb;      // Fwd: silu(z) in bf16; Bwd: g in bf16
+    const float* zin;    // Bwd: the pre-activation whose SiLU derivative multiplies
+    int ldz;
+    int k_split;         // reduction length per blockIdx.z (a multiple of 32)
+    long long ws_stride; // Ws: elements per split
+};
+
+struct LayerTable {
+    long long off[kLayers];
+    int kreal[kLayers], kpad[kLayers], n[kLayers], nld[kLayers];
+    uint16_t* wb[kLayers];
+    uint16_t* wt[kLayers];
+};
+
+// C = A B with A reduction-contiguous unless `at`, B given as B^T ([n][k]) when `bt`.
+hipError_t gemm(const GemmArgs& g, bool at, bool bt, int epi, int splits, hipStream_t s);
+hipError_t reduce_ws(const float* ws, int splits, long long stride, int rows, int N, int ld, float* dst,
+                     hipStream_t s);
+hipError_t colsum(const float* G, int B, int N, float* partial, int chunks, float* dst, hipStream_t s);
+hipError_t loss_head(const float* Y, const int8_t* cls, const float* tgt, int B, float w_sky, float w_disk,
+                     float* gy32, uint16_t* gyb, double* loss_sum, hipStream_t s);
+hipError_t adam(float* P, const float* G, float* M, float* V, long long n, float lr_t, float b1, float b2, float eps,
+                const LayerTable& lt, hipStream_t s);
+hipError_t refresh_bf16(const float* P, long long n, const LayerTable& lt, hipStream_t s);
+hipError_t gather(const float* feat, const int8_t* cls, const float* tgt, const long long* idx, int B, uint16_t* x0,
+                  int8_t* bcls, float* btgt, hipStream_t s);
+
+}  // namespace ktn

@@ -34,47 +34,74 @@ __device__ __forceinline__ const ViewDev* stage_views(const kti::TraceLaunch& L,
     return sv;
 }
 
-template <class R>
-__device__ __forceinline__ void start_ray(Lane<R>& s, const ViewDev* views, int n_views, long long idx) {
-    const int vid = find_view(views, n_views, idx);
-    const ViewDev& v = views[vid];
-    Photon<double> p;
-    double y0[5];
-    launch_photon(v, idx - v.offset, p, y0);
-    lane_start(s, v, p, y0);
+__device__ __forceinline__ void launch_of(const ViewDev* views, int n_views, long long idx, Launch& la) {
+    const ViewDev& v = views[find_view(views, n_views, idx)];
+    launch_photon(v, idx - v.offset, la);
 }
 
+// How one lane traces one ray, per precision.
 template <class R>
-__device__ __forceinline__ R tol_r(const kti::TraceLaunch& L);
+struct Plain {
+    using LaneT = Lane<R>;
+    __device__ static void start(LaneT& s, const Launch& la) { lane_start(s, la); }
+    __device__ static bool step(LaneT& s, const kti::TraceLaunch& L, Result& r);
+    __device__ static int steps(const LaneT& s) { return s.steps; }
+    __device__ static int fp64_steps(const LaneT&) { return 0; }
+};
 template <>
-__device__ __forceinline__ double tol_r<double>(const kti::TraceLaunch& L) { return L.rtol; }
+__device__ bool Plain<double>::step(Lane<double>& s, const kti::TraceLaunch& L, Result& r) {
+    return lane_step(s, L.rtol, L.atol, L.max_steps, r);
+}
 template <>
-__device__ __forceinline__ float tol_r<float>(const kti::TraceLaunch& L) { return (float)L.rtol32; }
-template <class R>
-__device__ __forceinline__ R tol_a(const kti::TraceLaunch& L);
-template <>
-__device__ __forceinline__ double tol_a<double>(const kti::TraceLaunch& L) { return L.atol; }
-template <>
-__device__ __forceinline__ float tol_a<float>(const kti::TraceLaunch& L) { return (float)L.atol32; }
+__device__ bool Plain<float>::step(Lane<float>& s, const kti::TraceLaunch& L, Result& r) {
+    return lane_step(s, (float)L.rtol32, (float)L.atol32, L.max_steps, r);
+}
+
+struct Mixed {
+    using LaneT = MixedLane;
+    __device__ static void start(LaneT& s, const Launch& la) {
+        const Photon<double>& p = la.p;
+        lane_start(s.d, la);
+        s.pf.a = (float)p.a;
+        s.pf.a2 = (float)p.a2;
+        s.pf.E = (float)p.E;
+        s.pf.L = (float)p.L;
+        s.pf.Q = (float)p.Q;
+        s.pf.aE = (float)p.aE;
+        s.pf.aL = (float)p.aL;
+        s.pf.K = (float)p.K;
+        s.pf.C1 = (float)p.C1;
+        s.pf.A2E2 = (float)p.A2E2;
+        s.pf.twoE = (float)p.twoE;
+        s.pf.twoA2E2 = (float)p.twoA2E2;
+        s.r_fp64 = (float)la.r_fp64;
+        s.fp64_steps = 0;
+    }
+    __device__ static bool step(LaneT& s, const kti::TraceLaunch& L, Result& r) {
+        return lane_step_mixed(s, L.rtol32, L.atol32, (float)L.restep_drift, L.max_steps, r);
+    }
+    __device__ static int steps(const LaneT& s) { return s.d.steps; }
+    __device__ static int fp64_steps(const LaneT& s) { return s.fp64_steps; }
+};
 
 __global__ void k_reset(unsigned long long* counter, unsigned long long* stats) {
     if (threadIdx.x == 0) *counter = 0ull;
     if (threadIdx.x < kti::kStatCount) stats[threadIdx.x] = 0ull;
 }
 
-template <class R>
-__global__ __launch_bounds__(256) void k_trace_persistent(kti::TraceLaunch L) {
+// W: the occupancy the register allocator aims for (waves per SIMD); a tuning knob.
+template <class Pol, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_trace_persistent(kti::TraceLaunch L) {
     __shared__ ViewDev sv[kti::kMaxLdsViews];
     const ViewDev* views = stage_views(L, sv);
     const int lane = (int)(threadIdx.x & 63u);
-    const R rtol = tol_r<R>(L), atol = tol_a<R>(L);
     const int K = L.steps_per_dequeue;
 
-    Lane<R> s;
+    typename Pol::LaneT s;
     bool active = false;
     bool drained = false;
     long long my = 0;
-    unsigned long long my_steps = 0;
+    unsigned long long my_steps = 0, my_fp64 = 0, my_restep_rays = 0;
     for (;;) {
         if (!drained) {
             const unsigned long long idle = __ballot(!active);
@@ -89,7 +116,9 @@ __global__ __launch_bounds__(256) void k_trace_persistent(kti::TraceLaunch L) {
                     const long long q = (long long)(base + (unsigned long long)__popcll(idle & below));
                     if (q < L.total) {
                         my = L.ray_list ? L.ray_list[q] : q;
-                        start_ray(s, views, L.n_views, my);
+                        Launch la;
+                        launch_of(views, L.n_views, my, la);
+                        Pol::start(s, la);
                         active = true;
                     }
                 }
@@ -99,53 +128,77 @@ __global__ __launch_bounds__(256) void k_trace_persistent(kti::TraceLaunch L) {
         for (int k = 0; k < K; ++k) {
             if (active) {
                 Result r;
-                if (lane_step(s, rtol, atol, L.max_steps, r)) {
-                    write_result(L.out, my, r, s.steps, 0);
-                    my_steps += (unsigned long long)s.steps;
+                if (Pol::step(s, L, r)) {
+                    const int f64 = Pol::fp64_steps(s);
+                    write_result(L.out, my, r, Pol::steps(s), f64 > 0 ? kFlagRestep : 0);
+                    my_steps += (unsigned long long)Pol::steps(s);
+                    my_fp64 += (unsigned long long)f64;
+                    my_restep_rays += f64 > 0 ? 1ull : 0ull;
                     active = false;
                 }
             }
         }
     }
     if (my_steps) atomicAdd(&L.stats[kti::kStatSteps], my_steps);
+    if (my_fp64) atomicAdd(&L.stats[kti::kStatFp64Steps], my_fp64);
+    if (my_restep_rays) atomicAdd(&L.stats[kti::kStatRestepRays], my_restep_rays);
 }
 
-template <class R>
+template <class Pol>
 __global__ __launch_bounds__(256) void k_trace_oneshot(kti::TraceLaunch L) {
     __shared__ ViewDev sv[kti::kMaxLdsViews];
     const ViewDev* views = stage_views(L, sv);
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= L.total) return;
-    const R rtol = tol_r<R>(L), atol = tol_a<R>(L);
     const long long my = L.ray_list ? L.ray_list[q] : q;
-    Lane<R> s;
-    start_ray(s, views, L.n_views, my);
+    typename Pol::LaneT s;
+    Launch la;
+    launch_of(views, L.n_views, my, la);
+    Pol::start(s, la);
     Result r;
-    while (!lane_step(s, rtol, atol, L.max_steps, r)) {
+    while (!Pol::step(s, L, r)) {
     }
-    write_result(L.out, my, r, s.steps, 0);
-    atomicAdd(&L.stats[kti::kStatSteps], (unsigned long long)s.steps);
+    const int f64 = Pol::fp64_steps(s);
+    write_result(L.out, my, r, Pol::steps(s), f64 > 0 ? kFlagRestep : 0);
+    atomicAdd(&L.stats[kti::kStatSteps], (unsigned long long)Pol::steps(s));
+    if (f64) {
+        atomicAdd(&L.stats[kti::kStatFp64Steps], (unsigned long long)f64);
+        atomicAdd(&L.stats[kti::kStatRestepRays], 1ull);
+    }
 }
 
 }  // namespace
 
 namespace kti {
 
+template <class Pol>
+void launch_persistent(const TraceLaunch& L, hipStream_t stream) {
+    const dim3 grid(L.grid), block(L.block);
+    switch (L.waves_per_simd) {
+        case 2: k_trace_persistent<Pol, 2><<<grid, block, 0, stream>>>(L); break;
+        case 3: k_trace_persistent<Pol, 3><<<grid, block, 0, stream>>>(L); break;
+        case 4: k_trace_persistent<Pol, 4><<<grid, block, 0, stream>>>(L); break;
+        default: k_trace_persistent<Pol, 1><<<grid, block, 0, stream>>>(L); break;
+    }
+}
+
+template <class Pol>
+void launch_oneshot(const TraceLaunch& L, hipStream_t stream) {
+    const dim3 grid((unsigned)((L.total + L.block - 1) / L.block)), block(L.block);
+    k_trace_oneshot<Pol><<<grid, block, 0, stream>>>(L);
+}
+
 hipError_t launch_trace(const TraceLaunch& L, hipStream_t stream) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, L.counter, L.stats);
     if (L.total <= 0) return hipGetLastError();
-    const bool fp32 = L.precision == 2;
     if (L.mode == 1) {
-        const long long blocks = (L.total + L.block - 1) / L.block;
-        if (fp32)
-            hipLaunchKernelGGL(k_trace_oneshot<float>, dim3((unsigned)blocks), dim3(L.block), 0, stream, L);
-        else
-            hipLaunchKernelGGL(k_trace_oneshot<double>, dim3((unsigned)blocks), dim3(L.block), 0, stream, L);
+        if (L.precision == 2) launch_oneshot<Plain<float>>(L, stream);
+        else if (L.precision == 1) launch_oneshot<Mixed>(L, stream);
+        else launch_oneshot<Plain<double>>(L, stream);
     } else {
-        if (fp32)
-            hipLaunchKernelGGL(k_trace_persistent<float>, dim3(L.grid), dim3(L.block), 0, stream, L);
-        else
-            hipLaunchKernelGGL(k_trace_persistent<double>, dim3(L.grid), dim3(L.block), 0, stream, L);
+        if (L.precision == 2) launch_persistent<Plain<float>>(L, stream);
+        else if (L.precision == 1) launch_persistent<Mixed>(L, stream);
+        else launch_persistent<Plain<double>>(L, stream);
     }
     return hipGetLastError();
 }

@@ -59,12 +59,13 @@ enum {
 /* flag bits OR-ed into cls */
 enum {
     KT_FLAG_FP64_RESTEP = 0x10,  /* mixed path: at least one FP64 re-step          */
-    KT_FLAG_SURROGATE = 0x20     /* answered by the learned surrogate              */
+    KT_FLAG_SURROGATE = 0x20,    /* answered by the learned surrogate              */
+    KT_FLAG_NONFINITE = 0x40     /* the state went non-finite: counted as captured */
 };
 
 enum { KT_CAMERA_BARDEEN = 0, KT_CAMERA_PINHOLE = 1 };
 enum { KT_PREC_FP64 = 0, KT_PREC_MIXED = 1, KT_PREC_FP32 = 2 };
-enum { KT_MODE_PERSISTENT = 0, KT_MODE_ONESHOT = 1 };
+enum { KT_MODE_PERSISTENT = 0, KT_MODE_ONESHOT = 1, KT_MODE_AUTO = 2 /* the tuned one, else persistent */ };
 
 /* One camera view.  Bardeen views (eht::View, src/eht.h:114-121) trace the image
  * plane of a camera at (r_cam, theta_cam = inclination, phi_cam): pixel centres
@@ -87,7 +88,7 @@ typedef struct kt_view {
 
 typedef struct kt_trace_opts {
     int precision;          /* KT_PREC_*                                            */
-    int mode;               /* KT_MODE_*                                            */
+    int mode;               /* KT_MODE_* (default KT_MODE_AUTO)                      */
     int use_surrogate;      /* 1: surrogate answers confident rays, others re-queued */
     int max_steps;          /* per ray, accepted + rejected (default 200000)        */
     double rtol, atol;      /* FP64 DP5 tolerances (default 1e-10, 1e-12)           */
@@ -98,6 +99,7 @@ typedef struct kt_trace_opts {
     int steps_per_dequeue;  /* persistent refill period (0 = tuned)                 */
     double gate_prob;       /* surrogate: min class probability (0.999)             */
     double gate_ring;       /* surrogate: min |rho/rho_ring - 1| (0.03)             */
+    int waves_per_simd;     /* register budget of the persistent kernel: 1-4 (0 = tuned) */
 } kt_trace_opts;
 
 /* SoA outputs; any pointer may be NULL.  on_device: 1 = device pointers
@@ -123,8 +125,9 @@ typedef struct kt_stats {
     long long rays;            /* rays traced                                               */
     long long steps;           /* DP5 attempts, summed over rays                            */
     long long fallback_rays;   /* surrogate path: rays re-queued to the integrator          */
-    long long restep_rays;     /* mixed path: rays with at least one FP64 re-step           */
-    int block, waves_per_cu, steps_per_dequeue, grid;
+    long long restep_rays;     /* mixed path: rays with at least one FP64 step              */
+    long long fp64_steps;      /* mixed path: DP5 attempts taken in FP64                     */
+    int block, waves_per_cu, steps_per_dequeue, grid, waves_per_simd, mode;
 } kt_stats;
 
 typedef struct kt_ctx kt_ctx;
@@ -147,6 +150,24 @@ int kt_trace_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
  * pointer unless out->on_device, then a device pointer. */
 int kt_trace_rays(kt_ctx* ctx, const kt_view* view, const double* params, long long n,
                   const kt_trace_opts* opts, const kt_rays_out* out, void* hip_stream);
+
+/* The launch-shape autotuner (no reference counterpart: the reference rasterises one
+ * fragment per pixel).  Times the trace of `views` under candidate register budgets
+ * (waves_per_simd), grid sizes (waves_per_cu), refill periods (steps_per_dequeue) and
+ * block sizes by coordinate descent with HIP events, writes the fastest into *opts and
+ * remembers it in ctx for later calls whose fields are 0.  Synchronous; the device
+ * buffers it uses are the context's own. */
+int kt_autotune(kt_ctx* ctx, const kt_view* views, int n_views, kt_trace_opts* opts);
+
+/* A prepared batch: the view table uploaded once and the launch sequence captured in a
+ * hipGraph, replayed by kt_plan_launch (no reference counterpart; the batched-views
+ * path of a frame loop).  `out` must hold device pointers; they are baked into the
+ * graph.  A plan is externally synchronised and owns its work-queue counters. */
+typedef struct kt_plan kt_plan;
+int kt_plan_create(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trace_opts* opts,
+                   const kt_rays_out* out, kt_plan** plan);
+int kt_plan_launch(kt_plan* plan, void* hip_stream);
+int kt_plan_destroy(kt_plan* plan);
 
 /* Host-side closed forms (src/kerr.cpp:157-206, 296-299). */
 double kt_horizon_radius(double a);

@@ -266,17 +266,22 @@ void ko_photon_init(ko_photon* p, double a, double E, double L, double Q) {
     p->K = x * x + Q;
     p->A2E2 = p->a2 * (E * E);
     p->C1 = (Q + L * L) - p->A2E2;
+    p->twoE = 2.0 * E;
+    p->twoA2E2 = 2.0 * p->A2E2;
 }
 
+/* P(r) = E (r^2 + a^2) - a L and Delta(r) = r^2 - 2r + a^2 */
+static double P_of(const ko_photon* p, double r) { return fma(p->E, fma(r, r, p->a2), -p->aL); }
+static double delta_of(const ko_photon* p, double r) { return fma(r, r - 2.0, p->a2); }
+
 double ko_R(const ko_photon* p, double r) {
-    const double P = p->E * (r * r + p->a2) - p->aL;
-    const double delta = (r * r - 2.0 * r) + p->a2;
-    return P * P - delta * p->K;
+    const double P = P_of(p, r);
+    return fma(P, P, -(delta_of(p, r) * p->K));
 }
 
 double ko_M(const ko_photon* p, double m) {
     const double m2 = m * m;
-    return (p->Q - p->C1 * m2) - p->A2E2 * (m2 * m2);
+    return fma(-p->A2E2, m2 * m2, fma(-p->C1, m2, p->Q));
 }
 
 /* Carter's equations in Mino time, backward (s = -tau):
@@ -284,16 +289,21 @@ double ko_M(const ko_photon* p, double m) {
  *   dmu/ds = w_mu, dw_mu/ds = M'(mu)/2 = -(Q + L^2 - a^2 E^2) mu - 2 a^2 E^2 mu^3
  *   dphi/ds = -(a P / Delta - a E + L / (1 - mu^2))                  (BL; kerr.cpp:348 less a vr/Delta)
  * M(mu) = sin^2(theta) Theta(theta); on the constraint surface w_mu = -ko_sin(theta) w_theta. */
+#define KO_TINY 1e-300
+
+/* dphi/ds = aE - [a P (1 - mu^2) + L Delta] / [Delta (1 - mu^2)]: one division.  The
+   floor on 1 - mu^2 keeps L = 0 rays finite at the pole (the ratio -> aP/Delta). */
 void ko_rhs(const ko_photon* p, const double y[5], double dy[5]) {
     const double r = y[0], m = y[1];
-    const double r2 = r * r;
-    const double delta = (r2 - 2.0 * r) + p->a2;
-    const double P = p->E * (r2 + p->a2) - p->aL;
+    const double delta = delta_of(p, r);
+    const double P = P_of(p, r);
+    const double s2 = fmax(fma(-m, m, 1.0), KO_TINY);
+    const double num = fma(p->a * P, s2, p->L * delta);
     dy[0] = y[3];
     dy[1] = y[4];
-    dy[2] = -(((p->a * P) / delta - p->aE) + p->L / (1.0 - m * m));
-    dy[3] = 2.0 * p->E * r * P - (r - 1.0) * p->K;
-    dy[4] = -(p->C1 * m) - 2.0 * p->A2E2 * (m * m * m);
+    dy[2] = p->aE - num / (delta * s2);
+    dy[3] = fma(p->twoE * r, P, -((r - 1.0) * p->K));
+    dy[4] = -(m * fma(p->twoA2E2, m * m, p->C1));
 }
 
 /* tail system in u = 1/r for z = (mu, phi, w_mu), outgoing (w_r = +sqrt(R)):
@@ -301,14 +311,16 @@ void ko_rhs(const ko_photon* p, const double y[5], double dy[5]) {
  *   Pu = E (1 + a^2 u^2) - a L u^2, Du = 1 - 2u + a^2 u^2 */
 static void tail_rhs(const ko_photon* p, double u, const double z[3], double dz[3]) {
     const double u2 = u * u;
-    const double Pu = p->E * (1.0 + p->a2 * u2) - p->aL * u2;
-    const double Du = (1.0 - 2.0 * u) + p->a2 * u2;
-    const double U = Pu * Pu - u2 * Du * p->K;
-    const double isq = 1.0 / sqrt(fmax(U, 1e-300));
+    const double Pu = fma(p->E, fma(p->a2, u2, 1.0), -(p->aL * u2));
+    const double Du = fma(p->a2, u2, fma(-2.0, u, 1.0));
+    const double U = fma(Pu, Pu, -(u2 * Du * p->K));
+    const double isq = 1.0 / sqrt(fmax(U, KO_TINY));
     const double m = z[0];
-    dz[0] = -z[2] * isq;
-    dz[1] = (((p->a * Pu) / Du - p->aE) + p->L / (1.0 - m * m)) * isq;
-    dz[2] = (p->C1 * m + 2.0 * p->A2E2 * (m * m * m)) * isq;
+    const double s2 = fmax(fma(-m, m, 1.0), KO_TINY);
+    const double num = fma(p->a * Pu, s2, p->L * Du);
+    dz[0] = -(z[2] * isq);
+    dz[1] = (num / (Du * s2) - p->aE) * isq;
+    dz[2] = (m * fma(p->twoA2E2, m * m, p->C1)) * isq;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -363,15 +375,16 @@ double ko_pow_m15(double x) {
     static const double T[5] = {1.0, 0.8705505632961241, 0.757858283255199, 0.6597539553864471,
                                 0.5743491774985174};
     int e;
-    if (!(x > 1e-200 && x < 1e200)) return x > 1.0 ? 0.0 : 1e40; /* inf/NaN/0: clamped by the caller */
+    if (!(x > 1e-200 && x < 1e200)) return x > 1.0 ? 0.0 : 1e20; /* inf/NaN/0: clamped by the caller */
     const double m = frexp(x, &e); /* x = m 2^e, m in [0.5, 1) */
     const int q = (e + 1100) / 5 - 220;
     const int s = e - 5 * q;
-    double y = 1.1487 - 0.2974 * (m - 0.5);
+    double y = fma(-0.2974, m - 0.5, 1.1487);
     y = ldexp(y * T[s], -q);
     for (int it = 0; it < 3; ++it) {
-        const double y5 = y * y * y * y * y;
-        y = y * (6.0 - x * y5) * 0.2;
+        const double y2 = y * y;
+        const double y5 = (y2 * y2) * y;
+        y = (y * fma(-x, y5, 6.0)) * 0.2;
     }
     return y;
 }
@@ -389,81 +402,93 @@ static double step_factor(double err, int rejected_before) {
 
 static double amax(double x, double y) { return x > y ? x : y; }
 
-/* one DP5 step of the main system; k1 holds f(y) on entry.  Fills ynew, k[7][5]
-   (k[0] = k1, ..., k[6] = f(ynew)) and returns the max-norm scaled error. */
+/* Stage sums as FMA chains: t = y + h (a_1 k_1 + ... + a_n k_n). */
+#define SUM2(c1, k1, c2, k2) fma(c2, k2, (c1) * (k1))
+#define SUM3(c1, k1, c2, k2, c3, k3) fma(c3, k3, SUM2(c1, k1, c2, k2))
+#define SUM4(c1, k1, c2, k2, c3, k3, c4, k4) fma(c4, k4, SUM3(c1, k1, c2, k2, c3, k3))
+#define SUM5(c1, k1, c2, k2, c3, k3, c4, k4, c5, k5) fma(c5, k5, SUM4(c1, k1, c2, k2, c3, k3, c4, k4))
+#define SUM6(c1, k1, c2, k2, c3, k3, c4, k4, c5, k5, c6, k6) fma(c6, k6, SUM5(c1, k1, c2, k2, c3, k3, c4, k4, c5, k5))
+
+/* The max-norm error picks the component with the largest |e_i| / sc_i by comparing
+   cross products, then divides once.  NaN anywhere in e makes the error NaN. */
+typedef struct { double be, bs, guard; } errpick;
+static void pick(errpick* ep, double e, double sc) {
+    const double ae = fabs(e);
+    ep->guard += ae;
+    if (ep->be < 0.0 || ae * ep->bs > ep->be * sc) {
+        ep->be = ae;
+        ep->bs = sc;
+    }
+}
+static double picked(const errpick* ep) { return ep->guard != ep->guard ? NAN : ep->be / ep->bs; }
+
+/* the five stage inputs and the solution; k[0] = f(y) on entry, k[1..6] filled */
+#define DP5_STAGES(N, EVAL)                                                                                  \
+    for (i = 0; i < N; ++i) t[i] = fma(h, A21 * k[0][i], y[i]);                                              \
+    EVAL(C2, t, k[1]);                                                                                      \
+    for (i = 0; i < N; ++i) t[i] = fma(h, SUM2(A31, k[0][i], A32, k[1][i]), y[i]);                          \
+    EVAL(C3, t, k[2]);                                                                                      \
+    for (i = 0; i < N; ++i) t[i] = fma(h, SUM3(A41, k[0][i], A42, k[1][i], A43, k[2][i]), y[i]);            \
+    EVAL(C4, t, k[3]);                                                                                      \
+    for (i = 0; i < N; ++i)                                                                                 \
+        t[i] = fma(h, SUM4(A51, k[0][i], A52, k[1][i], A53, k[2][i], A54, k[3][i]), y[i]);                  \
+    EVAL(C5, t, k[4]);                                                                                      \
+    for (i = 0; i < N; ++i)                                                                                 \
+        t[i] = fma(h, SUM5(A61, k[0][i], A62, k[1][i], A63, k[2][i], A64, k[3][i], A65, k[4][i]), y[i]);    \
+    EVAL(1.0, t, k[5]);                                                                                     \
+    for (i = 0; i < N; ++i)                                                                                 \
+        ynew[i] = fma(h, SUM5(A71, k[0][i], A73, k[2][i], A74, k[3][i], A75, k[4][i], A76, k[5][i]), y[i]); \
+    EVAL(1.0, ynew, k[6]);
+
+#define ERR_OF(i) (h * SUM6(E1, k[0][i], E3, k[2][i], E4, k[3][i], E5, k[4][i], E6, k[5][i], E7, k[6][i]))
+
 static double dp5_main(const ko_photon* p, const ko_opts* o, const double y[5], double h, double k[7][5],
                        double ynew[5]) {
     double t[5];
     int i;
-    for (i = 0; i < 5; ++i) t[i] = y[i] + h * (A21 * k[0][i]);
-    ko_rhs(p, t, k[1]);
-    for (i = 0; i < 5; ++i) t[i] = y[i] + h * (A31 * k[0][i] + A32 * k[1][i]);
-    ko_rhs(p, t, k[2]);
-    for (i = 0; i < 5; ++i) t[i] = y[i] + h * (A41 * k[0][i] + A42 * k[1][i] + A43 * k[2][i]);
-    ko_rhs(p, t, k[3]);
-    for (i = 0; i < 5; ++i)
-        t[i] = y[i] + h * (A51 * k[0][i] + A52 * k[1][i] + A53 * k[2][i] + A54 * k[3][i]);
-    ko_rhs(p, t, k[4]);
-    for (i = 0; i < 5; ++i)
-        t[i] = y[i] + h * (A61 * k[0][i] + A62 * k[1][i] + A63 * k[2][i] + A64 * k[3][i] + A65 * k[4][i]);
-    ko_rhs(p, t, k[5]);
-    for (i = 0; i < 5; ++i)
-        ynew[i] = y[i] + h * (A71 * k[0][i] + A73 * k[2][i] + A74 * k[3][i] + A75 * k[4][i] + A76 * k[5][i]);
-    ko_rhs(p, ynew, k[6]);
-    double err = 0.0;
+#define EVAL_MAIN(c, yy, kk) ko_rhs(p, yy, kk)
+    DP5_STAGES(5, EVAL_MAIN)
+#undef EVAL_MAIN
+    errpick ep = {-1.0, 1.0, 0.0};
     for (i = 0; i < 5; ++i) {
-        const double e = h * (E1 * k[0][i] + E3 * k[2][i] + E4 * k[3][i] + E5 * k[4][i] + E6 * k[5][i] +
-                              E7 * k[6][i]);
-        double sc;
         const double big = amax(fabs(y[i]), fabs(ynew[i]));
+        double sc;
         if (i == 0) sc = o->rtol * big;
         else if (i == 3) sc = o->rtol * amax(big, y[0] * y[0]);
-        else sc = o->atol + o->rtol * big;
-        const double q = fabs(e) / sc;
-        if (!(q <= err)) err = q; /* propagates NaN */
+        else sc = fma(o->rtol, big, o->atol);
+        pick(&ep, ERR_OF(i), sc);
     }
-    return err;
+    return picked(&ep);
 }
 
-static double dp5_tail(const ko_photon* p, const ko_opts* o, double u, const double z[3], double h,
-                       double k[7][3], double znew[3]) {
+static double dp5_tail(const ko_photon* p, const ko_opts* o, double u, const double y[3], double h,
+                       double k[7][3], double ynew[3]) {
     double t[3];
     int i;
-    for (i = 0; i < 3; ++i) t[i] = z[i] + h * (A21 * k[0][i]);
-    tail_rhs(p, u + C2 * h, t, k[1]);
-    for (i = 0; i < 3; ++i) t[i] = z[i] + h * (A31 * k[0][i] + A32 * k[1][i]);
-    tail_rhs(p, u + C3 * h, t, k[2]);
-    for (i = 0; i < 3; ++i) t[i] = z[i] + h * (A41 * k[0][i] + A42 * k[1][i] + A43 * k[2][i]);
-    tail_rhs(p, u + C4 * h, t, k[3]);
-    for (i = 0; i < 3; ++i)
-        t[i] = z[i] + h * (A51 * k[0][i] + A52 * k[1][i] + A53 * k[2][i] + A54 * k[3][i]);
-    tail_rhs(p, u + C5 * h, t, k[4]);
-    for (i = 0; i < 3; ++i)
-        t[i] = z[i] + h * (A61 * k[0][i] + A62 * k[1][i] + A63 * k[2][i] + A64 * k[3][i] + A65 * k[4][i]);
-    tail_rhs(p, u + h, t, k[5]);
-    for (i = 0; i < 3; ++i)
-        znew[i] = z[i] + h * (A71 * k[0][i] + A73 * k[2][i] + A74 * k[3][i] + A75 * k[4][i] + A76 * k[5][i]);
-    tail_rhs(p, u + h, znew, k[6]);
-    double err = 0.0;
-    for (i = 0; i < 3; ++i) {
-        const double e = h * (E1 * k[0][i] + E3 * k[2][i] + E4 * k[3][i] + E5 * k[4][i] + E6 * k[5][i] +
-                              E7 * k[6][i]);
-        const double sc = o->atol + o->rtol * amax(fabs(z[i]), fabs(znew[i]));
-        const double q = fabs(e) / sc;
-        if (!(q <= err)) err = q;
-    }
-    return err;
+#define EVAL_TAIL(c, yy, kk) tail_rhs(p, u + (c) * h, yy, kk)
+    DP5_STAGES(3, EVAL_TAIL)
+#undef EVAL_TAIL
+    errpick ep = {-1.0, 1.0, 0.0};
+    for (i = 0; i < 3; ++i) pick(&ep, ERR_OF(i), fma(o->rtol, amax(fabs(y[i]), fabs(ynew[i])), o->atol));
+    return picked(&ep);
 }
 
 /* dense output of component i at fraction th of the step */
-static double dense(const double y[5], const double ynew[5], double k[7][5], double h, int i, double th) {
-    const double ydiff = ynew[i] - y[i];
-    const double bspl = h * k[0][i] - ydiff;
-    const double c4 = ydiff - h * k[6][i] - bspl;
-    const double c5 = h * (D1 * k[0][i] + D3 * k[2][i] + D4 * k[3][i] + D5 * k[4][i] + D6 * k[5][i] +
-                           D7 * k[6][i]);
+typedef struct { double y, ydiff, bspl, c4, c5; } ko_dense;
+
+static ko_dense dense_coeffs(const double* y, const double* ynew, double (*k)[5], double h, int i) {
+    ko_dense d;
+    d.y = y[i];
+    d.ydiff = ynew[i] - y[i];
+    d.bspl = fma(h, k[0][i], -d.ydiff);
+    d.c4 = fma(-h, k[6][i], d.ydiff - d.bspl);
+    d.c5 = h * SUM6(D1, k[0][i], D3, k[2][i], D4, k[3][i], D5, k[4][i], D6, k[5][i], D7, k[6][i]);
+    return d;
+}
+
+static double dense_at(const ko_dense* d, double th) {
     const double th1 = 1.0 - th;
-    return y[i] + th * (ydiff + th1 * (bspl + th * (c4 + th1 * c5)));
+    return fma(th, fma(th1, fma(th, fma(th1, d->c5, d->c4), d->bspl), d->ydiff), d->y);
 }
 
 /* Constraint projection, kerr.cpp:521-542 (kTurningZone = 0.05) in (r, mu) form: away
@@ -473,15 +498,13 @@ static double dense(const double y[5], const double ynew[5], double k[7][5], dou
    and R ~ r^4: without this, a rounding-level error at r = 1000 M is O(1) near the hole. */
 static void project(const ko_photon* p, double y[5]) {
     const double r = y[0], m = y[1];
-    const double r2 = r * r;
-    const double P = p->E * (r2 + p->a2) - p->aL;
-    const double delta = (r2 - 2.0 * r) + p->a2;
-    const double PP = P * P;
-    const double R = PP - delta * p->K;
-    if (R > 0.05 * (PP + fabs(delta) * p->K)) y[3] = copysign(sqrt(R), y[3]);
+    const double P = P_of(p, r);
+    const double delta = delta_of(p, r);
+    const double R = fma(P, P, -(delta * p->K));
+    if (R > 0.05 * fma(P, P, fabs(delta) * p->K)) y[3] = copysign(sqrt(R), y[3]);
     const double m2 = m * m, s2 = 1.0 - m2;
-    const double M = (p->Q - p->C1 * m2) - p->A2E2 * (m2 * m2);
-    const double scale = (fabs(p->Q) * s2 + p->A2E2 * (m2 * s2)) + p->L * p->L * m2;
+    const double M = fma(-p->A2E2, m2 * m2, fma(-p->C1, m2, p->Q));
+    const double scale = fma(p->L * p->L, m2, fma(fabs(p->Q), s2, p->A2E2 * (m2 * s2)));
     if (M > 0.05 * scale) y[4] = copysign(sqrt(M), y[4]);
 }
 
@@ -663,17 +686,26 @@ void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, con
         if (p->L == 0.0 && y[1] * y[4] > 0.0 && yn[1] * yn[4] < 0.0) yn[2] += KO_PI;
         project(p, yn);
 
+        /* a non-finite state counts as captured (eht.cpp:285) */
+        if (!(fabs(yn[0]) < 1e300) || !(fabs(yn[1]) < 1e300)) {
+            out->cls = KO_CAPTURED;
+            out->flags = KO_FLAG_NONFINITE;
+            for (int i = 0; i < 5; ++i) out->y_end[i] = yn[i];
+            out->steps = steps;
+            return;
+        }
         /* events in the reference's order: disk, horizon, escape (trajectory.cpp:307-392) */
         if (disk && y[1] * yn[1] < 0.0) {
+            const ko_dense dr = dense_coeffs(y, yn, k, h, 0), dm = dense_coeffs(y, yn, k, h, 1);
             double lo = 0.0, hi = 1.0;
             const int pos = y[1] > 0.0;
             for (int it = 0; it < 48; ++it) {
                 const double mid = 0.5 * (lo + hi);
-                const double f = dense(y, yn, k, h, 1, mid);
+                const double f = dense_at(&dm, mid);
                 if ((f > 0.0) == pos) lo = mid;
                 else hi = mid;
             }
-            const double rc = dense(y, yn, k, h, 0, 0.5 * (lo + hi));
+            const double rc = dense_at(&dr, 0.5 * (lo + hi));
             if (rc >= r_in && rc <= r_out) {
                 /* trajectory.cpp:311-321 */
                 const double omega = ko_circular_omega(a, rc, v->disk_prograde);

@@ -30,7 +30,7 @@
  *     for the asymptotic sky angles.
  *
  * Arithmetic is plain IEEE binary64 with FP contraction disabled; the step loop
- * uses only +, -, *, /, sqrt and fma, each correctly rounded, so an
+ * uses only +, -, *, /, sqrt and explicit fma, each correctly rounded, so an
  * implementation that performs the same operations in the same order gets the
  * same bits.
  */
@@ -83,7 +83,7 @@ void ko_camera_basis(double yaw, double pitch, double roll, double basis[9]);
 /* ---- the integrated system: backward Mino time, (r, mu, phi, w_r, w_mu) ---- */
 typedef struct {
     double a, a2, E, L, Q;
-    double aE, aL, K, C1, A2E2;
+    double aE, aL, K, C1, A2E2, twoE, twoA2E2;
 } ko_photon;
 
 void ko_photon_init(ko_photon* p, double a, double E, double L, double Q);

@@ -51,7 +51,8 @@ class Opts(C.Structure):
 
 
 class Photon(C.Structure):
-    _fields_ = [(n, C.c_double) for n in ("a", "a2", "E", "L", "Q", "aE", "aL", "K", "C1", "A2E2")]
+    _fields_ = [(n, C.c_double) for n in ("a", "a2", "E", "L", "Q", "aE", "aL", "K", "C1", "A2E2", "twoE",
+                                          "twoA2E2")]
 
 
 class Result(C.Structure):

@@ -63,7 +63,7 @@ def test_defaults():
     lib = K.lib()
     o = K.KtTraceOpts()
     lib.kt_trace_opts_default(C.byref(o))
-    assert (o.rtol, o.atol, o.max_steps, o.precision) == (1e-10, 1e-12, 200000, K.PREC_FP64)
+    assert (o.rtol, o.atol, o.max_steps, o.precision, o.mode) == (1e-10, 1e-12, 200000, K.PREC_FP64, K.MODE_AUTO)
     assert (o.gate_prob, o.gate_ring, o.restep_drift) == (0.999, 0.03, 1e-6)
     v = K.KtView()
     lib.kt_view_default(C.byref(v))

@@ -1,0 +1,43 @@
+"""The C++20 host face (include/kerr_trace.hpp) through the kt_inspect program: the same
+rays as the oracle, bit for bit, as the Python path."""
+import json
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "black_simulation_amd", "kt_inspect")
+
+
+def run(*args):
+    out = subprocess.run([EXE, *map(str, args)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def test_inspect_single_pixels_match_oracle():
+    v = O.make_view(width=64, height=64)
+    for col, row in [(32, 32), (0, 0), (50, 32), (20, 33), (32, 20)]:
+        got = run(0.9, 30.0, 80.0, 12.0, 64, 64, col, row)
+        want = O.trace_pixel(v, O.make_opts(), col, row)
+        assert got["end"] == want.cls and got["steps"] == want.steps
+        if want.cls == O.ESCAPED:
+            assert abs(got["theta_inf"] - want.theta_inf) < 1e-13
+        else:
+            assert got["theta_inf"] is None
+        if want.cls == O.DISK:
+            assert got["r_hit"] == want.r_hit and got["g"] == want.g
+
+
+def test_render_matches_oracle_counts():
+    got = run(0.9, 30.0, 80.0, 12.0, 48, 48)
+    want = O.render(O.make_view(width=48, height=48), O.make_opts())
+    counts = np.bincount(want["cls"], minlength=4)
+    assert [got["escaped"], got["captured"], got["disk"], got["unfinished"]] == counts.tolist()
+    assert got["steps"] == int(want["steps"].sum())

@@ -1,0 +1,75 @@
+"""Mixed precision (FP32 bulk, FP64 in the turning-point and near-horizon zones and on
+H-drift re-steps) and pure FP32, judged against the FP64 oracle on identical rays.
+
+Bar: the same class for >= 99.9 % of rays, excluding rays within 1e-6 of a class boundary
+(the oracle's class changes when alpha or beta moves by 1e-6); sky-direction and g errors
+are measured and stated, with loose ceilings here."""
+import numpy as np
+import pytest
+
+import black_simulation_amd as bsa
+from black_simulation_amd.configs import c2_view
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_view(v):
+    return O.make_view(**{k: getattr(v, k) for k in (
+        "camera_kind", "width", "height", "spin", "r_cam", "theta_cam", "phi_cam", "half_width", "half_height",
+        "yaw", "pitch", "roll", "fov_deg", "disk_inner", "disk_outer", "disk_prograde", "disk_enabled",
+        "escape_radius", "capture_factor")})
+
+
+def sky_angle(t1, p1, t2, p2):
+    v1 = np.stack([np.sin(t1) * np.cos(p1), np.sin(t1) * np.sin(p1), np.cos(t1)])
+    v2 = np.stack([np.sin(t2) * np.cos(p2), np.sin(t2) * np.sin(p2), np.cos(t2)])
+    return 2 * np.arcsin(np.minimum(1, np.linalg.norm(v1 - v2, axis=0) / 2))
+
+
+def boundary_mask(v, want):
+    """rays whose oracle class changes under a 1e-6 M move of alpha or beta"""
+    ov = oracle_view(v)
+    n = v.width * v.height
+    cols = np.arange(n) % v.width
+    rows = np.arange(n) // v.width
+    alpha = -v.half_width + (cols + 0.5) * 2 * v.half_width / v.width
+    beta = v.half_height - (rows + 0.5) * 2 * v.half_height / v.height
+    near = np.zeros(n, bool)
+    opts = O.make_opts()
+    for da, db in ((1e-6, 0), (-1e-6, 0), (0, 1e-6), (0, -1e-6)):
+        moved = O.trace_params(ov, opts, np.stack([alpha + da, beta + db], axis=1))
+        near |= moved["cls"] != want["cls"]
+    return near
+
+
+@pytest.fixture(scope="module")
+def c2_sample():
+    v = c2_view(96, 54)
+    want = O.render(oracle_view(v), O.make_opts())
+    return v, want, boundary_mask(v, want)
+
+
+@pytest.mark.parametrize("prec", [bsa.PREC_MIXED, bsa.PREC_FP32])
+def test_precision_vs_fp64_oracle(tracer, c2_sample, prec):
+    v, want, near = c2_sample
+    got = bsa.to_numpy(tracer.trace_views([v], bsa.TraceOpts(precision=prec)))
+    st = tracer.stats()
+    cls = got["cls"] & 7
+    keep = ~near
+    agree = (cls[keep] == want["cls"][keep]).mean()
+    esc = keep & (cls == O.ESCAPED) & (want["cls"] == O.ESCAPED)
+    d = sky_angle(got["theta_inf"][esc], got["phi_inf"][esc], want["theta"][esc], want["phi"][esc])
+    disk = keep & (cls == O.DISK) & (want["cls"] == O.DISK)
+    gerr = np.abs(got["g"][disk] / want["g"][disk] - 1)
+    rerr = np.abs(got["r_hit"][disk] / want["r_hit"][disk] - 1)
+    restep = (got["cls"] & bsa.FLAG_FP64_RESTEP) != 0
+    print(f"prec={prec}: class agreement {agree:.5f} over {keep.sum()} rays ({near.sum()} boundary rays excluded); "
+          f"deflection median {np.median(d):.2e} p99 {np.quantile(d, 0.99):.2e} max {d.max():.2e} rad; "
+          f"g rel median {np.median(gerr):.2e} p99 {np.quantile(gerr, 0.99):.2e}; r_hit rel p99 "
+          f"{np.quantile(rerr, 0.99):.2e}; rays with FP64 steps {restep.mean():.3f}; "
+          f"fp64 steps {st['fp64_steps']} of {st['steps']}", flush=True)
+    assert agree >= 0.999
+    assert np.median(d) < 1e-4 and np.quantile(gerr, 0.99) < 1e-3
+    if prec == bsa.PREC_MIXED:
+        assert restep.any() and st["fp64_steps"] > 0

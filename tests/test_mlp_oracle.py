@@ -1,0 +1,56 @@
+"""The FP64 numpy surrogate network (oracle/mlp_oracle.py) against central finite
+differences: its backward pass is what the GPU kernels' gradients are checked against."""
+import numpy as np
+
+from oracle import mlp_oracle as M
+
+
+def small_problem(seed=1, B=16):
+    sizes = (5, 12, 12, 12, 12, 8)
+    params = M.init_params(seed, sizes)
+    params = [(W, np.random.default_rng(seed + l).normal(0, 0.1, b.shape)) for l, (W, b) in enumerate(params)]
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (B, 5))
+    cls = rng.integers(0, 3, B)
+    tgt = rng.uniform(-1, 1, (B, 5))
+    return params, x, cls, tgt
+
+
+def test_backward_matches_finite_differences():
+    params, x, cls, tgt = small_problem()
+    y, cache = M.forward(params, x)
+    _, gy = M.loss_and_grad_out(y, cls, tgt, w_sky=2.0, w_disk=0.5)
+    grads = M.backward(params, cache, gy)
+    rng = np.random.default_rng(0)
+    h = 1e-6
+    for l, ((W, b), (gW, gb)) in enumerate(zip(params, grads)):
+        for _ in range(6):
+            i, j = rng.integers(0, W.shape[0]), rng.integers(0, W.shape[1])
+            Wp, Wm = W.copy(), W.copy()
+            Wp[i, j] += h
+            Wm[i, j] -= h
+            pp = [(Wp if k == l else p[0], p[1]) for k, p in enumerate(params)]
+            pm = [(Wm if k == l else p[0], p[1]) for k, p in enumerate(params)]
+            fd = (M.loss_fn(pp, x, cls, tgt, w_sky=2.0, w_disk=0.5) - M.loss_fn(pm, x, cls, tgt, w_sky=2.0,
+                                                                                w_disk=0.5)) / (2 * h)
+            assert abs(fd - gW[i, j]) < 1e-7 + 1e-5 * abs(fd), (l, i, j, fd, gW[i, j])
+        j = rng.integers(0, b.shape[0])
+        bp, bm = b.copy(), b.copy()
+        bp[j] += h
+        bm[j] -= h
+        pp = [(p[0], bp if k == l else p[1]) for k, p in enumerate(params)]
+        pm = [(p[0], bm if k == l else p[1]) for k, p in enumerate(params)]
+        fd = (M.loss_fn(pp, x, cls, tgt, w_sky=2.0, w_disk=0.5) - M.loss_fn(pm, x, cls, tgt, w_sky=2.0,
+                                                                            w_disk=0.5)) / (2 * h)
+        assert abs(fd - gb[j]) < 1e-7 + 1e-5 * abs(fd)
+
+
+def test_adam_reduces_loss():
+    params, x, cls, tgt = small_problem(B=64)
+    st = M.adam_state(params)
+    first = M.loss_fn(params, x, cls, tgt)
+    for _ in range(200):
+        y, cache = M.forward(params, x)
+        _, gy = M.loss_and_grad_out(y, cls, tgt)
+        params = M.adam_step(params, M.backward(params, cache, gy), st, 1e-2)
+    assert M.loss_fn(params, x, cls, tgt) < 0.2 * first
