@@ -98,6 +98,33 @@ class KtStats(C.Structure):
     ]
 
 
+class KtSurrogateOpts(C.Structure):
+    _fields_ = [
+        ("n_train", C.c_longlong),
+        ("n_holdout", C.c_longlong),
+        ("epochs", C.c_int),
+        ("batch", C.c_int),
+        ("lr", C.c_double),
+        ("w_sky", C.c_double),
+        ("w_disk", C.c_double),
+        ("band", C.c_double),
+        ("band_factor", C.c_double),
+        ("seed", C.c_ulonglong),
+        ("label_precision", C.c_int),
+        ("disk_outer", C.c_double),
+        ("escape_radius", C.c_double),
+        ("capture_factor", C.c_double),
+    ]
+
+
+class KtSurrogateReport(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("datagen_seconds", "train_seconds", "eval_seconds", "first_loss",
+                                          "final_loss", "train_gemm_tflops")] + [("steps", C.c_longlong)] + \
+               [(n, C.c_double) for n in ("class_agreement", "gated_fraction", "gated_class_agreement",
+                                          "defl_median", "defl_p99", "defl_p999", "g_rel_median", "g_rel_p99",
+                                          "rhit_rel_median", "rhit_rel_p99")]
+
+
 # every entry point include/kerr_trace.h declares: name -> (restype, argtypes)
 _P = C.POINTER
 SIGNATURES = {
@@ -114,6 +141,18 @@ SIGNATURES = {
     "kt_plan_create": (C.c_int, [C.c_void_p, _P(KtView), C.c_int, _P(KtTraceOpts), _P(KtRaysOut), _P(C.c_void_p)]),
     "kt_plan_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kt_plan_destroy": (C.c_int, [C.c_void_p]),
+    "kt_surrogate_opts_default": (None, [_P(KtSurrogateOpts)]),
+    "kt_surrogate_param_count": (C.c_longlong, []),
+    "kt_surrogate_init": (C.c_int, [C.c_void_p, C.c_ulonglong]),
+    "kt_surrogate_load": (C.c_int, [C.c_void_p, C.c_void_p, C.c_longlong]),
+    "kt_surrogate_save": (C.c_int, [C.c_void_p, C.c_void_p, C.c_longlong]),
+    "kt_surrogate_train": (C.c_int, [C.c_void_p, _P(KtSurrogateOpts), _P(KtSurrogateReport)]),
+    "kt_surrogate_predict": (C.c_int, [C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p]),
+    "kt_surrogate_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                         _P(C.c_double)]),
+    "kt_surrogate_adam_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int]),
+    "kt_surrogate_table_radius": (C.c_double, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
+    "kt_surrogate_shadow_radius": (C.c_double, [C.c_void_p, C.c_double, C.c_double]),
     "kt_horizon_radius": (C.c_double, [C.c_double]),
     "kt_isco_radius": (C.c_double, [C.c_double, C.c_int]),
     "kt_photon_orbit_radius": (C.c_double, [C.c_double, C.c_int]),

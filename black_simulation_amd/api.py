@@ -206,6 +206,64 @@ class Tracer:
                                                  C.byref(handle)), "kt_plan_create")
         return Plan(self, handle, out)
 
+    # -- the learned surrogate ------------------------------------------------------------
+    def surrogate_init(self, seed: int = 1):
+        _check(self._ctx, K.lib().kt_surrogate_init(self._ctx, seed), "kt_surrogate_init")
+
+    def surrogate_train(self, **kw) -> dict:
+        """Generate data with the integrator, train on the GPU, judge the held-out set."""
+        o = K.KtSurrogateOpts()
+        K.lib().kt_surrogate_opts_default(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        r = K.KtSurrogateReport()
+        _check(self._ctx, K.lib().kt_surrogate_train(self._ctx, C.byref(o), C.byref(r)), "kt_surrogate_train")
+        return {f: getattr(r, f) for f, _ in K.KtSurrogateReport._fields_}
+
+    def surrogate_save(self) -> np.ndarray:
+        n = K.lib().kt_surrogate_param_count()
+        p = np.zeros(n, np.float32)
+        _check(self._ctx, K.lib().kt_surrogate_save(self._ctx, p.ctypes.data, n), "kt_surrogate_save")
+        return p
+
+    def surrogate_load(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        _check(self._ctx, K.lib().kt_surrogate_load(self._ctx, p.ctypes.data, p.size), "kt_surrogate_load")
+
+    def surrogate_predict(self, inputs):
+        """Network outputs for normalised inputs (n x 5) -> (n x 8), on the device."""
+        t = self.torch
+        x = t.as_tensor(np.asarray(inputs, np.float32)).to(f"cuda:{self.device}").contiguous()
+        y = t.empty((x.shape[0], 8), dtype=t.float32, device=x.device)
+        st = t.cuda.current_stream(self.device)
+        _check(self._ctx, K.lib().kt_surrogate_predict(self._ctx, x.data_ptr(), x.shape[0], y.data_ptr(),
+                                                       C.c_void_p(st.cuda_stream)), "kt_surrogate_predict")
+        return y
+
+    def surrogate_gradients(self, inputs, cls, targets):
+        t = self.torch
+        dev = f"cuda:{self.device}"
+        x = t.as_tensor(np.asarray(inputs, np.float32)).to(dev).contiguous()
+        c = t.as_tensor(np.asarray(cls, np.int8)).to(dev).contiguous()
+        y = t.as_tensor(np.asarray(targets, np.float32)).to(dev).contiguous()
+        g = t.empty(K.lib().kt_surrogate_param_count(), dtype=t.float32, device=dev)
+        loss = C.c_double()
+        t.cuda.synchronize()
+        _check(self._ctx, K.lib().kt_surrogate_gradients(self._ctx, x.data_ptr(), c.data_ptr(), y.data_ptr(),
+                                                         x.shape[0], g.data_ptr(), C.byref(loss)),
+               "kt_surrogate_gradients")
+        return g, loss.value
+
+    def surrogate_adam_step(self, grads, lr: float, t: int):
+        self.torch.cuda.synchronize()
+        _check(self._ctx, K.lib().kt_surrogate_adam_step(self._ctx, grads.data_ptr(), lr, t), "kt_surrogate_adam_step")
+
+    def surrogate_shadow_radius(self, a: float, inclination: float) -> float:
+        return K.lib().kt_surrogate_shadow_radius(self._ctx, a, inclination)
+
+    def surrogate_table_radius(self, a: float, inclination: float, psi: float) -> float:
+        return K.lib().kt_surrogate_table_radius(self._ctx, a, inclination, psi)
+
     def stats(self) -> dict:
         s = K.KtStats()
         _check(self._ctx, K.lib().kt_last_stats(self._ctx, C.byref(s)), "kt_last_stats")

@@ -17,6 +17,7 @@
 #include "../../include/kerr_trace.h"
 #include "kt_internal.hpp"
 #include "kt_surrogate.hpp"
+#include "kt_nn.hpp"
 
 using ktd::ViewDev;
 
@@ -331,6 +332,7 @@ kti::TraceLaunch make_launch(kt_ctx* ctx, const ViewDev* d_views, int n_views, l
     L.stats = counter + 1;
     L.out = dout;
     L.ray_list = nullptr;
+    L.total_dev = nullptr;
     L.rtol = o.rtol;
     L.atol = o.atol;
     L.rtol32 = o.rtol32;
@@ -768,6 +770,139 @@ int kt_plan_destroy(kt_plan* p) {
     if (p->d_counter) (void)hipFree(p->d_counter);
     delete p;
     return KT_OK;
+}
+
+static int ensure_surrogate(kt_ctx* ctx) {
+    if (ctx->surrogate) return KT_OK;
+    hipError_t e = hipSuccess;
+    ctx->surrogate = kts::create(ctx->device, &e);
+    if (!ctx->surrogate) return hip_fail(ctx, e, "surrogate create");
+    return KT_OK;
+}
+
+void kt_surrogate_opts_default(kt_surrogate_opts* o) {
+    if (!o) return;
+    kts::TrainOpts d;
+    o->n_train = d.n_train;
+    o->n_holdout = d.n_holdout;
+    o->epochs = d.epochs;
+    o->batch = d.batch;
+    o->lr = d.lr;
+    o->w_sky = d.w_sky;
+    o->w_disk = d.w_disk;
+    o->band = d.band;
+    o->band_factor = d.band_factor;
+    o->seed = d.seed;
+    o->label_precision = d.label_precision;
+    o->disk_outer = d.disk_outer;
+    o->escape_radius = d.escape_radius;
+    o->capture_factor = d.capture_factor;
+}
+
+long long kt_surrogate_param_count(void) { return ktn::kParams; }
+
+int kt_surrogate_init(kt_ctx* ctx, unsigned long long seed) {
+    if (!ctx) return KT_EINVAL;
+    if (int st = ensure_surrogate(ctx)) return st;
+    hipError_t e = kts::init_params(ctx->surrogate, seed);
+    return e == hipSuccess ? KT_OK : hip_fail(ctx, e, "surrogate init");
+}
+
+int kt_surrogate_load(kt_ctx* ctx, const float* params, long long n) {
+    if (!ctx || !params) return KT_EINVAL;
+    if (n != ktn::kParams) return fail(ctx, KT_EINVAL, "parameter count must be kt_surrogate_param_count()");
+    if (int st = ensure_surrogate(ctx)) return st;
+    hipError_t e = kts::set_params(ctx->surrogate, params, n);
+    return e == hipSuccess ? KT_OK : hip_fail(ctx, e, "surrogate load");
+}
+
+int kt_surrogate_save(kt_ctx* ctx, float* params, long long n) {
+    if (!ctx || !params) return KT_EINVAL;
+    if (n != ktn::kParams) return fail(ctx, KT_EINVAL, "parameter count must be kt_surrogate_param_count()");
+    if (!kts::ready(ctx->surrogate)) return fail(ctx, KT_EUNSUPPORTED, "no surrogate parameters");
+    hipError_t e = kts::get_params(ctx->surrogate, params, n);
+    return e == hipSuccess ? KT_OK : hip_fail(ctx, e, "surrogate save");
+}
+
+int kt_surrogate_train(kt_ctx* ctx, const kt_surrogate_opts* opts, kt_surrogate_report* report) {
+    if (!ctx) return KT_EINVAL;
+    if (int st = ensure_surrogate(ctx)) return st;
+    kts::TrainOpts o;
+    if (opts) {
+        o.n_train = opts->n_train > 0 ? opts->n_train : o.n_train;
+        o.n_holdout = opts->n_holdout > 0 ? opts->n_holdout : o.n_holdout;
+        o.epochs = opts->epochs > 0 ? opts->epochs : o.epochs;
+        o.batch = opts->batch > 0 ? opts->batch : o.batch;
+        o.lr = opts->lr > 0 ? opts->lr : o.lr;
+        o.w_sky = opts->w_sky > 0 ? opts->w_sky : o.w_sky;
+        o.w_disk = opts->w_disk > 0 ? opts->w_disk : o.w_disk;
+        o.band = opts->band > 0 ? opts->band : o.band;
+        o.band_factor = opts->band_factor > 0 ? opts->band_factor : o.band_factor;
+        o.seed = opts->seed ? opts->seed : o.seed;
+        o.label_precision = opts->label_precision;
+        o.disk_outer = opts->disk_outer > 0 ? opts->disk_outer : o.disk_outer;
+        o.escape_radius = opts->escape_radius > 0 ? opts->escape_radius : o.escape_radius;
+        o.capture_factor = opts->capture_factor >= 1.0 ? opts->capture_factor : o.capture_factor;
+    }
+    if (o.batch % 128 != 0) return fail(ctx, KT_EINVAL, "batch must be a multiple of 128");
+    kts::TrainReport r;
+    hipError_t e = kts::train(ctx->surrogate, o, r);
+    if (e != hipSuccess) return hip_fail(ctx, e, "surrogate train");
+    if (report) {
+        report->datagen_seconds = r.datagen_seconds;
+        report->train_seconds = r.train_seconds;
+        report->eval_seconds = r.eval_seconds;
+        report->first_loss = r.first_loss;
+        report->final_loss = r.final_loss;
+        report->train_gemm_tflops = r.train_gemm_tflops;
+        report->steps = r.steps;
+        report->class_agreement = r.class_agreement;
+        report->gated_fraction = r.gated_fraction;
+        report->gated_class_agreement = r.gated_class_agreement;
+        report->defl_median = r.defl_median;
+        report->defl_p99 = r.defl_p99;
+        report->defl_p999 = r.defl_p999;
+        report->g_rel_median = r.g_rel_median;
+        report->g_rel_p99 = r.g_rel_p99;
+        report->rhit_rel_median = r.rhit_rel_median;
+        report->rhit_rel_p99 = r.rhit_rel_p99;
+    }
+    return KT_OK;
+}
+
+int kt_surrogate_predict(kt_ctx* ctx, const float* inputs, long long n, float* outputs, void* hip_stream) {
+    if (!ctx || !inputs || !outputs || n <= 0) return KT_EINVAL;
+    if (!kts::ready(ctx->surrogate)) return fail(ctx, KT_EUNSUPPORTED, "no surrogate parameters");
+    hipError_t e = kts::predict(ctx->surrogate, inputs, n, outputs, static_cast<hipStream_t>(hip_stream));
+    return e == hipSuccess ? KT_OK : hip_fail(ctx, e, "surrogate predict");
+}
+
+int kt_surrogate_gradients(kt_ctx* ctx, const float* inputs, const int8_t* cls, const float* targets, int n,
+                           float* grads, double* loss) {
+    if (!ctx || !inputs || !cls || !targets || n <= 0) return KT_EINVAL;
+    if (n % 128 != 0) return fail(ctx, KT_EINVAL, "n must be a multiple of 128");
+    if (!kts::ready(ctx->surrogate)) return fail(ctx, KT_EUNSUPPORTED, "no surrogate parameters");
+    hipError_t e = kts::gradients(ctx->surrogate, inputs, cls, targets, n, grads, loss, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    return e == hipSuccess ? KT_OK : hip_fail(ctx, e, "surrogate gradients");
+}
+
+int kt_surrogate_adam_step(kt_ctx* ctx, const float* grads, double lr, int t) {
+    if (!ctx || t < 1) return KT_EINVAL;
+    if (!kts::ready(ctx->surrogate)) return fail(ctx, KT_EUNSUPPORTED, "no surrogate parameters");
+    hipError_t e = kts::adam_step(ctx->surrogate, grads, lr, t, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    return e == hipSuccess ? KT_OK : hip_fail(ctx, e, "surrogate adam");
+}
+
+double kt_surrogate_table_radius(kt_ctx* ctx, double a, double inclination, double psi) {
+    if (!ctx || ensure_surrogate(ctx) != KT_OK) return 0.0;
+    return kts::table_critical_radius(ctx->surrogate, a, inclination, psi);
+}
+
+double kt_surrogate_shadow_radius(kt_ctx* ctx, double a, double inclination) {
+    if (!ctx || ensure_surrogate(ctx) != KT_OK) return 0.0;
+    return kts::table_shadow_radius(ctx->surrogate, a, inclination);
 }
 
 double kt_horizon_radius(double a) { return horizon(a); }

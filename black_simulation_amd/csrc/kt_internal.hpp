@@ -20,6 +20,7 @@ struct TraceLaunch {
     unsigned long long* stats;    // kStatCount counters (zeroed by the launch)
     ktd::Outputs out;
     const long long* ray_list;  // optional: trace only these ray indices (fallback re-queue)
+    const long long* total_dev; // optional: the number of rays, read on the device (total = capacity)
     double rtol, atol, rtol32, atol32, restep_drift;
     int max_steps;
     int precision;              // KT_PREC_*

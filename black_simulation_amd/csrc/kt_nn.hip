@@ -176,8 +176,8 @@ __global__ void k_loss_head(const float* Y, const int8_t* cls, const float* tgt,
 
 // Adam on the FP32 master parameters, then the bf16 working copies of each weight:
 // Wb (K_pad x N_ld, row-major) and Wt (N x K_pad).
-__global__ void k_adam(float* P, const float* G, float* Mm, float* Vv, long long n, float lr_t, float b1, float b2,
-                       float eps, LayerTable lt) {
+__global__ void k_adam(float* P, const float* G, float* Mm, float* Vv, long long n, float lr, float b1, float b2,
+                       float eps, float c1, float c2, LayerTable lt) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const float g = G[p];
@@ -185,7 +185,8 @@ __global__ void k_adam(float* P, const float* G, float* Mm, float* Vv, long long
     const float v = b2 * Vv[p] + (1.f - b2) * g * g;
     Mm[p] = m;
     Vv[p] = v;
-    const float w = P[p] - lr_t * m / (sqrtf(v) + eps);
+    // bias-corrected moments: c1 = 1 / (1 - b1^t), c2 = 1 / (1 - b2^t)
+    const float w = P[p] - lr * (m * c1) / (sqrtf(v * c2) + eps);
     P[p] = w;
     // which layer / weight element
     int l = 0;
@@ -269,10 +270,10 @@ hipError_t loss_head(const float* Y, const int8_t* cls, const float* tgt, int B,
     return hipGetLastError();
 }
 
-hipError_t adam(float* P, const float* G, float* M, float* V, long long n, float lr_t, float b1, float b2, float eps,
-                const LayerTable& lt, hipStream_t s) {
-    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P, G, M, V, n, lr_t, b1, b2, eps,
-                       lt);
+hipError_t adam(float* P, const float* G, float* M, float* V, long long n, float lr, float b1, float b2, float eps,
+                float c1, float c2, const LayerTable& lt, hipStream_t s) {
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P, G, M, V, n, lr, b1, b2, eps,
+                       c1, c2, lt);
     return hipGetLastError();
 }
 

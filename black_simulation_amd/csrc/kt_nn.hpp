@@ -59,8 +59,8 @@ hipError_t reduce_ws(const float* ws, int splits, long long stride, int rows, in
 hipError_t colsum(const float* G, int B, int N, float* partial, int chunks, float* dst, hipStream_t s);
 hipError_t loss_head(const float* Y, const int8_t* cls, const float* tgt, int B, float w_sky, float w_disk,
                      float* gy32, uint16_t* gyb, double* loss_sum, hipStream_t s);
-hipError_t adam(float* P, const float* G, float* M, float* V, long long n, float lr_t, float b1, float b2, float eps,
-                const LayerTable& lt, hipStream_t s);
+hipError_t adam(float* P, const float* G, float* M, float* V, long long n, float lr, float b1, float b2, float eps,
+                float c1, float c2, const LayerTable& lt, hipStream_t s);
 hipError_t refresh_bf16(const float* P, long long n, const LayerTable& lt, hipStream_t s);
 hipError_t gather(const float* feat, const int8_t* cls, const float* tgt, const long long* idx, int B, uint16_t* x0,
                   int8_t* bcls, float* btgt, hipStream_t s);

@@ -1,16 +1,975 @@
-// kt_surrogate.hip -- surrogate inference/training kernels (placeholder until the MLP lands).
+// kt_surrogate.hip -- the learned surrogate: training data from the integrator, training
+// with the BF16 MFMA kernels of kt_nn.hip, evaluation against FP64 labels, and traced
+// inference with the gate and a device-side fallback queue for the integrator.
 #include "kt_surrogate.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "../../include/kerr_trace.h"
+
+using namespace ktm;
+using ktd::ViewDev;
 
 namespace kts {
 
-struct Surrogate {
-    int dummy;
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// ---- shadow table: critical radius over (a, i, psi) and the mean shadow radius ------------
+constexpr int kTabA = 41, kTabI = 45, kTabPsi = 129;
+constexpr double kTabA0 = 0.0, kTabA1 = 0.999;
+constexpr double kTabI0 = 1.0 * kPi / 180.0, kTabI1 = 89.0 * kPi / 180.0;
+
+struct Table {
+    const float* rho;  // [kTabA][kTabI][kTabPsi], psi in [0, pi]
+    const float* rsh;  // [kTabA][kTabI]
 };
 
-void destroy(Surrogate* s) { delete s; }
+__host__ __device__ inline void cell(double x, double x0, double x1, int n, int& i, float& f) {
+    double u = (x - x0) / (x1 - x0) * (n - 1);
+    if (u < 0.0) u = 0.0;
+    if (u > n - 1) u = n - 1;
+    i = (int)u;
+    if (i > n - 2) i = n - 2;
+    f = (float)(u - i);
+}
 
-hipError_t trace_with_surrogate(Surrogate*, const kti::TraceLaunch&, double, double, hipStream_t, hipEvent_t*) {
-    return hipErrorNotSupported;
+__host__ __device__ inline float table_rsh(const float* rsh, double a, double inc) {
+    int ia, ii;
+    float fa, fi;
+    cell(a, kTabA0, kTabA1, kTabA, ia, fa);
+    cell(inc, kTabI0, kTabI1, kTabI, ii, fi);
+    const float v00 = rsh[ia * kTabI + ii], v01 = rsh[ia * kTabI + ii + 1];
+    const float v10 = rsh[(ia + 1) * kTabI + ii], v11 = rsh[(ia + 1) * kTabI + ii + 1];
+    return (1 - fa) * ((1 - fi) * v00 + fi * v01) + fa * ((1 - fi) * v10 + fi * v11);
+}
+
+__host__ __device__ inline float table_rho(const float* rho, double a, double inc, double psi) {
+    int ia, ii, ip;
+    float fa, fi, fp;
+    cell(a, kTabA0, kTabA1, kTabA, ia, fa);
+    cell(inc, kTabI0, kTabI1, kTabI, ii, fi);
+    cell(psi, 0.0, kPi, kTabPsi, ip, fp);
+    auto at = [&](int x, int y, int z) { return rho[((long long)x * kTabI + y) * kTabPsi + z]; };
+    float r = 0.f;
+    for (int dx = 0; dx < 2; ++dx)
+        for (int dy = 0; dy < 2; ++dy) {
+            const float w = (dx ? fa : 1 - fa) * (dy ? fi : 1 - fi);
+            r += w * ((1 - fp) * at(ia + dx, ii + dy, ip) + fp * at(ia + dx, ii + dy, ip + 1));
+        }
+    return r;
+}
+
+// normalised inputs of one ray and its relative distance from the critical curve
+__host__ __device__ inline void features_of(const float* rho, const float* rsh, double a, double r_cam, double inc,
+                                            double alpha, double beta, float f[5], float* ring) {
+    const float R = table_rsh(rsh, a, inc);
+    f[0] = (float)(log10(r_cam) - 2.0);
+    f[1] = (float)((cos(inc) - 0.5415) / 0.4545);
+    f[2] = (float)((a - 0.499) / 0.499);
+    f[3] = (float)(alpha / (1.5 * R));
+    f[4] = (float)(beta / (1.5 * R));
+    const double psi = atan2(fabs(beta), alpha);
+    const double rc = table_rho(rho, a, inc, psi);
+    *ring = (float)fabs(sqrt(alpha * alpha + beta * beta) / rc - 1.0);
+}
+
+// ---- kernels -------------------------------------------------------------------------------
+
+__device__ __forceinline__ int find_view(const ViewDev* v, int n, long long idx) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (v[mid].offset <= idx) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Bardeen parameters of ray q of the batch: returns false for rays the surrogate does not
+// cover (pinhole cameras)
+__device__ __forceinline__ bool ray_params(const ViewDev* views, int n_views, long long q, double& a, double& r_cam,
+                                           double& inc, double& alpha, double& beta, double& phi_cam,
+                                           double& r_in) {
+    const ViewDev& v = views[find_view(views, n_views, q)];
+    const long long local = q - v.offset;
+    phi_cam = v.phi_cam;
+    if (v.kind == 2) {
+        const double* p = v.params + 5 * local;
+        a = p[0];
+        r_cam = p[1];
+        inc = p[2];
+        alpha = p[3];
+        beta = p[4];
+        r_in = ktd::isco_dev(a, v.prograde);
+        phi_cam = 0.0;
+        return true;
+    }
+    if (v.kind != 0) return false;
+    a = v.a;
+    r_cam = v.r_cam;
+    inc = atan2(v.sinI, v.cosI);
+    r_in = v.r_in;
+    if (v.params) {
+        alpha = v.params[2 * local];
+        beta = v.params[2 * local + 1];
+    } else {
+        const int row = (int)(local / v.width), col = (int)(local - (long long)row * v.width);
+        alpha = -v.half_width + ((double)col + 0.5) * v.px;
+        beta = v.half_height - ((double)row + 0.5) * v.py;
+    }
+    return true;
+}
+
+__global__ void k_features_views(const ViewDev* views, int n_views, long long n, Table t, uint16_t* x0, float* ring) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double a, r_cam, inc, alpha, beta, phi_cam, r_in;
+    float f[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, rg = -1.f;  // ring < 0: not coverable, always falls back
+    if (ray_params(views, n_views, q, a, r_cam, inc, alpha, beta, phi_cam, r_in))
+        features_of(t.rho, t.rsh, a, r_cam, inc, alpha, beta, f, &rg);
+    uint16_t* row = x0 + q * 32;
+    for (int k = 0; k < 32; ++k) row[k] = k < 5 ? f2bf(f[k]) : (uint16_t)0;
+    ring[q] = rg;
+}
+
+// features and targets of integrator-labelled rays (kind-2 params)
+__global__ void k_targets(const double* params, long long n, Table t, const int8_t* cls, const double* th,
+                          const double* ph, const double* rh, const double* gg, int prograde, float* feat,
+                          int8_t* ocls, float* tgt, float* ring) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double* p = params + 5 * q;
+    float f[5], rg;
+    features_of(t.rho, t.rsh, p[0], p[1], p[2], p[3], p[4], f, &rg);
+    for (int k = 0; k < 5; ++k) feat[q * 5 + k] = f[k];
+    if (ring) ring[q] = rg;
+    int c = cls[q] & 7;
+    if (c == 3) c = 1;  // a ray that exhausted its budget orbits on the critical curve: dark
+    ocls[q] = (int8_t)c;
+    float* o = tgt + q * 5;
+    for (int k = 0; k < 5; ++k) o[k] = 0.f;
+    if (c == 0) {
+        o[0] = (float)cos(th[q]);
+        o[1] = (float)cos(ph[q]);
+        o[2] = (float)sin(ph[q]);
+    } else if (c == 2) {
+        o[3] = (float)(rh[q] / ktd::isco_dev(p[0], prograde));
+        o[4] = (float)log(gg[q]);
+    }
+}
+
+// batch indices: a keyed Feistel bijection on [0, 2^bits), cycle-walked into [0, n)
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// balanced Feistel network on [0, 2^bits), bits even: (L, R) -> (R, L ^ F(R)), four rounds
+__device__ __forceinline__ uint64_t feistel(uint64_t x, int bits, uint64_t key) {
+    const int half = bits / 2;
+    const uint64_t mask = (1ull << half) - 1;
+    uint64_t L = x >> half, R = x & mask;
+    for (int r = 0; r < 4; ++r) {
+        const uint64_t F = mix32((uint32_t)R ^ (uint32_t)(key >> (16 * r)) ^ ((uint32_t)r * 0x9e3779b9u)) & mask;
+        const uint64_t nR = L ^ F;
+        L = R;
+        R = nR;
+    }
+    return (L << half) | R;
+}
+
+__global__ void k_batch_index(long long n, int bits, uint64_t key, long long start, int B, long long* idx) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    uint64_t x = (uint64_t)((start + b) % n);
+    do {
+        x = feistel(x, bits, key);
+    } while (x >= (uint64_t)n);
+    idx[b] = (long long)x;
+}
+
+// decode + gate: answered rays get their outputs, the others join the fallback queue
+__global__ void k_gate(const ViewDev* views, int n_views, long long n, const float* Y, const float* ring,
+                       float gate_prob, float gate_ring, ktd::Outputs out, long long* fallback,
+                       unsigned long long* n_fallback) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const float* y = Y + q * 8;
+    const float m = fmaxf(y[0], fmaxf(y[1], y[2]));
+    const float e0 = __expf(y[0] - m), e1 = __expf(y[1] - m), e2 = __expf(y[2] - m);
+    const float inv = 1.0f / (e0 + e1 + e2);
+    const float p0 = e0 * inv, p1 = e1 * inv, p2 = e2 * inv;
+    const int c = p0 >= p1 && p0 >= p2 ? 0 : (p1 >= p2 ? 1 : 2);
+    const float pm = c == 0 ? p0 : c == 1 ? p1 : p2;
+    const float rg = ring[q];
+    if (!(pm > gate_prob) || !(rg > gate_ring)) {
+        const unsigned long long at = atomicAdd(n_fallback, 1ull);
+        fallback[at] = q;
+        return;
+    }
+    double a, r_cam, inc, alpha, beta, phi_cam, r_in;
+    ray_params(views, n_views, q, a, r_cam, inc, alpha, beta, phi_cam, r_in);
+    const double nan = __builtin_nan("");
+    if (out.cls) out.cls[q] = (int8_t)(c | ktd::kFlagSurrogate);
+    double th = nan, ph = nan, rh = nan, g = nan;
+    if (c == 0) {
+        double ct = y[3];
+        ct = ct > 1.0 ? 1.0 : (ct < -1.0 ? -1.0 : ct);
+        th = acos(ct);
+        ph = ktd::wrap_2pi(atan2((double)y[5], (double)y[4]) + phi_cam);
+    } else if (c == 2) {
+        rh = (double)y[6] * r_in;
+        g = exp((double)y[7]);
+    }
+    if (out.theta) out.theta[q] = th;
+    if (out.phi) out.phi[q] = ph;
+    if (out.r_hit) out.r_hit[q] = rh;
+    if (out.g) out.g[q] = g;
+    if (out.steps) out.steps[q] = 0;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+__global__ void k_copy_count(const unsigned long long* c, long long* dst, unsigned long long* stat) {
+    if (threadIdx.x == 0) {
+        *dst = (long long)*c;
+        if (stat) *stat = *c;
+    }
+}
+
+__global__ void k_feat_to_x0(const float* feat, long long n, uint16_t* x0) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    for (int k = 0; k < 32; ++k) x0[q * 32 + k] = k < 5 ? f2bf(feat[q * 5 + k]) : (uint16_t)0;
+}
+
+template <class T>
+hipError_t dalloc(T** p, size_t n) {
+    return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1));
+}
+
+}  // namespace
+
+// ---- the surrogate object ---------------------------------------------------------------------
+
+struct Surrogate {
+    int device = 0;
+    bool have_params = false;
+    float* P = nullptr;     // FP32 master parameters
+    float* G = nullptr;     // gradients
+    float* M = nullptr;     // Adam moments
+    float* V = nullptr;
+    uint16_t* wb[ktn::kLayers] = {};
+    uint16_t* wt[ktn::kLayers] = {};
+    ktn::LayerTable lt{};
+    float* tab_rho = nullptr;
+    float* tab_rsh = nullptr;
+    std::vector<float> h_rho, h_rsh;
+    // work buffers, sized for `cap` rows
+    long long cap = 0;
+    uint16_t* x0 = nullptr;
+    float* Z[4] = {};
+    uint16_t* H[4] = {};
+    float* Y = nullptr;
+    float* gy32 = nullptr;
+    uint16_t* gyb = nullptr;
+    float* g32[2] = {};
+    uint16_t* gb[2] = {};
+    int8_t* bcls = nullptr;
+    float* btgt = nullptr;
+    long long* bidx = nullptr;
+    float* ws = nullptr;
+    float* partial = nullptr;
+    double* loss = nullptr;
+    // inference buffers
+    long long icap = 0;
+    uint16_t* ix0 = nullptr;
+    uint16_t* iH[2] = {};
+    float* iY = nullptr;
+    float* iring = nullptr;
+    long long* fallback = nullptr;
+    unsigned long long* nfb = nullptr;  // [0] fallback count
+    long long* nfb_ll = nullptr;
+    static constexpr int kSplits = 64, kChunks = 256;
+};
+
+static hipError_t free_all(Surrogate* s) {
+    auto f = [](void* p) {
+        if (p) (void)hipFree(p);
+    };
+    f(s->P);
+    f(s->G);
+    f(s->M);
+    f(s->V);
+    for (int l = 0; l < ktn::kLayers; ++l) {
+        f(s->wb[l]);
+        f(s->wt[l]);
+    }
+    f(s->tab_rho);
+    f(s->tab_rsh);
+    f(s->x0);
+    for (int l = 0; l < 4; ++l) {
+        f(s->Z[l]);
+        f(s->H[l]);
+    }
+    f(s->Y);
+    f(s->gy32);
+    f(s->gyb);
+    for (int k = 0; k < 2; ++k) {
+        f(s->g32[k]);
+        f(s->gb[k]);
+        f(s->iH[k]);
+    }
+    f(s->bcls);
+    f(s->btgt);
+    f(s->bidx);
+    f(s->ws);
+    f(s->partial);
+    f(s->loss);
+    f(s->ix0);
+    f(s->iY);
+    f(s->iring);
+    f(s->fallback);
+    f(s->nfb);
+    f(s->nfb_ll);
+    return hipSuccess;
+}
+
+#define KTS_TRY(x)                          \
+    do {                                    \
+        hipError_t e_ = (x);                \
+        if (e_ != hipSuccess) return e_;    \
+    } while (0)
+
+static hipError_t build_table(Surrogate* s) {
+    s->h_rho.assign((size_t)kTabA * kTabI * kTabPsi, 0.f);
+    s->h_rsh.assign((size_t)kTabA * kTabI, 0.f);
+    std::vector<double> al(16 * 200 + 8 + 200), be(al.size());
+    for (int ia = 0; ia < kTabA; ++ia) {
+        const double a = kTabA0 + (kTabA1 - kTabA0) * ia / (kTabA - 1);
+        for (int ii = 0; ii < kTabI; ++ii) {
+            const double inc = kTabI0 + (kTabI1 - kTabI0) * ii / (kTabI - 1);
+            const int n = kt_critical_curve(a < 1e-7 ? 0.0 : a, inc, 200, al.data(), be.data(), (int)al.size());
+            double mean = 0.0;
+            for (int k = 0; k < 72; ++k) mean += kt_critical_radius(al.data(), be.data(), n, 2.0 * kPi * k / 72);
+            s->h_rsh[(size_t)ia * kTabI + ii] = (float)(mean / 72.0);
+            for (int ip = 0; ip < kTabPsi; ++ip) {
+                const double psi = kPi * ip / (kTabPsi - 1);
+                s->h_rho[((size_t)ia * kTabI + ii) * kTabPsi + ip] =
+                    (float)kt_critical_radius(al.data(), be.data(), n, psi);
+            }
+        }
+    }
+    KTS_TRY(dalloc(&s->tab_rho, s->h_rho.size()));
+    KTS_TRY(dalloc(&s->tab_rsh, s->h_rsh.size()));
+    KTS_TRY(hipMemcpy(s->tab_rho, s->h_rho.data(), s->h_rho.size() * 4, hipMemcpyHostToDevice));
+    KTS_TRY(hipMemcpy(s->tab_rsh, s->h_rsh.data(), s->h_rsh.size() * 4, hipMemcpyHostToDevice));
+    return hipSuccess;
+}
+
+Surrogate* create(int device, hipError_t* err) {
+    Surrogate* s = new Surrogate();
+    s->device = device;
+    auto fail = [&](hipError_t e) {
+        if (err) *err = e;
+        free_all(s);
+        delete s;
+        return (Surrogate*)nullptr;
+    };
+    hipError_t e;
+    if ((e = hipSetDevice(device)) != hipSuccess) return fail(e);
+    const long long n = ktn::kParams;
+    if ((e = dalloc(&s->P, n)) != hipSuccess || (e = dalloc(&s->G, n)) != hipSuccess ||
+        (e = dalloc(&s->M, n)) != hipSuccess || (e = dalloc(&s->V, n)) != hipSuccess)
+        return fail(e);
+    for (int l = 0; l < ktn::kLayers; ++l) {
+        s->lt.off[l] = ktn::param_offset(l);
+        s->lt.kreal[l] = ktn::kKreal[l];
+        s->lt.kpad[l] = ktn::kKpad[l];
+        s->lt.n[l] = ktn::kN[l];
+        s->lt.nld[l] = ktn::kNld[l];
+        const size_t nb = (size_t)ktn::kKpad[l] * ktn::kNld[l], nt = (size_t)ktn::kN[l] * ktn::kKpad[l];
+        if ((e = dalloc(&s->wb[l], nb)) != hipSuccess || (e = dalloc(&s->wt[l], nt)) != hipSuccess) return fail(e);
+        if ((e = hipMemset(s->wb[l], 0, nb * 2)) != hipSuccess || (e = hipMemset(s->wt[l], 0, nt * 2)) != hipSuccess)
+            return fail(e);
+        s->lt.wb[l] = s->wb[l];
+        s->lt.wt[l] = s->wt[l];
+    }
+    if ((e = build_table(s)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&s->nfb, 4)) != hipSuccess || (e = dalloc(&s->nfb_ll, 1)) != hipSuccess ||
+        (e = dalloc(&s->loss, 1)) != hipSuccess)
+        return fail(e);
+    if (err) *err = hipSuccess;
+    return s;
+}
+
+void destroy(Surrogate* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    (void)hipDeviceSynchronize();
+    free_all(s);
+    delete s;
+}
+
+bool ready(const Surrogate* s) { return s && s->have_params; }
+
+double table_critical_radius(const Surrogate* s, double a, double inc, double psi) {
+    return table_rho(s->h_rho.data(), a, inc, psi);
+}
+double table_shadow_radius(const Surrogate* s, double a, double inc) { return table_rsh(s->h_rsh.data(), a, inc); }
+
+hipError_t set_params(Surrogate* s, const float* host, long long n) {
+    if (n != ktn::kParams) return hipErrorInvalidValue;
+    KTS_TRY(hipSetDevice(s->device));
+    KTS_TRY(hipMemcpy(s->P, host, sizeof(float) * n, hipMemcpyHostToDevice));
+    KTS_TRY(hipMemset(s->M, 0, sizeof(float) * n));
+    KTS_TRY(hipMemset(s->V, 0, sizeof(float) * n));
+    KTS_TRY(ktn::refresh_bf16(s->P, n, s->lt, nullptr));
+    KTS_TRY(hipDeviceSynchronize());
+    s->have_params = true;
+    return hipSuccess;
+}
+
+hipError_t get_params(Surrogate* s, float* host, long long n) {
+    if (n != ktn::kParams) return hipErrorInvalidValue;
+    KTS_TRY(hipSetDevice(s->device));
+    return hipMemcpy(host, s->P, sizeof(float) * n, hipMemcpyDeviceToHost);
+}
+
+hipError_t init_params(Surrogate* s, uint64_t seed) {
+    std::vector<float> p(ktn::kParams, 0.f);
+    std::mt19937_64 rng(seed);
+    for (int l = 0; l < ktn::kLayers; ++l) {
+        const double lim = std::sqrt(6.0 / ktn::kKreal[l]);
+        std::uniform_real_distribution<double> u(-lim, lim);
+        const long long off = ktn::param_offset(l);
+        const long long nw = (long long)ktn::kKreal[l] * ktn::kN[l];
+        for (long long i = 0; i < nw; ++i) p[off + i] = (float)u(rng);
+        // biases start at 0; the output layer starts small
+        if (l == ktn::kLayers - 1)
+            for (long long i = 0; i < nw; ++i) p[off + i] *= 0.1f;
+    }
+    return set_params(s, p.data(), ktn::kParams);
+}
+
+static hipError_t ensure_train_buffers(Surrogate* s, long long B) {
+    if (B <= s->cap) return hipSuccess;
+    auto f = [](void* p) {
+        if (p) (void)hipFree(p);
+    };
+    f(s->x0);
+    for (int l = 0; l < 4; ++l) {
+        f(s->Z[l]);
+        f(s->H[l]);
+    }
+    f(s->Y);
+    f(s->gy32);
+    f(s->gyb);
+    for (int k = 0; k < 2; ++k) {
+        f(s->g32[k]);
+        f(s->gb[k]);
+    }
+    f(s->bcls);
+    f(s->btgt);
+    f(s->bidx);
+    f(s->ws);
+    f(s->partial);
+    s->cap = 0;
+    KTS_TRY(dalloc(&s->x0, (size_t)B * 32));
+    for (int l = 0; l < 4; ++l) {
+        KTS_TRY(dalloc(&s->Z[l], (size_t)B * 256));
+        KTS_TRY(dalloc(&s->H[l], (size_t)B * 256));
+    }
+    KTS_TRY(dalloc(&s->Y, (size_t)B * 8));
+    KTS_TRY(dalloc(&s->gy32, (size_t)B * 8));
+    KTS_TRY(dalloc(&s->gyb, (size_t)B * 32));
+    for (int k = 0; k < 2; ++k) {
+        KTS_TRY(dalloc(&s->g32[k], (size_t)B * 256));
+        KTS_TRY(dalloc(&s->gb[k], (size_t)B * 256));
+    }
+    KTS_TRY(dalloc(&s->bcls, (size_t)B));
+    KTS_TRY(dalloc(&s->btgt, (size_t)B * 5));
+    KTS_TRY(dalloc(&s->bidx, (size_t)B));
+    KTS_TRY(dalloc(&s->ws, (size_t)Surrogate::kSplits * 256 * 256));
+    KTS_TRY(dalloc(&s->partial, (size_t)Surrogate::kChunks * 256));
+    s->cap = B;
+    return hipSuccess;
+}
+
+// forward over B rows of x0 into Z / H / Y (z stored when keep_z)
+static hipError_t forward(Surrogate* s, const uint16_t* x0, uint16_t** H, float** Z, float* Y, int B, bool keep_z,
+                          hipStream_t st) {
+    for (int l = 0; l < 4; ++l) {
+        ktn::GemmArgs g{};
+        g.M = B;
+        g.N = 256;
+        g.K = ktn::kKpad[l];
+        g.A = l == 0 ? x0 : H[l - 1];
+        g.lda = ktn::kKpad[l];
+        g.B = s->wt[l];
+        g.ldb = ktn::kKpad[l];
+        g.ldc = 256;
+        g.bias = s->P + ktn::param_offset(l) + (long long)ktn::kKreal[l] * ktn::kN[l];
+        g.out32 = keep_z ? Z[l] : nullptr;
+        g.outb = H[l];
+        g.k_split = g.K;
+        KTS_TRY(ktn::gemm(g, false, true, ktn::kEpiFwd, 1, st));
+    }
+    ktn::GemmArgs g{};
+    g.M = B;
+    g.N = 8;
+    g.K = 256;
+    g.A = H[3];
+    g.lda = 256;
+    g.B = s->wt[4];
+    g.ldb = 256;
+    g.ldc = 8;
+    g.bias = s->P + ktn::param_offset(4) + 256LL * 8;
+    g.out32 = Y;
+    g.k_split = 256;
+    return ktn::gemm(g, false, true, ktn::kEpiLin, 1, st);
+}
+
+// backward of the mean loss; gradients into s->G
+static hipError_t backward(Surrogate* s, int B, hipStream_t st) {
+    const int splits = std::max(1, std::min(Surrogate::kSplits, B / 1024));
+    const int ksplit = ((B / splits + 31) / 32) * 32;
+    auto dW = [&](const uint16_t* A, int lda, int M, const uint16_t* Bm, int ldb, int N, int l) -> hipError_t {
+        ktn::GemmArgs g{};
+        g.M = M;
+        g.N = N;
+        g.K = B;
+        g.A = A;
+        g.lda = lda;
+        g.B = Bm;
+        g.ldb = ldb;
+        g.ldc = N;
+        g.out32 = s->ws;
+        g.k_split = ksplit;
+        g.ws_stride = (long long)M * N;
+        const int nsplit = (B + ksplit - 1) / ksplit;
+        KTS_TRY(ktn::gemm(g, true, false, ktn::kEpiWs, nsplit, st));
+        return ktn::reduce_ws(s->ws, nsplit, (long long)M * N, ktn::kKreal[l], N, N, s->G + ktn::param_offset(l), st);
+    };
+    auto db = [&](const float* Gm, int N, int l) -> hipError_t {
+        return ktn::colsum(Gm, B, N, s->partial, Surrogate::kChunks,
+                           s->G + ktn::param_offset(l) + (long long)ktn::kKreal[l] * ktn::kN[l], st);
+    };
+    // output layer
+    KTS_TRY(dW(s->H[3], 256, 256, s->gyb, 32, 8, 4));
+    KTS_TRY(db(s->gy32, 8, 4));
+    {
+        ktn::GemmArgs g{};
+        g.M = B;
+        g.N = 256;
+        g.K = 32;
+        g.A = s->gyb;
+        g.lda = 32;
+        g.B = s->wb[4];
+        g.ldb = 32;
+        g.ldc = 256;
+        g.zin = s->Z[3];
+        g.ldz = 256;
+        g.out32 = s->g32[1];
+        g.outb = s->gb[1];
+        g.k_split = 32;
+        KTS_TRY(ktn::gemm(g, false, true, ktn::kEpiBwd, 1, st));
+    }
+    int cur = 1;
+    for (int l = 3; l >= 1; --l) {
+        KTS_TRY(dW(s->H[l - 1], 256, 256, s->gb[cur], 256, 256, l));
+        KTS_TRY(db(s->g32[cur], 256, l));
+        ktn::GemmArgs g{};
+        g.M = B;
+        g.N = 256;
+        g.K = 256;
+        g.A = s->gb[cur];
+        g.lda = 256;
+        g.B = s->wb[l];
+        g.ldb = 256;
+        g.ldc = 256;
+        g.zin = s->Z[l - 1];
+        g.ldz = 256;
+        g.out32 = s->g32[cur ^ 1];
+        g.outb = s->gb[cur ^ 1];
+        g.k_split = 256;
+        KTS_TRY(ktn::gemm(g, false, true, ktn::kEpiBwd, 1, st));
+        cur ^= 1;
+    }
+    KTS_TRY(dW(s->x0, 32, 32, s->gb[cur], 256, 256, 0));
+    return db(s->g32[cur], 256, 0);
+}
+
+hipError_t predict(Surrogate* s, const float* feat, long long n, float* out, hipStream_t st) {
+    KTS_TRY(hipSetDevice(s->device));
+    const long long chunk = 1 << 20;
+    KTS_TRY(ensure_train_buffers(s, std::min(n, chunk)));
+    for (long long o = 0; o < n; o += chunk) {
+        const int B = (int)std::min(chunk, n - o);
+        hipLaunchKernelGGL(k_feat_to_x0, dim3((B + 255) / 256), dim3(256), 0, st, feat + o * 5, (long long)B, s->x0);
+        KTS_TRY(forward(s, s->x0, s->H, s->Z, out + o * 8, B, false, st));
+    }
+    return hipGetLastError();
+}
+
+hipError_t gradients(Surrogate* s, const float* feat, const int8_t* cls, const float* tgt, int n, float* grads,
+                     double* loss, hipStream_t st) {
+    KTS_TRY(hipSetDevice(s->device));
+    KTS_TRY(ensure_train_buffers(s, n));
+    KTS_TRY(ktn::gather(feat, cls, tgt, nullptr, n, s->x0, s->bcls, s->btgt, st));
+    KTS_TRY(forward(s, s->x0, s->H, s->Z, s->Y, n, true, st));
+    KTS_TRY(hipMemsetAsync(s->loss, 0, sizeof(double), st));
+    KTS_TRY(ktn::loss_head(s->Y, s->bcls, s->btgt, n, 1.0f, 1.0f, s->gy32, s->gyb, s->loss, st));
+    KTS_TRY(backward(s, n, st));
+    if (grads) KTS_TRY(hipMemcpyAsync(grads, s->G, sizeof(float) * ktn::kParams, hipMemcpyDeviceToDevice, st));
+    if (loss) {
+        KTS_TRY(hipMemcpyAsync(loss, s->loss, sizeof(double), hipMemcpyDeviceToHost, st));
+        KTS_TRY(hipStreamSynchronize(st));
+        *loss /= n;
+    }
+    return hipSuccess;
+}
+
+hipError_t adam_step(Surrogate* s, const float* grads, double lr, int t, hipStream_t st) {
+    const double b1 = 0.9, b2 = 0.999;
+    const double c1 = 1.0 / (1.0 - std::pow(b1, t)), c2 = 1.0 / (1.0 - std::pow(b2, t));
+    return ktn::adam(s->P, grads ? grads : s->G, s->M, s->V, ktn::kParams, (float)lr, (float)b1, (float)b2, 1e-8f,
+                     (float)c1, (float)c2, s->lt, st);
+}
+
+// ---- data generation ----------------------------------------------------------------------
+
+struct Dataset {
+    long long n = 0;
+    double* params = nullptr;  // 5 per ray
+    float* feat = nullptr;
+    int8_t* cls = nullptr;
+    float* tgt = nullptr;
+    float* ring = nullptr;
+    // integrator outputs (kept for the held-out judgement)
+    int8_t* ocls = nullptr;
+    double *th = nullptr, *ph = nullptr, *rh = nullptr, *gg = nullptr;
+    unsigned long long* counter = nullptr;
+    ViewDev* view = nullptr;
+    void release() {
+        for (void* p : {(void*)params, (void*)feat, (void*)cls, (void*)tgt, (void*)ring, (void*)ocls, (void*)th,
+                        (void*)ph, (void*)rh, (void*)gg, (void*)counter, (void*)view})
+            if (p) (void)hipFree(p);
+        *this = Dataset();
+    }
+};
+
+// seeded uniform draws over the batch-view ranges, with the band around the critical
+// curve `band_factor` times denser (by thinning the rest)
+static void draw_rays(const Surrogate* s, long long n, uint64_t seed, double band, double factor,
+                      std::vector<double>& p) {
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    p.resize((size_t)n * 5);
+    long long k = 0;
+    const double keep_out = factor > 1.0 ? 1.0 / factor : 1.0;
+    while (k < n) {
+        const double a = 0.998 * U(rng);
+        const double inc = (5.0 + 80.0 * U(rng)) * kPi / 180.0;
+        const double rc = std::pow(10.0, 1.0 + 2.0 * U(rng));
+        const double R = table_rsh(s->h_rsh.data(), a, inc);
+        const double al = (3.0 * U(rng) - 1.5) * R, be = (3.0 * U(rng) - 1.5) * R;
+        const double rho_c = table_rho(s->h_rho.data(), a, inc, std::atan2(std::fabs(be), al));
+        const bool in_band = std::fabs(std::sqrt(al * al + be * be) / rho_c - 1.0) < band;
+        if (!in_band && U(rng) >= keep_out) continue;
+        double* q = &p[(size_t)k * 5];
+        q[0] = a;
+        q[1] = rc;
+        q[2] = inc;
+        q[3] = al;
+        q[4] = be;
+        ++k;
+    }
+}
+
+static hipError_t label(Surrogate* s, Dataset& d, const std::vector<double>& hp, int precision, const TrainOpts& o) {
+    const long long n = (long long)hp.size() / 5;
+    d.n = n;
+    KTS_TRY(dalloc(&d.params, (size_t)n * 5));
+    KTS_TRY(dalloc(&d.feat, (size_t)n * 5));
+    KTS_TRY(dalloc(&d.cls, (size_t)n));
+    KTS_TRY(dalloc(&d.tgt, (size_t)n * 5));
+    KTS_TRY(dalloc(&d.ring, (size_t)n));
+    KTS_TRY(dalloc(&d.ocls, (size_t)n));
+    KTS_TRY(dalloc(&d.th, (size_t)n));
+    KTS_TRY(dalloc(&d.ph, (size_t)n));
+    KTS_TRY(dalloc(&d.rh, (size_t)n));
+    KTS_TRY(dalloc(&d.gg, (size_t)n));
+    KTS_TRY(dalloc(&d.counter, 1 + kti::kStatCount));
+    KTS_TRY(dalloc(&d.view, 1));
+    KTS_TRY(hipMemcpy(d.params, hp.data(), sizeof(double) * hp.size(), hipMemcpyHostToDevice));
+    ViewDev v;
+    std::memset(&v, 0, sizeof v);
+    v.kind = 2;
+    v.width = (int)std::min<long long>(n, 0x7fffffff);
+    v.height = 1;
+    v.disk = 1;
+    v.prograde = 1;
+    v.r_cap = o.capture_factor;  // kind 2: the capture factor; r+ is per ray
+    v.r_out = o.disk_outer;
+    v.escape_radius_ = o.escape_radius;
+    v.params = d.params;
+    KTS_TRY(hipMemcpy(d.view, &v, sizeof v, hipMemcpyHostToDevice));
+    kti::TraceLaunch L{};
+    L.views = d.view;
+    L.n_views = 1;
+    L.total = n;
+    L.counter = d.counter;
+    L.stats = d.counter + 1;
+    L.out = ktd::Outputs{d.ocls, d.th, d.ph, d.rh, d.gg, nullptr};
+    L.rtol = o.rtol;
+    L.atol = o.atol;
+    L.rtol32 = o.rtol32;
+    L.atol32 = o.atol32;
+    L.restep_drift = o.restep_drift;
+    L.max_steps = 200000;
+    L.precision = precision;
+    L.mode = KT_MODE_ONESHOT;
+    L.block = 256;
+    KTS_TRY(kti::launch_trace(L, nullptr));
+    Table t{s->tab_rho, s->tab_rsh};
+    hipLaunchKernelGGL(k_targets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, d.params, n, t, d.ocls,
+                       d.th, d.ph, d.rh, d.gg, 1, d.feat, d.cls, d.tgt, d.ring);
+    KTS_TRY(hipGetLastError());
+    return hipDeviceSynchronize();
+}
+
+static double quantile(std::vector<double> v, double q) {
+    if (v.empty()) return std::nan("");
+    const size_t k = std::min(v.size() - 1, (size_t)(q * (v.size() - 1) + 0.5));
+    std::nth_element(v.begin(), v.begin() + k, v.end());
+    return v[k];
+}
+
+// held-out judgement: predictions against the FP64 integrator's outputs
+static hipError_t evaluate(Surrogate* s, Dataset& d, const TrainOpts& o, TrainReport& rep) {
+    const long long n = d.n;
+    float* y = nullptr;
+    KTS_TRY(dalloc(&y, (size_t)n * 8));
+    hipError_t e = predict(s, d.feat, n, y, nullptr);
+    std::vector<float> hy((size_t)n * 8), ring((size_t)n), tgt((size_t)n * 5);
+    std::vector<int8_t> cls((size_t)n);
+    std::vector<double> hp((size_t)n * 5), th((size_t)n), ph((size_t)n), rh((size_t)n), gg((size_t)n);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(hy.data(), y, hy.size() * 4, hipMemcpyDeviceToHost);
+    (void)hipFree(y);
+    KTS_TRY(e);
+    KTS_TRY(hipMemcpy(ring.data(), d.ring, ring.size() * 4, hipMemcpyDeviceToHost));
+    KTS_TRY(hipMemcpy(cls.data(), d.cls, cls.size(), hipMemcpyDeviceToHost));
+    KTS_TRY(hipMemcpy(th.data(), d.th, th.size() * 8, hipMemcpyDeviceToHost));
+    KTS_TRY(hipMemcpy(ph.data(), d.ph, ph.size() * 8, hipMemcpyDeviceToHost));
+    KTS_TRY(hipMemcpy(rh.data(), d.rh, rh.size() * 8, hipMemcpyDeviceToHost));
+    KTS_TRY(hipMemcpy(gg.data(), d.gg, gg.size() * 8, hipMemcpyDeviceToHost));
+    KTS_TRY(hipMemcpy(hp.data(), d.params, hp.size() * 8, hipMemcpyDeviceToHost));
+    long long agree = 0, gated = 0, gated_agree = 0;
+    std::vector<double> defl, gre, rre;
+    for (long long q = 0; q < n; ++q) {
+        const float* yy = &hy[(size_t)q * 8];
+        const float m = std::max(yy[0], std::max(yy[1], yy[2]));
+        const double e0 = std::exp(yy[0] - m), e1 = std::exp(yy[1] - m), e2 = std::exp(yy[2] - m);
+        const double sum = e0 + e1 + e2;
+        const int c = e0 >= e1 && e0 >= e2 ? 0 : (e1 >= e2 ? 1 : 2);
+        const double pm = std::max(e0, std::max(e1, e2)) / sum;
+        agree += c == cls[q];
+        if (!(pm > 0.999) || !(ring[q] > 0.03)) continue;
+        ++gated;
+        gated_agree += c == cls[q];
+        if (c != cls[q]) continue;
+        if (c == 0) {
+            const double ct = std::max(-1.0, std::min(1.0, (double)yy[3]));
+            const double pt = std::atan2((double)yy[5], (double)yy[4]);
+            const double st = std::sqrt(1.0 - ct * ct);
+            const double v1[3] = {st * std::cos(pt), st * std::sin(pt), ct};
+            const double v2[3] = {std::sin(th[q]) * std::cos(ph[q]), std::sin(th[q]) * std::sin(ph[q]), std::cos(th[q])};
+            const double dx = v1[0] - v2[0], dy = v1[1] - v2[1], dz = v1[2] - v2[2];
+            defl.push_back(2.0 * std::asin(std::min(1.0, std::sqrt(dx * dx + dy * dy + dz * dz) / 2.0)));
+        } else if (c == 2) {
+            const double risco = kt_isco_radius(hp[(size_t)q * 5], 1);
+            rre.push_back(std::fabs(yy[6] * risco / rh[q] - 1.0));
+            gre.push_back(std::fabs(std::exp((double)yy[7]) / gg[q] - 1.0));
+        }
+    }
+    rep.class_agreement = (double)agree / n;
+    rep.gated_fraction = (double)gated / n;
+    rep.gated_class_agreement = gated ? (double)gated_agree / gated : 0.0;
+    rep.defl_median = quantile(defl, 0.5);
+    rep.defl_p99 = quantile(defl, 0.99);
+    rep.defl_p999 = quantile(defl, 0.999);
+    rep.g_rel_median = quantile(gre, 0.5);
+    rep.g_rel_p99 = quantile(gre, 0.99);
+    rep.rhit_rel_median = quantile(rre, 0.5);
+    rep.rhit_rel_p99 = quantile(rre, 0.99);
+    (void)o;
+    return hipSuccess;
+}
+
+hipError_t train(Surrogate* s, const TrainOpts& o, TrainReport& rep) {
+    using clk = std::chrono::steady_clock;
+    KTS_TRY(hipSetDevice(s->device));
+    if (!s->have_params) KTS_TRY(init_params(s, o.seed ^ 0x1234567ull));
+    // data: the training set labelled by the mixed integrator, the held-out set by FP64
+    auto t0 = clk::now();
+    Dataset tr, ho;
+    std::vector<double> hp;
+    draw_rays(s, o.n_train, o.seed, o.band, o.band_factor, hp);
+    hipError_t e = label(s, tr, hp, o.label_precision, o);
+    if (e == hipSuccess) {
+        draw_rays(s, o.n_holdout, o.seed ^ 0xABCDEFull, o.band, 1.0, hp);
+        e = label(s, ho, hp, KT_PREC_FP64, o);
+    }
+    if (e != hipSuccess) {
+        tr.release();
+        ho.release();
+        return e;
+    }
+    rep.datagen_seconds = std::chrono::duration<double>(clk::now() - t0).count();
+
+    // training
+    const int B = (int)std::min<long long>(o.batch, tr.n);
+    e = ensure_train_buffers(s, B);
+    const long long steps_per_epoch = std::max<long long>(1, tr.n / B);
+    const long long total_steps = steps_per_epoch * o.epochs;
+    int bits = 2;
+    while ((1ll << bits) < tr.n) bits += 2;
+    std::mt19937_64 rng(o.seed ^ 0x77ull);
+    auto t1 = clk::now();
+    long long t = 0;
+    double loss_sum = 0.0;
+    for (int ep = 0; ep < o.epochs && e == hipSuccess; ++ep) {
+        const uint64_t key = rng();
+        double ep_loss = 0.0;
+        KTS_TRY(hipMemsetAsync(s->loss, 0, sizeof(double), nullptr));
+        for (long long k = 0; k < steps_per_epoch && e == hipSuccess; ++k) {
+            hipLaunchKernelGGL(k_batch_index, dim3((B + 255) / 256), dim3(256), 0, nullptr, tr.n, bits, key,
+                               k * (long long)B, B, s->bidx);
+            e = ktn::gather(tr.feat, tr.cls, tr.tgt, s->bidx, B, s->x0, s->bcls, s->btgt, nullptr);
+            if (e == hipSuccess) e = forward(s, s->x0, s->H, s->Z, s->Y, B, true, nullptr);
+            if (e == hipSuccess)
+                e = ktn::loss_head(s->Y, s->bcls, s->btgt, B, (float)o.w_sky, (float)o.w_disk, s->gy32, s->gyb,
+                                   s->loss, nullptr);
+            if (e == hipSuccess) e = backward(s, B, nullptr);
+            ++t;
+            const double lr = 0.5 * o.lr * (1.0 + std::cos(kPi * (double)(t - 1) / (double)total_steps));
+            if (e == hipSuccess) e = adam_step(s, nullptr, lr, (int)t, nullptr);
+        }
+        if (e == hipSuccess) e = hipMemcpy(&ep_loss, s->loss, sizeof(double), hipMemcpyDeviceToHost);
+        ep_loss /= (double)steps_per_epoch * B;
+        if (ep == 0) rep.first_loss = ep_loss;
+        loss_sum = ep_loss;
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    rep.train_seconds = std::chrono::duration<double>(clk::now() - t1).count();
+    rep.final_loss = loss_sum;
+    rep.steps = t;
+    // BF16 MFMA work per step: forward (2 B sum K N) + backward (2x that)
+    double macs = 0;
+    for (int l = 0; l < ktn::kLayers; ++l) macs += (double)ktn::kKpad[l] * ktn::kN[l];
+    rep.train_gemm_tflops = rep.train_seconds > 0 ? 6.0 * macs * B * (double)t / rep.train_seconds / 1e12 : 0.0;
+    auto t2 = clk::now();
+    if (e == hipSuccess) e = evaluate(s, ho, o, rep);
+    rep.eval_seconds = std::chrono::duration<double>(clk::now() - t2).count();
+    tr.release();
+    ho.release();
+    return e;
+}
+
+// ---- traced inference ------------------------------------------------------------------------
+
+static hipError_t ensure_infer_buffers(Surrogate* s, long long n) {
+    if (n <= s->icap) return hipSuccess;
+    auto f = [](void* p) {
+        if (p) (void)hipFree(p);
+    };
+    f(s->ix0);
+    f(s->iH[0]);
+    f(s->iH[1]);
+    f(s->iY);
+    f(s->iring);
+    f(s->fallback);
+    s->icap = 0;
+    KTS_TRY(dalloc(&s->ix0, (size_t)n * 32));
+    KTS_TRY(dalloc(&s->iH[0], (size_t)n * 256));
+    KTS_TRY(dalloc(&s->iH[1], (size_t)n * 256));
+    KTS_TRY(dalloc(&s->iY, (size_t)n * 8));
+    KTS_TRY(dalloc(&s->iring, (size_t)n));
+    KTS_TRY(dalloc(&s->fallback, (size_t)n));
+    s->icap = n;
+    return hipSuccess;
+}
+
+hipError_t trace_with_surrogate(Surrogate* s, const kti::TraceLaunch& L, double gate_prob, double gate_ring,
+                                hipStream_t st, hipEvent_t* ev) {
+    if (!s->have_params) return hipErrorNotReady;
+    const long long n = L.total;
+    KTS_TRY(ensure_infer_buffers(s, n));
+    Table t{s->tab_rho, s->tab_rsh};
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, s->nfb, 4);
+    hipLaunchKernelGGL(k_features_views, dim3(blocks), dim3(256), 0, st, L.views, L.n_views, n, t, s->ix0, s->iring);
+    KTS_TRY(hipGetLastError());
+    // the network, layer by layer (ping-pong activations)
+    for (int l = 0; l < 4; ++l) {
+        ktn::GemmArgs g{};
+        g.M = (int)n;
+        g.N = 256;
+        g.K = ktn::kKpad[l];
+        g.A = l == 0 ? s->ix0 : s->iH[(l - 1) & 1];
+        g.lda = ktn::kKpad[l];
+        g.B = s->wt[l];
+        g.ldb = ktn::kKpad[l];
+        g.ldc = 256;
+        g.bias = s->P + ktn::param_offset(l) + (long long)ktn::kKreal[l] * ktn::kN[l];
+        g.outb = s->iH[l & 1];
+        g.k_split = g.K;
+        KTS_TRY(ktn::gemm(g, false, true, ktn::kEpiFwd, 1, st));
+    }
+    {
+        ktn::GemmArgs g{};
+        g.M = (int)n;
+        g.N = 8;
+        g.K = 256;
+        g.A = s->iH[1];
+        g.lda = 256;
+        g.B = s->wt[4];
+        g.ldb = 256;
+        g.ldc = 8;
+        g.bias = s->P + ktn::param_offset(4) + 256LL * 8;
+        g.out32 = s->iY;
+        g.k_split = 256;
+        KTS_TRY(ktn::gemm(g, false, true, ktn::kEpiLin, 1, st));
+    }
+    hipLaunchKernelGGL(k_gate, dim3(blocks), dim3(256), 0, st, L.views, L.n_views, n, s->iY, s->iring,
+                       (float)gate_prob, (float)gate_ring, L.out, s->fallback, s->nfb);
+    hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(64), 0, st, s->nfb, s->nfb_ll, (unsigned long long*)nullptr);
+    KTS_TRY(hipGetLastError());
+    // the integrator on the queue the gate left (count read on the device)
+    kti::TraceLaunch F = L;
+    F.ray_list = s->fallback;
+    F.total_dev = s->nfb_ll;
+    KTS_TRY(hipEventRecord(ev[1], st));
+    KTS_TRY(kti::launch_trace(F, st));
+    KTS_TRY(hipEventRecord(ev[2], st));
+    // the fallback count into the context's counters (after launch_trace zeroed them)
+    hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(64), 0, st, s->nfb, s->nfb_ll, L.stats + kti::kStatFallback);
+    return hipGetLastError();
 }
 
 }  // namespace kts

@@ -97,6 +97,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
     const int lane = (int)(threadIdx.x & 63u);
     const int K = L.steps_per_dequeue;
 
+    // a device-side count (the surrogate's fallback queue) overrides the launch size
+    const long long total = L.total_dev ? *L.total_dev : L.total;
     typename Pol::LaneT s;
     bool active = false;
     bool drained = false;
@@ -110,11 +112,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
                 unsigned long long base = 0;
                 if (lane == 0) base = atomicAdd(L.counter, (unsigned long long)nidle);
                 base = __shfl(base, 0);
-                if (base + (unsigned long long)nidle >= (unsigned long long)L.total) drained = true;
+                if (base + (unsigned long long)nidle >= (unsigned long long)total) drained = true;
                 if (!active) {
                     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
                     const long long q = (long long)(base + (unsigned long long)__popcll(idle & below));
-                    if (q < L.total) {
+                    if (q < total) {
                         my = L.ray_list ? L.ray_list[q] : q;
                         Launch la;
                         launch_of(views, L.n_views, my, la);
@@ -149,7 +151,7 @@ __global__ __launch_bounds__(256) void k_trace_oneshot(kti::TraceLaunch L) {
     __shared__ ViewDev sv[kti::kMaxLdsViews];
     const ViewDev* views = stage_views(L, sv);
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= L.total) return;
+    if (q >= (L.total_dev ? *L.total_dev : L.total)) return;
     const long long my = L.ray_list ? L.ray_list[q] : q;
     typename Pol::LaneT s;
     Launch la;

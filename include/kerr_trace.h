@@ -169,6 +169,53 @@ int kt_plan_create(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
 int kt_plan_launch(kt_plan* plan, void* hip_stream);
 int kt_plan_destroy(kt_plan* plan);
 
+/* ---- the learned surrogate (no reference counterpart) ----------------------------------
+ * An MLP 5-256-256-256-256-8 (SiLU) with BF16 weights on the matrix cores, FP32
+ * accumulation and FP32 master weights.  It answers a ray when its largest class
+ * probability exceeds opts.gate_prob and the ray lies outside opts.gate_ring of the
+ * critical curve; every other ray goes to the integrator (kt_trace_views with
+ * use_surrogate = 1).  Parameters are FP32, layer by layer: W_l (n_in x n_out,
+ * row-major) then b_l. */
+typedef struct kt_surrogate_opts {
+    long long n_train, n_holdout;  /* 2^24, 2^20                                        */
+    int epochs, batch;             /* 64, 65536                                         */
+    double lr;                     /* 1e-3, cosine to 0                                 */
+    double w_sky, w_disk;          /* loss weights of the sky and disk regressions      */
+    double band, band_factor;      /* rays within `band` of the critical curve drawn    */
+                                   /* `band_factor` x more often (0.02, 4)              */
+    unsigned long long seed;
+    int label_precision;           /* KT_PREC_* labelling the training set (mixed)      */
+    double disk_outer, escape_radius, capture_factor;  /* the scene it learns (20, 1000, 1.01) */
+} kt_surrogate_opts;
+
+typedef struct kt_surrogate_report {
+    double datagen_seconds, train_seconds, eval_seconds;
+    double first_loss, final_loss, train_gemm_tflops;
+    long long steps;
+    double class_agreement, gated_fraction, gated_class_agreement;
+    double defl_median, defl_p99, defl_p999;        /* rad, gated escaped held-out rays */
+    double g_rel_median, g_rel_p99, rhit_rel_median, rhit_rel_p99;
+} kt_surrogate_report;
+
+void kt_surrogate_opts_default(kt_surrogate_opts* opts);
+long long kt_surrogate_param_count(void);
+int kt_surrogate_init(kt_ctx* ctx, unsigned long long seed);
+int kt_surrogate_load(kt_ctx* ctx, const float* params, long long n);
+int kt_surrogate_save(kt_ctx* ctx, float* params, long long n);
+/* generates the data with the integrator, trains on the GPU, judges the held-out set */
+int kt_surrogate_train(kt_ctx* ctx, const kt_surrogate_opts* opts, kt_surrogate_report* report);
+/* network outputs for normalised inputs (device pointers, n x 5 -> n x 8 floats) */
+int kt_surrogate_predict(kt_ctx* ctx, const float* inputs, long long n, float* outputs, void* hip_stream);
+/* gradient of the mean loss over n rays (device inputs: n x 5 floats, n classes, n x 5
+ * targets) into grads (device, param_count floats); the loss to *loss (host) */
+int kt_surrogate_gradients(kt_ctx* ctx, const float* inputs, const int8_t* cls, const float* targets, int n,
+                           float* grads, double* loss);
+/* one Adam step (beta 0.9, 0.999, eps 1e-8) with device gradients, step t >= 1 */
+int kt_surrogate_adam_step(kt_ctx* ctx, const float* grads, double lr, int t);
+/* the normalisation's shadow table: critical radius along psi and mean shadow radius */
+double kt_surrogate_table_radius(kt_ctx* ctx, double a, double inclination, double psi);
+double kt_surrogate_shadow_radius(kt_ctx* ctx, double a, double inclination);
+
 /* Host-side closed forms (src/kerr.cpp:157-206, 296-299). */
 double kt_horizon_radius(double a);
 double kt_isco_radius(double a, int prograde);

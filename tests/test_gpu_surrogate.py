@@ -1,0 +1,143 @@
+"""The learned surrogate's GPU kernels against the FP64 numpy network (oracle/mlp_oracle.py):
+forward outputs, loss, gradients and one Adam step; then a short training run on
+integrator-made data and the gated trace with its integrator fallback."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import black_simulation_amd as bsa
+from black_simulation_amd.configs import c2_view
+from oracle import mlp_oracle as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(x):
+    """round to bfloat16 (nearest even), as the kernels store weights and activations"""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def flat(params):
+    return np.concatenate([np.concatenate([W.ravel(), b]) for W, b in params]).astype(np.float32)
+
+
+def unflat(p):
+    out, o = [], 0
+    for n_in, n_out in zip(M.SIZES[:-1], M.SIZES[1:]):
+        W = p[o:o + n_in * n_out].reshape(n_in, n_out).astype(np.float64)
+        o += n_in * n_out
+        b = p[o:o + n_out].astype(np.float64)
+        o += n_out
+        out.append((W, b))
+    return out
+
+
+def emulated_forward(params, x):
+    """the kernels' arithmetic: bf16 weights and layer inputs, wide accumulation"""
+    Wq = [(bf16(W), b) for W, b in params]
+    h = bf16(x)
+    for l, (W, b) in enumerate(Wq):
+        z = h @ W + b
+        h = bf16(M.silu(z)) if l < len(Wq) - 1 else z
+    return h, Wq
+
+
+def problem(seed=3, n=1024):
+    rng = np.random.default_rng(seed)
+    params = M.init_params(seed)
+    params = [(W, rng.normal(0, 0.05, b.shape)) for W, b in params]
+    x = rng.uniform(-1, 1, (n, 5))
+    cls = rng.integers(0, 3, n).astype(np.int8)
+    tgt = rng.uniform(-1, 1, (n, 5))
+    return params, x, cls, tgt
+
+
+def test_forward_matches_numpy(tracer):
+    params, x, _, _ = problem()
+    tracer.surrogate_load(flat(params))
+    y = tracer.surrogate_predict(x).cpu().numpy()
+    want, _ = emulated_forward(unflat(flat(params)), x)
+    scale = np.abs(want).max()
+    err = np.abs(y - want)
+    print(f"forward: max err {err.max() / scale:.2e}, median {np.median(err) / scale:.2e} (relative to {scale:.2f})")
+    assert err.max() < 2e-2 * scale and np.median(err) < 1e-3 * scale
+
+
+def test_gradients_match_numpy(tracer):
+    params, x, cls, tgt = problem(n=2048)
+    p32 = flat(params)
+    tracer.surrogate_load(p32)
+    g, loss = tracer.surrogate_gradients(x, cls, tgt)
+    g = g.cpu().numpy()
+    P = unflat(p32)
+    Wq = [(bf16(W), b) for W, b in P]
+    # FP64 backward through the same (bf16-weight) network, activations rounded as on the GPU
+    h = bf16(x)
+    cache = []
+    for l, (W, b) in enumerate(Wq):
+        z = h @ W + b
+        cache.append((h, z))
+        h = bf16(M.silu(z)) if l < len(Wq) - 1 else z
+    want_loss, gy = M.loss_and_grad_out(h, cls.astype(int), tgt)
+    grads = M.backward(Wq, cache, gy)
+    assert loss == pytest.approx(want_loss, rel=2e-3)
+    o = 0
+    for l, (gW, gb) in enumerate(grads):
+        for ref in (gW.ravel(), gb):
+            got = g[o:o + ref.size]
+            o += ref.size
+            rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            cos = np.dot(got, ref) / (np.linalg.norm(got) * np.linalg.norm(ref))
+            print(f"layer {l} {'W' if ref.ndim else 'b'} size {ref.size}: rel err {rel:.2e} cos {cos:.6f}")
+            assert rel < 3e-2 and cos > 0.999
+
+
+def test_adam_step_matches_numpy(tracer):
+    params, _, _, _ = problem()
+    p32 = flat(params)
+    tracer.surrogate_load(p32)
+    rng = np.random.default_rng(9)
+    gr = rng.normal(0, 1e-3, p32.size).astype(np.float32)
+    tracer.surrogate_adam_step(torch.as_tensor(gr).cuda(), 1e-3, 1)
+    got = tracer.surrogate_save()
+    st = M.adam_state(unflat(p32))
+    want = flat(M.adam_step(unflat(p32), [(gW, gb) for gW, gb in unflat(gr)], st, 1e-3))
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-7)
+
+
+@pytest.fixture(scope="module")
+def trained(tracer):
+    tracer.surrogate_init(7)
+    rep = tracer.surrogate_train(n_train=1 << 17, n_holdout=1 << 14, epochs=6, batch=8192, seed=11)
+    print("short training:", {k: round(v, 6) if isinstance(v, float) else v for k, v in rep.items()}, flush=True)
+    return rep
+
+
+def test_short_training_learns(trained):
+    assert trained["final_loss"] < 0.7 * trained["first_loss"]
+    assert trained["class_agreement"] > 0.85
+
+
+def test_gated_trace_falls_back_to_the_integrator(tracer, trained):
+    """Rays the gate rejects are traced by the FP64 integrator (bit-equal to the oracle);
+    rays it accepts carry the surrogate flag and zero steps."""
+    v = c2_view(96, 54)
+    got = bsa.to_numpy(tracer.trace_views([v], bsa.TraceOpts(use_surrogate=1)))
+    st = tracer.stats()
+    ov = O.make_view(**{k: getattr(v, k) for k in (
+        "camera_kind", "width", "height", "spin", "r_cam", "theta_cam", "phi_cam", "half_width", "half_height",
+        "yaw", "pitch", "roll", "fov_deg", "disk_inner", "disk_outer", "disk_prograde", "disk_enabled",
+        "escape_radius", "capture_factor")})
+    want = O.render(ov, O.make_opts())
+    sur = (got["cls"] & bsa.FLAG_SURROGATE) != 0
+    fb = ~sur
+    print(f"surrogate answered {sur.mean():.3f} of rays; fallback {st['fallback_rays']}", flush=True)
+    assert st["fallback_rays"] == fb.sum()
+    assert np.array_equal(got["cls"][fb] & 7, want["cls"][fb])
+    assert np.array_equal(got["steps"][fb], want["steps"][fb])
+    assert np.all(got["steps"][sur] == 0)
