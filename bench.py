@@ -1,11 +1,19 @@
-"""Headline benchmark: Mrays/s tracing 1920x1080 frames of Kerr a=0.998.
+"""Headline benchmark: Mrays/s rendering 1920x1080 frames of Kerr a=0.998, surrogate +
+integrator fallback against the plain FP64 integrator on the same frames.
 
-One step = one full 1080p frame (2,073,600 rays, camera r=50 M, i=85 deg, 30 M-wide
-Bardeen plane, disk r_ISCO..40 M) traced by the FP64 persistent integrator (and, with
---path surrogate, by the surrogate + integrator fallback).  Inputs are the view
-constants; outputs stay in HBM.  Prints one JSON line.
+Workload (default): the 120-frame camera orbit at r = 40 M (phi_k = 2 pi k / 120,
+inclination 20 -> 85 deg), Bardeen plane 24 M wide, disk r_ISCO..20 M, escape 1000 M,
+capture 1.01 r+.  One step = one full frame (2,073,600 rays), replayed from a hipGraph
+plan.  --path surrogate: the MLP answers rays with max class probability > 0.999 lying
+outside +-3% of the critical curve, the mixed FP32/FP64 integrator the rest, all in
+one graph; --path plain: the FP64 integrator; --path mixed: the FP32/FP64 integrator.
+Both the chosen path and the plain FP64 integrator are timed on the same frames, and
+the chosen path's outputs are judged against the FP64 render of every frame.
+--workload c2 times the single a=0.998 frame at r = 50 M, i = 85 deg (disk to 40 M)
+instead; the surrogate does not cover that scene.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--path plain|mixed|surrogate] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--path surrogate|plain|mixed]
+                    [--workload orbit|c2] [--height 1080|2160] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -18,6 +26,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "Mrays/s 1080p a=0.998: surrogate+fallback vs plain integrator @≤1e-4 rad"
+
 # Floating-point operations of one DP5 attempt of the integrator (kt_device.hpp), each
 # +, -, *, /, sqrt counted once and an FMA as two:
 #   6 right-hand sides x 25 (one division each)                  = 150
@@ -26,6 +36,8 @@ sys.path.insert(0, ROOT)
 #   controller (x^-1/5 by three Newton steps, safety, h update)   =  30
 #   constraint projection (R, M, two zone tests, two sqrt)         =  31
 OPS_PER_ATTEMPT = 522
+# The network 5-256-256-256-256-8: 2 (5*256 + 3*256*256 + 256*8) multiply-adds per ray
+MLP_FLOP_PER_RAY = 2 * (5 * 256 + 3 * 256 * 256 + 256 * 8)  # 399,872
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 BF16_MFMA_PEAK_TFLOPS = 2500.0
@@ -33,7 +45,7 @@ BF16_MFMA_PEAK_TFLOPS = 2500.0
 
 def cpu_baseline(view, seconds_target: float = 15.0) -> dict:
     """The FP64 oracle (same DP5, rtol 1e-10) on this host's cores over a sparse sample
-    of the same frame: every `stride`-th pixel, the stride sized for ~10-30 s."""
+    of one frame of the workload: every `stride`-th pixel, the stride sized for ~15 s."""
     from oracle import oracle as O
 
     cores = os.cpu_count() or 1
@@ -42,7 +54,6 @@ def cpu_baseline(view, seconds_target: float = 15.0) -> dict:
         "yaw", "pitch", "roll", "fov_deg", "disk_inner", "disk_outer", "disk_prograde", "disk_enabled",
         "escape_radius", "capture_factor")})
     opts = O.make_opts()
-    # calibrate on a tiny sample first
     stride = 4099
     t0 = time.perf_counter()
     out = O.render(ov, opts, threads=cores, stride=stride)
@@ -54,91 +65,205 @@ def cpu_baseline(view, seconds_target: float = 15.0) -> dict:
     out = O.render(ov, opts, threads=cores, stride=stride)
     dt = time.perf_counter() - t0
     n = len(out["cls"])
-    return {"value": n / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
-            "sample": f"every {stride}th pixel of the same 1920x1080 a=0.998 frame ({n} rays, {dt:.1f} s, "
-                      f"FP64 DP5 rtol 1e-10, {cores} threads)"}
+    return {"value": round(n / dt / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"every {stride}th pixel of one {view.width}x{view.height} frame of the workload ({n} rays, "
+                      f"{dt:.1f} s, FP64 DP5 rtol 1e-10, {cores} threads)"}
+
+
+def percentile(xs, q):
+    s = sorted(xs)
+    return s[min(len(s) - 1, int(q * (len(s) - 1) + 0.5))]
+
+
+def timed_frames(orbit, warm, frames, torch):
+    """Replays each frame once, frames alternating over the orbit's streams; returns
+    (wall seconds, per-frame graph ms from each plan's HIP events)."""
+    for k in warm:
+        orbit.render(k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in frames:
+        orbit.render(k)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, [orbit.stats(k)["kernel_ms"] for k in frames]
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--path", choices=["plain", "mixed", "surrogate"], default="plain")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--steps", type=int, default=120)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--path", choices=["surrogate", "plain", "mixed"], default="surrogate")
+    ap.add_argument("--workload", choices=["orbit", "c2"], default="orbit")
     ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--weights", default=None)
+    ap.add_argument("--streams", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: rays are independent and this build runs on a single MI355X; "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.path == "surrogate" and args.workload != "orbit":
+        print("the surrogate is trained for the orbit scene (disk to 20 M); use --workload orbit", file=sys.stderr)
+        sys.exit(2)
 
     import torch
 
     import black_simulation_amd as bsa
-    from black_simulation_amd.configs import c2_view
+    from black_simulation_amd import orbit as O
+    from black_simulation_amd.configs import C5_FRAMES, c2_view, c5_view
+    from black_simulation_amd.train_surrogate import DEFAULT_WEIGHTS, load_params
 
-    view = c2_view(args.width, args.height)
-    prec = {"plain": bsa.PREC_FP64, "mixed": bsa.PREC_MIXED, "surrogate": bsa.PREC_FP64}[args.path]
-    opts = bsa.TraceOpts(precision=prec, use_surrogate=1 if args.path == "surrogate" else 0)
+    width = args.height * 16 // 9
+    if args.workload == "orbit":
+        views = [c5_view(k, width, args.height) for k in range(C5_FRAMES)]
+        frames = []
+        for i in range(args.steps):  # evenly spaced, consecutive frames on different streams
+            k = (i * C5_FRAMES) // args.steps % C5_FRAMES
+            if frames and k % args.streams == frames[-1] % args.streams:
+                k = (k + 1) % C5_FRAMES
+            frames.append(k)
+        warm = [(C5_FRAMES // 2 + i) % C5_FRAMES for i in range(args.warmup)]
+    else:  # one view, replayed; a copy per stream so that replays can overlap
+        views = [c2_view(width, args.height)] * args.streams
+        frames = [i % args.streams for i in range(args.steps)]
+        warm = [i % args.streams for i in range(args.warmup)]
+    rays = views[0].rays
+
     tr = bsa.Tracer(0)
-    out = tr.alloc_outputs(view.rays)
-    # launch shape from the autotuner (untimed), as a user would run it once per device
-    opts = tr.autotune([view], opts)
-    stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
-        tr.trace_views([view], opts, out=out, stream=stream)
-    torch.cuda.synchronize()
-    integ_ms, kern_ms, steps = [], [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        tr.trace_views([view], opts, out=out, stream=stream)
-        st = tr.stats()  # waits on the HIP events of this step's kernels
-        integ_ms.append(st["integrator_ms"])
-        kern_ms.append(st["kernel_ms"])
-        steps.append(st["steps"])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = view.rays * args.steps / elapsed / 1e6
+    plain = bsa.TraceOpts(precision=bsa.PREC_FP64)
+    plain = tr.autotune([views[frames[len(frames) // 2]]], plain)
+    if args.path == "plain":
+        opts = plain
+    else:
+        opts = tr.autotune([views[frames[len(frames) // 2]]], bsa.TraceOpts(precision=bsa.PREC_MIXED))
+        if args.path == "surrogate":
+            tr.surrogate_load(load_params(args.weights or DEFAULT_WEIGHTS))
+            opts.use_surrogate = 1
 
-    avg_integ = sum(integ_ms) / len(integ_ms)
-    attempts = sum(steps) / len(steps)
-    achieved = attempts * OPS_PER_ATTEMPT / (avg_integ * 1e-3) / 1e12
-    peak = FP64_VECTOR_PEAK_TFLOPS if prec == bsa.PREC_FP64 else FP32_VECTOR_PEAK_TFLOPS
-    cls = out["cls"].cpu()
-    counts = torch.bincount((cls & 7).to(torch.int64), minlength=4).tolist()
+    # the chosen path, timed
+    orb = O.Orbit(tr, views, opts, streams=args.streams)
+    wall, lat = timed_frames(orb, warm, frames, torch)
+    wall_frames = len(frames)
+    value = rays * wall_frames / wall / 1e6
+    # the plain FP64 integrator on the same frames, timed the same way
+    ref = O.Orbit(tr, views, plain, streams=args.streams) if args.path != "plain" else orb
+    if args.path != "plain":
+        wall_p, lat_p = timed_frames(ref, warm, frames, torch)
+    else:
+        wall_p, lat_p = wall, lat
+    value_p = rays * wall_frames / wall_p / 1e6
+
+    # per-kernel attribution (untimed pass without graphs: events around each kernel group)
+    integ_ms, sur_ms, attempts, fp64_att, fallback = [], [], [], [], []
+    for k in frames[:: max(1, len(frames) // 24)]:
+        tr.trace_views([views[k]], opts, out=orb.out)
+        st = tr.stats()
+        integ_ms.append(st["integrator_ms"])
+        sur_ms.append(st["surrogate_ms"])
+        attempts.append(st["steps"])
+        fp64_att.append(st["fp64_steps"] if opts.precision != bsa.PREC_FP64 else st["steps"])
+        fallback.append(st["fallback_rays"])
+    n_attr = len(integ_ms)
+
+    # accuracy of the chosen path against the FP64 render of every timed frame
+    acc = None
+    if args.path != "plain":
+        agree, n_rays, answered = 0, 0, 0
+        defl, defl_a, g_rel, g_rel_a, r_rel, r_rel_a = [], [], [], [], [], []
+        for k in sorted(set(frames)):
+            mine = orb.render(k)
+            theirs = ref.render(k)
+            orb.synchronize()
+            ref.synchronize()
+            e = O.errors(mine, theirs)
+            agree += int(e["agree"].sum())
+            answered += int(e["answered"].sum())
+            n_rays += rays
+            defl.append(e["defl"])
+            defl_a.append(e["defl_answered"])
+            g_rel.append(e["g_rel"])
+            g_rel_a.append(e["g_rel_answered"])
+            r_rel.append(e["r_rel"])
+            r_rel_a.append(e["r_rel_answered"])
+        q = lambda xs: [float("%.3g" % v) for v in O.quantiles(torch.cat(xs))]  # noqa: E731
+        acc = {"frames": len(set(frames)), "class_agreement": round(agree / n_rays, 6),
+               "answered_fraction": round(answered / n_rays, 4),
+               "deflection_rad_p50_p99_p999": q(defl), "deflection_rad_answered_p50_p99_p999": q(defl_a),
+               "g_rel_p50_p99_p999": q(g_rel), "g_rel_answered_p50_p99_p999": q(g_rel_a),
+               "r_hit_rel_p50_p99_p999": q(r_rel), "r_hit_rel_answered_p50_p99_p999": q(r_rel_a)}
+        d50, d999 = acc["deflection_rad_p50_p99_p999"][0], acc["deflection_rad_p50_p99_p999"][2]
+        acc["target_met"] = bool(value >= 20 * value_p and d50 <= 1e-4 and d999 <= 1e-3
+                                 and acc["class_agreement"] >= 0.999)
+
+    # roofline of the dominant kernel group (HIP-event times of the attribution pass)
+    avg_integ = sum(integ_ms) / n_attr
+    avg_sur = sum(sur_ms) / n_attr
+    att = sum(attempts) / n_attr
+    att64 = min(att, sum(fp64_att) / n_attr)
+    ops = att * OPS_PER_ATTEMPT
+    ideal_s = (att64 * OPS_PER_ATTEMPT / FP64_VECTOR_PEAK_TFLOPS + (att - att64) * OPS_PER_ATTEMPT
+               / FP32_VECTOR_PEAK_TFLOPS) / 1e12
+    integ_peak = ops / ideal_s / 1e12 if ideal_s > 0 else FP64_VECTOR_PEAK_TFLOPS
+    integ_ach = ops / (avg_integ * 1e-3) / 1e12 if avg_integ > 0 else 0.0
+    kname = ["k_trace_persistent", "k_trace_oneshot"][st["mode"]]
+    roof_integ = {"bound": "valu", "achieved": round(integ_ach, 3), "peak": round(integ_peak, 1), "unit": "TFLOP/s",
+                  "frac": round(integ_ach / integ_peak, 4), "traffic": None, "kernel": kname,
+                  "kernel_ms": round(avg_integ, 3), "dp5_attempts": int(att), "fp64_attempts": int(att64),
+                  "ops_per_attempt": OPS_PER_ATTEMPT,
+                  "peak_is": "FP64 78.6 / FP32 157.3 TFLOP/s blended by the FP64 share of attempts"}
+    roof = roof_integ
+    roof_mlp = None
+    if args.path == "surrogate":
+        mlp_ach = rays * MLP_FLOP_PER_RAY / (avg_sur * 1e-3) / 1e12
+        roof_mlp = {"bound": "mfma", "achieved": round(mlp_ach, 2), "peak": BF16_MFMA_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(mlp_ach / BF16_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+                    "kernel": "k_gemm (network) + features + gate", "kernel_ms": round(avg_sur, 3),
+                    "flop_per_ray": MLP_FLOP_PER_RAY}
+        if avg_sur > avg_integ:
+            roof, roof_mlp = roof_mlp, roof_integ
+
+    desc = {"orbit": f"120-frame orbit r=40 M, i 20->85 deg, a=0.998, {width}x{args.height} Bardeen plane 24 M "
+                     f"wide, disk r_ISCO..20 M",
+            "c2": f"single frame a=0.998, r=50 M, i=85 deg, {width}x{args.height} Bardeen plane 30 M wide, "
+                  f"disk r_ISCO..40 M"}[args.workload]
     rec = {
-        "metric": "Mrays/s 1080p a=0.998: surrogate+fallback vs plain integrator @<=1e-4 rad",
+        "metric": METRIC,
         "value": round(value, 3),
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(1e3 * wall / wall_frames, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == bsa.PREC_FP64 else "f32+f64",
-        "data": "synthetic: analytic Bardeen image plane, no input files",
-        "config": {"workload": f"{args.path} integrator, Kerr a=0.998, {view.width}x{view.height} Bardeen plane "
-                               f"30 M wide, camera r=50 M i=85 deg, disk r_ISCO..40 M, DP5 rtol 1e-10",
-                   "rays_per_step": view.rays, "path": args.path,
-                   "dp5_attempts_per_step": int(attempts), "class_counts": counts,
+        "dtype": {"plain": "f64", "mixed": "f32+f64", "surrogate": "bf16+f32+f64"}[args.path],
+        "data": "synthetic: analytic Bardeen image planes, no input files; surrogate weights trained by "
+                "black_simulation_amd.train_surrogate on integrator-labelled rays",
+        "config": {"workload": f"{args.path}: {desc}", "path": args.path, "rays_per_step": rays,
+                   "streams": args.streams,
+                   "frame_ms_p50": round(percentile(lat, 0.5), 3), "frame_ms_p99": round(percentile(lat, 0.99), 3),
+                   "plain_fp64": {"value": round(value_p, 3), "frame_ms_p50": round(percentile(lat_p, 0.5), 3),
+                                  "frame_ms_p99": round(percentile(lat_p, 0.99), 3)},
+                   "speedup_vs_plain_fp64": round(value / value_p, 3),
+                   "fallback_fraction": round(sum(fallback) / n_attr / rays, 4) if args.path == "surrogate" else None,
+                   "accuracy_vs_fp64": acc,
                    "launch": {"mode": ["persistent", "one-shot"][st["mode"]], "block": st["block"],
                               "grid": st["grid"], "waves_per_cu": st["waves_per_cu"],
                               "steps_per_dequeue": st["steps_per_dequeue"], "waves_per_simd": st["waves_per_simd"]}},
-        "roofline": {"bound": "valu_fp64" if prec == bsa.PREC_FP64 else "valu_fp32",
-                     "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
-                     "kernel": ["k_trace_persistent", "k_trace_oneshot"][st["mode"]], "kernel_ms": round(avg_integ, 3),
-                     "ops_per_attempt": OPS_PER_ATTEMPT},
+        "roofline": roof,
+        "roofline_other": roof_mlp,
         "cpu_baseline": None,
     }
+    orb.close()
+    if ref is not orb:
+        ref.close()
     tr.close()
     if not args.no_cpu:
-        rec["cpu_baseline"] = cpu_baseline(view)
+        rec["cpu_baseline"] = cpu_baseline(views[frames[len(frames) // 2]])
     print(json.dumps(rec), flush=True)
 
 

@@ -114,6 +114,7 @@ class KtSurrogateOpts(C.Structure):
         ("disk_outer", C.c_double),
         ("escape_radius", C.c_double),
         ("capture_factor", C.c_double),
+        ("verbose", C.c_int),
     ]
 
 
@@ -140,6 +141,7 @@ SIGNATURES = {
     "kt_autotune": (C.c_int, [C.c_void_p, _P(KtView), C.c_int, _P(KtTraceOpts)]),
     "kt_plan_create": (C.c_int, [C.c_void_p, _P(KtView), C.c_int, _P(KtTraceOpts), _P(KtRaysOut), _P(C.c_void_p)]),
     "kt_plan_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kt_plan_stats": (C.c_int, [C.c_void_p, _P(KtStats)]),
     "kt_plan_destroy": (C.c_int, [C.c_void_p]),
     "kt_surrogate_opts_default": (None, [_P(KtSurrogateOpts)]),
     "kt_surrogate_param_count": (C.c_longlong, []),

@@ -281,6 +281,12 @@ class Plan:
         _check(self.tracer._ctx, K.lib().kt_plan_launch(self._h, C.c_void_p(st.cuda_stream)), "kt_plan_launch")
         return self.out
 
+    def stats(self) -> dict:
+        """kt_plan_stats: counters and time of the last replay (waits for it)."""
+        s = K.KtStats()
+        _check(self.tracer._ctx, K.lib().kt_plan_stats(self._h, C.byref(s)), "kt_plan_stats")
+        return {f: getattr(s, f) for f, _ in K.KtStats._fields_}
+
     def close(self):
         if self._h:
             K.lib().kt_plan_destroy(self._h)

@@ -50,13 +50,15 @@ def c3_views(seed: int = C3_SEED, n: int = 64, sizes=(128, 256, 512, 1024, 2048)
 
 
 C5_FRAMES = 120
-C5_HALF_WIDTH = 12.0  # M across half the frame width (documented choice)
 
 
 def c5_view(k: int, width: int = 1920, height: int = 1080) -> View:
-    """Frame k of the orbit: phi_k = 2 pi k / 120, inclination 20 + 65 k / 119 deg."""
-    hw = C5_HALF_WIDTH
+    """Frame k of the orbit: phi_k = 2 pi k / 120, inclination 20 + 65 k / 119 deg.  The
+    half-width is 1.5x the frame's mean critical radius (the batch-view rule), so every
+    pixel lies inside the range the surrogate is trained on."""
+    inc = math.radians(20.0 + 65.0 * k / (C5_FRAMES - 1))
+    hw = 1.5 * shadow_radius(0.998, inc, samples=200, directions=72)
     return View(width=width, height=height, spin=0.998, r_cam=40.0,
-                theta_cam=math.radians(20.0 + 65.0 * k / (C5_FRAMES - 1)), phi_cam=2.0 * math.pi * k / C5_FRAMES,
+                theta_cam=inc, phi_cam=2.0 * math.pi * k / C5_FRAMES,
                 half_width=hw, half_height=hw * height / width, disk_inner=0.0, disk_outer=20.0,
                 escape_radius=1000.0, capture_factor=1.01)

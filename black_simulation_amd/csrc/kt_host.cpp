@@ -237,6 +237,10 @@ struct kt_plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     kti::TraceLaunch L{};
+    kts::Queue queue;           // the surrogate's fallback queue, owned by the plan
+    hipEvent_t ev[2] = {};      // around the last replay
+    bool timed = false;
+    int surrogate = 0;
 };
 
 namespace {
@@ -401,7 +405,7 @@ int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const k
     if (o.use_surrogate) {
         if (!ctx->surrogate) return fail(ctx, KT_EUNSUPPORTED, "use_surrogate set but no surrogate is loaded");
         KT_HIP(hipEventRecord(ctx->ev[0], stream));
-        hipError_t e = kts::trace_with_surrogate(ctx->surrogate, L, o.gate_prob, o.gate_ring, stream, ctx->ev);
+        hipError_t e = kts::trace_with_surrogate(ctx->surrogate, L, o.gate_prob, o.gate_ring, nullptr, stream, ctx->ev);
         if (e != hipSuccess) return hip_fail(ctx, e, "surrogate trace");
         KT_HIP(hipEventRecord(ctx->ev[3], stream));
     } else {
@@ -698,13 +702,15 @@ int kt_plan_create(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
         if (!out || !out->on_device) return fail(ctx, KT_EINVAL, "a plan needs device output pointers");
         kt_trace_opts o;
         fill_defaults(opts, o);
-        if (o.use_surrogate) return fail(ctx, KT_EUNSUPPORTED, "surrogate plans are created by kt_surrogate_plan");
+        if (o.use_surrogate && !kts::ready(ctx->surrogate))
+            return fail(ctx, KT_EUNSUPPORTED, "use_surrogate set but no surrogate is loaded");
         std::vector<ViewDev> vd;
         long long total = 0;
         if (int st = prepare_views(ctx, views, n_views, vd, total)) return st;
         KT_HIP(hipSetDevice(ctx->device));
         kt_plan* p = new kt_plan();
         p->ctx = ctx;
+        p->surrogate = o.use_surrogate;
         auto cleanup = [&](int code, const char* where, hipError_t e) {
             hip_fail(ctx, e, where);
             kt_plan_destroy(p);
@@ -717,13 +723,19 @@ int kt_plan_create(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
             return cleanup(KT_ENOMEM, "plan counters", e);
         if ((e = hipMemcpy(p->d_views, vd.data(), sizeof(ViewDev) * vd.size(), hipMemcpyHostToDevice)) != hipSuccess)
             return cleanup(KT_EHIP, "plan upload", e);
+        for (auto& ev : p->ev)
+            if ((e = hipEventCreate(&ev)) != hipSuccess) return cleanup(KT_EHIP, "plan events", e);
+        if (o.use_surrogate && (e = kts::queue_reserve(p->queue, total)) != hipSuccess)
+            return cleanup(KT_ENOMEM, "plan fallback queue", e);
         ktd::Outputs dout{out->cls, out->theta_inf, out->phi_inf, out->r_hit, out->g, out->steps};
         p->L = make_launch(ctx, p->d_views, (int)vd.size(), total, p->d_counter, dout, o);
         if ((e = hipStreamCreateWithFlags(&p->capture, hipStreamNonBlocking)) != hipSuccess)
             return cleanup(KT_EHIP, "plan stream", e);
         if ((e = hipStreamBeginCapture(p->capture, hipStreamCaptureModeThreadLocal)) != hipSuccess)
             return cleanup(KT_EHIP, "begin capture", e);
-        hipError_t le = kti::launch_trace(p->L, p->capture);
+        hipError_t le = o.use_surrogate ? kts::trace_with_surrogate(ctx->surrogate, p->L, o.gate_prob, o.gate_ring,
+                                                                    &p->queue, p->capture, nullptr)
+                                        : kti::launch_trace(p->L, p->capture);
         if ((e = hipStreamEndCapture(p->capture, &p->graph)) != hipSuccess) return cleanup(KT_EHIP, "end capture", e);
         if (le != hipSuccess) return cleanup(KT_EHIP, "captured launch", le);
         if ((e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0)) != hipSuccess)
@@ -740,11 +752,14 @@ int kt_plan_launch(kt_plan* p, void* hip_stream) {
     kt_ctx* ctx = p->ctx;
     hipStream_t stream = static_cast<hipStream_t>(hip_stream);
     KT_HIP(hipSetDevice(ctx->device));
+    KT_HIP(hipEventRecord(p->ev[0], stream));
     KT_HIP(hipEventRecord(ctx->ev[0], stream));
     KT_HIP(hipEventRecord(ctx->ev[1], stream));
     KT_HIP(hipGraphLaunch(p->exec, stream));
     KT_HIP(hipEventRecord(ctx->ev[2], stream));
     KT_HIP(hipEventRecord(ctx->ev[3], stream));
+    KT_HIP(hipEventRecord(p->ev[1], stream));
+    p->timed = true;
     ctx->stats = kt_stats{};
     ctx->stats.rays = p->L.total;
     ctx->stats.block = p->L.block;
@@ -766,9 +781,40 @@ int kt_plan_destroy(kt_plan* p) {
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->capture) (void)hipStreamDestroy(p->capture);
+    for (auto& ev : p->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    kts::queue_release(p->queue);
     if (p->d_views) (void)hipFree(p->d_views);
     if (p->d_counter) (void)hipFree(p->d_counter);
     delete p;
+    return KT_OK;
+}
+
+int kt_plan_stats(kt_plan* p, kt_stats* s) {
+    if (!p || !s) return KT_EINVAL;
+    kt_ctx* ctx = p->ctx;
+    KT_HIP(hipSetDevice(ctx->device));
+    kt_stats r{};
+    r.rays = p->L.total;
+    r.block = p->L.block;
+    r.grid = p->L.grid;
+    r.steps_per_dequeue = p->L.steps_per_dequeue;
+    r.waves_per_simd = p->L.waves_per_simd;
+    r.mode = p->L.mode;
+    r.waves_per_cu = (p->L.grid * p->L.block / 64) / std::max(1, ctx->cu_count);
+    if (p->timed) {
+        KT_HIP(hipEventSynchronize(p->ev[1]));
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p->ev[0], p->ev[1]) == hipSuccess) r.kernel_ms = ms;
+        if (!p->surrogate) r.integrator_ms = r.kernel_ms;
+        unsigned long long h[1 + kti::kStatCount];
+        KT_HIP(hipMemcpy(h, p->d_counter, sizeof h, hipMemcpyDeviceToHost));
+        r.steps = (long long)h[1 + kti::kStatSteps];
+        r.restep_rays = (long long)h[1 + kti::kStatRestepRays];
+        r.fallback_rays = (long long)h[1 + kti::kStatFallback];
+        r.fp64_steps = (long long)h[1 + kti::kStatFp64Steps];
+    }
+    *s = r;
     return KT_OK;
 }
 
@@ -797,6 +843,7 @@ void kt_surrogate_opts_default(kt_surrogate_opts* o) {
     o->disk_outer = d.disk_outer;
     o->escape_radius = d.escape_radius;
     o->capture_factor = d.capture_factor;
+    o->verbose = 0;
 }
 
 long long kt_surrogate_param_count(void) { return ktn::kParams; }
@@ -843,6 +890,7 @@ int kt_surrogate_train(kt_ctx* ctx, const kt_surrogate_opts* opts, kt_surrogate_
         o.disk_outer = opts->disk_outer > 0 ? opts->disk_outer : o.disk_outer;
         o.escape_radius = opts->escape_radius > 0 ? opts->escape_radius : o.escape_radius;
         o.capture_factor = opts->capture_factor >= 1.0 ? opts->capture_factor : o.capture_factor;
+        o.verbose = opts->verbose;
     }
     if (o.batch % 128 != 0) return fail(ctx, KT_EINVAL, "batch must be a multiple of 128");
     kts::TrainReport r;

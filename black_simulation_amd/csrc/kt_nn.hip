@@ -217,19 +217,80 @@ __global__ void k_refresh_bf16(const float* P, long long n, LayerTable lt) {
     }
 }
 
-// gather a batch: X0 (B x 32 bf16, 5 features then zeros), class and targets
+// gather a batch: inputs (B x 5 fp32), class and targets
 __global__ void k_gather(const float* feat, const int8_t* cls, const float* tgt, const long long* idx, int B,
-                         uint16_t* x0, int8_t* bcls, float* btgt) {
+                         float* xf, int8_t* bcls, float* btgt) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
     const long long s = idx ? idx[b] : b;
-    for (int k = 0; k < 32; ++k) x0[(long long)b * 32 + k] = k < 5 ? f2bf(feat[s * 5 + k]) : (uint16_t)0;
+    for (int k = 0; k < 5; ++k) xf[(long long)b * 5 + k] = feat[s * 5 + k];
     if (bcls) bcls[b] = cls[s];
     if (btgt)
         for (int k = 0; k < 5; ++k) btgt[(long long)b * 5 + k] = tgt[s * 5 + k];
 }
 
+// Input layer in FP32 on the vector units: z = x W0 + b0 with the master weights, then
+// SiLU rounded to bf16 for the matrix-core layers.  Thread j owns hidden unit j.
+constexpr int kL0Rows = 16;
+__global__ __launch_bounds__(256) void k_layer0(const float* xf, int B, const float* P, float* z0, uint16_t* h0) {
+    const int j = threadIdx.x;
+    float w[kInputs];
+#pragma unroll
+    for (int k = 0; k < kInputs; ++k) w[k] = P[k * kHidden + j];
+    const float bias = P[kInputs * kHidden + j];
+    const int r0 = blockIdx.x * kL0Rows;
+    for (int r = r0; r < r0 + kL0Rows && r < B; ++r) {
+        const float* x = xf + (long long)r * kInputs;
+        float z = bias;
+#pragma unroll
+        for (int k = 0; k < kInputs; ++k) z = fmaf(x[k], w[k], z);
+        if (z0) z0[(long long)r * kHidden + j] = z;
+        h0[(long long)r * kHidden + j] = f2bf(silu(z));
+    }
+}
+
+// Input-layer gradients in FP32: partial[c][k][j] = sum over chunk c of x[b][k] g[b][j]
+// (k < 5) and of g[b][j] (k = 5), then the sum over chunks into G (W0 then b0).
+__global__ __launch_bounds__(256) void k_layer0_grad_partial(const float* xf, const float* g, int B, int rows,
+                                                             float* partial) {
+    const int j = threadIdx.x;
+    const int b0 = blockIdx.x * rows;
+    const int b1 = b0 + rows < B ? b0 + rows : B;
+    float s[kInputs + 1] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int b = b0; b < b1; ++b) {
+        const float gj = g[(long long)b * kHidden + j];
+        const float* x = xf + (long long)b * kInputs;
+#pragma unroll
+        for (int k = 0; k < kInputs; ++k) s[k] = fmaf(x[k], gj, s[k]);
+        s[kInputs] += gj;
+    }
+    float* o = partial + (long long)blockIdx.x * (kInputs + 1) * kHidden;
+#pragma unroll
+    for (int k = 0; k <= kInputs; ++k) o[k * kHidden + j] = s[k];
+}
+
+__global__ void k_layer0_grad_final(const float* partial, int chunks, float* G) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;  // (k, j), k <= 5
+    if (e >= (kInputs + 1) * kHidden) return;
+    float s = 0.f;
+    for (int c = 0; c < chunks; ++c) s += partial[(long long)c * (kInputs + 1) * kHidden + e];
+    G[e] = s;  // W0 (5 x 256 row-major) is followed directly by b0
+}
+
 }  // namespace
+
+hipError_t layer0(const float* xf, int B, const float* P, float* z0, uint16_t* h0, hipStream_t s) {
+    hipLaunchKernelGGL(k_layer0, dim3((B + kL0Rows - 1) / kL0Rows), dim3(256), 0, s, xf, B, P, z0, h0);
+    return hipGetLastError();
+}
+
+hipError_t layer0_grad(const float* xf, const float* g, int B, float* partial, int chunks, float* G, hipStream_t s) {
+    const int rows = (B + chunks - 1) / chunks;
+    hipLaunchKernelGGL(k_layer0_grad_partial, dim3(chunks), dim3(256), 0, s, xf, g, B, rows, partial);
+    hipLaunchKernelGGL(k_layer0_grad_final, dim3(((kInputs + 1) * kHidden + 255) / 256), dim3(256), 0, s, partial,
+                       chunks, G);
+    return hipGetLastError();
+}
 
 hipError_t gemm(const GemmArgs& g, bool at, bool bt, int epi, int splits, hipStream_t stream) {
     const dim3 grid((unsigned)((g.M + 127) / 128), (unsigned)((g.N + 127) / 128), (unsigned)splits);
@@ -282,9 +343,9 @@ hipError_t refresh_bf16(const float* P, long long n, const LayerTable& lt, hipSt
     return hipGetLastError();
 }
 
-hipError_t gather(const float* feat, const int8_t* cls, const float* tgt, const long long* idx, int B, uint16_t* x0,
+hipError_t gather(const float* feat, const int8_t* cls, const float* tgt, const long long* idx, int B, float* xf,
                   int8_t* bcls, float* btgt, hipStream_t s) {
-    hipLaunchKernelGGL(k_gather, dim3((B + 255) / 256), dim3(256), 0, s, feat, cls, tgt, idx, B, x0, bcls, btgt);
+    hipLaunchKernelGGL(k_gather, dim3((B + 255) / 256), dim3(256), 0, s, feat, cls, tgt, idx, B, xf, bcls, btgt);
     return hipGetLastError();
 }
 

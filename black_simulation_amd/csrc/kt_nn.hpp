@@ -1,9 +1,11 @@
 // kt_nn.hpp -- interface of the surrogate network's kernels (kt_nn.hip).
 //
 // The network: 5 -> 256 -> 256 -> 256 -> 256 -> 8, SiLU after the hidden layers.  Layer l
-// maps x (rows) to x W_l + b_l, W_l stored K_l x N_l row-major in FP32 master copies;
-// the kernels read bf16 working copies W_l (K_pad x N_ld) and W_l^T (N x K_pad), where the
-// input layer's K is padded 5 -> 32 and the output layer's N 8 -> 32 with zeros.
+// maps x (rows) to x W_l + b_l, W_l stored K_l x N_l row-major in FP32 master copies.
+// The input layer runs in FP32 on the vector units (its K = 5 is too narrow for the
+// matrix cores and its inputs need more than bf16's 8 bits); layers 1-4 run on the
+// matrix cores from bf16 working copies W_l (K_pad x N_ld) and W_l^T (N x K_pad), the
+// output layer's N padded 8 -> 32 with zeros.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -62,7 +64,12 @@ hipError_t loss_head(const float* Y, const int8_t* cls, const float* tgt, int B,
 hipError_t adam(float* P, const float* G, float* M, float* V, long long n, float lr, float b1, float b2, float eps,
                 float c1, float c2, const LayerTable& lt, hipStream_t s);
 hipError_t refresh_bf16(const float* P, long long n, const LayerTable& lt, hipStream_t s);
-hipError_t gather(const float* feat, const int8_t* cls, const float* tgt, const long long* idx, int B, uint16_t* x0,
+hipError_t gather(const float* feat, const int8_t* cls, const float* tgt, const long long* idx, int B, float* xf,
                   int8_t* bcls, float* btgt, hipStream_t s);
+// input layer in FP32 from the master weights (P at layer 0): z0 (optional) and bf16 SiLU
+hipError_t layer0(const float* xf, int B, const float* P, float* z0, uint16_t* h0, hipStream_t s);
+// its gradients (W0, b0) in FP32 from the inputs and dL/dz0 (fp32 B x 256); partial holds
+// chunks x 6 x 256 floats
+hipError_t layer0_grad(const float* xf, const float* g, int B, float* partial, int chunks, float* G, hipStream_t s);
 
 }  // namespace ktn

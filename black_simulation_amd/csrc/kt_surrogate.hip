@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <random>
 #include <vector>
@@ -80,6 +81,62 @@ __host__ __device__ inline void features_of(const float* rho, const float* rsh, 
     *ring = (float)fabs(sqrt(alpha * alpha + beta * beta) / rc - 1.0);
 }
 
+// Frame of a ray's straight-line (M = 0) continuation: e1 the direction the backward-traced
+// ray would leave along (launch conventions: inward, -beta/r along e_theta, -alpha/r along
+// e_phi at the camera), e2 the unit vector from the line's closest point towards the hole
+// (the way gravity bends it), e3 = e1 x e2.  Cartesian, z along the spin axis.
+__host__ __device__ inline void ray_frame(double r, double inc, double phc, double al, double be, double e1[3],
+                                          double e2[3], double e3[3]) {
+    const double si = sin(inc), ci = cos(inc), sp = sin(phc), cp = cos(phc);
+    const double rh[3] = {si * cp, si * sp, ci}, th[3] = {ci * cp, ci * sp, -si}, ph[3] = {-sp, cp, 0.0};
+    double q = 1.0 - (al * al + be * be) / (r * r);
+    q = q > 0.0 ? sqrt(q) : 0.0;
+    double n = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        e1[k] = -q * rh[k] - (be / r) * th[k] - (al / r) * ph[k];
+        n += e1[k] * e1[k];
+    }
+    n = 1.0 / sqrt(n);
+    for (int k = 0; k < 3; ++k) e1[k] *= n;
+    // -(P - (P.e1) e1) with P = r rh
+    const double pe = r * (rh[0] * e1[0] + rh[1] * e1[1] + rh[2] * e1[2]);
+    double m = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        e2[k] = -(r * rh[k] - pe * e1[k]);
+        m += e2[k] * e2[k];
+    }
+    if (!(m > 1e-24)) {  // aimed at the hole: any perpendicular will do
+        const double te = th[0] * e1[0] + th[1] * e1[1] + th[2] * e1[2];
+        m = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            e2[k] = th[k] - te * e1[k];
+            m += e2[k] * e2[k];
+        }
+    }
+    m = 1.0 / sqrt(m);
+    for (int k = 0; k < 3; ++k) e2[k] *= m;
+    e3[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    e3[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    e3[2] = e1[0] * e2[1] - e1[1] * e2[0];
+}
+
+// sky angles from the network's components along (e1, e2, e3)
+__host__ __device__ inline void sky_from_frame(const double e1[3], const double e2[3], const double e3[3], double c1,
+                                               double c2, double c3, double& theta, double& phi) {
+    double N[3];
+    double n = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        N[k] = c1 * e1[k] + c2 * e2[k] + c3 * e3[k];
+        n += N[k] * N[k];
+    }
+    n = n > 0.0 ? 1.0 / sqrt(n) : 0.0;
+    double z = N[2] * n;
+    z = z > 1.0 ? 1.0 : (z < -1.0 ? -1.0 : z);
+    theta = acos(z);
+    phi = atan2(N[1], N[0]);
+    if (phi < 0.0) phi += 2.0 * kPi;
+}
+
 // ---- kernels -------------------------------------------------------------------------------
 
 __device__ __forceinline__ int find_view(const ViewDev* v, int n, long long idx) {
@@ -127,21 +184,9 @@ __device__ __forceinline__ bool ray_params(const ViewDev* views, int n_views, lo
     return true;
 }
 
-__global__ void k_features_views(const ViewDev* views, int n_views, long long n, Table t, uint16_t* x0, float* ring) {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    double a, r_cam, inc, alpha, beta, phi_cam, r_in;
-    float f[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, rg = -1.f;  // ring < 0: not coverable, always falls back
-    if (ray_params(views, n_views, q, a, r_cam, inc, alpha, beta, phi_cam, r_in))
-        features_of(t.rho, t.rsh, a, r_cam, inc, alpha, beta, f, &rg);
-    uint16_t* row = x0 + q * 32;
-    for (int k = 0; k < 32; ++k) row[k] = k < 5 ? f2bf(f[k]) : (uint16_t)0;
-    ring[q] = rg;
-}
-
 // features and targets of integrator-labelled rays (kind-2 params)
 __global__ void k_targets(const double* params, long long n, Table t, const int8_t* cls, const double* th,
-                          const double* ph, const double* rh, const double* gg, int prograde, float* feat,
+                          const double* ph, const double* rh, const double* gg, float* feat,
                           int8_t* ocls, float* tgt, float* ring) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
@@ -156,11 +201,15 @@ __global__ void k_targets(const double* params, long long n, Table t, const int8
     float* o = tgt + q * 5;
     for (int k = 0; k < 5; ++k) o[k] = 0.f;
     if (c == 0) {
-        o[0] = (float)cos(th[q]);
-        o[1] = (float)cos(ph[q]);
-        o[2] = (float)sin(ph[q]);
+        double e1[3], e2[3], e3[3];
+        ray_frame(p[1], p[2], 0.0, p[3], p[4], e1, e2, e3);
+        const double st = sin(th[q]);
+        const double N[3] = {st * cos(ph[q]), st * sin(ph[q]), cos(th[q])};
+        o[0] = (float)(N[0] * e1[0] + N[1] * e1[1] + N[2] * e1[2]);
+        o[1] = (float)(N[0] * e2[0] + N[1] * e2[1] + N[2] * e2[2]);
+        o[2] = (float)(N[0] * e3[0] + N[1] * e3[1] + N[2] * e3[2]);
     } else if (c == 2) {
-        o[3] = (float)(rh[q] / ktd::isco_dev(p[0], prograde));
+        o[3] = (float)log(rh[q]);
         o[4] = (float)log(gg[q]);
     }
 }
@@ -199,46 +248,6 @@ __global__ void k_batch_index(long long n, int bits, uint64_t key, long long sta
     idx[b] = (long long)x;
 }
 
-// decode + gate: answered rays get their outputs, the others join the fallback queue
-__global__ void k_gate(const ViewDev* views, int n_views, long long n, const float* Y, const float* ring,
-                       float gate_prob, float gate_ring, ktd::Outputs out, long long* fallback,
-                       unsigned long long* n_fallback) {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    const float* y = Y + q * 8;
-    const float m = fmaxf(y[0], fmaxf(y[1], y[2]));
-    const float e0 = __expf(y[0] - m), e1 = __expf(y[1] - m), e2 = __expf(y[2] - m);
-    const float inv = 1.0f / (e0 + e1 + e2);
-    const float p0 = e0 * inv, p1 = e1 * inv, p2 = e2 * inv;
-    const int c = p0 >= p1 && p0 >= p2 ? 0 : (p1 >= p2 ? 1 : 2);
-    const float pm = c == 0 ? p0 : c == 1 ? p1 : p2;
-    const float rg = ring[q];
-    if (!(pm > gate_prob) || !(rg > gate_ring)) {
-        const unsigned long long at = atomicAdd(n_fallback, 1ull);
-        fallback[at] = q;
-        return;
-    }
-    double a, r_cam, inc, alpha, beta, phi_cam, r_in;
-    ray_params(views, n_views, q, a, r_cam, inc, alpha, beta, phi_cam, r_in);
-    const double nan = __builtin_nan("");
-    if (out.cls) out.cls[q] = (int8_t)(c | ktd::kFlagSurrogate);
-    double th = nan, ph = nan, rh = nan, g = nan;
-    if (c == 0) {
-        double ct = y[3];
-        ct = ct > 1.0 ? 1.0 : (ct < -1.0 ? -1.0 : ct);
-        th = acos(ct);
-        ph = ktd::wrap_2pi(atan2((double)y[5], (double)y[4]) + phi_cam);
-    } else if (c == 2) {
-        rh = (double)y[6] * r_in;
-        g = exp((double)y[7]);
-    }
-    if (out.theta) out.theta[q] = th;
-    if (out.phi) out.phi[q] = ph;
-    if (out.r_hit) out.r_hit[q] = rh;
-    if (out.g) out.g[q] = g;
-    if (out.steps) out.steps[q] = 0;
-}
-
 __global__ void k_zero_u64(unsigned long long* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
@@ -250,10 +259,238 @@ __global__ void k_copy_count(const unsigned long long* c, long long* dst, unsign
     }
 }
 
-__global__ void k_feat_to_x0(const float* feat, long long n, uint16_t* x0) {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    for (int k = 0; k < 32; ++k) x0[q * 32 + k] = k < 5 ? f2bf(feat[q * 5 + k]) : (uint16_t)0;
+// ---- the fused network: one kernel from features to gated outputs -------------------------
+//
+// One wave owns 32 rays and keeps their activations in registers from layer to layer; the
+// weights stream through LDS in 16 KB slabs shared by the block's 4 waves.  Layers 1-4 use
+// v_mfma_f32_32x32x16_bf16 with the transposed product D[n][m] = sum_k W[k][n] X[m][k]:
+// the output feature n is the MFMA row, the ray m the column.  Lane l holds ray m = l % 32
+// and, of every 32-feature block fb, the 16 features 32 fb + 8 q + 4 h + r (h = l / 32,
+// q, r = 0..3) in accumulator element 4 q + r.  Read as the next layer's B operand, the
+// elements 8 j .. 8 j + 7 of block fb form k-block 2 fb + j, so k-slot s of lane half h in
+// k-block kb stands for feature
+//     F(kb, h, s) = 32 (kb / 2) + 8 (2 (kb % 2) + s / 4) + 4 h + s % 4,
+// and the packed weights hold W[F(kb, h, s)][n] in A-operand order (lane = 32 h + n % 32).
+// The accumulators therefore never leave the lane: SiLU and the bf16 rounding produce the
+// next layer's operand in place.
+constexpr int kFusedRays = 128;        // rays per block: 4 waves x 32
+constexpr int kSlabElems = 8192;       // bf16 per slab (16 KB): two k-blocks of all 8 feature blocks
+constexpr int kHiddenSlabs = 3 * 8;    // layers 1-3, eight slabs each; then one slab for layer 4
+constexpr int kPackedElems = (kHiddenSlabs + 1) * kSlabElems;
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__host__ __device__ inline int packed_feature(int kb, int h, int s) {
+    return 32 * (kb >> 1) + 8 * (2 * (kb & 1) + (s >> 2)) + 4 * h + (s & 3);
+}
+
+// packed weights from the FP32 masters: slab t < 24 is layer 1 + t / 8, k-blocks
+// 2 (t % 8) + j, laid out [j][fb][lane][8]; slab 24 is the output layer, [kb][lane][8]
+// with rows n >= 8 zero.
+__global__ void k_pack_fused(const float* P, uint16_t* wpk) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= kPackedElems) return;
+    const int t = e / kSlabElems, w = e % kSlabElems;
+    float v;
+    if (t < kHiddenSlabs) {
+        const int l = 1 + t / 8, sl = t % 8;
+        const int j = w / 4096, fb = (w / 512) % 8, lane = (w / 8) % 64, s = w % 8;
+        const int n = 32 * fb + (lane & 31);
+        const int f = packed_feature(2 * sl + j, lane >> 5, s);
+        v = P[ktn::param_offset(l) + (long long)f * ktn::kHidden + n];
+    } else {
+        const int kb = w / 512, lane = (w / 8) % 64, s = w % 8;
+        const int n = lane & 31;
+        const int f = packed_feature(kb, lane >> 5, s);
+        v = n < ktn::kOutputs ? P[ktn::param_offset(4) + (long long)f * ktn::kOutputs + n] : 0.f;
+    }
+    wpk[e] = f2bf(v);
+}
+
+struct FusedArgs {
+    long long n;
+    // inputs: either views (features computed per ray) or a feature array
+    const ViewDev* views;
+    int n_views;
+    Table t;
+    const float* xf;       // n x 5 when views == nullptr
+    const float* P;        // FP32 masters (input layer, biases)
+    const uint16_t* wpk;   // packed bf16 weights of layers 1-4
+    // outputs: raw network outputs (n x 8), or the gate's decisions
+    float* y;
+    float gate_prob, gate_ring;
+    ktd::Outputs out;
+    long long* fallback;
+    unsigned long long* n_fallback;
+};
+
+__device__ __forceinline__ float silu_fast(float z) {
+    return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+
+__device__ __forceinline__ f32x16 mfma32(const uint4& a, const bf16x8& b, const f32x16& c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a), b, c, 0, 0, 0);
+}
+
+template <bool VIEWS>
+// two waves per SIMD: 214 VGPRs, accumulators in the VGPR file, no spills
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mlp_fused(FusedArgs a) {
+    __shared__ uint4 slab[2][kSlabElems / 8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 31, h = lane >> 5;
+    const long long q = (long long)blockIdx.x * kFusedRays + wave * 32 + m;
+    const bool live = q < a.n;
+    const long long qc = live ? q : a.n - 1;
+
+    // stage slab 0 while the inputs are formed
+    const uint4* src = reinterpret_cast<const uint4*>(a.wpk);
+    uint4 pre[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) pre[i] = src[tid + 256 * i];
+
+    float f[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    float rg = -1.f;  // relative distance from the critical curve; < 0: not coverable
+    if (VIEWS) {
+        double ra, rc, inc, al, be, phc, rin;
+        if (ray_params(a.views, a.n_views, qc, ra, rc, inc, al, be, phc, rin))
+            features_of(a.t.rho, a.t.rsh, ra, rc, inc, al, be, f, &rg);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) f[k] = a.xf[qc * 5 + k];
+    }
+
+    // input layer in FP32: the 128 hidden units this lane holds, as layer 1's B operand
+    bf16x8 B[16];
+    {
+        const float* W0 = a.P;
+        const float* b0 = a.P + ktn::kInputs * ktn::kHidden;
+#pragma unroll
+        for (int fb = 0; fb < 8; ++fb)
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) {
+                const int n0 = 32 * fb + 8 * qq + 4 * h;
+                const float4 bb = *reinterpret_cast<const float4*>(b0 + n0);
+                float z[4] = {bb.x, bb.y, bb.z, bb.w};
+#pragma unroll
+                for (int k = 0; k < 5; ++k) {
+                    const float4 w = *reinterpret_cast<const float4*>(W0 + k * ktn::kHidden + n0);
+                    z[0] = fmaf(f[k], w.x, z[0]);
+                    z[1] = fmaf(f[k], w.y, z[1]);
+                    z[2] = fmaf(f[k], w.z, z[2]);
+                    z[3] = fmaf(f[k], w.w, z[3]);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) B[2 * fb + (qq >> 1)][4 * (qq & 1) + r] = (__bf16)silu_fast(z[r]);
+            }
+    }
+
+    f32x16 acc[8];
+    int buf = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) slab[0][tid + 256 * i] = pre[i];
+    __syncthreads();
+    for (int l = 1; l <= 3; ++l) {
+        {  // bias into the accumulators
+            const float* bl = a.P + ktn::param_offset(l) + (long long)ktn::kHidden * ktn::kHidden;
+#pragma unroll
+            for (int fb = 0; fb < 8; ++fb)
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq) {
+                    const float4 bb = *reinterpret_cast<const float4*>(bl + 32 * fb + 8 * qq + 4 * h);
+                    acc[fb][4 * qq + 0] = bb.x;
+                    acc[fb][4 * qq + 1] = bb.y;
+                    acc[fb][4 * qq + 2] = bb.z;
+                    acc[fb][4 * qq + 3] = bb.w;
+                }
+        }
+#pragma unroll
+        for (int sl = 0; sl < 8; ++sl) {
+            // next slab into registers (the output layer's after the last hidden one)
+            const uint4* nsrc = src + (long long)(8 * (l - 1) + sl + 1) * (kSlabElems / 8);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) pre[i] = nsrc[tid + 256 * i];
+            const uint4* S = slab[buf];
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int fb = 0; fb < 8; ++fb)
+                    acc[fb] = mfma32(S[(j * 8 + fb) * 64 + lane], B[2 * sl + j], acc[fb]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) slab[buf ^ 1][tid + 256 * i] = pre[i];
+            __syncthreads();
+            buf ^= 1;
+        }
+        // SiLU, rounded to bf16: the next layer's operand
+#pragma unroll
+        for (int fb = 0; fb < 8; ++fb)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) B[2 * fb + (i >> 3)][i & 7] = (__bf16)silu_fast(acc[fb][i]);
+    }
+
+    // output layer: four partial sums over the sixteen k-blocks
+    f32x16 o[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) o[p][i] = 0.f;
+    {
+        const uint4* S = slab[buf];
+#pragma unroll
+        for (int kb = 0; kb < 16; ++kb) o[kb & 3] = mfma32(S[kb * 64 + lane], B[kb], o[kb & 3]);
+    }
+    const float* b4 = a.P + ktn::param_offset(4) + (long long)ktn::kHidden * ktn::kOutputs;
+    float y4[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) y4[r] = o[0][r] + o[1][r] + o[2][r] + o[3][r] + b4[4 * h + r];
+    // lane m (h = 0) holds outputs 0..3 of ray m, lane m + 32 outputs 4..7
+    float y[8];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        y[r] = y4[r];
+        y[4 + r] = __shfl_down(y4[r], 32);
+    }
+    if (!VIEWS) {
+        if (live && h == 0)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) a.y[q * 8 + k] = y[k];
+        return;
+    }
+
+    // the gate, on the lower half-wave
+    const float mx = fmaxf(y[0], fmaxf(y[1], y[2]));
+    const float e0 = __expf(y[0] - mx), e1 = __expf(y[1] - mx), e2 = __expf(y[2] - mx);
+    const float inv = 1.0f / (e0 + e1 + e2);
+    const float p0 = e0 * inv, p1 = e1 * inv, p2 = e2 * inv;
+    const int c = p0 >= p1 && p0 >= p2 ? 0 : (p1 >= p2 ? 1 : 2);
+    const float pm = c == 0 ? p0 : c == 1 ? p1 : p2;
+    const bool to_fallback = live && h == 0 && (!(pm > a.gate_prob) || !(rg > a.gate_ring));
+    const unsigned long long mask = __ballot(to_fallback);
+    if (mask) {
+        const int first = __ffsll((long long)mask) - 1;
+        unsigned long long base = 0;
+        if (lane == first) base = atomicAdd(a.n_fallback, (unsigned long long)__popcll(mask));
+        base = __shfl(base, first);
+        if (to_fallback) a.fallback[base + __popcll(mask & ((1ull << lane) - 1))] = q;
+    }
+    if (!live || h != 0 || to_fallback) return;
+    double ra, rc, inc, al, be, phc, rin;
+    ray_params(a.views, a.n_views, q, ra, rc, inc, al, be, phc, rin);
+    const double nan = __builtin_nan("");
+    double th = nan, ph = nan, rh = nan, g = nan;
+    if (c == 0) {
+        double u1[3], u2[3], u3[3];
+        ray_frame(rc, inc, phc, al, be, u1, u2, u3);
+        sky_from_frame(u1, u2, u3, y[3], y[4], y[5], th, ph);
+    } else if (c == 2) {
+        rh = exp((double)y[6]);
+        g = exp((double)y[7]);
+    }
+    if (a.out.cls) a.out.cls[q] = (int8_t)(c | ktd::kFlagSurrogate);
+    if (a.out.theta) a.out.theta[q] = th;
+    if (a.out.phi) a.out.phi[q] = ph;
+    if (a.out.r_hit) a.out.r_hit[q] = rh;
+    if (a.out.g) a.out.g[q] = g;
+    if (a.out.steps) a.out.steps[q] = 0;
 }
 
 template <class T>
@@ -280,7 +517,7 @@ struct Surrogate {
     std::vector<float> h_rho, h_rsh;
     // work buffers, sized for `cap` rows
     long long cap = 0;
-    uint16_t* x0 = nullptr;
+    float* xf = nullptr;        // inputs, B x 5
     float* Z[4] = {};
     uint16_t* H[4] = {};
     float* Y = nullptr;
@@ -293,16 +530,10 @@ struct Surrogate {
     long long* bidx = nullptr;
     float* ws = nullptr;
     float* partial = nullptr;
+    float* partial0 = nullptr;  // input-layer gradient partials
     double* loss = nullptr;
-    // inference buffers
-    long long icap = 0;
-    uint16_t* ix0 = nullptr;
-    uint16_t* iH[2] = {};
-    float* iY = nullptr;
-    float* iring = nullptr;
-    long long* fallback = nullptr;
-    unsigned long long* nfb = nullptr;  // [0] fallback count
-    long long* nfb_ll = nullptr;
+    uint16_t* wpk = nullptr;  // layers 1-4 packed for the fused kernel (bf16)
+    Queue queue;              // fallback queue of eager calls
     static constexpr int kSplits = 64, kChunks = 256;
 };
 
@@ -320,7 +551,8 @@ static hipError_t free_all(Surrogate* s) {
     }
     f(s->tab_rho);
     f(s->tab_rsh);
-    f(s->x0);
+    f(s->xf);
+    f(s->partial0);
     for (int l = 0; l < 4; ++l) {
         f(s->Z[l]);
         f(s->H[l]);
@@ -331,7 +563,6 @@ static hipError_t free_all(Surrogate* s) {
     for (int k = 0; k < 2; ++k) {
         f(s->g32[k]);
         f(s->gb[k]);
-        f(s->iH[k]);
     }
     f(s->bcls);
     f(s->btgt);
@@ -339,12 +570,8 @@ static hipError_t free_all(Surrogate* s) {
     f(s->ws);
     f(s->partial);
     f(s->loss);
-    f(s->ix0);
-    f(s->iY);
-    f(s->iring);
-    f(s->fallback);
-    f(s->nfb);
-    f(s->nfb_ll);
+    f(s->wpk);
+    queue_release(s->queue);
     return hipSuccess;
 }
 
@@ -408,10 +635,9 @@ Surrogate* create(int device, hipError_t* err) {
         s->lt.wb[l] = s->wb[l];
         s->lt.wt[l] = s->wt[l];
     }
+    if ((e = dalloc(&s->wpk, kPackedElems)) != hipSuccess) return fail(e);
     if ((e = build_table(s)) != hipSuccess) return fail(e);
-    if ((e = dalloc(&s->nfb, 4)) != hipSuccess || (e = dalloc(&s->nfb_ll, 1)) != hipSuccess ||
-        (e = dalloc(&s->loss, 1)) != hipSuccess)
-        return fail(e);
+    if ((e = dalloc(&s->loss, 1)) != hipSuccess) return fail(e);
     if (err) *err = hipSuccess;
     return s;
 }
@@ -431,6 +657,11 @@ double table_critical_radius(const Surrogate* s, double a, double inc, double ps
 }
 double table_shadow_radius(const Surrogate* s, double a, double inc) { return table_rsh(s->h_rsh.data(), a, inc); }
 
+static hipError_t pack(Surrogate* s, hipStream_t st) {
+    hipLaunchKernelGGL(k_pack_fused, dim3((kPackedElems + 255) / 256), dim3(256), 0, st, s->P, s->wpk);
+    return hipGetLastError();
+}
+
 hipError_t set_params(Surrogate* s, const float* host, long long n) {
     if (n != ktn::kParams) return hipErrorInvalidValue;
     KTS_TRY(hipSetDevice(s->device));
@@ -438,6 +669,7 @@ hipError_t set_params(Surrogate* s, const float* host, long long n) {
     KTS_TRY(hipMemset(s->M, 0, sizeof(float) * n));
     KTS_TRY(hipMemset(s->V, 0, sizeof(float) * n));
     KTS_TRY(ktn::refresh_bf16(s->P, n, s->lt, nullptr));
+    KTS_TRY(pack(s, nullptr));
     KTS_TRY(hipDeviceSynchronize());
     s->have_params = true;
     return hipSuccess;
@@ -470,7 +702,8 @@ static hipError_t ensure_train_buffers(Surrogate* s, long long B) {
     auto f = [](void* p) {
         if (p) (void)hipFree(p);
     };
-    f(s->x0);
+    f(s->xf);
+    f(s->partial0);
     for (int l = 0; l < 4; ++l) {
         f(s->Z[l]);
         f(s->H[l]);
@@ -488,7 +721,8 @@ static hipError_t ensure_train_buffers(Surrogate* s, long long B) {
     f(s->ws);
     f(s->partial);
     s->cap = 0;
-    KTS_TRY(dalloc(&s->x0, (size_t)B * 32));
+    KTS_TRY(dalloc(&s->xf, (size_t)B * 5));
+    KTS_TRY(dalloc(&s->partial0, (size_t)Surrogate::kChunks * 6 * 256));
     for (int l = 0; l < 4; ++l) {
         KTS_TRY(dalloc(&s->Z[l], (size_t)B * 256));
         KTS_TRY(dalloc(&s->H[l], (size_t)B * 256));
@@ -509,15 +743,16 @@ static hipError_t ensure_train_buffers(Surrogate* s, long long B) {
     return hipSuccess;
 }
 
-// forward over B rows of x0 into Z / H / Y (z stored when keep_z)
-static hipError_t forward(Surrogate* s, const uint16_t* x0, uint16_t** H, float** Z, float* Y, int B, bool keep_z,
+// forward over B rows of inputs xf (B x 5) into Z / H / Y (z stored when keep_z)
+static hipError_t forward(Surrogate* s, const float* xf, uint16_t** H, float** Z, float* Y, int B, bool keep_z,
                           hipStream_t st) {
-    for (int l = 0; l < 4; ++l) {
+    KTS_TRY(ktn::layer0(xf, B, s->P, keep_z ? Z[0] : nullptr, H[0], st));
+    for (int l = 1; l < 4; ++l) {
         ktn::GemmArgs g{};
         g.M = B;
         g.N = 256;
         g.K = ktn::kKpad[l];
-        g.A = l == 0 ? x0 : H[l - 1];
+        g.A = H[l - 1];
         g.lda = ktn::kKpad[l];
         g.B = s->wt[l];
         g.ldb = ktn::kKpad[l];
@@ -609,19 +844,26 @@ static hipError_t backward(Surrogate* s, int B, hipStream_t st) {
         KTS_TRY(ktn::gemm(g, false, true, ktn::kEpiBwd, 1, st));
         cur ^= 1;
     }
-    KTS_TRY(dW(s->x0, 32, 32, s->gb[cur], 256, 256, 0));
-    return db(s->g32[cur], 256, 0);
+    // input layer in FP32 (its weights and bias are the first 6 x 256 parameters)
+    return ktn::layer0_grad(s->xf, s->g32[cur], B, s->partial0, Surrogate::kChunks, s->G, st);
+}
+
+static FusedArgs fused_args(Surrogate* s, long long n) {
+    FusedArgs a{};
+    a.n = n;
+    a.t = Table{s->tab_rho, s->tab_rsh};
+    a.P = s->P;
+    a.wpk = s->wpk;
+    return a;
 }
 
 hipError_t predict(Surrogate* s, const float* feat, long long n, float* out, hipStream_t st) {
     KTS_TRY(hipSetDevice(s->device));
-    const long long chunk = 1 << 20;
-    KTS_TRY(ensure_train_buffers(s, std::min(n, chunk)));
-    for (long long o = 0; o < n; o += chunk) {
-        const int B = (int)std::min(chunk, n - o);
-        hipLaunchKernelGGL(k_feat_to_x0, dim3((B + 255) / 256), dim3(256), 0, st, feat + o * 5, (long long)B, s->x0);
-        KTS_TRY(forward(s, s->x0, s->H, s->Z, out + o * 8, B, false, st));
-    }
+    if (n <= 0) return hipSuccess;
+    FusedArgs a = fused_args(s, n);
+    a.xf = feat;
+    a.y = out;
+    hipLaunchKernelGGL(k_mlp_fused<false>, dim3((unsigned)((n + kFusedRays - 1) / kFusedRays)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
@@ -629,8 +871,8 @@ hipError_t gradients(Surrogate* s, const float* feat, const int8_t* cls, const f
                      double* loss, hipStream_t st) {
     KTS_TRY(hipSetDevice(s->device));
     KTS_TRY(ensure_train_buffers(s, n));
-    KTS_TRY(ktn::gather(feat, cls, tgt, nullptr, n, s->x0, s->bcls, s->btgt, st));
-    KTS_TRY(forward(s, s->x0, s->H, s->Z, s->Y, n, true, st));
+    KTS_TRY(ktn::gather(feat, cls, tgt, nullptr, n, s->xf, s->bcls, s->btgt, st));
+    KTS_TRY(forward(s, s->xf, s->H, s->Z, s->Y, n, true, st));
     KTS_TRY(hipMemsetAsync(s->loss, 0, sizeof(double), st));
     KTS_TRY(ktn::loss_head(s->Y, s->bcls, s->btgt, n, 1.0f, 1.0f, s->gy32, s->gyb, s->loss, st));
     KTS_TRY(backward(s, n, st));
@@ -643,11 +885,16 @@ hipError_t gradients(Surrogate* s, const float* feat, const int8_t* cls, const f
     return hipSuccess;
 }
 
-hipError_t adam_step(Surrogate* s, const float* grads, double lr, int t, hipStream_t st) {
+static hipError_t adam_update(Surrogate* s, const float* grads, double lr, int t, hipStream_t st) {
     const double b1 = 0.9, b2 = 0.999;
     const double c1 = 1.0 / (1.0 - std::pow(b1, t)), c2 = 1.0 / (1.0 - std::pow(b2, t));
     return ktn::adam(s->P, grads ? grads : s->G, s->M, s->V, ktn::kParams, (float)lr, (float)b1, (float)b2, 1e-8f,
                      (float)c1, (float)c2, s->lt, st);
+}
+
+hipError_t adam_step(Surrogate* s, const float* grads, double lr, int t, hipStream_t st) {
+    KTS_TRY(adam_update(s, grads, lr, t, st));
+    return pack(s, st);
 }
 
 // ---- data generation ----------------------------------------------------------------------
@@ -747,7 +994,7 @@ static hipError_t label(Surrogate* s, Dataset& d, const std::vector<double>& hp,
     KTS_TRY(kti::launch_trace(L, nullptr));
     Table t{s->tab_rho, s->tab_rsh};
     hipLaunchKernelGGL(k_targets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, d.params, n, t, d.ocls,
-                       d.th, d.ph, d.rh, d.gg, 1, d.feat, d.cls, d.tgt, d.ring);
+                       d.th, d.ph, d.rh, d.gg, d.feat, d.cls, d.tgt, d.ring);
     KTS_TRY(hipGetLastError());
     return hipDeviceSynchronize();
 }
@@ -794,16 +1041,16 @@ static hipError_t evaluate(Surrogate* s, Dataset& d, const TrainOpts& o, TrainRe
         gated_agree += c == cls[q];
         if (c != cls[q]) continue;
         if (c == 0) {
-            const double ct = std::max(-1.0, std::min(1.0, (double)yy[3]));
-            const double pt = std::atan2((double)yy[5], (double)yy[4]);
-            const double st = std::sqrt(1.0 - ct * ct);
-            const double v1[3] = {st * std::cos(pt), st * std::sin(pt), ct};
+            const double* pq = &hp[(size_t)q * 5];
+            double e1[3], e2[3], e3[3], tt, pt;
+            ray_frame(pq[1], pq[2], 0.0, pq[3], pq[4], e1, e2, e3);
+            sky_from_frame(e1, e2, e3, yy[3], yy[4], yy[5], tt, pt);
+            const double v1[3] = {std::sin(tt) * std::cos(pt), std::sin(tt) * std::sin(pt), std::cos(tt)};
             const double v2[3] = {std::sin(th[q]) * std::cos(ph[q]), std::sin(th[q]) * std::sin(ph[q]), std::cos(th[q])};
             const double dx = v1[0] - v2[0], dy = v1[1] - v2[1], dz = v1[2] - v2[2];
             defl.push_back(2.0 * std::asin(std::min(1.0, std::sqrt(dx * dx + dy * dy + dz * dz) / 2.0)));
         } else if (c == 2) {
-            const double risco = kt_isco_radius(hp[(size_t)q * 5], 1);
-            rre.push_back(std::fabs(yy[6] * risco / rh[q] - 1.0));
+            rre.push_back(std::fabs(std::exp((double)yy[6]) / rh[q] - 1.0));
             gre.push_back(std::fabs(std::exp((double)yy[7]) / gg[q] - 1.0));
         }
     }
@@ -841,6 +1088,11 @@ hipError_t train(Surrogate* s, const TrainOpts& o, TrainReport& rep) {
         return e;
     }
     rep.datagen_seconds = std::chrono::duration<double>(clk::now() - t0).count();
+    if (o.verbose) {
+        std::fprintf(stderr, "surrogate: labelled %lld training + %lld held-out rays in %.1f s\n", tr.n, ho.n,
+                     rep.datagen_seconds);
+        std::fflush(stderr);
+    }
 
     // training
     const int B = (int)std::min<long long>(o.batch, tr.n);
@@ -860,21 +1112,27 @@ hipError_t train(Surrogate* s, const TrainOpts& o, TrainReport& rep) {
         for (long long k = 0; k < steps_per_epoch && e == hipSuccess; ++k) {
             hipLaunchKernelGGL(k_batch_index, dim3((B + 255) / 256), dim3(256), 0, nullptr, tr.n, bits, key,
                                k * (long long)B, B, s->bidx);
-            e = ktn::gather(tr.feat, tr.cls, tr.tgt, s->bidx, B, s->x0, s->bcls, s->btgt, nullptr);
-            if (e == hipSuccess) e = forward(s, s->x0, s->H, s->Z, s->Y, B, true, nullptr);
+            e = ktn::gather(tr.feat, tr.cls, tr.tgt, s->bidx, B, s->xf, s->bcls, s->btgt, nullptr);
+            if (e == hipSuccess) e = forward(s, s->xf, s->H, s->Z, s->Y, B, true, nullptr);
             if (e == hipSuccess)
                 e = ktn::loss_head(s->Y, s->bcls, s->btgt, B, (float)o.w_sky, (float)o.w_disk, s->gy32, s->gyb,
                                    s->loss, nullptr);
             if (e == hipSuccess) e = backward(s, B, nullptr);
             ++t;
             const double lr = 0.5 * o.lr * (1.0 + std::cos(kPi * (double)(t - 1) / (double)total_steps));
-            if (e == hipSuccess) e = adam_step(s, nullptr, lr, (int)t, nullptr);
+            if (e == hipSuccess) e = adam_update(s, nullptr, lr, (int)t, nullptr);
         }
         if (e == hipSuccess) e = hipMemcpy(&ep_loss, s->loss, sizeof(double), hipMemcpyDeviceToHost);
         ep_loss /= (double)steps_per_epoch * B;
         if (ep == 0) rep.first_loss = ep_loss;
         loss_sum = ep_loss;
+        if (o.verbose && e == hipSuccess) {
+            std::fprintf(stderr, "surrogate: epoch %d/%d loss %.6f (%.1f s)\n", ep + 1, o.epochs, ep_loss,
+                         std::chrono::duration<double>(clk::now() - t1).count());
+            std::fflush(stderr);
+        }
     }
+    if (e == hipSuccess) e = pack(s, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     rep.train_seconds = std::chrono::duration<double>(clk::now() - t1).count();
     rep.final_loss = loss_sum;
@@ -893,82 +1151,58 @@ hipError_t train(Surrogate* s, const TrainOpts& o, TrainReport& rep) {
 
 // ---- traced inference ------------------------------------------------------------------------
 
-static hipError_t ensure_infer_buffers(Surrogate* s, long long n) {
-    if (n <= s->icap) return hipSuccess;
-    auto f = [](void* p) {
-        if (p) (void)hipFree(p);
-    };
-    f(s->ix0);
-    f(s->iH[0]);
-    f(s->iH[1]);
-    f(s->iY);
-    f(s->iring);
-    f(s->fallback);
-    s->icap = 0;
-    KTS_TRY(dalloc(&s->ix0, (size_t)n * 32));
-    KTS_TRY(dalloc(&s->iH[0], (size_t)n * 256));
-    KTS_TRY(dalloc(&s->iH[1], (size_t)n * 256));
-    KTS_TRY(dalloc(&s->iY, (size_t)n * 8));
-    KTS_TRY(dalloc(&s->iring, (size_t)n));
-    KTS_TRY(dalloc(&s->fallback, (size_t)n));
-    s->icap = n;
+hipError_t queue_reserve(Queue& q, long long n) {
+    if (!q.count) {
+        KTS_TRY(dalloc(&q.count, 4));
+        KTS_TRY(dalloc(&q.total, 1));
+    }
+    if (n <= q.cap) return hipSuccess;
+    if (q.list) (void)hipFree(q.list);
+    q.list = nullptr;
+    q.cap = 0;
+    KTS_TRY(dalloc(&q.list, (size_t)n));
+    q.cap = n;
     return hipSuccess;
 }
 
+void queue_release(Queue& q) {
+    for (void* p : {(void*)q.list, (void*)q.count, (void*)q.total})
+        if (p) (void)hipFree(p);
+    q = Queue();
+}
+
 hipError_t trace_with_surrogate(Surrogate* s, const kti::TraceLaunch& L, double gate_prob, double gate_ring,
-                                hipStream_t st, hipEvent_t* ev) {
+                                Queue* qu, hipStream_t st, hipEvent_t* ev) {
     if (!s->have_params) return hipErrorNotReady;
     const long long n = L.total;
-    KTS_TRY(ensure_infer_buffers(s, n));
-    Table t{s->tab_rho, s->tab_rsh};
-    const unsigned blocks = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, s->nfb, 4);
-    hipLaunchKernelGGL(k_features_views, dim3(blocks), dim3(256), 0, st, L.views, L.n_views, n, t, s->ix0, s->iring);
-    KTS_TRY(hipGetLastError());
-    // the network, layer by layer (ping-pong activations)
-    for (int l = 0; l < 4; ++l) {
-        ktn::GemmArgs g{};
-        g.M = (int)n;
-        g.N = 256;
-        g.K = ktn::kKpad[l];
-        g.A = l == 0 ? s->ix0 : s->iH[(l - 1) & 1];
-        g.lda = ktn::kKpad[l];
-        g.B = s->wt[l];
-        g.ldb = ktn::kKpad[l];
-        g.ldc = 256;
-        g.bias = s->P + ktn::param_offset(l) + (long long)ktn::kKreal[l] * ktn::kN[l];
-        g.outb = s->iH[l & 1];
-        g.k_split = g.K;
-        KTS_TRY(ktn::gemm(g, false, true, ktn::kEpiFwd, 1, st));
+    Queue& Q = qu ? *qu : s->queue;
+    if (n > Q.cap || !Q.count) {
+        if (!ev) return hipErrorInvalidValue;  // under capture: the queue must exist already
+        KTS_TRY(queue_reserve(Q, n));
     }
-    {
-        ktn::GemmArgs g{};
-        g.M = (int)n;
-        g.N = 8;
-        g.K = 256;
-        g.A = s->iH[1];
-        g.lda = 256;
-        g.B = s->wt[4];
-        g.ldb = 256;
-        g.ldc = 8;
-        g.bias = s->P + ktn::param_offset(4) + 256LL * 8;
-        g.out32 = s->iY;
-        g.k_split = 256;
-        KTS_TRY(ktn::gemm(g, false, true, ktn::kEpiLin, 1, st));
-    }
-    hipLaunchKernelGGL(k_gate, dim3(blocks), dim3(256), 0, st, L.views, L.n_views, n, s->iY, s->iring,
-                       (float)gate_prob, (float)gate_ring, L.out, s->fallback, s->nfb);
-    hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(64), 0, st, s->nfb, s->nfb_ll, (unsigned long long*)nullptr);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, Q.count, 4);
+    // the network and the gate in one kernel: answered rays get their outputs, the
+    // others join the fallback queue
+    FusedArgs a = fused_args(s, n);
+    a.views = L.views;
+    a.n_views = L.n_views;
+    a.gate_prob = (float)gate_prob;
+    a.gate_ring = (float)gate_ring;
+    a.out = L.out;
+    a.fallback = Q.list;
+    a.n_fallback = Q.count;
+    hipLaunchKernelGGL(k_mlp_fused<true>, dim3((unsigned)((n + kFusedRays - 1) / kFusedRays)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(64), 0, st, Q.count, Q.total, (unsigned long long*)nullptr);
     KTS_TRY(hipGetLastError());
     // the integrator on the queue the gate left (count read on the device)
     kti::TraceLaunch F = L;
-    F.ray_list = s->fallback;
-    F.total_dev = s->nfb_ll;
-    KTS_TRY(hipEventRecord(ev[1], st));
+    F.ray_list = Q.list;
+    F.total_dev = Q.total;
+    if (ev) KTS_TRY(hipEventRecord(ev[1], st));
     KTS_TRY(kti::launch_trace(F, st));
-    KTS_TRY(hipEventRecord(ev[2], st));
+    if (ev) KTS_TRY(hipEventRecord(ev[2], st));
     // the fallback count into the context's counters (after launch_trace zeroed them)
-    hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(64), 0, st, s->nfb, s->nfb_ll, L.stats + kti::kStatFallback);
+    hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(64), 0, st, Q.count, Q.total, L.stats + kti::kStatFallback);
     return hipGetLastError();
 }
 

@@ -4,8 +4,10 @@
 //   log10 r_cam - 2,  (cos i - 0.5415) / 0.4545,  (a - 0.499) / 0.499,
 //   alpha / (1.5 R_sh),  beta / (1.5 R_sh)
 // with R_sh the mean critical-curve radius of (a, i) from the shadow table.  Outputs:
-// three class logits (escaped, captured, disk), cos theta_inf, cos(phi_inf - phi_cam),
-// sin(phi_inf - phi_cam), r_hit / r_ISCO and log g.  A ray is answered by the surrogate
+// three class logits (escaped, captured, disk); the escape direction's components along
+// the ray's straight-line frame (e1 the undeflected direction, e2 from the line's closest
+// point towards the hole, e3 = e1 x e2); log r_hit and log g.  The input layer
+// runs in FP32, layers 1-4 on the matrix cores.  A ray is answered by the surrogate
 // when its largest class probability exceeds gate_prob and it lies outside
 // |rho / rho_c(psi) - 1| <= gate_ring of the critical curve; all others go to the
 // integrator through a device-side queue.
@@ -24,16 +26,17 @@ struct Surrogate;
 struct TrainOpts {
     long long n_train = 1ll << 24;
     long long n_holdout = 1ll << 20;
-    int epochs = 64;
+    int epochs = 192;
     int batch = 65536;
     double lr = 1e-3;           // cosine schedule from lr to 0
-    double w_sky = 1.0, w_disk = 1.0;
+    double w_sky = 30.0, w_disk = 1.0;
     double band = 0.02;         // rays within this relative distance of the critical curve ...
     double band_factor = 4.0;   // ... are drawn this many times more often
     uint64_t seed = 0x5EED0004ull;
     int label_precision = 1;    // KT_PREC_* of the integrator labelling the training rays
     double disk_outer = 20.0, escape_radius = 1000.0, capture_factor = 1.01;
     double rtol = 1e-10, atol = 1e-12, rtol32 = 1e-6, atol32 = 1e-7, restep_drift = 1e-6;
+    int verbose = 0;            // progress lines on stderr
 };
 
 struct TrainReport {
@@ -67,11 +70,23 @@ hipError_t adam_step(Surrogate* s, const float* grads, double lr, int t, hipStre
 
 hipError_t train(Surrogate* s, const TrainOpts& o, TrainReport& rep);
 
+// The fallback queue the gate fills: ray indices and their count (device).  Calls that
+// share a queue must not overlap; a plan owns its own so that plans can run concurrently.
+struct Queue {
+    long long cap = 0;
+    long long* list = nullptr;
+    unsigned long long* count = nullptr;  // [0] rays queued
+    long long* total = nullptr;           // the same count, as the integrator reads it
+};
+hipError_t queue_reserve(Queue& q, long long n);
+void queue_release(Queue& q);
+
 // Surrogate inference over every ray of the batch, the gate, then the integrator over the
-// rays the gate rejects (re-queued through L.ray_list).  ev[1], ev[2] bracket the
-// integrator kernel.
+// rays the gate rejects (re-queued through L.ray_list).  q = nullptr uses the surrogate's
+// own queue.  ev[1], ev[2] bracket the integrator kernel; ev = nullptr records no events
+// (stream capture), and then q must already hold L.total rays.
 hipError_t trace_with_surrogate(Surrogate* s, const kti::TraceLaunch& L, double gate_prob, double gate_ring,
-                                hipStream_t stream, hipEvent_t* ev);
+                                Queue* q, hipStream_t stream, hipEvent_t* ev);
 
 // critical radius / shadow radius from the table (host), as the kernels see them
 double table_critical_radius(const Surrogate* s, double a, double inc, double psi);

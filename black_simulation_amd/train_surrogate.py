@@ -1,0 +1,75 @@
+"""Train the surrogate on the GPU and write its parameters and held-out report.
+
+    python -m black_simulation_amd.train_surrogate --out PATH [--n-train N] [--epochs E] ...
+
+Writes PATH.f32 (kt_surrogate_param_count() little-endian float32 values, the layout
+kt_surrogate_load takes) and PATH.json (the options and the kt_surrogate_report).  The
+default options are the full training set: 2^24 rays labelled by the mixed-precision
+integrator, 2^20 held-out rays judged by the FP64 integrator, Adam lr 1e-3 cosine,
+batch 65536, 192 epochs, sky regression weighted 30x.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import api
+from . import _lib as K
+
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate")
+
+
+def load_params(path: str = DEFAULT_WEIGHTS) -> np.ndarray:
+    """Parameters written by this script (PATH.f32); raises if absent or of the wrong size."""
+    p = np.fromfile(path + ".f32", dtype="<f4")
+    n = K.lib().kt_surrogate_param_count()
+    if p.size != n:
+        raise ValueError(f"{path}.f32 holds {p.size} values, the network has {n}")
+    return p
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--n-train", type=int, default=0)
+    ap.add_argument("--n-holdout", type=int, default=0)
+    ap.add_argument("--epochs", type=int, default=0)
+    ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--lr", type=float, default=0.0)
+    ap.add_argument("--w-sky", type=float, default=0.0)
+    ap.add_argument("--w-disk", type=float, default=0.0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--init-seed", type=int, default=1)
+    args = ap.parse_args(argv)
+    kw = {"verbose": 1}
+    for k in ("n_train", "n_holdout", "epochs", "batch", "lr", "seed", "w_sky", "w_disk"):
+        v = getattr(args, k)
+        if v:
+            kw[k] = v
+    tr = api.Tracer(0)
+    tr.surrogate_init(args.init_seed)
+    t0 = time.perf_counter()
+    rep = tr.surrogate_train(**kw)
+    rep["wall_seconds"] = time.perf_counter() - t0
+    params = tr.surrogate_save()
+    tr.close()
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    params.astype("<f4").tofile(args.out + ".f32")
+    o = K.KtSurrogateOpts()
+    K.lib().kt_surrogate_opts_default(o)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    with open(args.out + ".json", "w") as f:
+        json.dump({"opts": {n: getattr(o, n) for n, _ in K.KtSurrogateOpts._fields_},
+                   "init_seed": args.init_seed, "report": rep}, f, indent=1)
+    print(json.dumps(rep), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -162,11 +162,18 @@ int kt_autotune(kt_ctx* ctx, const kt_view* views, int n_views, kt_trace_opts* o
 /* A prepared batch: the view table uploaded once and the launch sequence captured in a
  * hipGraph, replayed by kt_plan_launch (no reference counterpart; the batched-views
  * path of a frame loop).  `out` must hold device pointers; they are baked into the
- * graph.  A plan is externally synchronised and owns its work-queue counters. */
+ * graph.  With opts->use_surrogate the graph holds the network, the gate and the
+ * integrator over the rays the gate rejected.  A plan is externally synchronised; it owns
+ * its work-queue counters and fallback queue, so different plans (with different
+ * outputs) may replay concurrently on different streams.  The surrogate's parameters
+ * are read at replay time. */
 typedef struct kt_plan kt_plan;
 int kt_plan_create(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trace_opts* opts,
                    const kt_rays_out* out, kt_plan** plan);
 int kt_plan_launch(kt_plan* plan, void* hip_stream);
+/* Counters and time of the plan's last replay (waits for it): kernel_ms spans the whole
+ * graph; integrator_ms equals it for integrator-only plans and is 0 for surrogate plans. */
+int kt_plan_stats(kt_plan* plan, kt_stats* stats);
 int kt_plan_destroy(kt_plan* plan);
 
 /* ---- the learned surrogate (no reference counterpart) ----------------------------------
@@ -178,14 +185,15 @@ int kt_plan_destroy(kt_plan* plan);
  * row-major) then b_l. */
 typedef struct kt_surrogate_opts {
     long long n_train, n_holdout;  /* 2^24, 2^20                                        */
-    int epochs, batch;             /* 64, 65536                                         */
+    int epochs, batch;             /* 192, 65536                                        */
     double lr;                     /* 1e-3, cosine to 0                                 */
-    double w_sky, w_disk;          /* loss weights of the sky and disk regressions      */
+    double w_sky, w_disk;          /* loss weights of the sky and disk regressions (30, 1) */
     double band, band_factor;      /* rays within `band` of the critical curve drawn    */
                                    /* `band_factor` x more often (0.02, 4)              */
     unsigned long long seed;
     int label_precision;           /* KT_PREC_* labelling the training set (mixed)      */
     double disk_outer, escape_radius, capture_factor;  /* the scene it learns (20, 1000, 1.01) */
+    int verbose;                   /* 1: one progress line per epoch on stderr          */
 } kt_surrogate_opts;
 
 typedef struct kt_surrogate_report {

@@ -37,10 +37,20 @@ def unflat(p):
     return out
 
 
+def f32(x):
+    return np.asarray(x, np.float32).astype(np.float64)
+
+
+def quantised(params):
+    """the weights as the kernels use them: FP32 input layer, bf16 matrix-core layers"""
+    return [(f32(W) if l == 0 else bf16(W), b) for l, (W, b) in enumerate(params)]
+
+
 def emulated_forward(params, x):
-    """the kernels' arithmetic: bf16 weights and layer inputs, wide accumulation"""
-    Wq = [(bf16(W), b) for W, b in params]
-    h = bf16(x)
+    """the kernels' arithmetic: FP32 input layer, then bf16 weights and layer inputs with
+    wide accumulation"""
+    Wq = quantised(params)
+    h = f32(x)
     for l, (W, b) in enumerate(Wq):
         z = h @ W + b
         h = bf16(M.silu(z)) if l < len(Wq) - 1 else z
@@ -75,9 +85,9 @@ def test_gradients_match_numpy(tracer):
     g, loss = tracer.surrogate_gradients(x, cls, tgt)
     g = g.cpu().numpy()
     P = unflat(p32)
-    Wq = [(bf16(W), b) for W, b in P]
-    # FP64 backward through the same (bf16-weight) network, activations rounded as on the GPU
-    h = bf16(x)
+    Wq = quantised(P)
+    # FP64 backward through the same network, activations rounded as on the GPU
+    h = f32(x)
     cache = []
     for l, (W, b) in enumerate(Wq):
         z = h @ W + b
@@ -141,3 +151,72 @@ def test_gated_trace_falls_back_to_the_integrator(tracer, trained):
     assert np.array_equal(got["cls"][fb] & 7, want["cls"][fb])
     assert np.array_equal(got["steps"][fb], want["steps"][fb])
     assert np.all(got["steps"][sur] == 0)
+
+
+def test_surrogate_plan_replays_equal_eager(tracer, trained):
+    """Network, gate and fallback integrator captured in one hipGraph: replays with eager
+    work and synchronisation in between reproduce the eager call every time."""
+    v = c2_view(96, 54)
+    opts = bsa.TraceOpts(use_surrogate=1, precision=bsa.PREC_MIXED)
+    eager = bsa.to_numpy(tracer.trace_views([v], opts))
+    plan = tracer.plan([v], opts)
+    for rep in range(3):
+        out = plan.launch()
+        torch.cuda.synchronize()
+        got = bsa.to_numpy(out)
+        for k in ("cls", "steps", "r_hit", "g", "theta_inf", "phi_inf"):
+            assert np.array_equal(got[k], eager[k], equal_nan=True), (rep, k)
+        tracer.trace_views([c2_view(32, 18)])  # unrelated eager work between replays
+        torch.cuda.synchronize()
+        for k in out:
+            out[k].zero_()
+    plan.close()
+
+
+def test_trained_network_answers_most_rays_of_an_orbit_frame(tracer):
+    """The shipped weights (black_simulation_amd/data/surrogate.f32) on a 1/16-size frame
+    of the a=0.998 orbit: the gate answers most rays, classes agree with the FP64
+    integrator, and the answered rays take no integrator steps."""
+    from black_simulation_amd import orbit as OR
+    from black_simulation_amd.configs import c5_view
+    from black_simulation_amd.train_surrogate import load_params
+
+    tracer.surrogate_load(load_params())
+    v = c5_view(60, 480, 270)
+    ref = tracer.trace_views([v], bsa.TraceOpts(precision=bsa.PREC_FP64))
+    plain_steps = int(ref["steps"].to(torch.int64).sum())
+    got = tracer.trace_views([v], bsa.TraceOpts(use_surrogate=1, precision=bsa.PREC_MIXED))
+    e = OR.errors(got, ref)
+    agree = float(e["agree"].float().mean())
+    answered = float(e["answered"].float().mean())
+    sur_steps = int(got["steps"].to(torch.int64).sum())
+    d50, d99, _ = OR.quantiles(e["defl_answered"])
+    print(f"answered {answered:.3f}, class agreement {agree:.5f}, integrator attempts {sur_steps} vs "
+          f"{plain_steps} plain, deflection of answered escaped rays p50 {d50:.3g} p99 {d99:.3g} rad", flush=True)
+    assert answered > 0.5
+    assert agree > 0.99
+    assert sur_steps < 0.7 * plain_steps
+
+
+def test_surrogate_plans_replay_concurrently_on_two_streams(tracer, trained):
+    """Two surrogate plans (each with its own outputs and fallback queue) replayed on two
+    streams at once give what the eager calls give."""
+    views = [c2_view(96, 54), bsa.View(**{**c2_view(96, 54).__dict__, "theta_cam": 1.0, "phi_cam": 0.7})]
+    opts = bsa.TraceOpts(use_surrogate=1, precision=bsa.PREC_MIXED)
+    eager = [bsa.to_numpy(tracer.trace_views([v], opts)) for v in views]
+    streams = [torch.cuda.current_stream(), torch.cuda.Stream()]
+    plans = [tracer.plan([v], opts) for v in views]
+    for rep in range(3):
+        outs = [p.launch(s) for p, s in zip(plans, streams)]
+        torch.cuda.synchronize()
+        for i, out in enumerate(outs):
+            got = bsa.to_numpy(out)
+            for k in ("cls", "steps", "r_hit", "g", "theta_inf", "phi_inf"):
+                assert np.array_equal(got[k], eager[i][k], equal_nan=True), (rep, i, k)
+            assert plans[i].stats()["fallback_rays"] == int(((eager[i]["cls"] & bsa.FLAG_SURROGATE) == 0).sum())
+        for out in outs:
+            for k in out:
+                out[k].zero_()
+        torch.cuda.synchronize()
+    for p in plans:
+        p.close()
