@@ -95,25 +95,41 @@ def main():
     ap.add_argument("--steps", type=int, default=120)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--path", choices=["surrogate", "plain", "mixed"], default="surrogate")
-    ap.add_argument("--workload", choices=["orbit", "c2"], default="orbit")
+    ap.add_argument("--workload", choices=["orbit", "c2", "c3"], default="orbit")
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--weights", default=None)
     ap.add_argument("--streams", type=int, default=2)
+    ap.add_argument("--lean", action="store_true",
+                    help="for profilers: no autotuning, no plain-FP64 timing, no accuracy pass")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="profiler calibration only: five device copies of this many MiB, then exit")
+    ap.add_argument("--mode", choices=["auto", "oneshot", "persistent"], default="auto",
+                    help="integrator kernel (auto: the autotuner's pick, or persistent with --lean)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: rays are independent and this build runs on a single MI355X; "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
-    if args.path == "surrogate" and args.workload != "orbit":
+    if args.path == "surrogate" and args.workload == "c2":
         print("the surrogate is trained for the orbit scene (disk to 20 M); use --workload orbit", file=sys.stderr)
         sys.exit(2)
 
     import torch
 
+    if args.copy_mib:  # a plain copy of known size, to scale FETCH_SIZE / WRITE_SIZE readings
+        n = args.copy_mib * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 5}), flush=True)
+        return
+
     import black_simulation_amd as bsa
     from black_simulation_amd import orbit as O
-    from black_simulation_amd.configs import C5_FRAMES, c2_view, c5_view
+    from black_simulation_amd.configs import C5_FRAMES, c2_view, c3_views, c5_view
     from black_simulation_amd.train_surrogate import DEFAULT_WEIGHTS, load_params
 
     width = args.height * 16 // 9
@@ -126,19 +142,22 @@ def main():
                 k = (k + 1) % C5_FRAMES
             frames.append(k)
         warm = [(C5_FRAMES // 2 + i) % C5_FRAMES for i in range(args.warmup)]
-    else:  # one view, replayed; a copy per stream so that replays can overlap
-        views = [c2_view(width, args.height)] * args.streams
+    else:  # one frame (a view, or the 64-view batch), a copy per stream so that replays can overlap
+        one = [c2_view(width, args.height)] if args.workload == "c2" else c3_views()
+        views = [one] * args.streams
         frames = [i % args.streams for i in range(args.steps)]
         warm = [i % args.streams for i in range(args.warmup)]
-    rays = views[0].rays
+    frame_views = [v if isinstance(v, list) else [v] for v in views]
+    rays = sum(v.rays for v in frame_views[0])
 
     tr = bsa.Tracer(0)
-    plain = bsa.TraceOpts(precision=bsa.PREC_FP64)
-    plain = tr.autotune([views[frames[len(frames) // 2]]], plain)
+    tune = (lambda o: o) if args.lean else (lambda o: tr.autotune(frame_views[frames[len(frames) // 2]], o))
+    mode = {"auto": bsa.MODE_AUTO, "oneshot": bsa.MODE_ONESHOT, "persistent": bsa.MODE_PERSISTENT}[args.mode]
+    plain = tune(bsa.TraceOpts(precision=bsa.PREC_FP64, mode=mode))
     if args.path == "plain":
         opts = plain
     else:
-        opts = tr.autotune([views[frames[len(frames) // 2]]], bsa.TraceOpts(precision=bsa.PREC_MIXED))
+        opts = tune(bsa.TraceOpts(precision=bsa.PREC_MIXED, mode=mode))
         if args.path == "surrogate":
             tr.surrogate_load(load_params(args.weights or DEFAULT_WEIGHTS))
             opts.use_surrogate = 1
@@ -149,8 +168,8 @@ def main():
     wall_frames = len(frames)
     value = rays * wall_frames / wall / 1e6
     # the plain FP64 integrator on the same frames, timed the same way
-    ref = O.Orbit(tr, views, plain, streams=args.streams) if args.path != "plain" else orb
-    if args.path != "plain":
+    ref = O.Orbit(tr, views, plain, streams=args.streams) if args.path != "plain" and not args.lean else orb
+    if ref is not orb:
         wall_p, lat_p = timed_frames(ref, warm, frames, torch)
     else:
         wall_p, lat_p = wall, lat
@@ -159,7 +178,7 @@ def main():
     # per-kernel attribution (untimed pass without graphs: events around each kernel group)
     integ_ms, sur_ms, attempts, fp64_att, fallback = [], [], [], [], []
     for k in frames[:: max(1, len(frames) // 24)]:
-        tr.trace_views([views[k]], opts, out=orb.out)
+        tr.trace_views(frame_views[k], opts, out=orb.out)
         st = tr.stats()
         integ_ms.append(st["integrator_ms"])
         sur_ms.append(st["surrogate_ms"])
@@ -170,7 +189,7 @@ def main():
 
     # accuracy of the chosen path against the FP64 render of every timed frame
     acc = None
-    if args.path != "plain":
+    if ref is not orb:
         agree, n_rays, answered = 0, 0, 0
         defl, defl_a, g_rel, g_rel_a, r_rel, r_rel_a = [], [], [], [], [], []
         for k in sorted(set(frames)):
@@ -228,7 +247,9 @@ def main():
     desc = {"orbit": f"120-frame orbit r=40 M, i 20->85 deg, a=0.998, {width}x{args.height} Bardeen plane 24 M "
                      f"wide, disk r_ISCO..20 M",
             "c2": f"single frame a=0.998, r=50 M, i=85 deg, {width}x{args.height} Bardeen plane 30 M wide, "
-                  f"disk r_ISCO..40 M"}[args.workload]
+                  f"disk r_ISCO..40 M",
+            "c3": f"64 seeded views (seed 0x5EED0003) of uneven size, spin and camera radius in one launch, "
+                  f"{rays} rays"}[args.workload]
     rec = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -244,7 +265,7 @@ def main():
         "data": "synthetic: analytic Bardeen image planes, no input files; surrogate weights trained by "
                 "black_simulation_amd.train_surrogate on integrator-labelled rays",
         "config": {"workload": f"{args.path}: {desc}", "path": args.path, "rays_per_step": rays,
-                   "streams": args.streams,
+                   "streams": args.streams, "lean": args.lean,
                    "frame_ms_p50": round(percentile(lat, 0.5), 3), "frame_ms_p99": round(percentile(lat, 0.99), 3),
                    "plain_fp64": {"value": round(value_p, 3), "frame_ms_p50": round(percentile(lat_p, 0.5), 3),
                                   "frame_ms_p99": round(percentile(lat_p, 0.99), 3)},
@@ -263,7 +284,7 @@ def main():
         ref.close()
     tr.close()
     if not args.no_cpu:
-        rec["cpu_baseline"] = cpu_baseline(views[frames[len(frames) // 2]])
+        rec["cpu_baseline"] = cpu_baseline(frame_views[frames[len(frames) // 2]][0])
     print(json.dumps(rec), flush=True)
 
 

@@ -633,17 +633,22 @@ __device__ __forceinline__ bool lane_step(Lane<R>& s, R rtol, R atol, int max_st
 // ---- mixed precision --------------------------------------------------------------
 //
 // The state and the controller live in FP64; a step is attempted in FP32 unless the ray
-// is in an FP64 zone: r < 1.2 r+, or inside the turning-point zones of kerr.cpp:521-542
-// (R or M below 0.05 of their scale), or in the tail.  An FP32 step that passes the
-// error test but moves H = 0 by more than `drift` (|w_r^2 - R| or |w_mu^2 - M| relative
-// to their scales; E, L and Q are exact constants here) is thrown away and re-taken in
-// FP64 with the same h.  Both precisions use the mixed tolerances.
+// is in an FP64 zone: r < 1.2 r+, inside the turning-point zones of kerr.cpp:521-542
+// (R or M below 0.05 of their scale), within sin^2(theta) < 0.05 of the axis (where
+// 1 - mu^2 loses its digits in FP32), or in the tail.  An FP32 step that fails the
+// error test, or passes it but moves H = 0 by more than `drift` (|w_r^2 - R| or
+// |w_mu^2 - M| relative to their scales; E, L and Q are exact constants here), is thrown
+// away and re-taken in FP64 with the same h, so only FP64 attempts steer the step size:
+// an FP32 error estimate held above tolerance by rounding noise cannot shrink h.  Both
+// precisions use the mixed tolerances.
 struct MixedLane {
     Lane<double> d;
     Photon<float> pf;
     float r_fp64;     // 1.2 r+
     int fp64_steps;
 };
+
+constexpr float kFp32AxisZone = 0.05f;  // FP64 while 1 - mu^2 is below this
 
 __device__ __forceinline__ bool fp64_zone(const MixedLane& s) {
     if (s.d.tail) return true;
@@ -654,6 +659,7 @@ __device__ __forceinline__ bool fp64_zone(const MixedLane& s) {
     const float Rr = fmaf(P, P, -(delta * p.K));
     if (!(Rr > 0.05f * fmaf(P, P, fabsf(delta) * p.K))) return true;
     const float m2 = m * m, s2 = 1.0f - m2;
+    if (!(s2 >= kFp32AxisZone)) return true;
     const float M = fmaf(-p.A2E2, m2 * m2, fmaf(-p.C1, m2, p.Q));
     const float scale = fmaf(p.L * p.L, m2, fmaf(fabsf(p.Q), s2, p.A2E2 * (m2 * s2)));
     return !(M > 0.05f * scale);
@@ -708,7 +714,7 @@ __device__ __forceinline__ bool lane_step_mixed(MixedLane& s, double rtol, doubl
         }
         const float hf = (float)h;
         const float errf = dp5_attempt(s.pf, false, 0.0f, y, hf, k, ynf, (float)rtol, (float)atol);
-        if (errf <= 1.0f && h_drift(s.pf, ynf) > drift) {
+        if (!(errf <= 1.0f) || h_drift(s.pf, ynf) > drift) {
             use64 = true;  // the re-step
         } else {
             err = errf;

@@ -17,19 +17,20 @@ from .api import TraceOpts, Tracer, View
 
 
 class Orbit:
-    """One plan per view (views of equal size); frame k writes outputs[k % streams] on
-    streams[k % streams]."""
+    """One plan per frame; a frame is a view or a list of views traced in one launch (all
+    frames of equal ray count).  Frame k writes outputs[k % streams] on streams[k % streams]."""
 
-    def __init__(self, tracer: Tracer, views: Sequence[View], opts: TraceOpts, streams: int = 1):
-        n = views[0].rays
-        if any(v.rays != n for v in views):
-            raise ValueError("orbit frames must have equal pixel counts")
+    def __init__(self, tracer: Tracer, frames: Sequence, opts: TraceOpts, streams: int = 1):
+        frames = [list(f) if isinstance(f, (list, tuple)) else [f] for f in frames]
+        n = sum(v.rays for v in frames[0])
+        if any(sum(v.rays for v in f) != n for f in frames):
+            raise ValueError("orbit frames must have equal ray counts")
         torch = tracer.torch
-        self.tracer, self.views = tracer, list(views)
+        self.tracer, self.frames, self.rays = tracer, frames, n
         self.streams = [torch.cuda.current_stream(tracer.device)] + \
                        [torch.cuda.Stream(tracer.device) for _ in range(streams - 1)]
         self.outputs = [tracer.alloc_outputs(n) for _ in range(streams)]
-        self.plans = [tracer.plan([v], opts, out=self.outputs[k % streams]) for k, v in enumerate(self.views)]
+        self.plans = [tracer.plan(f, opts, out=self.outputs[k % streams]) for k, f in enumerate(frames)]
 
     @property
     def out(self) -> dict:

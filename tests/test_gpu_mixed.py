@@ -73,3 +73,22 @@ def test_precision_vs_fp64_oracle(tracer, c2_sample, prec):
     assert np.median(d) < 1e-4 and np.quantile(gerr, 0.99) < 1e-3
     if prec == bsa.PREC_MIXED:
         assert restep.any() and st["fp64_steps"] > 0
+
+
+def test_mixed_finishes_rays_passing_over_the_pole(tracer):
+    """Rays of a near-polar batch view (a = 0.564, i = 15.5 deg, r = 26.8 M) launched with
+    alpha ~ 0 cross the axis.  FP32 loses 1 - mu^2 there; the mixed path must take those
+    steps in FP64 and finish in a step count close to the FP64 path's, with the same class
+    (before the axis zone and the FP64 retake of FP32 rejections, these rays ran out of
+    their 200,000-step budget)."""
+    v = bsa.View(width=2048, height=2048, spin=0.5640401548275981, r_cam=26.786079386450602,
+                 theta_cam=np.radians(15.457765328768993), half_width=7.644179312227107,
+                 half_height=7.644179312227107, disk_inner=0.0, disk_outer=20.0)
+    hw = v.half_width
+    params = [(-hw + (col + 0.5) * 2 * hw / v.width, hw - (row + 0.5) * 2 * hw / v.height)
+              for col in (1022, 1023, 1024, 1025) for row in range(0, 2048, 97)]
+    got = bsa.to_numpy(tracer.trace_params(v, params, bsa.TraceOpts(precision=bsa.PREC_MIXED)))
+    want = O.trace_params(oracle_view(v), O.make_opts(), np.asarray(params))
+    print(f"max mixed steps {got['steps'].max()}, max FP64 steps {want['steps'].max()}", flush=True)
+    assert np.array_equal(got["cls"] & 7, want["cls"])
+    assert got["steps"].max() < 2 * want["steps"].max()
