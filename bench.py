@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--streams", type=int, default=2)
     ap.add_argument("--lean", action="store_true",
                     help="for profilers: no autotuning, no plain-FP64 timing, no accuracy pass")
+    ap.add_argument("--host-outputs", action="store_true",
+                    help="also time the path with host output buffers (PCIe copy included, one stream)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="profiler calibration only: five device copies of this many MiB, then exit")
     ap.add_argument("--mode", choices=["auto", "oneshot", "persistent"], default="auto",
@@ -174,6 +176,16 @@ def main():
     else:
         wall_p, lat_p = wall, lat
     value_p = rays * wall_frames / wall_p / 1e6
+
+    # the same frames through host buffers (the library copies the outputs back over PCIe)
+    host_value = None
+    if args.host_outputs:
+        k0 = frames[len(frames) // 2]
+        tr.trace_views_host(frame_views[k0], opts)
+        t0 = time.perf_counter()
+        for k in frames:
+            tr.trace_views_host(frame_views[k], opts)
+        host_value = rays * len(frames) / (time.perf_counter() - t0) / 1e6
 
     # per-kernel attribution (untimed pass without graphs: events around each kernel group)
     integ_ms, sur_ms, attempts, fp64_att, fallback = [], [], [], [], []
@@ -270,6 +282,7 @@ def main():
                    "plain_fp64": {"value": round(value_p, 3), "frame_ms_p50": round(percentile(lat_p, 0.5), 3),
                                   "frame_ms_p99": round(percentile(lat_p, 0.99), 3)},
                    "speedup_vs_plain_fp64": round(value / value_p, 3),
+                   "host_outputs_value": round(host_value, 3) if host_value else None,
                    "fallback_fraction": round(sum(fallback) / n_attr / rays, 4) if args.path == "surrogate" else None,
                    "accuracy_vs_fp64": acc,
                    "launch": {"mode": ["persistent", "one-shot"][st["mode"]], "block": st["block"],

@@ -161,6 +161,24 @@ class Tracer:
         _check(self._ctx, rc, "kt_trace_views")
         return out
 
+    def trace_views_host(self, views: Sequence[View], opts: TraceOpts | None = None) -> dict:
+        """As trace_views, into host (numpy) buffers: the library copies the outputs back
+        over PCIe and synchronises before returning."""
+        n = sum(v.rays for v in views)
+        dt = {"cls": np.int8, "theta_inf": np.float64, "phi_inf": np.float64, "r_hit": np.float64,
+              "g": np.float64, "steps": np.int32}
+        out = {k: np.empty(n, dt[k]) for k in OUTPUT_FIELDS}
+        s = K.KtRaysOut()
+        for name in OUTPUT_FIELDS:
+            setattr(s, name, out[name].ctypes.data)
+        s.on_device = 0
+        arr = (K.KtView * len(views))(*[v.to_c() for v in views])
+        o = (opts or TraceOpts()).to_c()
+        st = self.torch.cuda.current_stream(self.device)
+        _check(self._ctx, K.lib().kt_trace_views(self._ctx, arr, len(views), C.byref(o), C.byref(s),
+                                                 C.c_void_p(st.cuda_stream)), "kt_trace_views")
+        return out
+
     def trace_params(self, view: View, params, opts: TraceOpts | None = None, out: dict | None = None,
                      stream=None) -> dict:
         """Explicit rays: (alpha, beta) rows for a Bardeen view, unit looks for a pinhole view."""
@@ -316,7 +334,7 @@ def _tracer() -> Tracer:
 def render(view: View, opts: TraceOpts | None = None) -> dict:
     """eht::render counterpart: every pixel of one view, as host numpy arrays reshaped to
     (height, width), row-major from the top."""
-    out = to_numpy(_tracer().trace_views([view], opts))
+    out = _tracer().trace_views_host([view], opts)
     return {k: v.reshape(view.height, view.width) for k, v in out.items()}
 
 

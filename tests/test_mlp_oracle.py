@@ -54,3 +54,30 @@ def test_adam_reduces_loss():
         _, gy = M.loss_and_grad_out(y, cls, tgt)
         params = M.adam_step(params, M.backward(params, cache, gy), st, 1e-2)
     assert M.loss_fn(params, x, cls, tgt) < 0.2 * first
+
+
+def test_adam_first_step_is_lr_times_sign():
+    """With bias correction the first Adam step moves every parameter by -lr sign(g)
+    (up to eps); without it the step would be sqrt(1 - beta2) / (1 - beta1) = 0.316 times
+    larger than lr in size."""
+    params, x, cls, tgt = small_problem(B=32)
+    y, cache = M.forward(params, x)
+    _, gy = M.loss_and_grad_out(y, cls, tgt)
+    grads = M.backward(params, cache, gy)
+    new = M.adam_step(params, grads, M.adam_state(params), 1e-3)
+    for (W, b), (gW, gb), (W1, b1) in zip(params, grads, new):
+        for p, g, p1 in ((W, gW, W1), (b, gb, b1)):
+            big = np.abs(g) > 1e-4
+            np.testing.assert_allclose((p1 - p)[big], -1e-3 * np.sign(g[big]), rtol=1e-3)
+
+
+def test_loss_closed_form():
+    """Zero outputs: cross-entropy log 3 per ray, plus the squared targets of the class's
+    regression outputs (sky for escaped, disk for disk rays) times their weights."""
+    B = 3
+    y = np.zeros((B, 8))
+    cls = np.array([0, 1, 2])
+    tgt = np.array([[0.5, -0.5, 0.25, 0.0, 0.0], [0.3, 0.3, 0.3, 0.3, 0.3], [0.0, 0.0, 0.0, 2.0, -1.0]])
+    loss, _ = M.loss_and_grad_out(y, cls, tgt, w_sky=2.0, w_disk=0.5)
+    want = (3 * np.log(3.0) + 2.0 * (0.25 + 0.25 + 0.0625) + 0.5 * (4.0 + 1.0)) / B
+    assert loss == np.float64(want) or abs(loss - want) < 1e-12
