@@ -335,7 +335,8 @@ __device__ __forceinline__ f32x16 mfma32(const uint4& a, const bf16x8& b, const 
 template <bool VIEWS>
 // two waves per SIMD: 214 VGPRs, accumulators in the VGPR file, no spills
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mlp_fused(FusedArgs a) {
-    __shared__ uint4 slab[2][kSlabElems / 8];
+    __shared__ uint4 slab0[kSlabElems / 8];
+    __shared__ uint4 slab1[kSlabElems / 8];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m = lane & 31, h = lane >> 5;
     const long long q = (long long)blockIdx.x * kFusedRays + wave * 32 + m;
@@ -344,9 +345,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
     // stage slab 0 while the inputs are formed
     const uint4* src = reinterpret_cast<const uint4*>(a.wpk);
-    uint4 pre[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) pre[i] = src[tid + 256 * i];
+    // four named registers, not an array: the compiler would place a private array in LDS
+    uint4 w0 = src[tid], w1 = src[tid + 256], w2 = src[tid + 512], w3 = src[tid + 768];
 
     float f[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
     float rg = -1.f;  // relative distance from the critical curve; < 0: not coverable
@@ -385,9 +385,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 
     f32x16 acc[8];
-    int buf = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) slab[0][tid + 256 * i] = pre[i];
+    slab0[tid] = w0;
+    slab0[tid + 256] = w1;
+    slab0[tid + 512] = w2;
+    slab0[tid + 768] = w3;
     __syncthreads();
     for (int l = 1; l <= 3; ++l) {
         {  // bias into the accumulators
@@ -407,18 +408,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         for (int sl = 0; sl < 8; ++sl) {
             // next slab into registers (the output layer's after the last hidden one)
             const uint4* nsrc = src + (long long)(8 * (l - 1) + sl + 1) * (kSlabElems / 8);
+            w0 = nsrc[tid];
+            w1 = nsrc[tid + 256];
+            w2 = nsrc[tid + 512];
+            w3 = nsrc[tid + 768];
+            // compiler-only fences: the next slab's loads stay ahead of the matrix work
+            // and their LDS stores after it, so the load latency hides behind the MFMAs
+            asm volatile("" ::: "memory");
+            // eight slabs per layer: the buffer parity is static (slab0 at even sl)
+            const uint4* S = (sl & 1) ? slab1 : slab0;
+            uint4* N = (sl & 1) ? slab0 : slab1;
+            // a k-block's eight A fragments are read together, so their LDS latency
+            // overlaps the MFMAs that consume them in turn
 #pragma unroll
-            for (int i = 0; i < 4; ++i) pre[i] = nsrc[tid + 256 * i];
-            const uint4* S = slab[buf];
+            for (int j = 0; j < 2; ++j) {
+                uint4 af[8];
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
+                for (int fb = 0; fb < 8; ++fb) af[fb] = S[(j * 8 + fb) * 64 + lane];
 #pragma unroll
-                for (int fb = 0; fb < 8; ++fb)
-                    acc[fb] = mfma32(S[(j * 8 + fb) * 64 + lane], B[2 * sl + j], acc[fb]);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) slab[buf ^ 1][tid + 256 * i] = pre[i];
+                for (int fb = 0; fb < 8; ++fb) acc[fb] = mfma32(af[fb], B[2 * sl + j], acc[fb]);
+            }
+            asm volatile("" ::: "memory");
+            N[tid] = w0;
+            N[tid + 256] = w1;
+            N[tid + 512] = w2;
+            N[tid + 768] = w3;
             __syncthreads();
-            buf ^= 1;
         }
         // SiLU, rounded to bf16: the next layer's operand
 #pragma unroll
@@ -434,7 +449,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
         for (int i = 0; i < 16; ++i) o[p][i] = 0.f;
     {
-        const uint4* S = slab[buf];
+        const uint4* S = slab0;  // 24 hidden slabs: the output layer's landed in slab0
 #pragma unroll
         for (int kb = 0; kb < 16; ++kb) o[kb & 3] = mfma32(S[kb * 64 + lane], B[kb], o[kb & 3]);
     }
