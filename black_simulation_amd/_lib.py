@@ -126,6 +126,21 @@ class KtSurrogateReport(C.Structure):
                                           "rhit_rel_median", "rhit_rel_p99")]
 
 
+class KtShadeOpts(C.Structure):
+    _fields_ = [
+        ("disk_temperature", C.c_double),
+        ("disk_brightness", C.c_double),
+        ("sky", C.c_int),
+        ("sky_brightness", C.c_double),
+        ("grid_line", C.c_double),
+        ("sky_image", C.c_void_p),
+        ("sky_width", C.c_int),
+        ("sky_height", C.c_int),
+    ]
+
+
+SKY_BLACK, SKY_GRID, SKY_IMAGE = 0, 1, 2
+
 # every entry point include/kerr_trace.h declares: name -> (restype, argtypes)
 _P = C.POINTER
 SIGNATURES = {
@@ -155,6 +170,10 @@ SIGNATURES = {
     "kt_surrogate_adam_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int]),
     "kt_surrogate_table_radius": (C.c_double, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
     "kt_surrogate_shadow_radius": (C.c_double, [C.c_void_p, C.c_double, C.c_double]),
+    "kt_shade_opts_default": (None, [_P(KtShadeOpts)]),
+    "kt_shade_views": (C.c_int, [C.c_void_p, _P(KtView), C.c_int, _P(KtRaysOut), _P(KtShadeOpts), C.c_void_p,
+                                 C.c_void_p]),
+    "kt_blackbody_table": (C.c_int, [C.c_int, _P(C.c_float)]),
     "kt_horizon_radius": (C.c_double, [C.c_double]),
     "kt_isco_radius": (C.c_double, [C.c_double, C.c_int]),
     "kt_photon_orbit_radius": (C.c_double, [C.c_double, C.c_int]),

@@ -282,6 +282,33 @@ class Tracer:
     def surrogate_table_radius(self, a: float, inclination: float, psi: float) -> float:
         return K.lib().kt_surrogate_table_radius(self._ctx, a, inclination, psi)
 
+    # -- HDR colour ----------------------------------------------------------------------
+    def shade(self, views: Sequence[View], rays: dict, sky: int = K.SKY_GRID, disk_temperature: float = 1e4,
+              disk_brightness: float = 1.0, sky_brightness: float = 1.0, grid_line: float = 0.02, sky_image=None,
+              stream=None):
+        """kt_shade_views: RGBA (half floats, n x 4, on the device) of the traced `rays`
+        (device outputs of trace_views on the same views).  sky_image: (H, W, 3) float32,
+        equirectangular, row 0 at latitude -90 deg."""
+        t = self.torch
+        n = sum(v.rays for v in views)
+        rgba = t.empty((n, 4), dtype=t.float16, device=f"cuda:{self.device}")
+        o = K.KtShadeOpts()
+        K.lib().kt_shade_opts_default(C.byref(o))
+        o.sky, o.disk_temperature, o.disk_brightness = sky, disk_temperature, disk_brightness
+        o.sky_brightness, o.grid_line = sky_brightness, grid_line
+        img = None
+        if sky_image is not None:
+            img = t.as_tensor(np.ascontiguousarray(sky_image, np.float32)).to(rgba.device)
+            o.sky_image, o.sky_width, o.sky_height = img.data_ptr(), img.shape[1], img.shape[0]
+        arr = (K.KtView * len(views))(*[v.to_c() for v in views])
+        s = self._out_struct(rays)
+        st = stream if stream is not None else t.cuda.current_stream(self.device)
+        _check(self._ctx, K.lib().kt_shade_views(self._ctx, arr, len(views), C.byref(s), C.byref(o),
+                                                 C.c_void_p(rgba.data_ptr()), C.c_void_p(st.cuda_stream)),
+               "kt_shade_views")
+        self._keep_img = img
+        return rgba
+
     def stats(self) -> dict:
         s = K.KtStats()
         _check(self._ctx, K.lib().kt_last_stats(self._ctx, C.byref(s)), "kt_last_stats")
@@ -347,6 +374,15 @@ def trace_ray(view: View, alpha: float, beta: float, opts: TraceOpts | None = No
 def trace_looks(view: View, looks: Iterable[Sequence[float]], opts: TraceOpts | None = None) -> dict:
     """trajectory::traceRay counterpart: unit look vectors of a pinhole view's ZAMO frame."""
     return to_numpy(_tracer().trace_params(view, np.asarray(list(looks), dtype=np.float64), opts))
+
+
+def blackbody_table(n: int = 1024) -> np.ndarray:
+    """kt_blackbody_table: (n, 3) linear-sRGB blackbody colours over log10 T in [2.5, 5.3]."""
+    t = np.zeros(3 * n, np.float32)
+    rc = K.lib().kt_blackbody_table(n, t.ctypes.data_as(C.POINTER(C.c_float)))
+    if rc != K.KT_OK:
+        raise RuntimeError(f"kt_blackbody_table failed with status {rc}")
+    return t.reshape(n, 3)
 
 
 def horizon_radius(a: float) -> float:

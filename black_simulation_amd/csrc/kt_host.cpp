@@ -18,6 +18,7 @@
 #include "kt_internal.hpp"
 #include "kt_surrogate.hpp"
 #include "kt_nn.hpp"
+#include "kt_shade.hpp"
 
 using ktd::ViewDev;
 
@@ -223,6 +224,8 @@ struct kt_ctx {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timed = false;
     kts::Surrogate* surrogate = nullptr;
+    float* d_blackbody = nullptr;  // shading: the blackbody colour table
+    std::vector<float> h_blackbody;
     // autotuned launch shape per precision (0 = not tuned)
     unsigned long long* stats_src = nullptr;  // counters of the last launch (a plan's or d_counter)
     int tuned_mode[3] = {0, 0, 0};  // 1 + KT_MODE_* once tuned
@@ -510,6 +513,7 @@ int kt_destroy(kt_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
     if (ctx->surrogate) kts::destroy(ctx->surrogate);
+    if (ctx->d_blackbody) (void)hipFree(ctx->d_blackbody);
     if (ctx->d_counter) (void)hipFree(ctx->d_counter);
     if (ctx->d_views) (void)hipFree(ctx->d_views);
     if (ctx->d_stage) (void)hipFree(ctx->d_stage);
@@ -816,6 +820,83 @@ int kt_plan_stats(kt_plan* p, kt_stats* s) {
     }
     *s = r;
     return KT_OK;
+}
+
+// ---- shading ---------------------------------------------------------------------------
+
+static constexpr int kBlackbodyEntries = 1024;
+
+void kt_shade_opts_default(kt_shade_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->disk_temperature = 10000.0;
+    o->disk_brightness = 1.0;
+    o->sky = KT_SKY_GRID;
+    o->sky_brightness = 1.0;
+    o->grid_line = 0.02;
+}
+
+int kt_blackbody_table(int n, float* rgb) {
+    if (n < 2 || !rgb) return KT_EINVAL;
+    ktsh::blackbody_table(n, rgb);
+    return KT_OK;
+}
+
+int kt_shade_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_rays_out* rays, const kt_shade_opts* opts,
+                   void* rgba, void* hip_stream) {
+    if (!ctx) return KT_EINVAL;
+    try {
+        if (!rays || !rgba) return fail(ctx, KT_EINVAL, "rays/rgba missing");
+        if (!rays->on_device) return fail(ctx, KT_EINVAL, "shading reads device outputs (on_device = 1)");
+        if (!rays->cls || !rays->theta_inf || !rays->phi_inf || !rays->r_hit || !rays->g)
+            return fail(ctx, KT_EINVAL, "shading needs cls, theta_inf, phi_inf, r_hit and g");
+        kt_shade_opts o;
+        kt_shade_opts_default(&o);
+        if (opts) o = *opts;
+        if (o.sky < KT_SKY_BLACK || o.sky > KT_SKY_IMAGE) return fail(ctx, KT_EINVAL, "unknown sky");
+        if (o.sky == KT_SKY_IMAGE && (!o.sky_image || o.sky_width < 1 || o.sky_height < 1))
+            return fail(ctx, KT_EINVAL, "image sky needs sky_image and its size");
+        if (!(o.disk_temperature > 0.0)) return fail(ctx, KT_EINVAL, "disk_temperature must be positive");
+        std::vector<ViewDev> vd;
+        long long total = 0;
+        if (int st = prepare_views(ctx, views, n_views, vd, total)) return st;
+        KT_HIP(hipSetDevice(ctx->device));
+        hipStream_t stream = static_cast<hipStream_t>(hip_stream);
+        if (!ctx->d_blackbody) {
+            ctx->h_blackbody.resize(3 * kBlackbodyEntries);
+            ktsh::blackbody_table(kBlackbodyEntries, ctx->h_blackbody.data());
+            KT_HIP(hipMalloc(&ctx->d_blackbody, sizeof(float) * ctx->h_blackbody.size()));
+            KT_HIP(hipMemcpy(ctx->d_blackbody, ctx->h_blackbody.data(), sizeof(float) * ctx->h_blackbody.size(),
+                             hipMemcpyHostToDevice));
+        }
+        if (ensure_views(ctx, (int)vd.size()) != KT_OK) return KT_ENOMEM;
+        KT_HIP(hipMemcpyAsync(ctx->d_views, vd.data(), sizeof(ViewDev) * vd.size(), hipMemcpyHostToDevice, stream));
+        ktsh::ShadeLaunch s{};
+        s.n = total;
+        s.views = ctx->d_views;
+        s.n_views = (int)vd.size();
+        s.cls = rays->cls;
+        s.theta = rays->theta_inf;
+        s.phi = rays->phi_inf;
+        s.r_hit = rays->r_hit;
+        s.g = rays->g;
+        s.table = ctx->d_blackbody;
+        s.table_n = kBlackbodyEntries;
+        s.disk_temperature = o.disk_temperature;
+        const float lum = ktsh::table_luminance(ctx->h_blackbody.data(), kBlackbodyEntries, o.disk_temperature);
+        s.disk_scale = (float)(o.disk_brightness / std::max((double)lum, 1e-30));
+        s.sky = o.sky;
+        s.sky_brightness = (float)o.sky_brightness;
+        s.grid_line = (float)o.grid_line;
+        s.sky_image = o.sky_image;
+        s.sky_w = o.sky_width;
+        s.sky_h = o.sky_height;
+        s.rgba = static_cast<__half*>(rgba);
+        hipError_t e = ktsh::shade(s, stream);
+        return e == hipSuccess ? KT_OK : hip_fail(ctx, e, "shade kernel launch");
+    } catch (const std::exception& ex) {
+        return fail(ctx, KT_EINVAL, ex.what());
+    }
 }
 
 static int ensure_surrogate(kt_ctx* ctx) {

@@ -224,6 +224,32 @@ int kt_surrogate_adam_step(kt_ctx* ctx, const float* grads, double lr, int t);
 double kt_surrogate_table_radius(kt_ctx* ctx, double a, double inclination, double psi);
 double kt_surrogate_shadow_radius(kt_ctx* ctx, double a, double inclination);
 
+/* ---- HDR colour of traced rays ---------------------------------------------------
+ * Replaces the colouring at the end of the simulator's trace (shaders/trace.frag:
+ * diskEmission 373-403, shiftedBlackbody 330-335, skyColor/gridSky/imageSky 1419-1458,
+ * output 1893-1906) for the traced per-ray outputs: disk rays get the thermal emission of
+ * a zero-torque thin disk (T^4 ~ r^-3 (1 - sqrt(r_in / r)), peak = disk_temperature)
+ * seen as a blackbody at g T, escaped rays the sky at their asymptotic direction,
+ * captured and unfinished rays black.  Output: 4 half floats (RGBA, A = 1) per ray. */
+enum { KT_SKY_BLACK = 0, KT_SKY_GRID = 1, KT_SKY_IMAGE = 2 };
+typedef struct kt_shade_opts {
+    double disk_temperature;   /* K at the emission peak (10000)                       */
+    double disk_brightness;    /* multiplies the disk colour (1)                        */
+    int sky;                   /* KT_SKY_* (grid)                                       */
+    double sky_brightness;     /* multiplies the sky colour (1)                         */
+    double grid_line;          /* grid sky: line half-width, in grid cells (0.02)       */
+    const float* sky_image;    /* KT_SKY_IMAGE: device, equirectangular RGB floats,     */
+    int sky_width, sky_height; /*   row 0 at latitude -90 deg, column 0 at longitude -pi */
+} kt_shade_opts;
+void kt_shade_opts_default(kt_shade_opts* opts);
+/* rays: the device outputs of a kt_trace_views call on the same views; rgba: device,
+ * 4 x uint16 (IEEE half) per ray */
+int kt_shade_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_rays_out* rays,
+                   const kt_shade_opts* opts, void* rgba, void* hip_stream);
+/* n linear-sRGB blackbody colours over log10 T in [2.5, 5.3], luminance 1 at 6500 K
+ * (src/blackbody.cpp:48-60) */
+int kt_blackbody_table(int n, float* rgb);
+
 /* Host-side closed forms (src/kerr.cpp:157-206, 296-299). */
 double kt_horizon_radius(double a);
 double kt_isco_radius(double a, int prograde);

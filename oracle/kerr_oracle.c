@@ -864,3 +864,108 @@ double ko_critical_radius(const double* alpha, const double* beta, int n, double
     }
     return best;
 }
+
+/* ---- HDR colour of a traced ray ------------------------------------------------------ */
+
+/* Wyman, Sloan and Shirley (2013) multi-lobe fit of the CIE 1931 2-degree observer, as
+   blackbody.cpp:16-26 uses it (lambda in nm) */
+static double ko_lobe(double x, double mu, double s1, double s2) {
+    double t = (x - mu) / (x < mu ? s1 : s2);
+    return exp(-0.5 * t * t);
+}
+
+void ko_planck_xyz(double T, double xyz[3]) {
+    const double h = 6.62607015e-34, c = 2.99792458e8, k = 1.380649e-23;
+    xyz[0] = xyz[1] = xyz[2] = 0.0;
+    for (int nm = 360; nm <= 830; nm += 2) {
+        double lam = nm * 1e-9, x = (double)nm;
+        double B = 2.0 * h * c * c / (lam * lam * lam * lam * lam) / expm1(h * c / (lam * k * T));
+        xyz[0] += B * (1.056 * ko_lobe(x, 599.8, 37.9, 31.0) + 0.362 * ko_lobe(x, 442.0, 16.0, 26.7) -
+                       0.065 * ko_lobe(x, 501.1, 20.4, 26.2));
+        xyz[1] += B * (0.821 * ko_lobe(x, 568.8, 46.9, 40.5) + 0.286 * ko_lobe(x, 530.9, 16.3, 31.1));
+        xyz[2] += B * (1.217 * ko_lobe(x, 437.0, 11.8, 36.0) + 0.681 * ko_lobe(x, 459.0, 26.0, 13.8));
+    }
+}
+
+void ko_blackbody_table(int n, float* rgb) {
+    double ref[3];
+    ko_planck_xyz(6500.0, ref);
+    for (int i = 0; i < n; ++i) {
+        double logT = KO_LOGT_MIN + (KO_LOGT_MAX - KO_LOGT_MIN) * i / (n - 1), v[3];
+        ko_planck_xyz(pow(10.0, logT), v);
+        /* XYZ -> linear sRGB (IEC 61966-2-1), negatives clipped (blackbody.cpp:36-41) */
+        double r = 3.2406 * v[0] - 1.5372 * v[1] - 0.4986 * v[2];
+        double g = -0.9689 * v[0] + 1.8758 * v[1] + 0.0415 * v[2];
+        double b = 0.0557 * v[0] - 0.2040 * v[1] + 1.0570 * v[2];
+        rgb[3 * i + 0] = (float)((r > 0.0 ? r : 0.0) / ref[1]);
+        rgb[3 * i + 1] = (float)((g > 0.0 ? g : 0.0) / ref[1]);
+        rgb[3 * i + 2] = (float)((b > 0.0 ? b : 0.0) / ref[1]);
+    }
+}
+
+/* the table at T, linearly interpolated in log10 T and clamped (trace.frag:308-313) */
+static void ko_table_rgb(const ko_shade* s, double T, double rgb[3]) {
+    double u = (log10(T > 1.0 ? T : 1.0) - KO_LOGT_MIN) / (KO_LOGT_MAX - KO_LOGT_MIN);
+    u = u < 0.0 ? 0.0 : (u > 1.0 ? 1.0 : u);
+    double x = u * (s->table_n - 1);
+    int i = (int)x;
+    if (i > s->table_n - 2) i = s->table_n - 2;
+    double f = x - i;
+    const float* a = s->table + 3 * i;
+    for (int k = 0; k < 3; ++k) rgb[k] = a[k] + f * ((double)a[k + 3] - a[k]);
+}
+
+static double ko_smoothstep(double e0, double e1, double x) {
+    double t = (x - e0) / (e1 - e0);
+    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    return t * t * (3.0 - 2.0 * t);
+}
+
+void ko_shade_ray(const ko_shade* s, int cls, double theta, double phi, double r_hit, double g, double r_in,
+                  double r_out, double rgb[3]) {
+    rgb[0] = rgb[1] = rgb[2] = 0.0;
+    if (cls == KO_ESCAPED) {
+        double d[3] = {sin(theta) * cos(phi), sin(theta) * sin(phi), cos(theta)};
+        double lon = atan2(d[1], d[0]), lat = asin(d[2] > 1.0 ? 1.0 : (d[2] < -1.0 ? -1.0 : d[2]));
+        if (s->sky == 1) { /* gridSky, trace.frag:1419-1436 */
+            static const double hue[4][3] = {{0.95, 0.3, 0.25}, {0.25, 0.75, 0.35}, {0.3, 0.45, 0.95},
+                                             {0.95, 0.8, 0.25}};
+            double u = lon / KO_PI * 12.0, v = lat / KO_PI * 12.0, cu = floor(u), cv = floor(v);
+            int quadrant = ((int)floor((lon + KO_PI) / (0.5 * KO_PI))) & 3;
+            double k = lat < 0.0 ? 0.55 : 1.0;
+            double parity = (cu + cv) - 2.0 * floor((cu + cv) * 0.5);
+            k *= 0.45 + 0.55 * parity;
+            double eu = 0.5 - fabs(u - cu - 0.5), ev = 0.5 - fabs(v - cv - 0.5);
+            double lu = ko_smoothstep(0.0, s->grid_line, eu), lv = ko_smoothstep(0.0, s->grid_line, ev);
+            k *= (0.15 + 0.85 * (lu < lv ? lu : lv)) * 0.6;
+            for (int c = 0; c < 3; ++c) rgb[c] = hue[quadrant][c] * k;
+        } else if (s->sky == 2) { /* imageSky, trace.frag:1438-1445, bilinear */
+            int W = s->sky_width, H = s->sky_height;
+            double x = (lon / (2.0 * KO_PI) + 0.5) * W - 0.5, y = (lat / KO_PI + 0.5) * H - 0.5;
+            y = y < 0.0 ? 0.0 : (y > H - 1 ? H - 1 : y);
+            double fx = floor(x), fy = floor(y), tx = x - fx, ty = y - fy;
+            int x0 = ((int)fx % W + W) % W, x1 = (x0 + 1) % W, y0 = (int)fy, y1 = y0 + 1 < H ? y0 + 1 : H - 1;
+            for (int c = 0; c < 3; ++c) {
+                double p00 = s->sky_image[3 * ((long)y0 * W + x0) + c], p01 = s->sky_image[3 * ((long)y0 * W + x1) + c];
+                double p10 = s->sky_image[3 * ((long)y1 * W + x0) + c], p11 = s->sky_image[3 * ((long)y1 * W + x1) + c];
+                rgb[c] = (1 - ty) * ((1 - tx) * p00 + tx * p01) + ty * ((1 - tx) * p10 + tx * p11);
+            }
+        }
+        for (int c = 0; c < 3; ++c) rgb[c] *= s->sky_brightness;
+    } else if (cls == KO_DISK) {
+        /* diskEmission (trace.frag:373-403): T^4 ~ x^-3 (1 - x^-1/2), peak (x = 49/36) at the
+           disk temperature; seen as a blackbody at g T; normalised by the luminance of the
+           disk temperature's colour; edges as trace.frag:396-398 */
+        double x = r_hit / r_in, peak = pow(36.0 / 49.0, 3.0) / 7.0;
+        double flux = pow(x, -3.0) * (1.0 - 1.0 / sqrt(x));
+        double T = s->disk_temperature * pow((flux > 0.0 ? flux : 0.0) / peak, 0.25), bb[3], ref[3];
+        ko_table_rgb(s, T * g, bb);
+        ko_table_rgb(s, s->disk_temperature, ref);
+        double lum = 0.2126 * ref[0] + 0.7152 * ref[1] + 0.0722 * ref[2];
+        double edges = (1.0 - ko_smoothstep(0.7 * r_out, r_out, r_hit)) * ko_smoothstep(r_in, 1.03 * r_in, r_hit);
+        for (int c = 0; c < 3; ++c) rgb[c] = bb[c] * s->disk_brightness / (lum > 1e-30 ? lum : 1e-30) * edges;
+    }
+    /* trace.frag:1905-1906: non-finite -> black, capped at 30000 */
+    if (!(fabs(rgb[0]) <= 1e300 && fabs(rgb[1]) <= 1e300 && fabs(rgb[2]) <= 1e300)) rgb[0] = rgb[1] = rgb[2] = 0.0;
+    for (int c = 0; c < 3; ++c) rgb[c] = rgb[c] < 30000.0 ? rgb[c] : 30000.0;
+}

@@ -142,6 +142,27 @@ double ko_critical_radius(const double* alpha, const double* beta, int n, double
 /* the controller's deterministic x^(-1/5) */
 double ko_pow_m15(double x);
 
+/* ---- HDR colour of a traced ray (trace.frag:330-335, 373-403, 1419-1458, 1893-1906;
+   blackbody.cpp:16-60) ---- */
+#define KO_LOGT_MIN 2.5
+#define KO_LOGT_MAX 5.3
+/* CIE XYZ of Planck radiance at T, 360-830 nm in 2 nm steps (blackbody.cpp:16-41) */
+void ko_planck_xyz(double T, double xyz[3]);
+/* n linear-sRGB triples over log10 T in [2.5, 5.3], luminance 1 at 6500 K (blackbody.cpp:48-60) */
+void ko_blackbody_table(int n, float* rgb);
+typedef struct ko_shade {
+    double disk_temperature, disk_brightness;
+    int sky;                      /* 0 black, 1 grid, 2 image */
+    double sky_brightness, grid_line;
+    const float* sky_image;       /* equirectangular RGB, row 0 at latitude -90 deg */
+    int sky_width, sky_height;
+    const float* table;           /* ko_blackbody_table output */
+    int table_n;
+} ko_shade;
+/* colour of one ray from its traced class, sky angles, disk radius and g */
+void ko_shade_ray(const ko_shade* s, int cls, double theta, double phi, double r_hit, double g, double r_in,
+                  double r_out, double rgb[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,6 +90,12 @@ def build() -> None:
 _lib = None
 
 
+class Shade(C.Structure):
+    _fields_ = [("disk_temperature", C.c_double), ("disk_brightness", C.c_double), ("sky", C.c_int),
+                ("sky_brightness", C.c_double), ("grid_line", C.c_double), ("sky_image", C.c_void_p),
+                ("sky_width", C.c_int), ("sky_height", C.c_int), ("table", C.c_void_p), ("table_n", C.c_int)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -127,6 +133,9 @@ def lib():
             ("ko_critical_curve", i, [d, d, i, P(d), P(d), i]),
             ("ko_critical_radius", d, [P(d), P(d), i, d]),
             ("ko_pow_m15", d, [d]),
+            ("ko_planck_xyz", None, [d, P(d)]),
+            ("ko_blackbody_table", None, [i, P(C.c_float)]),
+            ("ko_shade_ray", None, [P(Shade), i, d, d, d, d, d, d, P(d)]),
         ]:
             f = getattr(L, name)
             f.restype = res
@@ -227,4 +236,38 @@ def trace_params(view: View, opts: Opts, params) -> dict:
         if res.cls == DISK:
             out["r_hit"][i] = res.r_hit
             out["g"][i] = res.g
+    return out
+
+
+def planck_xyz(T: float) -> np.ndarray:
+    out = np.zeros(3)
+    lib().ko_planck_xyz(T, _dp(out))
+    return out
+
+
+def blackbody_table(n: int = 1024) -> np.ndarray:
+    t = np.zeros(3 * n, np.float32)
+    lib().ko_blackbody_table(n, t.ctypes.data_as(C.POINTER(C.c_float)))
+    return t.reshape(n, 3)
+
+
+def shade(rays: dict, r_in, r_out, disk_temperature=10000.0, disk_brightness=1.0, sky=1, sky_brightness=1.0,
+          grid_line=0.02, sky_image=None, table_n=1024) -> np.ndarray:
+    """Colour of every ray (rays: cls, theta_inf, phi_inf, r_hit, g as numpy arrays; r_in,
+    r_out per ray or scalars) -> (n, 3) float64."""
+    table = blackbody_table(table_n)
+    img = None if sky_image is None else np.ascontiguousarray(sky_image, np.float32)
+    s = Shade(disk_temperature, disk_brightness, sky, sky_brightness, grid_line,
+              None if img is None else img.ctypes.data, 0 if img is None else img.shape[1],
+              0 if img is None else img.shape[0], table.ctypes.data, table_n)
+    n = len(rays["cls"])
+    r_in = np.broadcast_to(np.asarray(r_in, np.float64), (n,))
+    r_out = np.broadcast_to(np.asarray(r_out, np.float64), (n,))
+    out = np.zeros((n, 3))
+    buf = np.zeros(3)
+    L = lib()
+    for q in range(n):
+        L.ko_shade_ray(C.byref(s), int(rays["cls"][q]) & 7, float(rays["theta_inf"][q]), float(rays["phi_inf"][q]),
+                       float(rays["r_hit"][q]), float(rays["g"][q]), float(r_in[q]), float(r_out[q]), _dp(buf))
+        out[q] = buf
     return out
