@@ -41,6 +41,7 @@ MLP_FLOP_PER_RAY = 2 * (5 * 256 + 3 * 256 * 256 + 256 * 8)  # 399,872
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 BF16_MFMA_PEAK_TFLOPS = 2500.0
+HBM_PEAK_GBS = 8000.0
 
 
 def cpu_baseline(view, seconds_target: float = 15.0) -> dict:
@@ -199,6 +200,22 @@ def main():
         fallback.append(st["fallback_rays"])
     n_attr = len(integ_ms)
 
+    # HDR colour of the last attribution frame (an HBM-bound pass: per ray it reads the
+    # class byte and four doubles and writes four halves)
+    k_last = frames[:: max(1, len(frames) // 24)][-1]
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    tr.shade(frame_views[k_last], orb.out)
+    ev0.record()
+    for _ in range(10):
+        tr.shade(frame_views[k_last], orb.out)
+    ev1.record()
+    torch.cuda.synchronize()
+    shade_ms = ev0.elapsed_time(ev1) / 10
+    shade_bytes = rays * (1 + 4 * 8 + 4 * 2)
+    shade = {"kernel": "k_shade", "kernel_ms": round(shade_ms, 4), "bound": "hbm",
+             "achieved": round(shade_bytes / (shade_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(shade_bytes / (shade_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_ray": 41}
+
     # accuracy of the chosen path against the FP64 render of every timed frame
     acc = None
     if ref is not orb:
@@ -285,6 +302,7 @@ def main():
                    "host_outputs_value": round(host_value, 3) if host_value else None,
                    "fallback_fraction": round(sum(fallback) / n_attr / rays, 4) if args.path == "surrogate" else None,
                    "accuracy_vs_fp64": acc,
+                   "shade": shade,
                    "launch": {"mode": ["persistent", "one-shot"][st["mode"]], "block": st["block"],
                               "grid": st["grid"], "waves_per_cu": st["waves_per_cu"],
                               "steps_per_dequeue": st["steps_per_dequeue"], "waves_per_simd": st["waves_per_simd"]}},
