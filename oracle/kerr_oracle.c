@@ -9,6 +9,7 @@
 #include <math.h>
 #include <pthread.h>
 #include <stddef.h>
+#include <stdlib.h>
 #include <string.h>
 
 /* ------------------------------------------------------------------------- */
@@ -968,4 +969,304 @@ void ko_shade_ray(const ko_shade* s, int cls, double theta, double phi, double r
     /* trace.frag:1905-1906: non-finite -> black, capped at 30000 */
     if (!(fabs(rgb[0]) <= 1e300 && fabs(rgb[1]) <= 1e300 && fabs(rgb[2]) <= 1e300)) rgb[0] = rgb[1] = rgb[2] = 0.0;
     for (int c = 0; c < 3; ++c) rgb[c] = rgb[c] < 30000.0 ? rgb[c] : 30000.0;
+}
+
+/* ---- EHT radiative transfer (eht.h:22-122; eht.cpp:98-316, 321-354, 432-457) ------------
+   Physical constants in cgs as eht.h:22-33.  The polynomial approximations of I0, I1, K0
+   and K1 are Abramowitz and Stegun 9.8.1-9.8.8, as eht.cpp:98-152 uses them. */
+#define KO_QE 4.80320425e-10
+#define KO_ME 9.1093837015e-28
+#define KO_MP 1.67262192369e-24
+#define KO_CL 2.99792458e10
+#define KO_KB 1.380649e-16
+#define KO_HP 6.62607015e-27
+#define KO_MSUN_CM 1.476625e5
+#define KO_JY 1e-23
+#define KO_UAS (KO_PI / 180.0 / 3600.0 / 1e6)
+#define KO_EHT_START 1000.0 /* eht.cpp:14 */
+#define KO_EHT_LIGHT 60.0   /* eht.cpp:15 */
+#define KO_EHT_DEPTH 30.0   /* eht.cpp:16 */
+
+static double ko_i0(double x) {
+    double ax = fabs(x);
+    if (ax < 3.75) {
+        double y = (x / 3.75) * (x / 3.75);
+        return 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 + y * (0.2659732 + y * (0.360768e-1 + y * 0.45813e-2)))));
+    }
+    double y = 3.75 / ax;
+    return (exp(ax) / sqrt(ax)) * (0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 + y * (-0.157565e-2 + y * (0.916281e-2 +
+           y * (-0.2057706e-1 + y * (0.2635537e-1 + y * (-0.1647633e-1 + y * 0.392377e-2))))))));
+}
+
+static double ko_i1(double x) {
+    double ax = fabs(x), res;
+    if (ax < 3.75) {
+        double y = (x / 3.75) * (x / 3.75);
+        res = ax * (0.5 + y * (0.87890594 + y * (0.51498869 + y * (0.15084934 + y * (0.2658733e-1 + y * (0.301532e-2 + y * 0.32411e-3))))));
+    } else {
+        double y = 3.75 / ax;
+        double t = 0.2282967e-1 + y * (-0.2895312e-1 + y * (0.1787654e-1 - y * 0.420059e-2));
+        t = 0.39894228 + y * (-0.3988024e-1 + y * (-0.362018e-2 + y * (0.163801e-2 + y * (-0.1031555e-1 + y * t))));
+        res = t * exp(ax) / sqrt(ax);
+    }
+    return x < 0.0 ? -res : res;
+}
+
+/* K0, K1 times exp(x) */
+static double ko_k0s(double x) {
+    if (x <= 2.0) {
+        double y = x * x / 4.0;
+        return exp(x) * ((-log(x / 2.0) * ko_i0(x)) + (-0.57721566 + y * (0.42278420 + y * (0.23069756 + y * (0.3488590e-1 +
+               y * (0.262698e-2 + y * (0.10750e-3 + y * 0.74e-5)))))));
+    }
+    double y = 2.0 / x;
+    return (1.0 / sqrt(x)) * (1.25331414 + y * (-0.7832358e-1 + y * (0.2189568e-1 + y * (-0.1062446e-1 + y * (0.587872e-2 +
+           y * (-0.251540e-2 + y * 0.53208e-3))))));
+}
+
+static double ko_k1s(double x) {
+    if (x <= 2.0) {
+        double y = x * x / 4.0;
+        return exp(x) * ((log(x / 2.0) * ko_i1(x)) + (1.0 / x) * (1.0 + y * (0.15443144 + y * (-0.67278579 + y * (-0.18156897 +
+               y * (-0.1919402e-1 + y * (-0.110404e-2 + y * (-0.4686e-4))))))));
+    }
+    double y = 2.0 / x;
+    return (1.0 / sqrt(x)) * (1.25331414 + y * (0.23498619 + y * (-0.3655620e-1 + y * (0.1504268e-1 + y * (-0.780353e-2 +
+           y * (0.325614e-2 + y * (-0.68245e-3)))))));
+}
+
+static double ko_k2s(double x) { return ko_k0s(x) + 2.0 / x * ko_k1s(x); } /* K2 = K0 + (2/x) K1 */
+
+double ko_bessel_k(int which, double x) {
+    double s = which == 0 ? ko_k0s(x) : which == 1 ? ko_k1s(x) : ko_k2s(x);
+    return s * exp(-x);
+}
+
+/* Leung, Gammie and Noble (2011) eq. 72, eht.cpp:163-177 */
+double ko_synchrotron_emissivity(double nu, double ne, double thetaE, double B, double sinThetaB) {
+    if (ne <= 0.0 || thetaE < 0.05 || B <= 0.0 || sinThetaB <= 0.0) return 0.0;
+    double nuC = KO_QE * B / (2.0 * KO_PI * KO_ME * KO_CL);
+    double nuS = (2.0 / 9.0) * nuC * thetaE * thetaE * sinThetaB;
+    double X = nu / nuS;
+    double shape = sqrt(X) + pow(2.0, 11.0 / 12.0) * pow(X, 1.0 / 6.0);
+    double inv = 1.0 / thetaE;
+    double expo = inv - cbrt(X); /* 1/K2(1/thetaE) = exp(1/thetaE)/scaledK2, with exp(-X^1/3) */
+    if (expo < -700.0) return 0.0;
+    return ne * sqrt(2.0) * KO_PI * KO_QE * KO_QE * nuS / (3.0 * ko_k2s(inv) * KO_CL) * shape * shape * exp(expo);
+}
+
+double ko_planck_nu(double nu, double T) {
+    double x = KO_HP * nu / (KO_KB * T);
+    return 2.0 * KO_HP * nu * nu * nu / (KO_CL * KO_CL) / expm1(x);
+}
+
+/* the flow's plunge from the prograde ISCO, per spin (eht.cpp:183-192, 222-233) */
+typedef struct ko_plunge {
+    double a, isco, horizon, E, L, start, start_speed;
+} ko_plunge;
+
+static double plunge_ur(const ko_plunge* q, double r) {
+    double delta = r * r - 2.0 * r + q->a * q->a, x = q->L - q->a * q->E;
+    double P = q->E * (r * r + q->a * q->a) - q->a * q->L;
+    return sqrt(fmax(P * P - delta * (r * r + x * x), 0.0)) / (r * r);
+}
+
+static void plunge_init(ko_plunge* q, double a) {
+    q->a = a;
+    q->isco = ko_isco_radius(a, 1);
+    q->horizon = ko_horizon_radius(a);
+    /* circular equatorial orbit at the ISCO (kerr.cpp:308-319, prograde) */
+    double r = q->isco, sr = sqrt(r), r32 = r * sr;
+    double den = pow(r, 0.75) * sqrt(r32 - 3.0 * sr + 2.0 * a);
+    q->E = (r32 - 2.0 * sr + a) / den;
+    q->L = (r * r - 2.0 * a * sr + a * a) / den;
+    q->start = fmax(q->isco - 0.25, 0.5 * (q->isco + q->horizon));
+    q->start_speed = plunge_ur(q, q->start);
+}
+
+/* n_e, T_e, B at (r, mu = cos theta) (eht.cpp:196-215) */
+static void plasma_mu(const ko_flow* f, const ko_plunge* q, double r, double mu, double out[3]) {
+    double z = r * mu, rho = fmax(r * sqrt(fmax(1.0 - mu * mu, 0.0)), 1e-6);
+    double h = fmax(f->height, 0.05) * rho;
+    double radial = pow(r, -1.1);
+    if (r < q->isco && r < q->start) {
+        double here = plunge_ur(q, r);
+        if (here > 0.0) radial = fmin(radial, pow(q->start, -1.1) * (q->start * q->start * q->start_speed) / (r * r * here));
+    }
+    out[0] = f->n0 * radial * exp(-z * z / (2.0 * h * h));
+    out[1] = f->T0 * pow(r, -0.84);
+    out[2] = sqrt(8.0 * KO_PI * out[0] * KO_MP * KO_CL * KO_CL / (6.0 * f->beta * r));
+}
+
+/* the gas's u^mu (BL, contravariant) at (r, mu) (eht.cpp:217-253) */
+static void flow_mu(const ko_flow* f, const ko_plunge* q, double r, double mu, double u[4]) {
+    double a = q->a, s2 = 1.0 - mu * mu, c2 = mu * mu;
+    double sigma = r * r + a * a * c2, delta = r * r - 2.0 * r + a * a;
+    double A = (r * r + a * a) * (r * r + a * a) - a * a * delta * s2;
+    double gtt = -(1.0 - 2.0 * r / sigma), gtp = -2.0 * a * r * s2 / sigma, gpp = A * s2 / sigma, grr = sigma / delta;
+    double omega, ur = 0.0;
+    if (r >= q->isco) {
+        omega = f->keplerian / (pow(r, 1.5) + a);
+    } else {
+        double x = q->L - a * q->E, P = q->E * (r * r + a * a) - a * q->L;
+        double R = fmax(P * P - delta * (r * r + x * x), 0.0);
+        double ut = ((r * r + a * a) * P / delta + a * x) / (r * r);
+        double up = (a * P / delta + x) / (r * r);
+        omega = up / ut;
+        ur = -sqrt(R) / (r * r);
+    }
+    double den = gtt + 2.0 * omega * gtp + omega * omega * gpp;
+    double num = -1.0 - grr * ur * ur;
+    if (!(den < 0.0) || !(num < 0.0)) { /* the local ZAMO */
+        omega = -gtp / gpp;
+        ur = 0.0;
+        den = gtt + 2.0 * omega * gtp + omega * omega * gpp;
+        num = -1.0;
+    }
+    double ut = sqrt(num / den);
+    u[0] = ut;
+    u[1] = ur;
+    u[2] = 0.0;
+    u[3] = omega * ut;
+}
+
+void ko_plasma_at(const ko_flow* f, double r, double theta, double a, double out[3]) {
+    ko_plunge q;
+    plunge_init(&q, a);
+    plasma_mu(f, &q, r, cos(theta), out);
+}
+
+void ko_flow_velocity(const ko_flow* f, double a, double r, double theta, double u[4]) {
+    ko_plunge q;
+    plunge_init(&q, a);
+    flow_mu(f, &q, r, cos(theta), u);
+}
+
+/* One pixel: the Bardeen ray from r = 1000 M (eht.cpp:262-271) stepped by the DP5 engine;
+   after every accepted step inside r = 60 M the transfer of eht.cpp:290-316 over that step
+   (emission and absorption at the step's end, affine length from the mean Sigma). */
+void ko_eht_trace(const ko_eht_view* v, const ko_flow* f, const ko_opts* o, double alpha, double beta,
+                  ko_eht_ray* out) {
+    ko_view kv;
+    memset(&kv, 0, sizeof kv);
+    kv.camera_kind = KO_CAMERA_BARDEEN;
+    kv.spin = v->a;
+    kv.r_cam = KO_EHT_START;
+    kv.theta_cam = v->inclination;
+    ko_photon p;
+    double y[5], yn[5], k[7][5];
+    ko_launch_bardeen(&kv, alpha, beta, &p, y);
+    ko_plunge q;
+    plunge_init(&q, v->a);
+    const double a = v->a, r_cap = q.horizon * (1.0 + 1e-4), len_cm = KO_MSUN_CM * v->mass_solar;
+    out->intensity = 0.0;
+    out->captured = 0;
+    out->min_radius = y[0];
+    double depth = 0.0;
+    int steps = 0, rejected = 0;
+    double h = 0.01 / KO_EHT_START;
+    ko_rhs(&p, y, k[0]);
+    while (steps < o->max_steps) {
+        const double err = dp5_main(&p, o, y, h, k, yn);
+        ++steps;
+        if (!(err <= 1.0)) {
+            h *= step_factor(err, 1);
+            rejected = 1;
+            continue;
+        }
+        const double fac = step_factor(err, rejected);
+        rejected = 0;
+        if (p.L == 0.0 && y[1] * y[4] > 0.0 && yn[1] * yn[4] < 0.0) yn[2] += KO_PI;
+        project(&p, yn);
+        if (!(fabs(yn[0]) < 1e300) || !(fabs(yn[1]) < 1e300) || yn[0] < r_cap) {
+            out->captured = 1;
+            break;
+        }
+        if (yn[0] < out->min_radius) out->min_radius = yn[0];
+        if (yn[0] > KO_EHT_START * 1.01 && yn[3] > 0.0) break;
+        if (yn[0] <= KO_EHT_LIGHT) {
+            const double r = yn[0], mu = yn[1];
+            const double sig_b = y[0] * y[0] + a * a * y[1] * y[1], sig_a = r * r + a * a * mu * mu;
+            const double dl = 0.5 * (sig_a + sig_b) * h * len_cm; /* affine length of the step, cm */
+            double pl[3], u[4];
+            plasma_mu(f, &q, r, mu, pl);
+            flow_mu(f, &q, r, mu, u);
+            const double delta = r * r - 2.0 * r + a * a;
+            /* -p.u with p_t = -1, p_phi = L, p_r = -w_r / Delta (towards the camera); u^theta = 0 */
+            const double mpu = u[0] - p.L * u[3] + yn[3] / delta * u[1];
+            if (mpu > 0.0) {
+                const double g = 1.0 / mpu, nu_f = v->frequency / g;
+                const double thetaE = KO_KB * pl[1] / (KO_ME * KO_CL * KO_CL);
+                const double j = ko_synchrotron_emissivity(nu_f, pl[0], thetaE, pl[2], f->sin_theta_b);
+                if (j > 0.0) {
+                    const double absorb = j / ko_planck_nu(nu_f, pl[1]);
+                    const double dtau = absorb * dl / g;
+                    const double add = dtau < 1e-6 ? g * g * j * dl : g * g * g * (j / absorb) * -expm1(-dtau);
+                    out->intensity += exp(-depth) * add;
+                    depth += dtau;
+                    if (depth > KO_EHT_DEPTH) break;
+                }
+            }
+        }
+        for (int i = 0; i < 5; ++i) y[i] = yn[i];
+        for (int i = 0; i < 5; ++i) k[0][i] = k[6][i];
+        k[0][0] = y[3];
+        k[0][1] = y[4];
+        h *= fac;
+    }
+    out->steps = steps;
+}
+
+/* eht.cpp:321-347: pixel centres, rows from the top */
+void ko_eht_render(const ko_eht_view* v, const ko_flow* f, const ko_opts* o, double* intensity, uint8_t* shadow) {
+    const double px = 2.0 * v->half_width / v->size;
+    for (int row = 0; row < v->size; ++row)
+        for (int col = 0; col < v->size; ++col) {
+            ko_eht_ray ray;
+            ko_eht_trace(v, f, o, -v->half_width + (col + 0.5) * px, v->half_width - (row + 0.5) * px, &ray);
+            intensity[(long)row * v->size + col] = ray.intensity;
+            if (shadow) shadow[(long)row * v->size + col] = (uint8_t)ray.captured;
+        }
+}
+
+/* eht.cpp:349-354: Jy, the pixel's solid angle from theta_g in microarcseconds */
+double ko_eht_flux(const double* intensity, int size, double half_width, double theta_guas) {
+    double pix = 2.0 * half_width / size, sa = pow(pix * theta_guas * KO_UAS, 2.0), sum = 0.0;
+    for (long i = 0; i < (long)size * size; ++i) sum += intensity[i];
+    return sum * sa / KO_JY;
+}
+
+/* eht.cpp:432-457: the density n0 that gives target_jy, by a secant in log n0 against log
+   flux on 48 x 48 renders of the view (a dark start is first brightened tenfold at a time) */
+double ko_eht_calibrate(const ko_eht_view* v, const ko_flow* flow, const ko_opts* o, double target_jy,
+                        double theta_guas) {
+    ko_eht_view small = *v;
+    small.size = 48;
+    ko_flow f = *flow;
+    const long n = (long)small.size * small.size;
+    double* img = (double*)malloc(sizeof(double) * n);
+    if (!img) return NAN;
+#define FLUX_AT(x) (f.n0 = (x), ko_eht_render(&small, &f, o, img, NULL), ko_eht_flux(img, small.size, small.half_width, theta_guas))
+    double n0 = f.n0, f0 = FLUX_AT(n0);
+    for (int k = 0; k < 8 && f0 <= 0.0; ++k) {
+        n0 *= 10.0;
+        f0 = FLUX_AT(n0);
+    }
+    if (f0 <= 0.0) {
+        free(img);
+        return n0;
+    }
+    double n1 = n0 * target_jy / f0, f1 = FLUX_AT(n1);
+    for (int k = 0; k < 8; ++k) {
+        if (f1 <= 0.0 || fabs(f1 / target_jy - 1.0) < 0.01) break;
+        const double slope = (log(f1) - log(f0)) / (log(n1) - log(n0));
+        const double next = exp(log(n1) + (log(target_jy) - log(f1)) / (fabs(slope) > 0.05 ? slope : 1.0));
+        n0 = n1;
+        f0 = f1;
+        n1 = next < n0 / 30.0 ? n0 / 30.0 : next > n0 * 30.0 ? n0 * 30.0 : next;
+        f1 = FLUX_AT(n1);
+    }
+#undef FLUX_AT
+    free(img);
+    return n1;
 }

@@ -159,6 +159,32 @@ typedef struct ko_shade {
     const float* table;           /* ko_blackbody_table output */
     int table_n;
 } ko_shade;
+/* ---- EHT radiative transfer (eht.h:88-140; eht.cpp:98-354) ---- */
+typedef struct ko_flow {
+    double n0, T0, beta, keplerian, height, sin_theta_b; /* eht::Flow (eht.h:97-104) */
+} ko_flow;
+typedef struct ko_eht_view {
+    double a, inclination, half_width; /* eht::View (eht.h:113-120) */
+    int size;
+    double frequency, mass_solar;
+} ko_eht_view;
+typedef struct ko_eht_ray {
+    double intensity, min_radius; /* erg s^-1 cm^-2 Hz^-1 sr^-1; M */
+    int captured, steps;
+} ko_eht_ray;
+double ko_bessel_k(int which, double x); /* K0, K1, K2 (A&S 9.8.5-9.8.8) */
+double ko_synchrotron_emissivity(double nu, double ne, double thetaE, double B, double sinThetaB);
+double ko_planck_nu(double nu, double T);
+void ko_plasma_at(const ko_flow* f, double r, double theta, double a, double out[3]); /* ne, Te, B */
+void ko_flow_velocity(const ko_flow* f, double a, double r, double theta, double u[4]);
+void ko_eht_trace(const ko_eht_view* v, const ko_flow* f, const ko_opts* o, double alpha, double beta,
+                  ko_eht_ray* out);
+void ko_eht_render(const ko_eht_view* v, const ko_flow* f, const ko_opts* o, double* intensity, uint8_t* shadow);
+double ko_eht_flux(const double* intensity, int size, double half_width, double theta_guas);
+/* eht.cpp:432-457: secant in log n0 on 48 x 48 renders */
+double ko_eht_calibrate(const ko_eht_view* v, const ko_flow* flow, const ko_opts* o, double target_jy,
+                        double theta_guas);
+
 /* colour of one ray from its traced class, sky angles, disk radius and g */
 void ko_shade_ray(const ko_shade* s, int cls, double theta, double phi, double r_hit, double g, double r_in,
                   double r_out, double rgb[3]);

@@ -96,6 +96,19 @@ class Shade(C.Structure):
                 ("sky_width", C.c_int), ("sky_height", C.c_int), ("table", C.c_void_p), ("table_n", C.c_int)]
 
 
+class Flow(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("n0", "T0", "beta", "keplerian", "height", "sin_theta_b")]
+
+
+class EhtView(C.Structure):
+    _fields_ = [("a", C.c_double), ("inclination", C.c_double), ("half_width", C.c_double), ("size", C.c_int),
+                ("frequency", C.c_double), ("mass_solar", C.c_double)]
+
+
+class EhtRay(C.Structure):
+    _fields_ = [("intensity", C.c_double), ("min_radius", C.c_double), ("captured", C.c_int), ("steps", C.c_int)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -136,6 +149,15 @@ def lib():
             ("ko_planck_xyz", None, [d, P(d)]),
             ("ko_blackbody_table", None, [i, P(C.c_float)]),
             ("ko_shade_ray", None, [P(Shade), i, d, d, d, d, d, d, P(d)]),
+            ("ko_bessel_k", d, [i, d]),
+            ("ko_synchrotron_emissivity", d, [d, d, d, d, d]),
+            ("ko_planck_nu", d, [d, d]),
+            ("ko_plasma_at", None, [P(Flow), d, d, d, P(d)]),
+            ("ko_flow_velocity", None, [P(Flow), d, d, d, P(d)]),
+            ("ko_eht_trace", None, [P(EhtView), P(Flow), P(Opts), d, d, P(EhtRay)]),
+            ("ko_eht_render", None, [P(EhtView), P(Flow), P(Opts), P(d), P(C.c_uint8)]),
+            ("ko_eht_flux", d, [P(d), i, d, d]),
+            ("ko_eht_calibrate", d, [P(EhtView), P(Flow), P(Opts), d, d]),
         ]:
             f = getattr(L, name)
             f.restype = res
@@ -271,3 +293,46 @@ def shade(rays: dict, r_in, r_out, disk_temperature=10000.0, disk_brightness=1.0
                        float(rays["r_hit"][q]), float(rays["g"][q]), float(r_in[q]), float(r_out[q]), _dp(buf))
         out[q] = buf
     return out
+
+
+def flow(n0=1e5, T0=4e11, beta=10.0, keplerian=0.9, height=1.0, sin_theta_b=0.8165) -> Flow:
+    return Flow(n0, T0, beta, keplerian, height, sin_theta_b)
+
+
+def eht_view(a=0.9, inclination=0.3, half_width=16.0, size=128, frequency=230e9, mass_solar=6.5e9) -> EhtView:
+    return EhtView(a, inclination, half_width, size, frequency, mass_solar)
+
+
+def plasma_at(f: Flow, r: float, theta: float, a: float) -> np.ndarray:
+    out = np.zeros(3)
+    lib().ko_plasma_at(C.byref(f), r, theta, a, _dp(out))
+    return out
+
+
+def flow_velocity(f: Flow, a: float, r: float, theta: float) -> np.ndarray:
+    out = np.zeros(4)
+    lib().ko_flow_velocity(C.byref(f), a, r, theta, _dp(out))
+    return out
+
+
+def eht_trace(v: EhtView, f: Flow, opts: Opts, alpha: float, beta: float) -> EhtRay:
+    r = EhtRay()
+    lib().ko_eht_trace(C.byref(v), C.byref(f), C.byref(opts), alpha, beta, C.byref(r))
+    return r
+
+
+def eht_render(v: EhtView, f: Flow, opts: Opts):
+    n = v.size * v.size
+    inten = np.zeros(n)
+    shadow = np.zeros(n, np.uint8)
+    lib().ko_eht_render(C.byref(v), C.byref(f), C.byref(opts), _dp(inten), shadow.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return inten.reshape(v.size, v.size), shadow.reshape(v.size, v.size)
+
+
+def eht_flux(intensity: np.ndarray, half_width: float, theta_guas: float) -> float:
+    a = np.ascontiguousarray(intensity, np.float64)
+    return lib().ko_eht_flux(_dp(a), a.shape[0], half_width, theta_guas)
+
+
+def eht_calibrate(v: EhtView, f: Flow, opts: Opts, target_jy: float, theta_guas: float) -> float:
+    return lib().ko_eht_calibrate(C.byref(v), C.byref(f), C.byref(opts), target_jy, theta_guas)

@@ -137,3 +137,66 @@ void ref_trajectory_trace(double a, double r, double theta, double phi, const do
 double ref_bending_angle(double R, double alpha) { return central::bendingAngle(R, alpha); }
 
 }  // extern "C"
+
+// ---- EHT radiative transfer: the physics functions and traceRay with the light ----------
+
+static eht::Flow make_flow(const double* f) {
+    eht::Flow flow;
+    flow.n0 = f[0];
+    flow.T0 = f[1];
+    flow.beta = f[2];
+    flow.keplerian = f[3];
+    flow.height = f[4];
+    flow.sinThetaB = f[5];
+    return flow;
+}
+
+extern "C" {
+
+double ref_bessel_k(int which, double x) {
+    return which == 0 ? eht::besselK0(x) : which == 1 ? eht::besselK1(x) : eht::besselK2(x);
+}
+
+double ref_synchrotron_emissivity(double nu, double ne, double thetaE, double B, double s) {
+    return eht::synchrotronEmissivity(nu, ne, thetaE, B, s);
+}
+
+double ref_planck_nu(double nu, double T) { return eht::planck(nu, T); }
+
+void ref_plasma_at(const double* flow, double r, double theta, double a, double* out) {
+    eht::Plasma p = eht::plasmaAt(make_flow(flow), r, theta, a);
+    out[0] = p.ne;
+    out[1] = p.Te;
+    out[2] = p.B;
+}
+
+void ref_flow_velocity(const double* flow, double a, double r, double theta, double* u) {
+    eht::flowVelocity(make_flow(flow), a, r, theta, u);
+}
+
+// out = {intensity, captured, minRadius}
+void ref_eht_trace_light(double a, double inclination, double frequency, double mass_solar, const double* flow,
+                         double alpha, double beta, double* out) {
+    eht::View v;
+    v.a = a;
+    v.inclination = inclination;
+    v.frequency = frequency;
+    v.massSolar = mass_solar;
+    eht::Ray ray = eht::traceRay(v, make_flow(flow), alpha, beta, true);
+    out[0] = ray.intensity;
+    out[1] = ray.captured ? 1.0 : 0.0;
+    out[2] = ray.minRadius;
+}
+
+double ref_eht_calibrate(double a, double inclination, double half_width, double frequency, double mass_solar,
+                         const double* flow, double target_jy, double theta_guas) {
+    eht::View v;
+    v.a = a;
+    v.inclination = inclination;
+    v.halfWidth = half_width;
+    v.frequency = frequency;
+    v.massSolar = mass_solar;
+    return eht::calibrate(v, make_flow(flow), target_jy, theta_guas, 8, nullptr);
+}
+
+}  // extern "C"
