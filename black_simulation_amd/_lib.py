@@ -126,6 +126,20 @@ class KtSurrogateReport(C.Structure):
                                           "rhit_rel_median", "rhit_rel_p99")]
 
 
+class KtEhtFlow(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("n0", "T0", "beta", "keplerian", "height", "sin_theta_b")]
+
+
+class KtEhtView(C.Structure):
+    _fields_ = [("a", C.c_double), ("inclination", C.c_double), ("half_width", C.c_double), ("size", C.c_int),
+                ("frequency", C.c_double), ("mass_solar", C.c_double)]
+
+
+class KtEhtOut(C.Structure):
+    _fields_ = [("intensity", C.c_void_p), ("shadow", C.c_void_p), ("min_radius", C.c_void_p), ("steps", C.c_void_p),
+                ("on_device", C.c_int)]
+
+
 class KtShadeOpts(C.Structure):
     _fields_ = [
         ("disk_temperature", C.c_double),
@@ -174,6 +188,15 @@ SIGNATURES = {
     "kt_shade_views": (C.c_int, [C.c_void_p, _P(KtView), C.c_int, _P(KtRaysOut), _P(KtShadeOpts), C.c_void_p,
                                  C.c_void_p]),
     "kt_blackbody_table": (C.c_int, [C.c_int, _P(C.c_float)]),
+    "kt_eht_flow_default": (None, [_P(KtEhtFlow)]),
+    "kt_eht_view_default": (None, [_P(KtEhtView)]),
+    "kt_eht_render": (C.c_int, [C.c_void_p, _P(KtEhtView), _P(KtEhtFlow), _P(KtTraceOpts), _P(KtEhtOut), C.c_void_p]),
+    "kt_eht_trace_rays": (C.c_int, [C.c_void_p, _P(KtEhtView), _P(KtEhtFlow), C.c_void_p, C.c_longlong,
+                                    _P(KtTraceOpts), _P(KtEhtOut), C.c_void_p]),
+    "kt_eht_flux": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double, _P(C.c_double), C.c_void_p]),
+    "kt_eht_calibrate": (C.c_int, [C.c_void_p, _P(KtEhtView), _P(KtEhtFlow), C.c_double, C.c_double,
+                                   _P(KtTraceOpts), _P(C.c_double), C.c_void_p]),
+    "kt_eht_angular_scale": (C.c_double, [C.c_double, C.c_double]),
     "kt_horizon_radius": (C.c_double, [C.c_double]),
     "kt_isco_radius": (C.c_double, [C.c_double, C.c_int]),
     "kt_photon_orbit_radius": (C.c_double, [C.c_double, C.c_int]),

@@ -1,0 +1,38 @@
+// kt_eht.hpp -- EHT radiative transfer over the DP5 geodesic engine (kt_eht.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kt_device.hpp"
+
+namespace kte {
+
+// partial sums the flux reduction needs at most (device scratch of this many doubles)
+constexpr int kSumPartials = 1024;
+
+struct EhtLaunch {
+    long long n;          // rays: size * size pixels, or the explicit list's length
+    const double* rays;   // explicit (alpha, beta) pairs, or null for the image's pixels
+    int size;
+    double half_width, px;
+    // the view: spin, inclination (sin, cos and the cosine of the clamped theta), capture
+    // radius r+ (1 + 1e-4), M in cm, observing frequency (Hz)
+    double a, sinI, cosI, m0, r_cap, len_cm, nu;
+    // the flow (eht::Flow) and its plunge from the prograde ISCO: the ISCO's E, L, the
+    // radius where the plunge density law starts and |u^r| there
+    double n0, T0, beta, keplerian, height, sin_theta_b;
+    double isco, E, L, start, start_speed;
+    double rtol, atol;
+    int max_steps;
+    double* intensity;   // erg s^-1 cm^-2 Hz^-1 sr^-1 per pixel
+    uint8_t* shadow;     // 1 where the ray fell in
+    double* min_radius;  // optional
+    int32_t* steps;      // optional
+};
+
+hipError_t render(const EhtLaunch& e, hipStream_t st);
+// *out = sum of x[0..n) (device), in a fixed order; partial holds kSumPartials doubles
+hipError_t sum(const double* x, long long n, double* partial, double* out, hipStream_t st);
+
+}  // namespace kte

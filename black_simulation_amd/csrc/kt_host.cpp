@@ -19,6 +19,7 @@
 #include "kt_surrogate.hpp"
 #include "kt_nn.hpp"
 #include "kt_shade.hpp"
+#include "kt_eht.hpp"
 
 using ktd::ViewDev;
 
@@ -226,6 +227,7 @@ struct kt_ctx {
     kts::Surrogate* surrogate = nullptr;
     float* d_blackbody = nullptr;  // shading: the blackbody colour table
     std::vector<float> h_blackbody;
+    double* d_eht = nullptr;  // EHT: flux partials, the sum, calibrate's 48 x 48 image
     // autotuned launch shape per precision (0 = not tuned)
     unsigned long long* stats_src = nullptr;  // counters of the last launch (a plan's or d_counter)
     int tuned_mode[3] = {0, 0, 0};  // 1 + KT_MODE_* once tuned
@@ -514,6 +516,7 @@ int kt_destroy(kt_ctx* ctx) {
     (void)hipDeviceSynchronize();
     if (ctx->surrogate) kts::destroy(ctx->surrogate);
     if (ctx->d_blackbody) (void)hipFree(ctx->d_blackbody);
+    if (ctx->d_eht) (void)hipFree(ctx->d_eht);
     if (ctx->d_counter) (void)hipFree(ctx->d_counter);
     if (ctx->d_views) (void)hipFree(ctx->d_views);
     if (ctx->d_stage) (void)hipFree(ctx->d_stage);
@@ -898,6 +901,254 @@ int kt_shade_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_rays
         return fail(ctx, KT_EINVAL, ex.what());
     }
 }
+
+}  // extern "C"
+
+// ---- EHT radiative transfer -------------------------------------------------------------
+
+namespace {
+
+constexpr double kSolarLengthCm = 1.476625e5;  // G M_sun / c^2 (src/eht.h:30)
+constexpr double kParsecCm = 3.0856775814913673e18;
+constexpr double kJansky = 1e-23;
+constexpr double kMicroarcsecond = kPi / 180.0 / 3600.0 / 1e6;
+constexpr int kCalibrateSize = 48;                 // eht.cpp:435
+constexpr size_t kEhtSumSlot = kte::kSumPartials;  // d_eht: partials, then the sum
+constexpr size_t kEhtImage = kte::kSumPartials + 8;
+constexpr size_t kEhtDoubles = kEhtImage + (size_t)kCalibrateSize * kCalibrateSize;
+
+const char* check_eht(const kt_eht_view& v, const kt_eht_flow& f) {
+    if (!(std::fabs(v.a) < 1.0)) return "spin must be in (-1, 1)";
+    if (!(v.inclination >= 0.0 && v.inclination <= kPi)) return "inclination must be in [0, pi]";
+    if (!(v.half_width > 0.0) || !std::isfinite(v.half_width)) return "half_width must be positive";
+    if (v.size < 1 || v.size > 32768) return "size must be in [1, 32768]";
+    if (!(v.frequency > 0.0) || !(v.mass_solar > 0.0)) return "frequency and mass_solar must be positive";
+    if (!(f.n0 >= 0.0) || !(f.T0 > 0.0) || !(f.beta > 0.0)) return "flow needs n0 >= 0, T0 > 0, beta > 0";
+    if (!std::isfinite(f.keplerian) || !std::isfinite(f.height) || !std::isfinite(f.sin_theta_b))
+        return "flow parameters must be finite";
+    return nullptr;
+}
+
+// the ISCO's plunging geodesic (src/eht.cpp:183-192, 222-233): |u^r| from its E, L
+double plunge_speed(double a, double E, double L, double r) {
+    const double delta = r * r - 2.0 * r + a * a, x = L - a * E;
+    const double P = E * (r * r + a * a) - a * L;
+    return std::sqrt(std::fmax(P * P - delta * (r * r + x * x), 0.0)) / (r * r);
+}
+
+kte::EhtLaunch make_eht(const kt_eht_view& v, const kt_eht_flow& f, const kt_trace_opts& o) {
+    kte::EhtLaunch e{};
+    e.size = v.size;
+    e.n = (long long)v.size * v.size;
+    e.half_width = v.half_width;
+    e.px = 2.0 * v.half_width / v.size;
+    e.a = v.a;
+    e.sinI = hsin(v.inclination);
+    e.cosI = hcos(v.inclination);
+    e.m0 = hcos(std::clamp(v.inclination, 1e-4, kPi - 1e-4));
+    const double rh = horizon(v.a);
+    e.r_cap = rh * (1.0 + 1e-4);
+    e.len_cm = kSolarLengthCm * v.mass_solar;
+    e.nu = v.frequency;
+    e.n0 = f.n0;
+    e.T0 = f.T0;
+    e.beta = f.beta;
+    e.keplerian = f.keplerian;
+    e.height = f.height;
+    e.sin_theta_b = f.sin_theta_b;
+    // circular equatorial orbit at the prograde ISCO (src/kerr.cpp:308-319)
+    const double r = isco(v.a, 1), sr = std::sqrt(r), r32 = r * sr;
+    const double den = std::pow(r, 0.75) * std::sqrt(r32 - 3.0 * sr + 2.0 * v.a);
+    e.isco = r;
+    e.E = (r32 - 2.0 * sr + v.a) / den;
+    e.L = (r * r - 2.0 * v.a * sr + v.a * v.a) / den;
+    e.start = std::fmax(r - 0.25, 0.5 * (r + rh));
+    e.start_speed = plunge_speed(v.a, e.E, e.L, e.start);
+    e.rtol = o.rtol;
+    e.atol = o.atol;
+    e.max_steps = o.max_steps;
+    return e;
+}
+
+int ensure_eht(kt_ctx* ctx) {
+    if (ctx->d_eht) return KT_OK;
+    KT_HIP(hipMalloc(&ctx->d_eht, sizeof(double) * kEhtDoubles));
+    return KT_OK;
+}
+
+// ray count n: explicit (ab != null) or the view's pixels; outputs staged when on the host
+int eht_run(kt_ctx* ctx, const kt_eht_view* view, const kt_eht_flow* flow, const double* ab, long long n,
+            const kt_trace_opts* opts, const kt_eht_out* out, hipStream_t stream) {
+    if (!view || !out || !out->intensity) return fail(ctx, KT_EINVAL, "view/out/intensity missing");
+    kt_eht_flow fl;
+    kt_eht_flow_default(&fl);
+    if (flow) fl = *flow;
+    if (const char* why = check_eht(*view, fl)) return fail(ctx, KT_EINVAL, why);
+    kt_trace_opts o;
+    fill_defaults(opts, o);
+    kte::EhtLaunch e = make_eht(*view, fl, o);
+    if (ab) e.n = n;
+    if (e.n <= 0) return KT_OK;
+    KT_HIP(hipSetDevice(ctx->device));
+    if (out->on_device) {
+        e.rays = ab;
+        e.intensity = out->intensity;
+        e.shadow = out->shadow;
+        e.min_radius = out->min_radius;
+        e.steps = out->steps;
+        hipError_t err = kte::render(e, stream);
+        return err == hipSuccess ? KT_OK : hip_fail(ctx, err, "EHT kernel launch");
+    }
+    // host buffers: explicit rays, then intensity, min radius, steps, shadow on the device
+    const size_t nr = (size_t)e.n, in_bytes = ab ? 16 * nr : 0;
+    if (int st = ensure_stage(ctx, in_bytes + nr * (8 + 8 + 4 + 1) + 64)) return st;
+    char* base = static_cast<char*>(ctx->d_stage);
+    if (ab) {
+        KT_HIP(hipMemcpyAsync(base, ab, in_bytes, hipMemcpyHostToDevice, stream));
+        e.rays = reinterpret_cast<const double*>(base);
+    }
+    e.intensity = reinterpret_cast<double*>(base + in_bytes);
+    e.min_radius = out->min_radius ? e.intensity + nr : nullptr;
+    e.steps = out->steps ? reinterpret_cast<int32_t*>(e.intensity + 2 * nr) : nullptr;
+    e.shadow = out->shadow ? reinterpret_cast<uint8_t*>(e.intensity + 2 * nr) + 4 * nr : nullptr;
+    hipError_t err = kte::render(e, stream);
+    if (err != hipSuccess) return hip_fail(ctx, err, "EHT kernel launch");
+    KT_HIP(hipMemcpyAsync(out->intensity, e.intensity, 8 * nr, hipMemcpyDeviceToHost, stream));
+    if (out->min_radius) KT_HIP(hipMemcpyAsync(out->min_radius, e.min_radius, 8 * nr, hipMemcpyDeviceToHost, stream));
+    if (out->steps) KT_HIP(hipMemcpyAsync(out->steps, e.steps, 4 * nr, hipMemcpyDeviceToHost, stream));
+    if (out->shadow) KT_HIP(hipMemcpyAsync(out->shadow, e.shadow, nr, hipMemcpyDeviceToHost, stream));
+    KT_HIP(hipStreamSynchronize(stream));
+    return KT_OK;
+}
+
+int eht_flux(kt_ctx* ctx, const double* intensity, int size, double half_width, double theta_guas, double* jy,
+             hipStream_t stream) {
+    if (int st = ensure_eht(ctx)) return st;
+    hipError_t err = kte::sum(intensity, (long long)size * size, ctx->d_eht, ctx->d_eht + kEhtSumSlot, stream);
+    if (err != hipSuccess) return hip_fail(ctx, err, "EHT flux reduction");
+    double sum = 0.0;
+    KT_HIP(hipMemcpyAsync(&sum, ctx->d_eht + kEhtSumSlot, sizeof sum, hipMemcpyDeviceToHost, stream));
+    KT_HIP(hipStreamSynchronize(stream));
+    const double pixel = 2.0 * half_width / size;
+    const double solid_angle = std::pow(pixel * theta_guas * kMicroarcsecond, 2.0);
+    *jy = sum * solid_angle / kJansky;
+    return KT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void kt_eht_flow_default(kt_eht_flow* f) {
+    if (!f) return;
+    f->n0 = 1e5;
+    f->T0 = 4e11;
+    f->beta = 10.0;
+    f->keplerian = 0.9;
+    f->height = 1.0;
+    f->sin_theta_b = 0.8165;
+}
+
+void kt_eht_view_default(kt_eht_view* v) {
+    if (!v) return;
+    v->a = 0.9;
+    v->inclination = 0.3;
+    v->half_width = 16.0;
+    v->size = 128;
+    v->frequency = 230e9;
+    v->mass_solar = 6.5e9;
+}
+
+int kt_eht_render(kt_ctx* ctx, const kt_eht_view* view, const kt_eht_flow* flow, const kt_trace_opts* opts,
+                  const kt_eht_out* out, void* hip_stream) {
+    if (!ctx) return KT_EINVAL;
+    try {
+        return eht_run(ctx, view, flow, nullptr, 0, opts, out, static_cast<hipStream_t>(hip_stream));
+    } catch (const std::exception& ex) {
+        return fail(ctx, KT_EINVAL, ex.what());
+    }
+}
+
+int kt_eht_trace_rays(kt_ctx* ctx, const kt_eht_view* view, const kt_eht_flow* flow, const double* ab, long long n,
+                      const kt_trace_opts* opts, const kt_eht_out* out, void* hip_stream) {
+    if (!ctx) return KT_EINVAL;
+    if (!ab || n <= 0) return fail(ctx, KT_EINVAL, "explicit rays: ab and n > 0 needed");
+    try {
+        return eht_run(ctx, view, flow, ab, n, opts, out, static_cast<hipStream_t>(hip_stream));
+    } catch (const std::exception& ex) {
+        return fail(ctx, KT_EINVAL, ex.what());
+    }
+}
+
+int kt_eht_flux(kt_ctx* ctx, const double* intensity, int size, double half_width, double theta_guas, double* jy,
+                void* hip_stream) {
+    if (!ctx) return KT_EINVAL;
+    if (!intensity || !jy || size < 1 || !(half_width > 0.0) || !(theta_guas > 0.0))
+        return fail(ctx, KT_EINVAL, "flux: intensity, jy, size >= 1, half_width > 0, theta_guas > 0 needed");
+    KT_HIP(hipSetDevice(ctx->device));
+    return eht_flux(ctx, intensity, size, half_width, theta_guas, jy, static_cast<hipStream_t>(hip_stream));
+}
+
+int kt_eht_calibrate(kt_ctx* ctx, const kt_eht_view* view, const kt_eht_flow* flow, double target_jy,
+                     double theta_guas, const kt_trace_opts* opts, double* n0_out, void* hip_stream) {
+    if (!ctx) return KT_EINVAL;
+    if (!view || !n0_out || !(target_jy > 0.0) || !(theta_guas > 0.0))
+        return fail(ctx, KT_EINVAL, "calibrate: view, n0, target_jy > 0 and theta_guas > 0 needed");
+    try {
+        hipStream_t stream = static_cast<hipStream_t>(hip_stream);
+        KT_HIP(hipSetDevice(ctx->device));
+        if (int st = ensure_eht(ctx)) return st;
+        kt_eht_view small = *view;
+        small.size = kCalibrateSize;
+        kt_eht_flow f;
+        kt_eht_flow_default(&f);
+        if (flow) f = *flow;
+        kt_eht_out out{};
+        out.intensity = ctx->d_eht + kEhtImage;
+        out.on_device = 1;
+        int st = KT_OK;
+        auto flux_at = [&](double n0) {
+            f.n0 = n0;
+            double jy = 0.0;
+            if (st == KT_OK) st = eht_run(ctx, &small, &f, nullptr, 0, opts, &out, stream);
+            if (st == KT_OK) st = eht_flux(ctx, out.intensity, small.size, small.half_width, theta_guas, &jy, stream);
+            return jy;
+        };
+        // src/eht.cpp:436-456: brighten a dark start tenfold at a time, then a secant in
+        // log n0 against log flux, each step held within a factor 30, to 1%
+        double n0 = f.n0, f0 = flux_at(n0);
+        for (int k = 0; k < 8 && f0 <= 0.0 && st == KT_OK; ++k) f0 = flux_at(n0 *= 10.0);
+        if (st != KT_OK) return st;
+        if (f0 <= 0.0) {
+            *n0_out = n0;
+            return KT_OK;
+        }
+        double n1 = n0 * target_jy / f0, f1 = flux_at(n1);
+        for (int k = 0; k < 8 && st == KT_OK; ++k) {
+            if (f1 <= 0.0 || std::fabs(f1 / target_jy - 1.0) < 0.01) break;
+            const double slope = (std::log(f1) - std::log(f0)) / (std::log(n1) - std::log(n0));
+            const double next =
+                std::exp(std::log(n1) + (std::log(target_jy) - std::log(f1)) / (std::fabs(slope) > 0.05 ? slope : 1.0));
+            n0 = n1;
+            f0 = f1;
+            n1 = std::clamp(next, n0 / 30.0, n0 * 30.0);
+            f1 = flux_at(n1);
+        }
+        if (st != KT_OK) return st;
+        *n0_out = n1;
+        return KT_OK;
+    } catch (const std::exception& ex) {
+        return fail(ctx, KT_EINVAL, ex.what());
+    }
+}
+
+double kt_eht_angular_scale(double mass_solar, double distance_pc) {
+    return kSolarLengthCm * mass_solar / (distance_pc * kParsecCm) / kMicroarcsecond;
+}
+
+}  // extern "C"
+
+extern "C" {
 
 static int ensure_surrogate(kt_ctx* ctx) {
     if (ctx->surrogate) return KT_OK;

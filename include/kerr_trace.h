@@ -17,6 +17,9 @@
  *                      circularOmega          (src/kerr.h:39-45, 71)
  *   kt_critical_curve, kt_critical_radius
  *                   <- eht::criticalCurve, eht::criticalRadius (src/eht.h:66-68)
+ *   kt_eht_render, kt_eht_trace_rays, kt_eht_flux, kt_eht_calibrate
+ *                   <- eht::render, eht::traceRay (with the light), eht::flux,
+ *                      eht::calibrate         (src/eht.h:130-152)
  *
  * Conventions
  *   - Units G = c = M = 1.  Angles in radians except fov_deg.
@@ -249,6 +252,56 @@ int kt_shade_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_rays
 /* n linear-sRGB blackbody colours over log10 T in [2.5, 5.3], luminance 1 at 6500 K
  * (src/blackbody.cpp:48-60) */
 int kt_blackbody_table(int n, float* rgb);
+
+/* ---- What the Event Horizon Telescope sees (src/eht.h:100-153) -----------------------
+ * Each pixel of a distant observer's Bardeen plane is traced back from r = 1000 M in FP64
+ * (the same DP5 stepper as kt_trace_views) and the thermal synchrotron light of a hot
+ * accretion flow gathered along it by radiative transfer after every accepted step inside
+ * r = 60 M (src/eht.cpp:260-319).  Plasma in cgs, intensities in erg s^-1 cm^-2 Hz^-1
+ * sr^-1, fluxes in janskys.  opts: rtol, atol and max_steps of kt_trace_opts are used
+ * (NULL = defaults); the path always computes in FP64. */
+typedef struct kt_eht_flow {   /* eht::Flow (src/eht.h:104-111)                       */
+    double n0;                 /* electrons cm^-3 at r = 1 M (1e5)                      */
+    double T0;                 /* electron temperature (K) at r = 1 M (4e11)            */
+    double beta;               /* plasma beta (10)                                      */
+    double keplerian;          /* rotation as a fraction of Keplerian (0.9)             */
+    double height;             /* H/R (1)                                               */
+    double sin_theta_b;        /* field angle to the ray, sqrt(2/3) (0.8165)            */
+} kt_eht_flow;
+typedef struct kt_eht_view {   /* eht::View (src/eht.h:118-125)                       */
+    double a;                  /* spin (0.9)                                            */
+    double inclination;        /* radians (0.3)                                         */
+    double half_width;         /* M (16)                                                */
+    int size;                  /* pixels across (128)                                   */
+    double frequency;          /* Hz (230e9)                                            */
+    double mass_solar;         /* sets the path lengths in cm (6.5e9)                   */
+} kt_eht_view;
+typedef struct kt_eht_out {    /* eht::Image / eht::Ray, SoA, rows from the top        */
+    double* intensity;         /* I_nu at the observer (required)                       */
+    uint8_t* shadow;           /* 1 where the ray fell in (optional)                    */
+    double* min_radius;        /* optional                                              */
+    int32_t* steps;            /* DP5 attempts (optional)                               */
+    int on_device;             /* 1: device pointers; 0: host (staged, synchronous)    */
+} kt_eht_out;
+void kt_eht_flow_default(kt_eht_flow* flow);
+void kt_eht_view_default(kt_eht_view* view);
+/* eht::render (src/eht.cpp:321-347): view->size^2 pixels */
+int kt_eht_render(kt_ctx* ctx, const kt_eht_view* view, const kt_eht_flow* flow, const kt_trace_opts* opts,
+                  const kt_eht_out* out, void* hip_stream);
+/* eht::traceRay with the light (src/eht.cpp:260-319) for n explicit rays: ab = n (alpha,
+ * beta) pairs in M, on the device when out->on_device, else on the host */
+int kt_eht_trace_rays(kt_ctx* ctx, const kt_eht_view* view, const kt_eht_flow* flow, const double* ab,
+                      long long n, const kt_trace_opts* opts, const kt_eht_out* out, void* hip_stream);
+/* eht::flux (src/eht.cpp:349-354): Jy of a size x size device image spanning
+ * [-half_width, half_width]^2 whose M subtends theta_guas microarcseconds; synchronous */
+int kt_eht_flux(kt_ctx* ctx, const double* intensity, int size, double half_width, double theta_guas,
+                double* jy, void* hip_stream);
+/* eht::calibrate (src/eht.cpp:432-457): the n0 that gives target_jy, by a secant in
+ * log n0 on 48 x 48 renders; synchronous */
+int kt_eht_calibrate(kt_ctx* ctx, const kt_eht_view* view, const kt_eht_flow* flow, double target_jy,
+                     double theta_guas, const kt_trace_opts* opts, double* n0, void* hip_stream);
+/* eht::angularScale (src/eht.cpp:46-48): microarcseconds that M subtends */
+double kt_eht_angular_scale(double mass_solar, double distance_pc);
 
 /* Host-side closed forms (src/kerr.cpp:157-206, 296-299). */
 double kt_horizon_radius(double a);

@@ -100,3 +100,27 @@ def test_product_never_touches_the_oracle():
             if fn.endswith((".py", ".cpp", ".hip", ".hpp", ".h")):
                 text = open(os.path.join(dirpath, fn)).read()
                 assert "oracle" not in text.lower(), f"{fn} mentions the oracle"
+
+
+def test_eht_defaults_scales_and_error_paths():
+    lib = K.lib()
+    f = K.KtEhtFlow()
+    lib.kt_eht_flow_default(C.byref(f))
+    assert (f.n0, f.T0, f.beta, f.keplerian, f.height, f.sin_theta_b) == (1e5, 4e11, 10.0, 0.9, 1.0, 0.8165)
+    v = K.KtEhtView()
+    lib.kt_eht_view_default(C.byref(v))
+    assert (v.a, v.inclination, v.half_width, v.size, v.frequency, v.mass_solar) == (0.9, 0.3, 16.0, 128, 230e9, 6.5e9)
+    # tests/test_kerr.cpp:2269-2270: M of M87* subtends 3.82 uas, of Sgr A* 5.13 uas
+    from black_simulation_amd import eht
+
+    assert eht.angular_scale(eht.TARGETS[0].mass_solar, eht.TARGETS[0].distance_pc) == pytest.approx(3.82, abs=0.01)
+    assert eht.angular_scale(eht.TARGETS[1].mass_solar, eht.TARGETS[1].distance_pc) == pytest.approx(5.13, abs=0.01)
+    # the shadow of a = 0, 2 sqrt(27) M: 39.7 uas for M87*, 53.3 for Sgr A* (test_kerr.cpp:2275-2278)
+    assert 2 * math.sqrt(27) * eht.angular_scale(6.5e9, 16.8e6) == pytest.approx(39.7, abs=0.1)
+    assert 2 * math.sqrt(27) * eht.angular_scale(4.3e6, 8.28e3) == pytest.approx(53.3, abs=0.1)
+    out = K.KtEhtOut()
+    assert lib.kt_eht_render(None, C.byref(v), C.byref(f), None, C.byref(out), None) == K.KT_EINVAL
+    assert lib.kt_eht_trace_rays(None, C.byref(v), C.byref(f), None, 1, None, C.byref(out), None) == K.KT_EINVAL
+    jy = C.c_double()
+    assert lib.kt_eht_flux(None, None, 8, 16.0, 3.8, C.byref(jy), None) == K.KT_EINVAL
+    assert lib.kt_eht_calibrate(None, C.byref(v), C.byref(f), 0.5, 3.8, None, C.byref(jy), None) == K.KT_EINVAL
