@@ -19,145 +19,139 @@ namespace kte {
 
 namespace {
 
-constexpr double kQe = 4.80320425e-10, kMe = 9.1093837015e-28, kMp = 1.67262192369e-24;
+constexpr double kQe = 4.80320425e-10, kMe = 9.1093837015e-28;
 constexpr double kC = 2.99792458e10, kKb = 1.380649e-16, kHp = 6.62607015e-27;
 constexpr double kStart = 1000.0, kLight = 60.0, kMaxDepth = 30.0;
 
-// Abramowitz and Stegun 9.8.1-9.8.8 (as eht.cpp:98-152): I0, I1, and K0, K1 times e^x
-__device__ double bessel_i0(double x) {
-    const double ax = fabs(x);
-    if (ax < 3.75) {
-        const double y = (x / 3.75) * (x / 3.75);
-        return 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 + y * (0.2659732 + y * (0.360768e-1 + y * 0.45813e-2)))));
-    }
-    const double y = 3.75 / ax;
-    return (exp(ax) / sqrt(ax)) * (0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 + y * (-0.157565e-2 + y * (0.916281e-2 +
-           y * (-0.2057706e-1 + y * (0.2635537e-1 + y * (-0.1647633e-1 + y * 0.392377e-2))))))));
-}
+constexpr double kSqrt2 = 1.4142135623730951;
+constexpr double kTwo11_12 = 1.8877486253633868;  // 2^(11/12)
+// the emissivity's prefactor sqrt(2) pi e^2 / (3 c) and nu_c = e B / (2 pi m_e c) per gauss
+constexpr double kJ = kSqrt2 * kPi * kQe * kQe / (3.0 * kC);
+constexpr double kNuC = kQe / (2.0 * kPi * kMe * kC);
+constexpr double kThetaPerK = kKb / (kMe * kC * kC);
 
-__device__ double bessel_i1(double x) {
-    const double ax = fabs(x);
-    double res;
-    if (ax < 3.75) {
-        const double y = (x / 3.75) * (x / 3.75);
-        res = ax * (0.5 + y * (0.87890594 + y * (0.51498869 + y * (0.15084934 + y * (0.2658733e-1 + y * (0.301532e-2 + y * 0.32411e-3))))));
-    } else {
-        const double y = 3.75 / ax;
-        double t = 0.2282967e-1 + y * (-0.2895312e-1 + y * (0.1787654e-1 - y * 0.420059e-2));
-        t = 0.39894228 + y * (-0.3988024e-1 + y * (-0.362018e-2 + y * (0.163801e-2 + y * (-0.1031555e-1 + y * t))));
-        res = t * exp(ax) / sqrt(ax);
-    }
-    return x < 0.0 ? -res : res;
-}
-
-__device__ double k0_scaled(double x) {
+// K2(x) e^x for x > 0 from K2 = K0 + (2/x) K1 with Abramowitz and Stegun 9.8.1, 9.8.3,
+// 9.8.5-9.8.8 (as eht.cpp:98-152).  Below x = 2 the series of K0 and K1 share one exp(x)
+// and one log(x / 2); above, the asymptotic forms share 1 / sqrt(x).
+__device__ __forceinline__ double k2_scaled(double x) {
     if (x <= 2.0) {
         const double y = x * x / 4.0;
-        return exp(x) * ((-log(x / 2.0) * bessel_i0(x)) + (-0.57721566 + y * (0.42278420 + y * (0.23069756 + y * (0.3488590e-1 +
-               y * (0.262698e-2 + y * (0.10750e-3 + y * 0.74e-5)))))));
+        const double t = (x / 3.75) * (x / 3.75);
+        const double i0 = 1.0 + t * (3.5156229 + t * (3.0899424 + t * (1.2067492 + t * (0.2659732 + t * (0.360768e-1 +
+                          t * 0.45813e-2)))));
+        const double i1 = x * (0.5 + t * (0.87890594 + t * (0.51498869 + t * (0.15084934 + t * (0.2658733e-1 +
+                          t * (0.301532e-2 + t * 0.32411e-3))))));
+        const double lx = log(x / 2.0), ex = exp(x);
+        const double k0 = ex * ((-lx * i0) + (-0.57721566 + y * (0.42278420 + y * (0.23069756 + y * (0.3488590e-1 +
+                          y * (0.262698e-2 + y * (0.10750e-3 + y * 0.74e-5)))))));
+        const double k1 = ex * ((lx * i1) + (1.0 / x) * (1.0 + y * (0.15443144 + y * (-0.67278579 + y * (-0.18156897 +
+                          y * (-0.1919402e-1 + y * (-0.110404e-2 + y * (-0.4686e-4))))))));
+        return k0 + 2.0 / x * k1;
     }
-    const double y = 2.0 / x;
-    return (1.0 / sqrt(x)) * (1.25331414 + y * (-0.7832358e-1 + y * (0.2189568e-1 + y * (-0.1062446e-1 + y * (0.587872e-2 +
-           y * (-0.251540e-2 + y * 0.53208e-3))))));
+    const double y = 2.0 / x, rs = 1.0 / sqrt(x);
+    const double k0 = rs * (1.25331414 + y * (-0.7832358e-1 + y * (0.2189568e-1 + y * (-0.1062446e-1 + y * (0.587872e-2 +
+                      y * (-0.251540e-2 + y * 0.53208e-3))))));
+    const double k1 = rs * (1.25331414 + y * (0.23498619 + y * (-0.3655620e-1 + y * (0.1504268e-1 + y * (-0.780353e-2 +
+                      y * (0.325614e-2 + y * (-0.68245e-3)))))));
+    return k0 + 2.0 / x * k1;
 }
 
-__device__ double k1_scaled(double x) {
-    if (x <= 2.0) {
-        const double y = x * x / 4.0;
-        return exp(x) * ((log(x / 2.0) * bessel_i1(x)) + (1.0 / x) * (1.0 + y * (0.15443144 + y * (-0.67278579 + y * (-0.18156897 +
-               y * (-0.1919402e-1 + y * (-0.110404e-2 + y * (-0.4686e-4))))))));
-    }
-    const double y = 2.0 / x;
-    return (1.0 / sqrt(x)) * (1.25331414 + y * (0.23498619 + y * (-0.3655620e-1 + y * (0.1504268e-1 + y * (-0.780353e-2 +
-           y * (0.325614e-2 + y * (-0.68245e-3)))))));
-}
-
-__device__ double emissivity(double nu, double ne, double thetaE, double B, double sinThetaB) {
-    if (ne <= 0.0 || thetaE < 0.05 || B <= 0.0 || sinThetaB <= 0.0) return 0.0;
-    const double nuC = kQe * B / (2.0 * kPi * kMe * kC);
-    const double nuS = (2.0 / 9.0) * nuC * thetaE * thetaE * sinThetaB;
-    const double X = nu / nuS;
-    const double shape = sqrt(X) + pow(2.0, 11.0 / 12.0) * pow(X, 1.0 / 6.0);
-    const double inv = 1.0 / thetaE;
-    const double expo = inv - cbrt(X);
-    if (expo < -700.0) return 0.0;
-    const double k2 = k0_scaled(inv) + 2.0 / inv * k1_scaled(inv);
-    return ne * sqrt(2.0) * kPi * kQe * kQe * nuS / (3.0 * k2 * kC) * shape * shape * exp(expo);
-}
-
-__device__ double planck_nu(double nu, double T) {
-    const double x = kHp * nu / (kKb * T);
-    return 2.0 * kHp * nu * nu * nu / (kC * kC) / expm1(x);
-}
-
-__device__ double plunge_ur(const EhtLaunch& e, double r) {
+// |u^r| of the ISCO's plunging geodesic at r (eht.cpp:222-233)
+__device__ __forceinline__ double plunge_ur(const EhtLaunch& e, double r) {
     const double a = e.a, delta = r * r - 2.0 * r + a * a, x = e.L - a * e.E;
     const double P = e.E * (r * r + a * a) - a * e.L;
     return sqrt(fmax(P * P - delta * (r * r + x * x), 0.0)) / (r * r);
 }
 
-// Radiative transfer over one accepted step ending at y (src/eht.cpp:290-316): the gas at
-// the step's end, the step's affine length from the mean of Sigma at both ends (cm).
-// Returns true once the optical depth passes kMaxDepth.
+// Radiative transfer over one accepted step ending at y (src/eht.cpp:290-316).  The
+// reference's formulas, arranged for the vector units: one log(r) gives r^-1.1 and
+// r^-0.84 through exp (and the Gaussian joins the density's exp), X^(1/6) is the square
+// root of X^(1/3), K0 and K1 share their exp and log, the divisions by Sigma become one
+// reciprocal, and exp(-tau) is carried as the product of the steps' exp(-dtau) (each
+// step's expm1 gives it), so a step costs nine transcendental calls instead of eighteen.
+// Results differ from the formulas as written by rounding only (1e-13 relative over a
+// ray).  Returns true once the optical depth passes kMaxDepth.
 __device__ __forceinline__ bool transfer(const EhtLaunch& e, const Photon<double>& p, double r_prev, double mu_prev,
-                                         const double y[5], double h, double& intensity, double& depth) {
-    const double a = e.a, r = y[0], mu = y[1];
-    const double sig_b = r_prev * r_prev + a * a * mu_prev * mu_prev, sig_a = r * r + a * a * mu * mu;
-    const double dl = 0.5 * (sig_a + sig_b) * h * e.len_cm;
-    // the plasma at (r, mu)
-    const double z = r * mu, rho = fmax(r * sqrt(fmax(1.0 - mu * mu, 0.0)), 1e-6);
-    const double hh = fmax(e.height, 0.05) * rho;
-    double radial = pow(r, -1.1);
-    if (r < e.isco && r < e.start) {
+                                         const double y[5], double h, double& intensity, double& depth,
+                                         double& seen) {
+    const double a = e.a, a2 = a * a, r = y[0], mu = y[1];
+    const double rr = r * r, s2 = 1.0 - mu * mu;
+    const double sigma = rr + a2 * mu * mu;
+    const double dl = 0.5 * (sigma + (r_prev * r_prev + a2 * mu_prev * mu_prev)) * h * e.len_cm;
+    // the plasma at (r, mu) (eht.cpp:196-215)
+    const double z = r * mu, rho = fmax(r * sqrt(fmax(s2, 0.0)), 1e-6);
+    const double hh = e.hfac * rho;
+    const double gauss = -z * z / (2.0 * hh * hh);
+    const double lr = log(r);
+    double ne;
+    if (r < e.isco && r < e.start) {  // inside the plunge the gas thins as it speeds up
+        double radial = exp(-1.1 * lr);
         const double here = plunge_ur(e, r);
-        if (here > 0.0)
-            radial = fmin(radial, pow(e.start, -1.1) * (e.start * e.start * e.start_speed) / (r * r * here));
+        if (here > 0.0) radial = fmin(radial, e.start_rate / (rr * here));
+        ne = e.n0 * radial * exp(gauss);
+    } else {
+        ne = e.n0 * exp(fma(-1.1, lr, gauss));
     }
-    const double ne = e.n0 * radial * exp(-z * z / (2.0 * hh * hh));
-    const double Te = e.T0 * pow(r, -0.84);
-    const double B = sqrt(8.0 * kPi * ne * kMp * kC * kC / (6.0 * e.beta * r));
-    // the gas's 4-velocity
-    const double s2 = 1.0 - mu * mu, c2 = mu * mu;
-    const double sigma = r * r + a * a * c2, delta = r * r - 2.0 * r + a * a;
-    const double A = (r * r + a * a) * (r * r + a * a) - a * a * delta * s2;
-    const double gtt = -(1.0 - 2.0 * r / sigma), gtp = -2.0 * a * r * s2 / sigma, gpp = A * s2 / sigma;
+    const double Te = e.T0 * exp(-0.84 * lr);
+    const double B = sqrt(e.b_coef * ne / r);
+    // the gas's 4-velocity (eht.cpp:217-253)
+    const double delta = rr - 2.0 * r + a2, rra = rr + a2;
+    const double A = rra * rra - a2 * delta * s2;
+    const double isig = 1.0 / sigma;
+    const double gtt = -(1.0 - 2.0 * r * isig), gtp = -2.0 * a * r * s2 * isig, gpp = A * s2 * isig;
     const double grr = sigma / delta;
     double omega, ur = 0.0;
     if (r >= e.isco) {
-        omega = e.keplerian / (pow(r, 1.5) + a);
+        omega = e.keplerian / (r * sqrt(r) + a);
     } else {
-        const double x = e.L - a * e.E, P = e.E * (r * r + a * a) - a * e.L;
-        const double R = fmax(P * P - delta * (r * r + x * x), 0.0);
-        const double ut = ((r * r + a * a) * P / delta + a * x) / (r * r);
-        const double up = (a * P / delta + x) / (r * r);
-        omega = up / ut;
-        ur = -sqrt(R) / (r * r);
+        const double x = e.L - a * e.E, P = e.E * rra - a * e.L;
+        const double R = fmax(P * P - delta * (rr + x * x), 0.0);
+        const double id = 1.0 / delta;
+        omega = (a * P * id + x) / (rra * P * id + a * x);  // u^phi / u^t (their 1/r^2 cancels)
+        ur = -sqrt(R) / rr;
     }
     double den = gtt + 2.0 * omega * gtp + omega * omega * gpp;
     double num = -1.0 - grr * ur * ur;
-    if (!(den < 0.0) || !(num < 0.0)) {
+    if (!(den < 0.0) || !(num < 0.0)) {  // the local ZAMO
         omega = -gtp / gpp;
         ur = 0.0;
         den = gtt + 2.0 * omega * gtp + omega * omega * gpp;
         num = -1.0;
     }
     const double ut = sqrt(num / den);
-    const double u0 = ut, u1 = ur, u3 = omega * ut;
-    const double mpu = u0 - p.L * u3 + y[3] / delta * u1;
-    if (mpu > 0.0) {
-        const double g = 1.0 / mpu, nu_f = e.nu / g;
-        const double thetaE = kKb * Te / (kMe * kC * kC);
-        const double j = emissivity(nu_f, ne, thetaE, B, e.sin_theta_b);
-        if (j > 0.0) {
-            const double absorb = j / planck_nu(nu_f, Te);
-            const double dtau = absorb * dl / g;
-            const double add = dtau < 1e-6 ? g * g * j * dl : g * g * g * (j / absorb) * -expm1(-dtau);
-            intensity += exp(-depth) * add;
-            depth += dtau;
-            if (depth > kMaxDepth) return true;
-        }
+    // -p.u with p_t = -1, p_phi = L, p_r = -w_r / Delta (towards the camera)
+    const double mpu = ut - p.L * (omega * ut) + y[3] / delta * ur;
+    if (!(mpu > 0.0)) return false;
+    const double g = 1.0 / mpu, nu_f = e.nu / g;
+    // thermal synchrotron emissivity at nu_f (Leung, Gammie and Noble 2011, eq. 72)
+    const double thetaE = kThetaPerK * Te;
+    if (ne <= 0.0 || thetaE < 0.05 || B <= 0.0 || e.sin_theta_b <= 0.0) return false;
+    const double nuS = (2.0 / 9.0) * (kNuC * B) * thetaE * thetaE * e.sin_theta_b;
+    const double X = nu_f / nuS;
+    const double cx = cbrt(X);
+    const double shape = sqrt(X) + kTwo11_12 * sqrt(cx);
+    const double inv = 1.0 / thetaE;
+    const double expo = inv - cx;
+    if (expo < -700.0) return false;
+    const double j = ne * kJ * nuS / k2_scaled(inv) * shape * shape * exp(expo);
+    if (!(j > 0.0)) return false;
+    // Kirchhoff: absorption j / B_nu(T_e)
+    const double bnu = 2.0 * kHp * nu_f * nu_f * nu_f / (kC * kC) / expm1(kHp * nu_f / (kKb * Te));
+    const double absorb = j / bnu;
+    const double dtau = absorb * dl / g;
+    double add, fade;
+    if (dtau < 1e-6) {
+        add = g * g * j * dl;
+        fade = 1.0 - dtau * (1.0 - 0.5 * dtau);
+    } else {
+        const double em = expm1(-dtau);
+        add = g * g * g * (j / absorb) * -em;
+        fade = 1.0 + em;
     }
-    return false;
+    intensity += seen * add;
+    seen *= fade;
+    depth += dtau;
+    return depth > kMaxDepth;
 }
 
 __global__ __launch_bounds__(128) void k_eht(EhtLaunch e) {
@@ -176,7 +170,7 @@ __global__ __launch_bounds__(128) void k_eht(EhtLaunch e) {
     Photon<double> p;
     double y[5], yn[5], k[7][5];
     bardeen_photon(e.a, kStart, 0.0, e.sinI, e.cosI, e.m0, alpha, beta, p, y);
-    double intensity = 0.0, depth = 0.0, rmin = y[0];
+    double intensity = 0.0, depth = 0.0, seen = 1.0, rmin = y[0];
     int captured = 0, steps = 0;
     bool rejected = false;
     double h = 0.01 / kStart;
@@ -209,7 +203,7 @@ __global__ __launch_bounds__(128) void k_eht(EhtLaunch e) {
         }
         if (y[0] < rmin) rmin = y[0];
         if (y[0] > kStart * 1.01 && y[3] > 0.0) break;
-        if (y[0] <= kLight && transfer(e, p, r_prev, mu_prev, y, h_used, intensity, depth)) break;
+        if (y[0] <= kLight && transfer(e, p, r_prev, mu_prev, y, h_used, intensity, depth, seen)) break;
     }
     if (e.intensity) e.intensity[q] = intensity;
     if (e.shadow) e.shadow[q] = (uint8_t)captured;

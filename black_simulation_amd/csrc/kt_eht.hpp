@@ -19,10 +19,12 @@ struct EhtLaunch {
     // the view: spin, inclination (sin, cos and the cosine of the clamped theta), capture
     // radius r+ (1 + 1e-4), M in cm, observing frequency (Hz)
     double a, sinI, cosI, m0, r_cap, len_cm, nu;
-    // the flow (eht::Flow) and its plunge from the prograde ISCO: the ISCO's E, L, the
-    // radius where the plunge density law starts and |u^r| there
-    double n0, T0, beta, keplerian, height, sin_theta_b;
-    double isco, E, L, start, start_speed;
+    // the flow (eht::Flow): n0, T0, keplerian, sin_theta_b as given; hfac = max(H/R, 0.05);
+    // b_coef = 8 pi m_p c^2 / (6 beta), so that B = sqrt(b_coef n_e / r)
+    double n0, T0, keplerian, sin_theta_b, hfac, b_coef;
+    // its plunge from the prograde ISCO: the ISCO's radius, E and L, the radius where the
+    // plunge's density law starts, and start^-1.1 start^2 |u^r(start)| (continuity)
+    double isco, E, L, start, start_rate;
     double rtol, atol;
     int max_steps;
     double* intensity;   // erg s^-1 cm^-2 Hz^-1 sr^-1 per pixel

@@ -909,6 +909,8 @@ int kt_shade_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_rays
 namespace {
 
 constexpr double kSolarLengthCm = 1.476625e5;  // G M_sun / c^2 (src/eht.h:30)
+constexpr double kProtonMass = 1.67262192369e-24;  // g
+constexpr double kLightCm = 2.99792458e10;         // cm/s
 constexpr double kParsecCm = 3.0856775814913673e18;
 constexpr double kJansky = 1e-23;
 constexpr double kMicroarcsecond = kPi / 180.0 / 3600.0 / 1e6;
@@ -952,10 +954,10 @@ kte::EhtLaunch make_eht(const kt_eht_view& v, const kt_eht_flow& f, const kt_tra
     e.nu = v.frequency;
     e.n0 = f.n0;
     e.T0 = f.T0;
-    e.beta = f.beta;
     e.keplerian = f.keplerian;
-    e.height = f.height;
     e.sin_theta_b = f.sin_theta_b;
+    e.hfac = std::fmax(f.height, 0.05);
+    e.b_coef = 8.0 * kPi * kProtonMass * kLightCm * kLightCm / (6.0 * f.beta);
     // circular equatorial orbit at the prograde ISCO (src/kerr.cpp:308-319)
     const double r = isco(v.a, 1), sr = std::sqrt(r), r32 = r * sr;
     const double den = std::pow(r, 0.75) * std::sqrt(r32 - 3.0 * sr + 2.0 * v.a);
@@ -963,7 +965,7 @@ kte::EhtLaunch make_eht(const kt_eht_view& v, const kt_eht_flow& f, const kt_tra
     e.E = (r32 - 2.0 * sr + v.a) / den;
     e.L = (r * r - 2.0 * v.a * sr + v.a * v.a) / den;
     e.start = std::fmax(r - 0.25, 0.5 * (r + rh));
-    e.start_speed = plunge_speed(v.a, e.E, e.L, e.start);
+    e.start_rate = std::pow(e.start, -1.1) * (e.start * e.start * plunge_speed(v.a, e.E, e.L, e.start));
     e.rtol = o.rtol;
     e.atol = o.atol;
     e.max_steps = o.max_steps;
