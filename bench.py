@@ -216,10 +216,15 @@ def main():
     ev1.record()
     torch.cuda.synchronize()
     shade_ms = ev0.elapsed_time(ev1) / 10
-    shade_bytes = rays * (1 + 4 * 8 + 4 * 2)
+    # algorithmic bytes: every ray reads its class byte and writes 8 B; escaped rays read
+    # (theta, phi) and disk rays (r_hit, g), 16 B each
+    cls = orb.out["cls"] & 7
+    n_esc, n_disk = int((cls == bsa.ESCAPED).sum().item()), int((cls == bsa.DISK).sum().item())
+    shade_bytes = rays * (1 + 8) + 16 * (n_esc + n_disk)
     shade = {"kernel": "k_shade", "kernel_ms": round(shade_ms, 4), "bound": "hbm",
              "achieved": round(shade_bytes / (shade_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-             "frac": round(shade_bytes / (shade_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_ray": 41}
+             "frac": round(shade_bytes / (shade_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+             "bytes_per_ray": round(shade_bytes / rays, 2)}
 
     # accuracy of the chosen path against the FP64 render of every timed frame
     acc = None
