@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libkerr_trace.so")
 KT_OK, KT_EINVAL, KT_EHIP, KT_ECANCELLED, KT_ENOMEM, KT_EUNSUPPORTED, KT_ENODEVICE = range(7)
 ESCAPED, CAPTURED, DISK, UNFINISHED = 0, 1, 2, 3
 FLAG_FP64_RESTEP, FLAG_SURROGATE = 0x10, 0x20
-CAMERA_BARDEEN, CAMERA_PINHOLE = 0, 1
+CAMERA_BARDEEN, CAMERA_PINHOLE, CAMERA_OBSERVER = 0, 1, 3
 PREC_FP64, PREC_MIXED, PREC_FP32 = 0, 1, 2
 MODE_PERSISTENT, MODE_ONESHOT, MODE_AUTO = 0, 1, 2
 
@@ -40,6 +40,8 @@ class KtView(C.Structure):
         ("disk_enabled", C.c_int),
         ("escape_radius", C.c_double),
         ("capture_factor", C.c_double),
+        ("u_obs", C.c_double * 4),
+        ("drift", C.c_double * 3),
     ]
 
 
@@ -197,6 +199,8 @@ SIGNATURES = {
     "kt_eht_calibrate": (C.c_int, [C.c_void_p, _P(KtEhtView), _P(KtEhtFlow), C.c_double, C.c_double,
                                    _P(KtTraceOpts), _P(C.c_double), C.c_void_p]),
     "kt_eht_angular_scale": (C.c_double, [C.c_double, C.c_double]),
+    "kt_geodesic_four_velocity": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                            C.c_int, C.c_int, _P(C.c_double)]),
     "kt_horizon_radius": (C.c_double, [C.c_double]),
     "kt_isco_radius": (C.c_double, [C.c_double, C.c_int]),
     "kt_photon_orbit_radius": (C.c_double, [C.c_double, C.c_int]),

@@ -28,7 +28,10 @@ OUTPUT_FIELDS = ("cls", "theta_inf", "phi_inf", "r_hit", "g", "steps")
 @dataclasses.dataclass
 class View:
     """One camera view (kt_view).  Bardeen: the image plane of eht::View; pinhole: the
-    simulator's Camera (src/camera.h:20-29) in the ZAMO frame."""
+    simulator's Camera (src/camera.h:20-29) in the ZAMO frame; observer: that camera
+    carried by the observer of 4-velocity u_obs (Kerr-Schild, e.g. from
+    geodesic_four_velocity; inside the horizon too), or by the ZAMO boosted by `drift`
+    (Camera::frame, src/camera.cpp:23-40)."""
 
     camera_kind: int = K.CAMERA_BARDEEN
     width: int = 256
@@ -49,11 +52,18 @@ class View:
     disk_enabled: int = 1
     escape_radius: float = 1000.0
     capture_factor: float = 1.01
+    u_obs: tuple = (0.0, 0.0, 0.0, 0.0)
+    drift: tuple = (0.0, 0.0, 0.0)
 
     def to_c(self) -> K.KtView:
         v = K.KtView()
         for f in dataclasses.fields(self):
-            setattr(v, f.name, getattr(self, f.name))
+            val = getattr(self, f.name)
+            if f.name == "u_obs":
+                val = (C.c_double * 4)(*[float(x) for x in val])
+            elif f.name == "drift":
+                val = (C.c_double * 3)(*[float(x) for x in val])
+            setattr(v, f.name, val)
         return v
 
     @property
@@ -383,6 +393,17 @@ def blackbody_table(n: int = 1024) -> np.ndarray:
     if rc != K.KT_OK:
         raise RuntimeError(f"kt_blackbody_table failed with status {rc}")
     return t.reshape(n, 3)
+
+
+def geodesic_four_velocity(a: float, E: float, L: float, Q: float, r: float, theta: float, radial_sign: int = -1,
+                           polar_sign: int = 1) -> tuple:
+    """kt_geodesic_four_velocity: (u^t, u^r, u^theta, u^phi) in Kerr-Schild components of a
+    free-falling massive observer (kerr::fourVelocity, src/kerr.cpp:414-418)."""
+    u = (C.c_double * 4)()
+    rc = K.lib().kt_geodesic_four_velocity(a, E, L, Q, r, theta, radial_sign, polar_sign, u)
+    if rc != K.KT_OK:
+        raise ValueError(f"no geodesic with E={E}, L={L}, Q={Q} at r={r}, theta={theta}")
+    return tuple(u)
 
 
 def horizon_radius(a: float) -> float:

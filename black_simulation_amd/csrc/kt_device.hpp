@@ -55,6 +55,7 @@ struct ViewDev {
     double sinI, cosI, m0;                   // sin/cos of inclination, cos of the clamped theta
     double r_cap, r_esc, r_in, r_out, h0, r_fp64;  // r_fp64 = 1.2 r+ (mixed precision)
     double escape_radius_;                         // kind 2: the view's escape_radius
+    double r_plus;                                 // kind 3: r+ (the far-side rule)
     double lt[4], lp[4], lh[4], er[4];       // pinhole: lowered frame (t, phi, theta) and e^r
     double B[9], tanh_fov, sn, cs, sigma;    // pinhole: camera basis (columns), trig at the camera
     const double* params;                    // explicit rays (alpha, beta) or look vectors, or null
@@ -142,16 +143,33 @@ __device__ __forceinline__ R pot_M(const Photon<R>& p, R m) {
 
 // dphi/ds = aE - [a P (1 - mu^2) + L Delta] / [Delta (1 - mu^2)]: one division; the floor
 // on 1 - mu^2 keeps L = 0 rays finite at the pole (the ratio tends to aP/Delta).
-template <class R>
+//
+// KS: phi is the ingoing Kerr-Schild phi (dphi_KS = dphi_BL + a dr / Delta), regular on the
+// future horizon, for cameras that may sit inside it: P / Delta becomes (P - w_r) / Delta,
+// which on the regular branch (P w_r > 0, the ray rising to the horizon) is K / (P + w_r)
+// by w_r^2 = P^2 - Delta K; on the other branch Delta is floored at 1e-12 as
+// src/kerr.cpp:323-329 (inflowFactor) does.  Still one division.
+template <class R, bool KS = false>
 __device__ __forceinline__ void rhs_main(const Photon<R>& p, const R y[5], R dy[5]) {
     const R r = y[0], m = y[1];
     const R delta = delta_of(p, r);
     const R P = P_of(p, r);
     const R s2 = fmax(fma(-m, m, R(1)), tiny_pos<R>());
-    const R num = fma(p.a * P, s2, p.L * delta);
+    R nd = P, dd = delta;
+    if (KS) {
+        const R w = y[3];
+        if (P * w > R(0)) {
+            nd = p.K;
+            dd = P + w;
+        } else {
+            nd = P - w;
+            dd = fabs(delta) > R(1e-12) ? delta : copysign(R(1e-12), delta);
+        }
+    }
+    const R num = fma(p.a * nd, s2, p.L * dd);
     dy[0] = y[3];
     dy[1] = y[4];
-    dy[2] = p.aE - qdiv(num, delta * s2);
+    dy[2] = p.aE - qdiv(num, dd * s2);
     dy[3] = fma(p.twoE * r, P, -((r - R(1)) * p.K));
     dy[4] = -(m * fma(p.twoA2E2, m * m, p.C1));
 }
@@ -175,10 +193,10 @@ __device__ __forceinline__ void rhs_tail(const Photon<R>& p, R u, const R y[5], 
     dy[4] = (m * fma(p.twoA2E2, m * m, p.C1)) * isq;
 }
 
-template <class R>
+template <class R, bool KS = false>
 __device__ __forceinline__ void eval(const Photon<R>& p, bool tail, R u, const R y[5], R dy[5]) {
     if (tail) rhs_tail(p, u, y, dy);
-    else rhs_main(p, y, dy);
+    else rhs_main<R, KS>(p, y, dy);
 }
 
 // x^(-1/5): exponent split and three Newton steps on y^-5 = x (+, *, frexp, ldexp only)
@@ -261,33 +279,33 @@ struct Dense {
 };
 
 // One DP5 attempt; k[0] = f(y) on entry; fills k[1..6] and yn; returns the scaled error.
-template <class R>
+template <class R, bool KS = false>
 __device__ __forceinline__ R dp5_attempt(const Photon<R>& p, bool tail, R u, const R y[5], R h, R k[7][5],
                                          R yn[5], R rtol, R atol) {
     R t[5];
 #pragma unroll
     for (int i = 0; i < 5; ++i) t[i] = fma(h, R(A21) * k[0][i], y[i]);
-    eval(p, tail, u + R(C2) * h, t, k[1]);
+    eval<R, KS>(p, tail, u + R(C2) * h, t, k[1]);
 #pragma unroll
     for (int i = 0; i < 5; ++i) t[i] = fma(h, sum2(R(A31), k[0][i], R(A32), k[1][i]), y[i]);
-    eval(p, tail, u + R(C3) * h, t, k[2]);
+    eval<R, KS>(p, tail, u + R(C3) * h, t, k[2]);
 #pragma unroll
     for (int i = 0; i < 5; ++i) t[i] = fma(h, sum3(R(A41), k[0][i], R(A42), k[1][i], R(A43), k[2][i]), y[i]);
-    eval(p, tail, u + R(C4) * h, t, k[3]);
+    eval<R, KS>(p, tail, u + R(C4) * h, t, k[3]);
 #pragma unroll
     for (int i = 0; i < 5; ++i)
         t[i] = fma(h, sum4(R(A51), k[0][i], R(A52), k[1][i], R(A53), k[2][i], R(A54), k[3][i]), y[i]);
-    eval(p, tail, u + R(C5) * h, t, k[4]);
+    eval<R, KS>(p, tail, u + R(C5) * h, t, k[4]);
 #pragma unroll
     for (int i = 0; i < 5; ++i)
         t[i] = fma(h, sum5(R(A61), k[0][i], R(A62), k[1][i], R(A63), k[2][i], R(A64), k[3][i], R(A65), k[4][i]),
                    y[i]);
-    eval(p, tail, u + h, t, k[5]);
+    eval<R, KS>(p, tail, u + h, t, k[5]);
 #pragma unroll
     for (int i = 0; i < 5; ++i)
         yn[i] = fma(h, sum5(R(A71), k[0][i], R(A73), k[2][i], R(A74), k[3][i], R(A75), k[4][i], R(A76), k[5][i]),
                     y[i]);
-    eval(p, tail, u + h, yn, k[6]);
+    eval<R, KS>(p, tail, u + h, yn, k[6]);
     ErrPick<R> ep;
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
@@ -349,6 +367,7 @@ struct Lane {
     R r_cap, r_esc, r_in, r_out;
     int steps;
     bool rejected, tail, disk, prograde;
+    bool dark;  // observer cameras: the ray came from the far side (ends captured at once)
 };
 
 struct Outputs {
@@ -371,7 +390,19 @@ struct Launch {
     double y0[5];
     double r_cap, r_esc, r_in, r_out, h0, r_fp64;
     int disk, prograde;
+    int dark;  // kind 3 inside the horizon: P(r+) = 2 r+ E - a L < 0
 };
+
+// phi_KS - phi_BL = psi(r) = integral from infinity to r of a / Delta, so that the two
+// agree at infinity: (a / (r+ - r-)) log((r - r+) / (r - r-)) outside the horizon, and the
+// limit -a / (r - 1) at |a| = 1.
+__device__ __forceinline__ double ks_shift(double a, double r) {
+    if (a == 0.0) return 0.0;
+    const double d = 2.0 * sqrt(fmax(0.0, 1.0 - a * a));
+    const double rm = 1.0 - 0.5 * d;
+    if (d == 0.0) return -a / (r - 1.0);
+    return a / d * log1p(-d / (r - rm));
+}
 
 // Bardeen launch (src/eht.cpp:262-271): E = 1, L = -alpha sin i,
 // Q = beta^2 + (alpha^2 - a^2) cos^2 i, inward, toward smaller theta when beta > 0.
@@ -408,6 +439,7 @@ __device__ __forceinline__ void launch_photon(const ViewDev& v, long long local,
     L.r_fp64 = v.r_fp64;
     L.disk = v.disk;
     L.prograde = v.prograde;
+    L.dark = 0;
     Photon<double>& p = L.p;
     double* y0 = L.y0;
     const double a = v.a;
@@ -439,7 +471,8 @@ __device__ __forceinline__ void launch_photon(const ViewDev& v, long long local,
         bardeen_photon(a, v.r_cam, v.phi_cam, v.sinI, v.cosI, v.m0, alpha, beta, p, y0);
         return;
     }
-    // pinhole camera: trace.frag:1494-1496 + launchPhoton (trace.frag:1470-1488)
+    // pinhole camera (kind 1: the static ZAMO; kind 3: any observer, frame from the host):
+    // trace.frag:1494-1496 + launchPhoton (trace.frag:1470-1488)
     double n0, n1, n2;
     if (v.params) {
         n0 = v.params[3 * local];
@@ -477,9 +510,13 @@ __device__ __forceinline__ void launch_photon(const ViewDev& v, long long local,
     y0[2] = v.phi_cam;
     y0[3] = -(v.sigma * qr);
     y0[4] = v.sn * qh;
+    // inside the horizon every ray, followed back, rises to it; one that arrives with
+    // P(r+) < 0 entered from the far side of the eternal spacetime: dark
+    // (src/trajectory.cpp:234-241, shaders/trace.frag:1538-1541)
+    if (v.kind == 3 && v.r_cam < v.r_plus && 2.0 * v.r_plus * E - a * Lz < 0.0) L.dark = 1;
 }
 
-template <class R>
+template <class R, bool KS = false>
 __device__ __forceinline__ void lane_start(Lane<R>& s, const Launch& la) {
     const Photon<double>& p = la.p;
     const double* y0 = la.y0;
@@ -508,13 +545,14 @@ __device__ __forceinline__ void lane_start(Lane<R>& s, const Launch& la) {
     s.tail = false;
     s.disk = la.disk != 0;
     s.prograde = la.prograde != 0;
-    rhs_main(s.p, s.y, s.k1);
+    s.dark = KS && la.dark != 0;
+    rhs_main<R, KS>(s.p, s.y, s.k1);
 }
 
 // After an attempt: the controller, the pole, the projection and the events.  k7 is the
 // attempt's f(yn); dr, dm its dense output of r and mu when the step crossed the plane.
 // Returns true when the ray has ended (res filled).
-template <class R>
+template <class R, bool KS = false>
 __device__ __forceinline__ bool lane_finish(Lane<R>& s, R err, R h, bool last, R yn[5], const R k7[5], bool crossing,
                                             const Dense<R>& dr, const Dense<R>& dm, Result& res) {
     if (!(err <= R(1))) {
@@ -578,11 +616,14 @@ __device__ __forceinline__ bool lane_finish(Lane<R>& s, R err, R h, bool last, R
             }
         }
     }
-    if (yn[0] < s.r_cap) {
+    // KS cameras may start inside the capture radius: there a ray is captured only when
+    // it heads inward (src/trajectory.cpp:377)
+    if (yn[0] < s.r_cap && (!KS || yn[3] < R(0))) {
         res.cls = kCaptured;
         return true;
     }
     if (yn[0] > s.r_esc && yn[3] > R(0)) {
+        if (KS) yn[2] -= R(ks_shift((double)s.p.a, (double)yn[0]));  // the tail runs in BL phi
 #pragma unroll
         for (int i = 0; i < 5; ++i) s.y[i] = yn[i];
         s.tail = true;
@@ -602,9 +643,13 @@ __device__ __forceinline__ bool lane_finish(Lane<R>& s, R err, R h, bool last, R
 }
 
 // One DP5 attempt plus events, all in precision R.
-template <class R>
+template <class R, bool KS = false>
 __device__ __forceinline__ bool lane_step(Lane<R>& s, R rtol, R atol, int max_steps, Result& res) {
     res.flags = 0;
+    if (KS && s.dark) {
+        res.cls = kCaptured;
+        return true;
+    }
     if (s.steps >= max_steps) {
         res.cls = kUnfinished;
         return true;
@@ -618,7 +663,7 @@ __device__ __forceinline__ bool lane_step(Lane<R>& s, R rtol, R atol, int max_st
         last = true;
     }
     const R h = s.h;
-    const R err = dp5_attempt(s.p, s.tail, s.u, s.y, h, k, yn, rtol, atol);
+    const R err = dp5_attempt<R, KS>(s.p, s.tail, s.u, s.y, h, k, yn, rtol, atol);
     ++s.steps;
     // dense output for the disk crossing, taken now so the stage vectors die here
     const bool crossing = !s.tail && s.disk && s.y[1] * yn[1] < R(0);
@@ -627,7 +672,7 @@ __device__ __forceinline__ bool lane_step(Lane<R>& s, R rtol, R atol, int max_st
         dr = dense_coeffs(s.y, yn, k, h, 0);
         dm = dense_coeffs(s.y, yn, k, h, 1);
     }
-    return lane_finish(s, err, h, last, yn, k[6], crossing, dr, dm, res);
+    return lane_finish<R, KS>(s, err, h, last, yn, k[6], crossing, dr, dm, res);
 }
 
 // ---- mixed precision --------------------------------------------------------------

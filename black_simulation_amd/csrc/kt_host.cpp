@@ -141,11 +141,58 @@ void camera_basis(double yaw, double pitch, double roll, double B[9]) {
     }
 }
 
+bool has_u_obs(const kt_view& v) { return v.u_obs[0] != 0.0 || v.u_obs[1] != 0.0 || v.u_obs[2] != 0.0 || v.u_obs[3] != 0.0; }
+
+// the observer of an observer view: u_obs normalised, or the ZAMO boosted by drift
+// (src/camera.cpp:23-36); false if it is not a future-directed timelike vector
+bool observer_u(const kt_view& v, double u[4]) {
+    const double a = v.spin, r = v.r_cam, t = v.theta_cam;
+    double g[4][4];
+    metric_tensor(a, r, t, g);
+    if (has_u_obs(v)) {
+        for (int i = 0; i < 4; ++i) u[i] = v.u_obs[i];
+        const double n = dot4(g, u, u);
+        if (!(n < 0.0) || !(u[0] > 0.0)) return false;
+        const double s = 1.0 / std::sqrt(-n);
+        for (int i = 0; i < 4; ++i) u[i] *= s;
+        return true;
+    }
+    zamo_u(a, r, t, u);
+    const double v2 = v.drift[0] * v.drift[0] + v.drift[1] * v.drift[1] + v.drift[2] * v.drift[2];
+    if (v2 > 0.0 && v2 < 1.0) {  // u = gamma (e0 + v^i e_i) of the ZAMO frame
+        double e[4][4];
+        observer_frame(a, r, t, u, e);
+        const double gamma = 1.0 / std::sqrt(1.0 - v2);
+        for (int i = 0; i < 4; ++i)
+            u[i] = gamma * (e[0][i] + v.drift[0] * e[1][i] + v.drift[1] * e[2][i] + v.drift[2] * e[3][i]);
+    }
+    return true;
+}
+
 const char* check_view(const kt_view& v) {
-    if (v.camera_kind != KT_CAMERA_BARDEEN && v.camera_kind != KT_CAMERA_PINHOLE) return "unknown camera_kind";
+    if (v.camera_kind != KT_CAMERA_BARDEEN && v.camera_kind != KT_CAMERA_PINHOLE &&
+        v.camera_kind != KT_CAMERA_OBSERVER)
+        return "unknown camera_kind";
     if (v.width <= 0 || v.height <= 0) return "image size must be positive";
     if ((long long)v.width * v.height > (1ll << 31)) return "view larger than 2^31 pixels";
     if (!(std::fabs(v.spin) <= 1.0)) return "spin must lie in [-1, 1] (naked singularities are not traced)";
+    if (v.camera_kind == KT_CAMERA_OBSERVER) {
+        if (!(v.fov_deg > 0.0 && v.fov_deg < 180.0)) return "fov out of range";
+        if (!(v.theta_cam > 0.0 && v.theta_cam < kPi)) return "an observer camera must be off the axis";
+        if (has_u_obs(v)) {
+            if (!(v.r_cam > 1.0 - std::sqrt(std::fmax(0.0, 1.0 - v.spin * v.spin)) + 1e-9))
+                return "an observer camera must be outside the inner horizon";
+        } else {
+            if (!(v.r_cam > horizon(v.spin))) return "no observer hovers inside the horizon: give u_obs";
+            const double v2 = v.drift[0] * v.drift[0] + v.drift[1] * v.drift[1] + v.drift[2] * v.drift[2];
+            if (!(v2 < 1.0)) return "drift must be slower than light";
+        }
+        double u[4];
+        if (!observer_u(v, u)) return "u_obs must be a future-directed timelike vector (u.u < 0, u^t > 0)";
+        if (!(v.capture_factor >= 1.0)) return "capture_factor must be >= 1";
+        if (v.disk_enabled && !(v.disk_outer > 0.0)) return "disk_outer must be positive";
+        return nullptr;
+    }
     if (!(v.r_cam > horizon(v.spin))) return "camera must be outside the horizon";
     if (!(v.theta_cam >= 0.0 && v.theta_cam <= kPi)) return "theta_cam must lie in [0, pi]";
     if (!(v.capture_factor >= 1.0)) return "capture_factor must be >= 1";
@@ -185,10 +232,12 @@ ViewDev make_view_dev(const kt_view& v, long long offset) {
     d.r_out = v.disk_outer;
     d.h0 = 0.01 / v.r_cam;
     d.r_fp64 = 1.2 * horizon(a);
-    if (v.camera_kind == KT_CAMERA_PINHOLE) {
+    d.r_plus = horizon(a);
+    if (v.camera_kind == KT_CAMERA_PINHOLE || v.camera_kind == KT_CAMERA_OBSERVER) {
         const double r = v.r_cam, t = v.theta_cam;
         double u[4], e[4][4], g[4][4], low[4][4];
-        zamo_u(a, r, t, u);
+        if (v.camera_kind == KT_CAMERA_OBSERVER) observer_u(v, u);
+        else zamo_u(a, r, t, u);
         observer_frame(a, r, t, u, e);
         metric_tensor(a, r, t, g);
         for (int j = 0; j < 4; ++j) lower4(g, e[j], low[j]);
@@ -331,6 +380,19 @@ int prepare_views(kt_ctx* ctx, const kt_view* views, int n_views, std::vector<Vi
     return KT_OK;
 }
 
+// observer views (Kerr-Schild phi, FP64 only) cannot share a launch with other kinds;
+// sets ks = 1 when the batch is all observer views
+int observer_batch(kt_ctx* ctx, const std::vector<ViewDev>& vd, const kt_trace_opts& o, int& ks) {
+    int n = 0;
+    for (const ViewDev& d : vd) n += d.kind == KT_CAMERA_OBSERVER ? 1 : 0;
+    ks = n > 0 ? 1 : 0;
+    if (n == 0) return KT_OK;
+    if (n != (int)vd.size()) return fail(ctx, KT_EINVAL, "observer cameras cannot share a call with other kinds");
+    if (o.precision != KT_PREC_FP64) return fail(ctx, KT_EINVAL, "observer cameras trace in FP64 (KT_PREC_FP64)");
+    if (o.use_surrogate) return fail(ctx, KT_EINVAL, "the surrogate does not cover observer cameras");
+    return KT_OK;
+}
+
 kti::TraceLaunch make_launch(kt_ctx* ctx, const ViewDev* d_views, int n_views, long long total,
                              unsigned long long* counter, const ktd::Outputs& dout, const kt_trace_opts& o) {
     kti::TraceLaunch L{};
@@ -397,6 +459,7 @@ int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const k
     KT_HIP(hipMemcpyAsync(ctx->d_views, vd.data(), sizeof(ViewDev) * vd.size(), hipMemcpyHostToDevice, stream));
 
     kti::TraceLaunch L = make_launch(ctx, ctx->d_views, (int)vd.size(), total, ctx->d_counter, dout, o);
+    if (int st = observer_batch(ctx, vd, o, L.ks)) return st;
 
     ctx->stats = kt_stats{};
     ctx->stats.rays = total;
@@ -625,10 +688,13 @@ int kt_autotune(kt_ctx* ctx, const kt_view* views, int n_views, kt_trace_opts* o
         if (ensure_views(ctx, (int)vd.size()) != KT_OK) return KT_ENOMEM;
         if (ensure_stage(ctx, (size_t)total + 256) != KT_OK) return KT_ENOMEM;
         KT_HIP(hipMemcpy(ctx->d_views, vd.data(), sizeof(ViewDev) * vd.size(), hipMemcpyHostToDevice));
+        int ks = 0;
+        if (int st = observer_batch(ctx, vd, o, ks)) return st;
         ktd::Outputs dout{};
         dout.cls = static_cast<int8_t*>(ctx->d_stage);  // classes only: the timing is the same
         auto time_of = [&](const kt_trace_opts& c) -> double {
             kti::TraceLaunch L = make_launch(ctx, ctx->d_views, (int)vd.size(), total, ctx->d_counter, dout, c);
+            L.ks = ks;
             double best = 1e30;
             for (int rep = 0; rep < 2; ++rep) {
                 if (hipEventRecord(ctx->ev[1], nullptr) != hipSuccess) return 1e30;
@@ -736,6 +802,10 @@ int kt_plan_create(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
             return cleanup(KT_ENOMEM, "plan fallback queue", e);
         ktd::Outputs dout{out->cls, out->theta_inf, out->phi_inf, out->r_hit, out->g, out->steps};
         p->L = make_launch(ctx, p->d_views, (int)vd.size(), total, p->d_counter, dout, o);
+        if (int st = observer_batch(ctx, vd, o, p->L.ks)) {
+            kt_plan_destroy(p);
+            return st;
+        }
         if ((e = hipStreamCreateWithFlags(&p->capture, hipStreamNonBlocking)) != hipSuccess)
             return cleanup(KT_EHIP, "plan stream", e);
         if ((e = hipStreamBeginCapture(p->capture, hipStreamCaptureModeThreadLocal)) != hipSuccess)
@@ -1285,6 +1355,37 @@ double kt_surrogate_table_radius(kt_ctx* ctx, double a, double inclination, doub
 double kt_surrogate_shadow_radius(kt_ctx* ctx, double a, double inclination) {
     if (!ctx || ensure_surrogate(ctx) != KT_OK) return 0.0;
     return kts::table_shadow_radius(ctx->surrogate, a, inclination);
+}
+
+int kt_geodesic_four_velocity(double a, double E, double L, double Q, double r, double theta, int radial_sign,
+                              int polar_sign, double u[4]) {
+    if (!u || !(std::fabs(a) <= 1.0) || !(r > 0.0)) return KT_EINVAL;
+    // kerr::fourVelocity (src/kerr.cpp:414-418) of a massive body (mu = 1): the Kerr-Schild
+    // derivatives of kerr.cpp:331-351 divided by Sigma, with inflowFactor (kerr.cpp:323-329)
+    const double sn = hsin(theta), cs = hcos(theta);
+    const double sn2 = std::fmax(sn * sn, 1e-18), a2 = a * a;
+    const double delta = r * r - 2.0 * r + a2, rra = r * r + a2;
+    const double P = E * rra - a * L, x = L - a * E;
+    const double K = r * r + x * x + Q;
+    const double R = P * P - delta * K;
+    const double Th = Q - cs * cs * (a2 * (1.0 - E * E) + L * L / sn2);
+    const double tol = 1e-12 * (P * P + std::fabs(delta) * K + std::fabs(Q) + L * L + 1.0);
+    if (R < -tol || Th < -tol) return KT_EINVAL;
+    const double vr = (radial_sign > 0 ? 1.0 : -1.0) * std::sqrt(std::fmax(R, 0.0));
+    const double vth = (polar_sign > 0 ? 1.0 : -1.0) * std::sqrt(std::fmax(Th, 0.0));
+    double inflow;
+    if (P * vr < 0.0) {
+        inflow = K / (P - vr);
+    } else {
+        const double d = std::fabs(delta) > 1e-12 ? delta : std::copysign(1e-12, delta);
+        inflow = (P + vr) / d;
+    }
+    const double sigma = r * r + a2 * cs * cs;
+    u[0] = (rra * inflow - a * (a * E * sn * sn - L) - vr) / sigma;
+    u[1] = vr / sigma;
+    u[2] = vth / sigma;
+    u[3] = (a * inflow - a * E + L / sn2) / sigma;
+    return KT_OK;
 }
 
 double kt_horizon_radius(double a) { return horizon(a); }

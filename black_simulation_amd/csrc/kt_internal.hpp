@@ -26,6 +26,7 @@ struct TraceLaunch {
     int precision;              // KT_PREC_*
     int mode;                   // KT_MODE_*
     int block, grid, steps_per_dequeue, waves_per_simd;
+    int ks;                     // 1: observer cameras (FP64, Kerr-Schild phi)
 };
 
 // launches the counter reset and the trace kernel on `stream`; returns a hipError_t

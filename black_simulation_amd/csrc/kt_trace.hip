@@ -57,6 +57,17 @@ __device__ bool Plain<float>::step(Lane<float>& s, const kti::TraceLaunch& L, Re
     return lane_step(s, (float)L.rtol32, (float)L.atol32, L.max_steps, r);
 }
 
+// Observer cameras (kind 3): FP64 with the Kerr-Schild phi, inside the horizon allowed.
+struct PlainKS {
+    using LaneT = Lane<double>;
+    __device__ static void start(LaneT& s, const Launch& la) { lane_start<double, true>(s, la); }
+    __device__ static bool step(LaneT& s, const kti::TraceLaunch& L, Result& r) {
+        return lane_step<double, true>(s, L.rtol, L.atol, L.max_steps, r);
+    }
+    __device__ static int steps(const LaneT& s) { return s.steps; }
+    __device__ static int fp64_steps(const LaneT&) { return 0; }
+};
+
 struct Mixed {
     using LaneT = MixedLane;
     __device__ static void start(LaneT& s, const Launch& la) {
@@ -193,7 +204,10 @@ void launch_oneshot(const TraceLaunch& L, hipStream_t stream) {
 hipError_t launch_trace(const TraceLaunch& L, hipStream_t stream) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, L.counter, L.stats);
     if (L.total <= 0) return hipGetLastError();
-    if (L.mode == 1) {
+    if (L.ks) {
+        if (L.mode == 1) launch_oneshot<PlainKS>(L, stream);
+        else launch_persistent<PlainKS>(L, stream);
+    } else if (L.mode == 1) {
         if (L.precision == 2) launch_oneshot<Plain<float>>(L, stream);
         else if (L.precision == 1) launch_oneshot<Mixed>(L, stream);
         else launch_oneshot<Plain<double>>(L, stream);

@@ -66,7 +66,7 @@ enum {
     KT_FLAG_NONFINITE = 0x40     /* the state went non-finite: counted as captured */
 };
 
-enum { KT_CAMERA_BARDEEN = 0, KT_CAMERA_PINHOLE = 1 };
+enum { KT_CAMERA_BARDEEN = 0, KT_CAMERA_PINHOLE = 1, KT_CAMERA_OBSERVER = 3 };
 enum { KT_PREC_FP64 = 0, KT_PREC_MIXED = 1, KT_PREC_FP32 = 2 };
 enum { KT_MODE_PERSISTENT = 0, KT_MODE_ONESHOT = 1, KT_MODE_AUTO = 2 /* the tuned one, else persistent */ };
 
@@ -75,7 +75,15 @@ enum { KT_MODE_PERSISTENT = 0, KT_MODE_ONESHOT = 1, KT_MODE_AUTO = 2 /* the tune
  * alpha = -half_width + (col + 0.5) 2 half_width / width, beta likewise from the top
  * (src/eht.cpp:332-334).  Pinhole views are the simulator's camera
  * (src/camera.h:20-29: pose, yaw/pitch/roll, vertical fov) in the ZAMO frame.
- * Disk fields follow trajectory::RayScene (src/trajectory.h:74-83). */
+ * Observer views are the same camera carried by any observer (Camera::frame,
+ * src/camera.cpp:23-40, and kerr::observerFrame, src/kerr.cpp:432-456): the 4-velocity
+ * u_obs (ingoing Kerr-Schild components, normalised here), or, when u_obs is all zero,
+ * the ZAMO boosted by `drift` (its 3-velocity along the ZAMO's e_r, e_theta, e_phi,
+ * |drift| < 1).  With u_obs the camera may sit inside the horizon (above r-): rays that
+ * reach the horizon from the far side are captured at once (src/trajectory.cpp:234-241),
+ * the others are followed out through it.  For observer views phi_cam is the Kerr-Schild
+ * phi, as in the simulator; they trace in FP64 only and cannot share a call with other
+ * kinds.  Disk fields follow trajectory::RayScene (src/trajectory.h:74-83). */
 typedef struct kt_view {
     int camera_kind;
     int width, height;
@@ -87,6 +95,8 @@ typedef struct kt_view {
     int disk_prograde, disk_enabled;
     double escape_radius;           /* escape when r > max(escape_radius, 1.5 r_cam), outgoing */
     double capture_factor;          /* captured when r < capture_factor * r_+ (e.g. 1.01) */
+    double u_obs[4];                /* KT_CAMERA_OBSERVER: u^t, u^r, u^theta, u^phi (KS)  */
+    double drift[3];                /* KT_CAMERA_OBSERVER with u_obs = 0: ZAMO boost      */
 } kt_view;
 
 typedef struct kt_trace_opts {
@@ -302,6 +312,16 @@ int kt_eht_calibrate(kt_ctx* ctx, const kt_eht_view* view, const kt_eht_flow* fl
                      double theta_guas, const kt_trace_opts* opts, double* n0, void* hip_stream);
 /* eht::angularScale (src/eht.cpp:46-48): microarcseconds that M subtends */
 double kt_eht_angular_scale(double mass_solar, double distance_pc);
+
+/* The 4-velocity (ingoing Kerr-Schild, contravariant) of a free-falling massive observer
+ * with constants E, L, Q at (r, theta), moving outward in r when radial_sign > 0 and
+ * toward larger theta when polar_sign > 0: kerr::fourVelocity (src/kerr.cpp:414-418) with
+ * Sigma dr/dtau = radial_sign sqrt(R), Sigma dtheta/dtau = polar_sign sqrt(Theta).  For
+ * KT_CAMERA_OBSERVER views (u_obs), e.g. a camera falling from rest at infinity (E = 1,
+ * L = Q = 0, radial_sign = -1), inside the horizon too.  Returns KT_EINVAL where R or
+ * Theta is negative by more than rounding. */
+int kt_geodesic_four_velocity(double a, double E, double L, double Q, double r, double theta, int radial_sign,
+                              int polar_sign, double u[4]);
 
 /* Host-side closed forms (src/kerr.cpp:157-206, 296-299). */
 double kt_horizon_radius(double a);

@@ -269,6 +269,7 @@ void ko_photon_init(ko_photon* p, double a, double E, double L, double Q) {
     p->C1 = (Q + L * L) - p->A2E2;
     p->twoE = 2.0 * E;
     p->twoA2E2 = 2.0 * p->A2E2;
+    p->ks = 0;
 }
 
 /* P(r) = E (r^2 + a^2) - a L and Delta(r) = r^2 - 2r + a^2 */
@@ -299,10 +300,24 @@ void ko_rhs(const ko_photon* p, const double y[5], double dy[5]) {
     const double delta = delta_of(p, r);
     const double P = P_of(p, r);
     const double s2 = fmax(fma(-m, m, 1.0), KO_TINY);
-    const double num = fma(p->a * P, s2, p->L * delta);
+    /* Kerr-Schild phi (ks): dphi_KS = dphi_BL + a dr / Delta, so P / Delta becomes
+       (P - w_r) / Delta = inflowFactor of kerr.cpp:323-329 in backward time: K / (P + w_r)
+       when P w_r > 0 (by w_r^2 = P^2 - Delta K), else with Delta floored at 1e-12 */
+    double nd = P, dd = delta;
+    if (p->ks) {
+        const double w = y[3];
+        if (P * w > 0.0) {
+            nd = p->K;
+            dd = P + w;
+        } else {
+            nd = P - w;
+            dd = fabs(delta) > 1e-12 ? delta : copysign(1e-12, delta);
+        }
+    }
+    const double num = fma(p->a * nd, s2, p->L * dd);
     dy[0] = y[3];
     dy[1] = y[4];
-    dy[2] = p->aE - num / (delta * s2);
+    dy[2] = p->aE - num / (dd * s2);
     dy[3] = fma(p->twoE * r, P, -((r - 1.0) * p->K));
     dy[4] = -(m * fma(p->twoA2E2, m * m, p->C1));
 }
@@ -564,11 +579,37 @@ void ko_launch_look(const ko_view* v, const double look[3], ko_photon* p, double
     pinhole_photon(v, look[0], look[1], look[2], p, y0);
 }
 
-/* the photon arriving from unit direction n of the camera's ZAMO frame */
+/* the observer of an observer view (camera.cpp:23-36): u_obs normalised to u.u = -1, or
+   the ZAMO boosted by drift, u = gamma (e0 + v^i e_i) */
+void ko_observer_u(const ko_view* v, double u[4]) {
+    const double a = v->spin, r = v->r_cam, th = v->theta_cam;
+    if (v->u_obs[0] != 0.0 || v->u_obs[1] != 0.0 || v->u_obs[2] != 0.0 || v->u_obs[3] != 0.0) {
+        double g[4][4], n = 0.0;
+        ko_metric_tensor(a, r, th, g);
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) n += g[i][j] * v->u_obs[i] * v->u_obs[j];
+        const double s = 1.0 / sqrt(-n);
+        for (int i = 0; i < 4; ++i) u[i] = v->u_obs[i] * s;
+        return;
+    }
+    ko_zamo_four_velocity(a, r, th, u);
+    const double v2 = v->drift[0] * v->drift[0] + v->drift[1] * v->drift[1] + v->drift[2] * v->drift[2];
+    if (v2 > 0.0 && v2 < 1.0) {
+        double e[4][4];
+        ko_observer_frame(a, r, th, u, e);
+        const double gamma = 1.0 / sqrt(1.0 - v2);
+        for (int i = 0; i < 4; ++i)
+            u[i] = gamma * (e[0][i] + v->drift[0] * e[1][i] + v->drift[1] * e[2][i] + v->drift[2] * e[3][i]);
+    }
+}
+
+/* the photon arriving from unit direction n of the camera's frame (the ZAMO's, or the
+   observer's for observer views) */
 static void pinhole_photon(const ko_view* v, double n0, double n1, double n2, ko_photon* p, double y0[5]) {
     const double a = v->spin, r = v->r_cam, th = v->theta_cam;
     double u[4], e[4][4], g[4][4], low[4][4];
-    ko_zamo_four_velocity(a, r, th, u);
+    if (v->camera_kind == KO_CAMERA_OBSERVER) ko_observer_u(v, u);
+    else ko_zamo_four_velocity(a, r, th, u);
     ko_observer_frame(a, r, th, u, e);
     ko_metric_tensor(a, r, th, g);
     for (int j = 0; j < 4; ++j) lower4(g, e[j], low[j]);
@@ -588,6 +629,7 @@ static void pinhole_photon(const ko_view* v, double n0, double n1, double n2, ko
         Q = qh * qh - cs2 * a2 * E * E;
     }
     ko_photon_init(p, a, E, L, Q);
+    p->ks = v->camera_kind == KO_CAMERA_OBSERVER;
     y0[0] = r;
     y0[1] = cs;
     y0[2] = v->phi_cam;
@@ -596,7 +638,7 @@ static void pinhole_photon(const ko_view* v, double n0, double n1, double n2, ko
 }
 
 void ko_launch_pixel(const ko_view* v, int col, int row, ko_photon* p, double y0[5]) {
-    if (v->camera_kind == KO_CAMERA_PINHOLE) {
+    if (v->camera_kind == KO_CAMERA_PINHOLE || v->camera_kind == KO_CAMERA_OBSERVER) {
         launch_pinhole(v, col, row, p, y0);
         return;
     }
@@ -610,6 +652,15 @@ void ko_launch_pixel(const ko_view* v, int col, int row, ko_photon* p, double y0
 /* ------------------------------------------------------------------------- */
 /* the tracer                                                                 */
 /* ------------------------------------------------------------------------- */
+
+/* phi_KS - phi_BL = psi(r), the integral of a / Delta from infinity: zero at infinity */
+double ko_ks_shift(double a, double r) {
+    if (a == 0.0) return 0.0;
+    const double d = 2.0 * sqrt(fmax(0.0, 1.0 - a * a));
+    const double rm = 1.0 - 0.5 * d;
+    if (d == 0.0) return -a / (r - 1.0);
+    return a / d * log1p(-d / (r - rm));
+}
 
 static void finish_escape(const ko_photon* p, const ko_opts* o, const double y[5], int* steps, ko_result* out) {
     double z[3] = {y[1], y[2], y[4]}, zn[3], k[7][3];
@@ -672,6 +723,15 @@ void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, con
     out->theta_inf = out->phi_inf = out->r_hit = out->g = NAN;
     int steps = 0, rejected = 0;
     double h = 0.01 / v->r_cam;
+    /* inside the horizon every ray, followed back, rises to it; one arriving with
+       P(r+) = 2 r+ E - a L < 0 came from the far side: dark (trajectory.cpp:234-241) */
+    const double rh = ko_horizon_radius(a);
+    if (p->ks && v->r_cam < rh && 2.0 * rh * p->E - a * p->L < 0.0) {
+        out->cls = KO_CAPTURED;
+        for (int i = 0; i < 5; ++i) out->y_end[i] = y[i];
+        out->steps = 0;
+        return;
+    }
     ko_rhs(p, y, k[0]);
     while (steps < o->max_steps) {
         const double err = dp5_main(p, o, y, h, k, yn);
@@ -723,13 +783,15 @@ void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, con
                 }
             }
         }
-        if (yn[0] < r_cap) {
+        /* observer cameras may start below r_cap: capture only heading inward (trajectory.cpp:377) */
+        if (yn[0] < r_cap && (!p->ks || yn[3] < 0.0)) {
             out->cls = KO_CAPTURED;
             for (int i = 0; i < 5; ++i) out->y_end[i] = yn[i];
             out->steps = steps;
             return;
         }
         if (yn[0] > r_esc && yn[3] > 0.0) {
+            if (p->ks) yn[2] -= ko_ks_shift(a, yn[0]); /* the tail runs in Boyer-Lindquist phi */
             finish_escape(p, o, yn, &steps, out);
             out->steps = steps;
             return;

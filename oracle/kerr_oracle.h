@@ -49,7 +49,7 @@ extern "C" {
 enum { KO_ESCAPED = 0, KO_CAPTURED = 1, KO_DISK = 2, KO_UNFINISHED = 3 };
 enum { KO_FLAG_NONFINITE = 0x40 };
 
-enum { KO_CAMERA_BARDEEN = 0, KO_CAMERA_PINHOLE = 1 };
+enum { KO_CAMERA_BARDEEN = 0, KO_CAMERA_PINHOLE = 1, KO_CAMERA_OBSERVER = 3 };
 
 /* ---- radii and circular orbits (kerr.cpp:157-223, 296-299) ---- */
 double ko_horizon_radius(double a);
@@ -84,6 +84,7 @@ void ko_camera_basis(double yaw, double pitch, double roll, double basis[9]);
 typedef struct {
     double a, a2, E, L, Q;
     double aE, aL, K, C1, A2E2, twoE, twoA2E2;
+    int ks; /* 1: phi is the ingoing Kerr-Schild phi (observer cameras) */
 } ko_photon;
 
 void ko_photon_init(ko_photon* p, double a, double E, double L, double Q);
@@ -105,6 +106,8 @@ typedef struct {
     int disk_prograde, disk_enabled;
     double escape_radius;                    /* effective: max(this, 1.5 r_cam)    */
     double capture_factor;                   /* captured when r < this * r_+       */
+    double u_obs[4];                         /* observer: KS 4-velocity, or zero   */
+    double drift[3];                         /* observer, u_obs zero: ZAMO boost   */
 } ko_view;
 
 typedef struct {
@@ -123,6 +126,10 @@ void ko_launch_pixel(const ko_view* v, int col, int row, ko_photon* p, double y0
 void ko_launch_bardeen(const ko_view* v, double alpha, double beta, ko_photon* p, double y0[5]);
 /* pinhole view, explicit unit look vector on the ZAMO frame's (r, theta, phi) axes */
 void ko_launch_look(const ko_view* v, const double look[3], ko_photon* p, double y0[5]);
+/* observer views: the camera's 4-velocity (camera.cpp:23-36) */
+void ko_observer_u(const ko_view* v, double u[4]);
+/* phi_KS - phi_BL at r (zero at infinity) */
+double ko_ks_shift(double a, double r);
 
 void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, const double y0[5],
                      ko_result* out);

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_LIB_PATH = os.path.join(HERE, "_ref", "libkerr_ref.so")
 
 ESCAPED, CAPTURED, DISK, UNFINISHED = 0, 1, 2, 3
-BARDEEN, PINHOLE = 0, 1
+BARDEEN, PINHOLE, OBSERVER = 0, 1, 3
 
 
 class View(C.Structure):
@@ -43,6 +43,8 @@ class View(C.Structure):
         ("disk_enabled", C.c_int),
         ("escape_radius", C.c_double),
         ("capture_factor", C.c_double),
+        ("u_obs", C.c_double * 4),
+        ("drift", C.c_double * 3),
     ]
 
 
@@ -52,7 +54,7 @@ class Opts(C.Structure):
 
 class Photon(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("a", "a2", "E", "L", "Q", "aE", "aL", "K", "C1", "A2E2", "twoE",
-                                          "twoA2E2")]
+                                          "twoA2E2")] + [("ks", C.c_int)]
 
 
 class Result(C.Structure):
@@ -139,6 +141,8 @@ def lib():
             ("ko_launch_pixel", None, [P(View), i, i, P(Photon), P(d)]),
             ("ko_launch_bardeen", None, [P(View), d, d, P(Photon), P(d)]),
             ("ko_launch_look", None, [P(View), P(d), P(Photon), P(d)]),
+            ("ko_observer_u", None, [P(View), P(d)]),
+            ("ko_ks_shift", d, [d, d]),
             ("ko_trace_photon", None, [P(View), P(Opts), P(Photon), P(d), P(Result)]),
             ("ko_trace_pixel", None, [P(View), P(Opts), i, i, P(Result)]),
             ("ko_render", C.c_long, [P(View), P(Opts), i, P(C.c_int), i, P(C.c_int8), P(d), P(d), P(d), P(d),
@@ -178,6 +182,10 @@ def make_view(**kw) -> View:
                     disk_prograde=1, disk_enabled=1, escape_radius=1000.0, capture_factor=1.01)
     defaults.update(kw)
     for k, val in defaults.items():
+        if k == "u_obs":
+            val = (C.c_double * 4)(*[float(x) for x in val])
+        elif k == "drift":
+            val = (C.c_double * 3)(*[float(x) for x in val])
         setattr(v, k, val)
     return v
 

@@ -200,3 +200,40 @@ double ref_eht_calibrate(double a, double inclination, double half_width, double
 }
 
 }  // extern "C"
+
+// ---- observers other than the static ZAMO: boosted, falling, inside the horizon ----------
+
+extern "C" {
+
+// kerr::fourVelocity; consts = {E, L, Q, mu}, state = {t, r, theta, phi, vr, vtheta}
+void ref_four_velocity(double a, const double* consts, const double* state, double* u) {
+    kerr::Vec4 v = kerr::fourVelocity(a, {consts[0], consts[1], consts[2], consts[3]},
+                                      {state[0], state[1], state[2], state[3], state[4], state[5]});
+    for (int i = 0; i < 4; ++i) u[i] = v[i];
+}
+
+// trajectory::traceRay from the frame of the observer of 4-velocity u (Kerr-Schild);
+// out as ref_trajectory_trace
+void ref_trajectory_trace_u(double a, double r, double theta, double phi, const double* u, const double* look,
+                            int disk, double disk_inner, double disk_outer, double escape, double* out) {
+    trajectory::RayScene scene;
+    scene.disk = disk != 0;
+    scene.diskInner = disk_inner;
+    scene.diskOuter = disk_outer;
+    scene.escapeRadius = escape;
+    kerr::Frame f = kerr::observerFrame(a, r, theta, {u[0], u[1], u[2], u[3]});
+    trajectory::RayReport rep = trajectory::traceRay(a, r, theta, phi, f, look, scene);
+    out[0] = static_cast<double>(static_cast<int>(rep.end));
+    out[1] = rep.rEnd;
+    out[2] = rep.g;
+    out[3] = rep.turns;
+    out[4] = rep.rMin;
+    out[5] = rep.skyDirection[0];
+    out[6] = rep.skyDirection[1];
+    out[7] = rep.skyDirection[2];
+    out[8] = rep.energy;
+    out[9] = rep.b;
+    out[10] = rep.q;
+}
+
+}  // extern "C"
