@@ -53,7 +53,7 @@ def cpu_baseline(view, seconds_target: float = 15.0) -> dict:
     ov = O.make_view(**{k: getattr(view, k) for k in (
         "camera_kind", "width", "height", "spin", "r_cam", "theta_cam", "phi_cam", "half_width", "half_height",
         "yaw", "pitch", "roll", "fov_deg", "disk_inner", "disk_outer", "disk_prograde", "disk_enabled",
-        "escape_radius", "capture_factor")})
+        "escape_radius", "capture_factor", "u_obs", "drift")})
     opts = O.make_opts()
     stride = 4099
     t0 = time.perf_counter()
@@ -96,7 +96,7 @@ def main():
     ap.add_argument("--steps", type=int, default=120)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--path", choices=["surrogate", "plain", "mixed"], default="surrogate")
-    ap.add_argument("--workload", choices=["orbit", "c2", "c3", "eht"], default="orbit")
+    ap.add_argument("--workload", choices=["orbit", "c2", "c3", "eht", "infall"], default="orbit")
     ap.add_argument("--size", type=int, default=1024, help="--workload eht: pixels across")
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--weights", default=None)
@@ -118,6 +118,9 @@ def main():
     if args.path == "surrogate" and args.workload == "c2":
         print("the surrogate is trained for the orbit scene (disk to 20 M); use --workload orbit", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "infall" and args.path != "plain":
+        print("observer cameras trace in FP64: --workload infall runs --path plain", file=sys.stderr)
+        args.path = "plain"
 
     import torch
 
@@ -137,7 +140,7 @@ def main():
 
     import black_simulation_amd as bsa
     from black_simulation_amd import orbit as O
-    from black_simulation_amd.configs import C5_FRAMES, c2_view, c3_views, c5_view
+    from black_simulation_amd.configs import C5_FRAMES, c2_view, c3_views, c5_view, infall_view
     from black_simulation_amd.train_surrogate import DEFAULT_WEIGHTS, load_params
 
     width = args.height * 16 // 9
@@ -151,7 +154,8 @@ def main():
             frames.append(k)
         warm = [(C5_FRAMES // 2 + i) % C5_FRAMES for i in range(args.warmup)]
     else:  # one frame (a view, or the 64-view batch), a copy per stream so that replays can overlap
-        one = [c2_view(width, args.height)] if args.workload == "c2" else c3_views()
+        one = ([c2_view(width, args.height)] if args.workload == "c2" else
+               [infall_view(width, args.height)] if args.workload == "infall" else c3_views())
         views = [one] * args.streams
         frames = [i % args.streams for i in range(args.steps)]
         warm = [i % args.streams for i in range(args.warmup)]
@@ -288,9 +292,12 @@ def main():
             "c2": f"single frame a=0.998, r=50 M, i=85 deg, {width}x{args.height} Bardeen plane 30 M wide, "
                   f"disk r_ISCO..40 M",
             "c3": f"64 seeded views (seed 0x5EED0003) of uneven size, spin and camera radius in one launch, "
-                  f"{rays} rays"}[args.workload]
+                  f"{rays} rays",
+            "infall": f"camera falling from rest at infinity, inside the a=0.9 horizon at r=1.2 M, theta 60 deg, "
+                      f"{width}x{args.height} pinhole 120 deg (Kerr-Schild phi, far-side rule), disk r_ISCO..20 M"}[
+        args.workload]
     rec = {
-        "metric": METRIC,
+        "metric": METRIC if args.workload == "orbit" else f"Mrays/s, {args.workload} workload, {args.path} path",
         "value": round(value, 3),
         "unit": "Mrays/s",
         "n_gpus": 1,

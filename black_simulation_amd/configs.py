@@ -62,3 +62,17 @@ def c5_view(k: int, width: int = 1920, height: int = 1080) -> View:
                 theta_cam=inc, phi_cam=2.0 * math.pi * k / C5_FRAMES,
                 half_width=hw, half_height=hw * height / width, disk_inner=0.0, disk_outer=20.0,
                 escape_radius=1000.0, capture_factor=1.01)
+
+
+def infall_view(width: int = 1920, height: int = 1080, r: float = 1.2, spin: float = 0.9) -> View:
+    """A camera falling from rest at infinity (E = 1, L = Q = 0), here inside the horizon
+    of a = 0.9 (r+ = 1.436 M) at r = 1.2 M, 60 degrees off the axis, looking down at the
+    hole with a 120-degree field (disk r_ISCO..20 M, escape 1000 M)."""
+    from .api import geodesic_four_velocity
+    from ._lib import CAMERA_OBSERVER
+
+    th = math.radians(60.0)
+    u = geodesic_four_velocity(spin, 1.0, 0.0, 0.0, r, th, -1, 1)
+    return View(camera_kind=CAMERA_OBSERVER, width=width, height=height, spin=spin, r_cam=r, theta_cam=th,
+                phi_cam=0.0, fov_deg=120.0, disk_inner=0.0, disk_outer=20.0, escape_radius=1000.0,
+                capture_factor=1.01, u_obs=u)
