@@ -36,6 +36,11 @@ METRIC = "Mrays/s 1080p a=0.998: surrogate+fallback vs plain integrator @≤1e-4
 #   controller (x^-1/5 by three Newton steps, safety, h update)   =  30
 #   constraint projection (R, M, two zone tests, two sqrt)         =  31
 OPS_PER_ATTEMPT = 522
+# One DP5 attempt of a massive particle in coordinate time (kt_particles.hip), counted the
+# same way: 6 right-hand sides x 40 (three divisions each: the inflow factor, L / sin^2 and
+# 1 / (dt/dlambda)) = 240, stage sums 230, error estimate and max-norm 81, controller 30,
+# projection 35, and the right-hand side re-evaluated after each accepted step 40
+OPS_PER_PARTICLE_ATTEMPT = 656
 # The network 5-256-256-256-256-8: 2 (5*256 + 3*256*256 + 256*8) multiply-adds per ray
 MLP_FLOP_PER_RAY = 2 * (5 * 256 + 3 * 256 * 256 + 256 * 8)  # 399,872
 FP64_VECTOR_PEAK_TFLOPS = 78.6
@@ -96,7 +101,8 @@ def main():
     ap.add_argument("--steps", type=int, default=120)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--path", choices=["surrogate", "plain", "mixed"], default="surrogate")
-    ap.add_argument("--workload", choices=["orbit", "c2", "c3", "eht", "infall"], default="orbit")
+    ap.add_argument("--workload", choices=["orbit", "c2", "c3", "eht", "infall", "debris"], default="orbit")
+    ap.add_argument("--particles", type=int, default=1 << 22, help="--workload debris: particles")
     ap.add_argument("--size", type=int, default=1024, help="--workload eht: pixels across")
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--weights", default=None)
@@ -136,6 +142,9 @@ def main():
 
     if args.workload == "eht":
         eht_bench(args, torch)
+        return
+    if args.workload == "debris":
+        debris_bench(args, torch)
         return
 
     import black_simulation_amd as bsa
@@ -431,6 +440,108 @@ def eht_bench(args, torch):
     if not args.no_cpu:
         rec["cpu_baseline"] = eht_cpu_baseline(view, flow)
     print(json.dumps(rec), flush=True)
+
+
+def debris_bench(args, torch):
+    """--workload debris: 2^22 particles (a = 0.9) launched from the local ZAMOs at r in
+    [2.5, 40] M with speeds up to 0.75 c, one step = one debris update of dt = 10 M of
+    Kerr-Schild time for every moving particle, FP64."""
+    import numpy as np
+
+    import black_simulation_amd as bsa
+    from black_simulation_amd.particles import COLUMNS, Particles
+
+    a, n, dt = 0.9, args.particles, 10.0
+    rng = np.random.default_rng(0xDEB2)
+    r = rng.uniform(2.5, 40.0, n)
+    th = np.arccos(rng.uniform(-0.95, 0.95, n))
+    ph = rng.uniform(0.0, 2.0 * np.pi, n)
+    v = rng.normal(size=(n, 3))
+    v *= (rng.uniform(0.0, 0.75, n) / np.linalg.norm(v, axis=1))[:, None]
+    # the launch closed form, vectorised (kerr::launchFromZamo with mu = 1, as kt_particle_from_zamo)
+    sn, cs = np.sin(th), np.cos(th)
+    sig, dl = r * r + a * a * cs * cs, r * r - 2 * r + a * a
+    A = (r * r + a * a) ** 2 - a * a * dl * sn * sn
+    g = 1.0 / np.sqrt(1.0 - np.minimum((v * v).sum(1), 1 - 1e-12))
+    L = g * v[:, 2] * np.sqrt(A / sig) * sn
+    E = g * np.sqrt(sig * dl / A) + 2 * a * r / A * L
+    vth = g * v[:, 1] * np.sqrt(sig)
+    cols = {"r": r, "mu": cs, "phi": ph, "vr": g * v[:, 0] * np.sqrt(sig * dl), "vmu": -sn * vth, "E": E, "L": L,
+            "Q": vth * vth + cs * cs * (a * a * (1 - E * E) + L * L / np.maximum(sn * sn, 1e-18))}
+    tr = bsa.Tracer(0)
+    p = Particles(cols)
+    start = {c: getattr(p, c).clone() for c in COLUMNS}
+    fate0 = p.fate.clone()
+    for _ in range(args.warmup):
+        p.advance(tr, a, dt)
+    for c in COLUMNS:  # timed from the same start
+        getattr(p, c).copy_(start[c])
+    p.fate.copy_(fate0)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    attempts = 0
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        p.advance(tr, a, dt)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1) / args.steps
+    # attempts of the last update, per particle (the steps column)
+    attempts = int(p.steps.to(torch.int64).sum().item())
+    moving = int((p.fate == 0).sum().item())
+    value = n * args.steps / wall / 1e6
+    ach = attempts * OPS_PER_PARTICLE_ATTEMPT / (kernel_ms * 1e-3) / 1e12
+    rec = {
+        "metric": "Mparticle-updates/s, debris workload (a=0.9, dt=10 M per update), FP64",
+        "value": round(value, 3), "unit": "Mparticles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic: seeded debris launched from local ZAMOs",
+        "config": {"workload": f"debris: {n} particles, a=0.9, r 2.5..40 M, |v| < 0.75 c, dt=10 M per update",
+                   "particles": n, "moving_after": moving,
+                   "mean_attempts_last_update": round(attempts / max(1, n), 1)},
+        "roofline": {"bound": "valu", "achieved": round(ach, 3), "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(ach / FP64_VECTOR_PEAK_TFLOPS, 4), "traffic": None, "kernel": "k_advance",
+                     "kernel_ms": round(kernel_ms, 3), "dp5_attempts_last_update": attempts,
+                     "ops_per_attempt": OPS_PER_PARTICLE_ATTEMPT,
+                     "note": "attempts of the last update over the mean update time"},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        rec["cpu_baseline"] = debris_cpu_baseline(a, {c: start[c][: 1 << 16].cpu().numpy() for c in COLUMNS}, dt)
+    tr.close()
+    print(json.dumps(rec), flush=True)
+
+
+def debris_cpu_baseline(a, cols, dt, seconds_target: float = 10.0) -> dict:
+    """The oracle's particle advance (same DP5) on the host cores, one particle per call
+    from a thread pool, over the first particles of the workload."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle as O
+
+    cores = os.cpu_count() or 1
+    opts = O.make_opts()
+    rs = O.lib().ko_horizon_radius(a) + 0.02
+
+    def one(i):
+        st = [cols[c][i] for c in ("r", "mu", "phi", "vr", "vmu")]
+        return O.particle_advance(a, cols["E"][i], cols["L"][i], cols["Q"][i], st, dt, rs, 1e4, opts)[0]
+
+    def run(k):
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(cores) as ex:
+            list(ex.map(one, range(k), chunksize=64))
+        return time.perf_counter() - t0
+
+    k = min(len(cols["r"]), 4096)
+    dt0 = run(k)
+    k = int(min(len(cols["r"]), max(k, k / dt0 * seconds_target)))
+    t = run(k)
+    return {"value": round(k / t / 1e6, 5), "unit": "Mparticles/s", "cores": cores, "kind": "port",
+            "sample": f"the first {k} particles of the workload, one update of dt={dt} M each ({t:.1f} s, {cores} threads)"}
 
 
 if __name__ == "__main__":

@@ -15,6 +15,7 @@ KT_OK, KT_EINVAL, KT_EHIP, KT_ECANCELLED, KT_ENOMEM, KT_EUNSUPPORTED, KT_ENODEVI
 ESCAPED, CAPTURED, DISK, UNFINISHED = 0, 1, 2, 3
 FLAG_FP64_RESTEP, FLAG_SURROGATE = 0x10, 0x20
 CAMERA_BARDEEN, CAMERA_PINHOLE, CAMERA_OBSERVER = 0, 1, 3
+PARTICLE_MOVING, PARTICLE_PLUNGED, PARTICLE_ESCAPED, PARTICLE_UNFINISHED = 0, 1, 2, 3
 PREC_FP64, PREC_MIXED, PREC_FP32 = 0, 1, 2
 MODE_PERSISTENT, MODE_ONESHOT, MODE_AUTO = 0, 1, 2
 
@@ -142,6 +143,11 @@ class KtEhtOut(C.Structure):
                 ("on_device", C.c_int)]
 
 
+class KtParticles(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("r", "mu", "phi", "vr", "vmu", "E", "L", "Q", "fate", "steps")] + \
+        [("on_device", C.c_int)]
+
+
 class KtShadeOpts(C.Structure):
     _fields_ = [
         ("disk_temperature", C.c_double),
@@ -201,6 +207,9 @@ SIGNATURES = {
     "kt_eht_angular_scale": (C.c_double, [C.c_double, C.c_double]),
     "kt_geodesic_four_velocity": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                             C.c_int, C.c_int, _P(C.c_double)]),
+    "kt_particles_advance": (C.c_int, [C.c_void_p, C.c_double, _P(KtParticles), C.c_longlong, C.c_double,
+                                       C.c_double, C.c_double, _P(KtTraceOpts), C.c_void_p]),
+    "kt_particle_from_zamo": (C.c_int, [C.c_double, C.c_double, C.c_double, _P(C.c_double), _P(C.c_double)]),
     "kt_horizon_radius": (C.c_double, [C.c_double]),
     "kt_isco_radius": (C.c_double, [C.c_double, C.c_int]),
     "kt_photon_orbit_radius": (C.c_double, [C.c_double, C.c_int]),

@@ -8,7 +8,7 @@ import sys
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG_DIR, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["kt_host.cpp", "kt_trace.hip", "kt_nn.hip", "kt_surrogate.hip", "kt_shade.hip", "kt_eht.hip"]
+SOURCES = ["kt_host.cpp", "kt_trace.hip", "kt_nn.hip", "kt_surrogate.hip", "kt_shade.hip", "kt_eht.hip", "kt_particles.hip"]
 
 # -ffp-contract=off: the FP64 step loop must round exactly as written (its results are
 # bit-identical to any IEEE implementation doing the same operations).  Packed FP32 is

@@ -20,6 +20,7 @@
 #include "kt_nn.hpp"
 #include "kt_shade.hpp"
 #include "kt_eht.hpp"
+#include "kt_particles.hpp"
 
 using ktd::ViewDev;
 
@@ -1386,6 +1387,102 @@ int kt_geodesic_four_velocity(double a, double E, double L, double Q, double r, 
     u[2] = vth / sigma;
     u[3] = (a * inflow - a * E + L / sn2) / sigma;
     return KT_OK;
+}
+
+int kt_particle_from_zamo(double a, double r, double theta, const double v[3], double out[5]) {
+    if (!v || !out || !(std::fabs(a) <= 1.0) || !(r > horizon(a))) return KT_EINVAL;
+    // kerr::launchFromZamo with mu = 1 (src/kerr.cpp:365-406): the ZAMO frame's lapse and
+    // circumference radius give E and L, Sigma Delta and Sigma the radial and polar momenta
+    const double sn = hsin(theta), cs = hcos(theta), a2 = a * a;
+    const double sigma = r * r + a2 * cs * cs, delta = r * r - 2.0 * r + a2;
+    const double A = (r * r + a2) * (r * r + a2) - a2 * delta * sn * sn;
+    const double omega = 2.0 * a * r / A;
+    const double v2 = std::min(v[0] * v[0] + v[1] * v[1] + v[2] * v[2], 1.0 - 1e-12);
+    const double gamma = 1.0 / std::sqrt(1.0 - v2);
+    const double L = gamma * v[2] * std::sqrt(A / sigma) * sn;
+    const double E = gamma * std::sqrt(sigma * delta / A) + omega * L;
+    const double vr = gamma * v[0] * std::sqrt(sigma * delta);
+    const double vth = gamma * v[1] * std::sqrt(sigma);
+    const double Q = vth * vth + cs * cs * (a2 * (1.0 - E * E) + L * L / std::max(sn * sn, 1e-18));
+    out[0] = E;
+    out[1] = L;
+    out[2] = Q;
+    out[3] = vr;
+    out[4] = -sn * vth;
+    return KT_OK;
+}
+
+int kt_particles_advance(kt_ctx* ctx, double a, const kt_particles* pt, long long n, double dt, double r_stop,
+                         double r_escape, const kt_trace_opts* opts, void* hip_stream) {
+    if (!ctx) return KT_EINVAL;
+    try {
+        if (!pt || n < 0) return fail(ctx, KT_EINVAL, "particles/n missing");
+        if (!pt->r || !pt->mu || !pt->phi || !pt->vr || !pt->vmu || !pt->E || !pt->L || !pt->Q || !pt->fate)
+            return fail(ctx, KT_EINVAL, "particles need r, mu, phi, vr, vmu, E, L, Q and fate");
+        if (!(std::fabs(a) <= 1.0)) return fail(ctx, KT_EINVAL, "spin must lie in [-1, 1]");
+        if (!(dt >= 0.0) || !std::isfinite(dt)) return fail(ctx, KT_EINVAL, "dt must be finite and >= 0");
+        if (!(r_stop > 0.0)) r_stop = horizon(a) + 0.02;
+        if (!(r_escape > r_stop)) return fail(ctx, KT_EINVAL, "r_escape must exceed r_stop");
+        if (n == 0) return KT_OK;
+        kt_trace_opts o;
+        fill_defaults(opts, o);
+        hipStream_t stream = static_cast<hipStream_t>(hip_stream);
+        KT_HIP(hipSetDevice(ctx->device));
+        ktp::ParticleLaunch L{};
+        L.n = n;
+        L.a = a;
+        L.dt = dt;
+        L.r_stop = r_stop;
+        L.r_escape = r_escape;
+        L.rtol = o.rtol;
+        L.atol = o.atol;
+        L.max_steps = o.max_steps;
+        const size_t nn = (size_t)n;
+        if (pt->on_device) {
+            L.r = pt->r;
+            L.mu = pt->mu;
+            L.phi = pt->phi;
+            L.vr = pt->vr;
+            L.vmu = pt->vmu;
+            L.E = pt->E;
+            L.L = pt->L;
+            L.Q = pt->Q;
+            L.fate = pt->fate;
+            L.steps = pt->steps;
+            hipError_t e = ktp::advance(L, stream);
+            return e == hipSuccess ? KT_OK : hip_fail(ctx, e, "particle kernel launch");
+        }
+        // host arrays: eight columns of doubles, then fate and steps, staged on the device
+        if (int st = ensure_stage(ctx, nn * (8 * 8 + 1 + 4) + 64)) return st;
+        double* base = static_cast<double*>(ctx->d_stage);
+        double* col[8];
+        for (int i = 0; i < 8; ++i) col[i] = base + i * nn;
+        const double* src[8] = {pt->r, pt->mu, pt->phi, pt->vr, pt->vmu, pt->E, pt->L, pt->Q};
+        for (int i = 0; i < 8; ++i) KT_HIP(hipMemcpyAsync(col[i], src[i], 8 * nn, hipMemcpyHostToDevice, stream));
+        int8_t* fate = reinterpret_cast<int8_t*>(base + 8 * nn);
+        int32_t* steps = pt->steps ? reinterpret_cast<int32_t*>(fate + ((nn + 7) & ~(size_t)7)) : nullptr;
+        KT_HIP(hipMemcpyAsync(fate, pt->fate, nn, hipMemcpyHostToDevice, stream));
+        L.r = col[0];
+        L.mu = col[1];
+        L.phi = col[2];
+        L.vr = col[3];
+        L.vmu = col[4];
+        L.E = col[5];
+        L.L = col[6];
+        L.Q = col[7];
+        L.fate = fate;
+        L.steps = steps;
+        hipError_t e = ktp::advance(L, stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "particle kernel launch");
+        double* dst[5] = {pt->r, pt->mu, pt->phi, pt->vr, pt->vmu};
+        for (int i = 0; i < 5; ++i) KT_HIP(hipMemcpyAsync(dst[i], col[i], 8 * nn, hipMemcpyDeviceToHost, stream));
+        KT_HIP(hipMemcpyAsync(pt->fate, fate, nn, hipMemcpyDeviceToHost, stream));
+        if (steps) KT_HIP(hipMemcpyAsync(pt->steps, steps, 4 * nn, hipMemcpyDeviceToHost, stream));
+        KT_HIP(hipStreamSynchronize(stream));
+        return KT_OK;
+    } catch (const std::exception& ex) {
+        return fail(ctx, KT_EINVAL, ex.what());
+    }
 }
 
 double kt_horizon_radius(double a) { return horizon(a); }

@@ -20,6 +20,11 @@
  *   kt_eht_render, kt_eht_trace_rays, kt_eht_flux, kt_eht_calibrate
  *                   <- eht::render, eht::traceRay (with the light), eht::flux,
  *                      eht::calibrate         (src/eht.h:130-152)
+ *   KT_CAMERA_OBSERVER views, kt_geodesic_four_velocity
+ *                   <- Camera::frame          (src/camera.cpp:23-40), kerr::fourVelocity
+ *   kt_particles_advance, kt_particle_from_zamo
+ *                   <- the debris update      (shaders/particles_update.comp:207-255),
+ *                      kerr::stepCoordinateTime, kerr::launchFromZamo (src/kerr.cpp)
  *
  * Conventions
  *   - Units G = c = M = 1.  Angles in radians except fov_deg.
@@ -312,6 +317,38 @@ int kt_eht_calibrate(kt_ctx* ctx, const kt_eht_view* view, const kt_eht_flow* fl
                      double theta_guas, const kt_trace_opts* opts, double* n0, void* hip_stream);
 /* eht::angularScale (src/eht.cpp:46-48): microarcseconds that M subtends */
 double kt_eht_angular_scale(double mass_solar, double distance_pc);
+
+/* ---- Massive particles (debris) ------------------------------------------------------
+ * Replaces the geodesic part of the debris update (shaders/particles_update.comp:207-255,
+ * particle_common.glsl) and kerr::stepCoordinateTime (src/kerr.cpp:544-561): each moving
+ * particle (mu = 1, constants E, L, Q) advances by dt of ingoing Kerr-Schild coordinate
+ * time, in FP64 with the DP5 stepper, on the device.  The state lives in the integrator's
+ * own variables so that frame after frame it never round-trips through theta:
+ * mu = cos(theta), vmu = Sigma dmu/dtau = -sin(theta) Sigma dtheta/dtau, vr = Sigma dr/dtau
+ * (kerr::State::vr), phi the Kerr-Schild phi.  A particle plunges below r_stop
+ * (<= 0: r+ + 0.02, src/trajectory.cpp:137) or escapes beyond r_escape; its fate says so
+ * and later calls skip it. */
+enum { KT_PARTICLE_MOVING = 0, KT_PARTICLE_PLUNGED = 1, KT_PARTICLE_ESCAPED = 2, KT_PARTICLE_UNFINISHED = 3 };
+typedef struct kt_particles {
+    double* r;
+    double* mu;
+    double* phi;
+    double* vr;
+    double* vmu;
+    const double* E;      /* per particle */
+    const double* L;
+    const double* Q;
+    int8_t* fate;         /* in: KT_PARTICLE_MOVING to advance; out: the fate        */
+    int32_t* steps;       /* optional: DP5 attempts of this call                     */
+    int on_device;        /* 1: device pointers on the call's stream; 0: host (sync) */
+} kt_particles;
+/* opts: rtol, atol and max_steps (per particle per call) of kt_trace_opts, NULL = defaults */
+int kt_particles_advance(kt_ctx* ctx, double a, const kt_particles* particles, long long n, double dt,
+                         double r_stop, double r_escape, const kt_trace_opts* opts, void* hip_stream);
+/* A particle launched with 3-velocity v (along e_r, e_theta, e_phi) relative to the ZAMO at
+ * (r, theta), r outside the horizon: kerr::launchFromZamo with mu = 1 (src/kerr.cpp:365-406).
+ * out = {E, L, Q, vr, vmu}. */
+int kt_particle_from_zamo(double a, double r, double theta, const double v[3], double out[5]);
 
 /* The 4-velocity (ingoing Kerr-Schild, contravariant) of a free-falling massive observer
  * with constants E, L, Q at (r, theta), moving outward in r when radial_sign > 0 and

@@ -533,6 +533,167 @@ static double wrap_2pi(double x) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* massive particles (particles_update.comp:207-255; kerr.cpp:323-351, 365-406, 544-561) */
+/* ------------------------------------------------------------------------- */
+
+/* a particle's constants; mu = 1 everywhere: A2E2 = a^2 (E^2 - 1) */
+typedef struct {
+    double a, a2, E, L, Q, aE, aL, K0, C1, A2E2, twoA2E2;
+    int pole; /* L = 0 with no polar turning point short of the pole: phi + pi per pass */
+} ko_body;
+
+static void body_init(ko_body* b, double a, double E, double L, double Q) {
+    /* within KO_AXIS_SNAP radians of the axis: through it (L = 0), as photons (kerr.cpp:399-405) */
+    if (fabs(L) < KO_AXIS_SNAP * sqrt(fmax(Q + a * a * (E * E - 1.0), 0.0))) L = 0.0;
+    b->a = a;
+    b->a2 = a * a;
+    b->E = E;
+    b->L = L;
+    b->Q = Q;
+    b->aE = a * E;
+    b->aL = a * L;
+    const double x = L - b->aE;
+    b->K0 = x * x + Q;
+    b->A2E2 = b->a2 * (E * E - 1.0);
+    b->C1 = (Q + L * L) - b->A2E2;
+    b->twoA2E2 = 2.0 * b->A2E2;
+    b->pole = L == 0.0 && Q + b->A2E2 > 0.0;
+}
+
+/* dy/dt for y = (r, mu, phi_KS, vr, vmu), t the ingoing Kerr-Schild time: the reference's
+   Kerr-Schild derivatives (kerr.cpp:331-351, mu = 1) in mu form, divided by dt/dlambda */
+static void particle_rhs(const ko_body* b, const double y[5], double dy[5]) {
+    const double r = y[0], m = y[1], w = y[3];
+    const double delta = fma(r, r - 2.0, b->a2);
+    const double rra = fma(r, r, b->a2);
+    const double P = fma(b->E, rra, -b->aL);
+    const double K = fma(r, r, b->K0);
+    const double s2t = fma(-m, m, 1.0);
+    const double s2 = fmax(s2t, KO_TINY);
+    double F;
+    if (P * w < 0.0) {
+        F = K / (P - w);
+    } else {
+        const double d = fabs(delta) > 1e-12 ? delta : copysign(1e-12, delta);
+        F = (P + w) / d;
+    }
+    const double tdot = fma(rra, F, -fma(b->a, fma(b->aE, s2t, -b->L), w));
+    const double phidot = fma(b->a, F, -b->aE) + b->L / s2;
+    const double accr = fma(2.0 * b->E * r, P, -fma(r - 1.0, K, delta * r));
+    const double accm = -(m * fma(b->twoA2E2, m * m, b->C1));
+    const double inv = 1.0 / tdot;
+    dy[0] = w * inv;
+    dy[1] = y[4] * inv;
+    dy[2] = phidot * inv;
+    dy[3] = accr * inv;
+    dy[4] = accm * inv;
+}
+
+/* particle_common.glsl enforceMassiveConstraints / kerr.cpp:521-542 in mu form */
+static void particle_project(const ko_body* b, double y[5]) {
+    const double r = y[0], m = y[1];
+    const double delta = fma(r, r - 2.0, b->a2);
+    const double P = fma(b->E, fma(r, r, b->a2), -b->aL);
+    const double K = fma(r, r, b->K0);
+    const double R = fma(P, P, -(delta * K));
+    if (R > 0.05 * fma(P, P, fabs(delta) * K)) y[3] = copysign(sqrt(R), y[3]);
+    const double m2 = m * m, s2 = 1.0 - m2;
+    const double M = fma(-b->A2E2, m2 * m2, fma(-b->C1, m2, b->Q));
+    const double scale = fma(b->L * b->L, m2, fma(fabs(b->Q), s2, fabs(b->A2E2) * (m2 * s2)));
+    if (M > 0.05 * scale) y[4] = copysign(sqrt(M), y[4]);
+}
+
+static double dp5_particle(const ko_body* b, const ko_opts* o, const double y[5], double h, double k[7][5],
+                           double ynew[5]) {
+    double t[5];
+    int i;
+#define EVAL_PARTICLE(c, yy, kk) particle_rhs(b, yy, kk)
+    DP5_STAGES(5, EVAL_PARTICLE)
+#undef EVAL_PARTICLE
+    errpick ep = {-1.0, 1.0, 0.0};
+    for (i = 0; i < 5; ++i) {
+        const double big = amax(fabs(y[i]), fabs(ynew[i]));
+        double sc;
+        if (i == 0) sc = o->rtol * big;
+        else if (i == 3) sc = o->rtol * amax(big, y[0] * y[0]);
+        else sc = fma(o->rtol, big, o->atol);
+        pick(&ep, ERR_OF(i), sc);
+    }
+    return picked(&ep);
+}
+
+int ko_particle_advance(double a, double E, double L, double Q, double st[5], double dt, double r_stop,
+                        double r_escape, const ko_opts* o, int* steps_out) {
+    ko_body b;
+    body_init(&b, a, E, L, Q);
+    double y[5], k[7][5], yn[5];
+    for (int i = 0; i < 5; ++i) y[i] = st[i];
+    int fate = KO_PARTICLE_MOVING, steps = 0, rejected = 0;
+    double left = dt;
+    double h = fmin(dt, 0.01 * y[0]);
+    particle_rhs(&b, y, k[0]);
+    while (left > 0.0) {
+        if (steps >= o->max_steps) {
+            fate = KO_PARTICLE_UNFINISHED;
+            break;
+        }
+        int last = 0;
+        if (h >= left) {
+            h = left;
+            last = 1;
+        }
+        const double err = dp5_particle(&b, o, y, h, k, yn);
+        ++steps;
+        if (!(err <= 1.0)) {
+            h *= step_factor(err, 1);
+            rejected = 1;
+            continue;
+        }
+        const double fac = step_factor(err, rejected);
+        rejected = 0;
+        if (b.pole && y[1] * y[4] > 0.0 && yn[1] * yn[4] < 0.0) yn[2] += KO_PI;
+        particle_project(&b, yn);
+        for (int i = 0; i < 5; ++i) y[i] = yn[i];
+        if (!(fabs(y[0]) < 1e300) || !(fabs(y[1]) < 1e300) || y[0] < r_stop) {
+            fate = KO_PARTICLE_PLUNGED;
+            break;
+        }
+        if (y[0] > r_escape) {
+            fate = KO_PARTICLE_ESCAPED;
+            break;
+        }
+        left = last ? 0.0 : left - h;
+        h *= fac;
+        particle_rhs(&b, y, k[0]);
+    }
+    for (int i = 0; i < 5; ++i) st[i] = y[i];
+    if (fabs(st[2]) < 1e300) st[2] = wrap_2pi(st[2]);
+    if (steps_out) *steps_out = steps;
+    return fate;
+}
+
+/* kerr::launchFromZamo with mu = 1 (kerr.cpp:365-406); out = {E, L, Q, vr, vmu} */
+void ko_particle_from_zamo(double a, double r, double theta, const double v[3], double out[5]) {
+    const double sn = ko_sin(theta), cs = ko_cos(theta), a2 = a * a;
+    const double sigma = r * r + a2 * cs * cs, delta = r * r - 2.0 * r + a2;
+    const double A = (r * r + a2) * (r * r + a2) - a2 * delta * sn * sn;
+    const double omega = 2.0 * a * r / A;
+    double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if (v2 > 1.0 - 1e-12) v2 = 1.0 - 1e-12;
+    const double gamma = 1.0 / sqrt(1.0 - v2);
+    const double L = gamma * v[2] * sqrt(A / sigma) * sn;
+    const double E = gamma * sqrt(sigma * delta / A) + omega * L;
+    const double vr = gamma * v[0] * sqrt(sigma * delta);
+    const double vth = gamma * v[1] * sqrt(sigma);
+    const double Q = vth * vth + cs * cs * (a2 * (1.0 - E * E) + L * L / fmax(sn * sn, 1e-18));
+    out[0] = E;
+    out[1] = L;
+    out[2] = Q;
+    out[3] = vr;
+    out[4] = -sn * vth;
+}
+
+/* ------------------------------------------------------------------------- */
 /* launch                                                                     */
 /* ------------------------------------------------------------------------- */
 

@@ -126,6 +126,14 @@ void ko_launch_pixel(const ko_view* v, int col, int row, ko_photon* p, double y0
 void ko_launch_bardeen(const ko_view* v, double alpha, double beta, ko_photon* p, double y0[5]);
 /* pinhole view, explicit unit look vector on the ZAMO frame's (r, theta, phi) axes */
 void ko_launch_look(const ko_view* v, const double look[3], ko_photon* p, double y0[5]);
+/* ---- massive particles (particles_update.comp:207-255, kerr.cpp:544-561, 365-406) ---- */
+enum { KO_PARTICLE_MOVING = 0, KO_PARTICLE_PLUNGED = 1, KO_PARTICLE_ESCAPED = 2, KO_PARTICLE_UNFINISHED = 3 };
+/* advances st = (r, mu, phi_KS, vr, vmu) of a particle with constants E, L, Q by dt of
+   ingoing Kerr-Schild time; returns its fate */
+int ko_particle_advance(double a, double E, double L, double Q, double st[5], double dt, double r_stop,
+                        double r_escape, const ko_opts* o, int* steps);
+void ko_particle_from_zamo(double a, double r, double theta, const double v[3], double out[5]);
+
 /* observer views: the camera's 4-velocity (camera.cpp:23-36) */
 void ko_observer_u(const ko_view* v, double u[4]);
 /* phi_KS - phi_BL at r (zero at infinity) */

@@ -142,6 +142,8 @@ def lib():
             ("ko_launch_bardeen", None, [P(View), d, d, P(Photon), P(d)]),
             ("ko_launch_look", None, [P(View), P(d), P(Photon), P(d)]),
             ("ko_observer_u", None, [P(View), P(d)]),
+            ("ko_particle_advance", i, [d, d, d, d, P(d), d, d, d, P(Opts), P(i)]),
+            ("ko_particle_from_zamo", None, [d, d, d, P(d), P(d)]),
             ("ko_ks_shift", d, [d, d]),
             ("ko_trace_photon", None, [P(View), P(Opts), P(Photon), P(d), P(Result)]),
             ("ko_trace_pixel", None, [P(View), P(Opts), i, i, P(Result)]),
@@ -344,3 +346,22 @@ def eht_flux(intensity: np.ndarray, half_width: float, theta_guas: float) -> flo
 
 def eht_calibrate(v: EhtView, f: Flow, opts: Opts, target_jy: float, theta_guas: float) -> float:
     return lib().ko_eht_calibrate(C.byref(v), C.byref(f), C.byref(opts), target_jy, theta_guas)
+
+
+PARTICLE_MOVING, PARTICLE_PLUNGED, PARTICLE_ESCAPED, PARTICLE_UNFINISHED = 0, 1, 2, 3
+
+
+def particle_from_zamo(a: float, r: float, theta: float, v) -> np.ndarray:
+    """kerr::launchFromZamo, mu = 1: (E, L, Q, vr, vmu)."""
+    out = np.zeros(5)
+    vv = np.ascontiguousarray(v, np.float64)
+    lib().ko_particle_from_zamo(a, r, theta, _dp(vv), _dp(out))
+    return out
+
+
+def particle_advance(a, E, L, Q, state, dt, r_stop, r_escape, opts: Opts):
+    """(fate, new state (r, mu, phi, vr, vmu), DP5 attempts)."""
+    st = np.ascontiguousarray(state, np.float64).copy()
+    steps = C.c_int()
+    fate = lib().ko_particle_advance(a, E, L, Q, _dp(st), dt, r_stop, r_escape, C.byref(opts), C.byref(steps))
+    return fate, st, steps.value

@@ -237,3 +237,39 @@ void ref_trajectory_trace_u(double a, double r, double theta, double phi, const 
 }
 
 }  // extern "C"
+
+// ---- massive particles --------------------------------------------------------------------
+
+extern "C" {
+
+// kerr::launchFromZamo with mu = 1: out = {E, L, Q, vr, vtheta}
+void ref_particle_from_zamo(double a, double r, double theta, const double* v, double* out) {
+    kerr::Constants c;
+    kerr::State s;
+    kerr::launchFromZamo(a, r, theta, 0.0, v, 1.0, c, s);
+    out[0] = c.E;
+    out[1] = c.L;
+    out[2] = c.Q;
+    out[3] = s.vr;
+    out[4] = s.vtheta;
+}
+
+// trajectory::predict up to max_time of coordinate time (escape beyond `escape`):
+// out = {fate, x, y, z of the last point, rMin, rMax, periapses, precessionDeg, duration}
+void ref_predict(double a, const double* consts, const double* state, double max_time, double escape, double* out) {
+    kerr::Constants c{consts[0], consts[1], consts[2], consts[3]};
+    kerr::State s{state[0], state[1], state[2], state[3], state[4], state[5]};
+    trajectory::Path p = trajectory::predict(a, c, s, max_time, 1e30, 4000, 0.0, escape);
+    const auto& x = p.points.back();
+    out[0] = static_cast<double>(static_cast<int>(p.fate));
+    out[1] = x[0];
+    out[2] = x[1];
+    out[3] = x[2];
+    out[4] = p.rMin;
+    out[5] = p.rMax;
+    out[6] = p.periapses;
+    out[7] = p.precessionDeg;
+    out[8] = p.duration;
+}
+
+}  // extern "C"
