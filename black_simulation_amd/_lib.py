@@ -144,7 +144,7 @@ class KtEhtOut(C.Structure):
 
 
 class KtParticles(C.Structure):
-    _fields_ = [(n, C.c_void_p) for n in ("r", "mu", "phi", "vr", "vmu", "E", "L", "Q", "fate", "steps")] + \
+    _fields_ = [(n, C.c_void_p) for n in ("r", "mu", "phi", "vr", "vmu", "E", "L", "Q", "fate", "steps", "h")] + \
         [("on_device", C.c_int)]
 
 

@@ -1389,6 +1389,9 @@ int kt_geodesic_four_velocity(double a, double E, double L, double Q, double r, 
     return KT_OK;
 }
 
+// persistent particle waves: 8 per CU (two blocks of four waves; the kernel holds 2 per SIMD)
+static int particle_grid(const kt_ctx* ctx) { return std::max(1, ctx->cu_count) * 2; }
+
 int kt_particle_from_zamo(double a, double r, double theta, const double v[3], double out[5]) {
     if (!v || !out || !(std::fabs(a) <= 1.0) || !(r > horizon(a))) return KT_EINVAL;
     // kerr::launchFromZamo with mu = 1 (src/kerr.cpp:365-406): the ZAMO frame's lapse and
@@ -1449,17 +1452,20 @@ int kt_particles_advance(kt_ctx* ctx, double a, const kt_particles* pt, long lon
             L.Q = pt->Q;
             L.fate = pt->fate;
             L.steps = pt->steps;
-            hipError_t e = ktp::advance(L, stream);
+            L.h = pt->h;
+            hipError_t e = ktp::advance(L, ctx->d_counter, particle_grid(ctx), stream);
             return e == hipSuccess ? KT_OK : hip_fail(ctx, e, "particle kernel launch");
         }
-        // host arrays: eight columns of doubles, then fate and steps, staged on the device
-        if (int st = ensure_stage(ctx, nn * (8 * 8 + 1 + 4) + 64)) return st;
+        // host arrays: nine columns of doubles (the ninth the step sizes), then fate and
+        // steps, staged on the device
+        if (int st = ensure_stage(ctx, nn * (9 * 8 + 1 + 4) + 64)) return st;
         double* base = static_cast<double*>(ctx->d_stage);
-        double* col[8];
-        for (int i = 0; i < 8; ++i) col[i] = base + i * nn;
+        double* col[9];
+        for (int i = 0; i < 9; ++i) col[i] = base + i * nn;
         const double* src[8] = {pt->r, pt->mu, pt->phi, pt->vr, pt->vmu, pt->E, pt->L, pt->Q};
         for (int i = 0; i < 8; ++i) KT_HIP(hipMemcpyAsync(col[i], src[i], 8 * nn, hipMemcpyHostToDevice, stream));
-        int8_t* fate = reinterpret_cast<int8_t*>(base + 8 * nn);
+        if (pt->h) KT_HIP(hipMemcpyAsync(col[8], pt->h, 8 * nn, hipMemcpyHostToDevice, stream));
+        int8_t* fate = reinterpret_cast<int8_t*>(base + 9 * nn);
         int32_t* steps = pt->steps ? reinterpret_cast<int32_t*>(fate + ((nn + 7) & ~(size_t)7)) : nullptr;
         KT_HIP(hipMemcpyAsync(fate, pt->fate, nn, hipMemcpyHostToDevice, stream));
         L.r = col[0];
@@ -1472,12 +1478,14 @@ int kt_particles_advance(kt_ctx* ctx, double a, const kt_particles* pt, long lon
         L.Q = col[7];
         L.fate = fate;
         L.steps = steps;
-        hipError_t e = ktp::advance(L, stream);
+        L.h = pt->h ? col[8] : nullptr;
+        hipError_t e = ktp::advance(L, ctx->d_counter, particle_grid(ctx), stream);
         if (e != hipSuccess) return hip_fail(ctx, e, "particle kernel launch");
         double* dst[5] = {pt->r, pt->mu, pt->phi, pt->vr, pt->vmu};
         for (int i = 0; i < 5; ++i) KT_HIP(hipMemcpyAsync(dst[i], col[i], 8 * nn, hipMemcpyDeviceToHost, stream));
         KT_HIP(hipMemcpyAsync(pt->fate, fate, nn, hipMemcpyDeviceToHost, stream));
         if (steps) KT_HIP(hipMemcpyAsync(pt->steps, steps, 4 * nn, hipMemcpyDeviceToHost, stream));
+        if (pt->h) KT_HIP(hipMemcpyAsync(pt->h, col[8], 8 * nn, hipMemcpyDeviceToHost, stream));
         KT_HIP(hipStreamSynchronize(stream));
         return KT_OK;
     } catch (const std::exception& ex) {

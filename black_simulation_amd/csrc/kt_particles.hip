@@ -52,7 +52,10 @@ __device__ __forceinline__ void body_init(Body& b, double a, double E, double L,
     b.pole = L == 0.0 && Q + b.A2E2 > 0.0;
 }
 
-// dy/dt from dy/dlambda and dt/dlambda
+// dy/dt = (dy/dlambda) / (dt/dlambda) with one division: the inflow factor F = nf / df
+// (kerr.cpp:323-329) and 1 / sin^2 are folded into g = 1 / (sin^2 (rra nf - df X)),
+// X = a (a E sin^2 - L) + vr, so dt/dlambda = (rra nf - df X) / df and
+// dphi/dlambda = (a nf sin^2 + L df - a E df sin^2) / (df sin^2)
 __device__ __forceinline__ void rhs(const Body& b, const double y[5], double dy[5]) {
     const double r = y[0], m = y[1], w = y[3];
     const double delta = fma(r, r - 2.0, b.a2);
@@ -60,24 +63,27 @@ __device__ __forceinline__ void rhs(const Body& b, const double y[5], double dy[
     const double P = fma(b.E, rra, -b.aL);
     const double K = fma(r, r, b.K0);
     const double s2t = fma(-m, m, 1.0);
-    const double s2 = fmax(s2t, 1e-300);
-    double F;
+    const double s2 = fmax(s2t, 1e-200);  // keeps df s2 a normal number at the pole
+    double nf, df;
     if (P * w < 0.0) {
-        F = K / (P - w);
+        nf = K;
+        df = P - w;
     } else {
-        const double d = fabs(delta) > 1e-12 ? delta : copysign(1e-12, delta);
-        F = (P + w) / d;
+        nf = P + w;
+        df = fabs(delta) > 1e-12 ? delta : copysign(1e-12, delta);
     }
-    const double tdot = fma(rra, F, -fma(b.a, fma(b.aE, s2t, -b.L), w));
-    const double phidot = fma(b.a, F, -b.aE) + b.L / s2;
+    const double X = fma(b.a, fma(b.aE, s2t, -b.L), w);
+    const double den = fma(rra, nf, -(df * X));
+    const double g = 1.0 / (s2 * den);
+    const double ts = (df * s2) * g;  // 1 / (dt/dlambda)
+    const double phin = fma(b.a * nf, s2, fma(b.L, df, -((b.aE * df) * s2)));
     const double accr = fma(2.0 * b.E * r, P, -fma(r - 1.0, K, delta * r));
     const double accm = -(m * fma(b.twoA2E2, m * m, b.C1));
-    const double inv = 1.0 / tdot;
-    dy[0] = w * inv;
-    dy[1] = y[4] * inv;
-    dy[2] = phidot * inv;
-    dy[3] = accr * inv;
-    dy[4] = accm * inv;
+    dy[0] = w * ts;
+    dy[1] = y[4] * ts;
+    dy[2] = phin * g;
+    dy[3] = accr * ts;
+    dy[4] = accm * ts;
 }
 
 __device__ __forceinline__ void project(const Body& b, double y[5]) {
@@ -130,71 +136,137 @@ __device__ __forceinline__ double dp5(const Body& b, const double y[5], double h
     return ep.value();
 }
 
-__global__ __launch_bounds__(128) void k_advance(ParticleLaunch p) {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= p.n) return;
-    if (p.fate[q] != kMoving) {
-        if (p.steps) p.steps[q] = 0;
-        return;
-    }
+// One particle in one lane: started from its arrays, stepped one DP5 attempt at a time.
+struct PLane {
     Body b;
-    body_init(b, p.a, p.E[q], p.L[q], p.Q[q]);
-    double y[5] = {p.r[q], p.mu[q], p.phi[q], p.vr[q], p.vmu[q]};
+    double y[5], k0[5], h, left, hwant;
+    int steps;
+    bool rejected;
+};
+
+__device__ __forceinline__ void lane_start(PLane& s, const ParticleLaunch& p, long long q) {
+    body_init(s.b, p.a, p.E[q], p.L[q], p.Q[q]);
+    s.y[0] = p.r[q];
+    s.y[1] = p.mu[q];
+    s.y[2] = p.phi[q];
+    s.y[3] = p.vr[q];
+    s.y[4] = p.vmu[q];
+    s.left = p.dt;
+    const double h0 = p.h ? p.h[q] : 0.0;
+    s.h = h0 > 0.0 ? fmin(p.dt, h0) : fmin(p.dt, 0.01 * s.y[0]);
+    s.hwant = s.h;
+    s.steps = 0;
+    s.rejected = false;
+    rhs(s.b, s.y, s.k0);
+}
+
+// one attempt; returns the fate once the particle is done with this call, else -1
+__device__ __forceinline__ int lane_step(PLane& s, const ParticleLaunch& p) {
+    if (!(s.left > 0.0)) return kMoving;
+    if (s.steps >= p.max_steps) return kUnfinished;
+    bool last = false;
+    double h = s.h;
+    s.hwant = h;  // the controller's own choice, before the span's end shortens it
+    if (h >= s.left) {
+        h = s.left;
+        last = true;
+    }
     double k[7][5], yn[5];
-    int fate = kMoving, steps = 0;
-    bool rejected = false;
-    double left = p.dt;
-    double h = fmin(p.dt, 0.01 * y[0]);
-    rhs(b, y, k[0]);
-    while (left > 0.0) {
-        if (steps >= p.max_steps) {
-            fate = kUnfinished;
-            break;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) k[0][i] = s.k0[i];
+    const double err = dp5(s.b, s.y, h, k, yn, p.rtol, p.atol);
+    ++s.steps;
+    if (!(err <= 1.0)) {
+        s.h = h * step_factor(err, true);
+        s.rejected = true;
+        return -1;
+    }
+    const double fac = step_factor(err, s.rejected);
+    s.rejected = false;
+    if (s.b.pole && s.y[1] * s.y[4] > 0.0 && yn[1] * yn[4] < 0.0) yn[2] += kPi;
+    project(s.b, yn);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) s.y[i] = yn[i];
+    if (!(fabs(s.y[0]) < 1e300) || !(fabs(s.y[1]) < 1e300) || s.y[0] < p.r_stop) return kPlunged;
+    if (s.y[0] > p.r_escape) return kEscaped;
+    s.left = last ? 0.0 : s.left - h;
+    if (!(s.left > 0.0)) return kMoving;
+    s.h = h * fac;
+    rhs(s.b, s.y, s.k0);
+    return -1;
+}
+
+__device__ __forceinline__ void lane_write(const PLane& s, const ParticleLaunch& p, long long q, int fate) {
+    p.r[q] = s.y[0];
+    p.mu[q] = s.y[1];
+    p.phi[q] = fabs(s.y[2]) < 1e300 ? wrap_2pi(s.y[2]) : s.y[2];
+    p.vr[q] = s.y[3];
+    p.vmu[q] = s.y[4];
+    p.fate[q] = (int8_t)fate;
+    if (p.steps) p.steps[q] = s.steps;
+    if (p.h) p.h[q] = s.hwant;
+}
+
+__global__ void k_zero_counter(unsigned long long* c) {
+    if (threadIdx.x == 0) *c = 0ull;
+}
+
+// Persistent waves pulling particles from one counter: every kRefill attempts a wave hands
+// its finished lanes new particles (one atomic per wave), so the few particles that need
+// hundreds of steps near the horizon or the axis do not hold 63 idle lanes each.
+constexpr int kRefill = 4;
+
+__global__ __launch_bounds__(256) void k_advance(ParticleLaunch p, unsigned long long* counter) {
+    const int lane = (int)(threadIdx.x & 63u);
+    PLane s;
+    bool active = false, drained = false;
+    long long my = 0;
+    for (;;) {
+        if (!drained) {
+            const unsigned long long idle = __ballot(!active);
+            const int nidle = __popcll(idle);
+            if (nidle > 0) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(counter, (unsigned long long)nidle);
+                base = __shfl(base, 0);
+                if (base + (unsigned long long)nidle >= (unsigned long long)p.n) drained = true;
+                if (!active) {
+                    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+                    const long long q = (long long)(base + (unsigned long long)__popcll(idle & below));
+                    if (q < p.n) {
+                        my = q;
+                        if (p.fate[q] == kMoving) {
+                            lane_start(s, p, q);
+                            active = true;
+                        } else if (p.steps) {
+                            p.steps[q] = 0;
+                        }
+                    }
+                }
+            }
         }
-        bool last = false;
-        if (h >= left) {
-            h = left;
-            last = true;
-        }
-        const double err = dp5(b, y, h, k, yn, p.rtol, p.atol);
-        ++steps;
-        if (!(err <= 1.0)) {
-            h *= step_factor(err, true);
-            rejected = true;
+        if (__ballot(active) == 0ull) {
+            if (drained) break;
             continue;
         }
-        const double fac = step_factor(err, rejected);
-        rejected = false;
-        if (b.pole && y[1] * y[4] > 0.0 && yn[1] * yn[4] < 0.0) yn[2] += kPi;
-        project(b, yn);
-#pragma unroll
-        for (int i = 0; i < 5; ++i) y[i] = yn[i];
-        if (!(fabs(y[0]) < 1e300) || !(fabs(y[1]) < 1e300) || y[0] < p.r_stop) {
-            fate = kPlunged;
-            break;
+        for (int k = 0; k < kRefill; ++k) {
+            if (active) {
+                const int fate = lane_step(s, p);
+                if (fate >= 0) {
+                    lane_write(s, p, my, fate);
+                    active = false;
+                }
+            }
         }
-        if (y[0] > p.r_escape) {
-            fate = kEscaped;
-            break;
-        }
-        left = last ? 0.0 : left - h;
-        h *= fac;
-        rhs(b, y, k[0]);
     }
-    p.r[q] = y[0];
-    p.mu[q] = y[1];
-    p.phi[q] = fabs(y[2]) < 1e300 ? wrap_2pi(y[2]) : y[2];
-    p.vr[q] = y[3];
-    p.vmu[q] = y[4];
-    p.fate[q] = (int8_t)fate;
-    if (p.steps) p.steps[q] = steps;
 }
 
 }  // namespace
 
-hipError_t advance(const ParticleLaunch& p, hipStream_t st) {
+hipError_t advance(const ParticleLaunch& p, unsigned long long* counter, int grid, hipStream_t st) {
     if (p.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_advance, dim3((unsigned)((p.n + 127) / 128)), dim3(128), 0, st, p);
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, counter);
+    hipLaunchKernelGGL(k_advance, dim3((unsigned)grid), dim3(256), 0, st, p, counter);
     return hipGetLastError();
 }
 

@@ -18,8 +18,10 @@ struct ParticleLaunch {
     const double *E, *L, *Q;
     int8_t* fate;
     int32_t* steps;  // optional
+    double* h;       // optional: first step size to try; out: the controller's next step
 };
 
-hipError_t advance(const ParticleLaunch& p, hipStream_t st);
+// counter: one device word (zeroed here by a kernel); grid: workgroups of 256 threads
+hipError_t advance(const ParticleLaunch& p, unsigned long long* counter, int grid, hipStream_t st);
 
 }  // namespace ktp

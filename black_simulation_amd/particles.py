@@ -37,6 +37,7 @@ class Particles:
         self.fate = (torch.zeros(n, dtype=torch.int8) if fate is None else torch.as_tensor(
             np.ascontiguousarray(fate, np.int8))).to(dev)
         self.steps = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.h = torch.zeros(n, dtype=torch.float64, device=dev)  # the controller's step, carried over
 
     @classmethod
     def from_zamo(cls, a: float, r, theta, phi, v, device: int = 0) -> "Particles":
@@ -58,7 +59,7 @@ class Particles:
 
     def _struct(self) -> K.KtParticles:
         return K.KtParticles(*[getattr(self, c).data_ptr() for c in COLUMNS], self.fate.data_ptr(),
-                             self.steps.data_ptr(), 1)
+                             self.steps.data_ptr(), self.h.data_ptr(), 1)
 
     def advance(self, tracer: Tracer, a: float, dt: float, r_stop: float = 0.0, r_escape: float = 1e4,
                 opts: TraceOpts | None = None, stream=None) -> "Particles":
@@ -77,4 +78,5 @@ class Particles:
         out = {c: getattr(self, c).cpu().numpy() for c in COLUMNS}
         out["fate"] = self.fate.cpu().numpy()
         out["steps"] = self.steps.cpu().numpy()
+        out["h"] = self.h.cpu().numpy()
         return out

@@ -340,6 +340,8 @@ typedef struct kt_particles {
     const double* Q;
     int8_t* fate;         /* in: KT_PARTICLE_MOVING to advance; out: the fate        */
     int32_t* steps;       /* optional: DP5 attempts of this call                     */
+    double* h;            /* optional: the first step size to try (<= 0: min(dt, 0.01 r)); */
+                          /*   out: the controller's next step, for the next call    */
     int on_device;        /* 1: device pointers on the call's stream; 0: host (sync) */
 } kt_particles;
 /* opts: rtol, atol and max_steps (per particle per call) of kt_trace_opts, NULL = defaults */

@@ -561,7 +561,9 @@ static void body_init(ko_body* b, double a, double E, double L, double Q) {
 }
 
 /* dy/dt for y = (r, mu, phi_KS, vr, vmu), t the ingoing Kerr-Schild time: the reference's
-   Kerr-Schild derivatives (kerr.cpp:331-351, mu = 1) in mu form, divided by dt/dlambda */
+   Kerr-Schild derivatives (kerr.cpp:331-351, mu = 1) in mu form over dt/dlambda, with one
+   division: F = nf / df (kerr.cpp:323-329) and 1 / sin^2 folded into
+   g = 1 / (sin^2 (rra nf - df X)), X = a (a E sin^2 - L) + vr */
 static void particle_rhs(const ko_body* b, const double y[5], double dy[5]) {
     const double r = y[0], m = y[1], w = y[3];
     const double delta = fma(r, r - 2.0, b->a2);
@@ -569,24 +571,27 @@ static void particle_rhs(const ko_body* b, const double y[5], double dy[5]) {
     const double P = fma(b->E, rra, -b->aL);
     const double K = fma(r, r, b->K0);
     const double s2t = fma(-m, m, 1.0);
-    const double s2 = fmax(s2t, KO_TINY);
-    double F;
+    const double s2 = fmax(s2t, 1e-200); /* keeps df s2 a normal number at the pole */
+    double nf, df;
     if (P * w < 0.0) {
-        F = K / (P - w);
+        nf = K;
+        df = P - w;
     } else {
-        const double d = fabs(delta) > 1e-12 ? delta : copysign(1e-12, delta);
-        F = (P + w) / d;
+        nf = P + w;
+        df = fabs(delta) > 1e-12 ? delta : copysign(1e-12, delta);
     }
-    const double tdot = fma(rra, F, -fma(b->a, fma(b->aE, s2t, -b->L), w));
-    const double phidot = fma(b->a, F, -b->aE) + b->L / s2;
+    const double X = fma(b->a, fma(b->aE, s2t, -b->L), w);
+    const double den = fma(rra, nf, -(df * X));
+    const double g = 1.0 / (s2 * den);
+    const double ts = (df * s2) * g;  /* 1 / (dt/dlambda) */
+    const double phin = fma(b->a * nf, s2, fma(b->L, df, -((b->aE * df) * s2)));
     const double accr = fma(2.0 * b->E * r, P, -fma(r - 1.0, K, delta * r));
     const double accm = -(m * fma(b->twoA2E2, m * m, b->C1));
-    const double inv = 1.0 / tdot;
-    dy[0] = w * inv;
-    dy[1] = y[4] * inv;
-    dy[2] = phidot * inv;
-    dy[3] = accr * inv;
-    dy[4] = accm * inv;
+    dy[0] = w * ts;
+    dy[1] = y[4] * ts;
+    dy[2] = phin * g;
+    dy[3] = accr * ts;
+    dy[4] = accm * ts;
 }
 
 /* particle_common.glsl enforceMassiveConstraints / kerr.cpp:521-542 in mu form */
@@ -623,14 +628,15 @@ static double dp5_particle(const ko_body* b, const ko_opts* o, const double y[5]
 }
 
 int ko_particle_advance(double a, double E, double L, double Q, double st[5], double dt, double r_stop,
-                        double r_escape, const ko_opts* o, int* steps_out) {
+                        double r_escape, const ko_opts* o, int* steps_out, double* h_io) {
     ko_body b;
     body_init(&b, a, E, L, Q);
     double y[5], k[7][5], yn[5];
     for (int i = 0; i < 5; ++i) y[i] = st[i];
     int fate = KO_PARTICLE_MOVING, steps = 0, rejected = 0;
     double left = dt;
-    double h = fmin(dt, 0.01 * y[0]);
+    double h = h_io && *h_io > 0.0 ? fmin(dt, *h_io) : fmin(dt, 0.01 * y[0]);
+    double hwant = h; /* the controller's own step before the span's end shortens it */
     particle_rhs(&b, y, k[0]);
     while (left > 0.0) {
         if (steps >= o->max_steps) {
@@ -638,6 +644,7 @@ int ko_particle_advance(double a, double E, double L, double Q, double st[5], do
             break;
         }
         int last = 0;
+        hwant = h;
         if (h >= left) {
             h = left;
             last = 1;
@@ -669,6 +676,7 @@ int ko_particle_advance(double a, double E, double L, double Q, double st[5], do
     for (int i = 0; i < 5; ++i) st[i] = y[i];
     if (fabs(st[2]) < 1e300) st[2] = wrap_2pi(st[2]);
     if (steps_out) *steps_out = steps;
+    if (h_io) *h_io = hwant;
     return fate;
 }
 

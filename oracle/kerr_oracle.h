@@ -131,7 +131,7 @@ enum { KO_PARTICLE_MOVING = 0, KO_PARTICLE_PLUNGED = 1, KO_PARTICLE_ESCAPED = 2,
 /* advances st = (r, mu, phi_KS, vr, vmu) of a particle with constants E, L, Q by dt of
    ingoing Kerr-Schild time; returns its fate */
 int ko_particle_advance(double a, double E, double L, double Q, double st[5], double dt, double r_stop,
-                        double r_escape, const ko_opts* o, int* steps);
+                        double r_escape, const ko_opts* o, int* steps, double* h); /* h: NULL or in/out */
 void ko_particle_from_zamo(double a, double r, double theta, const double v[3], double out[5]);
 
 /* observer views: the camera's 4-velocity (camera.cpp:23-36) */

@@ -142,7 +142,7 @@ def lib():
             ("ko_launch_bardeen", None, [P(View), d, d, P(Photon), P(d)]),
             ("ko_launch_look", None, [P(View), P(d), P(Photon), P(d)]),
             ("ko_observer_u", None, [P(View), P(d)]),
-            ("ko_particle_advance", i, [d, d, d, d, P(d), d, d, d, P(Opts), P(i)]),
+            ("ko_particle_advance", i, [d, d, d, d, P(d), d, d, d, P(Opts), P(i), P(d)]),
             ("ko_particle_from_zamo", None, [d, d, d, P(d), P(d)]),
             ("ko_ks_shift", d, [d, d]),
             ("ko_trace_photon", None, [P(View), P(Opts), P(Photon), P(d), P(Result)]),
@@ -359,9 +359,14 @@ def particle_from_zamo(a: float, r: float, theta: float, v) -> np.ndarray:
     return out
 
 
-def particle_advance(a, E, L, Q, state, dt, r_stop, r_escape, opts: Opts):
-    """(fate, new state (r, mu, phi, vr, vmu), DP5 attempts)."""
+def particle_advance(a, E, L, Q, state, dt, r_stop, r_escape, opts: Opts, h=None):
+    """(fate, new state (r, mu, phi, vr, vmu), DP5 attempts); with h given (the step to start
+    with, 0 = the default) also the controller's next step: (fate, state, steps, h)."""
     st = np.ascontiguousarray(state, np.float64).copy()
     steps = C.c_int()
-    fate = lib().ko_particle_advance(a, E, L, Q, _dp(st), dt, r_stop, r_escape, C.byref(opts), C.byref(steps))
-    return fate, st, steps.value
+    hh = None if h is None else C.c_double(h)
+    fate = lib().ko_particle_advance(a, E, L, Q, _dp(st), dt, r_stop, r_escape, C.byref(opts), C.byref(steps),
+                                     None if hh is None else C.byref(hh))
+    if h is None:
+        return fate, st, steps.value
+    return fate, st, steps.value, hh.value

@@ -1,7 +1,8 @@
 """Massive particles on the GPU (kt_particles_advance) against the oracle's
 ko_particle_advance on the same seeded debris.  Both perform the same correctly rounded
-operations in the same order, so fates, step counts and states are equal bit for bit,
-call after call; host buffers give what device buffers give."""
+operations in the same order, so fates, step counts, states and the step sizes carried to
+the next call are equal bit for bit, call after call; host buffers give what device
+buffers give."""
 import ctypes as C
 import math
 
@@ -37,11 +38,13 @@ def oracle_advance(a, p: dict, dt, r_stop, r_escape):
             out["steps"][i] = 0
             continue
         st = np.array([out[c][i] for c in ("r", "mu", "phi", "vr", "vmu")])
-        fate, st, steps = O.particle_advance(a, out["E"][i], out["L"][i], out["Q"][i], st, dt, r_stop, r_escape, opts)
+        fate, st, steps, h = O.particle_advance(a, out["E"][i], out["L"][i], out["Q"][i], st, dt, r_stop, r_escape,
+                                                opts, h=out["h"][i])
         for c, x in zip(("r", "mu", "phi", "vr", "vmu"), st):
             out[c][i] = x
         out["fate"][i] = fate
         out["steps"][i] = steps
+        out["h"][i] = h
     return out
 
 
@@ -49,7 +52,7 @@ def assert_same(got, want, what):
     print(f"{what}: fates {np.bincount(want['fate'], minlength=4)}, steps {want['steps'].sum()}", flush=True)
     assert np.array_equal(got["fate"], want["fate"]), what
     assert np.array_equal(got["steps"], want["steps"]), what
-    for c in ("r", "mu", "phi", "vr", "vmu"):
+    for c in ("r", "mu", "phi", "vr", "vmu", "h"):
         assert np.array_equal(got[c], want[c], equal_nan=True), (what, c)
 
 
@@ -73,11 +76,12 @@ def test_host_buffers_equal_device_buffers(tracer):
     cols = {c: np.ascontiguousarray(host[c]) for c in COLUMNS}
     fate = np.ascontiguousarray(host["fate"])
     steps = np.zeros(n, np.int32)
-    s = K.KtParticles(*[cols[c].ctypes.data for c in COLUMNS], fate.ctypes.data, steps.ctypes.data, 0)
+    h = np.zeros(n)
+    s = K.KtParticles(*[cols[c].ctypes.data for c in COLUMNS], fate.ctypes.data, steps.ctypes.data, h.ctypes.data, 0)
     o = bsa.TraceOpts().to_c()
     assert K.lib().kt_particles_advance(tracer._ctx, a, C.byref(s), n, 30.0, 0.0, 1e4, C.byref(o), None) == K.KT_OK
     dev.advance(tracer, a, 30.0)
     got = dev.to_numpy()
     for c in ("r", "mu", "phi", "vr", "vmu"):
         assert np.array_equal(cols[c], got[c], equal_nan=True), c
-    assert np.array_equal(fate, got["fate"]) and np.array_equal(steps, got["steps"])
+    assert np.array_equal(fate, got["fate"]) and np.array_equal(steps, got["steps"]) and np.array_equal(h, got["h"])

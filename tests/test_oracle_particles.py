@@ -160,3 +160,20 @@ def test_polar_orbit_passes_the_poles_smoothly_and_stays_on_the_constraints():
     assert passes >= 1
     fate, one, _ = O.particle_advance(a, E, Lz, Q, np.array([r, math.cos(th), 0.0, vr, vmu]), 150.0, 1.0, 1e4, OPTS)
     assert np.linalg.norm(cartesian(a, one) - x0) < 1e-7
+
+
+def test_carried_step_size_saves_attempts_and_keeps_the_orbit():
+    """The step size handed from one update to the next: the orbit is the same to the
+    tolerance, and the updates after the first take fewer attempts than cold starts."""
+    a, r, th = 0.9, 12.0, 1.2
+    E, Lz, Q, vr, vmu = O.particle_from_zamo(a, r, th, [0.1, 0.2, 0.3])
+    cold = warm = np.array([r, math.cos(th), 0.0, vr, vmu])
+    h, n_cold, n_warm = 0.0, 0, 0
+    for _ in range(30):
+        _, cold, s1 = O.particle_advance(a, E, Lz, Q, cold, 10.0, 1.45, 1e4, OPTS)
+        _, warm, s2, h = O.particle_advance(a, E, Lz, Q, warm, 10.0, 1.45, 1e4, OPTS, h=h)
+        n_cold += s1
+        n_warm += s2
+        assert h > 0.0
+    assert n_warm < 0.8 * n_cold
+    assert np.linalg.norm(cartesian(a, cold) - cartesian(a, warm)) < 1e-7
