@@ -291,7 +291,7 @@ def main():
         mlp_ach = rays * MLP_FLOP_PER_RAY / (avg_sur * 1e-3) / 1e12
         roof_mlp = {"bound": "mfma", "achieved": round(mlp_ach, 2), "peak": BF16_MFMA_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(mlp_ach / BF16_MFMA_PEAK_TFLOPS, 4), "traffic": None,
-                    "kernel": "k_gemm (network) + features + gate", "kernel_ms": round(avg_sur, 3),
+                    "kernel": "k_mlp_fused (features, network, gate)", "kernel_ms": round(avg_sur, 3),
                     "flop_per_ray": MLP_FLOP_PER_RAY}
         if avg_sur > avg_integ:
             roof, roof_mlp = roof_mlp, roof_integ

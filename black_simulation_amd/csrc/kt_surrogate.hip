@@ -262,7 +262,7 @@ __global__ void k_copy_count(const unsigned long long* c, long long* dst, unsign
 // ---- the fused network: one kernel from features to gated outputs -------------------------
 //
 // One wave owns 32 rays and keeps their activations in registers from layer to layer; the
-// weights stream through LDS in 16 KB slabs shared by the block's 4 waves.  Layers 1-4 use
+// weights stream through LDS in 16 KB slabs shared by the block's 4 waves.  The layers use
 // v_mfma_f32_32x32x16_bf16 with the transposed product D[n][m] = sum_k W[k][n] X[m][k]:
 // the output feature n is the MFMA row, the ray m the column.  Lane l holds ray m = l % 32
 // and, of every 32-feature block fb, the 16 features 32 fb + 8 q + 4 h + r (h = l / 32,
@@ -273,10 +273,31 @@ __global__ void k_copy_count(const unsigned long long* c, long long* dst, unsign
 // and the packed weights hold W[F(kb, h, s)][n] in A-operand order (lane = 32 h + n % 32).
 // The accumulators therefore never leave the lane: SiLU and the bf16 rounding produce the
 // next layer's operand in place.
+//
+// A hidden layer runs in output-block order: block fb is one chain of sixteen MFMAs over
+// the layer's whole input (one 16 KB slab: the weights of its 32 outputs), and the SiLU of
+// block fb - 1, which the chain does not wait for, issues between its MFMAs.  The vector
+// ALU, the kernel's busiest unit (two transcendentals per activation), so works while the
+// matrix cores do.
+//
+// Every pre-activation is kept scaled by -log2(e): z' = -log2(e) z, so exp(-z) = 2^z' is a
+// single v_exp_f32, and the stored activation is z' sigma(z) = -log2(e) silu(z).  A layer's
+// weights need no change (its input carries the factor and so must its output), its bias
+// carries it, and the output layer multiplies by -ln(2).
+//
+// The input layer also runs on the matrix cores: one 32x32x16 MFMA per output block with
+// split-bf16 operands, f = fh + fl and W = Wh + Wl (each part bf16), summing fh Wh + fl Wh +
+// fh Wl over the five inputs in k-slots 0-4, 5-9 and 10-14 (slot 15 zero).  The parts carry
+// 16 significant bits and the dropped fl Wl is 2^-18 relative, so its pre-activations are
+// good to ~1e-5, far below the bf16 rounding of the activations they feed.
 constexpr int kFusedRays = 128;        // rays per block: 4 waves x 32
-constexpr int kSlabElems = 8192;       // bf16 per slab (16 KB): two k-blocks of all 8 feature blocks
+constexpr int kSlabElems = 8192;       // bf16 per slab (16 KB): one output block, all 16 k-blocks
 constexpr int kHiddenSlabs = 3 * 8;    // layers 1-3, eight slabs each; then one slab for layer 4
 constexpr int kPackedElems = (kHiddenSlabs + 1) * kSlabElems;
+constexpr int kInputFragElems = 8 * 64 * 8;  // input layer: [fb][lane][8]
+// the scaled FP32 copies: input weights [5][256], input bias, biases of layers 1-3
+constexpr int kFoldElems = ktn::kInputs * ktn::kHidden + 4 * ktn::kHidden;
+constexpr float kNegLog2e = -1.44269504088896341f, kNegLn2 = -0.693147180559945309f;
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
@@ -284,27 +305,52 @@ __host__ __device__ inline int packed_feature(int kb, int h, int s) {
     return 32 * (kb >> 1) + 8 * (2 * (kb & 1) + (s >> 2)) + 4 * h + (s & 3);
 }
 
-// packed weights from the FP32 masters: slab t < 24 is layer 1 + t / 8, k-blocks
-// 2 (t % 8) + j, laid out [j][fb][lane][8]; slab 24 is the output layer, [kb][lane][8]
-// with rows n >= 8 zero.
+// packed bf16 weights from the FP32 masters: slab t < 24 is output block t % 8 of layer
+// 1 + t / 8 for all sixteen k-blocks, [kb][lane][8] with row n = 32 (t % 8) + lane % 32;
+// slab 24 is the output layer, [kb][lane][8] with rows n >= 8 zero
 __global__ void k_pack_fused(const float* P, uint16_t* wpk) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= kPackedElems) return;
     const int t = e / kSlabElems, w = e % kSlabElems;
+    const int kb = w / 512, lane = (w / 8) % 64, s = w % 8;
+    const int f = packed_feature(kb, lane >> 5, s);
     float v;
     if (t < kHiddenSlabs) {
-        const int l = 1 + t / 8, sl = t % 8;
-        const int j = w / 4096, fb = (w / 512) % 8, lane = (w / 8) % 64, s = w % 8;
-        const int n = 32 * fb + (lane & 31);
-        const int f = packed_feature(2 * sl + j, lane >> 5, s);
+        const int l = 1 + t / 8, n = 32 * (t % 8) + (lane & 31);
         v = P[ktn::param_offset(l) + (long long)f * ktn::kHidden + n];
     } else {
-        const int kb = w / 512, lane = (w / 8) % 64, s = w % 8;
         const int n = lane & 31;
-        const int f = packed_feature(kb, lane >> 5, s);
         v = n < ktn::kOutputs ? P[ktn::param_offset(4) + (long long)f * ktn::kOutputs + n] : 0.f;
     }
     wpk[e] = f2bf(v);
+}
+
+// the input layer and the biases of layers 0-3 times -log2(e)
+__global__ void k_fold(const float* P, float* fold) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= kFoldElems) return;
+    const int n0 = (ktn::kInputs + 1) * ktn::kHidden;  // input layer: weights then bias
+    const float v = e < n0 ? P[e]
+                           : P[ktn::param_offset(1 + (e - n0) / ktn::kHidden) + (long long)ktn::kHidden * ktn::kHidden +
+                               (e - n0) % ktn::kHidden];
+    fold[e] = v * kNegLog2e;
+}
+
+// the input layer's split-bf16 A fragments from the scaled weights: row n = 32 fb +
+// lane % 32, k-slot kk = 8 (lane / 32) + s holds Wh[kk] (kk < 5), Wh[kk - 5] (kk < 10),
+// Wl[kk - 10] (kk < 15), 0
+__global__ void k_pack_input(const float* fold, uint16_t* wpk0) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= kInputFragElems) return;
+    const int fb = e / 512, lane = (e / 8) % 64, kk = 8 * (lane >> 5) + e % 8;
+    const int n = 32 * fb + (lane & 31);
+    uint16_t v = 0;
+    if (kk < 15) {
+        const float w = fold[(kk % 5) * ktn::kHidden + n];
+        const uint16_t hi = f2bf(w);
+        v = kk < 10 ? hi : f2bf(w - bf2f(hi));
+    }
+    wpk0[e] = v;
 }
 
 struct FusedArgs {
@@ -314,7 +360,9 @@ struct FusedArgs {
     int n_views;
     Table t;
     const float* xf;       // n x 5 when views == nullptr
-    const float* P;        // FP32 masters (input layer, biases)
+    const float* P;        // FP32 masters (output bias)
+    const float* fold;     // input layer and biases of layers 0-3 times -log2(e)
+    const uint16_t* wpk0;  // the input layer's split-bf16 fragments
     const uint16_t* wpk;   // packed bf16 weights of layers 1-4
     // outputs: raw network outputs (n x 8), or the gate's decisions
     float* y;
@@ -324,8 +372,32 @@ struct FusedArgs {
     unsigned long long* n_fallback;
 };
 
-__device__ __forceinline__ float silu_fast(float z) {
-    return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+// -log2(e) silu(z) from z' = -log2(e) z
+__device__ __forceinline__ float silu_scaled(float zs) {
+    return zs * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(zs));
+}
+
+// one pair of the next layer's B operand (k-blocks 2 fb and 2 fb + 1) from block fb of a
+// layer's scaled pre-activations: SiLU, rounded to bf16
+__device__ __forceinline__ void silu_pair(const f32x16& z, bf16x8& p0, bf16x8& p1) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) p0[i] = (__bf16)silu_scaled(z[i]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) p1[i] = (__bf16)silu_scaled(z[8 + i]);
+}
+
+// the 16 values of block fb this lane holds (features 32 fb + 8 q + 4 h + r)
+__device__ __forceinline__ f32x16 block_bias(const float* b, int fb, int h) {
+    f32x16 z;
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+        const float4 bb = *reinterpret_cast<const float4*>(b + 32 * fb + 8 * qq + 4 * h);
+        z[4 * qq + 0] = bb.x;
+        z[4 * qq + 1] = bb.y;
+        z[4 * qq + 2] = bb.z;
+        z[4 * qq + 3] = bb.w;
+    }
+    return z;
 }
 
 __device__ __forceinline__ f32x16 mfma32(const uint4& a, const bf16x8& b, const f32x16& c) {
@@ -333,7 +405,7 @@ __device__ __forceinline__ f32x16 mfma32(const uint4& a, const bf16x8& b, const 
 }
 
 template <bool VIEWS>
-// two waves per SIMD: 214 VGPRs, accumulators in the VGPR file, no spills
+// two waves per SIMD: 248 VGPRs, accumulators in the VGPR file, no spills
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mlp_fused(FusedArgs a) {
     __shared__ uint4 slab0[kSlabElems / 8];
     __shared__ uint4 slab1[kSlabElems / 8];
@@ -359,87 +431,79 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         for (int k = 0; k < 5; ++k) f[k] = a.xf[qc * 5 + k];
     }
 
-    // input layer in FP32: the 128 hidden units this lane holds, as layer 1's B operand
+    // input layer: the ray's split features as the B operand (lane half 0: fh0-4, fl0-2;
+    // half 1: fl3-4, fh0-4, 0), one MFMA per output block, then layer 1's operand
     bf16x8 B[16];
     {
-        const float* W0 = a.P;
-        const float* b0 = a.P + ktn::kInputs * ktn::kHidden;
+        float fh[5], fl[5];
 #pragma unroll
-        for (int fb = 0; fb < 8; ++fb)
+        for (int k = 0; k < 5; ++k) {
+            fh[k] = (float)(__bf16)f[k];
+            fl[k] = f[k] - fh[k];  // exact
+        }
+        const float x0[8] = {fh[0], fh[1], fh[2], fh[3], fh[4], fl[0], fl[1], fl[2]};
+        const float x1[8] = {fl[3], fl[4], fh[0], fh[1], fh[2], fh[3], fh[4], 0.f};
+        bf16x8 xb;
 #pragma unroll
-            for (int qq = 0; qq < 4; ++qq) {
-                const int n0 = 32 * fb + 8 * qq + 4 * h;
-                const float4 bb = *reinterpret_cast<const float4*>(b0 + n0);
-                float z[4] = {bb.x, bb.y, bb.z, bb.w};
+        for (int k = 0; k < 8; ++k) xb[k] = (__bf16)(h ? x1[k] : x0[k]);
+        const uint4* a0 = reinterpret_cast<const uint4*>(a.wpk0);
+        const float* b0 = a.fold + ktn::kInputs * ktn::kHidden;
 #pragma unroll
-                for (int k = 0; k < 5; ++k) {
-                    const float4 w = *reinterpret_cast<const float4*>(W0 + k * ktn::kHidden + n0);
-                    z[0] = fmaf(f[k], w.x, z[0]);
-                    z[1] = fmaf(f[k], w.y, z[1]);
-                    z[2] = fmaf(f[k], w.z, z[2]);
-                    z[3] = fmaf(f[k], w.w, z[3]);
-                }
-#pragma unroll
-                for (int r = 0; r < 4; ++r) B[2 * fb + (qq >> 1)][4 * (qq & 1) + r] = (__bf16)silu_fast(z[r]);
-            }
+        for (int fb = 0; fb < 8; ++fb) {
+            const f32x16 z = mfma32(a0[fb * 64 + lane], xb, block_bias(b0, fb, h));
+            silu_pair(z, B[2 * fb], B[2 * fb + 1]);
+        }
     }
 
-    f32x16 acc[8];
     slab0[tid] = w0;
     slab0[tid + 256] = w1;
     slab0[tid + 512] = w2;
     slab0[tid + 768] = w3;
     __syncthreads();
-    for (int l = 1; l <= 3; ++l) {
-        {  // bias into the accumulators
-            const float* bl = a.P + ktn::param_offset(l) + (long long)ktn::kHidden * ktn::kHidden;
+    {
+        bf16x8 C[16];  // the next layer's operand, a pair per block
+        f32x16 acc, prev;
 #pragma unroll
-            for (int fb = 0; fb < 8; ++fb)
+        for (int l = 1; l <= 3; ++l) {
+            const float* bl = a.fold + (ktn::kInputs + l) * ktn::kHidden;
+            f32x16 bias = block_bias(bl, 0, h);
 #pragma unroll
-                for (int qq = 0; qq < 4; ++qq) {
-                    const float4 bb = *reinterpret_cast<const float4*>(bl + 32 * fb + 8 * qq + 4 * h);
-                    acc[fb][4 * qq + 0] = bb.x;
-                    acc[fb][4 * qq + 1] = bb.y;
-                    acc[fb][4 * qq + 2] = bb.z;
-                    acc[fb][4 * qq + 3] = bb.w;
-                }
-        }
+            for (int fb = 0; fb < 8; ++fb) {
+                const int t = 8 * (l - 1) + fb;
+                // the next slab (the output layer's after the last hidden one) into registers
+                const uint4* nsrc = src + (long long)(t + 1) * (kSlabElems / 8);
+                w0 = nsrc[tid];
+                w1 = nsrc[tid + 256];
+                w2 = nsrc[tid + 512];
+                w3 = nsrc[tid + 768];
+                acc = bias;
+                if (fb < 7) bias = block_bias(bl, fb + 1, h);
+                // scheduling barriers: the global loads above stay ahead of the matrix
+                // work, all sixteen fragment reads issue before the first MFMA (sunk between
+                // the MFMAs each would wait out its LDS latency), and the LDS stores follow
+                __builtin_amdgcn_sched_barrier(0);
+                // eight slabs per layer: the buffer parity is static (slab0 at even fb)
+                const uint4* S = (fb & 1) ? slab1 : slab0;
+                uint4* N = (fb & 1) ? slab0 : slab1;
+                uint4 af[16];
 #pragma unroll
-        for (int sl = 0; sl < 8; ++sl) {
-            // next slab into registers (the output layer's after the last hidden one)
-            const uint4* nsrc = src + (long long)(8 * (l - 1) + sl + 1) * (kSlabElems / 8);
-            w0 = nsrc[tid];
-            w1 = nsrc[tid + 256];
-            w2 = nsrc[tid + 512];
-            w3 = nsrc[tid + 768];
-            // compiler-only fences: the next slab's loads stay ahead of the matrix work
-            // and their LDS stores after it, so the load latency hides behind the MFMAs
-            asm volatile("" ::: "memory");
-            // eight slabs per layer: the buffer parity is static (slab0 at even sl)
-            const uint4* S = (sl & 1) ? slab1 : slab0;
-            uint4* N = (sl & 1) ? slab0 : slab1;
-            // a k-block's eight A fragments are read together, so their LDS latency
-            // overlaps the MFMAs that consume them in turn
+                for (int i = 0; i < 16; ++i) af[i] = S[i * 64 + lane];
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                uint4 af[8];
-#pragma unroll
-                for (int fb = 0; fb < 8; ++fb) af[fb] = S[(j * 8 + fb) * 64 + lane];
-#pragma unroll
-                for (int fb = 0; fb < 8; ++fb) acc[fb] = mfma32(af[fb], B[2 * sl + j], acc[fb]);
+                for (int i = 0; i < 16; ++i) acc = mfma32(af[i], B[i], acc);
+                if (fb > 0) silu_pair(prev, C[2 * fb - 2], C[2 * fb - 1]);
+                prev = acc;
+                __builtin_amdgcn_sched_barrier(0);
+                N[tid] = w0;
+                N[tid + 256] = w1;
+                N[tid + 512] = w2;
+                N[tid + 768] = w3;
+                __syncthreads();
             }
-            asm volatile("" ::: "memory");
-            N[tid] = w0;
-            N[tid + 256] = w1;
-            N[tid + 512] = w2;
-            N[tid + 768] = w3;
-            __syncthreads();
+            silu_pair(prev, C[14], C[15]);
+#pragma unroll
+            for (int k = 0; k < 16; ++k) B[k] = C[k];
         }
-        // SiLU, rounded to bf16: the next layer's operand
-#pragma unroll
-        for (int fb = 0; fb < 8; ++fb)
-#pragma unroll
-            for (int i = 0; i < 16; ++i) B[2 * fb + (i >> 3)][i & 7] = (__bf16)silu_fast(acc[fb][i]);
     }
 
     // output layer: four partial sums over the sixteen k-blocks
@@ -456,7 +520,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const float* b4 = a.P + ktn::param_offset(4) + (long long)ktn::kHidden * ktn::kOutputs;
     float y4[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) y4[r] = o[0][r] + o[1][r] + o[2][r] + o[3][r] + b4[4 * h + r];
+    for (int r = 0; r < 4; ++r) y4[r] = (o[0][r] + o[1][r] + o[2][r] + o[3][r]) * kNegLn2 + b4[4 * h + r];
     // lane m (h = 0) holds outputs 0..3 of ray m, lane m + 32 outputs 4..7
     float y[8];
 #pragma unroll
@@ -548,6 +612,8 @@ struct Surrogate {
     float* partial0 = nullptr;  // input-layer gradient partials
     double* loss = nullptr;
     uint16_t* wpk = nullptr;  // layers 1-4 packed for the fused kernel (bf16)
+    float* fold = nullptr;    // input layer and biases times -log2(e) for the fused kernel
+    uint16_t* wpk0 = nullptr; // the input layer's split-bf16 MFMA fragments
     Queue queue;              // fallback queue of eager calls
     static constexpr int kSplits = 64, kChunks = 256;
 };
@@ -586,6 +652,8 @@ static hipError_t free_all(Surrogate* s) {
     f(s->partial);
     f(s->loss);
     f(s->wpk);
+    f(s->fold);
+    f(s->wpk0);
     queue_release(s->queue);
     return hipSuccess;
 }
@@ -650,7 +718,9 @@ Surrogate* create(int device, hipError_t* err) {
         s->lt.wb[l] = s->wb[l];
         s->lt.wt[l] = s->wt[l];
     }
-    if ((e = dalloc(&s->wpk, kPackedElems)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&s->wpk, kPackedElems)) != hipSuccess || (e = dalloc(&s->fold, kFoldElems)) != hipSuccess ||
+        (e = dalloc(&s->wpk0, kInputFragElems)) != hipSuccess)
+        return fail(e);
     if ((e = build_table(s)) != hipSuccess) return fail(e);
     if ((e = dalloc(&s->loss, 1)) != hipSuccess) return fail(e);
     if (err) *err = hipSuccess;
@@ -673,6 +743,8 @@ double table_critical_radius(const Surrogate* s, double a, double inc, double ps
 double table_shadow_radius(const Surrogate* s, double a, double inc) { return table_rsh(s->h_rsh.data(), a, inc); }
 
 static hipError_t pack(Surrogate* s, hipStream_t st) {
+    hipLaunchKernelGGL(k_fold, dim3((kFoldElems + 255) / 256), dim3(256), 0, st, s->P, s->fold);
+    hipLaunchKernelGGL(k_pack_input, dim3(kInputFragElems / 256), dim3(256), 0, st, s->fold, s->wpk0);
     hipLaunchKernelGGL(k_pack_fused, dim3((kPackedElems + 255) / 256), dim3(256), 0, st, s->P, s->wpk);
     return hipGetLastError();
 }
@@ -868,6 +940,8 @@ static FusedArgs fused_args(Surrogate* s, long long n) {
     a.n = n;
     a.t = Table{s->tab_rho, s->tab_rsh};
     a.P = s->P;
+    a.fold = s->fold;
+    a.wpk0 = s->wpk0;
     a.wpk = s->wpk;
     return a;
 }

@@ -57,6 +57,30 @@ def emulated_forward(params, x):
     return h, Wq
 
 
+NEG_LOG2E = np.float32(-1.44269504088896341)
+NEG_LN2 = np.float32(-0.693147180559945309)
+
+
+def fused_forward(params, x):
+    """the fused inference kernel's arithmetic: pre-activations scaled by -log2(e) (so
+    exp(-z) = 2^z'), activations stored as bf16(-log2(e) silu(z)), the input layer as
+    split-bf16 products fh Wh + fl Wh + fh Wl, bf16 weights, the output times -ln(2)"""
+    (W0, b0), hidden, (W4, b4) = params[0], params[1:-1], params[-1]
+    scaled = lambda v: (np.asarray(v, np.float32) * NEG_LOG2E).astype(np.float64)  # float32 products
+    W0s = scaled(W0)
+    Wh = bf16(W0s)
+    Wl = bf16(W0s - Wh)
+    xf = f32(x)
+    fh = bf16(xf)
+    fl = bf16(xf - fh)
+    z = fh @ Wh + fl @ Wh + fh @ Wl + scaled(b0)
+    act = lambda zs: bf16(zs / (1.0 + np.exp2(zs)))
+    h = act(z)
+    for W, b in hidden:
+        h = act(h @ bf16(W) + scaled(b))
+    return float(NEG_LN2) * (h @ bf16(W4)) + f32(b4)
+
+
 def problem(seed=3, n=1024):
     rng = np.random.default_rng(seed)
     params = M.init_params(seed)
@@ -68,14 +92,23 @@ def problem(seed=3, n=1024):
 
 
 def test_forward_matches_numpy(tracer):
+    """Against the network (exact SiLU, FP32 input layer, bf16 weights and activations) to
+    the bf16 noise, and against the fused kernel's own arithmetic far tighter: what is left
+    there is FP32 accumulation order and the hardware exp2/rcp, which move an activation's
+    bf16 rounding now and then."""
     params, x, _, _ = problem()
     tracer.surrogate_load(flat(params))
     y = tracer.surrogate_predict(x).cpu().numpy()
-    want, _ = emulated_forward(unflat(flat(params)), x)
+    P = unflat(flat(params))
+    want, _ = emulated_forward(P, x)
     scale = np.abs(want).max()
     err = np.abs(y - want)
     print(f"forward: max err {err.max() / scale:.2e}, median {np.median(err) / scale:.2e} (relative to {scale:.2f})")
     assert err.max() < 2e-2 * scale and np.median(err) < 1e-3 * scale
+    exact = fused_forward(P, x)
+    err = np.abs(y - exact)
+    print(f"fused arithmetic: max err {err.max() / scale:.2e}, median {np.median(err) / scale:.2e}")
+    assert err.max() < 2e-3 * scale and np.median(err) < 2e-5 * scale
 
 
 def test_gradients_match_numpy(tracer):
