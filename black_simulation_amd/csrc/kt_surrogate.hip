@@ -137,6 +137,113 @@ __host__ __device__ inline void sky_from_frame(const double e1[3], const double 
     if (phi < 0.0) phi += 2.0 * kPi;
 }
 
+// ---- the same in FP32, for the fused inference kernel --------------------------------------
+//
+// The network reads its inputs through a split-bf16 input layer and carries bf16
+// activations, and its sky directions are good to ~1e-2 rad, so its inputs and the angles
+// formed from its outputs need float accuracy only (~1e-7; the polar angle comes from
+// atan2, well conditioned at the poles).  Training labels keep the FP64 forms above.
+
+__device__ __forceinline__ void cell_f(float x, float x0, float x1, int n, int& i, float& f) {
+    float u = (x - x0) * ((float)(n - 1) / (x1 - x0));
+    u = fminf(fmaxf(u, 0.f), (float)(n - 1));
+    i = (int)u;
+    if (i > n - 2) i = n - 2;
+    f = u - (float)i;
+}
+
+__device__ __forceinline__ float table_rsh_f(const float* rsh, float a, float inc) {
+    int ia, ii;
+    float fa, fi;
+    cell_f(a, (float)kTabA0, (float)kTabA1, kTabA, ia, fa);
+    cell_f(inc, (float)kTabI0, (float)kTabI1, kTabI, ii, fi);
+    const float v00 = rsh[ia * kTabI + ii], v01 = rsh[ia * kTabI + ii + 1];
+    const float v10 = rsh[(ia + 1) * kTabI + ii], v11 = rsh[(ia + 1) * kTabI + ii + 1];
+    return (1 - fa) * ((1 - fi) * v00 + fi * v01) + fa * ((1 - fi) * v10 + fi * v11);
+}
+
+__device__ __forceinline__ float table_rho_f(const float* rho, float a, float inc, float psi) {
+    int ia, ii, ip;
+    float fa, fi, fp;
+    cell_f(a, (float)kTabA0, (float)kTabA1, kTabA, ia, fa);
+    cell_f(inc, (float)kTabI0, (float)kTabI1, kTabI, ii, fi);
+    cell_f(psi, 0.f, (float)kPi, kTabPsi, ip, fp);
+    auto at = [&](int x, int y, int z) { return rho[(x * kTabI + y) * kTabPsi + z]; };
+    float r = 0.f;
+    for (int dx = 0; dx < 2; ++dx)
+        for (int dy = 0; dy < 2; ++dy) {
+            const float w = (dx ? fa : 1 - fa) * (dy ? fi : 1 - fi);
+            r += w * ((1 - fp) * at(ia + dx, ii + dy, ip) + fp * at(ia + dx, ii + dy, ip + 1));
+        }
+    return r;
+}
+
+// one ray's camera and image-plane position in float: spin, camera radius, inclination
+// (with its sine and cosine), sin/cos of the camera azimuth, Bardeen alpha and beta
+struct RayF {
+    float a, r_cam, inc, si, ci, sp, cp, alpha, beta;
+};
+
+// features_of in float
+__device__ __forceinline__ void features_f(const float* rho, const float* rsh, const RayF& p, float f[5],
+                                           float* ring) {
+    const float R = table_rsh_f(rsh, p.a, p.inc);
+    const float inv = __builtin_amdgcn_rcpf(1.5f * R);
+    f[0] = __log10f(p.r_cam) - 2.0f;
+    f[1] = (p.ci - 0.5415f) * (1.0f / 0.4545f);
+    f[2] = (p.a - 0.499f) * (1.0f / 0.499f);
+    f[3] = p.alpha * inv;
+    f[4] = p.beta * inv;
+    const float rc = table_rho_f(rho, p.a, p.inc, atan2f(fabsf(p.beta), p.alpha));
+    *ring = fabsf(sqrtf(p.alpha * p.alpha + p.beta * p.beta) * __builtin_amdgcn_rcpf(rc) - 1.0f);
+}
+
+// ray_frame in float
+__device__ __forceinline__ void ray_frame_f(const RayF& p, float e1[3], float e2[3], float e3[3]) {
+    const float r = p.r_cam, al = p.alpha, be = p.beta;
+    const float rh[3] = {p.si * p.cp, p.si * p.sp, p.ci}, th[3] = {p.ci * p.cp, p.ci * p.sp, -p.si},
+                ph[3] = {-p.sp, p.cp, 0.f};
+    float q = 1.0f - (al * al + be * be) / (r * r);
+    q = q > 0.f ? sqrtf(q) : 0.f;
+    float n = 0.f;
+    for (int k = 0; k < 3; ++k) {
+        e1[k] = -q * rh[k] - (be / r) * th[k] - (al / r) * ph[k];
+        n += e1[k] * e1[k];
+    }
+    n = __frsqrt_rn(n);
+    for (int k = 0; k < 3; ++k) e1[k] *= n;
+    const float pe = r * (rh[0] * e1[0] + rh[1] * e1[1] + rh[2] * e1[2]);
+    float m = 0.f;
+    for (int k = 0; k < 3; ++k) {
+        e2[k] = -(r * rh[k] - pe * e1[k]);
+        m += e2[k] * e2[k];
+    }
+    if (!(m > 1e-12f * r * r)) {  // aimed at the hole: any perpendicular will do
+        const float te = th[0] * e1[0] + th[1] * e1[1] + th[2] * e1[2];
+        m = 0.f;
+        for (int k = 0; k < 3; ++k) {
+            e2[k] = th[k] - te * e1[k];
+            m += e2[k] * e2[k];
+        }
+    }
+    m = __frsqrt_rn(m);
+    for (int k = 0; k < 3; ++k) e2[k] *= m;
+    e3[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    e3[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    e3[2] = e1[0] * e2[1] - e1[1] * e2[0];
+}
+
+// sky_from_frame in float: theta from atan2 (accurate at the poles, where acos is not)
+__device__ __forceinline__ void sky_from_frame_f(const float e1[3], const float e2[3], const float e3[3], float c1,
+                                                 float c2, float c3, double& theta, double& phi) {
+    float N[3];
+    for (int k = 0; k < 3; ++k) N[k] = c1 * e1[k] + c2 * e2[k] + c3 * e3[k];
+    theta = (double)atan2f(sqrtf(N[0] * N[0] + N[1] * N[1]), N[2]);
+    float f = atan2f(N[1], N[0]);
+    if (f < 0.f) f += 2.0f * (float)kPi;
+    phi = (double)f;
+}
+
 // ---- kernels -------------------------------------------------------------------------------
 
 __device__ __forceinline__ int find_view(const ViewDev* v, int n, long long idx) {
@@ -180,6 +287,41 @@ __device__ __forceinline__ bool ray_params(const ViewDev* views, int n_views, lo
         const int row = (int)(local / v.width), col = (int)(local - (long long)row * v.width);
         alpha = -v.half_width + ((double)col + 0.5) * v.px;
         beta = v.half_height - ((double)row + 0.5) * v.py;
+    }
+    return true;
+}
+
+// ray_params in float (the fused inference kernel): false for rays the surrogate does not
+// cover (pinhole cameras)
+__device__ __forceinline__ bool ray_params_f(const ViewDev* views, int n_views, long long q, RayF& p) {
+    const ViewDev& v = views[find_view(views, n_views, q)];
+    const long long local = q - v.offset;
+    if (v.kind == 2) {
+        const double* pp = v.params + 5 * local;
+        p.a = (float)pp[0];
+        p.r_cam = (float)pp[1];
+        p.inc = (float)pp[2];
+        p.alpha = (float)pp[3];
+        p.beta = (float)pp[4];
+        sincosf(p.inc, &p.si, &p.ci);
+        p.sp = 0.f;
+        p.cp = 1.f;
+        return true;
+    }
+    if (v.kind != 0) return false;
+    p.a = (float)v.a;
+    p.r_cam = (float)v.r_cam;
+    p.si = (float)v.sinI;
+    p.ci = (float)v.cosI;
+    p.inc = atan2f(p.si, p.ci);
+    sincosf((float)v.phi_cam, &p.sp, &p.cp);
+    if (v.params) {
+        p.alpha = (float)v.params[2 * local];
+        p.beta = (float)v.params[2 * local + 1];
+    } else {
+        const int row = (int)(local / v.width), col = (int)(local - (long long)row * v.width);
+        p.alpha = (float)(-v.half_width + ((double)col + 0.5) * v.px);
+        p.beta = (float)(v.half_height - ((double)row + 0.5) * v.py);
     }
     return true;
 }
@@ -423,9 +565,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     float f[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
     float rg = -1.f;  // relative distance from the critical curve; < 0: not coverable
     if (VIEWS) {
-        double ra, rc, inc, al, be, phc, rin;
-        if (ray_params(a.views, a.n_views, qc, ra, rc, inc, al, be, phc, rin))
-            features_of(a.t.rho, a.t.rsh, ra, rc, inc, al, be, f, &rg);
+        RayF p;
+        if (ray_params_f(a.views, a.n_views, qc, p)) features_f(a.t.rho, a.t.rsh, p, f, &rg);
     } else {
 #pragma unroll
         for (int k = 0; k < 5; ++k) f[k] = a.xf[qc * 5 + k];
@@ -552,17 +693,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         if (to_fallback) a.fallback[base + __popcll(mask & ((1ull << lane) - 1))] = q;
     }
     if (!live || h != 0 || to_fallback) return;
-    double ra, rc, inc, al, be, phc, rin;
-    ray_params(a.views, a.n_views, q, ra, rc, inc, al, be, phc, rin);
     const double nan = __builtin_nan("");
     double th = nan, ph = nan, rh = nan, g = nan;
     if (c == 0) {
-        double u1[3], u2[3], u3[3];
-        ray_frame(rc, inc, phc, al, be, u1, u2, u3);
-        sky_from_frame(u1, u2, u3, y[3], y[4], y[5], th, ph);
+        RayF p;
+        ray_params_f(a.views, a.n_views, q, p);
+        float u1[3], u2[3], u3[3];
+        ray_frame_f(p, u1, u2, u3);
+        sky_from_frame_f(u1, u2, u3, y[3], y[4], y[5], th, ph);
     } else if (c == 2) {
-        rh = exp((double)y[6]);
-        g = exp((double)y[7]);
+        rh = (double)__expf(y[6]);
+        g = (double)__expf(y[7]);
     }
     if (a.out.cls) a.out.cls[q] = (int8_t)(c | ktd::kFlagSurrogate);
     if (a.out.theta) a.out.theta[q] = th;

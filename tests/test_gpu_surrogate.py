@@ -253,3 +253,63 @@ def test_surrogate_plans_replay_concurrently_on_two_streams(tracer, trained):
         torch.cuda.synchronize()
     for p in plans:
         p.close()
+
+
+def frame64(r, inc, phc, al, be):
+    """the ray's straight-line frame (e1 undeflected direction, e2 towards the hole,
+    e3 = e1 x e2), FP64, as the training targets use it"""
+    si, ci, sp, cp = math.sin(inc), math.cos(inc), math.sin(phc), math.cos(phc)
+    rh, th, ph = np.array([si * cp, si * sp, ci]), np.array([ci * cp, ci * sp, -si]), np.array([-sp, cp, 0.0])
+    q = math.sqrt(max(1.0 - (al * al + be * be) / (r * r), 0.0))
+    e1 = -q * rh - (be / r) * th - (al / r) * ph
+    e1 /= np.linalg.norm(e1)
+    e2 = -(r * rh - r * np.dot(rh, e1) * e1)
+    e2 /= np.linalg.norm(e2)
+    return e1, e2, np.cross(e1, e2)
+
+
+def test_fused_features_and_sky_angles_match_fp64(tracer):
+    """The fused kernel forms the network's inputs and the answered rays' sky angles in
+    FP32.  Against the same steps in FP64 on the host (features through the shadow table,
+    the network through surrogate_predict, the frame and the angles): the same classes,
+    sky directions within 1e-5 rad at the median (a feature's last bit can move a bf16
+    rounding inside the network, so a few rays differ more), r_hit and g likewise."""
+    from black_simulation_amd.train_surrogate import load_params
+
+    tracer.surrogate_load(load_params())
+    v = bsa.View(width=96, height=64, spin=0.9, r_cam=40.0, theta_cam=math.radians(80.0), phi_cam=0.7,
+                 half_width=12.0, half_height=8.0, disk_inner=0.0, disk_outer=20.0, escape_radius=1000.0,
+                 capture_factor=1.01)
+    got = bsa.to_numpy(tracer.trace_views([v], bsa.TraceOpts(use_surrogate=1, precision=bsa.PREC_MIXED)))
+    sur = (got["cls"] & bsa.FLAG_SURROGATE) != 0
+    assert sur.mean() > 0.5
+    px, py = 2 * v.half_width / v.width, 2 * v.half_height / v.height
+    idx = np.nonzero(sur)[0]
+    al = -v.half_width + (idx % v.width + 0.5) * px
+    be = v.half_height - (idx // v.width + 0.5) * py
+    a, inc = v.spin, v.theta_cam
+    R = tracer.surrogate_shadow_radius(a, inc)
+    feat = np.stack([np.full(len(idx), math.log10(v.r_cam) - 2.0), np.full(len(idx), (math.cos(inc) - 0.5415) / 0.4545),
+                     np.full(len(idx), (a - 0.499) / 0.499), al / (1.5 * R), be / (1.5 * R)], axis=1)
+    y = tracer.surrogate_predict(feat).cpu().numpy().astype(np.float64)
+    cls = np.argmax(y[:, :3], axis=1)
+    assert np.array_equal(cls, got["cls"][idx] & 7)
+    esc = np.nonzero(cls == 0)[0]
+    d = []
+    for k in esc:
+        e1, e2, e3 = frame64(v.r_cam, inc, v.phi_cam, al[k], be[k])
+        N = y[k, 3] * e1 + y[k, 4] * e2 + y[k, 5] * e3
+        N /= np.linalg.norm(N)
+        t, p = got["theta_inf"][idx[k]], got["phi_inf"][idx[k]]
+        M = np.array([math.sin(t) * math.cos(p), math.sin(t) * math.sin(p), math.cos(t)])
+        d.append(math.atan2(np.linalg.norm(np.cross(N, M)), np.dot(N, M)))
+    d = np.array(d)
+    disk = np.nonzero(cls == 2)[0]
+    rr = np.abs(got["r_hit"][idx[disk]] / np.exp(y[disk, 6]) - 1.0)
+    gr = np.abs(got["g"][idx[disk]] / np.exp(y[disk, 7]) - 1.0)
+    q = lambda x: np.quantile(x, [0.5, 0.99, 1.0])
+    print(f"{len(esc)} escaped, {len(disk)} disk answered; p50 p99 max: sky {q(d)} rad, r_hit {q(rr)}, g {q(gr)}",
+          flush=True)
+    assert len(esc) > 100 and len(disk) > 100
+    for x in (d, rr, gr):
+        assert np.median(x) < 1e-5 and np.quantile(x, 0.99) < 2e-3 and x.max() < 5e-2
