@@ -63,6 +63,7 @@ class KtTraceOpts(C.Structure):
         ("gate_prob", C.c_double),
         ("gate_ring", C.c_double),
         ("waves_per_simd", C.c_int),
+        ("cancel", C.c_void_p),
     ]
 
 

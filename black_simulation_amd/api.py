@@ -88,11 +88,17 @@ class TraceOpts:
     gate_prob: float = 0.999
     gate_ring: float = 0.03
     waves_per_simd: int = 0
+    # EHT render / calibrate: a ctypes.c_int another thread may set to cancel (eht::render's
+    # cancel flag); None: not cancellable
+    cancel: object = None
 
     def to_c(self) -> K.KtTraceOpts:
         o = K.KtTraceOpts()
         for f in dataclasses.fields(self):
-            setattr(o, f.name, getattr(self, f.name))
+            if f.name == "cancel":
+                o.cancel = C.addressof(self.cancel) if self.cancel is not None else None
+            else:
+                setattr(o, f.name, getattr(self, f.name))
         return o
 
 

@@ -155,7 +155,7 @@ __device__ __forceinline__ bool transfer(const EhtLaunch& e, const Photon<double
 }
 
 __global__ __launch_bounds__(128) void k_eht(EhtLaunch e) {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long q = e.first + (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= e.n) return;
     double alpha, beta;
     if (e.rays) {
@@ -245,8 +245,8 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_final(const double* partial, 
 }  // namespace
 
 hipError_t render(const EhtLaunch& e, hipStream_t st) {
-    if (e.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_eht, dim3((unsigned)((e.n + 127) / 128)), dim3(128), 0, st, e);
+    if (e.n <= e.first) return hipSuccess;
+    hipLaunchKernelGGL(k_eht, dim3((unsigned)((e.n - e.first + 127) / 128)), dim3(128), 0, st, e);
     return hipGetLastError();
 }
 

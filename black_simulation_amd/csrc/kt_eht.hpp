@@ -13,6 +13,7 @@ constexpr int kSumPartials = 1024;
 
 struct EhtLaunch {
     long long n;          // rays: size * size pixels, or the explicit list's length
+    long long first;      // this launch traces rays [first, n)
     const double* rays;   // explicit (alpha, beta) pairs, or null for the image's pixels
     int size;
     double half_width, px;

@@ -1049,6 +1049,33 @@ int ensure_eht(kt_ctx* ctx) {
     return KT_OK;
 }
 
+// the launch itself: one kernel, or slice after slice while a cancel flag stays clear
+// (eht::render polls its flag per row, src/eht.cpp:330-331).  Rays of slices not traced
+// are zeroed.  A slice is an eighth of the rays, at least 2^17 (two waves per SIMD).
+int eht_launch(kt_ctx* ctx, kte::EhtLaunch e, const volatile int* cancel, hipStream_t stream) {
+    if (!cancel) {
+        hipError_t err = kte::render(e, stream);
+        return err == hipSuccess ? KT_OK : hip_fail(ctx, err, "EHT kernel launch");
+    }
+    const long long n = e.n, slice = std::max<long long>((n + 7) / 8, 1LL << 17);
+    for (long long first = 0; first < n; first += slice) {
+        if (*cancel) {
+            const size_t k = (size_t)first, rest = (size_t)(n - first);
+            KT_HIP(hipMemsetAsync(e.intensity + k, 0, 8 * rest, stream));
+            if (e.shadow) KT_HIP(hipMemsetAsync(e.shadow + k, 0, rest, stream));
+            if (e.min_radius) KT_HIP(hipMemsetAsync(e.min_radius + k, 0, 8 * rest, stream));
+            if (e.steps) KT_HIP(hipMemsetAsync(e.steps + k, 0, 4 * rest, stream));
+            return KT_ECANCELLED;
+        }
+        e.first = first;
+        e.n = std::min(n, first + slice);
+        hipError_t err = kte::render(e, stream);
+        if (err != hipSuccess) return hip_fail(ctx, err, "EHT kernel launch");
+        KT_HIP(hipStreamSynchronize(stream));
+    }
+    return KT_OK;
+}
+
 // ray count n: explicit (ab != null) or the view's pixels; outputs staged when on the host
 int eht_run(kt_ctx* ctx, const kt_eht_view* view, const kt_eht_flow* flow, const double* ab, long long n,
             const kt_trace_opts* opts, const kt_eht_out* out, hipStream_t stream) {
@@ -1069,8 +1096,9 @@ int eht_run(kt_ctx* ctx, const kt_eht_view* view, const kt_eht_flow* flow, const
         e.shadow = out->shadow;
         e.min_radius = out->min_radius;
         e.steps = out->steps;
-        hipError_t err = kte::render(e, stream);
-        return err == hipSuccess ? KT_OK : hip_fail(ctx, err, "EHT kernel launch");
+        const int st = eht_launch(ctx, e, o.cancel, stream);
+        if (st == KT_ECANCELLED) return fail(ctx, KT_ECANCELLED, "cancelled");
+        return st;
     }
     // host buffers: explicit rays, then intensity, min radius, steps, shadow on the device
     const size_t nr = (size_t)e.n, in_bytes = ab ? 16 * nr : 0;
@@ -1084,14 +1112,14 @@ int eht_run(kt_ctx* ctx, const kt_eht_view* view, const kt_eht_flow* flow, const
     e.min_radius = out->min_radius ? e.intensity + nr : nullptr;
     e.steps = out->steps ? reinterpret_cast<int32_t*>(e.intensity + 2 * nr) : nullptr;
     e.shadow = out->shadow ? reinterpret_cast<uint8_t*>(e.intensity + 2 * nr) + 4 * nr : nullptr;
-    hipError_t err = kte::render(e, stream);
-    if (err != hipSuccess) return hip_fail(ctx, err, "EHT kernel launch");
+    const int st = eht_launch(ctx, e, o.cancel, stream);
+    if (st != KT_OK && st != KT_ECANCELLED) return st;
     KT_HIP(hipMemcpyAsync(out->intensity, e.intensity, 8 * nr, hipMemcpyDeviceToHost, stream));
     if (out->min_radius) KT_HIP(hipMemcpyAsync(out->min_radius, e.min_radius, 8 * nr, hipMemcpyDeviceToHost, stream));
     if (out->steps) KT_HIP(hipMemcpyAsync(out->steps, e.steps, 4 * nr, hipMemcpyDeviceToHost, stream));
     if (out->shadow) KT_HIP(hipMemcpyAsync(out->shadow, e.shadow, nr, hipMemcpyDeviceToHost, stream));
     KT_HIP(hipStreamSynchronize(stream));
-    return KT_OK;
+    return st == KT_OK ? KT_OK : fail(ctx, KT_ECANCELLED, "cancelled");
 }
 
 int eht_flux(kt_ctx* ctx, const double* intensity, int size, double half_width, double theta_guas, double* jy,
@@ -1191,13 +1219,19 @@ int kt_eht_calibrate(kt_ctx* ctx, const kt_eht_view* view, const kt_eht_flow* fl
         // log n0 against log flux, each step held within a factor 30, to 1%
         double n0 = f.n0, f0 = flux_at(n0);
         for (int k = 0; k < 8 && f0 <= 0.0 && st == KT_OK; ++k) f0 = flux_at(n0 *= 10.0);
+        if (st == KT_ECANCELLED) *n0_out = n0;  // the estimate so far, as eht::calibrate returns it
         if (st != KT_OK) return st;
         if (f0 <= 0.0) {
             *n0_out = n0;
             return KT_OK;
         }
         double n1 = n0 * target_jy / f0, f1 = flux_at(n1);
+        const volatile int* cancel = opts ? opts->cancel : nullptr;
         for (int k = 0; k < 8 && st == KT_OK; ++k) {
+            if (cancel && *cancel) {
+                st = KT_ECANCELLED;
+                break;
+            }
             if (f1 <= 0.0 || std::fabs(f1 / target_jy - 1.0) < 0.01) break;
             const double slope = (std::log(f1) - std::log(f0)) / (std::log(n1) - std::log(n0));
             const double next =
@@ -1207,6 +1241,7 @@ int kt_eht_calibrate(kt_ctx* ctx, const kt_eht_view* view, const kt_eht_flow* fl
             n1 = std::clamp(next, n0 / 30.0, n0 * 30.0);
             f1 = flux_at(n1);
         }
+        if (st == KT_ECANCELLED) *n0_out = n1;
         if (st != KT_OK) return st;
         *n0_out = n1;
         return KT_OK;

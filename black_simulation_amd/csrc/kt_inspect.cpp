@@ -5,11 +5,16 @@
 // eht::render call: one line of JSON per request.
 //
 //   kt_inspect <spin> <r_cam> <incl_deg> <half_width> <width> <height> [col row]
+//   kt_inspect eht <spin> <incl_deg> <half_width> <size> [cancel_after_ms]
+//       eht::render of M87*'s defaults (230 GHz, 6.5e9 M_sun, the default flow); with
+//       cancel_after_ms >= 0 another thread sets the cancel flag after that many ms
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
-
+#include <cstring>
 #include <string>
+#include <thread>
 
 #include "../../include/kerr_trace.hpp"
 
@@ -21,7 +26,50 @@ static std::string num(double x) {
     return buf;
 }
 
+static int eht(int argc, char** argv) {
+    kt::EhtView v;
+    v.a = std::atof(argv[2]);
+    v.inclination = std::atof(argv[3]) * 3.14159265358979323846 / 180.0;
+    v.half_width = std::atof(argv[4]);
+    v.size = std::atoi(argv[5]);
+    const int after = argc > 6 ? std::atoi(argv[6]) : -1;
+    kt::Flow flow;
+    std::atomic<int> cancel{0};
+    std::thread setter;
+    if (after >= 0)
+        setter = std::thread([&] {
+            std::this_thread::sleep_for(std::chrono::milliseconds(after));
+            cancel = 1;
+        });
+    kt::Tracer tracer(0);
+    const kt::EhtImage img = tracer.ehtRender(v, flow, after >= 0 ? &cancel : nullptr);
+    if (setter.joinable()) setter.join();
+    // rows with any light or shadow: a cancelled render leaves the rest zero
+    int rows = 0;
+    double total = 0.0;
+    for (int r = 0; r < img.size; ++r) {
+        bool any = false;
+        for (int c = 0; c < img.size; ++c) {
+            const size_t k = static_cast<size_t>(r) * img.size + c;
+            total += img.intensity[k];
+            any = any || img.intensity[k] != 0.0 || img.shadow[k] != 0;
+        }
+        rows += any;
+    }
+    std::printf("{\"size\": %d, \"cancelled\": %s, \"rows_with_light\": %d, \"intensity_sum\": %s}\n", img.size,
+                img.cancelled ? "true" : "false", rows, num(total).c_str());
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc >= 6 && std::strcmp(argv[1], "eht") == 0) {
+        try {
+            return eht(argc, argv);
+        } catch (const kt::Error& e) {
+            std::fprintf(stderr, "kt_inspect: %s (status %d)\n", e.what(), e.status);
+            return 1;
+        }
+    }
     if (argc != 7 && argc != 9) {
         std::fprintf(stderr, "usage: %s spin r_cam incl_deg half_width width height [col row]\n", argv[0]);
         return 2;

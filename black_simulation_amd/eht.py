@@ -88,6 +88,7 @@ class Image:
     shadow: object     # (size, size) uint8
     min_radius: object = None
     steps: object = None
+    cancelled: bool = False  # the opts' cancel flag stopped the render: rows not traced are zero
 
     def pixel(self) -> float:
         return 2.0 * self.half_width / self.size
@@ -139,10 +140,12 @@ def render(view: EhtView, flow: Flow | None = None, opts: TraceOpts | None = Non
     out, s = _alloc(t, view.size * view.size, details)
     f = (flow or Flow()).to_c()
     v, o = view.to_c(), _opts(opts)
-    _check(t._ctx, K.lib().kt_eht_render(t._ctx, C.byref(v), C.byref(f), C.byref(o), C.byref(s), _stream(t, stream)),
-           "kt_eht_render")
+    rc = K.lib().kt_eht_render(t._ctx, C.byref(v), C.byref(f), C.byref(o), C.byref(s), _stream(t, stream))
+    if rc != K.KT_ECANCELLED:
+        _check(t._ctx, rc, "kt_eht_render")
     sq = {k: x.view(view.size, view.size) for k, x in out.items()}
-    return Image(view.size, view.half_width, sq["intensity"], sq["shadow"], sq.get("min_radius"), sq.get("steps"))
+    return Image(view.size, view.half_width, sq["intensity"], sq["shadow"], sq.get("min_radius"), sq.get("steps"),
+                 cancelled=rc == K.KT_ECANCELLED)
 
 
 def trace_rays(view: EhtView, flow: Flow | None, ab, opts: TraceOpts | None = None, tracer: Tracer | None = None,
@@ -195,13 +198,16 @@ def flux(image: Image, theta_guas: float, tracer: Tracer | None = None, stream=N
 
 def calibrate(view: EhtView, flow: Flow | None, target_jy: float, theta_guas: float, opts: TraceOpts | None = None,
               tracer: Tracer | None = None, stream=None) -> float:
-    """The density n0 for which a 48 x 48 render of the view has target_jy janskys."""
+    """The density n0 for which a 48 x 48 render of the view has target_jy janskys.  When
+    opts.cancel is set meanwhile, the estimate so far (as eht::calibrate returns it)."""
     t = _tracer(tracer)
     f = (flow or Flow()).to_c()
     v, o = view.to_c(), _opts(opts)
     n0 = C.c_double()
-    _check(t._ctx, K.lib().kt_eht_calibrate(t._ctx, C.byref(v), C.byref(f), target_jy, theta_guas, C.byref(o),
-                                            C.byref(n0), _stream(t, stream)), "kt_eht_calibrate")
+    rc = K.lib().kt_eht_calibrate(t._ctx, C.byref(v), C.byref(f), target_jy, theta_guas, C.byref(o), C.byref(n0),
+                                  _stream(t, stream))
+    if rc != K.KT_ECANCELLED:
+        _check(t._ctx, rc, "kt_eht_calibrate")
     return n0.value
 
 

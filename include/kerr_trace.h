@@ -118,6 +118,14 @@ typedef struct kt_trace_opts {
     double gate_prob;       /* surrogate: min class probability (0.999)             */
     double gate_ring;       /* surrogate: min |rho/rho_ring - 1| (0.03)             */
     int waves_per_simd;     /* register budget of the persistent kernel: 1-4 (0 = tuned) */
+    /* kt_eht_render, kt_eht_trace_rays and kt_eht_calibrate (as eht::render's and
+     * eht::calibrate's cancel, src/eht.h:140-153): a host flag read between slices of the
+     * image (and between calibration renders).  Once it reads nonzero no further slice is
+     * traced, the rays not traced are left zero (as eht::render leaves its rows) and the
+     * call returns KT_ECANCELLED; calibrate still writes its current estimate.  A cancellable
+     * render runs slice by slice, synchronising the stream after each.  NULL: not
+     * cancellable.  The other calls ignore it. */
+    const volatile int* cancel;
 } kt_trace_opts;
 
 /* SoA outputs; any pointer may be NULL.  on_device: 1 = device pointers

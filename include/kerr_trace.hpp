@@ -1,8 +1,14 @@
 // kerr_trace.hpp -- header-only C++20 face of libkerr_trace.so (over include/kerr_trace.h).
 //
 // Shaped like the reference's own calls so that a caller swaps one for the other:
-//   kt::Tracer::render(view)            for eht::render(View, Flow, threads, cancel)  (src/eht.h:140)
-//   kt::Tracer::traceRay(view, a, b)    for eht::traceRay(View, Flow, alpha, beta)   (src/eht.h:130)
+//   kt::Tracer::render(view)            per-ray classes, sky angles, disk r and g of a view
+//   kt::Tracer::traceRay(view, a, b)    one Bardeen-plane ray
+//   kt::Tracer::ehtRender(view, flow, cancel)
+//                                       for eht::render(View, Flow, threads, cancel)  (src/eht.h:140)
+//   kt::Tracer::ehtTraceRay(view, flow, a, b)
+//                                       for eht::traceRay(View, Flow, alpha, beta)   (src/eht.h:130)
+//   kt::Tracer::ehtCalibrate(view, flow, jy, theta, cancel)
+//                                       for eht::calibrate(...)                      (src/eht.h:152)
 //   kt::Tracer::traceLooks(view, looks) for trajectory::traceRay(..., look, scene)   (src/trajectory.h:118)
 //   kt::horizonRadius, iscoRadius, photonOrbitRadius, circularOmega              (src/kerr.h:39-45, 71)
 //   kt::criticalCurve, criticalRadius                                             (src/eht.h:66-68)
@@ -11,6 +17,7 @@
 #pragma once
 
 #include <array>
+#include <atomic>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -35,6 +42,38 @@ struct View : kt_view {
 struct Options : kt_trace_opts {
     Options() { kt_trace_opts_default(this); }
 };
+
+// eht::View and eht::Flow (src/eht.h:84-127) with the reference's defaults
+struct EhtView : kt_eht_view {
+    EhtView() { kt_eht_view_default(this); }
+};
+struct Flow : kt_eht_flow {
+    Flow() { kt_eht_flow_default(this); }
+};
+
+// eht::Image (src/eht.h:132-138): rows from the top; cancelled when the flag stopped it
+// (rows not traced are zero, as eht::render leaves them)
+struct EhtImage {
+    int size = 0;
+    double halfWidth = 0.0;
+    std::vector<double> intensity;
+    std::vector<uint8_t> shadow;
+    bool cancelled = false;
+    double pixel() const { return 2.0 * halfWidth / size; }
+};
+
+// eht::Ray (src/eht.h:123-128)
+struct EhtRay {
+    double intensity = 0.0;
+    bool captured = false;
+    double minRadius = 0.0;
+    int steps = 0;
+};
+
+// The cancel flag the C layer reads (const volatile int*): a std::atomic<int> with int's
+// representation.
+static_assert(sizeof(std::atomic<int>) == sizeof(int) && std::atomic<int>::is_always_lock_free,
+              "std::atomic<int> must be a plain int to serve as the cancel flag");
 
 struct Ray {
     RayEnd end = RayEnd::Unfinished;
@@ -100,6 +139,47 @@ public:
     Image traceLooks(const View& view, const std::vector<std::array<double, 3>>& looks,
                      const Options& opts = Options()) {
         return traceParams(view, looks.front().data(), static_cast<long long>(looks.size()), opts);
+    }
+
+    // eht::render: the view's size^2 pixels with the light; `cancel` (may be null) is read
+    // between slices of the image
+    EhtImage ehtRender(const EhtView& view, const Flow& flow, const std::atomic<int>* cancel = nullptr,
+                       Options opts = Options()) {
+        EhtImage img;
+        img.size = view.size;
+        img.halfWidth = view.half_width;
+        const size_t n = static_cast<size_t>(view.size) * view.size;
+        img.intensity.assign(n, 0.0);
+        img.shadow.assign(n, 0);
+        opts.cancel = reinterpret_cast<const volatile int*>(cancel);
+        kt_eht_out out{img.intensity.data(), img.shadow.data(), nullptr, nullptr, 0};
+        const int s = kt_eht_render(ctx_, &view, &flow, &opts, &out, nullptr);
+        img.cancelled = s == KT_ECANCELLED;
+        if (!img.cancelled) check(s);
+        return img;
+    }
+
+    // eht::traceRay with the light
+    EhtRay ehtTraceRay(const EhtView& view, const Flow& flow, double alpha, double beta,
+                       const Options& opts = Options()) {
+        const double ab[2] = {alpha, beta};
+        double intensity = 0.0, rmin = 0.0;
+        uint8_t shadow = 0;
+        int32_t steps = 0;
+        kt_eht_out out{&intensity, &shadow, &rmin, &steps, 0};
+        check(kt_eht_trace_rays(ctx_, &view, &flow, ab, 1, &opts, &out, nullptr));
+        return EhtRay{intensity, shadow != 0, rmin, steps};
+    }
+
+    // eht::calibrate: the density n0 for which a 48 x 48 render has targetJy janskys (the
+    // estimate so far when `cancel` turns nonzero)
+    double ehtCalibrate(const EhtView& view, const Flow& flow, double targetJy, double thetaGUas,
+                        const std::atomic<int>* cancel = nullptr, Options opts = Options()) {
+        opts.cancel = reinterpret_cast<const volatile int*>(cancel);
+        double n0 = flow.n0;
+        const int s = kt_eht_calibrate(ctx_, &view, &flow, targetJy, thetaGUas, &opts, &n0, nullptr);
+        if (s != KT_ECANCELLED) check(s);
+        return n0;
     }
 
     kt_stats stats() const {

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from black_simulation_amd import _lib as K
+from black_simulation_amd import api
 from oracle import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -65,6 +66,15 @@ def test_defaults():
     lib.kt_trace_opts_default(C.byref(o))
     assert (o.rtol, o.atol, o.max_steps, o.precision, o.mode) == (1e-10, 1e-12, 200000, K.PREC_FP64, K.MODE_AUTO)
     assert (o.gate_prob, o.gate_ring, o.restep_drift) == (0.999, 0.03, 1e-6)
+    assert o.cancel is None  # not cancellable
+    # the C struct is exactly the ctypes one: the default fills it and nothing past it
+    n = C.sizeof(K.KtTraceOpts)
+    buf = (C.c_ubyte * (n + 64))(*([0xAB] * (n + 64)))
+    lib.kt_trace_opts_default(C.cast(buf, C.POINTER(K.KtTraceOpts)))
+    assert bytes(buf[n:]) == b"\xab" * 64 and bytes(buf[n - 8:n]) == b"\0" * 8
+    flag = C.c_int(0)
+    assert api.TraceOpts(cancel=flag).to_c().cancel == C.addressof(flag)
+    assert api.TraceOpts().to_c().cancel is None
     v = K.KtView()
     lib.kt_view_default(C.byref(v))
     assert (v.width, v.spin, v.r_cam, v.capture_factor, v.escape_radius) == (256, 0.9, 30.0, 1.01, 1000.0)

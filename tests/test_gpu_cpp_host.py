@@ -41,3 +41,18 @@ def test_render_matches_oracle_counts():
     counts = np.bincount(want["cls"], minlength=4)
     assert [got["escaped"], got["captured"], got["disk"], got["unfinished"]] == counts.tolist()
     assert got["steps"] == int(want["steps"].sum())
+
+
+def test_eht_render_and_cancel_through_the_cpp_face():
+    """kt::Tracer::ehtRender (eht::render's shape): the image's total equals the Python
+    path's, and a cancel flag another thread sets before the render begins (the tracer's
+    creation takes longer than the thread's wait) leaves it untraced and cancelled."""
+    from black_simulation_amd import eht
+
+    got = run("eht", 0.9, 17.0, 16.0, 64)
+    assert got["cancelled"] is False and got["rows_with_light"] > 0
+    img = eht.render(eht.EhtView(0.9, math.radians(17.0), 16.0, 64))
+    want = float(img.intensity.sum().item())
+    assert got["intensity_sum"] == pytest.approx(want, rel=1e-12)
+    got = run("eht", 0.9, 17.0, 16.0, 64, 0)
+    assert got["cancelled"] is True and got["rows_with_light"] == 0 and got["intensity_sum"] == 0.0

@@ -172,3 +172,51 @@ def test_edge_on_image_is_mirror_symmetric_at_full_size():
     assert np.mean(shadow != shadow[::-1]) < 1e-3
     # not vacuous: the shadow and the lit flow both cover a good part of the image
     assert shadow.mean() > 0.05 and lit.mean() > 0.05, (shadow.mean(), lit.mean())
+
+
+def test_cancel_flag_stops_between_slices():
+    """eht::render's and eht::calibrate's cancel (src/eht.cpp:330-331, 446): a flag clear
+    throughout gives the uncancelled image bit for bit (slices trace the same rays); a flag
+    set before the call traces nothing (all zero, cancelled) and calibrate returns its
+    starting density; a flag set by another thread mid-render leaves whole slices of the
+    image either exactly the full render's or zero, the zero ones after the traced ones."""
+    import ctypes
+    import threading
+    import time
+
+    v = eht.EhtView(0.9, math.radians(60.0), 12.0, 768, 230e9, 6.5e9)
+    f = eht.Flow(n0=2e5)
+    full = host(eht.render(v, f, details=True))
+    flag = ctypes.c_int(0)
+    img = eht.render(v, f, TraceOpts(cancel=flag), details=True)
+    assert not img.cancelled
+    clear = host(img)
+    for k in full:
+        assert np.array_equal(clear[k], full[k]), k
+    flag.value = 1
+    img = eht.render(v, f, TraceOpts(cancel=flag), details=True)
+    assert img.cancelled
+    for k, x in host(img).items():
+        assert not x.any(), k
+    assert eht.calibrate(eht.EhtView(0.9, 1.0, 16.0, 48), f, 0.5, 3.8, TraceOpts(cancel=flag)) == 2e5
+    # from another thread, a few milliseconds in (a 768^2 render takes tens of ms)
+    flag.value = 0
+    setter = threading.Timer(0.004, lambda: setattr(flag, "value", 1))
+    setter.start()
+    t0 = time.perf_counter()
+    img = eht.render(v, f, TraceOpts(cancel=flag), details=True)
+    ms = 1e3 * (time.perf_counter() - t0)
+    setter.join()
+    got = host(img)
+    n = v.size * v.size
+    slice_ = max((n + 7) // 8, 1 << 17)
+    done = []
+    for s in range(0, n, slice_):
+        part = {k: x[s:s + slice_] for k, x in got.items()}
+        same = all(np.array_equal(part[k], full[k][s:s + slice_]) for k in part)
+        zero = all(not x.any() for x in part.values())
+        assert same or zero, s
+        done.append(same and not zero)
+    print(f"mid-render cancel after {ms:.1f} ms: cancelled {img.cancelled}, slices traced {done}", flush=True)
+    assert done == sorted(done, reverse=True)  # traced slices first
+    assert img.cancelled == (not all(done))
