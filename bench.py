@@ -41,6 +41,24 @@ OPS_PER_ATTEMPT = 522
 # 1 / (dt/dlambda)) = 240, stage sums 230, error estimate and max-norm 81, controller 30,
 # projection 35, and the right-hand side re-evaluated after each accepted step 40
 OPS_PER_PARTICLE_ATTEMPT = 656
+
+# HBM bytes per launch of each workload's dominant kernel, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+# (the factors a 1 GiB device copy reads) from rocprofv3 --pmc runs of the same command;
+# the summaries (commands included) are the named files under profiles/
+PROFILED_TRAFFIC = {
+    ("orbit", 1080, "k_mlp_fused"): (3.983e6 + 73.47e6, "profiles/k_mlp_fused_orbit_1080p.md"),
+    ("infall", 1080, "k_trace_persistent"): (0.374e6 + 149.1e6, "profiles/k_trace_persistent_plainks_infall_1080p.md"),
+    ("eht", 1024, "k_eht"): (0.3131e6 + 9.437e6, "profiles/k_eht_m87_1024.md"),
+    ("debris", 1 << 22, "k_advance"): (511.2e6 + 785.6e6, "profiles/k_advance_debris_4m.md"),
+}
+
+
+def profiled_traffic(workload, size, kernel):
+    """(bytes, source) of the profiled launch, or (None, None) for other workloads/sizes"""
+    for (w, n, k), v in PROFILED_TRAFFIC.items():
+        if w == workload and n == size and kernel.startswith(k):
+            return v
+    return None, None
 # The network 5-256-256-256-256-8: 2 (5*256 + 3*256*256 + 256*8) multiply-adds per ray
 MLP_FLOP_PER_RAY = 2 * (5 * 256 + 3 * 256 * 256 + 256 * 8)  # 399,872
 FP64_VECTOR_PEAK_TFLOPS = 78.6
@@ -280,8 +298,10 @@ def main():
     integ_peak = ops / ideal_s / 1e12 if ideal_s > 0 else FP64_VECTOR_PEAK_TFLOPS
     integ_ach = ops / (avg_integ * 1e-3) / 1e12 if avg_integ > 0 else 0.0
     kname = ["k_trace_persistent", "k_trace_oneshot"][st["mode"]]
+    tr_integ, src_integ = profiled_traffic(args.workload, args.height, kname)
     roof_integ = {"bound": "valu", "achieved": round(integ_ach, 3), "peak": round(integ_peak, 1), "unit": "TFLOP/s",
-                  "frac": round(integ_ach / integ_peak, 4), "traffic": None, "kernel": kname,
+                  "frac": round(integ_ach / integ_peak, 4), "traffic": tr_integ, "traffic_source": src_integ,
+                  "kernel": kname,
                   "kernel_ms": round(avg_integ, 3), "dp5_attempts": int(att), "fp64_attempts": int(att64),
                   "ops_per_attempt": OPS_PER_ATTEMPT,
                   "peak_is": "FP64 78.6 / FP32 157.3 TFLOP/s blended by the FP64 share of attempts"}
@@ -289,8 +309,10 @@ def main():
     roof_mlp = None
     if args.path == "surrogate":
         mlp_ach = rays * MLP_FLOP_PER_RAY / (avg_sur * 1e-3) / 1e12
+        tr_mlp, src_mlp = profiled_traffic(args.workload, args.height, "k_mlp_fused")
         roof_mlp = {"bound": "mfma", "achieved": round(mlp_ach, 2), "peak": BF16_MFMA_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(mlp_ach / BF16_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+                    "unit": "TFLOP/s", "frac": round(mlp_ach / BF16_MFMA_PEAK_TFLOPS, 4), "traffic": tr_mlp,
+                    "traffic_source": src_mlp,
                     "kernel": "k_mlp_fused (features, network, gate)", "kernel_ms": round(avg_sur, 3),
                     "flop_per_ray": MLP_FLOP_PER_RAY}
         if avg_sur > avg_integ:
@@ -431,7 +453,8 @@ def eht_bench(args, torch):
                    "captured_fraction": round(float(det.shadow.float().mean().item()), 4),
                    "mean_attempts_per_ray": round(attempts / rays, 1)},
         "roofline": {"bound": "valu", "achieved": round(ach, 3), "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(ach / FP64_VECTOR_PEAK_TFLOPS, 4), "traffic": None, "kernel": "k_eht",
+                     "frac": round(ach / FP64_VECTOR_PEAK_TFLOPS, 4), "traffic": profiled_traffic("eht", args.size, "k_eht")[0],
+                     "traffic_source": profiled_traffic("eht", args.size, "k_eht")[1], "kernel": "k_eht",
                      "kernel_ms": round(kernel_ms, 3), "dp5_attempts": attempts, "ops_per_attempt": OPS_PER_ATTEMPT,
                      "note": "counts the DP5 operations only; the transfer's exp/pow/Bessel work comes on top"},
         "cpu_baseline": None,
@@ -514,7 +537,9 @@ def debris_bench(args, torch):
                    "particles": n, "moving_after": moving,
                    "mean_attempts_per_update": round(attempts / max(1, n * args.steps), 2)},
         "roofline": {"bound": "valu", "achieved": round(ach, 3), "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(ach / FP64_VECTOR_PEAK_TFLOPS, 4), "traffic": None, "kernel": "k_advance",
+                     "frac": round(ach / FP64_VECTOR_PEAK_TFLOPS, 4),
+                     "traffic": profiled_traffic("debris", args.particles, "k_advance")[0],
+                     "traffic_source": profiled_traffic("debris", args.particles, "k_advance")[1], "kernel": "k_advance",
                      "kernel_ms": round(kernel_ms, 3), "dp5_attempts": attempts,
                      "ops_per_attempt": OPS_PER_PARTICLE_ATTEMPT,
                      "note": "attempts of all timed updates over their HIP-event time (persistent k_advance)"},
