@@ -144,16 +144,36 @@ struct PLane {
     bool rejected;
 };
 
-__device__ __forceinline__ void lane_start(PLane& s, const ParticleLaunch& p, long long q) {
-    body_init(s.b, p.a, p.E[q], p.L[q], p.Q[q]);
-    s.y[0] = p.r[q];
-    s.y[1] = p.mu[q];
-    s.y[2] = p.phi[q];
-    s.y[3] = p.vr[q];
-    s.y[4] = p.vmu[q];
+// the next particle of a lane, its columns loaded ahead of the lane's need for them
+struct Next {
+    long long q;
+    double r, mu, phi, vr, vmu, E, L, Q, h;
+    int fate;
+};
+
+__device__ __forceinline__ void next_load(Next& x, const ParticleLaunch& p, long long q) {
+    x.q = q;
+    x.fate = p.fate[q];
+    x.r = p.r[q];
+    x.mu = p.mu[q];
+    x.phi = p.phi[q];
+    x.vr = p.vr[q];
+    x.vmu = p.vmu[q];
+    x.E = p.E[q];
+    x.L = p.L[q];
+    x.Q = p.Q[q];
+    x.h = p.h ? p.h[q] : 0.0;
+}
+
+__device__ __forceinline__ void lane_start(PLane& s, const ParticleLaunch& p, const Next& x) {
+    body_init(s.b, p.a, x.E, x.L, x.Q);
+    s.y[0] = x.r;
+    s.y[1] = x.mu;
+    s.y[2] = x.phi;
+    s.y[3] = x.vr;
+    s.y[4] = x.vmu;
     s.left = p.dt;
-    const double h0 = p.h ? p.h[q] : 0.0;
-    s.h = h0 > 0.0 ? fmin(p.dt, h0) : fmin(p.dt, 0.01 * s.y[0]);
+    s.h = x.h > 0.0 ? fmin(p.dt, x.h) : fmin(p.dt, 0.01 * s.y[0]);
     s.hwant = s.h;
     s.steps = 0;
     s.rejected = false;
@@ -211,42 +231,64 @@ __global__ void k_zero_counter(unsigned long long* c) {
     if (threadIdx.x == 0) *c = 0ull;
 }
 
-// Persistent waves pulling particles from one counter: every kRefill attempts a wave hands
-// its finished lanes new particles (one atomic per wave), so the few particles that need
-// hundreds of steps near the horizon or the axis do not hold 63 idle lanes each.
+// Persistent waves pulling particles from one counter.  Every kRefill attempts the lanes
+// whose particle is done take the next one, so the few particles that need hundreds of
+// steps near the horizon or the axis do not hold 63 idle lanes each.  Each lane holds its
+// next particle's columns, loaded while it steps the current one, and a wave takes indices
+// 64 at a time (one atomic per 64 particles), so a refill waits on neither memory nor the
+// counter.
 constexpr int kRefill = 4;
+constexpr int kChunk = 64;
 
 __global__ __launch_bounds__(256) void k_advance(ParticleLaunch p, unsigned long long* counter) {
     const int lane = (int)(threadIdx.x & 63u);
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     PLane s;
-    bool active = false, drained = false;
+    Next nx;
+    bool active = false, has_next = false;
     long long my = 0;
+    long long cbase = 0;  // the wave's claimed indices not yet handed out: [cbase, cbase + cleft)
+    int cleft = 0;
+    bool exhausted = false;  // the counter has passed n
     for (;;) {
-        if (!drained) {
-            const unsigned long long idle = __ballot(!active);
-            const int nidle = __popcll(idle);
-            if (nidle > 0) {
-                unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(counter, (unsigned long long)nidle);
-                base = __shfl(base, 0);
-                if (base + (unsigned long long)nidle >= (unsigned long long)p.n) drained = true;
-                if (!active) {
-                    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-                    const long long q = (long long)(base + (unsigned long long)__popcll(idle & below));
-                    if (q < p.n) {
-                        my = q;
-                        if (p.fate[q] == kMoving) {
-                            lane_start(s, p, q);
-                            active = true;
-                        } else if (p.steps) {
-                            p.steps[q] = 0;
-                        }
-                    }
-                }
+        // lanes that are done take their prefetched particle
+        if (!active && has_next) {
+            has_next = false;
+            my = nx.q;
+            if (nx.fate == kMoving) {
+                lane_start(s, p, nx);
+                active = true;
+            } else if (p.steps) {
+                p.steps[my] = 0;
+            }
+        }
+        // lanes without a next particle get one from the wave's chunk, and its loads issue
+        const unsigned long long need = __ballot(!has_next);
+        const int nneed = __popcll(need);
+        if (nneed > 0 && (cleft > 0 || !exhausted)) {
+            const int rank = __popcll(need & below);
+            long long q = rank < cleft ? cbase + rank : -1;
+            if (nneed > cleft && !exhausted) {
+                unsigned long long nb = 0;
+                if (lane == 0) nb = atomicAdd(counter, (unsigned long long)kChunk);
+                nb = __shfl(nb, 0);
+                if (nb + kChunk >= (unsigned long long)p.n) exhausted = true;
+                const int from_new = nneed - cleft;
+                if (rank >= cleft) q = (long long)nb + (rank - cleft);
+                cbase = (long long)nb + from_new;
+                cleft = kChunk - from_new;
+            } else {
+                const int used = nneed < cleft ? nneed : cleft;
+                cbase += used;
+                cleft -= used;
+            }
+            if (!has_next && q >= 0 && q < p.n) {
+                next_load(nx, p, q);
+                has_next = true;
             }
         }
         if (__ballot(active) == 0ull) {
-            if (drained) break;
+            if (__ballot(has_next) == 0ull && exhausted && cleft == 0) break;
             continue;
         }
         for (int k = 0; k < kRefill; ++k) {
