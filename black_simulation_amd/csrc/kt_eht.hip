@@ -154,9 +154,17 @@ __device__ __forceinline__ bool transfer(const EhtLaunch& e, const Photon<double
     return depth > kMaxDepth;
 }
 
-__global__ __launch_bounds__(128) void k_eht(EhtLaunch e) {
-    const long long q = e.first + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= e.n) return;
+// One ray in one lane: started from its pixel (or list entry), stepped one DP5 attempt at
+// a time, written when done.
+struct ELane {
+    Photon<double> p;
+    double y[5], k0[5], h, intensity, depth, seen, rmin;
+    long long q;
+    int captured, steps;
+    bool rejected;
+};
+
+__device__ __forceinline__ void lane_start(ELane& L, const EhtLaunch& e, long long q) {
     double alpha, beta;
     if (e.rays) {
         alpha = e.rays[2 * q];
@@ -167,48 +175,118 @@ __global__ __launch_bounds__(128) void k_eht(EhtLaunch e) {
         alpha = -e.half_width + (col + 0.5) * e.px;
         beta = e.half_width - (row + 0.5) * e.px;
     }
-    Photon<double> p;
-    double y[5], yn[5], k[7][5];
-    bardeen_photon(e.a, kStart, 0.0, e.sinI, e.cosI, e.m0, alpha, beta, p, y);
-    double intensity = 0.0, depth = 0.0, seen = 1.0, rmin = y[0];
-    int captured = 0, steps = 0;
-    bool rejected = false;
-    double h = 0.01 / kStart;
-    rhs_main(p, y, k[0]);
-    while (steps < e.max_steps) {
-        const double err = dp5_attempt(p, false, 0.0, y, h, k, yn, e.rtol, e.atol);
-        ++steps;
-        if (!(err <= 1.0)) {
-            h *= step_factor(err, true);
-            rejected = true;
+    bardeen_photon(e.a, kStart, 0.0, e.sinI, e.cosI, e.m0, alpha, beta, L.p, L.y);
+    L.q = q;
+    L.intensity = 0.0;
+    L.depth = 0.0;
+    L.seen = 1.0;
+    L.rmin = L.y[0];
+    L.captured = 0;
+    L.steps = 0;
+    L.rejected = false;
+    L.h = 0.01 / kStart;
+    rhs_main(L.p, L.y, L.k0);
+}
+
+// one attempt; true once the ray is done (captured, back out, opaque or out of steps)
+__device__ __forceinline__ bool lane_step(ELane& L, const EhtLaunch& e) {
+    if (L.steps >= e.max_steps) return true;
+    double k[7][5], yn[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) k[0][i] = L.k0[i];
+    const double err = dp5_attempt(L.p, false, 0.0, L.y, L.h, k, yn, e.rtol, e.atol);
+    ++L.steps;
+    if (!(err <= 1.0)) {
+        L.h *= step_factor(err, true);
+        L.rejected = true;
+        return false;
+    }
+    const double fac = step_factor(err, L.rejected);
+    L.rejected = false;
+    if (L.p.L == 0.0 && L.y[1] * L.y[4] > 0.0 && yn[1] * yn[4] < 0.0) yn[2] += kPi;
+    project(L.p, yn);
+    // commit the step before the event tests (the previous point stays for the transfer's
+    // mean Sigma)
+    const double r_prev = L.y[0], mu_prev = L.y[1], h_used = L.h;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) L.y[i] = yn[i];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) L.k0[i] = k[6][i];
+    L.k0[0] = L.y[3];
+    L.k0[1] = L.y[4];
+    L.h *= fac;
+    if (!(fabs(L.y[0]) < 1e300) || !(fabs(L.y[1]) < 1e300) || L.y[0] < e.r_cap) {
+        L.captured = 1;
+        return true;
+    }
+    if (L.y[0] < L.rmin) L.rmin = L.y[0];
+    if (L.y[0] > kStart * 1.01 && L.y[3] > 0.0) return true;
+    return L.y[0] <= kLight && transfer(e, L.p, r_prev, mu_prev, L.y, h_used, L.intensity, L.depth, L.seen);
+}
+
+__device__ __forceinline__ void lane_write(const ELane& L, const EhtLaunch& e) {
+    if (e.intensity) e.intensity[L.q] = L.intensity;
+    if (e.shadow) e.shadow[L.q] = (uint8_t)L.captured;
+    if (e.min_radius) e.min_radius[L.q] = L.rmin;
+    if (e.steps) e.steps[L.q] = L.steps;
+}
+
+__global__ void k_zero_counter(unsigned long long* c) {
+    if (threadIdx.x == 0) *c = 0ull;
+}
+
+// Persistent waves over the rays [first, n): every kRefill attempts a wave's finished lanes
+// take new rays, handed out from 64 indices the wave claims per atomic.  Rays that the hole
+// captures end within tens of attempts while rays through the flow take hundreds, so lanes
+// of a one-ray-per-lane launch idled a fifth of the time.
+constexpr int kRefill = 8;
+constexpr int kChunk = 64;
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_eht(EhtLaunch e, unsigned long long* counter) {
+    const int lane = (int)(threadIdx.x & 63u);
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const long long total = e.n - e.first;
+    ELane L;
+    bool active = false;
+    long long cbase = 0;  // the wave's claimed offsets not yet handed out: [cbase, cbase + cleft)
+    int cleft = 0;
+    bool exhausted = false;
+    for (;;) {
+        const unsigned long long idle = __ballot(!active);
+        const int nidle = __popcll(idle);
+        if (nidle > 0 && (cleft > 0 || !exhausted)) {
+            const int rank = __popcll(idle & below);
+            long long q = rank < cleft ? cbase + rank : -1;
+            if (nidle > cleft && !exhausted) {
+                unsigned long long nb = 0;
+                if (lane == 0) nb = atomicAdd(counter, (unsigned long long)kChunk);
+                nb = __shfl(nb, 0);
+                if ((long long)nb + kChunk >= total) exhausted = true;
+                const int from_new = nidle - cleft;
+                if (rank >= cleft) q = (long long)nb + (rank - cleft);
+                cbase = (long long)nb + from_new;
+                cleft = kChunk - from_new;
+            } else {
+                const int used = nidle < cleft ? nidle : cleft;
+                cbase += used;
+                cleft -= used;
+            }
+            if (!active && q >= 0 && q < total) {
+                lane_start(L, e, e.first + q);
+                active = true;
+            }
+        }
+        if (__ballot(active) == 0ull) {
+            if (exhausted && cleft == 0) break;
             continue;
         }
-        const double fac = step_factor(err, rejected);
-        rejected = false;
-        if (p.L == 0.0 && y[1] * y[4] > 0.0 && yn[1] * yn[4] < 0.0) yn[2] += kPi;
-        project(p, yn);
-        // commit the step before the event tests (the previous point stays for the
-        // transfer's mean Sigma)
-        const double r_prev = y[0], mu_prev = y[1], h_used = h;
-#pragma unroll
-        for (int i = 0; i < 5; ++i) y[i] = yn[i];
-#pragma unroll
-        for (int i = 0; i < 5; ++i) k[0][i] = k[6][i];
-        k[0][0] = y[3];
-        k[0][1] = y[4];
-        h *= fac;
-        if (!(fabs(y[0]) < 1e300) || !(fabs(y[1]) < 1e300) || y[0] < e.r_cap) {
-            captured = 1;
-            break;
+        for (int k = 0; k < kRefill; ++k) {
+            if (active && lane_step(L, e)) {
+                lane_write(L, e);
+                active = false;
+            }
         }
-        if (y[0] < rmin) rmin = y[0];
-        if (y[0] > kStart * 1.01 && y[3] > 0.0) break;
-        if (y[0] <= kLight && transfer(e, p, r_prev, mu_prev, y, h_used, intensity, depth, seen)) break;
     }
-    if (e.intensity) e.intensity[q] = intensity;
-    if (e.shadow) e.shadow[q] = (uint8_t)captured;
-    if (e.min_radius) e.min_radius[q] = rmin;
-    if (e.steps) e.steps[q] = steps;
 }
 
 // sum of the intensities (for the flux) in a fixed order, so that it does not change from
@@ -244,9 +322,12 @@ __global__ __launch_bounds__(kSumBlock) void k_sum_final(const double* partial, 
 
 }  // namespace
 
-hipError_t render(const EhtLaunch& e, hipStream_t st) {
+hipError_t render(const EhtLaunch& e, unsigned long long* counter, int grid, hipStream_t st) {
     if (e.n <= e.first) return hipSuccess;
-    hipLaunchKernelGGL(k_eht, dim3((unsigned)((e.n - e.first + 127) / 128)), dim3(128), 0, st, e);
+    const long long need = (e.n - e.first + 255) / 256;  // no more workgroups than 256-ray groups
+    const int blocks = (int)(need < grid ? need : grid);
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, counter);
+    hipLaunchKernelGGL(k_eht, dim3((unsigned)blocks), dim3(256), 0, st, e, counter);
     return hipGetLastError();
 }
 

@@ -34,7 +34,9 @@ struct EhtLaunch {
     int32_t* steps;      // optional
 };
 
-hipError_t render(const EhtLaunch& e, hipStream_t st);
+// persistent waves over rays [first, n): counter is one device word (zeroed here by a
+// kernel), grid the most workgroups of 256 threads to launch
+hipError_t render(const EhtLaunch& e, unsigned long long* counter, int grid, hipStream_t st);
 // *out = sum of x[0..n) (device), in a fixed order; partial holds kSumPartials doubles
 hipError_t sum(const double* x, long long n, double* partial, double* out, hipStream_t st);
 

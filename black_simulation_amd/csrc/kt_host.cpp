@@ -1049,12 +1049,15 @@ int ensure_eht(kt_ctx* ctx) {
     return KT_OK;
 }
 
+// two workgroups of 256 per CU: the kernel's 2 waves per SIMD
+int eht_grid(const kt_ctx* ctx) { return std::max(1, ctx->cu_count) * 2; }
+
 // the launch itself: one kernel, or slice after slice while a cancel flag stays clear
 // (eht::render polls its flag per row, src/eht.cpp:330-331).  Rays of slices not traced
 // are zeroed.  A slice is an eighth of the rays, at least 2^17 (two waves per SIMD).
 int eht_launch(kt_ctx* ctx, kte::EhtLaunch e, const volatile int* cancel, hipStream_t stream) {
     if (!cancel) {
-        hipError_t err = kte::render(e, stream);
+        hipError_t err = kte::render(e, ctx->d_counter, eht_grid(ctx), stream);
         return err == hipSuccess ? KT_OK : hip_fail(ctx, err, "EHT kernel launch");
     }
     const long long n = e.n, slice = std::max<long long>((n + 7) / 8, 1LL << 17);
@@ -1069,7 +1072,7 @@ int eht_launch(kt_ctx* ctx, kte::EhtLaunch e, const volatile int* cancel, hipStr
         }
         e.first = first;
         e.n = std::min(n, first + slice);
-        hipError_t err = kte::render(e, stream);
+        hipError_t err = kte::render(e, ctx->d_counter, eht_grid(ctx), stream);
         if (err != hipSuccess) return hip_fail(ctx, err, "EHT kernel launch");
         KT_HIP(hipStreamSynchronize(stream));
     }
