@@ -400,6 +400,49 @@ def eht_cpu_baseline(view, flow, seconds_target: float = 15.0) -> dict:
                       f"FP64 DP5 rtol 1e-10 + transfer, {cores} threads)"}
 
 
+def eht_cpu_reference(view, flow, seconds_target: float = 10.0):
+    """The reference's own eht::traceRay with the light (src/eht.cpp:260-319: RK4 with its
+    step rule), compiled from its sources into oracle/_ref/libkerr_ref.so, on the same
+    pixels from a thread pool as eht::render does (src/eht.cpp:321-347); None when that
+    library was not built."""
+    import ctypes as C
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle as O
+
+    if not os.path.exists(O.REF_LIB_PATH):
+        return None
+    lib = C.CDLL(O.REF_LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    lib.ref_eht_trace_light.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, dp, C.c_double, C.c_double, dp]
+    lib.ref_eht_trace_light.restype = None
+    fl = (C.c_double * 6)(flow.n0, flow.T0, flow.beta, flow.keplerian, flow.height, flow.sin_theta_b)
+    cores = os.cpu_count() or 1
+    n = view.size * view.size
+    px = view.pixel()
+
+    def one(q):
+        out = (C.c_double * 3)()
+        row, col = divmod(q, view.size)
+        lib.ref_eht_trace_light(view.a, view.inclination, view.frequency, view.mass_solar, fl,
+                                -view.half_width + (col + 0.5) * px, view.half_width - (row + 0.5) * px, out)
+        return out[0]
+
+    def run(stride):
+        idx = list(range(stride // 2, n, stride))
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(cores) as ex:
+            list(ex.map(one, idx, chunksize=16))
+        return len(idx), time.perf_counter() - t0
+
+    k, dt = run(max(1, n // (64 * cores)))
+    stride = max(1, int(n / max(1.0, k / dt * seconds_target)))
+    k, dt = run(stride)
+    return {"value": round(k / dt / 1e6, 5), "unit": "Mrays/s", "cores": cores, "kind": "reference",
+            "sample": f"every {stride}th pixel of the {view.size}x{view.size} image ({k} rays, {dt:.1f} s): the "
+                      f"reference's eht::traceRay with the light (its RK4), {cores} threads"}
+
+
 def eht_bench(args, torch):
     """--workload eht: M87* at 230 GHz (a = 0.9 seen from 163 deg, 32 M field, the flow's
     density calibrated to 0.5 Jy), size x size pixels per step, FP64 throughout."""
@@ -462,6 +505,7 @@ def eht_bench(args, torch):
     tr.close()
     if not args.no_cpu:
         rec["cpu_baseline"] = eht_cpu_baseline(view, flow)
+        rec["cpu_reference"] = eht_cpu_reference(view, flow)
     print(json.dumps(rec), flush=True)
 
 
