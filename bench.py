@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -92,6 +93,82 @@ def cpu_baseline(view, seconds_target: float = 15.0) -> dict:
     return {"value": round(n / dt / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
             "sample": f"every {stride}th pixel of one {view.width}x{view.height} frame of the workload ({n} rays, "
                       f"{dt:.1f} s, FP64 DP5 rtol 1e-10, {cores} threads)"}
+
+
+def _ref_lib():
+    """oracle/_ref/libkerr_ref.so: the reference's own sources compiled as they are (with
+    oracle/ref_shim.cpp), or None where it was not built"""
+    import ctypes as C
+
+    from oracle import oracle as O
+
+    return C.CDLL(O.REF_LIB_PATH) if os.path.exists(O.REF_LIB_PATH) else None
+
+
+def _timed_pool(one, n, seconds_target):
+    """one(q) over every stride-th q < n from a thread pool of the host's cores (ctypes
+    releases the GIL), the stride sized for ~seconds_target; (rays, seconds, stride, cores)"""
+    from concurrent.futures import ThreadPoolExecutor
+
+    cores = os.cpu_count() or 1
+
+    def run(stride):
+        idx = list(range(stride // 2, n, stride))
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(cores) as ex:
+            list(ex.map(one, idx, chunksize=16))
+        return len(idx), time.perf_counter() - t0
+
+    k, dt = run(max(1, n // (64 * cores)))
+    stride = max(1, int(n / max(1.0, k / dt * seconds_target)))
+    k, dt = run(stride)
+    return k, dt, stride, cores
+
+
+def infall_cpu_reference(view, seconds_target: float = 10.0):
+    """The reference's own trajectory::traceRay (src/trajectory.cpp:214-396, its RK4 and
+    step rule) from the falling observer's frame (kerr::observerFrame), on the pixels' look
+    directions (camera.cpp:6-21 basis, trace.frag:1494-1496 pixel mapping) of the same
+    frame; None when oracle/_ref was not built."""
+    import ctypes as C
+
+    import numpy as np
+
+    lib = _ref_lib()
+    if lib is None:
+        return None
+    dp = C.POINTER(C.c_double)
+    lib.ref_trajectory_trace_u.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, C.c_int,
+                                           C.c_double, C.c_double, C.c_double, dp]
+    lib.ref_trajectory_trace_u.restype = None
+    cy, sy, cp, sp, cr, sr = (math.cos(view.yaw), math.sin(view.yaw), math.cos(view.pitch), math.sin(view.pitch),
+                              math.cos(view.roll), math.sin(view.roll))
+    f0, u0, r0 = np.array([-1.0, 0.0, 0.0]), np.array([0.0, -1.0, 0.0]), np.array([0.0, 0.0, 1.0])
+    yawed, right = cy * f0 + sy * r0, cy * r0 - sy * f0
+    fwd = cp * yawed + sp * u0
+    up = np.cross(right, fwd)
+    B = (cr * right + sr * up, cr * up - sr * right, fwd)
+    tanh_fov = math.tan(0.5 * view.fov_deg * math.pi / 180.0)
+    W, H = view.width, view.height
+    u = (C.c_double * 4)(*view.u_obs)
+    r_in = view.disk_inner if view.disk_inner > 0 else __import__("black_simulation_amd").isco_radius(view.spin, True)
+
+    def one(q):
+        row, col = divmod(q, W)
+        x = ((col + 0.5) - 0.5 * W) / (0.5 * H) * tanh_fov
+        y = (((H - row) - 0.5) - 0.5 * H) / (0.5 * H) * tanh_fov
+        d = x * B[0] + y * B[1] + B[2]
+        d /= np.linalg.norm(d)
+        look = (C.c_double * 3)(*d)
+        out = (C.c_double * 11)()
+        lib.ref_trajectory_trace_u(view.spin, view.r_cam, view.theta_cam, view.phi_cam, u, look,
+                                   1 if view.disk_enabled else 0, r_in, view.disk_outer, view.escape_radius, out)
+        return out[0]
+
+    k, dt, stride, cores = _timed_pool(one, W * H, seconds_target)
+    return {"value": round(k / dt / 1e6, 5), "unit": "Mrays/s", "cores": cores, "kind": "reference",
+            "sample": f"every {stride}th pixel of one {W}x{H} frame ({k} rays, {dt:.1f} s): the reference's "
+                      f"trajectory::traceRay from the observer's frame (its RK4), {cores} threads"}
 
 
 def percentile(xs, q):
@@ -364,6 +441,8 @@ def main():
     tr.close()
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(frame_views[frames[len(frames) // 2]][0])
+        if args.workload == "infall":
+            rec["cpu_reference"] = infall_cpu_reference(frame_views[frames[len(frames) // 2]][0])
     print(json.dumps(rec), flush=True)
 
 
@@ -590,7 +669,9 @@ def debris_bench(args, torch):
         "cpu_baseline": None,
     }
     if not args.no_cpu:
-        rec["cpu_baseline"] = debris_cpu_baseline(a, {c: start[c][: 1 << 16].cpu().numpy() for c in COLUMNS}, dt)
+        sample = {c: start[c][: 1 << 16].cpu().numpy() for c in COLUMNS}
+        rec["cpu_baseline"] = debris_cpu_baseline(a, sample, dt)
+        rec["cpu_reference"] = debris_cpu_reference(a, sample, dt)
     tr.close()
     print(json.dumps(rec), flush=True)
 
@@ -622,6 +703,35 @@ def debris_cpu_baseline(a, cols, dt, seconds_target: float = 10.0) -> dict:
     t = run(k)
     return {"value": round(k / t / 1e6, 5), "unit": "Mparticles/s", "cores": cores, "kind": "port",
             "sample": f"the first {k} particles of the workload, one update of dt={dt} M each ({t:.1f} s, {cores} threads)"}
+
+
+def debris_cpu_reference(a, cols, dt, seconds_target: float = 10.0):
+    """The reference's own trajectory::predict (src/trajectory.cpp:121-172: RK4 in
+    coordinate time) over dt for each of the workload's first particles, from a thread
+    pool; None when oracle/_ref was not built."""
+    import ctypes as C
+
+    lib = _ref_lib()
+    if lib is None:
+        return None
+    dp = C.POINTER(C.c_double)
+    lib.ref_predict.argtypes = [C.c_double, dp, dp, C.c_double, C.c_double, dp]
+    lib.ref_predict.restype = None
+
+    def one(i):
+        mu = min(1.0, max(-1.0, float(cols["mu"][i])))
+        th = math.acos(mu)
+        sn = max(math.sin(th), 1e-300)
+        consts = (C.c_double * 4)(cols["E"][i], cols["L"][i], cols["Q"][i], 1.0)
+        state = (C.c_double * 6)(0.0, cols["r"][i], th, cols["phi"][i], cols["vr"][i], -cols["vmu"][i] / sn)
+        out = (C.c_double * 9)()
+        lib.ref_predict(a, consts, state, dt, 1e4, out)
+        return out[0]
+
+    k, t, stride, cores = _timed_pool(one, len(cols["r"]), seconds_target)
+    return {"value": round(k / t / 1e6, 5), "unit": "Mparticles/s", "cores": cores, "kind": "reference",
+            "sample": f"every {stride}th of the first {len(cols['r'])} particles ({k}), one update of dt={dt} M each "
+                      f"({t:.1f} s): the reference's trajectory::predict (its RK4), {cores} threads"}
 
 
 if __name__ == "__main__":
