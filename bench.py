@@ -77,7 +77,7 @@ def cpu_baseline(view, seconds_target: float = 15.0) -> dict:
     ov = O.make_view(**{k: getattr(view, k) for k in (
         "camera_kind", "width", "height", "spin", "r_cam", "theta_cam", "phi_cam", "half_width", "half_height",
         "yaw", "pitch", "roll", "fov_deg", "disk_inner", "disk_outer", "disk_prograde", "disk_enabled",
-        "escape_radius", "capture_factor", "u_obs", "drift")})
+        "escape_radius", "capture_factor", "u_obs", "drift", "frame")})
     opts = O.make_opts()
     stride = 4099
     t0 = time.perf_counter()

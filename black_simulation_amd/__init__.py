@@ -4,7 +4,7 @@ The product is libkerr_trace.so, built in-tree from csrc/ (see build.py); this
 package loads it with ctypes.  Public names mirror the reference's entry points.
 """
 from .api import (TraceOpts, Tracer, View, blackbody_table, circular_omega, critical_curve, critical_radius,
-                  geodesic_four_velocity, horizon_radius, isco_radius, photon_orbit_radius, render, shadow_radius, to_numpy, trace_looks,
+                  geodesic_four_velocity, horizon_radius, observer_frame, isco_radius, photon_orbit_radius, render, shadow_radius, to_numpy, trace_looks,
                   trace_ray)
 from ._lib import (CAMERA_BARDEEN, CAMERA_OBSERVER, CAMERA_PINHOLE, CAPTURED, DISK, ESCAPED, FLAG_FP64_RESTEP, FLAG_SURROGATE,
                    MODE_AUTO, MODE_ONESHOT, MODE_PERSISTENT, PARTICLE_ESCAPED, PARTICLE_MOVING, PARTICLE_PLUNGED,
@@ -13,7 +13,7 @@ from ._lib import (CAMERA_BARDEEN, CAMERA_OBSERVER, CAMERA_PINHOLE, CAPTURED, DI
 
 __all__ = [
     "TraceOpts", "Tracer", "View", "blackbody_table", "circular_omega", "critical_curve", "critical_radius",
-    "geodesic_four_velocity", "horizon_radius", "isco_radius", "photon_orbit_radius", "render", "shadow_radius", "to_numpy", "trace_looks",
+    "geodesic_four_velocity", "horizon_radius", "observer_frame", "isco_radius", "photon_orbit_radius", "render", "shadow_radius", "to_numpy", "trace_looks",
     "trace_ray",
     "CAMERA_BARDEEN", "CAMERA_OBSERVER", "CAMERA_PINHOLE", "CAPTURED", "DISK", "ESCAPED", "FLAG_FP64_RESTEP", "FLAG_SURROGATE",
     "MODE_AUTO", "MODE_ONESHOT", "MODE_PERSISTENT", "PARTICLE_ESCAPED", "PARTICLE_MOVING", "PARTICLE_PLUNGED",

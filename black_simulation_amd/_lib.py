@@ -43,6 +43,7 @@ class KtView(C.Structure):
         ("capture_factor", C.c_double),
         ("u_obs", C.c_double * 4),
         ("drift", C.c_double * 3),
+        ("frame", C.c_double * 16),
     ]
 
 
@@ -206,6 +207,7 @@ SIGNATURES = {
     "kt_eht_calibrate": (C.c_int, [C.c_void_p, _P(KtEhtView), _P(KtEhtFlow), C.c_double, C.c_double,
                                    _P(KtTraceOpts), _P(C.c_double), C.c_void_p]),
     "kt_eht_angular_scale": (C.c_double, [C.c_double, C.c_double]),
+    "kt_observer_frame": (C.c_int, [C.c_double, C.c_double, C.c_double, _P(C.c_double), _P(C.c_double)]),
     "kt_geodesic_four_velocity": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                             C.c_int, C.c_int, _P(C.c_double)]),
     "kt_particles_advance": (C.c_int, [C.c_void_p, C.c_double, _P(KtParticles), C.c_longlong, C.c_double,

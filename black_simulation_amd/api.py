@@ -31,7 +31,9 @@ class View:
     simulator's Camera (src/camera.h:20-29) in the ZAMO frame; observer: that camera
     carried by the observer of 4-velocity u_obs (Kerr-Schild, e.g. from
     geodesic_four_velocity; inside the horizon too), or by the ZAMO boosted by `drift`
-    (Camera::frame, src/camera.cpp:23-40)."""
+    (Camera::frame, src/camera.cpp:23-40), or given its frame outright: `frame`, 16
+    values e_a^mu (rows: 4-velocity, r, theta, phi axes; Kerr-Schild), as uCamPhoton
+    carries Camera::frame (src/renderer.cpp:345-357); observer_frame builds one."""
 
     camera_kind: int = K.CAMERA_BARDEEN
     width: int = 256
@@ -54,6 +56,7 @@ class View:
     capture_factor: float = 1.01
     u_obs: tuple = (0.0, 0.0, 0.0, 0.0)
     drift: tuple = (0.0, 0.0, 0.0)
+    frame: tuple = (0.0,) * 16
 
     def to_c(self) -> K.KtView:
         v = K.KtView()
@@ -63,6 +66,8 @@ class View:
                 val = (C.c_double * 4)(*[float(x) for x in val])
             elif f.name == "drift":
                 val = (C.c_double * 3)(*[float(x) for x in val])
+            elif f.name == "frame":
+                val = (C.c_double * 16)(*[float(x) for x in np.ravel(val)])
             setattr(v, f.name, val)
         return v
 
@@ -399,6 +404,18 @@ def blackbody_table(n: int = 1024) -> np.ndarray:
     if rc != K.KT_OK:
         raise RuntimeError(f"kt_blackbody_table failed with status {rc}")
     return t.reshape(n, 3)
+
+
+def observer_frame(a: float, r: float, theta: float, u) -> np.ndarray:
+    """kerr::observerFrame (src/kerr.cpp:432-456) of the observer of 4-velocity u
+    (Kerr-Schild) at (r, theta): 4 x 4, rows e_0 = u normalised, then the r, theta, phi axes."""
+    uu = np.ascontiguousarray(u, np.float64)
+    e = np.zeros(16)
+    rc = K.lib().kt_observer_frame(a, r, theta, uu.ctypes.data_as(C.POINTER(C.c_double)),
+                                   e.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != K.KT_OK:
+        raise ValueError("observer_frame: u must be a future-directed timelike vector")
+    return e.reshape(4, 4)
 
 
 def geodesic_four_velocity(a: float, E: float, L: float, Q: float, r: float, theta: float, radial_sign: int = -1,

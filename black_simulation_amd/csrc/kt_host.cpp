@@ -144,6 +144,27 @@ void camera_basis(double yaw, double pitch, double roll, double B[9]) {
 
 bool has_u_obs(const kt_view& v) { return v.u_obs[0] != 0.0 || v.u_obs[1] != 0.0 || v.u_obs[2] != 0.0 || v.u_obs[3] != 0.0; }
 
+bool has_frame(const kt_view& v) {
+    for (double x : v.frame)
+        if (x != 0.0) return true;
+    return false;
+}
+
+// a caller's observer frame: orthonormal (g(e_a, e_b) = diag(-1, 1, 1, 1) to 1e-8) and
+// future-directed (e_0^t > 0)
+bool frame_ok(const kt_view& v) {
+    double g[4][4];
+    metric_tensor(v.spin, v.r_cam, v.theta_cam, g);
+    const double* e = v.frame;
+    if (!(e[0] > 0.0)) return false;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            const double d = dot4(g, e + 4 * i, e + 4 * j) - (i != j ? 0.0 : i == 0 ? -1.0 : 1.0);
+            if (!(std::fabs(d) <= 1e-8)) return false;
+        }
+    return true;
+}
+
 // the observer of an observer view: u_obs normalised, or the ZAMO boosted by drift
 // (src/camera.cpp:23-36); false if it is not a future-directed timelike vector
 bool observer_u(const kt_view& v, double u[4]) {
@@ -180,6 +201,14 @@ const char* check_view(const kt_view& v) {
     if (v.camera_kind == KT_CAMERA_OBSERVER) {
         if (!(v.fov_deg > 0.0 && v.fov_deg < 180.0)) return "fov out of range";
         if (!(v.theta_cam > 0.0 && v.theta_cam < kPi)) return "an observer camera must be off the axis";
+        if (has_frame(v)) {
+            if (!(v.r_cam > 1.0 - std::sqrt(std::fmax(0.0, 1.0 - v.spin * v.spin)) + 1e-9))
+                return "an observer camera must be outside the inner horizon";
+            if (!frame_ok(v)) return "frame must be orthonormal (to 1e-8) with a future-directed e_0";
+            if (!(v.capture_factor >= 1.0)) return "capture_factor must be >= 1";
+            if (v.disk_enabled && !(v.disk_outer > 0.0)) return "disk_outer must be positive";
+            return nullptr;
+        }
         if (has_u_obs(v)) {
             if (!(v.r_cam > 1.0 - std::sqrt(std::fmax(0.0, 1.0 - v.spin * v.spin)) + 1e-9))
                 return "an observer camera must be outside the inner horizon";
@@ -237,9 +266,14 @@ ViewDev make_view_dev(const kt_view& v, long long offset) {
     if (v.camera_kind == KT_CAMERA_PINHOLE || v.camera_kind == KT_CAMERA_OBSERVER) {
         const double r = v.r_cam, t = v.theta_cam;
         double u[4], e[4][4], g[4][4], low[4][4];
-        if (v.camera_kind == KT_CAMERA_OBSERVER) observer_u(v, u);
-        else zamo_u(a, r, t, u);
-        observer_frame(a, r, t, u, e);
+        if (v.camera_kind == KT_CAMERA_OBSERVER && has_frame(v)) {  // as given (uCamPhoton)
+            for (int j = 0; j < 4; ++j)
+                for (int i = 0; i < 4; ++i) e[j][i] = v.frame[4 * j + i];
+        } else {
+            if (v.camera_kind == KT_CAMERA_OBSERVER) observer_u(v, u);
+            else zamo_u(a, r, t, u);
+            observer_frame(a, r, t, u, e);
+        }
         metric_tensor(a, r, t, g);
         for (int j = 0; j < 4; ++j) lower4(g, e[j], low[j]);
         for (int j = 0; j < 4; ++j) {
@@ -1394,6 +1428,22 @@ double kt_surrogate_table_radius(kt_ctx* ctx, double a, double inclination, doub
 double kt_surrogate_shadow_radius(kt_ctx* ctx, double a, double inclination) {
     if (!ctx || ensure_surrogate(ctx) != KT_OK) return 0.0;
     return kts::table_shadow_radius(ctx->surrogate, a, inclination);
+}
+
+int kt_observer_frame(double a, double r, double theta, const double u[4], double e[16]) {
+    if (!u || !e || !(std::fabs(a) <= 1.0) || !(r > 0.0)) return KT_EINVAL;
+    double g[4][4], w[4];
+    metric_tensor(a, r, theta, g);
+    for (int i = 0; i < 4; ++i) w[i] = u[i];
+    const double n = dot4(g, w, w);
+    if (!(n < 0.0) || !(w[0] > 0.0)) return KT_EINVAL;
+    const double s = 1.0 / std::sqrt(-n);
+    for (int i = 0; i < 4; ++i) w[i] *= s;
+    double f[4][4];
+    observer_frame(a, r, theta, w, f);
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i) e[4 * j + i] = f[j][i];
+    return KT_OK;
 }
 
 int kt_geodesic_four_velocity(double a, double E, double L, double Q, double r, double theta, int radial_sign,

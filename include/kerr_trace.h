@@ -102,6 +102,12 @@ typedef struct kt_view {
     double capture_factor;          /* captured when r < capture_factor * r_+ (e.g. 1.01) */
     double u_obs[4];                /* KT_CAMERA_OBSERVER: u^t, u^r, u^theta, u^phi (KS)  */
     double drift[3];                /* KT_CAMERA_OBSERVER with u_obs = 0: ZAMO boost      */
+    /* KT_CAMERA_OBSERVER: the observer's frame itself (as uCamPhoton carries Camera::frame,
+     * src/renderer.cpp:345-357), e_a^mu at 4 a + mu: row 0 the 4-velocity, rows 1-3 the r,
+     * theta and phi axes of kerr::Frame (kerr.h), mu = (t, r, theta, phi) ingoing Kerr-Schild.
+     * It must be orthonormal (g(e_a, e_b) = diag(-1, 1, 1, 1) to 1e-8) and future-directed.
+     * All zero: the frame is built from u_obs or drift (kt_observer_frame). */
+    double frame[16];
 } kt_view;
 
 typedef struct kt_trace_opts {
@@ -360,6 +366,10 @@ int kt_particles_advance(kt_ctx* ctx, double a, const kt_particles* particles, l
  * out = {E, L, Q, vr, vmu}. */
 int kt_particle_from_zamo(double a, double r, double theta, const double v[3], double out[5]);
 
+/* kerr::observerFrame (src/kerr.cpp:432-456): the orthonormal frame of the observer of
+ * 4-velocity u (Kerr-Schild) at (r, theta), rows as kt_view.frame; u is normalised first.
+ * KT_EINVAL unless u is future-directed and timelike. */
+int kt_observer_frame(double a, double r, double theta, const double u[4], double e[16]);
 /* The 4-velocity (ingoing Kerr-Schild, contravariant) of a free-falling massive observer
  * with constants E, L, Q at (r, theta), moving outward in r when radial_sign > 0 and
  * toward larger theta when polar_sign > 0: kerr::fourVelocity (src/kerr.cpp:414-418) with

@@ -777,9 +777,17 @@ void ko_observer_u(const ko_view* v, double u[4]) {
 static void pinhole_photon(const ko_view* v, double n0, double n1, double n2, ko_photon* p, double y0[5]) {
     const double a = v->spin, r = v->r_cam, th = v->theta_cam;
     double u[4], e[4][4], g[4][4], low[4][4];
-    if (v->camera_kind == KO_CAMERA_OBSERVER) ko_observer_u(v, u);
-    else ko_zamo_four_velocity(a, r, th, u);
-    ko_observer_frame(a, r, th, u, e);
+    int given = 0;
+    if (v->camera_kind == KO_CAMERA_OBSERVER)
+        for (int k = 0; k < 16; ++k) given |= v->frame[k] != 0.0;
+    if (given) { /* the caller's frame, as uCamPhoton carries Camera::frame (renderer.cpp:345-357) */
+        for (int j = 0; j < 4; ++j)
+            for (int i = 0; i < 4; ++i) e[j][i] = v->frame[4 * j + i];
+    } else {
+        if (v->camera_kind == KO_CAMERA_OBSERVER) ko_observer_u(v, u);
+        else ko_zamo_four_velocity(a, r, th, u);
+        ko_observer_frame(a, r, th, u, e);
+    }
     ko_metric_tensor(a, r, th, g);
     for (int j = 0; j < 4; ++j) lower4(g, e[j], low[j]);
     const double sn = ko_sin(th), cs = ko_cos(th);

@@ -108,6 +108,7 @@ typedef struct {
     double capture_factor;                   /* captured when r < this * r_+       */
     double u_obs[4];                         /* observer: KS 4-velocity, or zero   */
     double drift[3];                         /* observer, u_obs zero: ZAMO boost   */
+    double frame[16];                        /* observer: e_a^mu given, or zero    */
 } ko_view;
 
 typedef struct {

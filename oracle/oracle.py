@@ -45,6 +45,7 @@ class View(C.Structure):
         ("capture_factor", C.c_double),
         ("u_obs", C.c_double * 4),
         ("drift", C.c_double * 3),
+        ("frame", C.c_double * 16),
     ]
 
 
@@ -188,6 +189,8 @@ def make_view(**kw) -> View:
             val = (C.c_double * 4)(*[float(x) for x in val])
         elif k == "drift":
             val = (C.c_double * 3)(*[float(x) for x in val])
+        elif k == "frame":
+            val = (C.c_double * 16)(*[float(x) for x in np.ravel(val)])
         setattr(v, k, val)
     return v
 

@@ -212,6 +212,13 @@ void ref_four_velocity(double a, const double* consts, const double* state, doub
     for (int i = 0; i < 4; ++i) u[i] = v[i];
 }
 
+// kerr::observerFrame of the observer of 4-velocity u (Kerr-Schild): e[4 k + i] = e_k^i
+void ref_observer_frame(double a, double r, double theta, const double* u, double* e) {
+    kerr::Frame f = kerr::observerFrame(a, r, theta, {u[0], u[1], u[2], u[3]});
+    for (int k = 0; k < 4; ++k)
+        for (int i = 0; i < 4; ++i) e[4 * k + i] = f.e[k][i];
+}
+
 // trajectory::traceRay from the frame of the observer of 4-velocity u (Kerr-Schild);
 // out as ref_trajectory_trace
 void ref_trajectory_trace_u(double a, double r, double theta, double phi, const double* u, const double* look,

@@ -9,7 +9,8 @@ oracle/ref_shim.cpp.  The fixture holds inputs and the reference's outputs only:
   - kerr::fourVelocity for massive geodesic states, outside and inside the horizon;
   - kerr::derivatives (Kerr-Schild dphi) for photon states inside and near the horizon;
   - trajectory::traceRay from boosted and falling observers' frames (RK4 with its own
-    step rule: class, the sky chord and the launch constants E, L/E).
+    step rule: class, the sky chord and the launch constants E, L/E);
+  - kerr::observerFrame of those observers (the frames themselves).
 
     python tests/golden/make_observer_golden.py
 """
@@ -41,6 +42,7 @@ def load_ref():
         ("ref_polar_potential", d, [d, P, d]),
         ("ref_four_velocity", None, [d, P, P, P]),
         ("ref_trajectory_trace_u", None, [d, d, d, d, P, P, i, d, d, d, P]),
+        ("ref_observer_frame", None, [d, d, d, P, P]),
     ]:
         f = getattr(L, name)
         f.restype = res
@@ -151,6 +153,14 @@ def main():
                                      o.ctypes.data_as(C.POINTER(C.c_double)))
             cases.append([a_, r_, th_, *u, *lk, disk, rin, rout, 1000.0])
             res.append(o)
+    frames_in, frames = [], []
+    for a_, r_, th_, u, disk, rin, rout in observers:
+        e = np.zeros(16)
+        L.ref_observer_frame(a_, r_, th_, ptr(u), e.ctypes.data_as(C.POINTER(C.c_double)))
+        frames_in.append([a_, r_, th_, *u])
+        frames.append(e)
+    out["frame_in"] = np.array(frames_in)
+    out["frame_out"] = np.array(frames)
     out["trace_cases"] = np.array(cases)
     out["trace_out"] = np.array(res)
     ends = np.array(res)[:, 0].astype(int)

@@ -134,3 +134,28 @@ def test_eht_defaults_scales_and_error_paths():
     jy = C.c_double()
     assert lib.kt_eht_flux(None, None, 8, 16.0, 3.8, C.byref(jy), None) == K.KT_EINVAL
     assert lib.kt_eht_calibrate(None, C.byref(v), C.byref(f), 0.5, 3.8, None, C.byref(jy), None) == K.KT_EINVAL
+
+
+def test_observer_frame_matches_the_oracle_and_the_reference():
+    """kt_observer_frame (kerr::observerFrame, host code) against the oracle's restatement
+    and the reference's own frames (tests/golden/observer_vectors.npz)."""
+    G = np.load(os.path.join(os.path.dirname(__file__), "golden", "observer_vectors.npz"))
+    lib, OL = K.lib(), O.lib()
+    for (a, r, th, *u), ref in zip(G["frame_in"], G["frame_out"]):
+        uu = np.array(u)
+        got, want = np.zeros(16), np.zeros(16)
+        assert lib.kt_observer_frame(a, r, th, uu.ctypes.data_as(C.POINTER(C.c_double)),
+                                     got.ctypes.data_as(C.POINTER(C.c_double))) == K.KT_OK
+        OL.ko_observer_frame(a, r, th, O._dp(uu / np.sqrt(-np.dot(uu, ks_metric(a, r, th) @ uu))), O._dp(want))
+        assert np.max(np.abs(got - ref)) <= 1e-13 * max(1.0, np.max(np.abs(ref)))
+        assert np.max(np.abs(got - want)) <= 1e-14 * max(1.0, np.max(np.abs(ref)))
+    bad = np.array([0.0, 1.0, 0.0, 0.0])  # spacelike
+    assert lib.kt_observer_frame(0.9, 10.0, 1.0, bad.ctypes.data_as(C.POINTER(C.c_double)),
+                                 np.zeros(16).ctypes.data_as(C.POINTER(C.c_double))) == K.KT_EINVAL
+
+
+def ks_metric(a, r, th):
+    """the Kerr-Schild metric g_{mu nu} at (r, theta) from the oracle"""
+    g = np.zeros(16)
+    O.lib().ko_metric_tensor(a, r, th, O._dp(g))
+    return g.reshape(4, 4)

@@ -27,7 +27,7 @@ def both(v: bsa.View, opts=None):
     ov = O.make_view(**{k: getattr(v, k) for k in (
         "camera_kind", "width", "height", "spin", "r_cam", "theta_cam", "phi_cam", "half_width", "half_height",
         "yaw", "pitch", "roll", "fov_deg", "disk_inner", "disk_outer", "disk_prograde", "disk_enabled",
-        "escape_radius", "capture_factor", "u_obs", "drift")})
+        "escape_radius", "capture_factor", "u_obs", "drift", "frame")})
     want = O.render(ov, O.make_opts(), threads=16)
     return got, want
 
@@ -122,4 +122,33 @@ def test_observer_batch_rules():
     inside = bsa.View(camera_kind=bsa.CAMERA_OBSERVER, width=4, height=4, spin=0.9, r_cam=1.2, theta_cam=1.0)
     with pytest.raises(RuntimeError, match="hovers"):
         tr.trace_views([inside], bsa.TraceOpts(precision=bsa.PREC_FP64))
+    tr.close()
+
+
+def test_explicit_observer_frame():
+    """A view given its frame outright (kt_view.frame, as uCamPhoton carries Camera::frame):
+    the frame kt_observer_frame builds from u_obs traces exactly as u_obs does, a frame with
+    its axes turned matches the oracle given the same frame, and a frame that is not
+    orthonormal is refused."""
+    a, r, th = 0.9, 1.2, 1.0
+    u = falling_u(a, r, th, 1.0, 2.0, 3.0)
+    base = dict(camera_kind=bsa.CAMERA_OBSERVER, width=48, height=32, spin=a, r_cam=r, theta_cam=th, phi_cam=0.4,
+                yaw=0.3, pitch=-0.2, fov_deg=100.0, disk_inner=0.0, disk_outer=20.0, disk_enabled=1,
+                escape_radius=1000.0, capture_factor=1.01)
+    e = bsa.observer_frame(a, r, th, u)
+    by_u, _ = both(bsa.View(**base, u_obs=u))
+    got, want = both(bsa.View(**base, frame=tuple(e.ravel())))
+    for k in ("cls", "steps", "theta_inf", "phi_inf", "r_hit", "g"):
+        assert np.array_equal(got[k], by_u[k], equal_nan=True), k
+    assert_parity(got, want, "frame from u_obs")
+    c, s = math.cos(0.5), math.sin(0.5)
+    e2 = e.copy()
+    e2[1:] = np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]]) @ e[1:]
+    got, want = both(bsa.View(**base, frame=tuple(e2.ravel())))
+    assert_parity(got, want, "turned frame")
+    e3 = e.copy()
+    e3[2] *= 1.001
+    tr = bsa.Tracer(0)
+    with pytest.raises(RuntimeError, match="orthonormal"):
+        tr.trace_views([bsa.View(**base, frame=tuple(e3.ravel()))], bsa.TraceOpts(precision=bsa.PREC_FP64))
     tr.close()
