@@ -292,14 +292,22 @@ def main():
     value_p = rays * wall_frames / wall_p / 1e6
 
     # the same frames through host buffers (the library copies the outputs back over PCIe)
-    host_value = None
+    # through page-locked buffers allocated once (host_outputs), and through fresh pageable
+    # numpy arrays per frame
+    host_value = host_value_pageable = None
     if args.host_outputs:
         k0 = frames[len(frames) // 2]
-        tr.trace_views_host(frame_views[k0], opts)
+        hb = tr.host_outputs(rays, pinned=True)
+        tr.trace_views_host(frame_views[k0], opts, out=hb)
         t0 = time.perf_counter()
         for k in frames:
-            tr.trace_views_host(frame_views[k], opts)
+            tr.trace_views_host(frame_views[k], opts, out=hb)
         host_value = rays * len(frames) / (time.perf_counter() - t0) / 1e6
+        tr.trace_views_host(frame_views[k0], opts)
+        t0 = time.perf_counter()
+        for k in frames[: max(1, len(frames) // 4)]:
+            tr.trace_views_host(frame_views[k], opts)
+        host_value_pageable = rays * max(1, len(frames) // 4) / (time.perf_counter() - t0) / 1e6
 
     # per-kernel attribution (untimed pass without graphs: events around each kernel group)
     integ_ms, sur_ms, attempts, fp64_att, fallback = [], [], [], [], []
@@ -425,6 +433,7 @@ def main():
                                   "frame_ms_p99": round(percentile(lat_p, 0.99), 3)},
                    "speedup_vs_plain_fp64": round(value / value_p, 3),
                    "host_outputs_value": round(host_value, 3) if host_value else None,
+                   "host_outputs_pageable_value": round(host_value_pageable, 3) if host_value_pageable else None,
                    "fallback_fraction": round(sum(fallback) / n_attr / rays, 4) if args.path == "surrogate" else None,
                    "accuracy_vs_fp64": acc,
                    "shade": shade,

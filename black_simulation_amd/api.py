@@ -182,16 +182,36 @@ class Tracer:
         _check(self._ctx, rc, "kt_trace_views")
         return out
 
-    def trace_views_host(self, views: Sequence[View], opts: TraceOpts | None = None) -> dict:
-        """As trace_views, into host (numpy) buffers: the library copies the outputs back
-        over PCIe and synchronises before returning."""
+    def host_outputs(self, n: int, pinned: bool = True) -> dict:
+        """Host buffers for n rays' outputs to reuse across trace_views_host calls: CPU
+        tensors, page-locked when pinned (the copy back then runs at the link's DMA rate
+        instead of through a pageable staging copy)."""
+        t = self.torch
+        dt = {"cls": t.int8, "theta_inf": t.float64, "phi_inf": t.float64, "r_hit": t.float64, "g": t.float64,
+              "steps": t.int32}
+        return {k: t.empty(n, dtype=dt[k], pin_memory=pinned) for k in OUTPUT_FIELDS}
+
+    def trace_views_host(self, views: Sequence[View], opts: TraceOpts | None = None, out: dict | None = None) -> dict:
+        """As trace_views, into host buffers: the library copies the outputs back over PCIe
+        and synchronises before returning.  `out`: reusable buffers (host_outputs, or
+        contiguous numpy arrays of the output dtypes); None: fresh numpy arrays."""
         n = sum(v.rays for v in views)
         dt = {"cls": np.int8, "theta_inf": np.float64, "phi_inf": np.float64, "r_hit": np.float64,
               "g": np.float64, "steps": np.int32}
-        out = {k: np.empty(n, dt[k]) for k in OUTPUT_FIELDS}
+        if out is None:
+            out = {k: np.empty(n, dt[k]) for k in OUTPUT_FIELDS}
         s = K.KtRaysOut()
         for name in OUTPUT_FIELDS:
-            setattr(s, name, out[name].ctypes.data)
+            buf = out[name]
+            if isinstance(buf, np.ndarray):
+                if buf.dtype != dt[name] or buf.size < n or not buf.flags["C_CONTIGUOUS"]:
+                    raise ValueError(f"host output {name}: {n} contiguous {np.dtype(dt[name])} values needed")
+                setattr(s, name, buf.ctypes.data)
+            else:
+                if buf.device.type != "cpu" or buf.numel() < n or not buf.is_contiguous() or \
+                        buf.element_size() != np.dtype(dt[name]).itemsize:
+                    raise ValueError(f"host output {name}: {n} contiguous CPU values of {np.dtype(dt[name])} needed")
+                setattr(s, name, buf.data_ptr())
         s.on_device = 0
         arr = (K.KtView * len(views))(*[v.to_c() for v in views])
         o = (opts or TraceOpts()).to_c()

@@ -146,3 +146,22 @@ def test_host_pointer_outputs():
     want = O.render(oracle_view(v), O.make_opts())
     assert np.array_equal(cls, want["cls"]) and np.array_equal(steps, want["steps"])
     L.kt_destroy(ctx)
+
+
+def test_pinned_host_buffers_reused_across_calls():
+    """trace_views_host into page-locked buffers allocated once (host_outputs) gives, call
+    after call, what device outputs give; fresh numpy buffers give the same."""
+    tr = bsa.Tracer(0)
+    v = c1_view()
+    v.width = v.height = 48
+    hb = tr.host_outputs(v.rays)
+    assert hb["theta_inf"].is_pinned()
+    dev = bsa.to_numpy(tr.trace_views([v]))
+    for _ in range(2):
+        got = tr.trace_views_host([v], out=hb)
+        for k, x in got.items():
+            assert np.array_equal(x.numpy(), dev[k], equal_nan=True), k
+    fresh = tr.trace_views_host([v])
+    for k, x in fresh.items():
+        assert np.array_equal(x, dev[k], equal_nan=True), k
+    tr.close()
