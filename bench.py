@@ -49,7 +49,7 @@ OPS_PER_PARTICLE_ATTEMPT = 656
 PROFILED_TRAFFIC = {
     ("orbit", 1080, "k_mlp_fused"): (3.983e6 + 73.47e6, "profiles/k_mlp_fused_orbit_1080p.md"),
     ("infall", 1080, "k_trace_persistent"): (0.374e6 + 149.1e6, "profiles/k_trace_persistent_plainks_infall_1080p.md"),
-    ("eht", 1024, "k_eht"): (0.3131e6 + 9.437e6, "profiles/k_eht_m87_1024.md"),
+    ("eht", 1024, "k_eht"): (2.468e6 + 12.13e6, "profiles/k_eht_m87_1024.md"),
     ("debris", 1 << 22, "k_advance"): (339.2e6 + 899.9e6, "profiles/k_advance_debris_4m.md"),
 }
 
