@@ -294,7 +294,7 @@ def main():
     # the same frames through host buffers (the library copies the outputs back over PCIe)
     # through page-locked buffers allocated once (host_outputs), and through fresh pageable
     # numpy arrays per frame
-    host_value = host_value_pageable = None
+    host_value = host_value_pageable = host_value_f32 = None
     if args.host_outputs:
         k0 = frames[len(frames) // 2]
         hb = tr.host_outputs(rays, pinned=True)
@@ -303,6 +303,12 @@ def main():
         for k in frames:
             tr.trace_views_host(frame_views[k], opts, out=hb)
         host_value = rays * len(frames) / (time.perf_counter() - t0) / 1e6
+        hb = tr.host_outputs(rays, pinned=True, f32=True)  # 21 B/ray: float angles, radius and g
+        tr.trace_views_host(frame_views[k0], opts, out=hb)
+        t0 = time.perf_counter()
+        for k in frames:
+            tr.trace_views_host(frame_views[k], opts, out=hb)
+        host_value_f32 = rays * len(frames) / (time.perf_counter() - t0) / 1e6
         tr.trace_views_host(frame_views[k0], opts)
         t0 = time.perf_counter()
         for k in frames[: max(1, len(frames) // 4)]:
@@ -434,6 +440,7 @@ def main():
                    "speedup_vs_plain_fp64": round(value / value_p, 3),
                    "host_outputs_value": round(host_value, 3) if host_value else None,
                    "host_outputs_pageable_value": round(host_value_pageable, 3) if host_value_pageable else None,
+                   "host_outputs_f32_value": round(host_value_f32, 3) if host_value_f32 else None,
                    "fallback_fraction": round(sum(fallback) / n_attr / rays, 4) if args.path == "surrogate" else None,
                    "accuracy_vs_fp64": acc,
                    "shade": shade,

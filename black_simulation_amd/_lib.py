@@ -77,6 +77,10 @@ class KtRaysOut(C.Structure):
         ("g", C.c_void_p),
         ("steps", C.c_void_p),
         ("on_device", C.c_int),
+        ("theta_inf_f32", C.c_void_p),
+        ("phi_inf_f32", C.c_void_p),
+        ("r_hit_f32", C.c_void_p),
+        ("g_f32", C.c_void_p),
     ]
 
 

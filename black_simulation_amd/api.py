@@ -23,6 +23,7 @@ import numpy as np
 from . import _lib as K
 
 OUTPUT_FIELDS = ("cls", "theta_inf", "phi_inf", "r_hit", "g", "steps")
+FLOAT_FIELDS = ("theta_inf", "phi_inf", "r_hit", "g")  # float64, or float32 (kt_rays_out *_f32)
 
 
 @dataclasses.dataclass
@@ -147,24 +148,32 @@ class Tracer:
         self.close()
 
     # -- device outputs -------------------------------------------------------------
-    def alloc_outputs(self, n: int) -> dict:
+    def alloc_outputs(self, n: int, f32: bool = False) -> dict:
+        """Device outputs for n rays; f32: the sky angles, disk radius and g in float (each
+        the FP64 result rounded once), 21 instead of 37 bytes per ray."""
         t = self.torch
         dev = t.device("cuda", self.device)
+        real = t.float32 if f32 else t.float64
         return {
             "cls": t.empty(n, dtype=t.int8, device=dev),
-            "theta_inf": t.empty(n, dtype=t.float64, device=dev),
-            "phi_inf": t.empty(n, dtype=t.float64, device=dev),
-            "r_hit": t.empty(n, dtype=t.float64, device=dev),
-            "g": t.empty(n, dtype=t.float64, device=dev),
+            "theta_inf": t.empty(n, dtype=real, device=dev),
+            "phi_inf": t.empty(n, dtype=real, device=dev),
+            "r_hit": t.empty(n, dtype=real, device=dev),
+            "g": t.empty(n, dtype=real, device=dev),
             "steps": t.empty(n, dtype=t.int32, device=dev),
         }
 
     @staticmethod
-    def _out_struct(out: dict) -> K.KtRaysOut:
+    def _set_out(s: K.KtRaysOut, name: str, ptr, is_f32: bool):
+        """one output field of kt_rays_out: the float pointer for FP32 buffers"""
+        setattr(s, name + "_f32" if is_f32 and name in FLOAT_FIELDS else name, ptr)
+
+    def _out_struct(self, out: dict) -> K.KtRaysOut:
         s = K.KtRaysOut()
         for name in OUTPUT_FIELDS:
             tens = out.get(name)
-            setattr(s, name, tens.data_ptr() if tens is not None else None)
+            if tens is not None:
+                self._set_out(s, name, tens.data_ptr(), tens.dtype == self.torch.float32)
         s.on_device = 1
         return s
 
@@ -182,13 +191,13 @@ class Tracer:
         _check(self._ctx, rc, "kt_trace_views")
         return out
 
-    def host_outputs(self, n: int, pinned: bool = True) -> dict:
+    def host_outputs(self, n: int, pinned: bool = True, f32: bool = False) -> dict:
         """Host buffers for n rays' outputs to reuse across trace_views_host calls: CPU
         tensors, page-locked when pinned (the copy back then runs at the link's DMA rate
-        instead of through a pageable staging copy)."""
+        instead of through a pageable staging copy); f32 as alloc_outputs."""
         t = self.torch
-        dt = {"cls": t.int8, "theta_inf": t.float64, "phi_inf": t.float64, "r_hit": t.float64, "g": t.float64,
-              "steps": t.int32}
+        real = t.float32 if f32 else t.float64
+        dt = {"cls": t.int8, "theta_inf": real, "phi_inf": real, "r_hit": real, "g": real, "steps": t.int32}
         return {k: t.empty(n, dtype=dt[k], pin_memory=pinned) for k in OUTPUT_FIELDS}
 
     def trace_views_host(self, views: Sequence[View], opts: TraceOpts | None = None, out: dict | None = None) -> dict:
@@ -204,14 +213,16 @@ class Tracer:
         for name in OUTPUT_FIELDS:
             buf = out[name]
             if isinstance(buf, np.ndarray):
-                if buf.dtype != dt[name] or buf.size < n or not buf.flags["C_CONTIGUOUS"]:
+                is_f32 = name in FLOAT_FIELDS and buf.dtype == np.float32
+                if (buf.dtype != dt[name] and not is_f32) or buf.size < n or not buf.flags["C_CONTIGUOUS"]:
                     raise ValueError(f"host output {name}: {n} contiguous {np.dtype(dt[name])} values needed")
-                setattr(s, name, buf.ctypes.data)
+                self._set_out(s, name, buf.ctypes.data, is_f32)
             else:
+                is_f32 = name in FLOAT_FIELDS and buf.dtype == self.torch.float32
                 if buf.device.type != "cpu" or buf.numel() < n or not buf.is_contiguous() or \
-                        buf.element_size() != np.dtype(dt[name]).itemsize:
+                        (buf.element_size() != np.dtype(dt[name]).itemsize and not is_f32):
                     raise ValueError(f"host output {name}: {n} contiguous CPU values of {np.dtype(dt[name])} needed")
-                setattr(s, name, buf.data_ptr())
+                self._set_out(s, name, buf.data_ptr(), is_f32)
         s.on_device = 0
         arr = (K.KtView * len(views))(*[v.to_c() for v in views])
         o = (opts or TraceOpts()).to_c()

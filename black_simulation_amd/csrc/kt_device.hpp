@@ -377,6 +377,7 @@ struct Outputs {
     double* r_hit;
     double* g;
     int32_t* steps;
+    float *theta32, *phi32, *r_hit32, *g32;  // optional FP32 copies (rounded once)
 };
 
 struct Result {
@@ -801,6 +802,10 @@ __device__ __forceinline__ void write_result(const Outputs& o, long long idx, co
     if (o.r_hit) o.r_hit[idx] = r.cls == kDisk ? r.r_hit : nan;
     if (o.g) o.g[idx] = r.cls == kDisk ? r.g : nan;
     if (o.steps) o.steps[idx] = steps;
+    if (o.theta32) o.theta32[idx] = (float)(r.cls == kEscaped ? r.theta : nan);
+    if (o.phi32) o.phi32[idx] = (float)(r.cls == kEscaped ? r.phi : nan);
+    if (o.r_hit32) o.r_hit32[idx] = (float)(r.cls == kDisk ? r.r_hit : nan);
+    if (o.g32) o.g32[idx] = (float)(r.cls == kDisk ? r.g : nan);
 }
 
 }  // namespace ktd

@@ -459,7 +459,7 @@ int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const k
 
     // outputs: device pointers as given, or staging for host pointers
     ktd::Outputs dout{};
-    size_t off[6] = {0, 0, 0, 0, 0, 0};
+    size_t off[10] = {};
     if (out->on_device) {
         dout.cls = out->cls;
         dout.theta = out->theta_inf;
@@ -467,6 +467,10 @@ int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const k
         dout.r_hit = out->r_hit;
         dout.g = out->g;
         dout.steps = out->steps;
+        dout.theta32 = out->theta_inf_f32;
+        dout.phi32 = out->phi_inf_f32;
+        dout.r_hit32 = out->r_hit_f32;
+        dout.g32 = out->g_f32;
     } else {
         size_t bytes = 0;
         const size_t n = (size_t)total;
@@ -481,6 +485,10 @@ int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const k
         off[3] = out->r_hit ? take(8) : 0;
         off[4] = out->g ? take(8) : 0;
         off[5] = out->steps ? take(4) : 0;
+        off[6] = out->theta_inf_f32 ? take(4) : 0;
+        off[7] = out->phi_inf_f32 ? take(4) : 0;
+        off[8] = out->r_hit_f32 ? take(4) : 0;
+        off[9] = out->g_f32 ? take(4) : 0;
         if (ensure_stage(ctx, bytes + 256) != KT_OK) return KT_ENOMEM;
         char* s = static_cast<char*>(ctx->d_stage);
         if (out->cls) dout.cls = reinterpret_cast<int8_t*>(s + off[0]);
@@ -489,6 +497,10 @@ int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const k
         if (out->r_hit) dout.r_hit = reinterpret_cast<double*>(s + off[3]);
         if (out->g) dout.g = reinterpret_cast<double*>(s + off[4]);
         if (out->steps) dout.steps = reinterpret_cast<int32_t*>(s + off[5]);
+        if (out->theta_inf_f32) dout.theta32 = reinterpret_cast<float*>(s + off[6]);
+        if (out->phi_inf_f32) dout.phi32 = reinterpret_cast<float*>(s + off[7]);
+        if (out->r_hit_f32) dout.r_hit32 = reinterpret_cast<float*>(s + off[8]);
+        if (out->g_f32) dout.g32 = reinterpret_cast<float*>(s + off[9]);
     }
 
     KT_HIP(hipMemcpyAsync(ctx->d_views, vd.data(), sizeof(ViewDev) * vd.size(), hipMemcpyHostToDevice, stream));
@@ -530,6 +542,11 @@ int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const k
         if (out->r_hit) KT_HIP(hipMemcpyAsync(out->r_hit, dout.r_hit, 8 * n, hipMemcpyDeviceToHost, stream));
         if (out->g) KT_HIP(hipMemcpyAsync(out->g, dout.g, 8 * n, hipMemcpyDeviceToHost, stream));
         if (out->steps) KT_HIP(hipMemcpyAsync(out->steps, dout.steps, 4 * n, hipMemcpyDeviceToHost, stream));
+        if (out->theta_inf_f32)
+            KT_HIP(hipMemcpyAsync(out->theta_inf_f32, dout.theta32, 4 * n, hipMemcpyDeviceToHost, stream));
+        if (out->phi_inf_f32) KT_HIP(hipMemcpyAsync(out->phi_inf_f32, dout.phi32, 4 * n, hipMemcpyDeviceToHost, stream));
+        if (out->r_hit_f32) KT_HIP(hipMemcpyAsync(out->r_hit_f32, dout.r_hit32, 4 * n, hipMemcpyDeviceToHost, stream));
+        if (out->g_f32) KT_HIP(hipMemcpyAsync(out->g_f32, dout.g32, 4 * n, hipMemcpyDeviceToHost, stream));
         KT_HIP(hipStreamSynchronize(stream));
     }
     return KT_OK;
@@ -835,7 +852,8 @@ int kt_plan_create(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
             if ((e = hipEventCreate(&ev)) != hipSuccess) return cleanup(KT_EHIP, "plan events", e);
         if (o.use_surrogate && (e = kts::queue_reserve(p->queue, total)) != hipSuccess)
             return cleanup(KT_ENOMEM, "plan fallback queue", e);
-        ktd::Outputs dout{out->cls, out->theta_inf, out->phi_inf, out->r_hit, out->g, out->steps};
+        ktd::Outputs dout{out->cls, out->theta_inf, out->phi_inf, out->r_hit, out->g, out->steps,
+                          out->theta_inf_f32, out->phi_inf_f32, out->r_hit_f32, out->g_f32};
         p->L = make_launch(ctx, p->d_views, (int)vd.size(), total, p->d_counter, dout, o);
         if (int st = observer_batch(ctx, vd, o, p->L.ks)) {
             kt_plan_destroy(p);

@@ -711,6 +711,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (a.out.r_hit) a.out.r_hit[q] = rh;
     if (a.out.g) a.out.g[q] = g;
     if (a.out.steps) a.out.steps[q] = 0;
+    if (a.out.theta32) a.out.theta32[q] = (float)th;
+    if (a.out.phi32) a.out.phi32[q] = (float)ph;
+    if (a.out.r_hit32) a.out.r_hit32[q] = (float)rh;
+    if (a.out.g32) a.out.g32[q] = (float)g;
 }
 
 template <class T>

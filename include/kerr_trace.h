@@ -144,6 +144,13 @@ typedef struct kt_rays_out {
     double* g;
     int32_t* steps;
     int on_device;
+    /* optional FP32 outputs, written beside or instead of the doubles above (a field may
+     * be given in either width or both): each the FP64 result rounded once, NaN where it
+     * does not apply.  Same placement as the others (on_device). */
+    float* theta_inf_f32;
+    float* phi_inf_f32;
+    float* r_hit_f32;
+    float* g_f32;
 } kt_rays_out;
 
 typedef struct kt_device_opts {

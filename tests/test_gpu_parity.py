@@ -165,3 +165,40 @@ def test_pinned_host_buffers_reused_across_calls():
     for k, x in fresh.items():
         assert np.array_equal(x, dev[k], equal_nan=True), k
     tr.close()
+
+
+def test_float_outputs_are_the_doubles_rounded_once():
+    """kt_rays_out's FP32 fields: on the device, through host buffers and from a plan, each
+    is the FP64 output rounded to float (NaN where the output does not apply); the surrogate
+    path's answered rays too."""
+    import torch
+
+    tr = bsa.Tracer(0)
+    v = c2_view(96, 54)
+    want = bsa.to_numpy(tr.trace_views([v]))
+    f = bsa.to_numpy(tr.trace_views([v], out=tr.alloc_outputs(v.rays, f32=True)))
+    h = tr.trace_views_host([v], out=tr.host_outputs(v.rays, f32=True))
+    plan = tr.plan([v], out=tr.alloc_outputs(v.rays, f32=True))
+    p = plan.launch()
+    torch.cuda.synchronize()
+    p = bsa.to_numpy(p)
+    plan.close()
+    for got in (f, {k: x.numpy() for k, x in h.items()}, p):
+        assert np.array_equal(got["cls"], want["cls"]) and np.array_equal(got["steps"], want["steps"])
+        for k in ("theta_inf", "phi_inf", "r_hit", "g"):
+            assert got[k].dtype == np.float32
+            assert np.array_equal(got[k], want[k].astype(np.float32), equal_nan=True), k
+    from black_simulation_amd.train_surrogate import load_params
+
+    tr.surrogate_load(load_params())
+    opts = bsa.TraceOpts(use_surrogate=1, precision=bsa.PREC_MIXED)
+    from black_simulation_amd.configs import c5_view
+
+    sv = c5_view(30, 96, 54)
+    d = bsa.to_numpy(tr.trace_views([sv], opts))
+    s = bsa.to_numpy(tr.trace_views([sv], opts, out=tr.alloc_outputs(sv.rays, f32=True)))
+    assert ((d["cls"] & bsa.FLAG_SURROGATE) != 0).any()
+    assert np.array_equal(s["cls"], d["cls"])
+    for k in ("theta_inf", "phi_inf", "r_hit", "g"):
+        assert np.array_equal(s[k], d[k].astype(np.float32), equal_nan=True), k
+    tr.close()
