@@ -95,6 +95,42 @@ def cpu_baseline(view, seconds_target: float = 15.0) -> dict:
                       f"{dt:.1f} s, FP64 DP5 rtol 1e-10, {cores} threads)"}
 
 
+def c1_timing(tr, torch) -> dict:
+    """The CPU-runnable configuration in full (a = 0.9, r = 30 M, i = 80 deg, 256^2 Bardeen
+    pixels, FP64 DP5 rtol 1e-10): the GPU's FP64 path (HIP events, best of 5) and the oracle
+    on every pixel over the host's cores; the classes agree bit for bit (the parity tests
+    check every output)."""
+    import numpy as np
+
+    import black_simulation_amd as bsa
+    from black_simulation_amd.configs import c1_view
+    from oracle import oracle as O
+
+    v = c1_view()
+    opts = bsa.TraceOpts(precision=bsa.PREC_FP64)
+    out = tr.trace_views([v], opts)
+    stream = torch.cuda.current_stream()
+    best = 1e30
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        tr.trace_views([v], opts, out=out)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    got = out["cls"].cpu().numpy()
+    cores = os.cpu_count() or 1
+    ov = O.make_view(**{k: getattr(v, k) for k in (
+        "camera_kind", "width", "height", "spin", "r_cam", "theta_cam", "phi_cam", "half_width", "half_height",
+        "disk_inner", "disk_outer", "disk_prograde", "disk_enabled", "escape_radius", "capture_factor")})
+    t0 = time.perf_counter()
+    want = O.render(ov, O.make_opts(), threads=cores)
+    cpu_s = time.perf_counter() - t0
+    return {"rays": v.rays, "gpu_fp64_ms": round(best, 3), "gpu_fp64_mrays_s": round(v.rays / best / 1e3, 2),
+            "cpu_port_s": round(cpu_s, 3), "cpu_port_mrays_s": round(v.rays / cpu_s / 1e6, 4), "cpu_threads": cores,
+            "classes_equal": bool(np.array_equal(got & 7, want["cls"]))}
+
+
 def _ref_lib():
     """oracle/_ref/libkerr_ref.so: the reference's own sources compiled as they are (with
     oracle/ref_shim.cpp), or None where it was not built"""
@@ -454,6 +490,8 @@ def main():
     orb.close()
     if ref is not orb:
         ref.close()
+    if not args.no_cpu and args.workload == "orbit":
+        rec["c1"] = c1_timing(tr, torch)
     tr.close()
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(frame_views[frames[len(frames) // 2]][0])
