@@ -974,8 +974,9 @@ int kt_shade_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_rays
     try {
         if (!rays || !rgba) return fail(ctx, KT_EINVAL, "rays/rgba missing");
         if (!rays->on_device) return fail(ctx, KT_EINVAL, "shading reads device outputs (on_device = 1)");
-        if (!rays->cls || !rays->theta_inf || !rays->phi_inf || !rays->r_hit || !rays->g)
-            return fail(ctx, KT_EINVAL, "shading needs cls, theta_inf, phi_inf, r_hit and g");
+        if (!rays->cls || !(rays->theta_inf || rays->theta_inf_f32) || !(rays->phi_inf || rays->phi_inf_f32) ||
+            !(rays->r_hit || rays->r_hit_f32) || !(rays->g || rays->g_f32))
+            return fail(ctx, KT_EINVAL, "shading needs cls, theta_inf, phi_inf, r_hit and g (FP64 or FP32)");
         kt_shade_opts o;
         kt_shade_opts_default(&o);
         if (opts) o = *opts;
@@ -1006,6 +1007,10 @@ int kt_shade_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_rays
         s.phi = rays->phi_inf;
         s.r_hit = rays->r_hit;
         s.g = rays->g;
+        s.theta32 = rays->theta_inf_f32;
+        s.phi32 = rays->phi_inf_f32;
+        s.r_hit32 = rays->r_hit_f32;
+        s.g32 = rays->g_f32;
         s.table = ctx->d_blackbody;
         s.table_n = kBlackbodyEntries;
         s.disk_temperature = o.disk_temperature;

@@ -103,7 +103,7 @@ __global__ void k_shade(ShadeLaunch s) {
     const int cls = s.cls[q] & 7;
     float3 c = make_float3(0.f, 0.f, 0.f);
     if (cls == 0) {
-        const double th = s.theta[q], ph = s.phi[q];
+        const double th = s.theta ? s.theta[q] : (double)s.theta32[q], ph = s.phi ? s.phi[q] : (double)s.phi32[q];
         const float st = (float)sin(th);
         const float dx = st * (float)cos(ph), dy = st * (float)sin(ph), dz = (float)cos(th);
         if (s.sky == 1) c = grid_sky(dx, dy, dz, s.grid_line);
@@ -112,7 +112,7 @@ __global__ void k_shade(ShadeLaunch s) {
         c.y *= s.sky_brightness;
         c.z *= s.sky_brightness;
     } else if (cls == 2) {
-        const double r = s.r_hit[q], g = s.g[q];
+        const double r = s.r_hit ? s.r_hit[q] : (double)s.r_hit32[q], g = s.g ? s.g[q] : (double)s.g32[q];
         const double x = r / v.r_in;
         const double flux = fmax(1.0 / (x * x * x) * (1.0 - 1.0 / sqrt(x)), 0.0);
         const float T = (float)(s.disk_temperature * sqrt(sqrt(flux / kFluxPeak)));

@@ -19,7 +19,8 @@ struct ShadeLaunch {
     const ktd::ViewDev* views;
     int n_views;
     const int8_t* cls;
-    const double *theta, *phi, *r_hit, *g;
+    const double *theta, *phi, *r_hit, *g;  // FP64 outputs, or null and:
+    const float *theta32, *phi32, *r_hit32, *g32;  // the FP32 ones
     const float* table;   // blackbody RGB, table_n entries
     int table_n;
     double disk_temperature;

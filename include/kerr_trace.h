@@ -281,8 +281,9 @@ typedef struct kt_shade_opts {
     int sky_width, sky_height; /*   row 0 at latitude -90 deg, column 0 at longitude -pi */
 } kt_shade_opts;
 void kt_shade_opts_default(kt_shade_opts* opts);
-/* rays: the device outputs of a kt_trace_views call on the same views; rgba: device,
- * 4 x uint16 (IEEE half) per ray */
+/* rays: the device outputs of a kt_trace_views call on the same views (each of the sky
+ * angles, disk radius and g in FP64, or else its FP32 field); rgba: device, 4 x uint16
+ * (IEEE half) per ray */
 int kt_shade_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_rays_out* rays,
                    const kt_shade_opts* opts, void* rgba, void* hip_stream);
 /* n linear-sRGB blackbody colours over log10 T in [2.5, 5.3], luminance 1 at 6500 K

@@ -53,3 +53,19 @@ def test_image_sky_matches_the_oracle(tracer):
     want = O.shade(host, 1.0, 20.0, sky=2, sky_image=img, sky_brightness=0.5)
     compare(rgba[:, :3], want, "image sky")
     assert (host["cls"] & 7 == bsa.ESCAPED).mean() > 0.5
+
+
+def test_shading_reads_fp32_outputs_too(tracer):
+    """kt_shade_views on the FP32 output fields: the colours of the FP64 outputs rounded to
+    float (the same function of values within a float ulp: within 2 half ulps)."""
+    import torch
+
+    v = bsa.View(width=64, height=48, spin=0.9, r_cam=30.0, theta_cam=1.3, half_width=14.0, half_height=10.5)
+    d = tracer.trace_views([v])
+    f = tracer.trace_views([v], out=tracer.alloc_outputs(v.rays, f32=True))
+    a = tracer.shade([v], d)
+    b = tracer.shade([v], f)
+    torch.cuda.synchronize()
+    a = a.float().cpu().numpy()
+    b = b.float().cpu().numpy()
+    assert np.allclose(a, b, rtol=2e-3, atol=1e-6)
