@@ -249,8 +249,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: rays are independent and this build runs on a single MI355X; "
-              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        print(f"bench.py measures one GPU: the tracer runs on a single MI355X (rays are independent; how a "
+              f"launch would shard across GPUs is described, not built); --gpus {args.gpus} is not supported",
+              file=sys.stderr)
         sys.exit(2)
     if args.path == "surrogate" and args.workload == "c2":
         print("the surrogate is trained for the orbit scene (disk to 20 M); use --workload orbit", file=sys.stderr)
