@@ -334,3 +334,45 @@ def test_step_budget_exhaustion_is_a_class_not_an_error():
     v = O.make_view()
     res = O.trace_params(v, O.make_opts(max_steps=10), [[3.0, 2.0]])
     assert res["cls"][0] == O.UNFINISHED and res["steps"][0] == 10
+
+
+def _kerr_equatorial_sweep(a, L, r_cam):
+    """Angle swept by an equatorial (Q = 0, E = 1) photon of angular momentum L around a
+    hole of spin a, from r_cam in to periapsis and out to infinity.  With u = 1/r the
+    orbit equations give dphi = F(u) du / sqrt(U(u)):
+        U(u) = 1 + (a^2 - L^2) u^2 + 2 (L - a)^2 u^3          (u^4 R(1/u))
+        F(u) = L - a + a (1 + a (a - L) u^2) / (1 - 2 u + a^2 u^2)   (L - aE + a P/Delta)
+    and u = u0 - t^2 removes the inverse square root at periapsis u0, with
+    U(u) = (u0 - u) V(u), V(u) = -(c3 (u^2 + u u0 + u0^2) + c2 (u + u0))."""
+    c3, c2 = 2.0 * (L - a) ** 2, a * a - L * L
+    roots = np.roots([c3, c2, 0.0, 1.0])
+    u0 = min(x.real for x in roots if abs(x.imag) < 1e-12 and x.real > 0)
+
+    def f(t):
+        u = u0 - t * t
+        V = -(c3 * (u * u + u * u0 + u0 * u0) + c2 * (u + u0))
+        F = L - a + a * (1.0 + a * (a - L) * u * u) / (1.0 - 2.0 * u + a * a * u * u)
+        return 2.0 * F / math.sqrt(V)
+
+    uc = 1.0 / r_cam
+    inward = integrate.quad(f, 0.0, math.sqrt(u0 - uc), epsabs=1e-14, epsrel=1e-13, limit=400)[0]
+    outward = integrate.quad(f, 0.0, math.sqrt(u0), epsabs=1e-14, epsrel=1e-13, limit=400)[0]
+    return inward, outward, 1.0 / u0
+
+
+@pytest.mark.parametrize("alpha", [-4.0, -8.0, 8.0, 12.0])
+def test_kerr_equatorial_deflection_exact(alpha):
+    """a = 0.9, edge-on Bardeen camera, beta = 0 (Q = 0, L = -alpha): the sky phi equals the
+    exact quadrature of the equatorial orbit equation to 1e-9 rad, for photons on the
+    prograde (L = 4, 8) and the retrograde side (L = -8, -12) of the critical impact
+    parameters (2.844 and -6.834); frame dragging makes the two sides differ."""
+    a, r_cam = 0.9, 50.0
+    L = -alpha
+    v = O.make_view(spin=a, r_cam=r_cam, theta_cam=math.pi / 2, disk_enabled=0)
+    out = O.trace_params(v, OPTS, [[alpha, 0.0]])
+    assert out["cls"][0] == O.ESCAPED
+    inward, outward, r_p = _kerr_equatorial_sweep(a, L, r_cam)
+    want = (-(inward + outward)) % (2 * math.pi)
+    d = (out["phi"][0] - want + math.pi) % (2 * math.pi) - math.pi
+    assert abs(d) < 1e-9, (out["phi"][0], want, r_p)
+    assert abs(out["theta"][0] - math.pi / 2) < 1e-12
