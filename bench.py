@@ -207,6 +207,84 @@ def infall_cpu_reference(view, seconds_target: float = 10.0):
                       f"trajectory::traceRay from the observer's frame (its RK4), {cores} threads"}
 
 
+def bardeen_cpu_reference(view, seconds_target: float = 10.0):
+    """The reference's own trajectory::traceRay (src/trajectory.cpp:214-396: RK4 with its
+    step rule at tol 0.01, linear disk crossing) from the camera's ZAMO frame, on every
+    stride-th pixel of one frame of a Bardeen-plane workload, from a thread pool of the
+    host's cores; None when oracle/_ref was not built.  Pixel (alpha, beta) becomes the
+    look direction whose photon has L = -alpha sin(i) and Q = beta^2 + (alpha^2 - a^2)
+    cos^2(i) (ZAMO-frame components of p: sqrt(R)/sqrt(Sigma Delta), sign(beta)
+    sqrt(Theta)/sqrt(Sigma), L/sqrt(g_phiphi), reversed and normalised).  Its classes, disk
+    radii and g are set beside the FP64 oracle's on the same pixels; its g, relative to the
+    ZAMO's unit photon energy, times the photon's conserved energy E is g seen from infinity."""
+    import ctypes as C
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    from oracle import oracle as O
+
+    lib = _ref_lib()
+    if lib is None:
+        return None
+    dp = C.POINTER(C.c_double)
+    lib.ref_trajectory_trace.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, dp, C.c_int,
+                                         C.c_double, C.c_double, C.c_double, dp]
+    lib.ref_trajectory_trace.restype = None
+    a, r, th, W, H = view.spin, view.r_cam, view.theta_cam, view.width, view.height
+    st, ct = math.sin(th), math.cos(th)
+    S, D = r * r + a * a * ct * ct, r * r - 2 * r + a * a
+    gpp = ((r * r + a * a) ** 2 - a * a * D * st * st) * st * st / S
+    r_in = view.disk_inner if view.disk_inner > 0 else __import__("black_simulation_amd").isco_radius(view.spin, True)
+    n = W * H
+
+    def one(q):
+        row, col = divmod(q, W)
+        al = -view.half_width + (col + 0.5) * 2.0 * view.half_width / W
+        be = view.half_height - (row + 0.5) * 2.0 * view.half_height / H
+        L, Q = -al * st, be * be + (al * al - a * a) * ct * ct
+        R = (r * r + a * a - a * L) ** 2 - D * (Q + (L - a) ** 2)
+        Th = Q - ct * ct * (L * L / (st * st) - a * a)
+        p = np.array([math.sqrt(max(R, 0.0)) / math.sqrt(S * D), math.copysign(math.sqrt(max(Th, 0.0)), be) / math.sqrt(S),
+                      L / math.sqrt(gpp)])
+        look = (C.c_double * 3)(*(-p / np.linalg.norm(p)))
+        out = (C.c_double * 11)()
+        lib.ref_trajectory_trace(a, r, th, view.phi_cam, look, 1 if view.disk_enabled else 0, r_in, view.disk_outer,
+                                 view.escape_radius, out)
+        return out[0], out[1], out[2] * out[8]  # g seen by the ZAMO times E: g seen from infinity
+
+    cores = os.cpu_count() or 1
+
+    def run(stride):
+        idx = list(range(0, n, stride))  # the oracle's sparse sample: idx % stride == 0
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(cores) as ex:
+            res = list(ex.map(one, idx, chunksize=16))
+        return res, time.perf_counter() - t0
+
+    res, dt = run(max(1, n // (64 * cores)))
+    stride = max(1, int(n / max(1.0, len(res) / dt * seconds_target)))
+    res, dt = run(stride)
+    k = len(res)
+    end = np.array([e for e, _, _ in res], dtype=np.int64)
+    cls = np.select([end == 0, end == 1, end == 2], [O.ESCAPED, O.CAPTURED, O.DISK], O.UNFINISHED)
+    ov = O.make_view(**{f: getattr(view, f) for f in (
+        "camera_kind", "width", "height", "spin", "r_cam", "theta_cam", "phi_cam", "half_width", "half_height",
+        "yaw", "pitch", "roll", "fov_deg", "disk_inner", "disk_outer", "disk_prograde", "disk_enabled",
+        "escape_radius", "capture_factor")})
+    want = O.render(ov, O.make_opts(), threads=cores, stride=stride)
+    both = (cls == O.DISK) & (want["cls"] == O.DISK)
+    r_rel = np.abs(np.array([x for _, x, _ in res])[both] / want["r_hit"][both] - 1.0)
+    g_rel = np.abs(np.array([x for _, _, x in res])[both] / want["g"][both] - 1.0)
+    pct = lambda x: [float("%.3g" % v) for v in np.percentile(x, [50, 99])] if x.size else None  # noqa: E731
+    return {"value": round(k / dt / 1e6, 5), "unit": "Mrays/s", "cores": cores, "kind": "reference",
+            "sample": f"every {stride}th pixel of one {W}x{H} frame ({k} rays, {dt:.1f} s): the reference's "
+                      f"trajectory::traceRay from the camera's ZAMO frame (its RK4), {cores} threads",
+            "vs_fp64_oracle": {"class_agreement": round(float((cls == want["cls"]).mean()), 5),
+                               "unfinished": int((cls == O.UNFINISHED).sum()),
+                               "r_hit_rel_p50_p99": pct(r_rel), "g_rel_p50_p99": pct(g_rel)}}
+
+
 def percentile(xs, q):
     s = sorted(xs)
     return s[min(len(s) - 1, int(q * (len(s) - 1) + 0.5))]
@@ -498,6 +576,8 @@ def main():
         rec["cpu_baseline"] = cpu_baseline(frame_views[frames[len(frames) // 2]][0])
         if args.workload == "infall":
             rec["cpu_reference"] = infall_cpu_reference(frame_views[frames[len(frames) // 2]][0])
+        if args.workload in ("orbit", "c2"):
+            rec["cpu_reference"] = bardeen_cpu_reference(frame_views[frames[len(frames) // 2]][0])
     print(json.dumps(rec), flush=True)
 
 
