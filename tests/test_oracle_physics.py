@@ -178,6 +178,47 @@ def test_disk_hit_radius_and_redshift_exact(b):
     assert out["g"][0] == pytest.approx(math.sqrt(1.0 - 3.0 / out["r_hit"][0]), rel=1e-14)
 
 
+@pytest.mark.parametrize("alpha,beta", [(0.0, -3.0), (3.0, -4.0), (-5.0, -2.0), (2.0, -7.0), (6.0, -1.5),
+                                        (-8.0, -3.0)])
+def test_kerr_disk_hit_radius_exact(alpha, beta):
+    """a = 0.9, Bardeen camera at r = 50 M, i = 60 deg, beta < 0: the ray runs from theta = i
+    down to the equator with neither r nor theta turning, so at the crossing the Mino
+    times agree: int_i^(pi/2) dtheta / sqrt(Theta) = int_(r_hit)^(r_cam) dr / sqrt(R).
+    r_hit from that equation (quadrature + root) to 1e-9 relative; dropping the spin term
+    of Theta moves it by ~3e-3.  g is the Keplerian prograde closed form at r_hit."""
+    a, r_cam, inc = 0.9, 50.0, math.radians(60.0)
+    Lz = -alpha * math.sin(inc)
+    Q = beta * beta + (alpha * alpha - a * a) * math.cos(inc) ** 2
+
+    def theta_pot(t):
+        return Q - math.cos(t) ** 2 * (Lz * Lz / math.sin(t) ** 2 - a * a)
+
+    def radial_pot(r):
+        return (r * r + a * a - a * Lz) ** 2 - (r * r - 2.0 * r + a * a) * (Q + (Lz - a) ** 2)
+
+    assert min(theta_pot(t) for t in np.linspace(inc, math.pi / 2, 2001)) > 0
+    lam = integrate.quad(lambda t: 1.0 / math.sqrt(theta_pot(t)), inc, math.pi / 2, epsabs=1e-15, epsrel=1e-13,
+                         limit=400)[0]
+    roots = np.roots([1.0, 0.0, a * a - Lz * Lz - Q, 2.0 * (Q + (Lz - a) ** 2), -a * a * Q])
+    r_turn = max([x.real for x in roots if abs(x.imag) < 1e-10 and x.real < r_cam] + [L.ko_horizon_radius(a)])
+
+    def mino_gap(r):
+        return integrate.quad(lambda x: 1.0 / math.sqrt(radial_pot(x)), r, r_cam, epsabs=1e-15, epsrel=1e-13,
+                              limit=400)[0] - lam
+
+    lo = r_turn + 1e-3
+    assert mino_gap(lo) > 0  # the equator comes before the radial turning point
+    r_want = optimize.brentq(mino_gap, lo, r_cam, xtol=1e-14, rtol=1e-15)
+    v = O.make_view(spin=a, r_cam=r_cam, theta_cam=inc, disk_enabled=1, disk_inner=1.5, disk_outer=100.0)
+    out = O.trace_params(v, OPTS, [[alpha, beta]])
+    assert out["cls"][0] == O.DISK, out
+    r = out["r_hit"][0]
+    assert r == pytest.approx(r_want, rel=1e-9)
+    om = 1.0 / (r ** 1.5 + a)
+    g_want = math.sqrt((1 - 2 / r) + 4 * a / r * om - (r * r + a * a + 2 * a * a / r) * om * om) / (1 - Lz * om)
+    assert out["g"][0] == pytest.approx(g_want, rel=1e-12)
+
+
 def test_equatorial_critical_impact_parameters():
     """testShadow (test_kerr.cpp:222-262): b = b_c (1 -+ 1e-3) falls in / escapes, for a = 0
     (b_c = 3 sqrt 3) and a = 0.9 (xi(r_ph) = 2.844 prograde, -6.834 retrograde)."""
