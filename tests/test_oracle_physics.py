@@ -1,12 +1,13 @@
 """The FP64 oracle against what the mathematics fixes: the reference's own theta-form
 equations (chain rule), finite differences of the potentials, conserved quantities,
-exact Schwarzschild quadratures, the critical curve and the reference's known answers
+exact Schwarzschild and Kerr quadratures (quadratures.py), the critical curve and the reference's known answers
 (test_kerr.cpp testShadow, testNearAxisPhotons, testEht)."""
 import math
 
 import numpy as np
 import pytest
-from scipy import integrate, optimize
+import quadratures as Qd
+from scipy import optimize
 
 from oracle import oracle as O
 
@@ -120,23 +121,6 @@ def test_constraints_hold_at_termination():
     assert n_checked > 50
 
 
-def _schwarzschild_sweep(b, r_cam):
-    """Angle swept by an equatorial a = 0 photon of impact parameter b from r_cam in to
-    periapsis and out to infinity: integral of du / sqrt(1/b^2 - u^2 + 2u^3), with
-    u = u0 - t^2 removing the inverse square root at the turning point u0."""
-    roots = np.roots([2.0, -1.0, 0.0, 1.0 / (b * b)])
-    u0 = min(x.real for x in roots if abs(x.imag) < 1e-12 and x.real > 0)
-
-    def g(u):  # B(u) = (u0 - u) g(u)
-        return (u + u0) - 2.0 * (u * u + u * u0 + u0 * u0)
-
-    f = lambda t: 2.0 / math.sqrt(g(u0 - t * t))
-    uc = 1.0 / r_cam
-    inward = integrate.quad(f, 0.0, math.sqrt(u0 - uc), epsabs=1e-14, epsrel=1e-13, limit=200)[0]
-    outward = integrate.quad(f, 0.0, math.sqrt(u0), epsabs=1e-14, epsrel=1e-13, limit=200)[0]
-    return inward, outward, 1.0 / u0
-
-
 @pytest.mark.parametrize("b", [6.0, 7.5, 10.0, 20.0])
 def test_schwarzschild_deflection_exact(b):
     """a = 0, edge-on camera, beta = 0: phi at infinity equals the exact quadrature (1e-9 rad)."""
@@ -144,7 +128,7 @@ def test_schwarzschild_deflection_exact(b):
     v = O.make_view(spin=0.0, r_cam=r_cam, theta_cam=math.pi / 2, disk_enabled=0)
     out = O.trace_params(v, OPTS, [[b, 0.0]])
     assert out["cls"][0] == O.ESCAPED
-    inward, outward, _ = _schwarzschild_sweep(b, r_cam)
+    inward, outward, _ = Qd.schwarzschild_sweep(b, r_cam)
     want = (inward + outward) % (2 * math.pi)
     assert abs(out["phi"][0] - want) < 1e-9, out["phi"][0] - want
     assert abs(out["theta"][0] - math.pi / 2) < 1e-12
@@ -156,20 +140,7 @@ def test_disk_hit_radius_and_redshift_exact(b):
     beta = b: it meets the far side of the equatorial disk where the swept angle is pi.
     r_hit against the quadrature; g = sqrt(1 - 3/r) for L = 0 Keplerian gas."""
     r_cam = 30.0
-    inward, outward, r_min = _schwarzschild_sweep(b, r_cam)
-    assert inward + outward > math.pi
-    u0 = 1.0 / r_min
-
-    def g(u):
-        return (u + u0) - 2.0 * (u * u + u * u0 + u0 * u0)
-
-    f = lambda t: 2.0 / math.sqrt(g(u0 - t * t))
-    need = math.pi - inward
-    assert need > 0
-    # swept angle from periapsis out to u = u0 - t^2 is the integral of f over [0, t]
-    t_hit = optimize.brentq(lambda t: integrate.quad(f, 0.0, t, epsabs=1e-14, epsrel=1e-13)[0] - need, 0.0,
-                            math.sqrt(u0), xtol=1e-15)
-    r_want = 1.0 / (u0 - t_hit * t_hit)
+    r_want = Qd.schwarzschild_pole_disk_hit(b, r_cam)
     # the crossing lies just past periapsis, inside r_ISCO = 6: the disk reaches in to 3.05 M
     v = O.make_view(spin=0.0, r_cam=r_cam, theta_cam=math.pi / 2, disk_enabled=1, disk_inner=3.05, disk_outer=40.0)
     out = O.trace_params(v, OPTS, [[0.0, b]])
@@ -187,36 +158,13 @@ def test_kerr_disk_hit_radius_exact(alpha, beta):
     r_hit from that equation (quadrature + root) to 1e-9 relative; dropping the spin term
     of Theta moves it by ~3e-3.  g is the Keplerian prograde closed form at r_hit."""
     a, r_cam, inc = 0.9, 50.0, math.radians(60.0)
-    Lz = -alpha * math.sin(inc)
-    Q = beta * beta + (alpha * alpha - a * a) * math.cos(inc) ** 2
-
-    def theta_pot(t):
-        return Q - math.cos(t) ** 2 * (Lz * Lz / math.sin(t) ** 2 - a * a)
-
-    def radial_pot(r):
-        return (r * r + a * a - a * Lz) ** 2 - (r * r - 2.0 * r + a * a) * (Q + (Lz - a) ** 2)
-
-    assert min(theta_pot(t) for t in np.linspace(inc, math.pi / 2, 2001)) > 0
-    lam = integrate.quad(lambda t: 1.0 / math.sqrt(theta_pot(t)), inc, math.pi / 2, epsabs=1e-15, epsrel=1e-13,
-                         limit=400)[0]
-    roots = np.roots([1.0, 0.0, a * a - Lz * Lz - Q, 2.0 * (Q + (Lz - a) ** 2), -a * a * Q])
-    r_turn = max([x.real for x in roots if abs(x.imag) < 1e-10 and x.real < r_cam] + [L.ko_horizon_radius(a)])
-
-    def mino_gap(r):
-        return integrate.quad(lambda x: 1.0 / math.sqrt(radial_pot(x)), r, r_cam, epsabs=1e-15, epsrel=1e-13,
-                              limit=400)[0] - lam
-
-    lo = r_turn + 1e-3
-    assert mino_gap(lo) > 0  # the equator comes before the radial turning point
-    r_want = optimize.brentq(mino_gap, lo, r_cam, xtol=1e-14, rtol=1e-15)
+    r_want = Qd.kerr_disk_hit(a, r_cam, inc, alpha, beta)
     v = O.make_view(spin=a, r_cam=r_cam, theta_cam=inc, disk_enabled=1, disk_inner=1.5, disk_outer=100.0)
     out = O.trace_params(v, OPTS, [[alpha, beta]])
     assert out["cls"][0] == O.DISK, out
     r = out["r_hit"][0]
     assert r == pytest.approx(r_want, rel=1e-9)
-    om = 1.0 / (r ** 1.5 + a)
-    g_want = math.sqrt((1 - 2 / r) + 4 * a / r * om - (r * r + a * a + 2 * a * a / r) * om * om) / (1 - Lz * om)
-    assert out["g"][0] == pytest.approx(g_want, rel=1e-12)
+    assert out["g"][0] == pytest.approx(Qd.keplerian_g(a, r, -alpha * math.sin(inc)), rel=1e-12)
 
 
 def test_equatorial_critical_impact_parameters():
@@ -377,30 +325,6 @@ def test_step_budget_exhaustion_is_a_class_not_an_error():
     assert res["cls"][0] == O.UNFINISHED and res["steps"][0] == 10
 
 
-def _kerr_equatorial_sweep(a, L, r_cam):
-    """Angle swept by an equatorial (Q = 0, E = 1) photon of angular momentum L around a
-    hole of spin a, from r_cam in to periapsis and out to infinity.  With u = 1/r the
-    orbit equations give dphi = F(u) du / sqrt(U(u)):
-        U(u) = 1 + (a^2 - L^2) u^2 + 2 (L - a)^2 u^3          (u^4 R(1/u))
-        F(u) = L - a + a (1 + a (a - L) u^2) / (1 - 2 u + a^2 u^2)   (L - aE + a P/Delta)
-    and u = u0 - t^2 removes the inverse square root at periapsis u0, with
-    U(u) = (u0 - u) V(u), V(u) = -(c3 (u^2 + u u0 + u0^2) + c2 (u + u0))."""
-    c3, c2 = 2.0 * (L - a) ** 2, a * a - L * L
-    roots = np.roots([c3, c2, 0.0, 1.0])
-    u0 = min(x.real for x in roots if abs(x.imag) < 1e-12 and x.real > 0)
-
-    def f(t):
-        u = u0 - t * t
-        V = -(c3 * (u * u + u * u0 + u0 * u0) + c2 * (u + u0))
-        F = L - a + a * (1.0 + a * (a - L) * u * u) / (1.0 - 2.0 * u + a * a * u * u)
-        return 2.0 * F / math.sqrt(V)
-
-    uc = 1.0 / r_cam
-    inward = integrate.quad(f, 0.0, math.sqrt(u0 - uc), epsabs=1e-14, epsrel=1e-13, limit=400)[0]
-    outward = integrate.quad(f, 0.0, math.sqrt(u0), epsabs=1e-14, epsrel=1e-13, limit=400)[0]
-    return inward, outward, 1.0 / u0
-
-
 @pytest.mark.parametrize("alpha", [-4.0, -8.0, 8.0, 12.0])
 def test_kerr_equatorial_deflection_exact(alpha):
     """a = 0.9, edge-on Bardeen camera, beta = 0 (Q = 0, L = -alpha): the sky phi equals the
@@ -408,12 +332,10 @@ def test_kerr_equatorial_deflection_exact(alpha):
     prograde (L = 4, 8) and the retrograde side (L = -8, -12) of the critical impact
     parameters (2.844 and -6.834); frame dragging makes the two sides differ."""
     a, r_cam = 0.9, 50.0
-    L = -alpha
     v = O.make_view(spin=a, r_cam=r_cam, theta_cam=math.pi / 2, disk_enabled=0)
     out = O.trace_params(v, OPTS, [[alpha, 0.0]])
     assert out["cls"][0] == O.ESCAPED
-    inward, outward, r_p = _kerr_equatorial_sweep(a, L, r_cam)
-    want = (-(inward + outward)) % (2 * math.pi)
+    want = Qd.kerr_equatorial_phi(a, alpha, r_cam)
     d = (out["phi"][0] - want + math.pi) % (2 * math.pi) - math.pi
-    assert abs(d) < 1e-9, (out["phi"][0], want, r_p)
+    assert abs(d) < 1e-9, (out["phi"][0], want)
     assert abs(out["theta"][0] - math.pi / 2) < 1e-12
