@@ -1,7 +1,7 @@
 """The FP64 oracle against what the mathematics fixes: the reference's own theta-form
 equations (chain rule), finite differences of the potentials, conserved quantities,
-exact Schwarzschild and Kerr quadratures (quadratures.py), the critical curve and the reference's known answers
-(test_kerr.cpp testShadow, testNearAxisPhotons, testEht)."""
+exact Schwarzschild and Kerr quadratures (quadratures.py), the critical curve and the
+reference's known answers (test_kerr.cpp testShadow, testNearAxisPhotons, testEht)."""
 import math
 
 import numpy as np
