@@ -195,6 +195,20 @@ const char* check_view(const kt_view& v) {
     if (v.camera_kind != KT_CAMERA_BARDEEN && v.camera_kind != KT_CAMERA_PINHOLE &&
         v.camera_kind != KT_CAMERA_OBSERVER)
         return "unknown camera_kind";
+    // every real field finite: a NaN or infinity would reach the rays unnoticed (NaN
+    // fails no range test written as !(x in range), but phi, the angles and the frame have
+    // no range to test)
+    const double scalars[] = {v.spin, v.r_cam, v.theta_cam, v.phi_cam, v.half_width, v.half_height,
+                              v.yaw, v.pitch, v.roll, v.fov_deg, v.disk_inner, v.disk_outer,
+                              v.escape_radius, v.capture_factor};
+    for (double x : scalars)
+        if (!std::isfinite(x)) return "view fields must be finite";
+    for (double x : v.u_obs)
+        if (!std::isfinite(x)) return "u_obs must be finite";
+    for (double x : v.drift)
+        if (!std::isfinite(x)) return "drift must be finite";
+    for (double x : v.frame)
+        if (!std::isfinite(x)) return "frame must be finite";
     if (v.width <= 0 || v.height <= 0) return "image size must be positive";
     if ((long long)v.width * v.height > (1ll << 31)) return "view larger than 2^31 pixels";
     if (!(std::fabs(v.spin) <= 1.0)) return "spin must lie in [-1, 1] (naked singularities are not traced)";

@@ -76,7 +76,9 @@ __device__ __forceinline__ float3 image_sky(const float* img, int W, int H, floa
     const float lon = atan2f(dy, dx);
     const float lat = asinf(fminf(fmaxf(dz, -1.f), 1.f));
     // texel centres at (i + 0.5) / W; longitude wraps, latitude clamps
-    float x = (lon / (2.f * pi) + 0.5f) * W - 0.5f;
+    // clamped before the integer conversion (a NaN direction gives texel 0, not an
+    // undefined index)
+    float x = fminf(fmaxf((lon / (2.f * pi) + 0.5f) * W - 0.5f, -0.5f), (float)W - 0.5f);
     float y = (lat / pi + 0.5f) * H - 0.5f;
     y = fminf(fmaxf(y, 0.f), (float)(H - 1));
     const float fx = floorf(x), fy = floorf(y);

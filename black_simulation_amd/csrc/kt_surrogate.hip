@@ -34,8 +34,7 @@ struct Table {
 
 __host__ __device__ inline void cell(double x, double x0, double x1, int n, int& i, float& f) {
     double u = (x - x0) / (x1 - x0) * (n - 1);
-    if (u < 0.0) u = 0.0;
-    if (u > n - 1) u = n - 1;
+    u = fmin(fmax(u, 0.0), (double)(n - 1));  // NaN -> 0: the index stays in the table
     i = (int)u;
     if (i > n - 2) i = n - 2;
     f = (float)(u - i);

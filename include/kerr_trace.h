@@ -88,7 +88,8 @@ enum { KT_MODE_PERSISTENT = 0, KT_MODE_ONESHOT = 1, KT_MODE_AUTO = 2 /* the tune
  * reach the horizon from the far side are captured at once (src/trajectory.cpp:234-241),
  * the others are followed out through it.  For observer views phi_cam is the Kerr-Schild
  * phi, as in the simulator; they trace in FP64 only and cannot share a call with other
- * kinds.  Disk fields follow trajectory::RayScene (src/trajectory.h:74-83). */
+ * kinds.  Disk fields follow trajectory::RayScene (src/trajectory.h:74-83).  Every real
+ * field must be finite: a NaN or infinity anywhere is KT_EINVAL. */
 typedef struct kt_view {
     int camera_kind;
     int width, height;

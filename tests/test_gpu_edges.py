@@ -62,6 +62,26 @@ def test_one_pixel_and_one_row_views(tracer):
     same(tracer, bsa.View(width=200, height=1, spin=0.9, r_cam=30.0, half_width=12.0, half_height=0.06), what="200x1")
 
 
+def test_non_finite_view_fields_are_refused(tracer):
+    """NaN or infinity in any real field of kt_view is KT_EINVAL with a message, also in the
+    fields no range check covers (phi, yaw, the observer's frame)"""
+    lib = K.lib()
+    o = bsa.TraceOpts().to_c()
+    out = K.KtRaysOut()
+    for field, value in (("phi_cam", math.nan), ("yaw", math.inf), ("spin", math.nan), ("r_cam", math.inf),
+                         ("escape_radius", math.nan), ("disk_inner", -math.inf)):
+        v = bsa.View(width=4, height=4)
+        setattr(v, field, value)
+        assert lib.kt_trace_views(tracer._ctx, C.byref(v.to_c()), 1, C.byref(o), C.byref(out), None) == K.KT_EINVAL
+        assert b"finite" in lib.kt_last_error_string(tracer._ctx), field
+    fr = [float(i % 5 == 0) for i in range(16)]
+    fr[5] = math.nan
+    v = bsa.View(camera_kind=bsa.CAMERA_OBSERVER, width=4, height=4, spin=0.9, r_cam=10.0, theta_cam=1.2,
+                 frame=tuple(fr))
+    assert lib.kt_trace_views(tracer._ctx, C.byref(v.to_c()), 1, C.byref(o), C.byref(out), None) == K.KT_EINVAL
+    assert b"frame must be finite" in lib.kt_last_error_string(tracer._ctx)
+
+
 def test_empty_calls_are_refused_or_do_nothing(tracer):
     lib = K.lib()
     out = K.KtRaysOut()
