@@ -1062,6 +1062,10 @@ constexpr size_t kEhtImage = kte::kSumPartials + 8;
 constexpr size_t kEhtDoubles = kEhtImage + (size_t)kCalibrateSize * kCalibrateSize;
 
 const char* check_eht(const kt_eht_view& v, const kt_eht_flow& f) {
+    const double reals[] = {v.a, v.inclination, v.half_width, v.frequency, v.mass_solar,
+                            f.n0, f.T0, f.beta, f.keplerian, f.height, f.sin_theta_b};
+    for (double x : reals)
+        if (!std::isfinite(x)) return "view and flow fields must be finite";
     if (!(std::fabs(v.a) < 1.0)) return "spin must be in (-1, 1)";
     if (!(v.inclination >= 0.0 && v.inclination <= kPi)) return "inclination must be in [0, pi]";
     if (!(v.half_width > 0.0) || !std::isfinite(v.half_width)) return "half_width must be positive";

@@ -174,6 +174,14 @@ def test_edge_on_image_is_mirror_symmetric_at_full_size():
     assert shadow.mean() > 0.05 and lit.mean() > 0.05, (shadow.mean(), lit.mean())
 
 
+def test_non_finite_view_or_flow_is_refused():
+    """an infinite frequency or a NaN flow parameter is KT_EINVAL, not an image of NaNs"""
+    with pytest.raises(RuntimeError, match="must be finite"):
+        eht.render(eht.EhtView(size=8, frequency=math.inf))
+    with pytest.raises(RuntimeError, match="must be finite"):
+        eht.render(eht.EhtView(size=8), eht.Flow(n0=math.nan))
+
+
 def test_cancel_flag_stops_between_slices():
     """eht::render's and eht::calibrate's cancel (src/eht.cpp:330-331, 446): a flag clear
     throughout gives the uncancelled image bit for bit (slices trace the same rays); a flag
