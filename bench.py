@@ -101,7 +101,7 @@ def cpu_baseline(view, seconds_target: float = 15.0, more=()) -> dict:
             if dt >= seconds_target:
                 break
         return {"value": round(n / dt / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
-                "sample": f"{used} whole {view.width}x{view.height} frames spread over the workload ({n} rays, "
+                "sample": f"{used} whole {view.width}x{view.height} frames of the workload ({n} rays, "
                           f"{dt:.1f} s, FP64 DP5 rtol 1e-10, {cores} threads)"}
     t0 = time.perf_counter()
     out = O.render(ov, opts, threads=cores, stride=stride)
@@ -591,7 +591,10 @@ def main():
     tr.close()
     if not args.no_cpu:
         spread = sorted(set(frames))
-        more = [frame_views[k][0] for k in spread[:: max(1, len(spread) // 8)]] if args.workload == "orbit" else []
+        # whole frames when one is too short a sample: spread over the orbit, or the one
+        # frame of the c2 workload again
+        more = ([frame_views[k][0] for k in spread[:: max(1, len(spread) // 8)]] if args.workload == "orbit" else
+                [frame_views[0][0]] * 8 if args.workload == "c2" else [])
         rec["cpu_baseline"] = cpu_baseline(frame_views[frames[len(frames) // 2]][0], more=more)
         if args.workload == "infall":
             rec["cpu_reference"] = infall_cpu_reference(frame_views[frames[len(frames) // 2]][0])
