@@ -377,7 +377,7 @@ def main():
     import black_simulation_amd as bsa
     from black_simulation_amd import orbit as O
     from black_simulation_amd.configs import C5_FRAMES, c2_view, c3_views, c5_view, infall_view
-    from black_simulation_amd.train_surrogate import DEFAULT_WEIGHTS, load_params
+    from black_simulation_amd.train_surrogate import DEFAULT_WEIGHTS, load_into
 
     width = args.height * 16 // 9
     if args.workload == "orbit":
@@ -407,7 +407,7 @@ def main():
     else:
         opts = tune(bsa.TraceOpts(precision=bsa.PREC_MIXED, mode=mode))
         if args.path == "surrogate":
-            tr.surrogate_load(load_params(args.weights or DEFAULT_WEIGHTS))
+            load_into(tr, args.weights or DEFAULT_WEIGHTS)
             opts.use_surrogate = 1
 
     # the chosen path, timed
@@ -484,6 +484,7 @@ def main():
     acc = None
     if ref is not orb:
         agree, n_rays, answered = 0, 0, 0
+        counts = {}
         defl, defl_a, g_rel, g_rel_a, r_rel, r_rel_a = [], [], [], [], [], []
         for k in sorted(set(frames)):
             mine = orb.render(k)
@@ -494,6 +495,10 @@ def main():
             agree += int(e["agree"].sum())
             answered += int(e["answered"].sum())
             n_rays += rays
+            for c, d in e["counts"].items():
+                t = counts.setdefault(c, {"rays": 0, "answered": 0, "answered_agree": 0})
+                for f in t:
+                    t[f] += d[f]
             defl.append(e["defl"])
             defl_a.append(e["defl_answered"])
             g_rel.append(e["g_rel"])
@@ -501,14 +506,27 @@ def main():
             r_rel.append(e["r_rel"])
             r_rel_a.append(e["r_rel_answered"])
         q = lambda xs: [float("%.3g" % v) for v in O.quantiles(torch.cat(xs))]  # noqa: E731
+        n_answered_esc = counts["escaped"]["answered"]
         acc = {"frames": len(set(frames)), "class_agreement": round(agree / n_rays, 6),
                "answered_fraction": round(answered / n_rays, 4),
+               # the frames' class mix (FP64 classes) and, per class, how many rays the network
+               # answered and how many of those it gave the right class
+               "rays_by_class": counts,
+               "answered_class_agreement": round(sum(c["answered_agree"] for c in counts.values())
+                                                 / max(1, answered), 6),
                "deflection_rad_p50_p99_p999": q(defl), "deflection_rad_answered_p50_p99_p999": q(defl_a),
                "g_rel_p50_p99_p999": q(g_rel), "g_rel_answered_p50_p99_p999": q(g_rel_a),
                "r_hit_rel_p50_p99_p999": q(r_rel), "r_hit_rel_answered_p50_p99_p999": q(r_rel_a)}
-        d50, d999 = acc["deflection_rad_p50_p99_p999"][0], acc["deflection_rad_p50_p99_p999"][2]
-        acc["target_met"] = bool(value >= 20 * value_p and d50 <= 1e-4 and d999 <= 1e-3
-                                 and acc["class_agreement"] >= 0.999)
+        # the target judges what the surrogate itself answered: its escaped rays' deflection
+        # (none answered: not met), all rays' classes, and the speed against plain FP64
+        if args.path == "surrogate":
+            d50, _, d999 = acc["deflection_rad_answered_p50_p99_p999"]
+            acc["target_met"] = bool(n_answered_esc > 0 and value >= 20 * value_p and d50 <= 1e-4
+                                     and d999 <= 1e-3 and acc["class_agreement"] >= 0.999)
+        else:
+            d50, _, d999 = acc["deflection_rad_p50_p99_p999"]
+            acc["target_met"] = bool(value >= 20 * value_p and d50 <= 1e-4 and d999 <= 1e-3
+                                     and acc["class_agreement"] >= 0.999)
 
     # roofline of the dominant kernel group (HIP-event times of the attribution pass)
     avg_integ = sum(integ_ms) / n_attr

@@ -127,6 +127,12 @@ class KtSurrogateOpts(C.Structure):
     ]
 
 
+class KtSurrogateDomain(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("spin_min", "spin_max", "inclination_min", "inclination_max",
+                                          "r_cam_min", "r_cam_max", "extent", "disk_outer", "escape_radius",
+                                          "capture_factor")]
+
+
 class KtSurrogateReport(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("datagen_seconds", "train_seconds", "eval_seconds", "first_loss",
                                           "final_loss", "train_gemm_tflops")] + [("steps", C.c_longlong)] + \
@@ -198,6 +204,9 @@ SIGNATURES = {
     "kt_surrogate_adam_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int]),
     "kt_surrogate_table_radius": (C.c_double, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
     "kt_surrogate_shadow_radius": (C.c_double, [C.c_void_p, C.c_double, C.c_double]),
+    "kt_surrogate_domain_default": (None, [_P(KtSurrogateDomain)]),
+    "kt_surrogate_get_domain": (C.c_int, [C.c_void_p, _P(KtSurrogateDomain)]),
+    "kt_surrogate_set_domain": (C.c_int, [C.c_void_p, _P(KtSurrogateDomain)]),
     "kt_shade_opts_default": (None, [_P(KtShadeOpts)]),
     "kt_shade_views": (C.c_int, [C.c_void_p, _P(KtView), C.c_int, _P(KtRaysOut), _P(KtShadeOpts), C.c_void_p,
                                  C.c_void_p]),

@@ -328,6 +328,22 @@ class Tracer:
         self.torch.cuda.synchronize()
         _check(self._ctx, K.lib().kt_surrogate_adam_step(self._ctx, grads.data_ptr(), lr, t), "kt_surrogate_adam_step")
 
+    def surrogate_domain(self) -> dict:
+        """What the loaded network covers (kt_surrogate_get_domain)."""
+        d = K.KtSurrogateDomain()
+        _check(self._ctx, K.lib().kt_surrogate_get_domain(self._ctx, C.byref(d)), "kt_surrogate_get_domain")
+        return {f: getattr(d, f) for f, _ in K.KtSurrogateDomain._fields_}
+
+    def surrogate_set_domain(self, **kw):
+        """Changes fields of the network's domain (kt_surrogate_set_domain)."""
+        d = K.KtSurrogateDomain()
+        _check(self._ctx, K.lib().kt_surrogate_get_domain(self._ctx, C.byref(d)), "kt_surrogate_get_domain")
+        for k, v in kw.items():
+            if k not in dict(K.KtSurrogateDomain._fields_):
+                raise KeyError(k)
+            setattr(d, k, float(v))
+        _check(self._ctx, K.lib().kt_surrogate_set_domain(self._ctx, C.byref(d)), "kt_surrogate_set_domain")
+
     def surrogate_shadow_radius(self, a: float, inclination: float) -> float:
         return K.lib().kt_surrogate_shadow_radius(self._ctx, a, inclination)
 

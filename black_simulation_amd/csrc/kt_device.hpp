@@ -50,6 +50,7 @@ enum : int { kFlagRestep = 0x10, kFlagSurrogate = 0x20, kFlagNonfinite = 0x40 };
 struct ViewDev {
     long long offset;  // first ray of this view in the batch
     int kind, width, height, disk, prograde, n_params;
+    int sur;  // 1: the view lies in the surrogate's domain (set only for surrogate calls)
     double a, r_cam, phi_cam;
     double half_width, half_height, px, py;  // Bardeen plane, pixel pitch
     double sinI, cosI, m0;                   // sin/cos of inclination, cos of the clamped theta

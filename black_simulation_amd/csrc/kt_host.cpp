@@ -380,6 +380,39 @@ void fill_defaults(const kt_trace_opts* in, kt_trace_opts& o) {
     if (!(o.gate_ring > 0.0)) o.gate_ring = d.gate_ring;
 }
 
+// 1 when a view lies in the surrogate's domain: a Bardeen plane of the trained scene (disk
+// on, prograde, from the ISCO to disk_outer; escape radius and capture factor as trained)
+// with spin, inclination and camera radius inside the trained ranges
+int surrogate_covers(const kts::Domain& d, const ViewDev& v) {
+    if (v.kind != KT_CAMERA_BARDEEN || !v.disk || !v.prograde) return 0;
+    const auto near = [](double x, double y) { return std::fabs(x - y) <= 1e-12 * std::fabs(y); };
+    const double inc = std::atan2(v.sinI, v.cosI);
+    if (!(v.a >= d.spin_min && v.a <= d.spin_max)) return 0;
+    if (!(inc >= d.inc_min && inc <= d.inc_max)) return 0;
+    if (!(v.r_cam >= d.r_cam_min && v.r_cam <= d.r_cam_max)) return 0;
+    if (!near(v.r_in, isco(v.a, 1)) || !near(v.r_out, d.disk_outer)) return 0;
+    if (!near(v.r_cap, d.capture_factor * horizon(v.a))) return 0;
+    if (!near(v.r_esc, std::max(d.escape_radius, 1.5 * v.r_cam))) return 0;
+    return 1;
+}
+
+void mark_coverage(const kt_ctx* ctx, std::vector<ViewDev>& vd) {
+    const kts::Domain d = kts::get_domain(ctx->surrogate);
+    for (ViewDev& v : vd) v.sur = surrogate_covers(d, v);
+}
+
+// the launch-shape and precision fields a caller may set; null when valid
+const char* check_opts(const kt_trace_opts& o) {
+    if (o.precision < KT_PREC_FP64 || o.precision > KT_PREC_FP32) return "unknown precision";
+    if (o.mode < KT_MODE_PERSISTENT || o.mode > KT_MODE_AUTO) return "unknown mode";
+    // the kernels are built for at most 256 threads per block and count in whole waves
+    if (o.block != 0 && (o.block < 64 || o.block > 256 || o.block % 64 != 0))
+        return "block must be 0 (tuned) or 64, 128, 192 or 256";
+    if (o.waves_per_cu < 0 || o.steps_per_dequeue < 0) return "negative launch shape";
+    if (o.waves_per_simd < 0 || o.waves_per_simd > 4) return "waves_per_simd must be 0 (tuned) to 4";
+    return nullptr;
+}
+
 int ensure_stage(kt_ctx* ctx, size_t bytes) {
     if (bytes <= ctx->stage_cap) return KT_OK;
     if (ctx->d_stage) (void)hipFree(ctx->d_stage);
@@ -468,6 +501,8 @@ kti::TraceLaunch make_launch(kt_ctx* ctx, const ViewDev* d_views, int n_views, l
 
 int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const kt_trace_opts& o,
                  const kt_rays_out* out, hipStream_t stream) {
+    if (const char* why = check_opts(o)) return fail(ctx, KT_EINVAL, why);
+    if (o.use_surrogate && ctx->surrogate) mark_coverage(ctx, vd);
     KT_HIP(hipSetDevice(ctx->device));
     if (ensure_views(ctx, (int)vd.size()) != KT_OK) return KT_ENOMEM;
 
@@ -686,7 +721,6 @@ int kt_trace_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
         if (!views || n_views <= 0 || !out) return fail(ctx, KT_EINVAL, "views/out missing");
         kt_trace_opts o;
         fill_defaults(opts, o);
-        if (o.precision < 0 || o.precision > 2) return fail(ctx, KT_EINVAL, "unknown precision");
         std::vector<ViewDev> vd;
         long long total = 0;
         if (int st = prepare_views(ctx, views, n_views, vd, total)) return st;
@@ -744,7 +778,7 @@ int kt_autotune(kt_ctx* ctx, const kt_view* views, int n_views, kt_trace_opts* o
     try {
         kt_trace_opts o;
         fill_defaults(opts, o);
-        if (o.precision < 0 || o.precision > 2) return fail(ctx, KT_EINVAL, "unknown precision");
+        if (const char* why = check_opts(o)) return fail(ctx, KT_EINVAL, why);
         o.mode = KT_MODE_PERSISTENT;
         o.use_surrogate = 0;
         std::vector<ViewDev> vd;
@@ -841,11 +875,13 @@ int kt_plan_create(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
         if (!out || !out->on_device) return fail(ctx, KT_EINVAL, "a plan needs device output pointers");
         kt_trace_opts o;
         fill_defaults(opts, o);
+        if (const char* why = check_opts(o)) return fail(ctx, KT_EINVAL, why);
         if (o.use_surrogate && !kts::ready(ctx->surrogate))
             return fail(ctx, KT_EUNSUPPORTED, "use_surrogate set but no surrogate is loaded");
         std::vector<ViewDev> vd;
         long long total = 0;
         if (int st = prepare_views(ctx, views, n_views, vd, total)) return st;
+        if (o.use_surrogate) mark_coverage(ctx, vd);
         KT_HIP(hipSetDevice(ctx->device));
         kt_plan* p = new kt_plan();
         p->ctx = ctx;
@@ -1469,6 +1505,62 @@ double kt_surrogate_table_radius(kt_ctx* ctx, double a, double inclination, doub
 double kt_surrogate_shadow_radius(kt_ctx* ctx, double a, double inclination) {
     if (!ctx || ensure_surrogate(ctx) != KT_OK) return 0.0;
     return kts::table_shadow_radius(ctx->surrogate, a, inclination);
+}
+
+void kt_surrogate_domain_default(kt_surrogate_domain* o) {
+    if (!o) return;
+    const kts::Domain d;
+    o->spin_min = d.spin_min;
+    o->spin_max = d.spin_max;
+    o->inclination_min = d.inc_min;
+    o->inclination_max = d.inc_max;
+    o->r_cam_min = d.r_cam_min;
+    o->r_cam_max = d.r_cam_max;
+    o->extent = d.extent;
+    o->disk_outer = d.disk_outer;
+    o->escape_radius = d.escape_radius;
+    o->capture_factor = d.capture_factor;
+}
+
+int kt_surrogate_get_domain(kt_ctx* ctx, kt_surrogate_domain* o) {
+    if (!ctx || !o) return KT_EINVAL;
+    if (int st = ensure_surrogate(ctx)) return st;
+    const kts::Domain d = kts::get_domain(ctx->surrogate);
+    o->spin_min = d.spin_min;
+    o->spin_max = d.spin_max;
+    o->inclination_min = d.inc_min;
+    o->inclination_max = d.inc_max;
+    o->r_cam_min = d.r_cam_min;
+    o->r_cam_max = d.r_cam_max;
+    o->extent = d.extent;
+    o->disk_outer = d.disk_outer;
+    o->escape_radius = d.escape_radius;
+    o->capture_factor = d.capture_factor;
+    return KT_OK;
+}
+
+int kt_surrogate_set_domain(kt_ctx* ctx, const kt_surrogate_domain* o) {
+    if (!ctx || !o) return KT_EINVAL;
+    const bool ok = o->spin_min >= 0.0 && o->spin_min <= o->spin_max && o->spin_max <= 1.0 &&
+                    o->inclination_min >= 0.0 && o->inclination_min <= o->inclination_max &&
+                    o->inclination_max <= kPi && o->r_cam_min > 0.0 && o->r_cam_min <= o->r_cam_max &&
+                    o->extent > 0.0 && o->extent <= 1.5 && o->disk_outer > 0.0 && o->escape_radius > 0.0 &&
+                    o->capture_factor >= 1.0;
+    if (!ok) return fail(ctx, KT_EINVAL, "invalid surrogate domain (ranges reversed, or extent beyond 1.5)");
+    if (int st = ensure_surrogate(ctx)) return st;
+    kts::Domain d;
+    d.spin_min = o->spin_min;
+    d.spin_max = o->spin_max;
+    d.inc_min = o->inclination_min;
+    d.inc_max = o->inclination_max;
+    d.r_cam_min = o->r_cam_min;
+    d.r_cam_max = o->r_cam_max;
+    d.extent = o->extent;
+    d.disk_outer = o->disk_outer;
+    d.escape_radius = o->escape_radius;
+    d.capture_factor = o->capture_factor;
+    kts::set_domain(ctx->surrogate, d);
+    return KT_OK;
 }
 
 int kt_observer_frame(double a, double r, double theta, const double u[4], double e[16]) {

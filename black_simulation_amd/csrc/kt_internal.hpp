@@ -32,7 +32,4 @@ struct TraceLaunch {
 // launches the counter reset and the trace kernel on `stream`; returns a hipError_t
 hipError_t launch_trace(const TraceLaunch& L, hipStream_t stream);
 
-// registers per lane etc. of the trace kernels, for the autotuner (0 if unknown)
-int trace_kernel_vgprs(int precision);
-
 }  // namespace kti

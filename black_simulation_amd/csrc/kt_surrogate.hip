@@ -291,7 +291,7 @@ __device__ __forceinline__ bool ray_params(const ViewDev* views, int n_views, lo
 }
 
 // ray_params in float (the fused inference kernel): false for rays the surrogate does not
-// cover (pinhole cameras)
+// cover (cameras other than Bardeen planes, views outside the network's domain)
 __device__ __forceinline__ bool ray_params_f(const ViewDev* views, int n_views, long long q, RayF& p) {
     const ViewDev& v = views[find_view(views, n_views, q)];
     const long long local = q - v.offset;
@@ -307,7 +307,7 @@ __device__ __forceinline__ bool ray_params_f(const ViewDev* views, int n_views, 
         p.cp = 1.f;
         return true;
     }
-    if (v.kind != 0) return false;
+    if (v.kind != 0 || !v.sur) return false;
     p.a = (float)v.a;
     p.r_cam = (float)v.r_cam;
     p.si = (float)v.sinI;
@@ -508,6 +508,7 @@ struct FusedArgs {
     // outputs: raw network outputs (n x 8), or the gate's decisions
     float* y;
     float gate_prob, gate_ring;
+    float extent;          // answered rays have |alpha|, |beta| <= extent x 1.5 R_sh
     ktd::Outputs out;
     long long* fallback;
     unsigned long long* n_fallback;
@@ -565,7 +566,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     float rg = -1.f;  // relative distance from the critical curve; < 0: not coverable
     if (VIEWS) {
         RayF p;
-        if (ray_params_f(a.views, a.n_views, qc, p)) features_f(a.t.rho, a.t.rsh, p, f, &rg);
+        if (ray_params_f(a.views, a.n_views, qc, p)) {
+            features_f(a.t.rho, a.t.rsh, p, f, &rg);
+            // outside the image-plane square the network was trained on: not coverable
+            if (!(fabsf(f[3]) <= a.extent) || !(fabsf(f[4]) <= a.extent)) rg = -1.f;
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < 5; ++k) f[k] = a.xf[qc * 5 + k];
@@ -759,6 +764,7 @@ struct Surrogate {
     float* fold = nullptr;    // input layer and biases times -log2(e) for the fused kernel
     uint16_t* wpk0 = nullptr; // the input layer's split-bf16 MFMA fragments
     Queue queue;              // fallback queue of eager calls
+    Domain dom;               // what the parameters cover
     static constexpr int kSplits = 64, kChunks = 256;
 };
 
@@ -880,6 +886,12 @@ void destroy(Surrogate* s) {
 }
 
 bool ready(const Surrogate* s) { return s && s->have_params; }
+
+Domain get_domain(const Surrogate* s) { return s ? s->dom : Domain(); }
+
+void set_domain(Surrogate* s, const Domain& d) {
+    if (s) s->dom = d;
+}
 
 double table_critical_radius(const Surrogate* s, double a, double inc, double psi) {
     return table_rho(s->h_rho.data(), a, inc, psi);
@@ -1305,6 +1317,11 @@ hipError_t train(Surrogate* s, const TrainOpts& o, TrainReport& rep) {
     using clk = std::chrono::steady_clock;
     KTS_TRY(hipSetDevice(s->device));
     if (!s->have_params) KTS_TRY(init_params(s, o.seed ^ 0x1234567ull));
+    // the ranges draw_rays samples and the scene the labels come from
+    s->dom = Domain();
+    s->dom.disk_outer = o.disk_outer;
+    s->dom.escape_radius = o.escape_radius;
+    s->dom.capture_factor = o.capture_factor;
     // data: the training set labelled by the mixed integrator, the held-out set by FP64
     auto t0 = clk::now();
     Dataset tr, ho;
@@ -1421,13 +1438,19 @@ hipError_t trace_with_surrogate(Surrogate* s, const kti::TraceLaunch& L, double 
     a.n_views = L.n_views;
     a.gate_prob = (float)gate_prob;
     a.gate_ring = (float)gate_ring;
+    a.extent = (float)(std::min(s->dom.extent, 1.5) / 1.5);
     a.out = L.out;
     a.fallback = Q.list;
     a.n_fallback = Q.count;
     hipLaunchKernelGGL(k_mlp_fused<true>, dim3((unsigned)((n + kFusedRays - 1) / kFusedRays)), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(64), 0, st, Q.count, Q.total, (unsigned long long*)nullptr);
     KTS_TRY(hipGetLastError());
-    // the integrator on the queue the gate left (count read on the device)
+    // the integrator on the queue the gate left (count read on the device).  The one-shot
+    // form's grid covers the whole batch and the blocks past the queue's end leave at once;
+    // the rays queued are the slowest of the frame (photon ring, class boundaries), so the
+    // launch is bound by their latency either way, and the persistent form measured slower
+    // on the 1080p orbit (0.42 against 0.35 ms per frame: its periodic refill costs and
+    // the few rays leave nothing to refill)
     kti::TraceLaunch F = L;
     F.ray_list = Q.list;
     F.total_dev = Q.total;

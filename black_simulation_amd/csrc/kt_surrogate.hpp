@@ -39,6 +39,16 @@ struct TrainOpts {
     int verbose = 0;            // progress lines on stderr
 };
 
+// What a trained network covers (the ranges its training rays are drawn from and the scene
+// its labels come from).  Views or rays outside it go to the integrator.
+struct Domain {
+    double spin_min = 0.0, spin_max = 0.998;
+    double inc_min = 5.0 * 3.14159265358979323846 / 180.0, inc_max = 85.0 * 3.14159265358979323846 / 180.0;
+    double r_cam_min = 10.0, r_cam_max = 1000.0;
+    double extent = 1.5;  // |alpha|, |beta| <= extent x the mean shadow radius (at most 1.5)
+    double disk_outer = 20.0, escape_radius = 1000.0, capture_factor = 1.01;
+};
+
 struct TrainReport {
     double datagen_seconds = 0, train_seconds = 0, eval_seconds = 0;
     double first_loss = 0, final_loss = 0;
@@ -69,6 +79,10 @@ hipError_t gradients(Surrogate* s, const float* feat, const int8_t* cls, const f
 hipError_t adam_step(Surrogate* s, const float* grads, double lr, int t, hipStream_t stream);
 
 hipError_t train(Surrogate* s, const TrainOpts& o, TrainReport& rep);
+
+// the network's domain: set by train() from its options, else the default training's
+Domain get_domain(const Surrogate* s);
+void set_domain(Surrogate* s, const Domain& d);
 
 // The fallback queue the gate fills: ray indices and their count (device).  Calls that
 // share a queue must not overlap; a plan owns its own so that plans can run concurrently.

@@ -34,6 +34,13 @@ __device__ __forceinline__ const ViewDev* stage_views(const kti::TraceLaunch& L,
     return sv;
 }
 
+// sum of x over the 64 lanes of the wave (all of them must execute the call)
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    return x;
+}
+
 __device__ __forceinline__ void launch_of(const ViewDev* views, int n_views, long long idx, Launch& la) {
     const ViewDev& v = views[find_view(views, n_views, idx)];
     launch_photon(v, idx - v.offset, la);
@@ -162,21 +169,32 @@ __global__ __launch_bounds__(256) void k_trace_oneshot(kti::TraceLaunch L) {
     __shared__ ViewDev sv[kti::kMaxLdsViews];
     const ViewDev* views = stage_views(L, sv);
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (L.total_dev ? *L.total_dev : L.total)) return;
-    const long long my = L.ray_list ? L.ray_list[q] : q;
-    typename Pol::LaneT s;
-    Launch la;
-    launch_of(views, L.n_views, my, la);
-    Pol::start(s, la);
-    Result r;
-    while (!Pol::step(s, L, r)) {
+    const long long total = L.total_dev ? *L.total_dev : L.total;
+    if ((long long)blockIdx.x * blockDim.x >= total) return;  // the whole block is past the end
+    unsigned long long st = 0, s64 = 0;
+    if (q < total) {
+        const long long my = L.ray_list ? L.ray_list[q] : q;
+        typename Pol::LaneT s;
+        Launch la;
+        launch_of(views, L.n_views, my, la);
+        Pol::start(s, la);
+        Result r;
+        while (!Pol::step(s, L, r)) {
+        }
+        const int f64 = Pol::fp64_steps(s);
+        write_result(L.out, my, r, Pol::steps(s), f64 > 0 ? kFlagRestep : 0);
+        st = (unsigned long long)Pol::steps(s);
+        s64 = (unsigned long long)f64;
     }
-    const int f64 = Pol::fp64_steps(s);
-    write_result(L.out, my, r, Pol::steps(s), f64 > 0 ? kFlagRestep : 0);
-    atomicAdd(&L.stats[kti::kStatSteps], (unsigned long long)Pol::steps(s));
-    if (f64) {
-        atomicAdd(&L.stats[kti::kStatFp64Steps], (unsigned long long)f64);
-        atomicAdd(&L.stats[kti::kStatRestepRays], 1ull);
+    // the counters, summed over the wave first: one atomic per wave each (every lane of
+    // the wave reaches this point; lanes past the end add 0)
+    const unsigned long long nre = __popcll(__ballot(s64 > 0));
+    st = wave_sum(st);
+    s64 = wave_sum(s64);
+    if ((threadIdx.x & 63u) == 0u) {
+        if (st) atomicAdd(&L.stats[kti::kStatSteps], st);
+        if (s64) atomicAdd(&L.stats[kti::kStatFp64Steps], s64);
+        if (nre) atomicAdd(&L.stats[kti::kStatRestepRays], nre);
     }
 }
 
@@ -218,7 +236,5 @@ hipError_t launch_trace(const TraceLaunch& L, hipStream_t stream) {
     }
     return hipGetLastError();
 }
-
-int trace_kernel_vgprs(int) { return 0; }
 
 }  // namespace kti

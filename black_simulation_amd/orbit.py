@@ -67,7 +67,9 @@ def errors(out: dict, ref: dict) -> dict:
     Returns device tensors: `agree` (class match, bool per ray), `answered` (ray carries the
     surrogate flag), and for rays whose class matches: `defl` (angle between the two sky
     directions, rad) and `defl_answered` for escaped rays, `g_rel` / `r_rel` (relative
-    errors of g and r_hit) and `*_answered` for disk rays."""
+    errors of g and r_hit) and `*_answered` for disk rays.  `counts` (host ints) holds, per
+    class of the reference, its rays, how many of them the surrogate answered and how many
+    of those it gave the reference's class."""
     c = out["cls"].to(dtype=ref["cls"].dtype) & 7
     cr = ref["cls"] & 7
     ans = (out["cls"].to(dtype=ref["cls"].dtype) & K.FLAG_SURROGATE) != 0
@@ -82,8 +84,14 @@ def errors(out: dict, ref: dict) -> dict:
     disk = agree & (cr == K.DISK)
     g_rel = (out["g"][disk] / ref["g"][disk] - 1.0).abs()
     r_rel = (out["r_hit"][disk] / ref["r_hit"][disk] - 1.0).abs()
+    counts = {}
+    for name, k in (("escaped", K.ESCAPED), ("captured", K.CAPTURED), ("disk", K.DISK), ("unfinished", K.UNFINISHED)):
+        of = cr == k
+        counts[name] = {"rays": int(of.sum()), "answered": int((of & ans).sum()),
+                        "answered_agree": int((of & ans & agree).sum())}
     return {"agree": agree, "answered": ans, "defl": defl, "defl_answered": defl[ans[esc]],
-            "g_rel": g_rel, "g_rel_answered": g_rel[ans[disk]], "r_rel": r_rel, "r_rel_answered": r_rel[ans[disk]]}
+            "g_rel": g_rel, "g_rel_answered": g_rel[ans[disk]], "r_rel": r_rel, "r_rel_answered": r_rel[ans[disk]],
+            "counts": counts}
 
 
 def quantiles(x, qs=(0.5, 0.99, 0.999)) -> list:

@@ -33,6 +33,22 @@ def load_params(path: str = DEFAULT_WEIGHTS) -> np.ndarray:
     return p
 
 
+def load_into(tracer, path: str = DEFAULT_WEIGHTS):
+    """Loads PATH.f32 into `tracer` and sets the network's domain to the one PATH.json
+    records (the scene it was trained on); a network without that record keeps the
+    default training's domain."""
+    tracer.surrogate_load(load_params(path))
+    meta = path + ".json"
+    if os.path.exists(meta):
+        with open(meta) as f:
+            rec = json.load(f)
+        dom = rec.get("domain")
+        if dom is None:  # written before the domain was recorded: the scene from its options
+            o = rec.get("opts", {})
+            dom = {k: o[k] for k in ("disk_outer", "escape_radius", "capture_factor") if k in o}
+        tracer.surrogate_set_domain(**dom)
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
@@ -57,6 +73,7 @@ def main(argv=None) -> int:
     rep = tr.surrogate_train(**kw)
     rep["wall_seconds"] = time.perf_counter() - t0
     params = tr.surrogate_save()
+    domain = tr.surrogate_domain()
     tr.close()
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     params.astype("<f4").tofile(args.out + ".f32")
@@ -66,7 +83,7 @@ def main(argv=None) -> int:
         setattr(o, k, v)
     with open(args.out + ".json", "w") as f:
         json.dump({"opts": {n: getattr(o, n) for n, _ in K.KtSurrogateOpts._fields_},
-                   "init_seed": args.init_seed, "report": rep}, f, indent=1)
+                   "init_seed": args.init_seed, "domain": domain, "report": rep}, f, indent=1)
     print(json.dumps(rep), flush=True)
     return 0
 

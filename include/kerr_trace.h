@@ -264,6 +264,25 @@ int kt_surrogate_adam_step(kt_ctx* ctx, const float* grads, double lr, int t);
 double kt_surrogate_table_radius(kt_ctx* ctx, double a, double inclination, double psi);
 double kt_surrogate_shadow_radius(kt_ctx* ctx, double a, double inclination);
 
+/* What the loaded network covers.  With use_surrogate = 1 every ray of a view outside it
+ * goes to the integrator: cameras other than Bardeen planes; a spin, inclination or camera
+ * radius outside the ranges; a disk that is disabled, retrograde, starts anywhere but the
+ * ISCO or ends anywhere but disk_outer; another escape radius or capture factor.  Per ray,
+ * |alpha| or |beta| beyond `extent` mean shadow radii goes there too.  kt_surrogate_train
+ * sets the domain to the ranges it draws from and the scene of its options; a network given
+ * by kt_surrogate_load or kt_surrogate_init keeps the current one (the default training's
+ * until set), so a caller loading parameters trained on another scene sets it here. */
+typedef struct kt_surrogate_domain {
+    double spin_min, spin_max;                /* 0, 0.998                                  */
+    double inclination_min, inclination_max;  /* rad: 5 deg, 85 deg                        */
+    double r_cam_min, r_cam_max;              /* 10, 1000                                  */
+    double extent;                            /* 1.5 (at most: the inputs' normalisation)  */
+    double disk_outer, escape_radius, capture_factor;  /* 20, 1000, 1.01               */
+} kt_surrogate_domain;
+void kt_surrogate_domain_default(kt_surrogate_domain* domain);
+int kt_surrogate_get_domain(kt_ctx* ctx, kt_surrogate_domain* domain);
+int kt_surrogate_set_domain(kt_ctx* ctx, const kt_surrogate_domain* domain);
+
 /* ---- HDR colour of traced rays ---------------------------------------------------
  * Replaces the colouring at the end of the simulator's trace (shaders/trace.frag:
  * diskEmission 373-403, shiftedBlackbody 330-335, skyColor/gridSky/imageSky 1419-1458,

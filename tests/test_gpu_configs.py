@@ -1,0 +1,157 @@
+"""The batch and orbit workloads against the FP64 oracle on the GPU, at reduced image size
+with their launch shapes kept.
+
+* The 64 seeded uneven views (spin U[0, 0.998], inclination U[5, 85] deg, r_cam
+  10^U[1, 3], half-width 1.5 mean shadow radii) in ONE launch on the mixed FP32/FP64
+  path, each view's size divided by 16 (8..128 pixels across instead of 128..2048).
+* Frames of the a = 0.998 orbit (r = 40 M, inclination 20 -> 85 deg) through the
+  surrogate, its gate and the integrator fallback, as the frame loop runs them.
+
+Bar (as for the single frame): the same class as the oracle for >= 99.9 % of rays, a
+mismatch being excused only where the oracle's own class changes when alpha or beta moves
+by 1e-6 M; the per-output errors are measured and printed, with ceilings that catch a
+broken path.  The surrogate's answered rays are judged separately, per class."""
+import math
+
+import numpy as np
+import pytest
+
+import black_simulation_amd as bsa
+from black_simulation_amd.configs import C5_FRAMES, c3_views, c5_view
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("camera_kind", "width", "height", "spin", "r_cam", "theta_cam", "phi_cam", "half_width", "half_height",
+          "yaw", "pitch", "roll", "fov_deg", "disk_inner", "disk_outer", "disk_prograde", "disk_enabled",
+          "escape_radius", "capture_factor")
+
+
+def oracle_view(v):
+    return O.make_view(**{k: getattr(v, k) for k in FIELDS})
+
+
+def plane(v, idx):
+    al = -v.half_width + (idx % v.width + 0.5) * (2 * v.half_width / v.width)
+    be = v.half_height - (idx // v.width + 0.5) * (2 * v.half_height / v.height)
+    return al, be
+
+
+def boundary(v, idx, cls_want):
+    """which of the rays `idx` of view v change their oracle class under a 1e-6 M move"""
+    ov = oracle_view(v)
+    al, be = plane(v, np.asarray(idx))
+    near = np.zeros(len(idx), bool)
+    for da, db in ((1e-6, 0), (-1e-6, 0), (0, 1e-6), (0, -1e-6)):
+        moved = O.trace_params(ov, O.make_opts(), np.stack([al + da, be + db], axis=1))
+        near |= moved["cls"] != cls_want
+    return near
+
+
+def sky_angle(t1, p1, t2, p2):
+    v1 = np.stack([np.sin(t1) * np.cos(p1), np.sin(t1) * np.sin(p1), np.cos(t1)])
+    v2 = np.stack([np.sin(t2) * np.cos(p2), np.sin(t2) * np.sin(p2), np.cos(t2)])
+    return 2 * np.arcsin(np.minimum(1, np.linalg.norm(v1 - v2, axis=0) / 2))
+
+
+def q3(x):
+    return np.quantile(x, [0.5, 0.99, 0.999]) if len(x) else np.full(3, np.nan)
+
+
+def compare(views, got, fp64_fallback_exact=False):
+    """Class agreement (boundary-excused) and per-class errors of `got` (one launch over
+    `views`) against the oracle's render of each view."""
+    off = 0
+    tot = dict(rays=0, agree=0, excused=0)
+    defl, g_rel, r_rel = [], [], []
+    ans = dict(rays=0, agree=0, defl=[], g_rel=[], r_rel=[], by_class=np.zeros(4, int))
+    for v in views:
+        n = v.rays
+        want = O.render(oracle_view(v), O.make_opts())
+        cls = got["cls"][off:off + n] & 7
+        answered = (got["cls"][off:off + n] & bsa.FLAG_SURROGATE) != 0
+        miss = np.nonzero(cls != want["cls"])[0]
+        excused = boundary(v, miss, want["cls"][miss]) if len(miss) else np.zeros(0, bool)
+        tot["rays"] += n
+        tot["agree"] += n - len(miss)
+        tot["excused"] += int(excused.sum())
+        both = cls == want["cls"]
+        esc = both & (cls == O.ESCAPED)
+        d = sky_angle(got["theta_inf"][off:off + n][esc], got["phi_inf"][off:off + n][esc], want["theta"][esc],
+                      want["phi"][esc])
+        disk = both & (cls == O.DISK)
+        gr = np.abs(got["g"][off:off + n][disk] / want["g"][disk] - 1)
+        rr = np.abs(got["r_hit"][off:off + n][disk] / want["r_hit"][disk] - 1)
+        fb = ~answered
+        defl.append(d[fb[esc]])
+        g_rel.append(gr[fb[disk]])
+        r_rel.append(rr[fb[disk]])
+        ans["rays"] += int(answered.sum())
+        ans["agree"] += int((answered & both).sum())
+        ans["defl"].append(d[answered[esc]])
+        ans["g_rel"].append(gr[answered[disk]])
+        ans["r_rel"].append(rr[answered[disk]])
+        ans["by_class"] += np.bincount(want["cls"][answered], minlength=4)[:4]
+        if fp64_fallback_exact:  # the FP64 fallback reproduces the oracle bit for bit
+            for k, w in (("steps", "steps"), ("r_hit", "r_hit"), ("g", "g")):
+                assert np.array_equal(got[k][off:off + n][fb], want[w][fb], equal_nan=True), k
+            assert np.array_equal(cls[fb], want["cls"][fb])
+        off += n
+    cat = lambda xs: np.concatenate(xs) if xs else np.zeros(0)  # noqa: E731
+    for k in ("defl", "g_rel", "r_rel"):
+        ans[k] = cat(ans[k])
+    return tot, cat(defl), cat(g_rel), cat(r_rel), ans
+
+
+def test_c3_batch_of_64_views_in_one_mixed_launch(tracer):
+    """All 64 seeded views of the batch workload at 1/16 of their size, one mixed-precision
+    launch (one work queue for all views, persistent kernel)."""
+    views = c3_views(sizes=(8, 16, 32, 64, 128))
+    assert len(views) == 64
+    opts = bsa.TraceOpts(precision=bsa.PREC_MIXED, mode=bsa.MODE_PERSISTENT)
+    got = bsa.to_numpy(tracer.trace_views(views, opts))
+    st = tracer.stats()
+    assert st["rays"] == sum(v.rays for v in views)
+    tot, d, g, r, _ = compare(views, got)
+    agree = (tot["agree"] + tot["excused"]) / tot["rays"]
+    print(f"C3 at 1/16 size: {tot['rays']} rays in 64 views; class agreement {tot['agree'] / tot['rays']:.6f} "
+          f"({tot['rays'] - tot['agree']} differ, {tot['excused']} of them within 1e-6 of a boundary); "
+          f"deflection p50/p99/p99.9 {q3(d)} rad; g rel {q3(g)}; r_hit rel {q3(r)}; "
+          f"FP64 attempts {st['fp64_steps']} of {st['steps']}", flush=True)
+    assert agree >= 0.999
+    assert np.median(d) < 1e-5 and np.quantile(d, 0.99) < 1e-3
+    assert np.quantile(g, 0.99) < 1e-4 and np.quantile(r, 0.99) < 1e-4
+
+
+@pytest.mark.parametrize("precision", [bsa.PREC_MIXED, bsa.PREC_FP64])
+def test_c5_orbit_frames_surrogate_and_fallback(tracer, precision):
+    """Four frames spread over the a = 0.998 orbit (1/10 size, 192x108) through the network,
+    the gate and the integrator fallback in one launch each.  All rays: the same class as
+    the oracle (boundary-excused) for >= 99.9 %.  Fallback rays: with the FP64 fallback
+    bit-equal to the oracle, with the mixed one within its stated errors.  Answered rays
+    (the network's own outputs): class agreement >= 99.9 %, and their errors against the
+    oracle printed per class with ceilings at a few times what the shipped network gives."""
+    from black_simulation_amd.train_surrogate import load_params
+
+    tracer.surrogate_load(load_params())
+    frames = [0, 40, 80, C5_FRAMES - 1]
+    views = [c5_view(k, 192, 108) for k in frames]
+    opts = bsa.TraceOpts(precision=precision, use_surrogate=1)
+    got = bsa.to_numpy(tracer.trace_views(views, opts))
+    tot, d, g, r, ans = compare(views, got, fp64_fallback_exact=precision == bsa.PREC_FP64)
+    agree = (tot["agree"] + tot["excused"]) / tot["rays"]
+    a_agree = ans["agree"] / max(1, ans["rays"])
+    print(f"C5 frames {frames} at 192x108, fallback {'mixed' if precision == bsa.PREC_MIXED else 'FP64'}: "
+          f"class agreement {tot['agree'] / tot['rays']:.6f} ({tot['excused']} boundary rays excused); "
+          f"answered {ans['rays']} of {tot['rays']} (escaped, captured, disk, unfinished: {ans['by_class'].tolist()}), "
+          f"their class agreement {a_agree:.6f}; answered deflection p50/p99/p99.9 {q3(ans['defl'])} rad, "
+          f"g rel {q3(ans['g_rel'])}, r_hit rel {q3(ans['r_rel'])}; fallback deflection {q3(d)} rad, "
+          f"g rel {q3(g)}, r_hit rel {q3(r)}", flush=True)
+    assert agree >= 0.999
+    assert ans["rays"] > 0.5 * tot["rays"] and a_agree >= 0.999
+    assert ans["by_class"][O.DISK] > 0 and ans["by_class"][O.CAPTURED] > 0
+    if len(ans["defl"]):
+        assert np.median(ans["defl"]) < 0.1
+    assert np.median(ans["g_rel"]) < 0.05 and np.median(ans["r_rel"]) < 0.05
+    if precision == bsa.PREC_MIXED:
+        assert np.median(d) < 1e-4 and np.quantile(g, 0.99) < 1e-3 and np.quantile(r, 0.99) < 1e-3
