@@ -185,6 +185,8 @@ SIGNATURES = {
     "kt_last_error_string": (C.c_char_p, [C.c_void_p]),
     "kt_last_stats": (C.c_int, [C.c_void_p, _P(KtStats)]),
     "kt_trace_views": (C.c_int, [C.c_void_p, _P(KtView), C.c_int, _P(KtTraceOpts), _P(KtRaysOut), C.c_void_p]),
+    "kt_trace_views_noalloc": (C.c_int, [C.c_void_p, _P(KtView), C.c_int, _P(KtTraceOpts), _P(KtRaysOut),
+                                         C.c_void_p]),
     "kt_trace_rays": (C.c_int, [C.c_void_p, _P(KtView), C.c_void_p, C.c_longlong, _P(KtTraceOpts), _P(KtRaysOut),
                                 C.c_void_p]),
     "kt_autotune": (C.c_int, [C.c_void_p, _P(KtView), C.c_int, _P(KtTraceOpts)]),

@@ -323,6 +323,7 @@ struct kt_ctx {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timed = false;
     kts::Surrogate* surrogate = nullptr;
+    kti::Scratch scratch{};  // the no-allocation entry point's fixed device arrays (per device)
     float* d_blackbody = nullptr;  // shading: the blackbody colour table
     std::vector<float> h_blackbody;
     double* d_eht = nullptr;  // EHT: flux partials, the sum, calibrate's 48 x 48 image
@@ -670,6 +671,12 @@ int kt_create(const kt_device_opts* opts, kt_ctx** out) {
             return KT_EHIP;
         }
     }
+    // the device address of this GPU's fixed scratch (kt_trace_views_noalloc); the device
+    // is current here
+    if (kti::scratch_setup(ctx->device, &ctx->scratch) != hipSuccess) {
+        kt_destroy(ctx);
+        return KT_EHIP;
+    }
     *out = ctx;
     return KT_OK;
 }
@@ -725,6 +732,68 @@ int kt_trace_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
         long long total = 0;
         if (int st = prepare_views(ctx, views, n_views, vd, total)) return st;
         return trace_common(ctx, vd, total, o, out, static_cast<hipStream_t>(hip_stream));
+    } catch (const std::exception& e) {
+        return fail(ctx, KT_EINVAL, e.what());
+    } catch (...) {
+        return fail(ctx, KT_EINVAL, "unknown error");
+    }
+}
+
+static_assert(KT_NOALLOC_MAX_VIEWS == kti::kMaxLdsViews, "the scratch holds the LDS-staged view table");
+
+int kt_trace_views_noalloc(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trace_opts* opts,
+                           const kt_rays_out* out, void* hip_stream) {
+    if (!ctx) return KT_EINVAL;
+    try {
+        if (!views || n_views <= 0 || !out) return fail(ctx, KT_EINVAL, "views/out missing");
+        if (!ctx->scratch.views) return fail(ctx, KT_EHIP, "no device scratch on this context's GPU");
+        if (n_views > ctx->scratch.max_views)
+            return fail(ctx, KT_EINVAL, "kt_trace_views_noalloc takes at most KT_NOALLOC_MAX_VIEWS (64) views");
+        if (!out->on_device)
+            return fail(ctx, KT_EINVAL, "kt_trace_views_noalloc writes device outputs only (on_device = 1)");
+        kt_trace_opts o;
+        fill_defaults(opts, o);
+        if (const char* why = check_opts(o)) return fail(ctx, KT_EINVAL, why);
+        if (o.use_surrogate)
+            return fail(ctx, KT_EINVAL, "kt_trace_views_noalloc does not run the surrogate (its queue is allocated)");
+        // the view table on the stack: no allocation on the way to the kernels
+        ViewDev vd[kti::kMaxLdsViews];
+        long long total = 0;
+        int n_obs = 0;
+        for (int i = 0; i < n_views; ++i) {
+            if (const char* why = check_view(views[i])) return fail(ctx, KT_EINVAL, why);
+            vd[i] = make_view_dev(views[i], total);
+            total += (long long)views[i].width * views[i].height;
+            n_obs += views[i].camera_kind == KT_CAMERA_OBSERVER ? 1 : 0;
+        }
+        if (n_obs && n_obs != n_views) return fail(ctx, KT_EINVAL, "observer cameras cannot share a call with other kinds");
+        if (n_obs && o.precision != KT_PREC_FP64) return fail(ctx, KT_EINVAL, "observer cameras trace in FP64 (KT_PREC_FP64)");
+        hipStream_t stream = static_cast<hipStream_t>(hip_stream);
+        KT_HIP(hipSetDevice(ctx->device));
+        // a pageable source: the copy has read vd before the call returns
+        KT_HIP(hipMemcpyAsync(ctx->scratch.views, vd, sizeof(ViewDev) * (size_t)n_views, hipMemcpyHostToDevice,
+                              stream));
+        ktd::Outputs dout{out->cls, out->theta_inf, out->phi_inf, out->r_hit, out->g, out->steps,
+                          out->theta_inf_f32, out->phi_inf_f32, out->r_hit_f32, out->g_f32};
+        kti::TraceLaunch L = make_launch(ctx, ctx->scratch.views, n_views, total, ctx->scratch.counter, dout, o);
+        L.ks = n_obs ? 1 : 0;
+        ctx->stats = kt_stats{};
+        ctx->stats.rays = total;
+        ctx->stats.block = L.block;
+        ctx->stats.grid = L.grid;
+        ctx->stats.steps_per_dequeue = L.steps_per_dequeue;
+        ctx->stats.waves_per_simd = L.waves_per_simd;
+        ctx->stats.mode = L.mode;
+        ctx->stats.waves_per_cu = (L.grid * L.block / 64) / std::max(1, ctx->cu_count);
+        KT_HIP(hipEventRecord(ctx->ev[0], stream));
+        KT_HIP(hipEventRecord(ctx->ev[1], stream));
+        hipError_t e = kti::launch_trace(L, stream);  // the counters are cleared by a kernel
+        if (e != hipSuccess) return hip_fail(ctx, e, "trace kernel launch");
+        KT_HIP(hipEventRecord(ctx->ev[2], stream));
+        KT_HIP(hipEventRecord(ctx->ev[3], stream));
+        ctx->timed = true;
+        ctx->stats_src = ctx->scratch.counter;
+        return KT_OK;
     } catch (const std::exception& e) {
         return fail(ctx, KT_EINVAL, e.what());
     } catch (...) {

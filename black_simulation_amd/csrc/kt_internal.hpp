@@ -32,4 +32,16 @@ struct TraceLaunch {
 // launches the counter reset and the trace kernel on `stream`; returns a hipError_t
 hipError_t launch_trace(const TraceLaunch& L, hipStream_t stream);
 
+// The fixed device scratch of the no-allocation entry point, on one GPU: the counters
+// (queue head + kStatCount) and a view table of max_views views.
+struct Scratch {
+    unsigned long long* counter = nullptr;
+    ktd::ViewDev* views = nullptr;
+    int max_views = 0;
+};
+// Resolves the scratch's device address on `device`, which must be the current device
+// (once per device; later calls return the same address).  Allocates nothing.
+hipError_t scratch_setup(int device, Scratch* out);
+size_t scratch_bytes();
+
 }  // namespace kti

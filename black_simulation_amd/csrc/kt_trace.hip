@@ -10,6 +10,8 @@
 // The per-view table is staged in LDS when it fits (<= 64 views).
 #include "kt_internal.hpp"
 
+#include <mutex>
+
 using namespace ktd;
 
 namespace {
@@ -200,7 +202,49 @@ __global__ __launch_bounds__(256) void k_trace_oneshot(kti::TraceLaunch L) {
 
 }  // namespace
 
+// ---- the scratch of the entry point that allocates nothing (kt_trace_views_noalloc) --------
+// The work-queue head and counters, then a view table of kMaxLdsViews views, in a fixed
+// device array: each GPU the library runs on holds one copy until the process ends.  Host
+// code reaches it only through the address hipGetSymbolAddress gives per device.
+namespace {
+
+constexpr size_t kScratchCounterBytes = 256;
+constexpr size_t kScratchBytes = kScratchCounterBytes + sizeof(ViewDev) * kti::kMaxLdsViews;
+static_assert(sizeof(unsigned long long) * (1 + kti::kStatCount) <= kScratchCounterBytes, "counters fit");
+
+constexpr int kScratchDevices = 64;
+std::mutex scratch_mu;
+void* scratch_at[kScratchDevices] = {};  // per device ordinal; null until set up
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace kti {
+
+size_t scratch_bytes() { return kScratchBytes; }
+
+hipError_t scratch_setup(int device, Scratch* out) {
+    *out = Scratch{};
+    if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+    int current = -1;
+    hipError_t e = hipGetDevice(&current);
+    if (e != hipSuccess) return e;
+    if (current != device) return hipErrorInvalidDevice;  // the caller makes `device` current
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_at[device]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        scratch_at[device] = p;
+    }
+    char* base = static_cast<char*>(scratch_at[device]);
+    out->counter = reinterpret_cast<unsigned long long*>(base);
+    out->views = reinterpret_cast<ViewDev*>(base + kScratchCounterBytes);
+    out->max_views = kMaxLdsViews;
+    return hipSuccess;
+}
 
 template <class Pol>
 void launch_persistent(const TraceLaunch& L, hipStream_t stream) {

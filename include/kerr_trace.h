@@ -184,6 +184,19 @@ int kt_last_stats(const kt_ctx* ctx, kt_stats* stats);
 int kt_trace_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trace_opts* opts,
                    const kt_rays_out* out, void* hip_stream);
 
+/* kt_trace_views for callers that cannot allocate: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  No counterpart in the reference.  It
+ * calls no allocating routine: the view table and the work-queue counters live in a fixed
+ * device array of each GPU (about 26 KB: KT_NOALLOC_MAX_VIEWS views), whose address
+ * kt_create resolves and which is held until the process ends.  Limits, each refused with
+ * KT_EINVAL: at most KT_NOALLOC_MAX_VIEWS views, outputs on the device (on_device = 1), no
+ * surrogate (its fallback queue is allocated).  The array is shared by every call on one
+ * GPU, from any context: calls that overlap on one GPU race and must be serialised by the
+ * caller.  Results equal kt_trace_views' for the same arguments; kt_last_stats reports it. */
+#define KT_NOALLOC_MAX_VIEWS 64
+int kt_trace_views_noalloc(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trace_opts* opts,
+                           const kt_rays_out* out, void* hip_stream);
+
 /* Explicit rays of one view: for a Bardeen view `params` holds n (alpha, beta) pairs;
  * for a pinhole view n unit look vectors (x, y, z) along the frame's (r, theta, phi)
  * axes (the `look` of kerr::photonFromFrame, src/kerr.cpp:458).  params is a host
