@@ -2,7 +2,7 @@
 integrator fallback against the plain FP64 integrator on the same frames.
 
 Workload (default): the 120-frame camera orbit at r = 40 M (phi_k = 2 pi k / 120,
-inclination 20 -> 85 deg), Bardeen plane 24 M wide, disk r_ISCO..20 M, escape 1000 M,
+inclination 20 -> 85 deg), Bardeen plane +-1.5 mean shadow radii, disk r_ISCO..20 M, escape 1000 M,
 capture 1.01 r+.  One step = one full frame (2,073,600 rays), replayed from a hipGraph
 plan.  --path surrogate: the MLP answers rays with max class probability > 0.999 lying
 outside +-3% of the critical curve, the mixed FP32/FP64 integrator the rest, all in
@@ -61,7 +61,10 @@ def profiled_traffic(workload, size, kernel):
             return v
     return None, None
 # The network 5-256-256-256-256-8: 2 (5*256 + 3*256*256 + 256*8) multiply-adds per ray
-MLP_FLOP_PER_RAY = 2 * (5 * 256 + 3 * 256 * 256 + 256 * 8)  # 399,872
+def mlp_flop_per_ray(hidden: int) -> int:
+    """multiply-adds x 2 of the network 5 -> H -> H -> H -> H -> 8: 399,872 at H = 256 (the
+    wide network), 101,632 at H = 128 (the fast one)"""
+    return 2 * (5 * hidden + 3 * hidden * hidden + hidden * 8)
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 BF16_MFMA_PEAK_TFLOPS = 2500.0
@@ -342,6 +345,9 @@ def main():
     ap.add_argument("--mode", choices=["auto", "oneshot", "persistent"], default="auto",
                     help="integrator kernel (auto: the autotuner's pick, or persistent with --lean)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rtol", type=float, default=0.0,
+                    help="DP5 tolerance of the chosen path (FP64 rtol for --path plain, the FP32 bulk's for mixed "
+                         "and surrogate; atol = rtol / 10); the reference render stays FP64 at rtol 1e-10")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the tracer runs on a single MI355X (rays are independent; how a "
@@ -404,11 +410,15 @@ def main():
     plain = tune(bsa.TraceOpts(precision=bsa.PREC_FP64, mode=mode))
     if args.path == "plain":
         opts = plain
+        if args.rtol:  # FP64 at another tolerance, judged against FP64 at rtol 1e-10
+            opts = tune(bsa.TraceOpts(precision=bsa.PREC_FP64, mode=mode, rtol=args.rtol, atol=args.rtol / 10))
     else:
-        opts = tune(bsa.TraceOpts(precision=bsa.PREC_MIXED, mode=mode))
+        opts = tune(bsa.TraceOpts(precision=bsa.PREC_MIXED, mode=mode,
+                                  **({"rtol32": args.rtol, "atol32": args.rtol / 10} if args.rtol else {})))
         if args.path == "surrogate":
             load_into(tr, args.weights or DEFAULT_WEIGHTS)
             opts.use_surrogate = 1
+    tr_net = tr.surrogate_net() if args.path == "surrogate" else None
 
     # the chosen path, timed
     orb = O.Orbit(tr, views, opts, streams=args.streams)
@@ -416,7 +426,8 @@ def main():
     wall_frames = len(frames)
     value = rays * wall_frames / wall / 1e6
     # the plain FP64 integrator on the same frames, timed the same way
-    ref = O.Orbit(tr, views, plain, streams=args.streams) if args.path != "plain" and not args.lean else orb
+    ref = O.Orbit(tr, views, plain, streams=args.streams) if (args.path != "plain" or args.rtol) and not args.lean \
+        else orb
     if ref is not orb:
         wall_p, lat_p = timed_frames(ref, warm, frames, torch)
     else:
@@ -549,18 +560,22 @@ def main():
     roof = roof_integ
     roof_mlp = None
     if args.path == "surrogate":
-        mlp_ach = rays * MLP_FLOP_PER_RAY / (avg_sur * 1e-3) / 1e12
-        tr_mlp, src_mlp = profiled_traffic(args.workload, args.height, "k_mlp_fused")
+        hidden = 128 if tr_net == bsa.NET_FAST else 256
+        flop_ray = mlp_flop_per_ray(hidden)
+        mlp_ach = rays * flop_ray / (avg_sur * 1e-3) / 1e12
+        kmlp = "k_mlp_fast" if hidden == 128 else "k_mlp_fused"
+        tr_mlp, src_mlp = profiled_traffic(args.workload, args.height, kmlp)
         roof_mlp = {"bound": "mfma", "achieved": round(mlp_ach, 2), "peak": BF16_MFMA_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(mlp_ach / BF16_MFMA_PEAK_TFLOPS, 4), "traffic": tr_mlp,
                     "traffic_source": src_mlp,
-                    "kernel": "k_mlp_fused (features, network, gate)", "kernel_ms": round(avg_sur, 3),
-                    "flop_per_ray": MLP_FLOP_PER_RAY}
+                    "kernel": f"{kmlp} (features, network, gate)", "kernel_ms": round(avg_sur, 3),
+                    "flop_per_ray": flop_ray,
+                    "network": f"5-{hidden}x4-8 " + ("hard SiLU" if hidden == 128 else "SiLU")}
         if avg_sur > avg_integ:
             roof, roof_mlp = roof_mlp, roof_integ
 
-    desc = {"orbit": f"120-frame orbit r=40 M, i 20->85 deg, a=0.998, {width}x{args.height} Bardeen plane 24 M "
-                     f"wide, disk r_ISCO..20 M",
+    desc = {"orbit": f"120-frame orbit r=40 M, i 20->85 deg, a=0.998, {width}x{args.height} Bardeen plane "
+                     f"+-1.5 mean shadow radii wide, disk r_ISCO..20 M",
             "c2": f"single frame a=0.998, r=50 M, i=85 deg, {width}x{args.height} Bardeen plane 30 M wide, "
                   f"disk r_ISCO..40 M",
             "c3": f"64 seeded views (seed 0x5EED0003) of uneven size, spin and camera radius in one launch, "

@@ -15,6 +15,7 @@ KT_OK, KT_EINVAL, KT_EHIP, KT_ECANCELLED, KT_ENOMEM, KT_EUNSUPPORTED, KT_ENODEVI
 ESCAPED, CAPTURED, DISK, UNFINISHED = 0, 1, 2, 3
 FLAG_FP64_RESTEP, FLAG_SURROGATE = 0x10, 0x20
 CAMERA_BARDEEN, CAMERA_PINHOLE, CAMERA_OBSERVER = 0, 1, 3
+NET_WIDE, NET_FAST = 0, 1
 PARTICLE_MOVING, PARTICLE_PLUNGED, PARTICLE_ESCAPED, PARTICLE_UNFINISHED = 0, 1, 2, 3
 PREC_FP64, PREC_MIXED, PREC_FP32 = 0, 1, 2
 MODE_PERSISTENT, MODE_ONESHOT, MODE_AUTO = 0, 1, 2
@@ -124,6 +125,7 @@ class KtSurrogateOpts(C.Structure):
         ("escape_radius", C.c_double),
         ("capture_factor", C.c_double),
         ("verbose", C.c_int),
+        ("net", C.c_int),
     ]
 
 
@@ -196,6 +198,9 @@ SIGNATURES = {
     "kt_plan_destroy": (C.c_int, [C.c_void_p]),
     "kt_surrogate_opts_default": (None, [_P(KtSurrogateOpts)]),
     "kt_surrogate_param_count": (C.c_longlong, []),
+    "kt_surrogate_net_param_count": (C.c_longlong, [C.c_int]),
+    "kt_surrogate_set_net": (C.c_int, [C.c_void_p, C.c_int]),
+    "kt_surrogate_get_net": (C.c_int, [C.c_void_p]),
     "kt_surrogate_init": (C.c_int, [C.c_void_p, C.c_ulonglong]),
     "kt_surrogate_load": (C.c_int, [C.c_void_p, C.c_void_p, C.c_longlong]),
     "kt_surrogate_save": (C.c_int, [C.c_void_p, C.c_void_p, C.c_longlong]),

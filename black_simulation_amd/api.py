@@ -290,8 +290,15 @@ class Tracer:
         _check(self._ctx, K.lib().kt_surrogate_train(self._ctx, C.byref(o), C.byref(r)), "kt_surrogate_train")
         return {f: getattr(r, f) for f, _ in K.KtSurrogateReport._fields_}
 
+    def surrogate_set_net(self, net: int):
+        """Chooses the network (NET_WIDE or NET_FAST); its parameters are dropped."""
+        _check(self._ctx, K.lib().kt_surrogate_set_net(self._ctx, net), "kt_surrogate_set_net")
+
+    def surrogate_net(self) -> int:
+        return K.lib().kt_surrogate_get_net(self._ctx)
+
     def surrogate_save(self) -> np.ndarray:
-        n = K.lib().kt_surrogate_param_count()
+        n = K.lib().kt_surrogate_net_param_count(self.surrogate_net())
         p = np.zeros(n, np.float32)
         _check(self._ctx, K.lib().kt_surrogate_save(self._ctx, p.ctypes.data, n), "kt_surrogate_save")
         return p
@@ -316,7 +323,7 @@ class Tracer:
         x = t.as_tensor(np.asarray(inputs, np.float32)).to(dev).contiguous()
         c = t.as_tensor(np.asarray(cls, np.int8)).to(dev).contiguous()
         y = t.as_tensor(np.asarray(targets, np.float32)).to(dev).contiguous()
-        g = t.empty(K.lib().kt_surrogate_param_count(), dtype=t.float32, device=dev)
+        g = t.empty(K.lib().kt_surrogate_net_param_count(self.surrogate_net()), dtype=t.float32, device=dev)
         loss = C.c_double()
         t.cuda.synchronize()
         _check(self._ctx, K.lib().kt_surrogate_gradients(self._ctx, x.data_ptr(), c.data_ptr(), y.data_ptr(),

@@ -739,6 +739,7 @@ int kt_trace_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
     }
 }
 
+static_assert(KT_NET_WIDE == ktn::kNetWide && KT_NET_FAST == ktn::kNetFast, "network ids");
 static_assert(KT_NOALLOC_MAX_VIEWS == kti::kMaxLdsViews, "the scratch holds the LDS-staged view table");
 
 int kt_trace_views_noalloc(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trace_opts* opts,
@@ -1467,9 +1468,28 @@ void kt_surrogate_opts_default(kt_surrogate_opts* o) {
     o->escape_radius = d.escape_radius;
     o->capture_factor = d.capture_factor;
     o->verbose = 0;
+    o->net = KT_NET_WIDE;
 }
 
-long long kt_surrogate_param_count(void) { return ktn::kParams; }
+long long kt_surrogate_param_count(void) { return ktn::param_count(ktn::net_of(KT_NET_WIDE).hidden); }
+
+long long kt_surrogate_net_param_count(int net) {
+    if (net != KT_NET_WIDE && net != KT_NET_FAST) return 0;
+    return ktn::param_count(ktn::net_of(net).hidden);
+}
+
+int kt_surrogate_set_net(kt_ctx* ctx, int net) {
+    if (!ctx) return KT_EINVAL;
+    if (net != KT_NET_WIDE && net != KT_NET_FAST) return fail(ctx, KT_EINVAL, "unknown network (KT_NET_*)");
+    if (int st = ensure_surrogate(ctx)) return st;
+    hipError_t e = kts::set_net(ctx->surrogate, net);
+    return e == hipSuccess ? KT_OK : hip_fail(ctx, e, "surrogate set_net");
+}
+
+int kt_surrogate_get_net(kt_ctx* ctx) {
+    if (!ctx || ensure_surrogate(ctx) != KT_OK) return -1;
+    return kts::get_net(ctx->surrogate);
+}
 
 int kt_surrogate_init(kt_ctx* ctx, unsigned long long seed) {
     if (!ctx) return KT_EINVAL;
@@ -1480,15 +1500,21 @@ int kt_surrogate_init(kt_ctx* ctx, unsigned long long seed) {
 
 int kt_surrogate_load(kt_ctx* ctx, const float* params, long long n) {
     if (!ctx || !params) return KT_EINVAL;
-    if (n != ktn::kParams) return fail(ctx, KT_EINVAL, "parameter count must be kt_surrogate_param_count()");
+    // the parameter count names the network
+    const int net = n == kt_surrogate_net_param_count(KT_NET_WIDE) ? KT_NET_WIDE
+                  : n == kt_surrogate_net_param_count(KT_NET_FAST) ? KT_NET_FAST : -1;
+    if (net < 0) return fail(ctx, KT_EINVAL, "parameter count must be kt_surrogate_net_param_count() of a network");
     if (int st = ensure_surrogate(ctx)) return st;
-    hipError_t e = kts::set_params(ctx->surrogate, params, n);
+    hipError_t e = kts::set_net(ctx->surrogate, net);
+    if (e == hipSuccess) e = kts::set_params(ctx->surrogate, params, n);
     return e == hipSuccess ? KT_OK : hip_fail(ctx, e, "surrogate load");
 }
 
 int kt_surrogate_save(kt_ctx* ctx, float* params, long long n) {
     if (!ctx || !params) return KT_EINVAL;
-    if (n != ktn::kParams) return fail(ctx, KT_EINVAL, "parameter count must be kt_surrogate_param_count()");
+    if (int st = ensure_surrogate(ctx)) return st;
+    if (n != kt_surrogate_net_param_count(kts::get_net(ctx->surrogate)))
+        return fail(ctx, KT_EINVAL, "parameter count must be kt_surrogate_net_param_count() of the loaded network");
     if (!kts::ready(ctx->surrogate)) return fail(ctx, KT_EUNSUPPORTED, "no surrogate parameters");
     hipError_t e = kts::get_params(ctx->surrogate, params, n);
     return e == hipSuccess ? KT_OK : hip_fail(ctx, e, "surrogate save");
@@ -1514,6 +1540,8 @@ int kt_surrogate_train(kt_ctx* ctx, const kt_surrogate_opts* opts, kt_surrogate_
         o.escape_radius = opts->escape_radius > 0 ? opts->escape_radius : o.escape_radius;
         o.capture_factor = opts->capture_factor >= 1.0 ? opts->capture_factor : o.capture_factor;
         o.verbose = opts->verbose;
+        if (opts->net != KT_NET_WIDE && opts->net != KT_NET_FAST) return fail(ctx, KT_EINVAL, "unknown network");
+        o.net = opts->net;
     }
     if (o.batch % 128 != 0) return fail(ctx, KT_EINVAL, "batch must be a multiple of 128");
     kts::TrainReport r;

@@ -39,4 +39,15 @@ __device__ __forceinline__ float dsilu(float z) {
     return s * (1.0f + z * (1.0f - s));
 }
 
+// hard SiLU, z clamp(z / 6 + 1/2, 0, 1), and its derivative (one-sided at z = +-3)
+__device__ __forceinline__ float hsilu(float z) { return z * fminf(fmaxf(z * (1.0f / 6.0f) + 0.5f, 0.f), 1.f); }
+
+__device__ __forceinline__ float dhsilu(float z) {
+    return z <= -3.0f ? 0.f : (z >= 3.0f ? 1.f : z * (1.0f / 3.0f) + 0.5f);
+}
+
+// the activation of kind `act` (0: SiLU, 1: hard SiLU) and its derivative
+__device__ __forceinline__ float activate(float z, int act) { return act ? hsilu(z) : silu(z); }
+__device__ __forceinline__ float dactivate(float z, int act) { return act ? dhsilu(z) : dsilu(z); }
+
 }  // namespace ktm

@@ -85,11 +85,11 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
                 if (EPI == kEpiFwd) {
                     const float z = v + g.bias[c];
                     if (g.out32) g.out32[o] = z;
-                    g.outb[o] = f2bf(silu(z));
+                    g.outb[o] = f2bf(activate(z, g.act));
                 } else if (EPI == kEpiLin) {
                     g.out32[o] = v + g.bias[c];
                 } else if (EPI == kEpiBwd) {
-                    const float gz = v * dsilu(g.zin[(long long)r * g.ldz + c]);
+                    const float gz = v * dactivate(g.zin[(long long)r * g.ldz + c], g.act);
                     g.out32[o] = gz;
                     g.outb[o] = f2bf(gz);
                 } else {  // kEpiWs: split-K partial
@@ -230,65 +230,71 @@ __global__ void k_gather(const float* feat, const int8_t* cls, const float* tgt,
 }
 
 // Input layer in FP32 on the vector units: z = x W0 + b0 with the master weights, then
-// SiLU rounded to bf16 for the matrix-core layers.  Thread j owns hidden unit j.
+// the activation rounded to bf16 for the matrix-core layers.  Thread j owns hidden unit j
+// (blockDim.x = H).
 constexpr int kL0Rows = 16;
-__global__ __launch_bounds__(256) void k_layer0(const float* xf, int B, const float* P, float* z0, uint16_t* h0) {
-    const int j = threadIdx.x;
+__global__ __launch_bounds__(256) void k_layer0(const float* xf, int B, const float* P, int act, float* z0,
+                                                uint16_t* h0) {
+    const int j = threadIdx.x, H = blockDim.x;
     float w[kInputs];
 #pragma unroll
-    for (int k = 0; k < kInputs; ++k) w[k] = P[k * kHidden + j];
-    const float bias = P[kInputs * kHidden + j];
+    for (int k = 0; k < kInputs; ++k) w[k] = P[k * H + j];
+    const float bias = P[kInputs * H + j];
     const int r0 = blockIdx.x * kL0Rows;
     for (int r = r0; r < r0 + kL0Rows && r < B; ++r) {
         const float* x = xf + (long long)r * kInputs;
         float z = bias;
 #pragma unroll
         for (int k = 0; k < kInputs; ++k) z = fmaf(x[k], w[k], z);
-        if (z0) z0[(long long)r * kHidden + j] = z;
-        h0[(long long)r * kHidden + j] = f2bf(silu(z));
+        if (z0) z0[(long long)r * H + j] = z;
+        h0[(long long)r * H + j] = f2bf(activate(z, act));
     }
 }
 
 // Input-layer gradients in FP32: partial[c][k][j] = sum over chunk c of x[b][k] g[b][j]
-// (k < 5) and of g[b][j] (k = 5), then the sum over chunks into G (W0 then b0).
+// (k < 5) and of g[b][j] (k = 5), then the sum over chunks into G (W0 then b0).  Thread j
+// owns hidden unit j (blockDim.x = H).
 __global__ __launch_bounds__(256) void k_layer0_grad_partial(const float* xf, const float* g, int B, int rows,
                                                              float* partial) {
-    const int j = threadIdx.x;
+    const int j = threadIdx.x, H = blockDim.x;
     const int b0 = blockIdx.x * rows;
     const int b1 = b0 + rows < B ? b0 + rows : B;
     float s[kInputs + 1] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int b = b0; b < b1; ++b) {
-        const float gj = g[(long long)b * kHidden + j];
+        const float gj = g[(long long)b * H + j];
         const float* x = xf + (long long)b * kInputs;
 #pragma unroll
         for (int k = 0; k < kInputs; ++k) s[k] = fmaf(x[k], gj, s[k]);
         s[kInputs] += gj;
     }
-    float* o = partial + (long long)blockIdx.x * (kInputs + 1) * kHidden;
+    float* o = partial + (long long)blockIdx.x * (kInputs + 1) * H;
 #pragma unroll
-    for (int k = 0; k <= kInputs; ++k) o[k * kHidden + j] = s[k];
+    for (int k = 0; k <= kInputs; ++k) o[k * H + j] = s[k];
 }
 
-__global__ void k_layer0_grad_final(const float* partial, int chunks, float* G) {
+__global__ void k_layer0_grad_final(const float* partial, int chunks, int H, float* G) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;  // (k, j), k <= 5
-    if (e >= (kInputs + 1) * kHidden) return;
+    if (e >= (kInputs + 1) * H) return;
     float s = 0.f;
-    for (int c = 0; c < chunks; ++c) s += partial[(long long)c * (kInputs + 1) * kHidden + e];
-    G[e] = s;  // W0 (5 x 256 row-major) is followed directly by b0
+    for (int c = 0; c < chunks; ++c) s += partial[(long long)c * (kInputs + 1) * H + e];
+    G[e] = s;  // W0 (5 x H row-major) is followed directly by b0
 }
 
 }  // namespace
 
-hipError_t layer0(const float* xf, int B, const float* P, float* z0, uint16_t* h0, hipStream_t s) {
-    hipLaunchKernelGGL(k_layer0, dim3((B + kL0Rows - 1) / kL0Rows), dim3(256), 0, s, xf, B, P, z0, h0);
+hipError_t layer0(const float* xf, int B, const float* P, int H, int act, float* z0, uint16_t* h0, hipStream_t s) {
+    if (H < 64 || H > kMaxHidden || H % 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_layer0, dim3((B + kL0Rows - 1) / kL0Rows), dim3(H), 0, s, xf, B, P, act, z0, h0);
     return hipGetLastError();
 }
 
-hipError_t layer0_grad(const float* xf, const float* g, int B, float* partial, int chunks, float* G, hipStream_t s) {
+hipError_t layer0_grad(const float* xf, const float* g, int B, int H, float* partial, int chunks, float* G,
+                       hipStream_t s) {
+    if (H < 64 || H > kMaxHidden || H % 64) return hipErrorInvalidValue;
     const int rows = (B + chunks - 1) / chunks;
-    hipLaunchKernelGGL(k_layer0_grad_partial, dim3(chunks), dim3(256), 0, s, xf, g, B, rows, partial);
-    hipLaunchKernelGGL(k_layer0_grad_final, dim3(((kInputs + 1) * kHidden + 255) / 256), dim3(256), 0, s, partial,
-                       chunks, G);
+    hipLaunchKernelGGL(k_layer0_grad_partial, dim3(chunks), dim3(H), 0, s, xf, g, B, rows, partial);
+    hipLaunchKernelGGL(k_layer0_grad_final, dim3(((kInputs + 1) * H + 255) / 256), dim3(256), 0, s, partial,
+                       chunks, H, G);
     return hipGetLastError();
 }
 

@@ -290,8 +290,11 @@ __device__ __forceinline__ bool ray_params(const ViewDev* views, int n_views, lo
     return true;
 }
 
-// ray_params in float (the fused inference kernel): false for rays the surrogate does not
-// cover (cameras other than Bardeen planes, views outside the network's domain)
+// ray_params in float (the fused inference kernels): false for rays the surrogate does not
+// cover (cameras other than Bardeen planes, views outside the network's domain).  The
+// camera azimuth's sine and cosine (sp, cp) only WITH_PHI: only answered escaped rays need
+// them, for their sky angles.
+template <bool WITH_PHI = true>
 __device__ __forceinline__ bool ray_params_f(const ViewDev* views, int n_views, long long q, RayF& p) {
     const ViewDev& v = views[find_view(views, n_views, q)];
     const long long local = q - v.offset;
@@ -313,12 +316,14 @@ __device__ __forceinline__ bool ray_params_f(const ViewDev* views, int n_views, 
     p.si = (float)v.sinI;
     p.ci = (float)v.cosI;
     p.inc = atan2f(p.si, p.ci);
-    sincosf((float)v.phi_cam, &p.sp, &p.cp);
+    if (WITH_PHI) sincosf((float)v.phi_cam, &p.sp, &p.cp);
     if (v.params) {
         p.alpha = (float)v.params[2 * local];
         p.beta = (float)v.params[2 * local + 1];
     } else {
-        const int row = (int)(local / v.width), col = (int)(local - (long long)row * v.width);
+        // a view holds at most 2^31 pixels: the row and column in 32 bits
+        const unsigned lu = (unsigned)local, w = (unsigned)v.width;
+        const int row = (int)(lu / w), col = (int)(lu - (unsigned)row * w);
         p.alpha = (float)(-v.half_width + ((double)col + 0.5) * v.px);
         p.beta = (float)(v.half_height - ((double)row + 0.5) * v.py);
     }
@@ -431,13 +436,14 @@ __global__ void k_copy_count(const unsigned long long* c, long long* dst, unsign
 // fh Wl over the five inputs in k-slots 0-4, 5-9 and 10-14 (slot 15 zero).  The parts carry
 // 16 significant bits and the dropped fl Wl is 2^-18 relative, so its pre-activations are
 // good to ~1e-5, far below the bf16 rounding of the activations they feed.
+constexpr int kWideH = 256;            // the wide network's hidden width
 constexpr int kFusedRays = 128;        // rays per block: 4 waves x 32
 constexpr int kSlabElems = 8192;       // bf16 per slab (16 KB): one output block, all 16 k-blocks
 constexpr int kHiddenSlabs = 3 * 8;    // layers 1-3, eight slabs each; then one slab for layer 4
 constexpr int kPackedElems = (kHiddenSlabs + 1) * kSlabElems;
 constexpr int kInputFragElems = 8 * 64 * 8;  // input layer: [fb][lane][8]
 // the scaled FP32 copies: input weights [5][256], input bias, biases of layers 1-3
-constexpr int kFoldElems = ktn::kInputs * ktn::kHidden + 4 * ktn::kHidden;
+constexpr int kFoldElems = ktn::kInputs * kWideH + 4 * kWideH;
 constexpr float kNegLog2e = -1.44269504088896341f, kNegLn2 = -0.693147180559945309f;
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -458,10 +464,10 @@ __global__ void k_pack_fused(const float* P, uint16_t* wpk) {
     float v;
     if (t < kHiddenSlabs) {
         const int l = 1 + t / 8, n = 32 * (t % 8) + (lane & 31);
-        v = P[ktn::param_offset(l) + (long long)f * ktn::kHidden + n];
+        v = P[ktn::param_offset(kWideH, l) + (long long)f * kWideH + n];
     } else {
         const int n = lane & 31;
-        v = n < ktn::kOutputs ? P[ktn::param_offset(4) + (long long)f * ktn::kOutputs + n] : 0.f;
+        v = n < ktn::kOutputs ? P[ktn::param_offset(kWideH, 4) + (long long)f * ktn::kOutputs + n] : 0.f;
     }
     wpk[e] = f2bf(v);
 }
@@ -470,10 +476,10 @@ __global__ void k_pack_fused(const float* P, uint16_t* wpk) {
 __global__ void k_fold(const float* P, float* fold) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= kFoldElems) return;
-    const int n0 = (ktn::kInputs + 1) * ktn::kHidden;  // input layer: weights then bias
+    const int n0 = (ktn::kInputs + 1) * kWideH;  // input layer: weights then bias
     const float v = e < n0 ? P[e]
-                           : P[ktn::param_offset(1 + (e - n0) / ktn::kHidden) + (long long)ktn::kHidden * ktn::kHidden +
-                               (e - n0) % ktn::kHidden];
+                           : P[ktn::param_offset(kWideH, 1 + (e - n0) / kWideH) + (long long)kWideH * kWideH +
+                               (e - n0) % kWideH];
     fold[e] = v * kNegLog2e;
 }
 
@@ -487,7 +493,7 @@ __global__ void k_pack_input(const float* fold, uint16_t* wpk0) {
     const int n = 32 * fb + (lane & 31);
     uint16_t v = 0;
     if (kk < 15) {
-        const float w = fold[(kk % 5) * ktn::kHidden + n];
+        const float w = fold[(kk % 5) * kWideH + n];
         const uint16_t hi = f2bf(w);
         v = kk < 10 ? hi : f2bf(w - bf2f(hi));
     }
@@ -505,6 +511,10 @@ struct FusedArgs {
     const float* fold;     // input layer and biases of layers 0-3 times -log2(e)
     const uint16_t* wpk0;  // the input layer's split-bf16 fragments
     const uint16_t* wpk;   // packed bf16 weights of layers 1-4
+    // the fast network (k_mlp_fast): its packed weights (hidden layers, output layer, the
+    // input layer's split fragments) and biases, all staged into LDS once per block
+    const uint4* wfast;
+    const float* bfast;
     // outputs: raw network outputs (n x 8), or the gate's decisions
     float* y;
     float gate_prob, gate_ring;
@@ -546,6 +556,53 @@ __device__ __forceinline__ f32x16 mfma32(const uint4& a, const bf16x8& b, const 
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a), b, c, 0, 0, 0);
 }
 
+// The gate and the answered rays' outputs, from the network outputs y of the ray this lane
+// holds (lane m and m + 32 hold the same ray; the lower half-wave, h = 0, decides): a ray
+// is answered when its largest class probability exceeds gate_prob and it lies outside
+// gate_ring of the critical curve (rg < 0: not coverable); the others join the fallback
+// queue, one atomic per wave.  Every lane of the wave calls it.
+__device__ __forceinline__ void gate_and_write(const FusedArgs& a, long long q, bool live, int h, int lane,
+                                               const float y[8], float rg) {
+    const float mx = fmaxf(y[0], fmaxf(y[1], y[2]));
+    const float e0 = __expf(y[0] - mx), e1 = __expf(y[1] - mx), e2 = __expf(y[2] - mx);
+    const float inv = 1.0f / (e0 + e1 + e2);
+    const float p0 = e0 * inv, p1 = e1 * inv, p2 = e2 * inv;
+    const int c = p0 >= p1 && p0 >= p2 ? 0 : (p1 >= p2 ? 1 : 2);
+    const float pm = c == 0 ? p0 : c == 1 ? p1 : p2;
+    const bool to_fallback = live && h == 0 && (!(pm > a.gate_prob) || !(rg > a.gate_ring));
+    const unsigned long long mask = __ballot(to_fallback);
+    if (mask) {
+        const int first = __ffsll((long long)mask) - 1;
+        unsigned long long base = 0;
+        if (lane == first) base = atomicAdd(a.n_fallback, (unsigned long long)__popcll(mask));
+        base = __shfl(base, first);
+        if (to_fallback) a.fallback[base + __popcll(mask & ((1ull << lane) - 1))] = q;
+    }
+    if (!live || h != 0 || to_fallback) return;
+    const double nan = __builtin_nan("");
+    double th = nan, ph = nan, rh = nan, g = nan;
+    if (c == 0) {
+        RayF p;
+        ray_params_f(a.views, a.n_views, q, p);
+        float u1[3], u2[3], u3[3];
+        ray_frame_f(p, u1, u2, u3);
+        sky_from_frame_f(u1, u2, u3, y[3], y[4], y[5], th, ph);
+    } else if (c == 2) {
+        rh = (double)__expf(y[6]);
+        g = (double)__expf(y[7]);
+    }
+    if (a.out.cls) a.out.cls[q] = (int8_t)(c | ktd::kFlagSurrogate);
+    if (a.out.theta) a.out.theta[q] = th;
+    if (a.out.phi) a.out.phi[q] = ph;
+    if (a.out.r_hit) a.out.r_hit[q] = rh;
+    if (a.out.g) a.out.g[q] = g;
+    if (a.out.steps) a.out.steps[q] = 0;
+    if (a.out.theta32) a.out.theta32[q] = (float)th;
+    if (a.out.phi32) a.out.phi32[q] = (float)ph;
+    if (a.out.r_hit32) a.out.r_hit32[q] = (float)rh;
+    if (a.out.g32) a.out.g32[q] = (float)g;
+}
+
 template <bool VIEWS>
 // two waves per SIMD: 248 VGPRs, accumulators in the VGPR file, no spills
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mlp_fused(FusedArgs a) {
@@ -566,7 +623,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     float rg = -1.f;  // relative distance from the critical curve; < 0: not coverable
     if (VIEWS) {
         RayF p;
-        if (ray_params_f(a.views, a.n_views, qc, p)) {
+        if (ray_params_f<false>(a.views, a.n_views, qc, p)) {
             features_f(a.t.rho, a.t.rsh, p, f, &rg);
             // outside the image-plane square the network was trained on: not coverable
             if (!(fabsf(f[3]) <= a.extent) || !(fabsf(f[4]) <= a.extent)) rg = -1.f;
@@ -592,7 +649,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
         for (int k = 0; k < 8; ++k) xb[k] = (__bf16)(h ? x1[k] : x0[k]);
         const uint4* a0 = reinterpret_cast<const uint4*>(a.wpk0);
-        const float* b0 = a.fold + ktn::kInputs * ktn::kHidden;
+        const float* b0 = a.fold + ktn::kInputs * kWideH;
 #pragma unroll
         for (int fb = 0; fb < 8; ++fb) {
             const f32x16 z = mfma32(a0[fb * 64 + lane], xb, block_bias(b0, fb, h));
@@ -610,7 +667,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         f32x16 acc, prev;
 #pragma unroll
         for (int l = 1; l <= 3; ++l) {
-            const float* bl = a.fold + (ktn::kInputs + l) * ktn::kHidden;
+            const float* bl = a.fold + (ktn::kInputs + l) * kWideH;
             f32x16 bias = block_bias(bl, 0, h);
 #pragma unroll
             for (int fb = 0; fb < 8; ++fb) {
@@ -662,7 +719,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
         for (int kb = 0; kb < 16; ++kb) o[kb & 3] = mfma32(S[kb * 64 + lane], B[kb], o[kb & 3]);
     }
-    const float* b4 = a.P + ktn::param_offset(4) + (long long)ktn::kHidden * ktn::kOutputs;
+    const float* b4 = a.P + ktn::param_offset(kWideH, 4) + (long long)kWideH * ktn::kOutputs;
     float y4[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) y4[r] = (o[0][r] + o[1][r] + o[2][r] + o[3][r]) * kNegLn2 + b4[4 * h + r];
@@ -680,45 +737,184 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         return;
     }
 
-    // the gate, on the lower half-wave
-    const float mx = fmaxf(y[0], fmaxf(y[1], y[2]));
-    const float e0 = __expf(y[0] - mx), e1 = __expf(y[1] - mx), e2 = __expf(y[2] - mx);
-    const float inv = 1.0f / (e0 + e1 + e2);
-    const float p0 = e0 * inv, p1 = e1 * inv, p2 = e2 * inv;
-    const int c = p0 >= p1 && p0 >= p2 ? 0 : (p1 >= p2 ? 1 : 2);
-    const float pm = c == 0 ? p0 : c == 1 ? p1 : p2;
-    const bool to_fallback = live && h == 0 && (!(pm > a.gate_prob) || !(rg > a.gate_ring));
-    const unsigned long long mask = __ballot(to_fallback);
-    if (mask) {
-        const int first = __ffsll((long long)mask) - 1;
-        unsigned long long base = 0;
-        if (lane == first) base = atomicAdd(a.n_fallback, (unsigned long long)__popcll(mask));
-        base = __shfl(base, first);
-        if (to_fallback) a.fallback[base + __popcll(mask & ((1ull << lane) - 1))] = q;
+    gate_and_write(a, q, live, h, lane, y, rg);
+}
+
+// ---- the fast network: 5 -> 128 -> 128 -> 128 -> 128 -> 8, hard SiLU --------------------
+// The same transposed product and in-lane operand layout as k_mlp_fused, over four 32-feature
+// output blocks and eight k-blocks per layer.  Its bf16 weights, 108 KB, fit in LDS whole:
+// a block of eight waves stages them once and then runs over tiles of 256 rays until the
+// batch is done (one block per CU), so no weight traffic or barrier remains in the loop.
+//
+// Hard SiLU, a = z clamp(z / 6 + 1/2, 0, 1), is two full-rate vector instructions.  The
+// kernel keeps every activation divided by 6: with z' = z / 6 it is z' clamp(z' + 1/2, 0, 1),
+// an add with the clamp modifier and a multiply.  The hidden layers' weights are unchanged
+// by this (their input carries the 1/6 and so does their output), the input layer's weights
+// and the biases of layers 0-3 are divided by 6, and the output layer multiplies by 6.
+constexpr int kFastH = 128;
+constexpr int kFastBlocks = kFastH / 32, kFastK = kFastH / 16;   // output blocks, k-blocks
+constexpr int kFastLayerFrags = kFastBlocks * kFastK * 64;       // uint4 per hidden layer
+constexpr int kFastOutFrags = kFastK * 64;                       // uint4 of the output layer
+constexpr int kFastInFrags = kFastBlocks * 64;                   // uint4 of the input layer
+constexpr int kFastFrags = 3 * kFastLayerFrags + kFastOutFrags + kFastInFrags;  // 6912 (108 KB)
+constexpr int kFastBias = 4 * kFastH + 8;                        // b0..b3 / 6, then b4
+constexpr int kFastThreads = 512, kFastTile = 256;               // 8 waves x 32 rays
+
+// packed fragments from the FP32 masters: hidden layer l (1-3) block fb, k-block kb, lane
+// (row n = 32 fb + lane % 32, k-slots F(kb, lane / 32, 0..7)); then the output layer
+// [kb][lane] (rows n >= 8 zero); then the input layer's split fragments of W0 / 6 [fb][lane]
+// (k-slot kk holds Wh[kk] for kk < 5, Wh[kk - 5] for kk < 10, Wl[kk - 10] for kk < 15, 0)
+__global__ void k_pack_fast(const float* P, uint16_t* w, float* bias) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int H = kFastH;
+    if (e < kFastBias) {  // biases
+        float v;
+        if (e < 4 * H) {
+            const int l = e / H, n = e % H;
+            v = P[ktn::param_offset(H, l) + (long long)ktn::kreal(H, l) * H + n] * (1.0f / 6.0f);
+        } else {
+            v = P[ktn::param_offset(H, 4) + (long long)H * ktn::kOutputs + (e - 4 * H)];
+        }
+        bias[e] = v;
     }
-    if (!live || h != 0 || to_fallback) return;
-    const double nan = __builtin_nan("");
-    double th = nan, ph = nan, rh = nan, g = nan;
-    if (c == 0) {
-        RayF p;
-        ray_params_f(a.views, a.n_views, q, p);
-        float u1[3], u2[3], u3[3];
-        ray_frame_f(p, u1, u2, u3);
-        sky_from_frame_f(u1, u2, u3, y[3], y[4], y[5], th, ph);
-    } else if (c == 2) {
-        rh = (double)__expf(y[6]);
-        g = (double)__expf(y[7]);
+    if (e >= kFastFrags * 8) return;
+    const int frag = e / 8, s = e % 8;
+    uint16_t out = 0;
+    if (frag < 3 * kFastLayerFrags) {
+        const int l = 1 + frag / kFastLayerFrags, r = frag % kFastLayerFrags;
+        const int fb = r / (kFastK * 64), kb = (r / 64) % kFastK, lane = r % 64;
+        const int f = packed_feature(kb, lane >> 5, s), n = 32 * fb + (lane & 31);
+        out = f2bf(P[ktn::param_offset(H, l) + (long long)f * H + n]);
+    } else if (frag < 3 * kFastLayerFrags + kFastOutFrags) {
+        const int r = frag - 3 * kFastLayerFrags;
+        const int kb = r / 64, lane = r % 64, n = lane & 31;
+        const int f = packed_feature(kb, lane >> 5, s);
+        out = n < ktn::kOutputs ? f2bf(P[ktn::param_offset(H, 4) + (long long)f * ktn::kOutputs + n]) : 0;
+    } else {
+        const int r = frag - 3 * kFastLayerFrags - kFastOutFrags;
+        const int fb = r / 64, lane = r % 64, kk = 8 * (lane >> 5) + s, n = 32 * fb + (lane & 31);
+        if (kk < 15) {
+            const float wv = P[(kk % 5) * H + n] * (1.0f / 6.0f);
+            const uint16_t hi = f2bf(wv);
+            out = kk < 10 ? hi : f2bf(wv - bf2f(hi));
+        }
     }
-    if (a.out.cls) a.out.cls[q] = (int8_t)(c | ktd::kFlagSurrogate);
-    if (a.out.theta) a.out.theta[q] = th;
-    if (a.out.phi) a.out.phi[q] = ph;
-    if (a.out.r_hit) a.out.r_hit[q] = rh;
-    if (a.out.g) a.out.g[q] = g;
-    if (a.out.steps) a.out.steps[q] = 0;
-    if (a.out.theta32) a.out.theta32[q] = (float)th;
-    if (a.out.phi32) a.out.phi32[q] = (float)ph;
-    if (a.out.r_hit32) a.out.r_hit32[q] = (float)rh;
-    if (a.out.g32) a.out.g32[q] = (float)g;
+    w[e] = out;
+}
+
+// hard SiLU of a scaled pre-activation z' = z / 6: z' clamp(z' + 1/2, 0, 1) = hsilu(z) / 6.
+// The clamp is one v_med3_f32 (a min/max pair compiles to compares and selects).
+__device__ __forceinline__ float hsilu_scaled(float zs) {
+    return zs * __builtin_amdgcn_fmed3f(zs + 0.5f, 0.0f, 1.0f);
+}
+
+__device__ __forceinline__ void hsilu_pair(const f32x16& z, bf16x8& p0, bf16x8& p1) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) p0[i] = (__bf16)hsilu_scaled(z[i]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) p1[i] = (__bf16)hsilu_scaled(z[8 + i]);
+}
+
+template <bool VIEWS>
+__global__ __launch_bounds__(kFastThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mlp_fast(FusedArgs a) {
+    __shared__ uint4 W[kFastFrags];
+    __shared__ __attribute__((aligned(16))) float Bs[kFastBias];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 31, h = lane >> 5;
+    for (int i = tid; i < kFastFrags; i += kFastThreads) W[i] = a.wfast[i];
+    for (int i = tid; i < kFastBias; i += kFastThreads) Bs[i] = a.bfast[i];
+    __syncthreads();
+    const uint4* Win = W + 3 * kFastLayerFrags + kFastOutFrags;
+    const uint4* Wout = W + 3 * kFastLayerFrags;
+    const long long tiles = (a.n + kFastTile - 1) / kFastTile;
+    for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const long long q = tile * kFastTile + wave * 32 + m;
+        const bool live = q < a.n;
+        const long long qc = live ? q : a.n - 1;
+        float f[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+        float rg = -1.f;
+        if (VIEWS) {
+            RayF p;
+            if (ray_params_f<false>(a.views, a.n_views, qc, p)) {
+                features_f(a.t.rho, a.t.rsh, p, f, &rg);
+                if (!(fabsf(f[3]) <= a.extent) || !(fabsf(f[4]) <= a.extent)) rg = -1.f;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) f[k] = a.xf[qc * 5 + k];
+        }
+        // input layer: split-bf16 features as the B operand, one MFMA per output block
+        bf16x8 B[kFastK];
+        {
+            float fh[5], fl[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                fh[k] = (float)(__bf16)f[k];
+                fl[k] = f[k] - fh[k];
+            }
+            const float x0[8] = {fh[0], fh[1], fh[2], fh[3], fh[4], fl[0], fl[1], fl[2]};
+            const float x1[8] = {fl[3], fl[4], fh[0], fh[1], fh[2], fh[3], fh[4], 0.f};
+            bf16x8 xb;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) xb[k] = (__bf16)(h ? x1[k] : x0[k]);
+#pragma unroll
+            for (int fb = 0; fb < kFastBlocks; ++fb) {
+                const f32x16 z = mfma32(Win[fb * 64 + lane], xb, block_bias(Bs, fb, h));
+                hsilu_pair(z, B[2 * fb], B[2 * fb + 1]);
+            }
+        }
+        // hidden layers 1-3 in output-block order, the previous block's activation beside
+        // the current block's MFMA chain
+#pragma unroll
+        for (int l = 1; l <= 3; ++l) {
+            const uint4* Wl = W + (l - 1) * kFastLayerFrags;
+            const float* bl = Bs + l * kFastH;
+            bf16x8 C[kFastK];
+            f32x16 prev;
+#pragma unroll
+            for (int fb = 0; fb < kFastBlocks; ++fb) {
+                f32x16 acc = block_bias(bl, fb, h);
+                uint4 af[kFastK];
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int i = 0; i < kFastK; ++i) af[i] = Wl[(fb * kFastK + i) * 64 + lane];
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int i = 0; i < kFastK; ++i) acc = mfma32(af[i], B[i], acc);
+                if (fb > 0) hsilu_pair(prev, C[2 * fb - 2], C[2 * fb - 1]);
+                prev = acc;
+            }
+            hsilu_pair(prev, C[kFastK - 2], C[kFastK - 1]);
+#pragma unroll
+            for (int k = 0; k < kFastK; ++k) B[k] = C[k];
+        }
+        // output layer: two partial sums over the eight k-blocks, times 6 (the scaled
+        // activations), plus the bias
+        f32x16 o[2];
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) o[p][i] = 0.f;
+#pragma unroll
+        for (int kb = 0; kb < kFastK; ++kb) o[kb & 1] = mfma32(Wout[kb * 64 + lane], B[kb], o[kb & 1]);
+        const float* b4 = Bs + 4 * kFastH;
+        float y4[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) y4[r] = (o[0][r] + o[1][r]) * 6.0f + b4[4 * h + r];
+        float y[8];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            y[r] = y4[r];
+            y[4 + r] = __shfl_down(y4[r], 32);
+        }
+        if (!VIEWS) {
+            if (live && h == 0)
+#pragma unroll
+                for (int k = 0; k < 8; ++k) a.y[q * 8 + k] = y[k];
+            continue;
+        }
+        gate_and_write(a, q, live, h, lane, y, rg);
+    }
 }
 
 template <class T>
@@ -732,7 +928,10 @@ hipError_t dalloc(T** p, size_t n) {
 
 struct Surrogate {
     int device = 0;
+    int cu_count = 0;
     bool have_params = false;
+    int net = ktn::kNetWide;      // which network (ktn::kNet*): its width and activation
+    int hid = 256, act = ktn::kActSilu;
     float* P = nullptr;     // FP32 master parameters
     float* G = nullptr;     // gradients
     float* M = nullptr;     // Adam moments
@@ -763,6 +962,8 @@ struct Surrogate {
     uint16_t* wpk = nullptr;  // layers 1-4 packed for the fused kernel (bf16)
     float* fold = nullptr;    // input layer and biases times -log2(e) for the fused kernel
     uint16_t* wpk0 = nullptr; // the input layer's split-bf16 MFMA fragments
+    uint16_t* wfast = nullptr;  // the fast network's packed fragments (k_mlp_fast)
+    float* bfast = nullptr;     // and its biases
     Queue queue;              // fallback queue of eager calls
     Domain dom;               // what the parameters cover
     static constexpr int kSplits = 64, kChunks = 256;
@@ -805,6 +1006,8 @@ static void free_all(Surrogate* s) {
     f(s->wpk);
     f(s->fold);
     f(s->wpk0);
+    f(s->wfast);
+    f(s->bfast);
     queue_release(s->queue);
 }
 
@@ -840,6 +1043,39 @@ static hipError_t build_table(Surrogate* s) {
     return hipSuccess;
 }
 
+// the layer table of the current network (the bf16 copies keep their maximal allocation)
+static void set_layers(Surrogate* s) {
+    const int H = s->hid;
+    for (int l = 0; l < ktn::kLayers; ++l) {
+        s->lt.off[l] = ktn::param_offset(H, l);
+        s->lt.kreal[l] = ktn::kreal(H, l);
+        s->lt.kpad[l] = ktn::kpad(H, l);
+        s->lt.n[l] = ktn::nout(H, l);
+        s->lt.nld[l] = ktn::nld(H, l);
+    }
+}
+
+hipError_t set_net(Surrogate* s, int net) {
+    if (net != ktn::kNetWide && net != ktn::kNetFast) return hipErrorInvalidValue;
+    if (net == s->net) return hipSuccess;
+    KTS_TRY(hipSetDevice(s->device));
+    KTS_TRY(hipDeviceSynchronize());
+    s->net = net;
+    s->hid = ktn::net_of(net).hidden;
+    s->act = ktn::net_of(net).act;
+    set_layers(s);
+    // the bf16 working copies change their strides: clear them (padding must stay zero)
+    for (int l = 0; l < ktn::kLayers; ++l) {
+        const int H = ktn::kMaxHidden;
+        KTS_TRY(hipMemset(s->wb[l], 0, (size_t)ktn::kpad(H, l) * ktn::nld(H, l) * 2));
+        KTS_TRY(hipMemset(s->wt[l], 0, (size_t)ktn::nout(H, l) * ktn::kpad(H, l) * 2));
+    }
+    s->have_params = false;
+    return hipSuccess;
+}
+
+int get_net(const Surrogate* s) { return s ? s->net : ktn::kNetWide; }
+
 Surrogate* create(int device, hipError_t* err) {
     Surrogate* s = new Surrogate();
     s->device = device;
@@ -851,25 +1087,26 @@ Surrogate* create(int device, hipError_t* err) {
     };
     hipError_t e;
     if ((e = hipSetDevice(device)) != hipSuccess) return fail(e);
-    const long long n = ktn::kParams;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) s->cu_count = prop.multiProcessorCount;
+    // every buffer sized for the wider network, so that either fits
+    const long long n = ktn::kMaxParams;
     if ((e = dalloc(&s->P, n)) != hipSuccess || (e = dalloc(&s->G, n)) != hipSuccess ||
         (e = dalloc(&s->M, n)) != hipSuccess || (e = dalloc(&s->V, n)) != hipSuccess)
         return fail(e);
     for (int l = 0; l < ktn::kLayers; ++l) {
-        s->lt.off[l] = ktn::param_offset(l);
-        s->lt.kreal[l] = ktn::kKreal[l];
-        s->lt.kpad[l] = ktn::kKpad[l];
-        s->lt.n[l] = ktn::kN[l];
-        s->lt.nld[l] = ktn::kNld[l];
-        const size_t nb = (size_t)ktn::kKpad[l] * ktn::kNld[l], nt = (size_t)ktn::kN[l] * ktn::kKpad[l];
+        const int H = ktn::kMaxHidden;
+        const size_t nb = (size_t)ktn::kpad(H, l) * ktn::nld(H, l), nt = (size_t)ktn::nout(H, l) * ktn::kpad(H, l);
         if ((e = dalloc(&s->wb[l], nb)) != hipSuccess || (e = dalloc(&s->wt[l], nt)) != hipSuccess) return fail(e);
         if ((e = hipMemset(s->wb[l], 0, nb * 2)) != hipSuccess || (e = hipMemset(s->wt[l], 0, nt * 2)) != hipSuccess)
             return fail(e);
         s->lt.wb[l] = s->wb[l];
         s->lt.wt[l] = s->wt[l];
     }
+    set_layers(s);
     if ((e = dalloc(&s->wpk, kPackedElems)) != hipSuccess || (e = dalloc(&s->fold, kFoldElems)) != hipSuccess ||
-        (e = dalloc(&s->wpk0, kInputFragElems)) != hipSuccess)
+        (e = dalloc(&s->wpk0, kInputFragElems)) != hipSuccess || (e = dalloc(&s->wfast, (size_t)kFastFrags * 8)) !=
+        hipSuccess || (e = dalloc(&s->bfast, kFastBias)) != hipSuccess)
         return fail(e);
     if ((e = build_table(s)) != hipSuccess) return fail(e);
     if ((e = dalloc(&s->loss, 1)) != hipSuccess) return fail(e);
@@ -899,6 +1136,11 @@ double table_critical_radius(const Surrogate* s, double a, double inc, double ps
 double table_shadow_radius(const Surrogate* s, double a, double inc) { return table_rsh(s->h_rsh.data(), a, inc); }
 
 static hipError_t pack(Surrogate* s, hipStream_t st) {
+    if (s->net == ktn::kNetFast) {
+        hipLaunchKernelGGL(k_pack_fast, dim3((kFastFrags * 8 + 255) / 256), dim3(256), 0, st, s->P, s->wfast,
+                           s->bfast);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_fold, dim3((kFoldElems + 255) / 256), dim3(256), 0, st, s->P, s->fold);
     hipLaunchKernelGGL(k_pack_input, dim3(kInputFragElems / 256), dim3(256), 0, st, s->fold, s->wpk0);
     hipLaunchKernelGGL(k_pack_fused, dim3((kPackedElems + 255) / 256), dim3(256), 0, st, s->P, s->wpk);
@@ -906,7 +1148,7 @@ static hipError_t pack(Surrogate* s, hipStream_t st) {
 }
 
 hipError_t set_params(Surrogate* s, const float* host, long long n) {
-    if (n != ktn::kParams) return hipErrorInvalidValue;
+    if (n != ktn::param_count(s->hid)) return hipErrorInvalidValue;
     KTS_TRY(hipSetDevice(s->device));
     KTS_TRY(hipMemcpy(s->P, host, sizeof(float) * n, hipMemcpyHostToDevice));
     KTS_TRY(hipMemset(s->M, 0, sizeof(float) * n));
@@ -919,25 +1161,26 @@ hipError_t set_params(Surrogate* s, const float* host, long long n) {
 }
 
 hipError_t get_params(Surrogate* s, float* host, long long n) {
-    if (n != ktn::kParams) return hipErrorInvalidValue;
+    if (n != ktn::param_count(s->hid)) return hipErrorInvalidValue;
     KTS_TRY(hipSetDevice(s->device));
     return hipMemcpy(host, s->P, sizeof(float) * n, hipMemcpyDeviceToHost);
 }
 
 hipError_t init_params(Surrogate* s, uint64_t seed) {
-    std::vector<float> p(ktn::kParams, 0.f);
+    const int H = s->hid;
+    std::vector<float> p(ktn::param_count(H), 0.f);
     std::mt19937_64 rng(seed);
     for (int l = 0; l < ktn::kLayers; ++l) {
-        const double lim = std::sqrt(6.0 / ktn::kKreal[l]);
+        const double lim = std::sqrt(6.0 / ktn::kreal(H, l));
         std::uniform_real_distribution<double> u(-lim, lim);
-        const long long off = ktn::param_offset(l);
-        const long long nw = (long long)ktn::kKreal[l] * ktn::kN[l];
+        const long long off = ktn::param_offset(H, l);
+        const long long nw = (long long)ktn::kreal(H, l) * ktn::nout(H, l);
         for (long long i = 0; i < nw; ++i) p[off + i] = (float)u(rng);
         // biases start at 0; the output layer starts small
         if (l == ktn::kLayers - 1)
             for (long long i = 0; i < nw; ++i) p[off + i] *= 0.1f;
     }
-    return set_params(s, p.data(), ktn::kParams);
+    return set_params(s, p.data(), (long long)p.size());
 }
 
 static hipError_t ensure_train_buffers(Surrogate* s, long long B) {
@@ -989,40 +1232,43 @@ static hipError_t ensure_train_buffers(Surrogate* s, long long B) {
 // forward over B rows of inputs xf (B x 5) into Z / H / Y (z stored when keep_z)
 static hipError_t forward(Surrogate* s, const float* xf, uint16_t** H, float** Z, float* Y, int B, bool keep_z,
                           hipStream_t st) {
-    KTS_TRY(ktn::layer0(xf, B, s->P, keep_z ? Z[0] : nullptr, H[0], st));
+    const int W = s->hid;
+    KTS_TRY(ktn::layer0(xf, B, s->P, W, s->act, keep_z ? Z[0] : nullptr, H[0], st));
     for (int l = 1; l < 4; ++l) {
         ktn::GemmArgs g{};
         g.M = B;
-        g.N = 256;
-        g.K = ktn::kKpad[l];
+        g.N = W;
+        g.K = ktn::kpad(W, l);
         g.A = H[l - 1];
-        g.lda = ktn::kKpad[l];
+        g.lda = ktn::kpad(W, l);
         g.B = s->wt[l];
-        g.ldb = ktn::kKpad[l];
-        g.ldc = 256;
-        g.bias = s->P + ktn::param_offset(l) + (long long)ktn::kKreal[l] * ktn::kN[l];
+        g.ldb = ktn::kpad(W, l);
+        g.ldc = W;
+        g.bias = s->P + ktn::param_offset(W, l) + (long long)ktn::kreal(W, l) * ktn::nout(W, l);
         g.out32 = keep_z ? Z[l] : nullptr;
         g.outb = H[l];
         g.k_split = g.K;
+        g.act = s->act;
         KTS_TRY(ktn::gemm(g, false, true, ktn::kEpiFwd, 1, st));
     }
     ktn::GemmArgs g{};
     g.M = B;
     g.N = 8;
-    g.K = 256;
+    g.K = W;
     g.A = H[3];
-    g.lda = 256;
+    g.lda = W;
     g.B = s->wt[4];
-    g.ldb = 256;
+    g.ldb = W;
     g.ldc = 8;
-    g.bias = s->P + ktn::param_offset(4) + 256LL * 8;
+    g.bias = s->P + ktn::param_offset(W, 4) + (long long)W * 8;
     g.out32 = Y;
-    g.k_split = 256;
+    g.k_split = W;
     return ktn::gemm(g, false, true, ktn::kEpiLin, 1, st);
 }
 
 // backward of the mean loss; gradients into s->G
 static hipError_t backward(Surrogate* s, int B, hipStream_t st) {
+    const int W = s->hid;
     const int splits = std::max(1, std::min(Surrogate::kSplits, B / 1024));
     const int ksplit = ((B / splits + 31) / 32) * 32;
     auto dW = [&](const uint16_t* A, int lda, int M, const uint16_t* Bm, int ldb, int N, int l) -> hipError_t {
@@ -1040,55 +1286,58 @@ static hipError_t backward(Surrogate* s, int B, hipStream_t st) {
         g.ws_stride = (long long)M * N;
         const int nsplit = (B + ksplit - 1) / ksplit;
         KTS_TRY(ktn::gemm(g, true, false, ktn::kEpiWs, nsplit, st));
-        return ktn::reduce_ws(s->ws, nsplit, (long long)M * N, ktn::kKreal[l], N, N, s->G + ktn::param_offset(l), st);
+        return ktn::reduce_ws(s->ws, nsplit, (long long)M * N, ktn::kreal(W, l), N, N,
+                              s->G + ktn::param_offset(W, l), st);
     };
     auto db = [&](const float* Gm, int N, int l) -> hipError_t {
         return ktn::colsum(Gm, B, N, s->partial, Surrogate::kChunks,
-                           s->G + ktn::param_offset(l) + (long long)ktn::kKreal[l] * ktn::kN[l], st);
+                           s->G + ktn::param_offset(W, l) + (long long)ktn::kreal(W, l) * ktn::nout(W, l), st);
     };
     // output layer
-    KTS_TRY(dW(s->H[3], 256, 256, s->gyb, 32, 8, 4));
+    KTS_TRY(dW(s->H[3], W, W, s->gyb, 32, 8, 4));
     KTS_TRY(db(s->gy32, 8, 4));
     {
         ktn::GemmArgs g{};
         g.M = B;
-        g.N = 256;
+        g.N = W;
         g.K = 32;
         g.A = s->gyb;
         g.lda = 32;
         g.B = s->wb[4];
         g.ldb = 32;
-        g.ldc = 256;
+        g.ldc = W;
         g.zin = s->Z[3];
-        g.ldz = 256;
+        g.ldz = W;
         g.out32 = s->g32[1];
         g.outb = s->gb[1];
         g.k_split = 32;
+        g.act = s->act;
         KTS_TRY(ktn::gemm(g, false, true, ktn::kEpiBwd, 1, st));
     }
     int cur = 1;
     for (int l = 3; l >= 1; --l) {
-        KTS_TRY(dW(s->H[l - 1], 256, 256, s->gb[cur], 256, 256, l));
-        KTS_TRY(db(s->g32[cur], 256, l));
+        KTS_TRY(dW(s->H[l - 1], W, W, s->gb[cur], W, W, l));
+        KTS_TRY(db(s->g32[cur], W, l));
         ktn::GemmArgs g{};
         g.M = B;
-        g.N = 256;
-        g.K = 256;
+        g.N = W;
+        g.K = W;
         g.A = s->gb[cur];
-        g.lda = 256;
+        g.lda = W;
         g.B = s->wb[l];
-        g.ldb = 256;
-        g.ldc = 256;
+        g.ldb = W;
+        g.ldc = W;
         g.zin = s->Z[l - 1];
-        g.ldz = 256;
+        g.ldz = W;
         g.out32 = s->g32[cur ^ 1];
         g.outb = s->gb[cur ^ 1];
-        g.k_split = 256;
+        g.k_split = W;
+        g.act = s->act;
         KTS_TRY(ktn::gemm(g, false, true, ktn::kEpiBwd, 1, st));
         cur ^= 1;
     }
-    // input layer in FP32 (its weights and bias are the first 6 x 256 parameters)
-    return ktn::layer0_grad(s->xf, s->g32[cur], B, s->partial0, Surrogate::kChunks, s->G, st);
+    // input layer in FP32 (its weights and bias are the first 6 x W parameters)
+    return ktn::layer0_grad(s->xf, s->g32[cur], B, W, s->partial0, Surrogate::kChunks, s->G, st);
 }
 
 static FusedArgs fused_args(Surrogate* s, long long n) {
@@ -1099,7 +1348,23 @@ static FusedArgs fused_args(Surrogate* s, long long n) {
     a.fold = s->fold;
     a.wpk0 = s->wpk0;
     a.wpk = s->wpk;
+    a.wfast = reinterpret_cast<const uint4*>(s->wfast);
+    a.bfast = s->bfast;
     return a;
+}
+
+// the network over n rays: k_mlp_fast (one block per CU, over tiles) or k_mlp_fused
+static void launch_network(Surrogate* s, const FusedArgs& a, bool views, hipStream_t st) {
+    if (s->net == ktn::kNetFast) {
+        const long long tiles = (a.n + kFastTile - 1) / kFastTile;
+        const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(tiles, std::max(1, s->cu_count)));
+        if (views) hipLaunchKernelGGL(k_mlp_fast<true>, dim3(grid), dim3(kFastThreads), 0, st, a);
+        else hipLaunchKernelGGL(k_mlp_fast<false>, dim3(grid), dim3(kFastThreads), 0, st, a);
+        return;
+    }
+    const dim3 grid((unsigned)((a.n + kFusedRays - 1) / kFusedRays));
+    if (views) hipLaunchKernelGGL(k_mlp_fused<true>, grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_mlp_fused<false>, grid, dim3(256), 0, st, a);
 }
 
 hipError_t predict(Surrogate* s, const float* feat, long long n, float* out, hipStream_t st) {
@@ -1108,7 +1373,7 @@ hipError_t predict(Surrogate* s, const float* feat, long long n, float* out, hip
     FusedArgs a = fused_args(s, n);
     a.xf = feat;
     a.y = out;
-    hipLaunchKernelGGL(k_mlp_fused<false>, dim3((unsigned)((n + kFusedRays - 1) / kFusedRays)), dim3(256), 0, st, a);
+    launch_network(s, a, false, st);
     return hipGetLastError();
 }
 
@@ -1121,7 +1386,8 @@ hipError_t gradients(Surrogate* s, const float* feat, const int8_t* cls, const f
     KTS_TRY(hipMemsetAsync(s->loss, 0, sizeof(double), st));
     KTS_TRY(ktn::loss_head(s->Y, s->bcls, s->btgt, n, 1.0f, 1.0f, s->gy32, s->gyb, s->loss, st));
     KTS_TRY(backward(s, n, st));
-    if (grads) KTS_TRY(hipMemcpyAsync(grads, s->G, sizeof(float) * ktn::kParams, hipMemcpyDeviceToDevice, st));
+    if (grads)
+        KTS_TRY(hipMemcpyAsync(grads, s->G, sizeof(float) * ktn::param_count(s->hid), hipMemcpyDeviceToDevice, st));
     if (loss) {
         KTS_TRY(hipMemcpyAsync(loss, s->loss, sizeof(double), hipMemcpyDeviceToHost, st));
         KTS_TRY(hipStreamSynchronize(st));
@@ -1133,8 +1399,8 @@ hipError_t gradients(Surrogate* s, const float* feat, const int8_t* cls, const f
 static hipError_t adam_update(Surrogate* s, const float* grads, double lr, int t, hipStream_t st) {
     const double b1 = 0.9, b2 = 0.999;
     const double c1 = 1.0 / (1.0 - std::pow(b1, t)), c2 = 1.0 / (1.0 - std::pow(b2, t));
-    return ktn::adam(s->P, grads ? grads : s->G, s->M, s->V, ktn::kParams, (float)lr, (float)b1, (float)b2, 1e-8f,
-                     (float)c1, (float)c2, s->lt, st);
+    return ktn::adam(s->P, grads ? grads : s->G, s->M, s->V, ktn::param_count(s->hid), (float)lr, (float)b1,
+                     (float)b2, 1e-8f, (float)c1, (float)c2, s->lt, st);
 }
 
 hipError_t adam_step(Surrogate* s, const float* grads, double lr, int t, hipStream_t st) {
@@ -1316,6 +1582,7 @@ static hipError_t evaluate(Surrogate* s, Dataset& d, const TrainOpts& o, TrainRe
 hipError_t train(Surrogate* s, const TrainOpts& o, TrainReport& rep) {
     using clk = std::chrono::steady_clock;
     KTS_TRY(hipSetDevice(s->device));
+    if (o.net != s->net) KTS_TRY(set_net(s, o.net));
     if (!s->have_params) KTS_TRY(init_params(s, o.seed ^ 0x1234567ull));
     // the ranges draw_rays samples and the scene the labels come from
     s->dom = Domain();
@@ -1389,7 +1656,7 @@ hipError_t train(Surrogate* s, const TrainOpts& o, TrainReport& rep) {
     rep.steps = t;
     // BF16 MFMA work per step: forward (2 B sum K N) + backward (2x that)
     double macs = 0;
-    for (int l = 0; l < ktn::kLayers; ++l) macs += (double)ktn::kKpad[l] * ktn::kN[l];
+    for (int l = 0; l < ktn::kLayers; ++l) macs += (double)ktn::kpad(s->hid, l) * ktn::nout(s->hid, l);
     rep.train_gemm_tflops = rep.train_seconds > 0 ? 6.0 * macs * B * (double)t / rep.train_seconds / 1e12 : 0.0;
     auto t2 = clk::now();
     if (e == hipSuccess) e = evaluate(s, ho, o, rep);
@@ -1442,7 +1709,7 @@ hipError_t trace_with_surrogate(Surrogate* s, const kti::TraceLaunch& L, double 
     a.out = L.out;
     a.fallback = Q.list;
     a.n_fallback = Q.count;
-    hipLaunchKernelGGL(k_mlp_fused<true>, dim3((unsigned)((n + kFusedRays - 1) / kFusedRays)), dim3(256), 0, st, a);
+    launch_network(s, a, true, st);
     hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(64), 0, st, Q.count, Q.total, (unsigned long long*)nullptr);
     KTS_TRY(hipGetLastError());
     // the integrator on the queue the gate left (count read on the device).  The one-shot

@@ -36,6 +36,7 @@ struct TrainOpts {
     int label_precision = 1;    // KT_PREC_* of the integrator labelling the training rays
     double disk_outer = 20.0, escape_radius = 1000.0, capture_factor = 1.01;
     double rtol = 1e-10, atol = 1e-12, rtol32 = 1e-6, atol32 = 1e-7, restep_drift = 1e-6;
+    int net = 0;                // ktn::kNet*: the network to train
     int verbose = 0;            // progress lines on stderr
 };
 
@@ -65,6 +66,11 @@ struct TrainReport {
 Surrogate* create(int device, hipError_t* err);
 void destroy(Surrogate* s);
 bool ready(const Surrogate* s);
+
+// which network the surrogate holds (ktn::kNetWide or ktn::kNetFast); changing it drops the
+// parameters (init_params, set_params or train give new ones)
+hipError_t set_net(Surrogate* s, int net);
+int get_net(const Surrogate* s);
 
 hipError_t init_params(Surrogate* s, uint64_t seed);
 hipError_t set_params(Surrogate* s, const float* host, long long n);

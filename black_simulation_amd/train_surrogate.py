@@ -1,6 +1,6 @@
 """Train the surrogate on the GPU and write its parameters and held-out report.
 
-    python -m black_simulation_amd.train_surrogate --out PATH [--n-train N] [--epochs E] ...
+    python -m black_simulation_amd.train_surrogate --out PATH [--net wide|fast] [--n-train N] [--epochs E] ...
 
 Writes PATH.f32 (kt_surrogate_param_count() little-endian float32 values, the layout
 kt_surrogate_load takes) and PATH.json (the options and the kt_surrogate_report).  The
@@ -21,15 +21,18 @@ import numpy as np
 from . import api
 from . import _lib as K
 
-DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate")
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+DEFAULT_WEIGHTS = os.path.join(DATA, "surrogate")        # the wide network (5-256x4-8, SiLU)
+FAST_WEIGHTS = os.path.join(DATA, "surrogate_fast")      # the fast one (5-128x4-8, hard SiLU)
 
 
 def load_params(path: str = DEFAULT_WEIGHTS) -> np.ndarray:
-    """Parameters written by this script (PATH.f32); raises if absent or of the wrong size."""
+    """Parameters written by this script (PATH.f32); raises if absent or if the count is
+    not one of the networks' (the count names the network)."""
     p = np.fromfile(path + ".f32", dtype="<f4")
-    n = K.lib().kt_surrogate_param_count()
-    if p.size != n:
-        raise ValueError(f"{path}.f32 holds {p.size} values, the network has {n}")
+    sizes = {K.lib().kt_surrogate_net_param_count(n) for n in (K.NET_WIDE, K.NET_FAST)}
+    if p.size not in sizes:
+        raise ValueError(f"{path}.f32 holds {p.size} values; the networks have {sorted(sizes)}")
     return p
 
 
@@ -61,13 +64,15 @@ def main(argv=None) -> int:
     ap.add_argument("--w-disk", type=float, default=0.0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--init-seed", type=int, default=1)
+    ap.add_argument("--net", choices=["wide", "fast"], default="wide")
     args = ap.parse_args(argv)
-    kw = {"verbose": 1}
+    kw = {"verbose": 1, "net": K.NET_FAST if args.net == "fast" else K.NET_WIDE}
     for k in ("n_train", "n_holdout", "epochs", "batch", "lr", "seed", "w_sky", "w_disk"):
         v = getattr(args, k)
         if v:
             kw[k] = v
     tr = api.Tracer(0)
+    tr.surrogate_set_net(kw["net"])
     tr.surrogate_init(args.init_seed)
     t0 = time.perf_counter()
     rep = tr.surrogate_train(**kw)

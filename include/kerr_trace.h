@@ -230,12 +230,15 @@ int kt_plan_stats(kt_plan* plan, kt_stats* stats);
 int kt_plan_destroy(kt_plan* plan);
 
 /* ---- the learned surrogate (no reference counterpart) ----------------------------------
- * An MLP 5-256-256-256-256-8 (SiLU) with BF16 weights on the matrix cores, FP32
- * accumulation and FP32 master weights.  It answers a ray when its largest class
- * probability exceeds opts.gate_prob and the ray lies outside opts.gate_ring of the
- * critical curve; every other ray goes to the integrator (kt_trace_views with
- * use_surrogate = 1).  Parameters are FP32, layer by layer: W_l (n_in x n_out,
- * row-major) then b_l. */
+ * An MLP with BF16 weights on the matrix cores, FP32 accumulation and FP32 master weights,
+ * one of two networks: KT_NET_WIDE, 5-256-256-256-256-8 with SiLU, and KT_NET_FAST,
+ * 5-128-128-128-128-8 with hard SiLU (x clamp(x/6 + 1/2, 0, 1)), a quarter of the matrix
+ * work and no transcendentals.  It answers a ray when its largest class probability
+ * exceeds opts.gate_prob and the ray lies outside opts.gate_ring of the critical curve;
+ * every other ray goes to the integrator (kt_trace_views with use_surrogate = 1).
+ * Parameters are FP32, layer by layer: W_l (n_in x n_out, row-major) then b_l; their
+ * count names the network (kt_surrogate_net_param_count). */
+enum { KT_NET_WIDE = 0, KT_NET_FAST = 1 };
 typedef struct kt_surrogate_opts {
     long long n_train, n_holdout;  /* 2^24, 2^20                                        */
     int epochs, batch;             /* 192, 65536                                        */
@@ -247,6 +250,7 @@ typedef struct kt_surrogate_opts {
     int label_precision;           /* KT_PREC_* labelling the training set (mixed)      */
     double disk_outer, escape_radius, capture_factor;  /* the scene it learns (20, 1000, 1.01) */
     int verbose;                   /* 1: one progress line per epoch on stderr          */
+    int net;                       /* KT_NET_* to train (KT_NET_WIDE)                   */
 } kt_surrogate_opts;
 
 typedef struct kt_surrogate_report {
@@ -259,7 +263,14 @@ typedef struct kt_surrogate_report {
 } kt_surrogate_report;
 
 void kt_surrogate_opts_default(kt_surrogate_opts* opts);
+/* parameters of KT_NET_WIDE; of either network; 0 for an unknown one */
 long long kt_surrogate_param_count(void);
+long long kt_surrogate_net_param_count(int net);
+/* the network the context holds (KT_NET_WIDE until set): set_net drops its parameters;
+ * kt_surrogate_load picks the network from the parameter count, kt_surrogate_train from
+ * opts->net, kt_surrogate_init initialises the current one */
+int kt_surrogate_set_net(kt_ctx* ctx, int net);
+int kt_surrogate_get_net(kt_ctx* ctx);
 int kt_surrogate_init(kt_ctx* ctx, unsigned long long seed);
 int kt_surrogate_load(kt_ctx* ctx, const float* params, long long n);
 int kt_surrogate_save(kt_ctx* ctx, float* params, long long n);

@@ -4,20 +4,24 @@ TEST INFRASTRUCTURE ONLY (tests/ and bench.py's checks import it; the product ne
 The surrogate has no counterpart in the reference; this module is pinned by central
 finite differences (tests/test_mlp_oracle.py), not by reference vectors.
 
-Network: 5 -> 256 -> 256 -> 256 -> 256 -> 8, SiLU after each hidden layer.  Layer l maps
-x (B x n_in) to x @ W_l + b_l with W_l stored (n_in x n_out), row-major.
+Networks: 5 -> 256 -> 256 -> 256 -> 256 -> 8 with SiLU after each hidden layer (the wide
+network), and 5 -> 128 -> 128 -> 128 -> 128 -> 8 with hard SiLU, z clamp(z/6 + 1/2, 0, 1)
+(the fast one).  Layer l maps x (B x n_in) to x @ W_l + b_l with W_l stored (n_in x n_out),
+row-major.
 
 Loss per ray (mean over the batch):
     cross-entropy of the 3 class logits (escaped, captured, disk)
-    + sky weight * [(cos th - t0)^2 + (cos ph - t1)^2 + (sin ph - t2)^2]   escaped rays only
-    + disk weight * [(r/r_isco - t3)^2 + (log g - t4)^2]                    disk rays only
-Outputs 3..7 are (cos theta_inf, cos phi_inf, sin phi_inf, r_hit/r_ISCO, log g).
+    + sky weight * sum_k (y_{3+k} - t_k)^2,  k = 0..2                      escaped rays only
+    + disk weight * [(y_6 - t_3)^2 + (y_7 - t_4)^2]                          disk rays only
+Outputs 3..5 are the escape direction's components along the ray's straight-line frame,
+6 and 7 are log r_hit and log g.
 """
 from __future__ import annotations
 
 import numpy as np
 
 SIZES = (5, 256, 256, 256, 256, 8)
+FAST_SIZES = (5, 128, 128, 128, 128, 8)
 
 
 def silu(z):
@@ -27,6 +31,17 @@ def silu(z):
 def dsilu(z):
     s = 1.0 / (1.0 + np.exp(-z))
     return s * (1.0 + z * (1.0 - s))
+
+
+def hsilu(z):
+    return z * np.clip(z / 6.0 + 0.5, 0.0, 1.0)
+
+
+def dhsilu(z):
+    return np.where(z <= -3.0, 0.0, np.where(z >= 3.0, 1.0, z / 3.0 + 0.5))
+
+
+ACT = {"silu": (silu, dsilu), "hard": (hsilu, dhsilu)}
 
 
 def init_params(seed: int = 0, sizes=SIZES):
@@ -39,14 +54,15 @@ def init_params(seed: int = 0, sizes=SIZES):
     return params
 
 
-def forward(params, x):
+def forward(params, x, act="silu"):
     """Returns the output (B x 8) and the per-layer (input, pre-activation) cache."""
+    f = ACT[act][0]
     cache = []
     h = x
     for l, (W, b) in enumerate(params):
         z = h @ W + b
         cache.append((h, z))
-        h = silu(z) if l < len(params) - 1 else z
+        h = f(z) if l < len(params) - 1 else z
     return h, cache
 
 
@@ -72,22 +88,23 @@ def loss_and_grad_out(y, cls, tgt, w_sky=1.0, w_disk=1.0):
     return loss.mean(), g / B
 
 
-def backward(params, cache, gy):
+def backward(params, cache, gy, act="silu"):
     """Gradients [(dW, db)] for dL/dy = gy."""
+    df = ACT[act][1]
     grads = [None] * len(params)
     g = gy
     for l in range(len(params) - 1, -1, -1):
         W, _ = params[l]
         h, z = cache[l]
         if l < len(params) - 1:
-            g = g * dsilu(z)
+            g = g * df(z)
         grads[l] = (h.T @ g, g.sum(axis=0))
         g = g @ W.T
     return grads
 
 
-def loss_fn(params, x, cls, tgt, **kw):
-    y, _ = forward(params, x)
+def loss_fn(params, x, cls, tgt, act="silu", **kw):
+    y, _ = forward(params, x, act)
     return loss_and_grad_out(y, cls, tgt, **kw)[0]
 
 

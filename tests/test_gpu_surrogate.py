@@ -26,9 +26,9 @@ def flat(params):
     return np.concatenate([np.concatenate([W.ravel(), b]) for W, b in params]).astype(np.float32)
 
 
-def unflat(p):
+def unflat(p, sizes=M.SIZES):
     out, o = [], 0
-    for n_in, n_out in zip(M.SIZES[:-1], M.SIZES[1:]):
+    for n_in, n_out in zip(sizes[:-1], sizes[1:]):
         W = p[o:o + n_in * n_out].reshape(n_in, n_out).astype(np.float64)
         o += n_in * n_out
         b = p[o:o + n_out].astype(np.float64)
@@ -81,9 +81,9 @@ def fused_forward(params, x):
     return float(NEG_LN2) * (h @ bf16(W4)) + f32(b4)
 
 
-def problem(seed=3, n=1024):
+def problem(seed=3, n=1024, sizes=M.SIZES):
     rng = np.random.default_rng(seed)
-    params = M.init_params(seed)
+    params = M.init_params(seed, sizes)
     params = [(W, rng.normal(0, 0.05, b.shape)) for W, b in params]
     x = rng.uniform(-1, 1, (n, 5))
     cls = rng.integers(0, 3, n).astype(np.int8)
@@ -150,6 +150,88 @@ def test_adam_step_matches_numpy(tracer):
     got = tracer.surrogate_save()
     st = M.adam_state(unflat(p32))
     want = flat(M.adam_step(unflat(p32), [(gW, gb) for gW, gb in unflat(gr)], st, 1e-3))
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-7)
+
+
+def fast_forward(params, x):
+    """the fast kernel's arithmetic: activations kept divided by 6 (bf16(z' clamp(z' + 1/2,
+    0, 1)) with z' = z / 6), the input layer as split-bf16 products of W0 / 6, bf16 hidden
+    and output weights, the output times 6 plus b4"""
+    (W0, b0), hidden, (W4, b4) = params[0], params[1:-1], params[-1]
+    sixth = lambda v: (np.asarray(v, np.float32) * np.float32(1.0 / 6.0)).astype(np.float64)  # noqa: E731
+    W0s = sixth(W0)
+    Wh = bf16(W0s)
+    Wl = bf16(W0s - Wh)
+    xf = f32(x)
+    fh = bf16(xf)
+    fl = bf16(xf - fh)
+    act = lambda zs: bf16(zs * np.clip(zs + 0.5, 0.0, 1.0))  # noqa: E731
+    h = act(fh @ Wh + fl @ Wh + fh @ Wl + sixth(b0))
+    for W, b in hidden:
+        h = act(h @ bf16(W) + sixth(b))
+    return 6.0 * (h @ bf16(W4)) + f32(b4)
+
+
+def test_fast_network_forward_matches_numpy(tracer):
+    """The 128-wide hard-SiLU network in the fast kernel: against the network itself (FP32
+    input layer, bf16 weights and activations) to the bf16 noise, and against the kernel's
+    own arithmetic (activations divided by 6, split-bf16 input layer) far tighter."""
+    params, x, _, _ = problem(n=2048, sizes=M.FAST_SIZES)
+    tracer.surrogate_load(flat(params))
+    assert tracer.surrogate_net() == bsa.NET_FAST
+    y = tracer.surrogate_predict(x).cpu().numpy()
+    P = unflat(flat(params), M.FAST_SIZES)
+    Wq = quantised(P)
+    h = f32(x)
+    for l, (W, b) in enumerate(Wq):
+        z = h @ W + b
+        h = bf16(M.hsilu(z)) if l < len(Wq) - 1 else z
+    scale = np.abs(h).max()
+    err = np.abs(y - h)
+    print(f"fast forward: max err {err.max() / scale:.2e}, median {np.median(err) / scale:.2e} (of {scale:.2f})")
+    assert err.max() < 2e-2 * scale and np.median(err) < 1e-3 * scale
+    exact = fast_forward(P, x)
+    err = np.abs(y - exact)
+    print(f"fast kernel arithmetic: max err {err.max() / scale:.2e}, median {np.median(err) / scale:.2e}")
+    assert err.max() < 2e-3 * scale and np.median(err) < 2e-5 * scale
+
+
+def test_fast_network_gradients_and_adam_match_numpy(tracer):
+    """Training kernels on the fast network: loss and every layer's weight and bias
+    gradient against the FP64 network (hard SiLU, activations rounded as on the GPU), then
+    one Adam step."""
+    params, x, cls, tgt = problem(n=2048, sizes=M.FAST_SIZES)
+    p32 = flat(params)
+    tracer.surrogate_load(p32)
+    g, loss = tracer.surrogate_gradients(x, cls, tgt)
+    g = g.cpu().numpy()
+    P = unflat(p32, M.FAST_SIZES)
+    Wq = quantised(P)
+    h = f32(x)
+    cache = []
+    for l, (W, b) in enumerate(Wq):
+        z = h @ W + b
+        cache.append((h, z))
+        h = bf16(M.hsilu(z)) if l < len(Wq) - 1 else z
+    want_loss, gy = M.loss_and_grad_out(h, cls.astype(int), tgt)
+    grads = M.backward(Wq, cache, gy, "hard")
+    assert loss == pytest.approx(want_loss, rel=2e-3)
+    o = 0
+    for l, (gW, gb) in enumerate(grads):
+        for ref in (gW.ravel(), gb):
+            got = g[o:o + ref.size]
+            o += ref.size
+            rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            cos = np.dot(got, ref) / (np.linalg.norm(got) * np.linalg.norm(ref))
+            print(f"fast layer {l} {'W' if ref.ndim else 'b'} size {ref.size}: rel err {rel:.2e} cos {cos:.6f}")
+            assert rel < 3e-2 and cos > 0.999
+    assert o == p32.size
+    rng = np.random.default_rng(9)
+    gr = rng.normal(0, 1e-3, p32.size).astype(np.float32)
+    tracer.surrogate_adam_step(torch.as_tensor(gr).cuda(), 1e-3, 1)
+    got = tracer.surrogate_save()
+    st = M.adam_state(P)
+    want = flat(M.adam_step(P, [(gW, gb) for gW, gb in unflat(gr, M.FAST_SIZES)], st, 1e-3))
     np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-7)
 
 

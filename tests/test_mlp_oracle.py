@@ -1,6 +1,7 @@
 """The FP64 numpy surrogate network (oracle/mlp_oracle.py) against central finite
 differences: its backward pass is what the GPU kernels' gradients are checked against."""
 import numpy as np
+import pytest
 
 from oracle import mlp_oracle as M
 
@@ -16,11 +17,14 @@ def small_problem(seed=1, B=16):
     return params, x, cls, tgt
 
 
-def test_backward_matches_finite_differences():
+@pytest.mark.parametrize("act", ["silu", "hard"])
+def test_backward_matches_finite_differences(act):
     params, x, cls, tgt = small_problem()
-    y, cache = M.forward(params, x)
+    y, cache = M.forward(params, x, act)
     _, gy = M.loss_and_grad_out(y, cls, tgt, w_sky=2.0, w_disk=0.5)
-    grads = M.backward(params, cache, gy)
+    grads = M.backward(params, cache, gy, act)
+    # the hard SiLU's derivative jumps at +-3: keep every pre-activation clear of the kinks
+    assert all(np.abs(np.abs(z) - 3.0).min() > 1e-4 for _, z in cache[:-1])
     rng = np.random.default_rng(0)
     h = 1e-6
     for l, ((W, b), (gW, gb)) in enumerate(zip(params, grads)):
@@ -31,8 +35,8 @@ def test_backward_matches_finite_differences():
             Wm[i, j] -= h
             pp = [(Wp if k == l else p[0], p[1]) for k, p in enumerate(params)]
             pm = [(Wm if k == l else p[0], p[1]) for k, p in enumerate(params)]
-            fd = (M.loss_fn(pp, x, cls, tgt, w_sky=2.0, w_disk=0.5) - M.loss_fn(pm, x, cls, tgt, w_sky=2.0,
-                                                                                w_disk=0.5)) / (2 * h)
+            fd = (M.loss_fn(pp, x, cls, tgt, act, w_sky=2.0, w_disk=0.5)
+                  - M.loss_fn(pm, x, cls, tgt, act, w_sky=2.0, w_disk=0.5)) / (2 * h)
             assert abs(fd - gW[i, j]) < 1e-7 + 1e-5 * abs(fd), (l, i, j, fd, gW[i, j])
         j = rng.integers(0, b.shape[0])
         bp, bm = b.copy(), b.copy()
@@ -40,8 +44,8 @@ def test_backward_matches_finite_differences():
         bm[j] -= h
         pp = [(p[0], bp if k == l else p[1]) for k, p in enumerate(params)]
         pm = [(p[0], bm if k == l else p[1]) for k, p in enumerate(params)]
-        fd = (M.loss_fn(pp, x, cls, tgt, w_sky=2.0, w_disk=0.5) - M.loss_fn(pm, x, cls, tgt, w_sky=2.0,
-                                                                            w_disk=0.5)) / (2 * h)
+        fd = (M.loss_fn(pp, x, cls, tgt, act, w_sky=2.0, w_disk=0.5)
+              - M.loss_fn(pm, x, cls, tgt, act, w_sky=2.0, w_disk=0.5)) / (2 * h)
         assert abs(fd - gb[j]) < 1e-7 + 1e-5 * abs(fd)
 
 
@@ -81,3 +85,15 @@ def test_loss_closed_form():
     loss, _ = M.loss_and_grad_out(y, cls, tgt, w_sky=2.0, w_disk=0.5)
     want = (3 * np.log(3.0) + 2.0 * (0.25 + 0.25 + 0.0625) + 0.5 * (4.0 + 1.0)) / B
     assert loss == np.float64(want) or abs(loss - want) < 1e-12
+
+
+def test_hard_silu_closed_forms():
+    """z clamp(z/6 + 1/2, 0, 1): zero below -3, the identity above 3, z (z + 3) / 6 between
+    (continuous at both ends), with derivative (2z + 3) / 6 there; SiLU's odd part is the
+    same: f(z) - f(-z) = z."""
+    z = np.array([-7.0, -3.0, -1.5, 0.0, 1.5, 3.0, 7.0])
+    np.testing.assert_allclose(M.hsilu(z), [0.0, 0.0, -0.375, 0.0, 1.125, 3.0, 7.0], atol=1e-15)
+    np.testing.assert_allclose(M.dhsilu(np.array([-4.0, -1.5, 0.0, 1.5, 4.0])), [0.0, 0.0, 0.5, 1.0, 1.0])
+    zz = np.linspace(-6, 6, 101)
+    np.testing.assert_allclose(M.hsilu(zz) - M.hsilu(-zz), zz, atol=1e-14)
+    np.testing.assert_allclose(M.silu(zz) - M.silu(-zz), zz, atol=1e-14)
