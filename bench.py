@@ -4,9 +4,9 @@ integrator fallback against the plain FP64 integrator on the same frames.
 Workload (default): the 120-frame camera orbit at r = 40 M (phi_k = 2 pi k / 120,
 inclination 20 -> 85 deg), Bardeen plane +-1.5 mean shadow radii, disk r_ISCO..20 M, escape 1000 M,
 capture 1.01 r+.  One step = one full frame (2,073,600 rays), replayed from a hipGraph
-plan.  --path surrogate: the MLP answers rays with max class probability > 0.999 lying
-outside +-3% of the critical curve, the mixed FP32/FP64 integrator the rest, all in
-one graph; --path plain: the FP64 integrator; --path mixed: the FP32/FP64 integrator.
+plan.  --path surrogate: the MLP (by default the fast network, 5-128x4-8 with hard SiLU)
+answers rays with max class probability > 0.999 lying outside +-3% of the critical
+curve, the mixed FP32/FP64 integrator the rest, all in one graph; --path plain: the FP64 integrator; --path mixed: the FP32/FP64 integrator.
 Both the chosen path and the plain FP64 integrator are timed on the same frames, and
 the chosen path's outputs are judged against the FP64 render of every frame.
 --workload c2 times the single a=0.998 frame at r = 50 M, i = 85 deg (disk to 40 M)
@@ -334,7 +334,9 @@ def main():
     ap.add_argument("--particles", type=int, default=1 << 22, help="--workload debris: particles")
     ap.add_argument("--size", type=int, default=1024, help="--workload eht: pixels across")
     ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--weights", default=None)
+    ap.add_argument("--weights", default=None,
+                    help="surrogate parameters (PATH.f32 and PATH.json); default the fast network, "
+                         "black_simulation_amd/data/surrogate_fast (the wide one: .../data/surrogate)")
     ap.add_argument("--streams", type=int, default=2)
     ap.add_argument("--lean", action="store_true",
                     help="for profilers: no autotuning, no plain-FP64 timing, no accuracy pass")
@@ -383,7 +385,7 @@ def main():
     import black_simulation_amd as bsa
     from black_simulation_amd import orbit as O
     from black_simulation_amd.configs import C5_FRAMES, c2_view, c3_views, c5_view, infall_view
-    from black_simulation_amd.train_surrogate import DEFAULT_WEIGHTS, load_into
+    from black_simulation_amd.train_surrogate import FAST_WEIGHTS, load_into
 
     width = args.height * 16 // 9
     if args.workload == "orbit":
@@ -416,7 +418,7 @@ def main():
         opts = tune(bsa.TraceOpts(precision=bsa.PREC_MIXED, mode=mode,
                                   **({"rtol32": args.rtol, "atol32": args.rtol / 10} if args.rtol else {})))
         if args.path == "surrogate":
-            load_into(tr, args.weights or DEFAULT_WEIGHTS)
+            load_into(tr, args.weights or FAST_WEIGHTS)
             opts.use_surrogate = 1
     tr_net = tr.surrogate_net() if args.path == "surrogate" else None
 
@@ -559,18 +561,20 @@ def main():
                   "peak_is": "FP64 78.6 / FP32 157.3 TFLOP/s blended by the FP64 share of attempts"}
     roof = roof_integ
     roof_mlp = None
+    net_desc = None
     if args.path == "surrogate":
         hidden = 128 if tr_net == bsa.NET_FAST else 256
         flop_ray = mlp_flop_per_ray(hidden)
         mlp_ach = rays * flop_ray / (avg_sur * 1e-3) / 1e12
         kmlp = "k_mlp_fast" if hidden == 128 else "k_mlp_fused"
+        net_desc = f"5-{hidden}x4-8 " + ("hard SiLU" if hidden == 128 else "SiLU")
         tr_mlp, src_mlp = profiled_traffic(args.workload, args.height, kmlp)
         roof_mlp = {"bound": "mfma", "achieved": round(mlp_ach, 2), "peak": BF16_MFMA_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(mlp_ach / BF16_MFMA_PEAK_TFLOPS, 4), "traffic": tr_mlp,
                     "traffic_source": src_mlp,
                     "kernel": f"{kmlp} (features, network, gate)", "kernel_ms": round(avg_sur, 3),
                     "flop_per_ray": flop_ray,
-                    "network": f"5-{hidden}x4-8 " + ("hard SiLU" if hidden == 128 else "SiLU")}
+                    "network": net_desc}
         if avg_sur > avg_integ:
             roof, roof_mlp = roof_mlp, roof_integ
 
@@ -598,6 +602,7 @@ def main():
         "data": "synthetic: analytic Bardeen image planes, no input files; surrogate weights trained by "
                 "black_simulation_amd.train_surrogate on integrator-labelled rays",
         "config": {"workload": f"{args.path}: {desc}", "path": args.path, "rays_per_step": rays,
+                   "network": net_desc,
                    "streams": args.streams, "lean": args.lean,
                    "frame_ms_p50": round(percentile(lat, 0.5), 3), "frame_ms_p99": round(percentile(lat, 0.99), 3),
                    "plain_fp64": {"value": round(value_p, 3), "frame_ms_p50": round(percentile(lat_p, 0.5), 3),

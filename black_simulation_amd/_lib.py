@@ -126,6 +126,8 @@ class KtSurrogateOpts(C.Structure):
         ("capture_factor", C.c_double),
         ("verbose", C.c_int),
         ("net", C.c_int),
+        ("edge_band", C.c_double),
+        ("edge_copies", C.c_int),
     ]
 
 

@@ -739,7 +739,7 @@ int kt_trace_views(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
     }
 }
 
-static_assert(KT_NET_WIDE == ktn::kNetWide && KT_NET_FAST == ktn::kNetFast, "network ids");
+static_assert((int)KT_NET_WIDE == (int)ktn::kNetWide && (int)KT_NET_FAST == (int)ktn::kNetFast, "network ids");
 static_assert(KT_NOALLOC_MAX_VIEWS == kti::kMaxLdsViews, "the scratch holds the LDS-staged view table");
 
 int kt_trace_views_noalloc(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trace_opts* opts,
@@ -1469,6 +1469,8 @@ void kt_surrogate_opts_default(kt_surrogate_opts* o) {
     o->capture_factor = d.capture_factor;
     o->verbose = 0;
     o->net = KT_NET_WIDE;
+    o->edge_band = d.edge_band;
+    o->edge_copies = d.edge_copies;
 }
 
 long long kt_surrogate_param_count(void) { return ktn::param_count(ktn::net_of(KT_NET_WIDE).hidden); }
@@ -1542,6 +1544,8 @@ int kt_surrogate_train(kt_ctx* ctx, const kt_surrogate_opts* opts, kt_surrogate_
         o.verbose = opts->verbose;
         if (opts->net != KT_NET_WIDE && opts->net != KT_NET_FAST) return fail(ctx, KT_EINVAL, "unknown network");
         o.net = opts->net;
+        o.edge_band = opts->edge_band > 0 ? opts->edge_band : o.edge_band;
+        o.edge_copies = opts->edge_copies < 0 ? 0 : (opts->edge_copies > 0 ? opts->edge_copies : o.edge_copies);
     }
     if (o.batch % 128 != 0) return fail(ctx, KT_EINVAL, "batch must be a multiple of 128");
     kts::TrainReport r;

@@ -255,42 +255,7 @@ __device__ __forceinline__ int find_view(const ViewDev* v, int n, long long idx)
     return lo;
 }
 
-// Bardeen parameters of ray q of the batch: returns false for rays the surrogate does not
-// cover (pinhole cameras)
-__device__ __forceinline__ bool ray_params(const ViewDev* views, int n_views, long long q, double& a, double& r_cam,
-                                           double& inc, double& alpha, double& beta, double& phi_cam,
-                                           double& r_in) {
-    const ViewDev& v = views[find_view(views, n_views, q)];
-    const long long local = q - v.offset;
-    phi_cam = v.phi_cam;
-    if (v.kind == 2) {
-        const double* p = v.params + 5 * local;
-        a = p[0];
-        r_cam = p[1];
-        inc = p[2];
-        alpha = p[3];
-        beta = p[4];
-        r_in = ktd::isco_dev(a, v.prograde);
-        phi_cam = 0.0;
-        return true;
-    }
-    if (v.kind != 0) return false;
-    a = v.a;
-    r_cam = v.r_cam;
-    inc = atan2(v.sinI, v.cosI);
-    r_in = v.r_in;
-    if (v.params) {
-        alpha = v.params[2 * local];
-        beta = v.params[2 * local + 1];
-    } else {
-        const int row = (int)(local / v.width), col = (int)(local - (long long)row * v.width);
-        alpha = -v.half_width + ((double)col + 0.5) * v.px;
-        beta = v.half_height - ((double)row + 0.5) * v.py;
-    }
-    return true;
-}
-
-// ray_params in float (the fused inference kernels): false for rays the surrogate does not
+// Bardeen parameters of ray q of the batch, in float (the fused inference kernels): false for rays the surrogate does not
 // cover (cameras other than Bardeen planes, views outside the network's domain).  The
 // camera azimuth's sine and cosine (sp, cp) only WITH_PHI: only answered escaped rays need
 // them, for their sky angles.
@@ -556,12 +521,12 @@ __device__ __forceinline__ f32x16 mfma32(const uint4& a, const bf16x8& b, const 
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a), b, c, 0, 0, 0);
 }
 
-// The gate and the answered rays' outputs, from the network outputs y of the ray this lane
-// holds (lane m and m + 32 hold the same ray; the lower half-wave, h = 0, decides): a ray
-// is answered when its largest class probability exceeds gate_prob and it lies outside
+// The gate and the answered rays' outputs, from the network outputs y of ray q (its owner
+// lane decides and writes; lanes that hold a ray twice pass owner only on one): a ray is
+// answered when its largest class probability exceeds gate_prob and it lies outside
 // gate_ring of the critical curve (rg < 0: not coverable); the others join the fallback
 // queue, one atomic per wave.  Every lane of the wave calls it.
-__device__ __forceinline__ void gate_and_write(const FusedArgs& a, long long q, bool live, int h, int lane,
+__device__ __forceinline__ void gate_and_write(const FusedArgs& a, long long q, bool live, bool owner, int lane,
                                                const float y[8], float rg) {
     const float mx = fmaxf(y[0], fmaxf(y[1], y[2]));
     const float e0 = __expf(y[0] - mx), e1 = __expf(y[1] - mx), e2 = __expf(y[2] - mx);
@@ -569,7 +534,7 @@ __device__ __forceinline__ void gate_and_write(const FusedArgs& a, long long q, 
     const float p0 = e0 * inv, p1 = e1 * inv, p2 = e2 * inv;
     const int c = p0 >= p1 && p0 >= p2 ? 0 : (p1 >= p2 ? 1 : 2);
     const float pm = c == 0 ? p0 : c == 1 ? p1 : p2;
-    const bool to_fallback = live && h == 0 && (!(pm > a.gate_prob) || !(rg > a.gate_ring));
+    const bool to_fallback = live && owner && (!(pm > a.gate_prob) || !(rg > a.gate_ring));
     const unsigned long long mask = __ballot(to_fallback);
     if (mask) {
         const int first = __ffsll((long long)mask) - 1;
@@ -578,7 +543,7 @@ __device__ __forceinline__ void gate_and_write(const FusedArgs& a, long long q, 
         base = __shfl(base, first);
         if (to_fallback) a.fallback[base + __popcll(mask & ((1ull << lane) - 1))] = q;
     }
-    if (!live || h != 0 || to_fallback) return;
+    if (!live || !owner || to_fallback) return;
     const double nan = __builtin_nan("");
     double th = nan, ph = nan, rh = nan, g = nan;
     if (c == 0) {
@@ -737,7 +702,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         return;
     }
 
-    gate_and_write(a, q, live, h, lane, y, rg);
+    gate_and_write(a, q, live, h == 0, lane, y, rg);
 }
 
 // ---- the fast network: 5 -> 128 -> 128 -> 128 -> 128 -> 8, hard SiLU --------------------
@@ -758,7 +723,7 @@ constexpr int kFastOutFrags = kFastK * 64;                       // uint4 of the
 constexpr int kFastInFrags = kFastBlocks * 64;                   // uint4 of the input layer
 constexpr int kFastFrags = 3 * kFastLayerFrags + kFastOutFrags + kFastInFrags;  // 6912 (108 KB)
 constexpr int kFastBias = 4 * kFastH + 8;                        // b0..b3 / 6, then b4
-constexpr int kFastThreads = 512, kFastTile = 256;               // 8 waves x 32 rays
+constexpr int kFastThreads = 512, kFastTile = 256;               // 8 waves x 32 rays per network pass
 
 // packed fragments from the FP32 masters: hidden layer l (1-3) block fb, k-block kb, lane
 // (row n = 32 fb + lane % 32, k-slots F(kb, lane / 32, 0..7)); then the output layer
@@ -815,7 +780,78 @@ __device__ __forceinline__ void hsilu_pair(const f32x16& z, bf16x8& p0, bf16x8& 
     for (int i = 0; i < 8; ++i) p1[i] = (__bf16)hsilu_scaled(z[8 + i]);
 }
 
+// the network for the 32 rays of a wave's group on features f (lane m and m + 32 hold ray
+// m): returns the four outputs 4 h .. 4 h + 3 of ray m in lane (m, h)
+__device__ __forceinline__ void fast_network(const uint4* W, const float* Bs, int lane, int h, const float f[5],
+                                             float y4[4]) {
+    const uint4* Win = W + 3 * kFastLayerFrags + kFastOutFrags;
+    const uint4* Wout = W + 3 * kFastLayerFrags;
+    // input layer: split-bf16 features as the B operand, one MFMA per output block
+    bf16x8 B[kFastK];
+    {
+        float fh[5], fl[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            fh[k] = (float)(__bf16)f[k];
+            fl[k] = f[k] - fh[k];
+        }
+        const float x0[8] = {fh[0], fh[1], fh[2], fh[3], fh[4], fl[0], fl[1], fl[2]};
+        const float x1[8] = {fl[3], fl[4], fh[0], fh[1], fh[2], fh[3], fh[4], 0.f};
+        bf16x8 xb;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) xb[k] = (__bf16)(h ? x1[k] : x0[k]);
+#pragma unroll
+        for (int fb = 0; fb < kFastBlocks; ++fb) {
+            const f32x16 z = mfma32(Win[fb * 64 + lane], xb, block_bias(Bs, fb, h));
+            hsilu_pair(z, B[2 * fb], B[2 * fb + 1]);
+        }
+    }
+    // hidden layers 1-3 in output-block order, the previous block's activation beside the
+    // current block's MFMA chain
+#pragma unroll
+    for (int l = 1; l <= 3; ++l) {
+        const uint4* Wl = W + (l - 1) * kFastLayerFrags;
+        const float* bl = Bs + l * kFastH;
+        bf16x8 C[kFastK];
+        f32x16 prev;
+#pragma unroll
+        for (int fb = 0; fb < kFastBlocks; ++fb) {
+            f32x16 acc = block_bias(bl, fb, h);
+            uint4 af[kFastK];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < kFastK; ++i) af[i] = Wl[(fb * kFastK + i) * 64 + lane];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < kFastK; ++i) acc = mfma32(af[i], B[i], acc);
+            if (fb > 0) hsilu_pair(prev, C[2 * fb - 2], C[2 * fb - 1]);
+            prev = acc;
+        }
+        hsilu_pair(prev, C[kFastK - 2], C[kFastK - 1]);
+#pragma unroll
+        for (int k = 0; k < kFastK; ++k) B[k] = C[k];
+    }
+    // output layer: two partial sums over the eight k-blocks, times 6 (the scaled
+    // activations), plus the bias
+    f32x16 o[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) o[p][i] = 0.f;
+#pragma unroll
+    for (int kb = 0; kb < kFastK; ++kb) o[kb & 1] = mfma32(Wout[kb * 64 + lane], B[kb], o[kb & 1]);
+    const float* b4 = Bs + 4 * kFastH;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) y4[r] = (o[0][r] + o[1][r]) * 6.0f + b4[4 * h + r];
+}
+
+// A wave takes 64 rays at a time, one per lane for everything but the network: lane l forms
+// the inputs of ray l and, after the network has run on rays 0-31 and then on rays 32-63
+// (each with the 32x32 layout, in which lanes m and m + 32 share ray m), gates and writes
+// ray l.  The inputs, the gate and the stores, which do not scale with the network, so cost
+// one pass per 64 rays, not two.
 template <bool VIEWS>
+// two waves per SIMD leave room on a CU for the integrator's waves beside it (three measured no faster)
 __global__ __launch_bounds__(kFastThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mlp_fast(FusedArgs a) {
     __shared__ uint4 W[kFastFrags];
     __shared__ __attribute__((aligned(16))) float Bs[kFastBias];
@@ -824,11 +860,10 @@ __global__ __launch_bounds__(kFastThreads) __attribute__((amdgpu_waves_per_eu(2,
     for (int i = tid; i < kFastFrags; i += kFastThreads) W[i] = a.wfast[i];
     for (int i = tid; i < kFastBias; i += kFastThreads) Bs[i] = a.bfast[i];
     __syncthreads();
-    const uint4* Win = W + 3 * kFastLayerFrags + kFastOutFrags;
-    const uint4* Wout = W + 3 * kFastLayerFrags;
-    const long long tiles = (a.n + kFastTile - 1) / kFastTile;
+    constexpr int kPair = 2 * kFastTile;  // rays per block and iteration: 8 waves x 64
+    const long long tiles = (a.n + kPair - 1) / kPair;
     for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        const long long q = tile * kFastTile + wave * 32 + m;
+        const long long q = tile * kPair + wave * 64 + lane;  // this lane's ray
         const bool live = q < a.n;
         const long long qc = live ? q : a.n - 1;
         float f[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
@@ -843,77 +878,33 @@ __global__ __launch_bounds__(kFastThreads) __attribute__((amdgpu_waves_per_eu(2,
 #pragma unroll
             for (int k = 0; k < 5; ++k) f[k] = a.xf[qc * 5 + k];
         }
-        // input layer: split-bf16 features as the B operand, one MFMA per output block
-        bf16x8 B[kFastK];
-        {
-            float fh[5], fl[5];
+        // the two groups' inputs in the network's layout: ray m of group g sits in lane
+        // 32 g + m
+        float g0[5], g1[5];
 #pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                fh[k] = (float)(__bf16)f[k];
-                fl[k] = f[k] - fh[k];
-            }
-            const float x0[8] = {fh[0], fh[1], fh[2], fh[3], fh[4], fl[0], fl[1], fl[2]};
-            const float x1[8] = {fl[3], fl[4], fh[0], fh[1], fh[2], fh[3], fh[4], 0.f};
-            bf16x8 xb;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) xb[k] = (__bf16)(h ? x1[k] : x0[k]);
-#pragma unroll
-            for (int fb = 0; fb < kFastBlocks; ++fb) {
-                const f32x16 z = mfma32(Win[fb * 64 + lane], xb, block_bias(Bs, fb, h));
-                hsilu_pair(z, B[2 * fb], B[2 * fb + 1]);
-            }
+        for (int k = 0; k < 5; ++k) {
+            g0[k] = __shfl(f[k], m);
+            g1[k] = __shfl(f[k], m + 32);
         }
-        // hidden layers 1-3 in output-block order, the previous block's activation beside
-        // the current block's MFMA chain
-#pragma unroll
-        for (int l = 1; l <= 3; ++l) {
-            const uint4* Wl = W + (l - 1) * kFastLayerFrags;
-            const float* bl = Bs + l * kFastH;
-            bf16x8 C[kFastK];
-            f32x16 prev;
-#pragma unroll
-            for (int fb = 0; fb < kFastBlocks; ++fb) {
-                f32x16 acc = block_bias(bl, fb, h);
-                uint4 af[kFastK];
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int i = 0; i < kFastK; ++i) af[i] = Wl[(fb * kFastK + i) * 64 + lane];
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int i = 0; i < kFastK; ++i) acc = mfma32(af[i], B[i], acc);
-                if (fb > 0) hsilu_pair(prev, C[2 * fb - 2], C[2 * fb - 1]);
-                prev = acc;
-            }
-            hsilu_pair(prev, C[kFastK - 2], C[kFastK - 1]);
-#pragma unroll
-            for (int k = 0; k < kFastK; ++k) B[k] = C[k];
-        }
-        // output layer: two partial sums over the eight k-blocks, times 6 (the scaled
-        // activations), plus the bias
-        f32x16 o[2];
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int i = 0; i < 16; ++i) o[p][i] = 0.f;
-#pragma unroll
-        for (int kb = 0; kb < kFastK; ++kb) o[kb & 1] = mfma32(Wout[kb * 64 + lane], B[kb], o[kb & 1]);
-        const float* b4 = Bs + 4 * kFastH;
-        float y4[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) y4[r] = (o[0][r] + o[1][r]) * 6.0f + b4[4 * h + r];
+        float y0[4], y1[4];
+        fast_network(W, Bs, lane, h, g0, y0);
+        fast_network(W, Bs, lane, h, g1, y1);
+        // outputs 0-3 of group 0's ray m sit in lane m, 4-7 in lane m + 32; group 1's alike:
+        // one exchange across the half-waves gives lane l all eight of ray l
         float y[8];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            y[r] = y4[r];
-            y[4 + r] = __shfl_down(y4[r], 32);
+            const float x = __shfl_xor(h ? y0[r] : y1[r], 32);
+            y[r] = h ? x : y0[r];
+            y[4 + r] = h ? y1[r] : x;
         }
         if (!VIEWS) {
-            if (live && h == 0)
+            if (live)
 #pragma unroll
                 for (int k = 0; k < 8; ++k) a.y[q * 8 + k] = y[k];
             continue;
         }
-        gate_and_write(a, q, live, h, lane, y, rg);
+        gate_and_write(a, q, live, true, lane, y, rg);
     }
 }
 
@@ -1356,7 +1347,7 @@ static FusedArgs fused_args(Surrogate* s, long long n) {
 // the network over n rays: k_mlp_fast (one block per CU, over tiles) or k_mlp_fused
 static void launch_network(Surrogate* s, const FusedArgs& a, bool views, hipStream_t st) {
     if (s->net == ktn::kNetFast) {
-        const long long tiles = (a.n + kFastTile - 1) / kFastTile;
+        const long long tiles = (a.n + 2 * kFastTile - 1) / (2 * kFastTile);
         const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(tiles, std::max(1, s->cu_count)));
         if (views) hipLaunchKernelGGL(k_mlp_fast<true>, dim3(grid), dim3(kFastThreads), 0, st, a);
         else hipLaunchKernelGGL(k_mlp_fast<false>, dim3(grid), dim3(kFastThreads), 0, st, a);
@@ -1510,6 +1501,67 @@ static hipError_t label(Surrogate* s, Dataset& d, const std::vector<double>& hp,
     return hipDeviceSynchronize();
 }
 
+// Rays that hit the disk just outside its inner edge, where the disk's image meets the
+// hole's shadow away from the critical curve: the class changes there over a thin band
+// the uniform draw samples sparsely, and a network trained on it is confidently wrong
+// along that edge.  Every training ray that hits within `band` of the inner edge gets
+// `copies` neighbours, moved by up to +-0.005 mean shadow radii in alpha and beta, to be
+// labelled by the integrator like the rest.
+static hipError_t edge_rays(const Surrogate* s, const Dataset& d, const std::vector<double>& hp, double band,
+                            int copies, uint64_t seed, std::vector<double>& out) {
+    out.clear();
+    if (copies <= 0 || !(band > 0.0)) return hipSuccess;
+    const long long n = d.n;
+    std::vector<int8_t> cls((size_t)n);
+    std::vector<double> rh((size_t)n);
+    KTS_TRY(hipMemcpy(cls.data(), d.ocls, (size_t)n, hipMemcpyDeviceToHost));
+    KTS_TRY(hipMemcpy(rh.data(), d.rh, (size_t)n * 8, hipMemcpyDeviceToHost));
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    for (long long q = 0; q < n; ++q) {
+        if ((cls[q] & 7) != 2) continue;
+        const double* p = &hp[(size_t)q * 5];
+        if (!(rh[q] < (1.0 + band) * kt_isco_radius(p[0], 1))) continue;
+        const double R = table_rsh(s->h_rsh.data(), p[0], p[2]);
+        for (int c = 0; c < copies; ++c) {
+            out.insert(out.end(), p, p + 5);
+            out[out.size() - 2] += 0.005 * R * U(rng);
+            out[out.size() - 1] += 0.005 * R * U(rng);
+        }
+    }
+    return hipSuccess;
+}
+
+// appends b's training inputs, classes and targets to a's (b is released)
+static hipError_t append(Dataset& a, Dataset& b) {
+    const long long n = a.n + b.n;
+    float *feat = nullptr, *tgt = nullptr;
+    int8_t* cls = nullptr;
+    hipError_t e = dalloc(&feat, (size_t)n * 5);
+    if (e == hipSuccess) e = dalloc(&tgt, (size_t)n * 5);
+    if (e == hipSuccess) e = dalloc(&cls, (size_t)n);
+    if (e == hipSuccess) e = hipMemcpy(feat, a.feat, (size_t)a.n * 5 * 4, hipMemcpyDeviceToDevice);
+    if (e == hipSuccess) e = hipMemcpy(feat + a.n * 5, b.feat, (size_t)b.n * 5 * 4, hipMemcpyDeviceToDevice);
+    if (e == hipSuccess) e = hipMemcpy(tgt, a.tgt, (size_t)a.n * 5 * 4, hipMemcpyDeviceToDevice);
+    if (e == hipSuccess) e = hipMemcpy(tgt + a.n * 5, b.tgt, (size_t)b.n * 5 * 4, hipMemcpyDeviceToDevice);
+    if (e == hipSuccess) e = hipMemcpy(cls, a.cls, (size_t)a.n, hipMemcpyDeviceToDevice);
+    if (e == hipSuccess) e = hipMemcpy(cls + a.n, b.cls, (size_t)b.n, hipMemcpyDeviceToDevice);
+    if (e != hipSuccess) {
+        for (void* p : {(void*)feat, (void*)tgt, (void*)cls})
+            if (p) (void)hipFree(p);
+        return e;
+    }
+    (void)hipFree(a.feat);
+    (void)hipFree(a.tgt);
+    (void)hipFree(a.cls);
+    a.feat = feat;
+    a.tgt = tgt;
+    a.cls = cls;
+    a.n = n;
+    b.release();
+    return hipSuccess;
+}
+
 static double quantile(std::vector<double> v, double q) {
     if (v.empty()) return std::nan("");
     const size_t k = std::min(v.size() - 1, (size_t)(q * (v.size() - 1) + 0.5));
@@ -1595,6 +1647,16 @@ hipError_t train(Surrogate* s, const TrainOpts& o, TrainReport& rep) {
     std::vector<double> hp;
     draw_rays(s, o.n_train, o.seed, o.band, o.band_factor, hp);
     hipError_t e = label(s, tr, hp, o.label_precision, o);
+    if (e == hipSuccess && o.edge_copies > 0) {  // more rays along the disk's inner edge
+        std::vector<double> ep;
+        e = edge_rays(s, tr, hp, o.edge_band, o.edge_copies, o.seed ^ 0xED6Eull, ep);
+        if (e == hipSuccess && !ep.empty()) {
+            Dataset ed;
+            e = label(s, ed, ep, o.label_precision, o);
+            if (e == hipSuccess) e = append(tr, ed);
+            ed.release();
+        }
+    }
     if (e == hipSuccess) {
         draw_rays(s, o.n_holdout, o.seed ^ 0xABCDEFull, o.band, 1.0, hp);
         e = label(s, ho, hp, KT_PREC_FP64, o);

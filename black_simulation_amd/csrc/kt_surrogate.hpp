@@ -37,6 +37,8 @@ struct TrainOpts {
     double disk_outer = 20.0, escape_radius = 1000.0, capture_factor = 1.01;
     double rtol = 1e-10, atol = 1e-12, rtol32 = 1e-6, atol32 = 1e-7, restep_drift = 1e-6;
     int net = 0;                // ktn::kNet*: the network to train
+    double edge_band = 0.06;    // rays hitting the disk within this of its inner edge ...
+    int edge_copies = 4;        // ... get this many jittered neighbours in the training set
     int verbose = 0;            // progress lines on stderr
 };
 

@@ -26,6 +26,16 @@ __device__ __forceinline__ int find_view(const ViewDev* v, int n, long long idx)
     return lo;
 }
 
+// The view table in LDS, sized to the launch's views (dynamic shared memory, n_views x
+// sizeof(ViewDev) when n_views <= kMaxLdsViews, else none and the table is read from
+// global memory).  Sizing it to the views, not to the maximum, leaves a CU's LDS to other
+// kernels running beside the integrator: the surrogate's network keeps its weights there.
+extern __shared__ ViewDev view_lds[];
+
+__host__ __device__ inline size_t view_lds_bytes(int n_views) {
+    return n_views <= kti::kMaxLdsViews ? sizeof(ViewDev) * (size_t)n_views : 0;
+}
+
 __device__ __forceinline__ const ViewDev* stage_views(const kti::TraceLaunch& L, ViewDev* sv) {
     if (L.n_views > kti::kMaxLdsViews) return L.views;
     const int words = (int)(sizeof(ViewDev) / sizeof(unsigned long long)) * L.n_views;
@@ -112,8 +122,7 @@ __global__ void k_reset(unsigned long long* counter, unsigned long long* stats) 
 // W: the occupancy the register allocator aims for (waves per SIMD); a tuning knob.
 template <class Pol, int W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_trace_persistent(kti::TraceLaunch L) {
-    __shared__ ViewDev sv[kti::kMaxLdsViews];
-    const ViewDev* views = stage_views(L, sv);
+    const ViewDev* views = stage_views(L, view_lds);
     const int lane = (int)(threadIdx.x & 63u);
     const int K = L.steps_per_dequeue;
 
@@ -168,8 +177,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
 
 template <class Pol>
 __global__ __launch_bounds__(256) void k_trace_oneshot(kti::TraceLaunch L) {
-    __shared__ ViewDev sv[kti::kMaxLdsViews];
-    const ViewDev* views = stage_views(L, sv);
+    const ViewDev* views = stage_views(L, view_lds);
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = L.total_dev ? *L.total_dev : L.total;
     if ((long long)blockIdx.x * blockDim.x >= total) return;  // the whole block is past the end
@@ -249,18 +257,19 @@ hipError_t scratch_setup(int device, Scratch* out) {
 template <class Pol>
 void launch_persistent(const TraceLaunch& L, hipStream_t stream) {
     const dim3 grid(L.grid), block(L.block);
+    const size_t lds = view_lds_bytes(L.n_views);
     switch (L.waves_per_simd) {
-        case 2: k_trace_persistent<Pol, 2><<<grid, block, 0, stream>>>(L); break;
-        case 3: k_trace_persistent<Pol, 3><<<grid, block, 0, stream>>>(L); break;
-        case 4: k_trace_persistent<Pol, 4><<<grid, block, 0, stream>>>(L); break;
-        default: k_trace_persistent<Pol, 1><<<grid, block, 0, stream>>>(L); break;
+        case 2: k_trace_persistent<Pol, 2><<<grid, block, lds, stream>>>(L); break;
+        case 3: k_trace_persistent<Pol, 3><<<grid, block, lds, stream>>>(L); break;
+        case 4: k_trace_persistent<Pol, 4><<<grid, block, lds, stream>>>(L); break;
+        default: k_trace_persistent<Pol, 1><<<grid, block, lds, stream>>>(L); break;
     }
 }
 
 template <class Pol>
 void launch_oneshot(const TraceLaunch& L, hipStream_t stream) {
     const dim3 grid((unsigned)((L.total + L.block - 1) / L.block)), block(L.block);
-    k_trace_oneshot<Pol><<<grid, block, 0, stream>>>(L);
+    k_trace_oneshot<Pol><<<grid, block, view_lds_bytes(L.n_views), stream>>>(L);
 }
 
 hipError_t launch_trace(const TraceLaunch& L, hipStream_t stream) {

@@ -251,6 +251,9 @@ typedef struct kt_surrogate_opts {
     double disk_outer, escape_radius, capture_factor;  /* the scene it learns (20, 1000, 1.01) */
     int verbose;                   /* 1: one progress line per epoch on stderr          */
     int net;                       /* KT_NET_* to train (KT_NET_WIDE)                   */
+    double edge_band;              /* training rays hitting the disk within edge_band of */
+    int edge_copies;               /* its inner edge get edge_copies jittered neighbours */
+                                   /* (0.06, 4; edge_copies < 0: none)                  */
 } kt_surrogate_opts;
 
 typedef struct kt_surrogate_report {

@@ -123,17 +123,19 @@ def test_c3_batch_of_64_views_in_one_mixed_launch(tracer):
     assert np.quantile(g, 0.99) < 1e-4 and np.quantile(r, 0.99) < 1e-4
 
 
-@pytest.mark.parametrize("precision", [bsa.PREC_MIXED, bsa.PREC_FP64])
-def test_c5_orbit_frames_surrogate_and_fallback(tracer, precision):
+@pytest.mark.parametrize("precision,net", [(bsa.PREC_MIXED, "fast"), (bsa.PREC_FP64, "fast"),
+                                           (bsa.PREC_MIXED, "wide")])
+def test_c5_orbit_frames_surrogate_and_fallback(tracer, precision, net):
     """Four frames spread over the a = 0.998 orbit (1/10 size, 192x108) through the network,
     the gate and the integrator fallback in one launch each.  All rays: the same class as
     the oracle (boundary-excused) for >= 99.9 %.  Fallback rays: with the FP64 fallback
     bit-equal to the oracle, with the mixed one within its stated errors.  Answered rays
     (the network's own outputs): class agreement >= 99.9 %, and their errors against the
     oracle printed per class with ceilings at a few times what the shipped network gives."""
-    from black_simulation_amd.train_surrogate import load_params
+    from black_simulation_amd.train_surrogate import DEFAULT_WEIGHTS, FAST_WEIGHTS, load_into
 
-    tracer.surrogate_load(load_params())
+    load_into(tracer, FAST_WEIGHTS if net == "fast" else DEFAULT_WEIGHTS)
+    assert tracer.surrogate_net() == (bsa.NET_FAST if net == "fast" else bsa.NET_WIDE)
     frames = [0, 40, 80, C5_FRAMES - 1]
     views = [c5_view(k, 192, 108) for k in frames]
     opts = bsa.TraceOpts(precision=precision, use_surrogate=1)
@@ -141,7 +143,7 @@ def test_c5_orbit_frames_surrogate_and_fallback(tracer, precision):
     tot, d, g, r, ans = compare(views, got, fp64_fallback_exact=precision == bsa.PREC_FP64)
     agree = (tot["agree"] + tot["excused"]) / tot["rays"]
     a_agree = ans["agree"] / max(1, ans["rays"])
-    print(f"C5 frames {frames} at 192x108, fallback {'mixed' if precision == bsa.PREC_MIXED else 'FP64'}: "
+    print(f"C5 frames {frames} at 192x108, {net} network, fallback {'mixed' if precision == bsa.PREC_MIXED else 'FP64'}: "
           f"class agreement {tot['agree'] / tot['rays']:.6f} ({tot['excused']} boundary rays excused); "
           f"answered {ans['rays']} of {tot['rays']} (escaped, captured, disk, unfinished: {ans['by_class'].tolist()}), "
           f"their class agreement {a_agree:.6f}; answered deflection p50/p99/p99.9 {q3(ans['defl'])} rad, "
