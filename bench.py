@@ -6,8 +6,9 @@ inclination 20 -> 85 deg), Bardeen plane +-1.5 mean shadow radii, disk r_ISCO..2
 capture 1.01 r+.  One step = one full frame (2,073,600 rays), replayed from a hipGraph
 plan.  --path surrogate: the MLP (by default the fast network, 5-128x4-8 with hard SiLU)
 answers rays with max class probability > 0.999 lying outside +-3% of the critical
-curve, the mixed FP32/FP64 integrator the rest, all in one graph; --path plain: the FP64 integrator; --path mixed: the FP32/FP64 integrator.
-Both the chosen path and the plain FP64 integrator are timed on the same frames, and
+curve, the integrator the rest (FP64 DP5 at rtol 1e-6; --fallback mixed: the FP32/FP64
+one), all in one graph; --path plain: the FP64 integrator (rtol 1e-10); --path mixed:
+the FP32/FP64 integrator (rtol 1e-6).  Both the chosen path and the plain FP64 integrator are timed on the same frames, and
 the chosen path's outputs are judged against the FP64 render of every frame.
 --workload c2 times the single a=0.998 frame at r = 50 M, i = 85 deg (disk to 40 M)
 instead; the surrogate does not cover that scene.
@@ -347,6 +348,9 @@ def main():
     ap.add_argument("--mode", choices=["auto", "oneshot", "persistent"], default="auto",
                     help="integrator kernel (auto: the autotuner's pick, or persistent with --lean)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fallback", choices=["mixed", "fp64"], default="fp64",
+                    help="the integrator of --path surrogate: FP64 (default) at rtol 1e-6, or the mixed FP32/FP64 "
+                         "one; --rtol sets its tolerance")
     ap.add_argument("--rtol", type=float, default=0.0,
                     help="DP5 tolerance of the chosen path (FP64 rtol for --path plain, the FP32 bulk's for mixed "
                          "and surrogate; atol = rtol / 10); the reference render stays FP64 at rtol 1e-10")
@@ -415,8 +419,12 @@ def main():
         if args.rtol:  # FP64 at another tolerance, judged against FP64 at rtol 1e-10
             opts = tune(bsa.TraceOpts(precision=bsa.PREC_FP64, mode=mode, rtol=args.rtol, atol=args.rtol / 10))
     else:
-        opts = tune(bsa.TraceOpts(precision=bsa.PREC_MIXED, mode=mode,
-                                  **({"rtol32": args.rtol, "atol32": args.rtol / 10} if args.rtol else {})))
+        if args.path == "surrogate" and args.fallback == "fp64":
+            rtol = args.rtol or 1e-6
+            opts = tune(bsa.TraceOpts(precision=bsa.PREC_FP64, mode=mode, rtol=rtol, atol=rtol / 10))
+        else:
+            opts = tune(bsa.TraceOpts(precision=bsa.PREC_MIXED, mode=mode,
+                                      **({"rtol32": args.rtol, "atol32": args.rtol / 10} if args.rtol else {})))
         if args.path == "surrogate":
             load_into(tr, args.weights or FAST_WEIGHTS)
             opts.use_surrogate = 1
@@ -603,6 +611,9 @@ def main():
                 "black_simulation_amd.train_surrogate on integrator-labelled rays",
         "config": {"workload": f"{args.path}: {desc}", "path": args.path, "rays_per_step": rays,
                    "network": net_desc,
+                   "fallback": (f"{'FP64' if opts.precision == bsa.PREC_FP64 else 'mixed FP32/FP64'} DP5 rtol "
+                                f"{opts.rtol if opts.precision == bsa.PREC_FP64 else opts.rtol32:g}")
+                   if args.path == "surrogate" else None,
                    "streams": args.streams, "lean": args.lean,
                    "frame_ms_p50": round(percentile(lat, 0.5), 3), "frame_ms_p99": round(percentile(lat, 0.99), 3),
                    "plain_fp64": {"value": round(value_p, 3), "frame_ms_p50": round(percentile(lat_p, 0.5), 3),
