@@ -16,33 +16,63 @@ namespace {
 
 constexpr int kLdsStride = 40;  // bf16 per LDS row: 32 + 8 pad (80 B, keeps 16-byte alignment)
 
-// 128 rows x 32 k of a tile into LDS [row][k].  RC = false: element (row, k) at P[row * ld + k]
-// (reduction-contiguous); RC = true: at P[k * ld + row].  Rows >= R read as zero.
+// 128 rows x 32 k of a tile, from global memory into registers (fetch) and from there into
+// LDS [row][k] (put).  RC = false: element (row, k) at P[row * ld + k] (reduction-
+// contiguous), two 16-byte chunks per thread.  RC = true: at P[k * ld + row]; an 8 (k) x 8
+// (row) block per thread of the first wave, eight 16-byte loads along the rows, transposed
+// in registers (v_perm_b32 pairs the halves of two words) into eight 16-byte stores along
+// k.  Rows >= R read as zero.
 template <bool RC>
-__device__ __forceinline__ void load_tile(uint16_t* T, const uint16_t* P, int ld, int r0, int k0, int R, int tid) {
+__device__ __forceinline__ void fetch(uint4 (&v)[8], const uint16_t* P, int ld, int r0, int k0, int R, int tid) {
     if (!RC) {
-        for (int c = tid; c < 512; c += 256) {
-            const int row = c >> 2, kc = (c & 3) * 8;
-            uint4 v = {0u, 0u, 0u, 0u};
-            if (r0 + row < R) v = *reinterpret_cast<const uint4*>(P + (long long)(r0 + row) * ld + k0 + kc);
-            *reinterpret_cast<uint4*>(T + row * kLdsStride + kc) = v;
-        }
-    } else {
-        for (int c = tid; c < 512; c += 256) {
-            const int kr = c >> 4, rc = (c & 15) * 8;
-            uint4 v = {0u, 0u, 0u, 0u};
-            if (r0 + rc < R) v = *reinterpret_cast<const uint4*>(P + (long long)(k0 + kr) * ld + r0 + rc);
-            const uint16_t* e = reinterpret_cast<const uint16_t*>(&v);
 #pragma unroll
-            for (int j = 0; j < 8; ++j) T[(rc + j) * kLdsStride + kr] = e[j];
+        for (int i = 0; i < 2; ++i) {
+            const int c = tid + 256 * i, row = c >> 2, kc = (c & 3) * 8;
+            v[i] = uint4{0u, 0u, 0u, 0u};
+            if (r0 + row < R) v[i] = *reinterpret_cast<const uint4*>(P + (long long)(r0 + row) * ld + k0 + kc);
+        }
+    } else if (tid < 64) {
+        const int kb = (tid >> 4) * 8, rc = (tid & 15) * 8;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            v[i] = uint4{0u, 0u, 0u, 0u};
+            if (r0 + rc < R) v[i] = *reinterpret_cast<const uint4*>(P + (long long)(k0 + kb + i) * ld + r0 + rc);
         }
     }
 }
 
+template <bool RC>
+__device__ __forceinline__ void put(const uint4 (&v)[8], uint16_t* T, int tid) {
+    if (!RC) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int c = tid + 256 * i, row = c >> 2, kc = (c & 3) * 8;
+            *reinterpret_cast<uint4*>(T + row * kLdsStride + kc) = v[i];
+        }
+    } else if (tid < 64) {
+        const int kb = (tid >> 4) * 8, rc = (tid & 15) * 8;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            // row rc + j: element j of each of the eight k-rows
+            const unsigned sel = (j & 1) ? 0x07060302u : 0x05040100u;  // high halves : low halves
+            const int w = j >> 1;
+            auto word = [w](const uint4& x) -> unsigned { return w == 0 ? x.x : w == 1 ? x.y : w == 2 ? x.z : x.w; };
+            uint4 o;
+            o.x = __builtin_amdgcn_perm(word(v[1]), word(v[0]), sel);
+            o.y = __builtin_amdgcn_perm(word(v[3]), word(v[2]), sel);
+            o.z = __builtin_amdgcn_perm(word(v[5]), word(v[4]), sel);
+            o.w = __builtin_amdgcn_perm(word(v[7]), word(v[6]), sel);
+            *reinterpret_cast<uint4*>(T + (rc + j) * kLdsStride + kb) = o;
+        }
+    }
+}
+
+// Double-buffered: the next k-slice's global loads are in flight while the matrix cores work
+// on the current one from LDS; one barrier per slice.
 template <bool AT, bool BT, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
-    __shared__ __attribute__((aligned(16))) uint16_t As[128 * kLdsStride];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[128 * kLdsStride];
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][128 * kLdsStride];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][128 * kLdsStride];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int m0 = blockIdx.x * 128, n0 = blockIdx.y * 128;
@@ -54,58 +84,133 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int fr = lane & 15, fk = (lane >> 4) * 8;
-    for (int k0 = kb; k0 < ke; k0 += 32) {
-        load_tile<AT>(As, g.A, g.lda, m0, k0, g.M, tid);
-        load_tile<!BT>(Bs, g.B, g.ldb, n0, k0, g.N, tid);
-        __syncthreads();
+    uint4 va[8], vb[8];
+    if (kb < ke) {
+        fetch<AT>(va, g.A, g.lda, m0, kb, g.M, tid);
+        fetch<!BT>(vb, g.B, g.ldb, n0, kb, g.N, tid);
+        put<AT>(va, As[0], tid);
+        put<!BT>(vb, Bs[0], tid);
+    }
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = kb; k0 < ke; k0 += 32, buf ^= 1) {
+        const bool more = k0 + 32 < ke;
+        if (more) {
+            fetch<AT>(va, g.A, g.lda, m0, k0 + 32, g.M, tid);
+            fetch<!BT>(vb, g.B, g.ldb, n0, k0 + 32, g.N, tid);
+        }
         bf16x8 af[4], bfr[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-            af[i] = as_bf16x8(*reinterpret_cast<const uint4*>(As + (64 * wm + 16 * i + fr) * kLdsStride + fk));
+            af[i] = as_bf16x8(*reinterpret_cast<const uint4*>(As[buf] + (64 * wm + 16 * i + fr) * kLdsStride + fk));
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-            bfr[j] = as_bf16x8(*reinterpret_cast<const uint4*>(Bs + (64 * wn + 16 * j + fr) * kLdsStride + fk));
+            bfr[j] = as_bf16x8(*reinterpret_cast<const uint4*>(Bs[buf] + (64 * wn + 16 * j + fr) * kLdsStride + fk));
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = mfma16(af[i], bfr[j], acc[i][j]);
+            for (int j = 0; j < 4; ++j) acc[i][j] = mfma16(bfr[j], af[i], acc[i][j]);  // C^T: columns in-lane
+        if (more) {
+            put<AT>(va, As[buf ^ 1], tid);
+            put<!BT>(vb, Bs[buf ^ 1], tid);
+        }
         __syncthreads();
     }
-    const int col0 = n0 + 64 * wn + fr, row0 = m0 + 64 * wm + (lane >> 4) * 4;
+    // the tile was formed transposed, so lane l holds C[row0 + 16 i][col0 + 16 j + q], q < 4:
+    // four consecutive columns, stored as one 16-byte (FP32) or 8-byte (BF16) access
+    const int row0 = m0 + 64 * wm + fr, col0 = n0 + 64 * wn + (lane >> 4) * 4;
+    f32x4 cs[4];  // Bwd: this lane's column sums of g over its 4 rows
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) cs[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = row0 + 16 * i;
+        if (r >= g.M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int c = col0 + 16 * j;  // N is a multiple of 4: the four columns are in or out together
+            if (c >= g.N) continue;
+            const f32x4 v = acc[i][j];
+            const long long o = (long long)r * g.ldc + c;
+            if (EPI == kEpiFwd || EPI == kEpiLin) {
+                f32x4 z;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) z[q] = v[q] + g.bias[c + q];
+                if (EPI == kEpiLin || g.out32) *reinterpret_cast<f32x4*>(g.out32 + o) = z;
+                if (EPI == kEpiFwd) {
+                    uint2 h;
+                    h.x = (unsigned)f2bf(activate(z[0], g.act)) | ((unsigned)f2bf(activate(z[1], g.act)) << 16);
+                    h.y = (unsigned)f2bf(activate(z[2], g.act)) | ((unsigned)f2bf(activate(z[3], g.act)) << 16);
+                    *reinterpret_cast<uint2*>(g.outb + o) = h;
+                }
+            } else if (EPI == kEpiBwd) {
+                const f32x4 z = *reinterpret_cast<const f32x4*>(g.zin + (long long)r * g.ldz + c);
+                f32x4 gz;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) gz[q] = v[q] * dactivate(z[q], g.act);
+                if (g.out32) *reinterpret_cast<f32x4*>(g.out32 + o) = gz;
+                uint2 h;
+                h.x = (unsigned)f2bf(gz[0]) | ((unsigned)f2bf(gz[1]) << 16);
+                h.y = (unsigned)f2bf(gz[2]) | ((unsigned)f2bf(gz[3]) << 16);
+                *reinterpret_cast<uint2*>(g.outb + o) = h;
+                cs[j] += gz;
+            } else {  // kEpiWs: split-K partial
+                *reinterpret_cast<f32x4*>(g.out32 + (long long)blockIdx.z * g.ws_stride + o) = v;
+            }
+        }
+    }
+    if (EPI == kEpiBwd && g.colpart) {
+        // sum over the 16 rows the lanes of one column group hold (lanes differing in bits
+        // 0-3), in a fixed order; one partial row per (row tile, wave row): 2 per 128 rows
 #pragma unroll
         for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int r = row0 + 16 * i + q, c = col0 + 16 * j;
-                if (r >= g.M || c >= g.N) continue;
-                const float v = acc[i][j][q];
-                const long long o = (long long)r * g.ldc + c;
-                if (EPI == kEpiFwd) {
-                    const float z = v + g.bias[c];
-                    if (g.out32) g.out32[o] = z;
-                    g.outb[o] = f2bf(activate(z, g.act));
-                } else if (EPI == kEpiLin) {
-                    g.out32[o] = v + g.bias[c];
-                } else if (EPI == kEpiBwd) {
-                    const float gz = v * dactivate(g.zin[(long long)r * g.ldz + c], g.act);
-                    g.out32[o] = gz;
-                    g.outb[o] = f2bf(gz);
-                } else {  // kEpiWs: split-K partial
-                    g.out32[(long long)blockIdx.z * g.ws_stride + o] = v;
-                }
+                float x = cs[j][q];
+                x += __shfl_xor(x, 1);
+                x += __shfl_xor(x, 2);
+                x += __shfl_xor(x, 4);
+                x += __shfl_xor(x, 8);
+                cs[j][q] = x;
             }
+        if (fr == 0) {
+            float* dst = g.colpart + (long long)(2 * blockIdx.x + wm) * g.N;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int c = col0 + 16 * j;
+                if (c < g.N) *reinterpret_cast<f32x4*>(dst + c) = cs[j];
+            }
+        }
+    }
 }
 
 // sum over splits: dst[i * N + j] = sum_s ws[s][i][j] for i < rows
-__global__ void k_reduce_ws(const float* ws, int splits, long long stride, int rows, int N, int ld, float* dst) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= rows * N) return;
-    const int i = idx / N, j = idx - i * N;
-    float s = 0.f;
-    for (int k = 0; k < splits; ++k) s += ws[k * stride + (long long)i * ld + j];
-    dst[idx] = s;
+// dst[i * N + j] = sum over k < slices of src[k * stride + i * ld + j], for i < rows, j < N
+// (N, ld and stride multiples of 4, src 16-byte aligned; dst dense).  A block of 256 threads
+// takes 16 consecutive outputs as four 16-byte groups, 64 threads per group each summing
+// every 64th slice; then one thread per output adds the 64 sums in a fixed order, so the
+// result does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_sum_slices(const float* src, int slices, long long stride, int rows, int N,
+                                                    int ld, float* dst) {
+    __shared__ f32x4 red[64][4];
+    const int cg = threadIdx.x & 3, sg = threadIdx.x >> 2;
+    const long long total = (long long)rows * N;
+    const long long o = (long long)blockIdx.x * 16 + cg * 4;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    if (o < total) {
+        const int i = (int)(o / N), j = (int)(o - (long long)i * N);
+        const float* p = src + (long long)i * ld + j;
+#pragma unroll 4
+        for (int k = sg; k < slices; k += 64) acc += *reinterpret_cast<const f32x4*>(p + (long long)k * stride);
+    }
+    red[sg][cg] = acc;
+    __syncthreads();
+    const long long oo = (long long)blockIdx.x * 16 + threadIdx.x;
+    if (threadIdx.x < 16 && oo < total) {
+        float t = 0.f;
+        for (int k = 0; k < 64; ++k) t += red[k][threadIdx.x >> 2][threadIdx.x & 3];
+        dst[oo] = t;
+    }
 }
 
 // column sums of G (B x N fp32, row-major) in two passes: partial[chunk][j], then dst[j]
@@ -119,16 +224,6 @@ __global__ void k_colsum_partial(const float* G, int B, int N, int rows_per_chun
     partial[(long long)blockIdx.x * N + j] = s;
 }
 
-__global__ void k_colsum_final(const float* partial, int chunks, int N, float* dst) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= N) return;
-    float s = 0.f;
-    for (int c = 0; c < chunks; ++c) s += partial[(long long)c * N + j];
-    dst[j] = s;
-}
-
-// loss head: softmax cross-entropy on outputs 0..2, squared error on 3..5 (escaped) and
-// 6..7 (disk); writes dL/dy / B (fp32 B x 8 and bf16 padded B x 32) and accumulates the loss
 __global__ void k_loss_head(const float* Y, const int8_t* cls, const float* tgt, int B, float w_sky, float w_disk,
                             float* gy32, uint16_t* gyb, double* loss_sum) {
     __shared__ float red[256];
@@ -272,13 +367,6 @@ __global__ __launch_bounds__(256) void k_layer0_grad_partial(const float* xf, co
     for (int k = 0; k <= kInputs; ++k) o[k * H + j] = s[k];
 }
 
-__global__ void k_layer0_grad_final(const float* partial, int chunks, int H, float* G) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;  // (k, j), k <= 5
-    if (e >= (kInputs + 1) * H) return;
-    float s = 0.f;
-    for (int c = 0; c < chunks; ++c) s += partial[(long long)c * (kInputs + 1) * H + e];
-    G[e] = s;  // W0 (5 x H row-major) is followed directly by b0
-}
 
 }  // namespace
 
@@ -293,12 +381,14 @@ hipError_t layer0_grad(const float* xf, const float* g, int B, int H, float* par
     if (H < 64 || H > kMaxHidden || H % 64) return hipErrorInvalidValue;
     const int rows = (B + chunks - 1) / chunks;
     hipLaunchKernelGGL(k_layer0_grad_partial, dim3(chunks), dim3(H), 0, s, xf, g, B, rows, partial);
-    hipLaunchKernelGGL(k_layer0_grad_final, dim3(((kInputs + 1) * H + 255) / 256), dim3(256), 0, s, partial,
-                       chunks, H, G);
-    return hipGetLastError();
+    // W0 (5 x H row-major) is followed directly by b0: one (k, j) column per parameter
+    const int ncols = (kInputs + 1) * H;
+    const hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : sum_chunks(partial, chunks, ncols, G, s);
 }
 
 hipError_t gemm(const GemmArgs& g, bool at, bool bt, int epi, int splits, hipStream_t stream) {
+    if (g.N % 4 || g.ldc % 4 || (epi == kEpiBwd && g.ldz % 4)) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((g.M + 127) / 128), (unsigned)((g.N + 127) / 128), (unsigned)splits);
     const dim3 block(256);
 #define KTN_CASE(A_, B_, E_)                                                          \
@@ -318,16 +408,22 @@ hipError_t gemm(const GemmArgs& g, bool at, bool bt, int epi, int splits, hipStr
 
 hipError_t reduce_ws(const float* ws, int splits, long long stride, int rows, int N, int ld, float* dst,
                      hipStream_t s) {
-    const int n = rows * N;
-    hipLaunchKernelGGL(k_reduce_ws, dim3((n + 255) / 256), dim3(256), 0, s, ws, splits, stride, rows, N, ld, dst);
+    if (N % 4 || ld % 4 || stride % 4) return hipErrorInvalidValue;
+    const long long n = (long long)rows * N;
+    hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, s, ws, splits, stride, rows, N, ld,
+                       dst);
     return hipGetLastError();
+}
+
+hipError_t sum_chunks(const float* partial, int chunks, int ncols, float* dst, hipStream_t s) {
+    return reduce_ws(partial, chunks, ncols, 1, ncols, 0, dst, s);
 }
 
 hipError_t colsum(const float* G, int B, int N, float* partial, int chunks, float* dst, hipStream_t s) {
     const int rpc = (B + chunks - 1) / chunks;
     hipLaunchKernelGGL(k_colsum_partial, dim3(chunks), dim3(256), 0, s, G, B, N, rpc, partial);
-    hipLaunchKernelGGL(k_colsum_final, dim3((N + 255) / 256), dim3(256), 0, s, partial, chunks, N, dst);
-    return hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : sum_chunks(partial, chunks, N, dst, s);
 }
 
 hipError_t loss_head(const float* Y, const int8_t* cls, const float* tgt, int B, float w_sky, float w_disk,

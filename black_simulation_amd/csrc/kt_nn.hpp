@@ -59,6 +59,7 @@ struct GemmArgs {
     int ldz;
     int k_split;         // reduction length per blockIdx.z (a multiple of 32)
     long long ws_stride; // Ws: elements per split
+    float* colpart;      // Bwd (optional): column sums of g, one row of N per 64 rows of C
 };
 
 struct LayerTable {
@@ -68,11 +69,14 @@ struct LayerTable {
     uint16_t* wt[kLayers];
 };
 
-// C = A B with A reduction-contiguous unless `at`, B given as B^T ([n][k]) when `bt`.
+// C = A B with A reduction-contiguous unless `at`, B given as B^T ([n][k]) when `bt`.  N and
+// ldc are multiples of 4 (four consecutive columns leave one lane together).
 hipError_t gemm(const GemmArgs& g, bool at, bool bt, int epi, int splits, hipStream_t s);
 hipError_t reduce_ws(const float* ws, int splits, long long stride, int rows, int N, int ld, float* dst,
                      hipStream_t s);
 hipError_t colsum(const float* G, int B, int N, float* partial, int chunks, float* dst, hipStream_t s);
+// dst[j] = sum over c < chunks of partial[c * ncols + j], in a fixed order
+hipError_t sum_chunks(const float* partial, int chunks, int ncols, float* dst, hipStream_t s);
 hipError_t loss_head(const float* Y, const int8_t* cls, const float* tgt, int B, float w_sky, float w_disk,
                      float* gy32, uint16_t* gyb, double* loss_sum, hipStream_t s);
 hipError_t adam(float* P, const float* G, float* M, float* V, long long n, float lr, float b1, float b2, float eps,

@@ -949,6 +949,7 @@ struct Surrogate {
     float* ws = nullptr;
     float* partial = nullptr;
     float* partial0 = nullptr;  // input-layer gradient partials
+    float* colpart = nullptr;   // hidden-layer bias gradient partials, one row per 64 batch rows
     double* loss = nullptr;
     uint16_t* wpk = nullptr;  // layers 1-4 packed for the fused kernel (bf16)
     float* fold = nullptr;    // input layer and biases times -log2(e) for the fused kernel
@@ -957,7 +958,9 @@ struct Surrogate {
     float* bfast = nullptr;     // and its biases
     Queue queue;              // fallback queue of eager calls
     Domain dom;               // what the parameters cover
-    static constexpr int kSplits = 64, kChunks = 256;
+    // split-K slices of the weight gradients; row chunks of the bias and input-layer
+    // gradient sums (64 rows each at the default batch)
+    static constexpr int kSplits = 128, kChunks = 1024;
 };
 
 // frees what a surrogate holds; device frees during teardown report nothing actionable
@@ -1197,6 +1200,7 @@ static hipError_t ensure_train_buffers(Surrogate* s, long long B) {
     f(s->bidx);
     f(s->ws);
     f(s->partial);
+    f(s->colpart);
     s->cap = 0;
     KTS_TRY(dalloc(&s->xf, (size_t)B * 5));
     KTS_TRY(dalloc(&s->partial0, (size_t)Surrogate::kChunks * 6 * 256));
@@ -1216,6 +1220,7 @@ static hipError_t ensure_train_buffers(Surrogate* s, long long B) {
     KTS_TRY(dalloc(&s->bidx, (size_t)B));
     KTS_TRY(dalloc(&s->ws, (size_t)Surrogate::kSplits * 256 * 256));
     KTS_TRY(dalloc(&s->partial, (size_t)Surrogate::kChunks * 256));
+    KTS_TRY(dalloc(&s->colpart, (size_t)((B + 127) / 128) * 2 * 256));
     s->cap = B;
     return hipSuccess;
 }
@@ -1260,7 +1265,7 @@ static hipError_t forward(Surrogate* s, const float* xf, uint16_t** H, float** Z
 // backward of the mean loss; gradients into s->G
 static hipError_t backward(Surrogate* s, int B, hipStream_t st) {
     const int W = s->hid;
-    const int splits = std::max(1, std::min(Surrogate::kSplits, B / 1024));
+    const int splits = std::max(1, std::min(Surrogate::kSplits, B / 512));
     const int ksplit = ((B / splits + 31) / 32) * 32;
     auto dW = [&](const uint16_t* A, int lda, int M, const uint16_t* Bm, int ldb, int N, int l) -> hipError_t {
         ktn::GemmArgs g{};
@@ -1284,47 +1289,40 @@ static hipError_t backward(Surrogate* s, int B, hipStream_t st) {
         return ktn::colsum(Gm, B, N, s->partial, Surrogate::kChunks,
                            s->G + ktn::param_offset(W, l) + (long long)ktn::kreal(W, l) * ktn::nout(W, l), st);
     };
+    // the bias gradients of hidden layers 3..1 come from the column sums the Bwd GEMM that
+    // forms g leaves in colpart; layer 0 takes g in FP32 instead
+    const int parts = 2 * ((B + 127) / 128);
+    auto bwd = [&](const uint16_t* A, int lda, int K, const uint16_t* Wt, int ldw, const float* Z, int dst_l,
+                   int out) -> hipError_t {
+        ktn::GemmArgs g{};
+        g.M = B;
+        g.N = W;
+        g.K = K;
+        g.A = A;
+        g.lda = lda;
+        g.B = Wt;
+        g.ldb = ldw;
+        g.ldc = W;
+        g.zin = Z;
+        g.ldz = W;
+        g.out32 = dst_l == 0 ? s->g32[out] : nullptr;
+        g.outb = s->gb[out];
+        g.colpart = dst_l == 0 ? nullptr : s->colpart;
+        g.k_split = K;
+        g.act = s->act;
+        KTS_TRY(ktn::gemm(g, false, true, ktn::kEpiBwd, 1, st));
+        if (dst_l == 0) return hipSuccess;
+        return ktn::sum_chunks(s->colpart, parts, W,
+                               s->G + ktn::param_offset(W, dst_l) + (long long)ktn::kreal(W, dst_l) * W, st);
+    };
     // output layer
     KTS_TRY(dW(s->H[3], W, W, s->gyb, 32, 8, 4));
     KTS_TRY(db(s->gy32, 8, 4));
-    {
-        ktn::GemmArgs g{};
-        g.M = B;
-        g.N = W;
-        g.K = 32;
-        g.A = s->gyb;
-        g.lda = 32;
-        g.B = s->wb[4];
-        g.ldb = 32;
-        g.ldc = W;
-        g.zin = s->Z[3];
-        g.ldz = W;
-        g.out32 = s->g32[1];
-        g.outb = s->gb[1];
-        g.k_split = 32;
-        g.act = s->act;
-        KTS_TRY(ktn::gemm(g, false, true, ktn::kEpiBwd, 1, st));
-    }
+    KTS_TRY(bwd(s->gyb, 32, 32, s->wb[4], 32, s->Z[3], 3, 1));
     int cur = 1;
     for (int l = 3; l >= 1; --l) {
         KTS_TRY(dW(s->H[l - 1], W, W, s->gb[cur], W, W, l));
-        KTS_TRY(db(s->g32[cur], W, l));
-        ktn::GemmArgs g{};
-        g.M = B;
-        g.N = W;
-        g.K = W;
-        g.A = s->gb[cur];
-        g.lda = W;
-        g.B = s->wb[l];
-        g.ldb = W;
-        g.ldc = W;
-        g.zin = s->Z[l - 1];
-        g.ldz = W;
-        g.out32 = s->g32[cur ^ 1];
-        g.outb = s->gb[cur ^ 1];
-        g.k_split = W;
-        g.act = s->act;
-        KTS_TRY(ktn::gemm(g, false, true, ktn::kEpiBwd, 1, st));
+        KTS_TRY(bwd(s->gb[cur], W, W, s->wb[l], W, s->Z[l - 1], l - 1, cur ^ 1));
         cur ^= 1;
     }
     // input layer in FP32 (its weights and bias are the first 6 x W parameters)
