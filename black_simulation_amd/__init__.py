@@ -3,7 +3,7 @@
 The product is libkerr_trace.so, built in-tree from csrc/ (see build.py); this
 package loads it with ctypes.  Public names mirror the reference's entry points.
 """
-from .api import (TraceOpts, Tracer, View, blackbody_table, circular_omega, critical_curve, critical_radius,
+from .api import (REPORT_FIELDS, TraceOpts, Tracer, View, blackbody_table, circular_omega, critical_curve, critical_radius,
                   geodesic_four_velocity, horizon_radius, observer_frame, isco_radius, photon_orbit_radius, render, shadow_radius, to_numpy, trace_looks,
                   trace_ray)
 from ._lib import (CAMERA_BARDEEN, CAMERA_OBSERVER, CAMERA_PINHOLE, CAPTURED, DISK, ESCAPED, FLAG_FP64_RESTEP, FLAG_SURROGATE,
@@ -12,7 +12,7 @@ from ._lib import (CAMERA_BARDEEN, CAMERA_OBSERVER, CAMERA_PINHOLE, CAPTURED, DI
                    SKY_IMAGE, UNFINISHED)
 
 __all__ = [
-    "TraceOpts", "Tracer", "View", "blackbody_table", "circular_omega", "critical_curve", "critical_radius",
+    "REPORT_FIELDS", "TraceOpts", "Tracer", "View", "blackbody_table", "circular_omega", "critical_curve", "critical_radius",
     "geodesic_four_velocity", "horizon_radius", "observer_frame", "isco_radius", "photon_orbit_radius", "render", "shadow_radius", "to_numpy", "trace_looks",
     "trace_ray",
     "CAMERA_BARDEEN", "CAMERA_OBSERVER", "CAMERA_PINHOLE", "CAPTURED", "DISK", "ESCAPED", "FLAG_FP64_RESTEP", "FLAG_SURROGATE",

@@ -82,6 +82,11 @@ class KtRaysOut(C.Structure):
         ("phi_inf_f32", C.c_void_p),
         ("r_hit_f32", C.c_void_p),
         ("g_f32", C.c_void_p),
+        ("turns", C.c_void_p),
+        ("r_min", C.c_void_p),
+        ("delay", C.c_void_p),
+        ("b", C.c_void_p),
+        ("q", C.c_void_p),
     ]
 
 

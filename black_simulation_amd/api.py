@@ -24,6 +24,9 @@ from . import _lib as K
 
 OUTPUT_FIELDS = ("cls", "theta_inf", "phi_inf", "r_hit", "g", "steps")
 FLOAT_FIELDS = ("theta_inf", "phi_inf", "r_hit", "g")  # float64, or float32 (kt_rays_out *_f32)
+# the single-ray report (trajectory::RayReport, src/trajectory.h:97-113), float64; any of
+# them in `out` selects the FP64 report trace
+REPORT_FIELDS = ("turns", "r_min", "delay", "b", "q")
 
 
 @dataclasses.dataclass
@@ -148,13 +151,14 @@ class Tracer:
         self.close()
 
     # -- device outputs -------------------------------------------------------------
-    def alloc_outputs(self, n: int, f32: bool = False) -> dict:
+    def alloc_outputs(self, n: int, f32: bool = False, report: bool = False) -> dict:
         """Device outputs for n rays; f32: the sky angles, disk radius and g in float (each
-        the FP64 result rounded once), 21 instead of 37 bytes per ray."""
+        the FP64 result rounded once), 21 instead of 37 bytes per ray; report: also the
+        single-ray report fields (REPORT_FIELDS, float64)."""
         t = self.torch
         dev = t.device("cuda", self.device)
         real = t.float32 if f32 else t.float64
-        return {
+        out = {
             "cls": t.empty(n, dtype=t.int8, device=dev),
             "theta_inf": t.empty(n, dtype=real, device=dev),
             "phi_inf": t.empty(n, dtype=real, device=dev),
@@ -162,6 +166,9 @@ class Tracer:
             "g": t.empty(n, dtype=real, device=dev),
             "steps": t.empty(n, dtype=t.int32, device=dev),
         }
+        if report:
+            out.update({k: t.empty(n, dtype=t.float64, device=dev) for k in REPORT_FIELDS})
+        return out
 
     @staticmethod
     def _set_out(s: K.KtRaysOut, name: str, ptr, is_f32: bool):
@@ -174,15 +181,22 @@ class Tracer:
             tens = out.get(name)
             if tens is not None:
                 self._set_out(s, name, tens.data_ptr(), tens.dtype == self.torch.float32)
+        for name in REPORT_FIELDS:
+            tens = out.get(name)
+            if tens is not None:
+                if tens.dtype != self.torch.float64 or not tens.is_contiguous():
+                    raise ValueError(f"report output {name}: contiguous float64 needed")
+                setattr(s, name, tens.data_ptr())
         s.on_device = 1
         return s
 
     def trace_views(self, views: Sequence[View], opts: TraceOpts | None = None, out: dict | None = None,
-                    stream=None) -> dict:
-        """Trace every pixel of `views` in one launch; returns device tensors (views concatenated)."""
+                    stream=None, report: bool = False) -> dict:
+        """Trace every pixel of `views` in one launch; returns device tensors (views
+        concatenated).  report: fresh outputs include the single-ray report (FP64 only)."""
         n = sum(v.rays for v in views)
         if out is None:
-            out = self.alloc_outputs(n)
+            out = self.alloc_outputs(n, report=report)
         arr = (K.KtView * len(views))(*[v.to_c() for v in views])
         o = (opts or TraceOpts()).to_c()
         s = self._out_struct(out)
@@ -200,17 +214,23 @@ class Tracer:
         dt = {"cls": t.int8, "theta_inf": real, "phi_inf": real, "r_hit": real, "g": real, "steps": t.int32}
         return {k: t.empty(n, dtype=dt[k], pin_memory=pinned) for k in OUTPUT_FIELDS}
 
-    def trace_views_host(self, views: Sequence[View], opts: TraceOpts | None = None, out: dict | None = None) -> dict:
+    def trace_views_host(self, views: Sequence[View], opts: TraceOpts | None = None, out: dict | None = None,
+                         report: bool = False) -> dict:
         """As trace_views, into host buffers: the library copies the outputs back over PCIe
         and synchronises before returning.  `out`: reusable buffers (host_outputs, or
-        contiguous numpy arrays of the output dtypes); None: fresh numpy arrays."""
+        contiguous numpy arrays of the output dtypes, with REPORT_FIELDS too for the
+        report); None: fresh numpy arrays (with the report when `report`)."""
         n = sum(v.rays for v in views)
         dt = {"cls": np.int8, "theta_inf": np.float64, "phi_inf": np.float64, "r_hit": np.float64,
-              "g": np.float64, "steps": np.int32}
+              "g": np.float64, "steps": np.int32, **{k: np.float64 for k in REPORT_FIELDS}}
         if out is None:
-            out = {k: np.empty(n, dt[k]) for k in OUTPUT_FIELDS}
+            out = {k: np.empty(n, dt[k]) for k in OUTPUT_FIELDS + (REPORT_FIELDS if report else ())}
         s = K.KtRaysOut()
-        for name in OUTPUT_FIELDS:
+        for name in OUTPUT_FIELDS + REPORT_FIELDS:
+            if name not in out:
+                if name in REPORT_FIELDS:
+                    continue
+                raise ValueError(f"host output {name} missing")
             buf = out[name]
             if isinstance(buf, np.ndarray):
                 is_f32 = name in FLOAT_FIELDS and buf.dtype == np.float32
@@ -232,8 +252,9 @@ class Tracer:
         return out
 
     def trace_params(self, view: View, params, opts: TraceOpts | None = None, out: dict | None = None,
-                     stream=None) -> dict:
-        """Explicit rays: (alpha, beta) rows for a Bardeen view, unit looks for a pinhole view."""
+                     stream=None, report: bool = False) -> dict:
+        """Explicit rays: (alpha, beta) rows for a Bardeen view, unit looks for a pinhole view.
+        report: fresh outputs include the single-ray report (FP64 only)."""
         t = self.torch
         per = 2 if view.camera_kind == K.CAMERA_BARDEEN else 3
         p = t.as_tensor(np.asarray(params, dtype=np.float64).reshape(-1, per)) if not isinstance(params, t.Tensor) \
@@ -241,7 +262,7 @@ class Tracer:
         p = p.to(f"cuda:{self.device}").contiguous()
         n = p.shape[0]
         if out is None:
-            out = self.alloc_outputs(n)
+            out = self.alloc_outputs(n, report=report)
         o = (opts or TraceOpts()).to_c()
         s = self._out_struct(out)
         v = view.to_c()
@@ -440,15 +461,19 @@ def render(view: View, opts: TraceOpts | None = None) -> dict:
     return {k: v.reshape(view.height, view.width) for k, v in out.items()}
 
 
-def trace_ray(view: View, alpha: float, beta: float, opts: TraceOpts | None = None) -> dict:
-    """eht::traceRay counterpart for one (alpha, beta) of a Bardeen view."""
-    out = to_numpy(_tracer().trace_params(view, [[alpha, beta]], opts))
+def trace_ray(view: View, alpha: float, beta: float, opts: TraceOpts | None = None, report: bool = False) -> dict:
+    """eht::traceRay counterpart for one (alpha, beta) of a Bardeen view; with `report`
+    also the RayReport fields turns, r_min, delay, b, q (FP64)."""
+    out = to_numpy(_tracer().trace_params(view, [[alpha, beta]], opts, report=report))
     return {k: v[0].item() for k, v in out.items()}
 
 
-def trace_looks(view: View, looks: Iterable[Sequence[float]], opts: TraceOpts | None = None) -> dict:
-    """trajectory::traceRay counterpart: unit look vectors of a pinhole view's ZAMO frame."""
-    return to_numpy(_tracer().trace_params(view, np.asarray(list(looks), dtype=np.float64), opts))
+def trace_looks(view: View, looks: Iterable[Sequence[float]], opts: TraceOpts | None = None,
+                report: bool = False) -> dict:
+    """trajectory::traceRay counterpart: unit look vectors of a pinhole view's ZAMO frame;
+    with `report` also its RayReport fields (src/trajectory.h:97-113): turns, r_min, delay,
+    b, q (FP64)."""
+    return to_numpy(_tracer().trace_params(view, np.asarray(list(looks), dtype=np.float64), opts, report=report))
 
 
 def blackbody_table(n: int = 1024) -> np.ndarray:

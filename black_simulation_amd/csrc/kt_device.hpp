@@ -379,6 +379,7 @@ struct Outputs {
     double* g;
     int32_t* steps;
     float *theta32, *phi32, *r_hit32, *g32;  // optional FP32 copies (rounded once)
+    double *turns, *r_min, *delay, *b, *q;   // optional single-ray report (FP64 kernels only)
 };
 
 struct Result {
@@ -793,6 +794,189 @@ __device__ __forceinline__ bool lane_step_mixed(MixedLane& s, double rtol, doubl
     }
     ++d.steps;
     return lane_finish(d, err, h, last, yn, k7, crossing, dr, dm, res);
+}
+
+// ---- the single-ray report: RayReport's turns, rMin and delay -------------------------
+// (src/trajectory.h:97-113, accumulated along the march of src/trajectory.cpp:259-396)
+//
+// Over each accepted step, up to the step's event when it has one (the disk crossing, or
+// r = r_cap, or r = r_esc, each located on the step's dense output):
+//   turns  the angle swept by the direction of the position in Kerr-Schild Cartesian form
+//          (kerr::toCartesian, src/kerr.cpp:638-641), over 2 pi: great-circle angles
+//          between the directions at 32 equal sub-steps of the dense output, summed;
+//   r_min  the least r: the camera, the step ends, the event radius, and where w_r turns
+//          from in- to outgoing inside a step, the root of R(r) = 0 refined by Newton from
+//          the dense output's estimate;
+//   delay  ingoing Kerr-Schild coordinate time between the event and the camera (the
+//          reference's State::t, src/kerr.cpp:350: dt/dtau = (r^2 + a^2) I - a (a E
+//          sin^2 - L) - v_r with I its inflowFactor), by 5-point Gauss-Legendre on 16 equal
+//          parts of each step's dense output (the integrand grows as 1/(r - r+) toward the
+//          capture radius).
+struct ReportAcc {
+    double swept, r_min, delay;
+    bool done;  // the event has been reached: nothing more accumulates
+};
+
+constexpr int kReportSub = 32, kReportQuad = 16;
+// Gauss-Legendre on [0, 1], 5 nodes (x = (1 + t) / 2, w = W / 2 of the [-1, 1] rule)
+constexpr double kGLx[5] = {0.04691007703066800, 0.23076534494715845, 0.5, 0.76923465505284155,
+                            0.95308992296933200};
+constexpr double kGLw[5] = {0.11846344252809454, 0.23931433524968324, 0.28444444444444444,
+                            0.23931433524968324, 0.11846344252809454};
+
+// unit direction of kerr::toCartesian(a, r, theta, phi_ks)
+__device__ __forceinline__ void ks_direction(double a, double r, double mu, double phi_ks, double d[3]) {
+    const double sn = sqrt(fmax(0.0, 1.0 - mu * mu));
+    const double cp = cos(phi_ks), sp = sin(phi_ks);
+    const double x = (r * cp - a * sp) * sn, y = (r * sp + a * cp) * sn, z = r * mu;
+    const double n = sqrt(x * x + y * y + z * z);
+    d[0] = x / n;
+    d[1] = y / n;
+    d[2] = z / n;
+}
+
+// the angle between unit vectors, accurate for small ones too
+__device__ __forceinline__ double angle_between(const double u[3], const double v[3]) {
+    const double cx = u[1] * v[2] - u[2] * v[1], cy = u[2] * v[0] - u[0] * v[2], cz = u[0] * v[1] - u[1] * v[0];
+    return atan2(sqrt(cx * cx + cy * cy + cz * cz), u[0] * v[0] + u[1] * v[1] + u[2] * v[2]);
+}
+
+// ingoing Kerr-Schild dt/dtau of the photon at (r, mu) with w = dr/ds = -v_r
+__device__ __forceinline__ double dt_dtau(const Photon<double>& p, double r, double mu, double w) {
+    const double delta = delta_of(p, r), P = P_of(p, r);
+    double inflow;
+    if (P * w > 0.0) {
+        inflow = p.K / (P + w);
+    } else {
+        const double dd = fabs(delta) > 1e-12 ? delta : copysign(1e-12, delta);
+        inflow = (P - w) / dd;
+    }
+    return fma(r, r, p.a2) * inflow + p.a * (p.L - p.aE * (1.0 - mu * mu)) + w;
+}
+
+// the fraction of a step where the dense output d first reaches `target` from `from_below`
+__device__ __forceinline__ double dense_root(const Dense<double>& d, double target, bool from_below) {
+    double lo = 0.0, hi = 1.0;
+    for (int it = 0; it < 48; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if ((d.at(mid) < target) == from_below) lo = mid;
+        else hi = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+template <bool KS>
+__device__ __forceinline__ void report_start(const Lane<double>& s, ReportAcc& acc) {
+    acc.swept = 0.0;
+    acc.r_min = s.y[0];
+    acc.delay = 0.0;
+    acc.done = false;
+}
+
+// lane_step in FP64 with the report accumulated over the accepted steps (the integration
+// itself is the same operation for operation, so every other output equals lane_step's)
+template <bool KS>
+__device__ __forceinline__ bool report_step(Lane<double>& s, ReportAcc& acc, double rtol, double atol, int max_steps,
+                                            Result& res) {
+    res.flags = 0;
+    if (KS && s.dark) {
+        res.cls = kCaptured;
+        return true;
+    }
+    if (s.steps >= max_steps) {
+        res.cls = kUnfinished;
+        return true;
+    }
+    double k[7][5], yn[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) k[0][i] = s.k1[i];
+    bool last = false;
+    if (s.tail && s.h <= -s.u) {
+        s.h = -s.u;
+        last = true;
+    }
+    const double h = s.h;
+    const double err = dp5_attempt<double, KS>(s.p, s.tail, s.u, s.y, h, k, yn, rtol, atol);
+    ++s.steps;
+    const bool main = !s.tail;
+    const bool crossing = main && s.disk && s.y[1] * yn[1] < 0.0;
+    Dense<double> d[5];
+    double y0[5];
+    if (main) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            d[i] = dense_coeffs(s.y, yn, k, h, i);
+            y0[i] = s.y[i];
+        }
+    }
+    const bool flip = s.p.L == 0.0 && s.y[1] * s.y[4] > 0.0 && yn[1] * yn[4] < 0.0;
+    const bool ended = lane_finish<double, KS>(s, err, h, last, yn, k[6], crossing, d[0], d[1], res);
+    if (!main || acc.done || (!ended && s.rejected)) return ended;
+    if (ended && (res.flags & kFlagNonfinite)) {
+        acc.done = true;
+        return ended;
+    }
+    // the part of this step before its event
+    double th_end = 1.0;
+    if (ended) {
+        if (res.cls == kDisk) th_end = dense_root(d[1], 0.0, !(y0[1] > 0.0));
+        else if (res.cls == kCaptured && y0[0] >= s.r_cap) th_end = dense_root(d[0], s.r_cap, false);
+        acc.done = true;
+    } else if (s.tail) {
+        th_end = dense_root(d[0], s.r_esc, true);
+        acc.done = true;
+    }
+    const double a = (double)s.p.a;
+    // r_min
+    double rm = fmin(acc.r_min, d[0].at(th_end));
+    if (y0[3] < 0.0 && d[3].at(th_end) > 0.0) {
+        const double tt = dense_root(d[3], 0.0, true);
+        const double r0 = d[0].at(tt);
+        double r = r0;
+        for (int it = 0; it < 4; ++it) {
+            const double P = P_of(s.p, r);
+            const double dR = 4.0 * s.p.E * r * P - 2.0 * (r - 1.0) * s.p.K;
+            if (dR == 0.0) break;
+            r -= pot_R(s.p, r) / dR;
+        }
+        rm = fmin(rm, fabs(r - r0) <= 1e-6 * r0 ? r : r0);
+    }
+    acc.r_min = rm;
+    // turns: sub-step directions (through the pole for L = 0 the dense phi lacks the
+    // half-turn the step adds, so it is added where mu w_mu has changed sign)
+    double dir0[3], dir1[3];
+    ks_direction(a, y0[0], y0[1], KS ? y0[2] : y0[2] + ks_shift(a, y0[0]), dir0);
+    for (int j = 1; j <= kReportSub; ++j) {
+        const double th = th_end * (double)j / (double)kReportSub;
+        const double r = d[0].at(th), mu = d[1].at(th);
+        double ph = d[2].at(th);
+        if (flip && mu * d[4].at(th) < 0.0) ph += kPi;
+        ks_direction(a, r, mu, KS ? ph : ph + ks_shift(a, r), dir1);
+        acc.swept += angle_between(dir0, dir1);
+        dir0[0] = dir1[0];
+        dir0[1] = dir1[1];
+        dir0[2] = dir1[2];
+    }
+    // delay
+    double q = 0.0;
+    for (int m = 0; m < kReportQuad; ++m)
+#pragma unroll
+        for (int g = 0; g < 5; ++g) {
+            const double th = th_end * ((double)m + kGLx[g]) / (double)kReportQuad;
+            q = fma(kGLw[g], dt_dtau(s.p, d[0].at(th), d[1].at(th), d[3].at(th)), q);
+        }
+    acc.delay = fma(h * th_end / (double)kReportQuad, q, acc.delay);
+    return ended;
+}
+
+__device__ __forceinline__ void write_report(const Outputs& o, long long idx, const Photon<double>& p,
+                                             const ReportAcc& acc) {
+    const double invE = fabs(p.E) > 1e-12 ? 1.0 / p.E : 0.0;  // src/trajectory.cpp:221
+    if (o.turns) o.turns[idx] = acc.swept / (2.0 * kPi);
+    if (o.r_min) o.r_min[idx] = acc.r_min;
+    if (o.delay) o.delay[idx] = acc.delay;
+    if (o.b) o.b[idx] = p.L * invE;
+    if (o.q) o.q[idx] = p.Q * invE * invE;
 }
 
 __device__ __forceinline__ void write_result(const Outputs& o, long long idx, const Result& r, int steps, int flags) {

@@ -508,9 +508,17 @@ int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const k
     if (ensure_views(ctx, (int)vd.size()) != KT_OK) return KT_ENOMEM;
 
     // outputs: device pointers as given, or staging for host pointers
+    const bool report = out->turns || out->r_min || out->delay || out->b || out->q;
+    if (report && o.use_surrogate) return fail(ctx, KT_EINVAL, "the ray report is not given with the surrogate");
+    if (report && o.precision != KT_PREC_FP64) return fail(ctx, KT_EINVAL, "the ray report is traced in FP64 (KT_PREC_FP64)");
     ktd::Outputs dout{};
-    size_t off[10] = {};
+    size_t off[15] = {};
     if (out->on_device) {
+        dout.turns = out->turns;
+        dout.r_min = out->r_min;
+        dout.delay = out->delay;
+        dout.b = out->b;
+        dout.q = out->q;
         dout.cls = out->cls;
         dout.theta = out->theta_inf;
         dout.phi = out->phi_inf;
@@ -539,6 +547,11 @@ int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const k
         off[7] = out->phi_inf_f32 ? take(4) : 0;
         off[8] = out->r_hit_f32 ? take(4) : 0;
         off[9] = out->g_f32 ? take(4) : 0;
+        off[10] = out->turns ? take(8) : 0;
+        off[11] = out->r_min ? take(8) : 0;
+        off[12] = out->delay ? take(8) : 0;
+        off[13] = out->b ? take(8) : 0;
+        off[14] = out->q ? take(8) : 0;
         if (ensure_stage(ctx, bytes + 256) != KT_OK) return KT_ENOMEM;
         char* s = static_cast<char*>(ctx->d_stage);
         if (out->cls) dout.cls = reinterpret_cast<int8_t*>(s + off[0]);
@@ -551,6 +564,11 @@ int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const k
         if (out->phi_inf_f32) dout.phi32 = reinterpret_cast<float*>(s + off[7]);
         if (out->r_hit_f32) dout.r_hit32 = reinterpret_cast<float*>(s + off[8]);
         if (out->g_f32) dout.g32 = reinterpret_cast<float*>(s + off[9]);
+        if (out->turns) dout.turns = reinterpret_cast<double*>(s + off[10]);
+        if (out->r_min) dout.r_min = reinterpret_cast<double*>(s + off[11]);
+        if (out->delay) dout.delay = reinterpret_cast<double*>(s + off[12]);
+        if (out->b) dout.b = reinterpret_cast<double*>(s + off[13]);
+        if (out->q) dout.q = reinterpret_cast<double*>(s + off[14]);
     }
 
     KT_HIP(hipMemcpyAsync(ctx->d_views, vd.data(), sizeof(ViewDev) * vd.size(), hipMemcpyHostToDevice, stream));
@@ -597,6 +615,11 @@ int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const k
         if (out->phi_inf_f32) KT_HIP(hipMemcpyAsync(out->phi_inf_f32, dout.phi32, 4 * n, hipMemcpyDeviceToHost, stream));
         if (out->r_hit_f32) KT_HIP(hipMemcpyAsync(out->r_hit_f32, dout.r_hit32, 4 * n, hipMemcpyDeviceToHost, stream));
         if (out->g_f32) KT_HIP(hipMemcpyAsync(out->g_f32, dout.g32, 4 * n, hipMemcpyDeviceToHost, stream));
+        if (out->turns) KT_HIP(hipMemcpyAsync(out->turns, dout.turns, 8 * n, hipMemcpyDeviceToHost, stream));
+        if (out->r_min) KT_HIP(hipMemcpyAsync(out->r_min, dout.r_min, 8 * n, hipMemcpyDeviceToHost, stream));
+        if (out->delay) KT_HIP(hipMemcpyAsync(out->delay, dout.delay, 8 * n, hipMemcpyDeviceToHost, stream));
+        if (out->b) KT_HIP(hipMemcpyAsync(out->b, dout.b, 8 * n, hipMemcpyDeviceToHost, stream));
+        if (out->q) KT_HIP(hipMemcpyAsync(out->q, dout.q, 8 * n, hipMemcpyDeviceToHost, stream));
         KT_HIP(hipStreamSynchronize(stream));
     }
     return KT_OK;
@@ -752,6 +775,8 @@ int kt_trace_views_noalloc(kt_ctx* ctx, const kt_view* views, int n_views, const
             return fail(ctx, KT_EINVAL, "kt_trace_views_noalloc takes at most KT_NOALLOC_MAX_VIEWS (64) views");
         if (!out->on_device)
             return fail(ctx, KT_EINVAL, "kt_trace_views_noalloc writes device outputs only (on_device = 1)");
+        if (out->turns || out->r_min || out->delay || out->b || out->q)
+            return fail(ctx, KT_EINVAL, "kt_trace_views_noalloc does not give the ray report (use kt_trace_views)");
         kt_trace_opts o;
         fill_defaults(opts, o);
         if (const char* why = check_opts(o)) return fail(ctx, KT_EINVAL, why);
@@ -943,6 +968,8 @@ int kt_plan_create(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
     *plan_out = nullptr;
     try {
         if (!out || !out->on_device) return fail(ctx, KT_EINVAL, "a plan needs device output pointers");
+        if (out->turns || out->r_min || out->delay || out->b || out->q)
+            return fail(ctx, KT_EINVAL, "a plan does not give the ray report (use kt_trace_views)");
         kt_trace_opts o;
         fill_defaults(opts, o);
         if (const char* why = check_opts(o)) return fail(ctx, KT_EINVAL, why);

@@ -1,8 +1,9 @@
 // kt_inspect.cpp -- command-line ray inspector and image driver over include/kerr_trace.hpp.
 //
 // The C++20 counterpart of the reference's `--inspect X,Y` (App::inspectPixel,
-// src/app.cpp:1145-1192, printing RayReport fields as src/app.cpp:771-784) and of an
-// eht::render call: one line of JSON per request.
+// src/app.cpp:1145-1192, printing the RayReport fields of src/app.cpp:771-784: the end,
+// r_end, g, turns, r_min, delay, b and q) and of an eht::render call: one line of JSON per
+// request.
 //
 //   kt_inspect <spin> <r_cam> <incl_deg> <half_width> <width> <height> [col row]
 //   kt_inspect eht <spin> <incl_deg> <half_width> <size> [cancel_after_ms]
@@ -88,11 +89,15 @@ int main(int argc, char** argv) {
             const int col = std::atoi(argv[7]), row = std::atoi(argv[8]);
             const double px = 2.0 * v.half_width / v.width, py = 2.0 * v.half_height / v.height;
             const double alpha = -v.half_width + (col + 0.5) * px, beta = v.half_height - (row + 0.5) * py;
-            const kt::Ray r = tracer.traceRay(v, alpha, beta);
+            const kt::RayReport rep = tracer.inspect(v, alpha, beta);
+            const kt::Ray& r = rep.ray;
             std::printf("{\"alpha\": %s, \"beta\": %s, \"end\": %d, \"theta_inf\": %s, \"phi_inf\": %s, "
-                        "\"r_hit\": %s, \"g\": %s, \"steps\": %d}\n",
+                        "\"r_hit\": %s, \"g\": %s, \"steps\": %d, \"r_end\": %s, \"turns\": %s, \"r_min\": %s, "
+                        "\"delay\": %s, \"b\": %s, \"q\": %s}\n",
                         num(alpha).c_str(), num(beta).c_str(), static_cast<int>(r.end), num(r.thetaInf).c_str(),
-                        num(r.phiInf).c_str(), num(r.rHit).c_str(), num(r.g).c_str(), r.steps);
+                        num(r.phiInf).c_str(), num(r.rHit).c_str(), num(r.g).c_str(), r.steps, num(rep.rEnd).c_str(),
+                        num(rep.turns).c_str(), num(rep.rMin).c_str(), num(rep.delay).c_str(), num(rep.b).c_str(),
+                        num(rep.q).c_str());
             return 0;
         }
         const kt::Image img = tracer.render(v);

@@ -208,6 +208,33 @@ __global__ __launch_bounds__(256) void k_trace_oneshot(kti::TraceLaunch L) {
     }
 }
 
+// The single-ray report: the one-shot FP64 trace (Plain<double>, or PlainKS for observer
+// cameras) with report_step, which integrates exactly as lane_step does and accumulates
+// turns, r_min and delay; they are written beside the usual outputs.
+template <bool KS>
+__global__ __launch_bounds__(256) void k_trace_report(kti::TraceLaunch L) {
+    const ViewDev* views = stage_views(L, view_lds);
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if ((long long)blockIdx.x * blockDim.x >= L.total) return;  // the whole block is past the end
+    unsigned long long st = 0;
+    if (q < L.total) {
+        Lane<double> s;
+        Launch la;
+        launch_of(views, L.n_views, q, la);
+        lane_start<double, KS>(s, la);
+        ReportAcc acc;
+        report_start<KS>(s, acc);
+        Result r;
+        while (!report_step<KS>(s, acc, L.rtol, L.atol, L.max_steps, r)) {
+        }
+        write_result(L.out, q, r, s.steps, 0);
+        write_report(L.out, q, s.p, acc);
+        st = (unsigned long long)s.steps;
+    }
+    st = wave_sum(st);
+    if ((threadIdx.x & 63u) == 0u && st) atomicAdd(&L.stats[kti::kStatSteps], st);
+}
+
 }  // namespace
 
 // ---- the scratch of the entry point that allocates nothing (kt_trace_views_noalloc) --------
@@ -275,7 +302,11 @@ void launch_oneshot(const TraceLaunch& L, hipStream_t stream) {
 hipError_t launch_trace(const TraceLaunch& L, hipStream_t stream) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, L.counter, L.stats);
     if (L.total <= 0) return hipGetLastError();
-    if (L.ks) {
+    if (L.out.turns || L.out.r_min || L.out.delay || L.out.b || L.out.q) {
+        const dim3 grid((unsigned)((L.total + L.block - 1) / L.block)), block(L.block);
+        if (L.ks) k_trace_report<true><<<grid, block, view_lds_bytes(L.n_views), stream>>>(L);
+        else k_trace_report<false><<<grid, block, view_lds_bytes(L.n_views), stream>>>(L);
+    } else if (L.ks) {
         if (L.mode == 1) launch_oneshot<PlainKS>(L, stream);
         else launch_persistent<PlainKS>(L, stream);
     } else if (L.mode == 1) {

@@ -152,6 +152,21 @@ typedef struct kt_rays_out {
     float* phi_inf_f32;
     float* r_hit_f32;
     float* g_f32;
+    /* optional single-ray report, trajectory::RayReport (src/trajectory.h:97-113) as
+     * App::inspectPixel prints it (src/app.cpp:776-777); any of them given selects an FP64
+     * one-shot trace (KT_PREC_FP64 only, no surrogate) whose other outputs are the same as
+     * without them.  Each ray is followed to its event: the disk crossing, r = r_cap or
+     * r = r_esc (located on the step's dense output), or its step budget.
+     *   turns  angle swept by the direction of the position in Kerr-Schild Cartesian form
+     *          (kerr::toCartesian), over 2 pi, up to the event
+     *   r_min  least r on the way (camera, event, and radial turning points as roots of R)
+     *   delay  ingoing Kerr-Schild coordinate time from the event to the camera
+     *   b, q   L/E and Q/E^2 (0 when |E| <= 1e-12, as the reference) */
+    double* turns;
+    double* r_min;
+    double* delay;
+    double* b;
+    double* q;
 } kt_rays_out;
 
 typedef struct kt_device_opts {

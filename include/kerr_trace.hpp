@@ -10,6 +10,7 @@
 //   kt::Tracer::ehtCalibrate(view, flow, jy, theta, cancel)
 //                                       for eht::calibrate(...)                      (src/eht.h:152)
 //   kt::Tracer::traceLooks(view, looks) for trajectory::traceRay(..., look, scene)   (src/trajectory.h:118)
+//   kt::Tracer::inspect(view, ...)      its RayReport: turns, rMin, delay, b, q       (src/trajectory.h:97-113)
 //   kt::horizonRadius, iscoRadius, photonOrbitRadius, circularOmega              (src/kerr.h:39-45, 71)
 //   kt::criticalCurve, criticalRadius                                             (src/eht.h:66-68)
 // Values come back by value in std::vectors (as eht::Image, trajectory::RayReport);
@@ -83,6 +84,15 @@ struct Ray {
     int steps = 0;
 };
 
+// trajectory::RayReport (src/trajectory.h:97-113) of one ray, as App::inspectPixel prints it
+// (src/app.cpp:776-777): the ray, where it ended (the disk radius, or r_cap or r_esc, where
+// the trace stops), how many times it went around, its closest approach, the Kerr-Schild
+// coordinate time it spent, and b = L/E, q = Q/E^2.
+struct RayReport {
+    Ray ray;
+    double rEnd = 0.0, turns = 0.0, rMin = 0.0, delay = 0.0, b = 0.0, q = 0.0;
+};
+
 // Per-pixel outputs of one or more views, row-major from the top of each view.
 struct Image {
     int width = 0, height = 0;
@@ -139,6 +149,43 @@ public:
     Image traceLooks(const View& view, const std::vector<std::array<double, 3>>& looks,
                      const Options& opts = Options()) {
         return traceParams(view, looks.front().data(), static_cast<long long>(looks.size()), opts);
+    }
+
+    // The full report of explicit rays (FP64): (alpha, beta) pairs of a Bardeen view, or unit
+    // looks of a pinhole or observer view; the counterpart of trajectory::traceRay's return.
+    std::vector<RayReport> inspect(const View& view, const double* params, long long n,
+                                   Options opts = Options()) {
+        opts.precision = KT_PREC_FP64;
+        opts.use_surrogate = 0;
+        Image img;
+        resize(img, static_cast<size_t>(n));
+        std::vector<double> turns(n), rmin(n), delay(n), b(n), q(n);
+        kt_rays_out out = hostOut(img);
+        out.turns = turns.data();
+        out.r_min = rmin.data();
+        out.delay = delay.data();
+        out.b = b.data();
+        out.q = q.data();
+        check(kt_trace_rays(ctx_, &view, params, n, &opts, &out, nullptr));
+        const double r_cap = view.capture_factor * kt_horizon_radius(view.spin);
+        const double r_esc = view.escape_radius > 1.5 * view.r_cam ? view.escape_radius : 1.5 * view.r_cam;
+        std::vector<RayReport> rep(static_cast<size_t>(n));
+        for (size_t k = 0; k < rep.size(); ++k) {
+            rep[k].ray = img.at(k);
+            const RayEnd e = rep[k].ray.end;
+            rep[k].rEnd = e == RayEnd::Disk ? rep[k].ray.rHit : e == RayEnd::Horizon ? r_cap
+                        : e == RayEnd::Sky ? r_esc : 0.0;
+            rep[k].turns = turns[k];
+            rep[k].rMin = rmin[k];
+            rep[k].delay = delay[k];
+            rep[k].b = b[k];
+            rep[k].q = q[k];
+        }
+        return rep;
+    }
+    RayReport inspect(const View& view, double alpha, double beta, const Options& opts = Options()) {
+        const double ab[2] = {alpha, beta};
+        return inspect(view, ab, 1, opts).front();
     }
 
     // eht::render: the view's size^2 pixels with the light; `cancel` (may be null) is read

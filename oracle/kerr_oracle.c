@@ -884,8 +884,120 @@ static void finish_escape(const ko_photon* p, const ko_opts* o, const double y[5
     out->cls = KO_UNFINISHED;
 }
 
+/* ---- the single-ray report: RayReport's turns, rMin and delay (trajectory.h:97-113,
+   accumulated along the march of trajectory.cpp:259-396) ----
+   Over each accepted step, up to the step's event when it has one (the disk crossing,
+   r = r_cap or r = r_esc, located on the step's dense output):
+     turns  the angle swept by the direction of kerr::toCartesian(a, r, theta, phi_KS)
+            (kerr.cpp:638-641), over 2 pi, as great-circle angles between the directions at
+            32 equal sub-steps of the dense output;
+     r_min  the least r: the camera, the step ends, the event radius, and where w_r turns
+            from in- to outgoing inside a step the root of R(r) = 0, Newton-refined from
+            the dense output's estimate;
+     delay  ingoing Kerr-Schild coordinate time from the event to the camera, the integral
+            of dt/dtau (kerr.cpp:350, with inflowFactor kerr.cpp:322-329 and v_r = -w_r)
+            over s by 5-point Gauss-Legendre on 16 equal parts of each step's dense output
+            (the integrand grows as 1/(r - r+) toward the capture radius). */
+static const double GL_X[5] = {0.04691007703066800, 0.23076534494715845, 0.5, 0.76923465505284155,
+                               0.95308992296933200};
+static const double GL_W[5] = {0.11846344252809454, 0.23931433524968324, 0.28444444444444444,
+                               0.23931433524968324, 0.11846344252809454};
+
+static void ks_direction(double a, double r, double mu, double phi_ks, double d[3]) {
+    const double sn = sqrt(fmax(0.0, 1.0 - mu * mu));
+    const double cp = cos(phi_ks), sp = sin(phi_ks);
+    const double x = (r * cp - a * sp) * sn, y = (r * sp + a * cp) * sn, z = r * mu;
+    const double n = sqrt(x * x + y * y + z * z);
+    d[0] = x / n;
+    d[1] = y / n;
+    d[2] = z / n;
+}
+
+static double angle_between(const double u[3], const double v[3]) {
+    const double cx = u[1] * v[2] - u[2] * v[1], cy = u[2] * v[0] - u[0] * v[2], cz = u[0] * v[1] - u[1] * v[0];
+    return atan2(sqrt(cx * cx + cy * cy + cz * cz), u[0] * v[0] + u[1] * v[1] + u[2] * v[2]);
+}
+
+/* ingoing Kerr-Schild dt/dtau at (r, mu) with w = dr/ds = -v_r */
+static double dt_dtau(const ko_photon* p, double r, double mu, double w) {
+    const double delta = delta_of(p, r), P = P_of(p, r);
+    double inflow;
+    if (P * w > 0.0) {
+        inflow = p->K / (P + w);
+    } else {
+        const double dd = fabs(delta) > 1e-12 ? delta : copysign(1e-12, delta);
+        inflow = (P - w) / dd;
+    }
+    return fma(r, r, p->a2) * inflow + p->a * (p->L - p->aE * (1.0 - mu * mu)) + w;
+}
+
+/* the fraction of a step where the dense output first reaches `target` */
+static double dense_root(const ko_dense* d, double target, int from_below) {
+    double lo = 0.0, hi = 1.0;
+    for (int it = 0; it < 48; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if ((dense_at(d, mid) < target) == (from_below != 0)) lo = mid;
+        else hi = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+typedef struct { double swept, r_min, delay; } report_acc;
+
+static void report_accumulate(const ko_photon* p, const double y0[5], const ko_dense d[5], double h, double th_end,
+                              int flip, report_acc* acc) {
+    const double a = p->a;
+    double rm = fmin(acc->r_min, dense_at(&d[0], th_end));
+    if (y0[3] < 0.0 && dense_at(&d[3], th_end) > 0.0) {
+        const double tt = dense_root(&d[3], 0.0, 1);
+        const double r0 = dense_at(&d[0], tt);
+        double r = r0;
+        for (int it = 0; it < 4; ++it) {
+            const double P = P_of(p, r);
+            const double dR = 4.0 * p->E * r * P - 2.0 * (r - 1.0) * p->K;
+            if (dR == 0.0) break;
+            r -= ko_R(p, r) / dR;
+        }
+        rm = fmin(rm, fabs(r - r0) <= 1e-6 * r0 ? r : r0);
+    }
+    acc->r_min = rm;
+    double dir0[3], dir1[3];
+    ks_direction(a, y0[0], y0[1], p->ks ? y0[2] : y0[2] + ko_ks_shift(a, y0[0]), dir0);
+    for (int j = 1; j <= 32; ++j) {
+        const double th = th_end * (double)j / 32.0;
+        const double r = dense_at(&d[0], th), mu = dense_at(&d[1], th);
+        double ph = dense_at(&d[2], th);
+        if (flip && mu * dense_at(&d[4], th) < 0.0) ph += KO_PI;
+        ks_direction(a, r, mu, p->ks ? ph : ph + ko_ks_shift(a, r), dir1);
+        acc->swept += angle_between(dir0, dir1);
+        dir0[0] = dir1[0];
+        dir0[1] = dir1[1];
+        dir0[2] = dir1[2];
+    }
+    double q = 0.0;
+    for (int m = 0; m < 16; ++m)
+        for (int g = 0; g < 5; ++g) {
+            const double th = th_end * ((double)m + GL_X[g]) / 16.0;
+            q = fma(GL_W[g], dt_dtau(p, dense_at(&d[0], th), dense_at(&d[1], th), dense_at(&d[3], th)), q);
+        }
+    acc->delay = fma(h * th_end / 16.0, q, acc->delay);
+}
+
+static void trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, const double y0[5], ko_result* out,
+                         ko_report* rep);
+
 void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, const double y0[5],
                      ko_result* out) {
+    trace_photon(v, o, p, y0, out, NULL);
+}
+
+void ko_trace_photon_report(const ko_view* v, const ko_opts* o, const ko_photon* p, const double y0[5],
+                            ko_result* out, ko_report* rep) {
+    trace_photon(v, o, p, y0, out, rep);
+}
+
+static void trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, const double y0[5], ko_result* out,
+                         ko_report* rep) {
     const double a = v->spin;
     const double r_cap = v->capture_factor * ko_horizon_radius(a);
     const double r_esc = amax(v->escape_radius, 1.5 * v->r_cam);
@@ -900,6 +1012,20 @@ void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, con
     out->theta_inf = out->phi_inf = out->r_hit = out->g = NAN;
     int steps = 0, rejected = 0;
     double h = 0.01 / v->r_cam;
+    report_acc acc = {0.0, y0[0], 0.0};
+    if (rep) {
+        const double invE = fabs(p->E) > 1e-12 ? 1.0 / p->E : 0.0; /* trajectory.cpp:221 */
+        rep->b = p->L * invE;
+        rep->q = p->Q * invE * invE;
+    }
+#define KO_REPORT_OUT()                              \
+    do {                                             \
+        if (rep) {                                   \
+            rep->turns = acc.swept / (2.0 * KO_PI);  \
+            rep->r_min = acc.r_min;                  \
+            rep->delay = acc.delay;                  \
+        }                                            \
+    } while (0)
     /* inside the horizon every ray, followed back, rises to it; one arriving with
        P(r+) = 2 r+ E - a L < 0 came from the far side: dark (trajectory.cpp:234-241) */
     const double rh = ko_horizon_radius(a);
@@ -907,6 +1033,7 @@ void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, con
         out->cls = KO_CAPTURED;
         for (int i = 0; i < 5; ++i) out->y_end[i] = y[i];
         out->steps = 0;
+        KO_REPORT_OUT();
         return;
     }
     ko_rhs(p, y, k[0]);
@@ -920,8 +1047,13 @@ void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, con
         }
         const double fac = step_factor(err, rejected);
         rejected = 0;
+        /* the report's dense output, from the step as taken (before the half-turn and the projection) */
+        ko_dense dd[5];
+        if (rep)
+            for (int i = 0; i < 5; ++i) dd[i] = dense_coeffs(y, yn, k, h, i);
         /* through the pole (L = 0 only): same point, phi + pi */
-        if (p->L == 0.0 && y[1] * y[4] > 0.0 && yn[1] * yn[4] < 0.0) yn[2] += KO_PI;
+        const int flip = p->L == 0.0 && y[1] * y[4] > 0.0 && yn[1] * yn[4] < 0.0;
+        if (flip) yn[2] += KO_PI;
         project(p, yn);
 
         /* a non-finite state counts as captured (eht.cpp:285) */
@@ -930,6 +1062,7 @@ void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, con
             out->flags = KO_FLAG_NONFINITE;
             for (int i = 0; i < 5; ++i) out->y_end[i] = yn[i];
             out->steps = steps;
+            KO_REPORT_OUT();
             return;
         }
         /* events in the reference's order: disk, horizon, escape (trajectory.cpp:307-392) */
@@ -956,6 +1089,8 @@ void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, con
                     out->g = sqrt(norm) / (p->E - p->L * omega);
                     for (int i = 0; i < 5; ++i) out->y_end[i] = yn[i];
                     out->steps = steps;
+                    if (rep) report_accumulate(p, y, dd, h, dense_root(&dd[1], 0.0, !(y[1] > 0.0)), flip, &acc);
+                    KO_REPORT_OUT();
                     return;
                 }
             }
@@ -965,14 +1100,19 @@ void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, con
             out->cls = KO_CAPTURED;
             for (int i = 0; i < 5; ++i) out->y_end[i] = yn[i];
             out->steps = steps;
+            if (rep) report_accumulate(p, y, dd, h, y[0] >= r_cap ? dense_root(&dd[0], r_cap, 0) : 1.0, flip, &acc);
+            KO_REPORT_OUT();
             return;
         }
         if (yn[0] > r_esc && yn[3] > 0.0) {
+            if (rep) report_accumulate(p, y, dd, h, dense_root(&dd[0], r_esc, 1), flip, &acc);
+            KO_REPORT_OUT();
             if (p->ks) yn[2] -= ko_ks_shift(a, yn[0]); /* the tail runs in Boyer-Lindquist phi */
             finish_escape(p, o, yn, &steps, out);
             out->steps = steps;
             return;
         }
+        if (rep) report_accumulate(p, y, dd, h, 1.0, flip, &acc);
         for (int i = 0; i < 5; ++i) y[i] = yn[i];
         for (int i = 0; i < 5; ++i) k[0][i] = k[6][i];
         k[0][0] = y[3]; /* FSAL after the projection: only dr/ds = w_r, dmu/ds = w_mu move */
@@ -981,6 +1121,8 @@ void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, con
     }
     for (int i = 0; i < 5; ++i) out->y_end[i] = y[i];
     out->steps = steps;
+    KO_REPORT_OUT();
+#undef KO_REPORT_OUT
 }
 
 void ko_trace_pixel(const ko_view* v, const ko_opts* o, int col, int row, ko_result* out) {

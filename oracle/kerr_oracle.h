@@ -144,6 +144,14 @@ void ko_trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, con
                      ko_result* out);
 void ko_trace_pixel(const ko_view* v, const ko_opts* o, int col, int row, ko_result* out);
 
+/* the single-ray report (trajectory::RayReport, trajectory.h:97-113): the angle the
+   position's direction sweeps over 2 pi, the least r and the ingoing Kerr-Schild time from
+   the event to the camera, each up to the event (disk crossing, r_cap, r_esc); b = L/E and
+   q = Q/E^2 (0 for |E| <= 1e-12).  ko_trace_photon with the report beside it. */
+typedef struct { double turns, r_min, delay, b, q; } ko_report;
+void ko_trace_photon_report(const ko_view* v, const ko_opts* o, const ko_photon* p, const double y0[5],
+                            ko_result* out, ko_report* rep);
+
 /* whole image, `threads` workers claiming rows from an atomic counter (eht.cpp:321-347).
    Outputs are row-major from the top row; any pointer may be NULL. `stride` > 1 traces
    only pixels with (row * width + col) % stride == 0 (a sparse timing sample) and writes

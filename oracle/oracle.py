@@ -71,6 +71,10 @@ class Result(C.Structure):
     ]
 
 
+class Report(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("turns", "r_min", "delay", "b", "q")]
+
+
 class Consts(C.Structure):
     _fields_ = [("E", C.c_double), ("L", C.c_double), ("Q", C.c_double), ("mu", C.c_double)]
 
@@ -147,6 +151,7 @@ def lib():
             ("ko_particle_from_zamo", None, [d, d, d, P(d), P(d)]),
             ("ko_ks_shift", d, [d, d]),
             ("ko_trace_photon", None, [P(View), P(Opts), P(Photon), P(d), P(Result)]),
+            ("ko_trace_photon_report", None, [P(View), P(Opts), P(Photon), P(d), P(Result), P(Report)]),
             ("ko_trace_pixel", None, [P(View), P(Opts), i, i, P(Result)]),
             ("ko_render", C.c_long, [P(View), P(Opts), i, P(C.c_int), i, P(C.c_int8), P(d), P(d), P(d), P(d),
                                      P(C.c_int32)]),
@@ -241,8 +246,12 @@ def critical_radius(alpha: np.ndarray, beta: np.ndarray, psi: float) -> float:
     return lib().ko_critical_radius(_dp(al), _dp(be), len(al), psi)
 
 
-def trace_params(view: View, opts: Opts, params) -> dict:
-    """Explicit rays: (alpha, beta) rows for a Bardeen view, unit looks for a pinhole view."""
+REPORT_FIELDS = ("turns", "r_min", "delay", "b", "q")
+
+
+def trace_params(view: View, opts: Opts, params, report: bool = False) -> dict:
+    """Explicit rays: (alpha, beta) rows for a Bardeen view, unit looks for a pinhole view;
+    with `report`, also the single-ray report (REPORT_FIELDS, ko_trace_photon_report)."""
     params = np.asarray(params, dtype=np.float64)
     per = 2 if view.camera_kind == BARDEEN else 3
     params = params.reshape(-1, per)
@@ -250,6 +259,8 @@ def trace_params(view: View, opts: Opts, params) -> dict:
     out = {"cls": np.zeros(n, np.int8), "theta": np.full(n, np.nan), "phi": np.full(n, np.nan),
            "r_hit": np.full(n, np.nan), "g": np.full(n, np.nan), "steps": np.zeros(n, np.int32),
            "y_end": np.zeros((n, 5)), "consts": np.zeros((n, 3))}
+    if report:
+        out.update({k: np.zeros(n) for k in REPORT_FIELDS})
     L = lib()
     for i, row in enumerate(params):
         p = Photon()
@@ -260,7 +271,13 @@ def trace_params(view: View, opts: Opts, params) -> dict:
             lk = (C.c_double * 3)(*row)
             L.ko_launch_look(C.byref(view), lk, C.byref(p), y0)
         res = Result()
-        L.ko_trace_photon(C.byref(view), C.byref(opts), C.byref(p), y0, C.byref(res))
+        if report:
+            rep = Report()
+            L.ko_trace_photon_report(C.byref(view), C.byref(opts), C.byref(p), y0, C.byref(res), C.byref(rep))
+            for k in REPORT_FIELDS:
+                out[k][i] = getattr(rep, k)
+        else:
+            L.ko_trace_photon(C.byref(view), C.byref(opts), C.byref(p), y0, C.byref(res))
         out["cls"][i] = res.cls
         out["steps"][i] = res.steps
         out["y_end"][i] = list(res.y_end)

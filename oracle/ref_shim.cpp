@@ -134,6 +134,28 @@ void ref_trajectory_trace(double a, double r, double theta, double phi, const do
     out[10] = rep.q;
 }
 
+// trajectory::traceRay from a ZAMO camera with the whole RayReport the inspector prints
+// (src/app.cpp:776-777); out = {end, rEnd, g, turns, rMin, delay, energy, b, q}
+void ref_trajectory_report(double a, double r, double theta, double phi, const double* look, int disk,
+                           double disk_inner, double disk_outer, double escape, double* out) {
+    trajectory::RayScene scene;
+    scene.disk = disk != 0;
+    scene.diskInner = disk_inner;
+    scene.diskOuter = disk_outer;
+    scene.escapeRadius = escape;
+    kerr::Frame f = kerr::observerFrame(a, r, theta, kerr::zamoFourVelocity(a, r, theta));
+    trajectory::RayReport rep = trajectory::traceRay(a, r, theta, phi, f, look, scene);
+    out[0] = static_cast<double>(static_cast<int>(rep.end));
+    out[1] = rep.rEnd;
+    out[2] = rep.g;
+    out[3] = rep.turns;
+    out[4] = rep.rMin;
+    out[5] = rep.delay;
+    out[6] = rep.energy;
+    out[7] = rep.b;
+    out[8] = rep.q;
+}
+
 double ref_bending_angle(double R, double alpha) { return central::bendingAngle(R, alpha); }
 
 }  // extern "C"

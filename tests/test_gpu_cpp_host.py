@@ -12,6 +12,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
+OL = O.lib()
 EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "black_simulation_amd", "kt_inspect")
 
 
@@ -22,11 +23,19 @@ def run(*args):
 
 
 def test_inspect_single_pixels_match_oracle():
+    """One pixel's full report, as App::inspectPixel prints it (src/app.cpp:776-777): the
+    end, r_end, g, turns, r_min, delay, b and q."""
     v = O.make_view(width=64, height=64)
     for col, row in [(32, 32), (0, 0), (50, 32), (20, 33), (32, 20)]:
         got = run(0.9, 30.0, 80.0, 12.0, 64, 64, col, row)
         want = O.trace_pixel(v, O.make_opts(), col, row)
         assert got["end"] == want.cls and got["steps"] == want.steps
+        rep = O.trace_params(v, O.make_opts(), [[got["alpha"], got["beta"]]], report=True)
+        for k in ("turns", "r_min", "delay"):
+            assert got[k] == pytest.approx(rep[k][0], rel=1e-12, abs=1e-13), k
+        assert got["b"] == rep["b"][0] and got["q"] == rep["q"][0]
+        r_end = {O.ESCAPED: 1000.0, O.CAPTURED: 1.01 * OL.ko_horizon_radius(0.9), O.DISK: want.r_hit}[want.cls]
+        assert got["r_end"] == pytest.approx(r_end, rel=1e-15)
         if want.cls == O.ESCAPED:
             assert abs(got["theta_inf"] - want.theta_inf) < 1e-13
         else:
