@@ -338,7 +338,7 @@ def main():
     ap.add_argument("--weights", default=None,
                     help="surrogate parameters (PATH.f32 and PATH.json); default the fast network, "
                          "black_simulation_amd/data/surrogate_fast (the wide one: .../data/surrogate)")
-    ap.add_argument("--streams", type=int, default=2)
+    ap.add_argument("--streams", type=int, default=3)
     ap.add_argument("--lean", action="store_true",
                     help="for profilers: no autotuning, no plain-FP64 timing, no accuracy pass")
     ap.add_argument("--host-outputs", action="store_true",
@@ -351,6 +351,10 @@ def main():
     ap.add_argument("--fallback", choices=["mixed", "fp64"], default="fp64",
                     help="the integrator of --path surrogate: FP64 (default) at rtol 1e-6, or the mixed FP32/FP64 "
                          "one; --rtol sets its tolerance")
+    ap.add_argument("--gate-ring", type=float, default=0.0,
+                    help="surrogate gate: least |rho / rho_c - 1| of an answered ray (0: the library's default)")
+    ap.add_argument("--gate-prob", type=float, default=0.0,
+                    help="surrogate gate: least class probability of an answered ray (0: the library's default)")
     ap.add_argument("--rtol", type=float, default=0.0,
                     help="DP5 tolerance of the chosen path (FP64 rtol for --path plain, the FP32 bulk's for mixed "
                          "and surrogate; atol = rtol / 10); the reference render stays FP64 at rtol 1e-10")
@@ -428,6 +432,10 @@ def main():
         if args.path == "surrogate":
             load_into(tr, args.weights or FAST_WEIGHTS)
             opts.use_surrogate = 1
+            if args.gate_ring:
+                opts.gate_ring = args.gate_ring
+            if args.gate_prob:
+                opts.gate_prob = args.gate_prob
     tr_net = tr.surrogate_net() if args.path == "surrogate" else None
 
     # the chosen path, timed
@@ -614,6 +622,7 @@ def main():
                    "fallback": (f"{'FP64' if opts.precision == bsa.PREC_FP64 else 'mixed FP32/FP64'} DP5 rtol "
                                 f"{opts.rtol if opts.precision == bsa.PREC_FP64 else opts.rtol32:g}")
                    if args.path == "surrogate" else None,
+                   "gate": {"prob": opts.gate_prob, "ring": opts.gate_ring} if args.path == "surrogate" else None,
                    "streams": args.streams, "lean": args.lean,
                    "frame_ms_p50": round(percentile(lat, 0.5), 3), "frame_ms_p99": round(percentile(lat, 0.99), 3),
                    "plain_fp64": {"value": round(value_p, 3), "frame_ms_p50": round(percentile(lat_p, 0.5), 3),

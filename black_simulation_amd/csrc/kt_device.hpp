@@ -52,6 +52,8 @@ struct ViewDev {
     int kind, width, height, disk, prograde, n_params;
     int sur;  // 1: the view lies in the surrogate's domain (set only for surrogate calls)
     double a, r_cam, phi_cam;
+    double phi0;  // phi at the camera as the integrator carries it: phi_cam - psi(r_cam) (outgoing
+                  // Kerr-Schild), or phi_cam itself for observer cameras (ingoing)
     double half_width, half_height, px, py;  // Bardeen plane, pixel pitch
     double sinI, cosI, m0;                   // sin/cos of inclination, cos of the clamped theta
     double r_cap, r_esc, r_in, r_out, h0, r_fp64;  // r_fp64 = 1.2 r+ (mixed precision)
@@ -145,25 +147,44 @@ __device__ __forceinline__ R pot_M(const Photon<R>& p, R m) {
 // dphi/ds = aE - [a P (1 - mu^2) + L Delta] / [Delta (1 - mu^2)]: one division; the floor
 // on 1 - mu^2 keeps L = 0 rays finite at the pole (the ratio tends to aP/Delta).
 //
-// KS: phi is the ingoing Kerr-Schild phi (dphi_KS = dphi_BL + a dr / Delta), regular on the
-// future horizon, for cameras that may sit inside it: P / Delta becomes (P - w_r) / Delta,
-// which on the regular branch (P w_r > 0, the ray rising to the horizon) is K / (P + w_r)
-// by w_r^2 = P^2 - Delta K; on the other branch Delta is floored at 1e-12 as
-// src/kerr.cpp:323-329 (inflowFactor) does.  Still one division.
-template <class R, bool KS = false>
+// Which phi a lane carries.  kPhiIn: the ingoing Kerr-Schild phi (dphi_in = dphi_BL +
+// a dr / Delta), regular on the future horizon, for cameras that may sit inside it:
+// P / Delta becomes (P - w_r) / Delta, which on the regular branch (P w_r > 0, the ray
+// rising to the horizon) is K / (P + w_r) by w_r^2 = P^2 - Delta K; on the other branch
+// Delta is floored at 1e-12 as src/kerr.cpp:323-329 (inflowFactor) does.  kPhiOut: the
+// outgoing one (dphi_out = dphi_BL - a dr / Delta), regular on the past horizon, which a
+// ray from a camera outside, followed back into the hole, approaches: P / Delta becomes
+// (P + w_r) / Delta = K / (P - w_r) on its regular branch (P w_r < 0).  Its phi stays
+// finite down to the capture radius, where the Boyer-Lindquist phi winds without bound
+// and would set the step size of rays whose phi is not an output; phi_out - phi_BL =
+// -psi(r) vanishes at infinity, so the sky angle needs no conversion.  kPhiBL: the
+// Boyer-Lindquist phi (the EHT transfer, whose outputs do not use phi).  One division in
+// every form.
+enum : int { kPhiBL = 0, kPhiIn = 1, kPhiOut = 2 };
+
+template <class R, int PHI = kPhiBL>
 __device__ __forceinline__ void rhs_main(const Photon<R>& p, const R y[5], R dy[5]) {
     const R r = y[0], m = y[1];
     const R delta = delta_of(p, r);
     const R P = P_of(p, r);
     const R s2 = fmax(fma(-m, m, R(1)), tiny_pos<R>());
     R nd = P, dd = delta;
-    if (KS) {
+    if (PHI == kPhiIn) {
         const R w = y[3];
         if (P * w > R(0)) {
             nd = p.K;
             dd = P + w;
         } else {
             nd = P - w;
+            dd = fabs(delta) > R(1e-12) ? delta : copysign(R(1e-12), delta);
+        }
+    } else if (PHI == kPhiOut) {
+        const R w = y[3];
+        if (P * w < R(0)) {
+            nd = p.K;
+            dd = P - w;
+        } else {
+            nd = P + w;
             dd = fabs(delta) > R(1e-12) ? delta : copysign(R(1e-12), delta);
         }
     }
@@ -176,8 +197,9 @@ __device__ __forceinline__ void rhs_main(const Photon<R>& p, const R y[5], R dy[
 }
 
 // tail in u = 1/r on slots (1, 2, 4) = (mu, phi, w_mu); ds/du = -1/sqrt(U),
-// U = u^4 R(1/u) = Pu^2 - u^2 Du K
-template <class R>
+// U = u^4 R(1/u) = Pu^2 - u^2 Du K.  The tail carries the Boyer-Lindquist phi, or for
+// kPhiOut the outgoing Kerr-Schild one: dphi_out/du = dphi_BL/du + a / Du.
+template <class R, int PHI = kPhiBL>
 __device__ __forceinline__ void rhs_tail(const Photon<R>& p, R u, const R y[5], R dy[5]) {
     const R u2 = u * u;
     const R Pu = fma(p.E, fma(p.a2, u2, R(1)), -(p.aL * u2));
@@ -191,13 +213,14 @@ __device__ __forceinline__ void rhs_tail(const Photon<R>& p, R u, const R y[5], 
     dy[3] = R(0);
     dy[1] = -(y[4] * isq);
     dy[2] = (qdiv(num, Du * s2) - p.aE) * isq;
+    if (PHI == kPhiOut) dy[2] = dy[2] + qdiv(p.a, Du);
     dy[4] = (m * fma(p.twoA2E2, m * m, p.C1)) * isq;
 }
 
-template <class R, bool KS = false>
+template <class R, int PHI = kPhiBL>
 __device__ __forceinline__ void eval(const Photon<R>& p, bool tail, R u, const R y[5], R dy[5]) {
-    if (tail) rhs_tail(p, u, y, dy);
-    else rhs_main<R, KS>(p, y, dy);
+    if (tail) rhs_tail<R, PHI>(p, u, y, dy);
+    else rhs_main<R, PHI>(p, y, dy);
 }
 
 // x^(-1/5): exponent split and three Newton steps on y^-5 = x (+, *, frexp, ldexp only)
@@ -280,33 +303,33 @@ struct Dense {
 };
 
 // One DP5 attempt; k[0] = f(y) on entry; fills k[1..6] and yn; returns the scaled error.
-template <class R, bool KS = false>
+template <class R, int PHI = kPhiBL>
 __device__ __forceinline__ R dp5_attempt(const Photon<R>& p, bool tail, R u, const R y[5], R h, R k[7][5],
                                          R yn[5], R rtol, R atol) {
     R t[5];
 #pragma unroll
     for (int i = 0; i < 5; ++i) t[i] = fma(h, R(A21) * k[0][i], y[i]);
-    eval<R, KS>(p, tail, u + R(C2) * h, t, k[1]);
+    eval<R, PHI>(p, tail, u + R(C2) * h, t, k[1]);
 #pragma unroll
     for (int i = 0; i < 5; ++i) t[i] = fma(h, sum2(R(A31), k[0][i], R(A32), k[1][i]), y[i]);
-    eval<R, KS>(p, tail, u + R(C3) * h, t, k[2]);
+    eval<R, PHI>(p, tail, u + R(C3) * h, t, k[2]);
 #pragma unroll
     for (int i = 0; i < 5; ++i) t[i] = fma(h, sum3(R(A41), k[0][i], R(A42), k[1][i], R(A43), k[2][i]), y[i]);
-    eval<R, KS>(p, tail, u + R(C4) * h, t, k[3]);
+    eval<R, PHI>(p, tail, u + R(C4) * h, t, k[3]);
 #pragma unroll
     for (int i = 0; i < 5; ++i)
         t[i] = fma(h, sum4(R(A51), k[0][i], R(A52), k[1][i], R(A53), k[2][i], R(A54), k[3][i]), y[i]);
-    eval<R, KS>(p, tail, u + R(C5) * h, t, k[4]);
+    eval<R, PHI>(p, tail, u + R(C5) * h, t, k[4]);
 #pragma unroll
     for (int i = 0; i < 5; ++i)
         t[i] = fma(h, sum5(R(A61), k[0][i], R(A62), k[1][i], R(A63), k[2][i], R(A64), k[3][i], R(A65), k[4][i]),
                    y[i]);
-    eval<R, KS>(p, tail, u + h, t, k[5]);
+    eval<R, PHI>(p, tail, u + h, t, k[5]);
 #pragma unroll
     for (int i = 0; i < 5; ++i)
         yn[i] = fma(h, sum5(R(A71), k[0][i], R(A73), k[2][i], R(A74), k[3][i], R(A75), k[4][i], R(A76), k[5][i]),
                     y[i]);
-    eval<R, KS>(p, tail, u + h, yn, k[6]);
+    eval<R, PHI>(p, tail, u + h, yn, k[6]);
     ErrPick<R> ep;
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
@@ -458,7 +481,7 @@ __device__ __forceinline__ void launch_photon(const ViewDev& v, long long local,
         L.r_in = isco_dev(ar, v.prograde);
         L.h0 = 0.01 / rc;
         L.r_fp64 = 1.2 * rp;
-        bardeen_photon(ar, rc, 0.0, sin(inc), cos(inc), cos(th), q[3], q[4], p, y0);
+        bardeen_photon(ar, rc, -ks_shift(ar, rc), sin(inc), cos(inc), cos(th), q[3], q[4], p, y0);  // phi_out
         return;
     }
     if (v.kind == 0) {  // Bardeen image plane, src/eht.cpp:262-271, 332-334
@@ -471,7 +494,7 @@ __device__ __forceinline__ void launch_photon(const ViewDev& v, long long local,
             alpha = -v.half_width + ((double)col + 0.5) * v.px;
             beta = v.half_height - ((double)row + 0.5) * v.py;
         }
-        bardeen_photon(a, v.r_cam, v.phi_cam, v.sinI, v.cosI, v.m0, alpha, beta, p, y0);
+        bardeen_photon(a, v.r_cam, v.phi0, v.sinI, v.cosI, v.m0, alpha, beta, p, y0);
         return;
     }
     // pinhole camera (kind 1: the static ZAMO; kind 3: any observer, frame from the host):
@@ -510,7 +533,7 @@ __device__ __forceinline__ void launch_photon(const ViewDev& v, long long local,
     photon_init(p, a, E, Lz, Q);
     y0[0] = v.r_cam;
     y0[1] = v.cs;
-    y0[2] = v.phi_cam;
+    y0[2] = v.phi0;
     y0[3] = -(v.sigma * qr);
     y0[4] = v.sn * qh;
     // inside the horizon every ray, followed back, rises to it; one that arrives with
@@ -519,7 +542,7 @@ __device__ __forceinline__ void launch_photon(const ViewDev& v, long long local,
     if (v.kind == 3 && v.r_cam < v.r_plus && 2.0 * v.r_plus * E - a * Lz < 0.0) L.dark = 1;
 }
 
-template <class R, bool KS = false>
+template <class R, int PHI>
 __device__ __forceinline__ void lane_start(Lane<R>& s, const Launch& la) {
     const Photon<double>& p = la.p;
     const double* y0 = la.y0;
@@ -548,14 +571,14 @@ __device__ __forceinline__ void lane_start(Lane<R>& s, const Launch& la) {
     s.tail = false;
     s.disk = la.disk != 0;
     s.prograde = la.prograde != 0;
-    s.dark = KS && la.dark != 0;
-    rhs_main<R, KS>(s.p, s.y, s.k1);
+    s.dark = PHI == kPhiIn && la.dark != 0;
+    rhs_main<R, PHI>(s.p, s.y, s.k1);
 }
 
 // After an attempt: the controller, the pole, the projection and the events.  k7 is the
 // attempt's f(yn); dr, dm its dense output of r and mu when the step crossed the plane.
 // Returns true when the ray has ended (res filled).
-template <class R, bool KS = false>
+template <class R, int PHI>
 __device__ __forceinline__ bool lane_finish(Lane<R>& s, R err, R h, bool last, R yn[5], const R k7[5], bool crossing,
                                             const Dense<R>& dr, const Dense<R>& dm, Result& res) {
     if (!(err <= R(1))) {
@@ -621,18 +644,18 @@ __device__ __forceinline__ bool lane_finish(Lane<R>& s, R err, R h, bool last, R
     }
     // KS cameras may start inside the capture radius: there a ray is captured only when
     // it heads inward (src/trajectory.cpp:377)
-    if (yn[0] < s.r_cap && (!KS || yn[3] < R(0))) {
+    if (yn[0] < s.r_cap && (PHI != kPhiIn || yn[3] < R(0))) {
         res.cls = kCaptured;
         return true;
     }
     if (yn[0] > s.r_esc && yn[3] > R(0)) {
-        if (KS) yn[2] -= R(ks_shift((double)s.p.a, (double)yn[0]));  // the tail runs in BL phi
+        if (PHI == kPhiIn) yn[2] -= R(ks_shift((double)s.p.a, (double)yn[0]));  // its tail runs in BL phi
 #pragma unroll
         for (int i = 0; i < 5; ++i) s.y[i] = yn[i];
         s.tail = true;
         s.u = R(1) / yn[0];
         s.h = -s.u;
-        rhs_tail(s.p, s.u, s.y, s.k1);
+        rhs_tail<R, PHI>(s.p, s.u, s.y, s.k1);
         return false;
     }
 #pragma unroll
@@ -646,10 +669,10 @@ __device__ __forceinline__ bool lane_finish(Lane<R>& s, R err, R h, bool last, R
 }
 
 // One DP5 attempt plus events, all in precision R.
-template <class R, bool KS = false>
+template <class R, int PHI>
 __device__ __forceinline__ bool lane_step(Lane<R>& s, R rtol, R atol, int max_steps, Result& res) {
     res.flags = 0;
-    if (KS && s.dark) {
+    if (PHI == kPhiIn && s.dark) {
         res.cls = kCaptured;
         return true;
     }
@@ -666,7 +689,7 @@ __device__ __forceinline__ bool lane_step(Lane<R>& s, R rtol, R atol, int max_st
         last = true;
     }
     const R h = s.h;
-    const R err = dp5_attempt<R, KS>(s.p, s.tail, s.u, s.y, h, k, yn, rtol, atol);
+    const R err = dp5_attempt<R, PHI>(s.p, s.tail, s.u, s.y, h, k, yn, rtol, atol);
     ++s.steps;
     // dense output for the disk crossing, taken now so the stage vectors die here
     const bool crossing = !s.tail && s.disk && s.y[1] * yn[1] < R(0);
@@ -675,7 +698,7 @@ __device__ __forceinline__ bool lane_step(Lane<R>& s, R rtol, R atol, int max_st
         dr = dense_coeffs(s.y, yn, k, h, 0);
         dm = dense_coeffs(s.y, yn, k, h, 1);
     }
-    return lane_finish<R, KS>(s, err, h, last, yn, k[6], crossing, dr, dm, res);
+    return lane_finish<R, PHI>(s, err, h, last, yn, k[6], crossing, dr, dm, res);
 }
 
 // ---- mixed precision --------------------------------------------------------------
@@ -761,7 +784,7 @@ __device__ __forceinline__ bool lane_step_mixed(MixedLane& s, double rtol, doubl
             k[0][i] = (float)d.k1[i];
         }
         const float hf = (float)h;
-        const float errf = dp5_attempt(s.pf, false, 0.0f, y, hf, k, ynf, (float)rtol, (float)atol);
+        const float errf = dp5_attempt<float, kPhiOut>(s.pf, false, 0.0f, y, hf, k, ynf, (float)rtol, (float)atol);
         if (!(errf <= 1.0f) || h_drift(s.pf, ynf) > drift) {
             use64 = true;  // the re-step
         } else {
@@ -782,7 +805,7 @@ __device__ __forceinline__ bool lane_step_mixed(MixedLane& s, double rtol, doubl
         double k[7][5];
 #pragma unroll
         for (int i = 0; i < 5; ++i) k[0][i] = d.k1[i];
-        err = dp5_attempt(d.p, d.tail, d.u, d.y, h, k, yn, rtol, atol);
+        err = dp5_attempt<double, kPhiOut>(d.p, d.tail, d.u, d.y, h, k, yn, rtol, atol);
 #pragma unroll
         for (int i = 0; i < 5; ++i) k7[i] = k[6][i];
         crossing = !d.tail && d.disk && d.y[1] * yn[1] < 0.0;
@@ -793,7 +816,7 @@ __device__ __forceinline__ bool lane_step_mixed(MixedLane& s, double rtol, doubl
         ++s.fp64_steps;
     }
     ++d.steps;
-    return lane_finish(d, err, h, last, yn, k7, crossing, dr, dm, res);
+    return lane_finish<double, kPhiOut>(d, err, h, last, yn, k7, crossing, dr, dm, res);
 }
 
 // ---- the single-ray report: RayReport's turns, rMin and delay -------------------------
@@ -803,21 +826,36 @@ __device__ __forceinline__ bool lane_step_mixed(MixedLane& s, double rtol, doubl
 // r = r_cap, or r = r_esc, each located on the step's dense output):
 //   turns  the angle swept by the direction of the position in Kerr-Schild Cartesian form
 //          (kerr::toCartesian, src/kerr.cpp:638-641), over 2 pi: great-circle angles
-//          between the directions at 32 equal sub-steps of the dense output, summed;
+//          between the directions at equal sub-steps of the dense output (report_parts),
+//          summed;
 //   r_min  the least r: the camera, the step ends, the event radius, and where w_r turns
 //          from in- to outgoing inside a step, the root of R(r) = 0 refined by Newton from
 //          the dense output's estimate;
 //   delay  ingoing Kerr-Schild coordinate time between the event and the camera (the
 //          reference's State::t, src/kerr.cpp:350: dt/dtau = (r^2 + a^2) I - a (a E
-//          sin^2 - L) - v_r with I its inflowFactor), by 5-point Gauss-Legendre on 16 equal
-//          parts of each step's dense output (the integrand grows as 1/(r - r+) toward the
-//          capture radius).
+//          sin^2 - L) - v_r with I its inflowFactor), by 5-point Gauss-Legendre on equal
+//          parts of each step's dense output, half as many as the swept angle's (at least
+//          16: the integrand grows as 1/(r - r+) toward the capture radius).
 struct ReportAcc {
     double swept, r_min, delay;
     bool done;  // the event has been reached: nothing more accumulates
 };
 
-constexpr int kReportSub = 32, kReportQuad = 16;
+// parts of a step for the swept angle (and half as many for the delay): at least 32, and
+// one per 0.02 rad of the ingoing phi the step turns through, which near the capture
+// radius the outgoing phi carried (phi_in = phi_out + 2 psi(r)) does not show: bounded by
+// |dr| |a| / Delta at the step's lower end.  Plain arithmetic, so host and device agree.
+constexpr int kReportSub = 32, kReportMaxSub = 65536;
+__device__ __forceinline__ int report_parts(const Photon<double>& p, bool ingoing, double r0, double r1, double dphi) {
+    double turn = fabs(dphi);
+    if (!ingoing) {
+        const double lo = fmin(r0, r1);
+        const double dlo = fmax(delta_of(p, lo), 1e-12);
+        turn += 2.0 * fabs(r1 - r0) * fabs(p.a) / dlo;
+    }
+    const double n = (double)kReportSub + ceil(turn / 0.02);
+    return n < (double)kReportMaxSub ? (int)n : kReportMaxSub;
+}
 // Gauss-Legendre on [0, 1], 5 nodes (x = (1 + t) / 2, w = W / 2 of the [-1, 1] rule)
 constexpr double kGLx[5] = {0.04691007703066800, 0.23076534494715845, 0.5, 0.76923465505284155,
                             0.95308992296933200};
@@ -865,7 +903,7 @@ __device__ __forceinline__ double dense_root(const Dense<double>& d, double targ
     return 0.5 * (lo + hi);
 }
 
-template <bool KS>
+template <int PHI>
 __device__ __forceinline__ void report_start(const Lane<double>& s, ReportAcc& acc) {
     acc.swept = 0.0;
     acc.r_min = s.y[0];
@@ -875,11 +913,11 @@ __device__ __forceinline__ void report_start(const Lane<double>& s, ReportAcc& a
 
 // lane_step in FP64 with the report accumulated over the accepted steps (the integration
 // itself is the same operation for operation, so every other output equals lane_step's)
-template <bool KS>
+template <int PHI>
 __device__ __forceinline__ bool report_step(Lane<double>& s, ReportAcc& acc, double rtol, double atol, int max_steps,
                                             Result& res) {
     res.flags = 0;
-    if (KS && s.dark) {
+    if (PHI == kPhiIn && s.dark) {
         res.cls = kCaptured;
         return true;
     }
@@ -896,7 +934,7 @@ __device__ __forceinline__ bool report_step(Lane<double>& s, ReportAcc& acc, dou
         last = true;
     }
     const double h = s.h;
-    const double err = dp5_attempt<double, KS>(s.p, s.tail, s.u, s.y, h, k, yn, rtol, atol);
+    const double err = dp5_attempt<double, PHI>(s.p, s.tail, s.u, s.y, h, k, yn, rtol, atol);
     ++s.steps;
     const bool main = !s.tail;
     const bool crossing = main && s.disk && s.y[1] * yn[1] < 0.0;
@@ -910,7 +948,7 @@ __device__ __forceinline__ bool report_step(Lane<double>& s, ReportAcc& acc, dou
         }
     }
     const bool flip = s.p.L == 0.0 && s.y[1] * s.y[4] > 0.0 && yn[1] * yn[4] < 0.0;
-    const bool ended = lane_finish<double, KS>(s, err, h, last, yn, k[6], crossing, d[0], d[1], res);
+    const bool ended = lane_finish<double, PHI>(s, err, h, last, yn, k[6], crossing, d[0], d[1], res);
     if (!main || acc.done || (!ended && s.rejected)) return ended;
     if (ended && (res.flags & kFlagNonfinite)) {
         acc.done = true;
@@ -945,27 +983,30 @@ __device__ __forceinline__ bool report_step(Lane<double>& s, ReportAcc& acc, dou
     // turns: sub-step directions (through the pole for L = 0 the dense phi lacks the
     // half-turn the step adds, so it is added where mu w_mu has changed sign)
     double dir0[3], dir1[3];
-    ks_direction(a, y0[0], y0[1], KS ? y0[2] : y0[2] + ks_shift(a, y0[0]), dir0);
-    for (int j = 1; j <= kReportSub; ++j) {
-        const double th = th_end * (double)j / (double)kReportSub;
+    const int nsub = report_parts(s.p, PHI == kPhiIn, y0[0], d[0].at(th_end), d[2].at(th_end) - y0[2]);
+    // toCartesian takes the ingoing phi: phi_in = phi_out + 2 psi(r)
+    ks_direction(a, y0[0], y0[1], PHI == kPhiIn ? y0[2] : y0[2] + 2.0 * ks_shift(a, y0[0]), dir0);
+    for (int j = 1; j <= nsub; ++j) {
+        const double th = th_end * (double)j / (double)nsub;
         const double r = d[0].at(th), mu = d[1].at(th);
         double ph = d[2].at(th);
         if (flip && mu * d[4].at(th) < 0.0) ph += kPi;
-        ks_direction(a, r, mu, KS ? ph : ph + ks_shift(a, r), dir1);
+        ks_direction(a, r, mu, PHI == kPhiIn ? ph : ph + 2.0 * ks_shift(a, r), dir1);
         acc.swept += angle_between(dir0, dir1);
         dir0[0] = dir1[0];
         dir0[1] = dir1[1];
         dir0[2] = dir1[2];
     }
     // delay
+    const int nquad = nsub / 2;
     double q = 0.0;
-    for (int m = 0; m < kReportQuad; ++m)
+    for (int m = 0; m < nquad; ++m)
 #pragma unroll
         for (int g = 0; g < 5; ++g) {
-            const double th = th_end * ((double)m + kGLx[g]) / (double)kReportQuad;
+            const double th = th_end * ((double)m + kGLx[g]) / (double)nquad;
             q = fma(kGLw[g], dt_dtau(s.p, d[0].at(th), d[1].at(th), d[3].at(th)), q);
         }
-    acc.delay = fma(h * th_end / (double)kReportQuad, q, acc.delay);
+    acc.delay = fma(h * th_end / (double)nquad, q, acc.delay);
     return ended;
 }
 

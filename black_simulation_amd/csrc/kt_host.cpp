@@ -33,6 +33,16 @@ constexpr double kPi = 3.14159265358979323846;
 __attribute__((noinline)) double hsin(double x) { return std::sin(x); }
 __attribute__((noinline)) double hcos(double x) { return std::cos(x); }
 
+// phi_in - phi_BL = psi(r) = integral from infinity to r of a / Delta (zero at infinity);
+// phi_out - phi_BL = -psi(r).  The same operations as the device's ks_shift.
+double ks_shift_host(double a, double r) {
+    if (a == 0.0) return 0.0;
+    const double d = 2.0 * std::sqrt(std::fmax(0.0, 1.0 - a * a));
+    const double rm = 1.0 - 0.5 * d;
+    if (d == 0.0) return -a / (r - 1.0);
+    return a / d * std::log1p(-d / (r - rm));
+}
+
 // ---- closed forms (src/kerr.cpp:157-206, 296-299) ----
 
 double horizon(double a) {
@@ -260,6 +270,8 @@ ViewDev make_view_dev(const kt_view& v, long long offset) {
     d.a = a;
     d.r_cam = v.r_cam;
     d.phi_cam = v.phi_cam;
+    // the integrator carries the outgoing Kerr-Schild phi (ingoing for observer cameras)
+    d.phi0 = v.camera_kind == KT_CAMERA_OBSERVER ? v.phi_cam : v.phi_cam - ks_shift_host(a, v.r_cam);
     d.half_width = v.half_width;
     d.half_height = v.half_height;
     d.px = 2.0 * v.half_width / (double)v.width;
@@ -487,6 +499,8 @@ kti::TraceLaunch make_launch(kt_ctx* ctx, const ViewDev* d_views, int n_views, l
     L.out = dout;
     L.ray_list = nullptr;
     L.total_dev = nullptr;
+    L.bucket_count = nullptr;
+    L.bucket_stride = 0;
     L.rtol = o.rtol;
     L.atol = o.atol;
     L.rtol32 = o.rtol32;
@@ -642,7 +656,7 @@ void kt_trace_opts_default(kt_trace_opts* o) {
     o->atol32 = 1e-7;
     o->restep_drift = 1e-6;
     o->gate_prob = 0.999;
-    o->gate_ring = 0.03;
+    o->gate_ring = 0.015;
 }
 
 void kt_view_default(kt_view* v) {

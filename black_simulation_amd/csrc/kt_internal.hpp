@@ -21,6 +21,10 @@ struct TraceLaunch {
     ktd::Outputs out;
     const long long* ray_list;  // optional: trace only these ray indices (fallback re-queue)
     const long long* total_dev; // optional: the number of rays, read on the device (total = capacity)
+    // optional, with ray_list: the list holds kQueueBuckets regions bucket_stride apart, region
+    // b holding bucket_count[b] entries; ray q of the launch is taken from them in order
+    const unsigned long long* bucket_count;
+    long long bucket_stride;
     double rtol, atol, rtol32, atol32, restep_drift;
     int max_steps;
     int precision;              // KT_PREC_*
@@ -28,6 +32,8 @@ struct TraceLaunch {
     int block, grid, steps_per_dequeue, waves_per_simd;
     int ks;                     // 1: observer cameras (FP64, Kerr-Schild phi)
 };
+
+constexpr int kQueueBuckets = 3;
 
 // launches the counter reset and the trace kernel on `stream`; returns a hipError_t
 hipError_t launch_trace(const TraceLaunch& L, hipStream_t stream);

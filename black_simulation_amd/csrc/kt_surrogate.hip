@@ -363,10 +363,13 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
+// the fallback queue's length (the sum over its regions) for the integrator and the stats
 __global__ void k_copy_count(const unsigned long long* c, long long* dst, unsigned long long* stat) {
     if (threadIdx.x == 0) {
-        *dst = (long long)*c;
-        if (stat) *stat = *c;
+        unsigned long long n = 0;
+        for (int b = 0; b < kti::kQueueBuckets; ++b) n += c[b];
+        *dst = (long long)n;
+        if (stat) *stat = n;
     }
 }
 
@@ -485,8 +488,12 @@ struct FusedArgs {
     float gate_prob, gate_ring;
     float extent;          // answered rays have |alpha|, |beta| <= extent x 1.5 R_sh
     ktd::Outputs out;
-    long long* fallback;
-    unsigned long long* n_fallback;
+    long long* fallback;             // kti::kQueueBuckets regions fallback_stride apart
+    unsigned long long* n_fallback;  // their counts
+    long long fallback_stride;
+    // k_mlp_fast: tiles past the first gridDim.x are claimed from this counter (zero at
+    // launch), so blocks that start late, on CUs still busy with other work, take fewer
+    unsigned long long* next_tile;
 };
 
 // -log2(e) silu(z) from z' = -log2(e) z
@@ -535,13 +542,20 @@ __device__ __forceinline__ void gate_and_write(const FusedArgs& a, long long q, 
     const int c = p0 >= p1 && p0 >= p2 ? 0 : (p1 >= p2 ? 1 : 2);
     const float pm = c == 0 ? p0 : c == 1 ? p1 : p2;
     const bool to_fallback = live && owner && (!(pm > a.gate_prob) || !(rg > a.gate_ring));
-    const unsigned long long mask = __ballot(to_fallback);
-    if (mask) {
-        const int first = __ffsll((long long)mask) - 1;
-        unsigned long long base = 0;
-        if (lane == first) base = atomicAdd(a.n_fallback, (unsigned long long)__popcll(mask));
-        base = __shfl(base, first);
-        if (to_fallback) a.fallback[base + __popcll(mask & ((1ull << lane) - 1))] = q;
+    // the fallback queue in one region per predicted class, so that a wave of the integrator
+    // takes rays of like length (escaped rays near the ring run longest, disk rays shortest):
+    // one atomic per wave and region
+#pragma unroll
+    for (int b = 0; b < kti::kQueueBuckets; ++b) {
+        const unsigned long long mask = __ballot(to_fallback && c == b);
+        if (mask) {
+            const int first = __ffsll((long long)mask) - 1;
+            unsigned long long base = 0;
+            if (lane == first) base = atomicAdd(a.n_fallback + b, (unsigned long long)__popcll(mask));
+            base = __shfl(base, first);
+            if (to_fallback && c == b)
+                a.fallback[b * a.fallback_stride + (long long)(base + __popcll(mask & ((1ull << lane) - 1)))] = q;
+        }
     }
     if (!live || !owner || to_fallback) return;
     const double nan = __builtin_nan("");
@@ -780,6 +794,93 @@ __device__ __forceinline__ void hsilu_pair(const f32x16& z, bf16x8& p0, bf16x8& 
     for (int i = 0; i < 8; ++i) p1[i] = (__bf16)hsilu_scaled(z[8 + i]);
 }
 
+// the input layer's B operand of the 32 rays of a group (lane m and m + 32 hold ray m):
+// split-bf16 features, k-slots (hi 0-4, lo 0-2) in the lower half-wave and (lo 3-4, hi 0-4,
+// 0) in the upper, as the input layer's fragments expect
+__device__ __forceinline__ bf16x8 input_split(const float f[5], int h) {
+    float fh[5], fl[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        fh[k] = (float)(__bf16)f[k];
+        fl[k] = f[k] - fh[k];
+    }
+    const float x0[8] = {fh[0], fh[1], fh[2], fh[3], fh[4], fl[0], fl[1], fl[2]};
+    const float x1[8] = {fl[3], fl[4], fh[0], fh[1], fh[2], fh[3], fh[4], 0.f};
+    bf16x8 xb;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) xb[k] = (__bf16)(h ? x1[k] : x0[k]);
+    return xb;
+}
+
+// The network for both 32-ray groups of a wave at once (group g's ray m in lanes m and
+// m + 32), layer by layer: each weight fragment read from LDS feeds the two groups' MFMAs,
+// which halves the LDS traffic of the single-group form, and the two accumulation chains
+// interleave, so an MFMA never waits on the one before it.  Returns the four outputs
+// 4 h .. 4 h + 3 of group g's ray m in lane (m, h) of y[g].
+__device__ __forceinline__ void fast_network2(const uint4* W, const float* Bs, int lane, int h, const float f0[5],
+                                              const float f1[5], float y0[4], float y1[4]) {
+    const uint4* Win = W + 3 * kFastLayerFrags + kFastOutFrags;
+    const uint4* Wout = W + 3 * kFastLayerFrags;
+    bf16x8 B0[kFastK], B1[kFastK];
+    {
+        const bf16x8 xb0 = input_split(f0, h), xb1 = input_split(f1, h);
+#pragma unroll
+        for (int fb = 0; fb < kFastBlocks; ++fb) {
+            const uint4 w = Win[fb * 64 + lane];
+            const f32x16 bias = block_bias(Bs, fb, h);
+            const f32x16 z0 = mfma32(w, xb0, bias);
+            const f32x16 z1 = mfma32(w, xb1, bias);
+            hsilu_pair(z0, B0[2 * fb], B0[2 * fb + 1]);
+            hsilu_pair(z1, B1[2 * fb], B1[2 * fb + 1]);
+        }
+    }
+#pragma unroll
+    for (int l = 1; l <= 3; ++l) {
+        const uint4* Wl = W + (l - 1) * kFastLayerFrags;
+        const float* bl = Bs + l * kFastH;
+        bf16x8 C0[kFastK], C1[kFastK];
+#pragma unroll
+        for (int fb = 0; fb < kFastBlocks; ++fb) {
+            f32x16 acc0 = block_bias(bl, fb, h);
+            f32x16 acc1 = acc0;
+            uint4 af[kFastK];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < kFastK; ++i) af[i] = Wl[(fb * kFastK + i) * 64 + lane];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < kFastK; ++i) {
+                acc0 = mfma32(af[i], B0[i], acc0);
+                acc1 = mfma32(af[i], B1[i], acc1);
+            }
+            hsilu_pair(acc0, C0[2 * fb], C0[2 * fb + 1]);
+            hsilu_pair(acc1, C1[2 * fb], C1[2 * fb + 1]);
+        }
+#pragma unroll
+        for (int k = 0; k < kFastK; ++k) {
+            B0[k] = C0[k];
+            B1[k] = C1[k];
+        }
+    }
+    f32x16 o0[2], o1[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) o0[p][i] = o1[p][i] = 0.f;
+#pragma unroll
+    for (int kb = 0; kb < kFastK; ++kb) {
+        const uint4 w = Wout[kb * 64 + lane];
+        o0[kb & 1] = mfma32(w, B0[kb], o0[kb & 1]);
+        o1[kb & 1] = mfma32(w, B1[kb], o1[kb & 1]);
+    }
+    const float* b4 = Bs + 4 * kFastH;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        y0[r] = (o0[0][r] + o0[1][r]) * 6.0f + b4[4 * h + r];
+        y1[r] = (o1[0][r] + o1[1][r]) * 6.0f + b4[4 * h + r];
+    }
+}
+
 // the network for the 32 rays of a wave's group on features f (lane m and m + 32 hold ray
 // m): returns the four outputs 4 h .. 4 h + 3 of ray m in lane (m, h)
 __device__ __forceinline__ void fast_network(const uint4* W, const float* Bs, int lane, int h, const float f[5],
@@ -862,7 +963,8 @@ __global__ __launch_bounds__(kFastThreads) __attribute__((amdgpu_waves_per_eu(2,
     __syncthreads();
     constexpr int kPair = 2 * kFastTile;  // rays per block and iteration: 8 waves x 64
     const long long tiles = (a.n + kPair - 1) / kPair;
-    for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    __shared__ long long claimed;
+    for (long long tile = blockIdx.x; tile < tiles;) {
         const long long q = tile * kPair + wave * 64 + lane;  // this lane's ray
         const bool live = q < a.n;
         const long long qc = live ? q : a.n - 1;
@@ -887,8 +989,7 @@ __global__ __launch_bounds__(kFastThreads) __attribute__((amdgpu_waves_per_eu(2,
             g1[k] = __shfl(f[k], m + 32);
         }
         float y0[4], y1[4];
-        fast_network(W, Bs, lane, h, g0, y0);
-        fast_network(W, Bs, lane, h, g1, y1);
+        fast_network2(W, Bs, lane, h, g0, g1, y0, y1);
         // outputs 0-3 of group 0's ray m sit in lane m, 4-7 in lane m + 32; group 1's alike:
         // one exchange across the half-waves gives lane l all eight of ray l
         float y[8];
@@ -902,9 +1003,17 @@ __global__ __launch_bounds__(kFastThreads) __attribute__((amdgpu_waves_per_eu(2,
             if (live)
 #pragma unroll
                 for (int k = 0; k < 8; ++k) a.y[q * 8 + k] = y[k];
-            continue;
+        } else {
+            gate_and_write(a, q, live, true, lane, y, rg);
         }
-        gate_and_write(a, q, live, true, lane, y, rg);
+        if (a.next_tile) {
+            __syncthreads();  // every wave has read `claimed` for this tile
+            if (tid == 0) claimed = (long long)gridDim.x + (long long)atomicAdd(a.next_tile, 1ull);
+            __syncthreads();
+            tile = claimed;
+        } else {
+            tile += gridDim.x;
+        }
     }
 }
 
@@ -1597,7 +1706,7 @@ static hipError_t evaluate(Surrogate* s, Dataset& d, const TrainOpts& o, TrainRe
         const int c = e0 >= e1 && e0 >= e2 ? 0 : (e1 >= e2 ? 1 : 2);
         const double pm = std::max(e0, std::max(e1, e2)) / sum;
         agree += c == cls[q];
-        if (!(pm > 0.999) || !(ring[q] > 0.03)) continue;
+        if (!(pm > 0.999) || !(ring[q] > 0.015)) continue;
         ++gated;
         gated_agree += c == cls[q];
         if (c != cls[q]) continue;
@@ -1737,7 +1846,7 @@ hipError_t queue_reserve(Queue& q, long long n) {
     if (q.list) (void)hipFree(q.list);
     q.list = nullptr;
     q.cap = 0;
-    KTS_TRY(dalloc(&q.list, (size_t)n));
+    KTS_TRY(dalloc(&q.list, (size_t)n * kti::kQueueBuckets));
     q.cap = n;
     return hipSuccess;
 }
@@ -1769,6 +1878,8 @@ hipError_t trace_with_surrogate(Surrogate* s, const kti::TraceLaunch& L, double 
     a.out = L.out;
     a.fallback = Q.list;
     a.n_fallback = Q.count;
+    a.fallback_stride = Q.cap;
+    a.next_tile = Q.count + kti::kQueueBuckets;  // zeroed above with the fallback counts
     launch_network(s, a, true, st);
     hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(64), 0, st, Q.count, Q.total, (unsigned long long*)nullptr);
     KTS_TRY(hipGetLastError());
@@ -1781,6 +1892,8 @@ hipError_t trace_with_surrogate(Surrogate* s, const kti::TraceLaunch& L, double 
     kti::TraceLaunch F = L;
     F.ray_list = Q.list;
     F.total_dev = Q.total;
+    F.bucket_count = Q.count;
+    F.bucket_stride = Q.cap;
     if (ev) KTS_TRY(hipEventRecord(ev[1], st));
     KTS_TRY(kti::launch_trace(F, st));
     if (ev) KTS_TRY(hipEventRecord(ev[2], st));

@@ -97,7 +97,7 @@ void set_domain(Surrogate* s, const Domain& d);
 struct Queue {
     long long cap = 0;
     long long* list = nullptr;
-    unsigned long long* count = nullptr;  // [0] rays queued
+    unsigned long long* count = nullptr;  // [0..2] rays queued per region, [3] the network's tile counter
     long long* total = nullptr;           // the same count, as the integrator reads it
 };
 hipError_t queue_reserve(Queue& q, long long n);

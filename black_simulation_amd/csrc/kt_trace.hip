@@ -53,6 +53,26 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
     return x;
 }
 
+// the queue slot of a launch's q-th ray: regions of a bucketed queue are read in order
+struct QueueMap {
+    long long c0, c1, stride;
+    bool bucketed;
+    __device__ __forceinline__ explicit QueueMap(const kti::TraceLaunch& L) : c0(0), c1(0), stride(0), bucketed(false) {
+        if (L.bucket_count) {
+            bucketed = true;
+            c0 = (long long)L.bucket_count[0];
+            c1 = (long long)L.bucket_count[1];
+            stride = L.bucket_stride;
+        }
+    }
+    __device__ __forceinline__ long long slot(long long q) const {
+        if (!bucketed || q < c0) return q;
+        if (q < c0 + c1) return stride + (q - c0);
+        return 2 * stride + (q - c0 - c1);
+    }
+};
+static_assert(kti::kQueueBuckets == 3, "QueueMap reads three regions");
+
 __device__ __forceinline__ void launch_of(const ViewDev* views, int n_views, long long idx, Launch& la) {
     const ViewDev& v = views[find_view(views, n_views, idx)];
     launch_photon(v, idx - v.offset, la);
@@ -62,26 +82,26 @@ __device__ __forceinline__ void launch_of(const ViewDev* views, int n_views, lon
 template <class R>
 struct Plain {
     using LaneT = Lane<R>;
-    __device__ static void start(LaneT& s, const Launch& la) { lane_start(s, la); }
+    __device__ static void start(LaneT& s, const Launch& la) { lane_start<R, kPhiOut>(s, la); }
     __device__ static bool step(LaneT& s, const kti::TraceLaunch& L, Result& r);
     __device__ static int steps(const LaneT& s) { return s.steps; }
     __device__ static int fp64_steps(const LaneT&) { return 0; }
 };
 template <>
 __device__ bool Plain<double>::step(Lane<double>& s, const kti::TraceLaunch& L, Result& r) {
-    return lane_step(s, L.rtol, L.atol, L.max_steps, r);
+    return lane_step<double, kPhiOut>(s, L.rtol, L.atol, L.max_steps, r);
 }
 template <>
 __device__ bool Plain<float>::step(Lane<float>& s, const kti::TraceLaunch& L, Result& r) {
-    return lane_step(s, (float)L.rtol32, (float)L.atol32, L.max_steps, r);
+    return lane_step<float, kPhiOut>(s, (float)L.rtol32, (float)L.atol32, L.max_steps, r);
 }
 
 // Observer cameras (kind 3): FP64 with the Kerr-Schild phi, inside the horizon allowed.
 struct PlainKS {
     using LaneT = Lane<double>;
-    __device__ static void start(LaneT& s, const Launch& la) { lane_start<double, true>(s, la); }
+    __device__ static void start(LaneT& s, const Launch& la) { lane_start<double, kPhiIn>(s, la); }
     __device__ static bool step(LaneT& s, const kti::TraceLaunch& L, Result& r) {
-        return lane_step<double, true>(s, L.rtol, L.atol, L.max_steps, r);
+        return lane_step<double, kPhiIn>(s, L.rtol, L.atol, L.max_steps, r);
     }
     __device__ static int steps(const LaneT& s) { return s.steps; }
     __device__ static int fp64_steps(const LaneT&) { return 0; }
@@ -91,7 +111,7 @@ struct Mixed {
     using LaneT = MixedLane;
     __device__ static void start(LaneT& s, const Launch& la) {
         const Photon<double>& p = la.p;
-        lane_start(s.d, la);
+        lane_start<double, kPhiOut>(s.d, la);
         s.pf.a = (float)p.a;
         s.pf.a2 = (float)p.a2;
         s.pf.E = (float)p.E;
@@ -128,6 +148,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
 
     // a device-side count (the surrogate's fallback queue) overrides the launch size
     const long long total = L.total_dev ? *L.total_dev : L.total;
+    const QueueMap qm(L);
     typename Pol::LaneT s;
     bool active = false;
     bool drained = false;
@@ -146,7 +167,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
                     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
                     const long long q = (long long)(base + (unsigned long long)__popcll(idle & below));
                     if (q < total) {
-                        my = L.ray_list ? L.ray_list[q] : q;
+                        my = L.ray_list ? L.ray_list[qm.slot(q)] : q;
                         Launch la;
                         launch_of(views, L.n_views, my, la);
                         Pol::start(s, la);
@@ -175,15 +196,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
     if (my_restep_rays) atomicAdd(&L.stats[kti::kStatRestepRays], my_restep_rays);
 }
 
+// One lane per ray.  With a count read on the device (the surrogate's fallback queue) the
+// grid is sized for the likely count, not for the batch, and strides over the queue when
+// it holds more rays than the grid has lanes.
 template <class Pol>
 __global__ __launch_bounds__(256) void k_trace_oneshot(kti::TraceLaunch L) {
+    // issue priority over other kernels' waves on the same SIMD: a fallback queue's time is
+    // the latency of its longest rays, while the surrogate's network beside it (another
+    // frame, another stream) is throughput work that fills the issue slots left over
+    if (L.total_dev) __builtin_amdgcn_s_setprio(3);
     const ViewDev* views = stage_views(L, view_lds);
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = L.total_dev ? *L.total_dev : L.total;
-    if ((long long)blockIdx.x * blockDim.x >= total) return;  // the whole block is past the end
-    unsigned long long st = 0, s64 = 0;
-    if (q < total) {
-        const long long my = L.ray_list ? L.ray_list[q] : q;
+    const QueueMap qm(L);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    unsigned long long st = 0, s64 = 0, nre = 0;
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < total; base += stride) {  // block-uniform
+        const long long q = base + threadIdx.x;
+        if (q >= total) break;
+        const long long my = L.ray_list ? L.ray_list[qm.slot(q)] : q;
         typename Pol::LaneT s;
         Launch la;
         launch_of(views, L.n_views, my, la);
@@ -193,14 +223,15 @@ __global__ __launch_bounds__(256) void k_trace_oneshot(kti::TraceLaunch L) {
         }
         const int f64 = Pol::fp64_steps(s);
         write_result(L.out, my, r, Pol::steps(s), f64 > 0 ? kFlagRestep : 0);
-        st = (unsigned long long)Pol::steps(s);
-        s64 = (unsigned long long)f64;
+        st += (unsigned long long)Pol::steps(s);
+        s64 += (unsigned long long)f64;
+        nre += f64 > 0 ? 1ull : 0ull;
     }
     // the counters, summed over the wave first: one atomic per wave each (every lane of
     // the wave reaches this point; lanes past the end add 0)
-    const unsigned long long nre = __popcll(__ballot(s64 > 0));
     st = wave_sum(st);
     s64 = wave_sum(s64);
+    nre = wave_sum(nre);
     if ((threadIdx.x & 63u) == 0u) {
         if (st) atomicAdd(&L.stats[kti::kStatSteps], st);
         if (s64) atomicAdd(&L.stats[kti::kStatFp64Steps], s64);
@@ -211,7 +242,7 @@ __global__ __launch_bounds__(256) void k_trace_oneshot(kti::TraceLaunch L) {
 // The single-ray report: the one-shot FP64 trace (Plain<double>, or PlainKS for observer
 // cameras) with report_step, which integrates exactly as lane_step does and accumulates
 // turns, r_min and delay; they are written beside the usual outputs.
-template <bool KS>
+template <int PHI>
 __global__ __launch_bounds__(256) void k_trace_report(kti::TraceLaunch L) {
     const ViewDev* views = stage_views(L, view_lds);
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -221,11 +252,11 @@ __global__ __launch_bounds__(256) void k_trace_report(kti::TraceLaunch L) {
         Lane<double> s;
         Launch la;
         launch_of(views, L.n_views, q, la);
-        lane_start<double, KS>(s, la);
+        lane_start<double, PHI>(s, la);
         ReportAcc acc;
-        report_start<KS>(s, acc);
+        report_start<PHI>(s, acc);
         Result r;
-        while (!report_step<KS>(s, acc, L.rtol, L.atol, L.max_steps, r)) {
+        while (!report_step<PHI>(s, acc, L.rtol, L.atol, L.max_steps, r)) {
         }
         write_result(L.out, q, r, s.steps, 0);
         write_report(L.out, q, s.p, acc);
@@ -295,7 +326,11 @@ void launch_persistent(const TraceLaunch& L, hipStream_t stream) {
 
 template <class Pol>
 void launch_oneshot(const TraceLaunch& L, hipStream_t stream) {
-    const dim3 grid((unsigned)((L.total + L.block - 1) / L.block)), block(L.block);
+    long long blocks = (L.total + L.block - 1) / L.block;
+    // a queue counted on the device: twice the persistent grid's lanes cover the usual
+    // share of rejected rays in one pass (the kernel strides over a longer queue)
+    if (L.total_dev) blocks = std::min<long long>(blocks, 2LL * std::max(1, L.grid));
+    const dim3 grid((unsigned)std::max(1LL, blocks)), block(L.block);
     k_trace_oneshot<Pol><<<grid, block, view_lds_bytes(L.n_views), stream>>>(L);
 }
 
@@ -304,8 +339,8 @@ hipError_t launch_trace(const TraceLaunch& L, hipStream_t stream) {
     if (L.total <= 0) return hipGetLastError();
     if (L.out.turns || L.out.r_min || L.out.delay || L.out.b || L.out.q) {
         const dim3 grid((unsigned)((L.total + L.block - 1) / L.block)), block(L.block);
-        if (L.ks) k_trace_report<true><<<grid, block, view_lds_bytes(L.n_views), stream>>>(L);
-        else k_trace_report<false><<<grid, block, view_lds_bytes(L.n_views), stream>>>(L);
+        if (L.ks) k_trace_report<kPhiIn><<<grid, block, view_lds_bytes(L.n_views), stream>>>(L);
+        else k_trace_report<kPhiOut><<<grid, block, view_lds_bytes(L.n_views), stream>>>(L);
     } else if (L.ks) {
         if (L.mode == 1) launch_oneshot<PlainKS>(L, stream);
         else launch_persistent<PlainKS>(L, stream);

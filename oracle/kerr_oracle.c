@@ -304,13 +304,24 @@ void ko_rhs(const ko_photon* p, const double y[5], double dy[5]) {
        (P - w_r) / Delta = inflowFactor of kerr.cpp:323-329 in backward time: K / (P + w_r)
        when P w_r > 0 (by w_r^2 = P^2 - Delta K), else with Delta floored at 1e-12 */
     double nd = P, dd = delta;
-    if (p->ks) {
-        const double w = y[3];
+    const double w = y[3];
+    if (p->ks == 1) {
         if (P * w > 0.0) {
             nd = p->K;
             dd = P + w;
         } else {
             nd = P - w;
+            dd = fabs(delta) > 1e-12 ? delta : copysign(1e-12, delta);
+        }
+    } else if (p->ks == 2) {
+        /* outgoing Kerr-Schild phi (dphi_out = dphi_BL - a dr / Delta), regular on the past
+           horizon the ray approaches when followed back into the hole from outside:
+           (P + w_r) / Delta = K / (P - w_r) when P w_r < 0 */
+        if (P * w < 0.0) {
+            nd = p->K;
+            dd = P - w;
+        } else {
+            nd = P + w;
             dd = fabs(delta) > 1e-12 ? delta : copysign(1e-12, delta);
         }
     }
@@ -336,6 +347,7 @@ static void tail_rhs(const ko_photon* p, double u, const double z[3], double dz[
     const double num = fma(p->a * Pu, s2, p->L * Du);
     dz[0] = -(z[2] * isq);
     dz[1] = (num / (Du * s2) - p->aE) * isq;
+    if (p->ks == 2) dz[1] = dz[1] + p->a / Du; /* dphi_out/du = dphi_BL/du + a / Du */
     dz[2] = (m * fma(p->twoA2E2, m * m, p->C1)) * isq;
 }
 
@@ -890,14 +902,15 @@ static void finish_escape(const ko_photon* p, const ko_opts* o, const double y[5
    r = r_cap or r = r_esc, located on the step's dense output):
      turns  the angle swept by the direction of kerr::toCartesian(a, r, theta, phi_KS)
             (kerr.cpp:638-641), over 2 pi, as great-circle angles between the directions at
-            32 equal sub-steps of the dense output;
+            equal sub-steps of the dense output (report_parts);
      r_min  the least r: the camera, the step ends, the event radius, and where w_r turns
             from in- to outgoing inside a step the root of R(r) = 0, Newton-refined from
             the dense output's estimate;
      delay  ingoing Kerr-Schild coordinate time from the event to the camera, the integral
             of dt/dtau (kerr.cpp:350, with inflowFactor kerr.cpp:322-329 and v_r = -w_r)
-            over s by 5-point Gauss-Legendre on 16 equal parts of each step's dense output
-            (the integrand grows as 1/(r - r+) toward the capture radius). */
+            over s by 5-point Gauss-Legendre on equal parts of each step's dense output, half
+            as many as the swept angle's (the integrand grows as 1/(r - r+) toward the capture
+            radius). */
 static const double GL_X[5] = {0.04691007703066800, 0.23076534494715845, 0.5, 0.76923465505284155,
                                0.95308992296933200};
 static const double GL_W[5] = {0.11846344252809454, 0.23931433524968324, 0.28444444444444444,
@@ -944,6 +957,20 @@ static double dense_root(const ko_dense* d, double target, int from_below) {
 
 typedef struct { double swept, r_min, delay; } report_acc;
 
+/* parts of a step for the swept angle (half as many for the delay): at least 32, and one per
+   0.02 rad of the ingoing phi the step turns through, bounded for an outgoing phi by
+   |dr| |a| / Delta at the step's lower end (phi_in = phi_out + 2 psi(r)) */
+static int report_parts(const ko_photon* p, double r0, double r1, double dphi) {
+    double turn = fabs(dphi);
+    if (p->ks != 1) {
+        const double lo = fmin(r0, r1);
+        const double dlo = fmax(delta_of(p, lo), 1e-12);
+        turn += 2.0 * fabs(r1 - r0) * fabs(p->a) / dlo;
+    }
+    const double n = 32.0 + ceil(turn / 0.02);
+    return n < 65536.0 ? (int)n : 65536;
+}
+
 static void report_accumulate(const ko_photon* p, const double y0[5], const ko_dense d[5], double h, double th_end,
                               int flip, report_acc* acc) {
     const double a = p->a;
@@ -962,25 +989,28 @@ static void report_accumulate(const ko_photon* p, const double y0[5], const ko_d
     }
     acc->r_min = rm;
     double dir0[3], dir1[3];
-    ks_direction(a, y0[0], y0[1], p->ks ? y0[2] : y0[2] + ko_ks_shift(a, y0[0]), dir0);
-    for (int j = 1; j <= 32; ++j) {
-        const double th = th_end * (double)j / 32.0;
+    const int nsub = report_parts(p, y0[0], dense_at(&d[0], th_end), dense_at(&d[2], th_end) - y0[2]);
+    /* toCartesian takes the ingoing phi: phi_in = phi_out + 2 psi(r) */
+    ks_direction(a, y0[0], y0[1], p->ks == 1 ? y0[2] : y0[2] + 2.0 * ko_ks_shift(a, y0[0]), dir0);
+    for (int j = 1; j <= nsub; ++j) {
+        const double th = th_end * (double)j / (double)nsub;
         const double r = dense_at(&d[0], th), mu = dense_at(&d[1], th);
         double ph = dense_at(&d[2], th);
         if (flip && mu * dense_at(&d[4], th) < 0.0) ph += KO_PI;
-        ks_direction(a, r, mu, p->ks ? ph : ph + ko_ks_shift(a, r), dir1);
+        ks_direction(a, r, mu, p->ks == 1 ? ph : ph + 2.0 * ko_ks_shift(a, r), dir1);
         acc->swept += angle_between(dir0, dir1);
         dir0[0] = dir1[0];
         dir0[1] = dir1[1];
         dir0[2] = dir1[2];
     }
+    const int nquad = nsub / 2;
     double q = 0.0;
-    for (int m = 0; m < 16; ++m)
+    for (int m = 0; m < nquad; ++m)
         for (int g = 0; g < 5; ++g) {
-            const double th = th_end * ((double)m + GL_X[g]) / 16.0;
+            const double th = th_end * ((double)m + GL_X[g]) / (double)nquad;
             q = fma(GL_W[g], dt_dtau(p, dense_at(&d[0], th), dense_at(&d[1], th), dense_at(&d[3], th)), q);
         }
-    acc->delay = fma(h * th_end / 16.0, q, acc->delay);
+    acc->delay = fma(h * th_end / (double)nquad, q, acc->delay);
 }
 
 static void trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, const double y0[5], ko_result* out,
@@ -996,8 +1026,18 @@ void ko_trace_photon_report(const ko_view* v, const ko_opts* o, const ko_photon*
     trace_photon(v, o, p, y0, out, rep);
 }
 
-static void trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p, const double y0[5], ko_result* out,
-                         ko_report* rep) {
+static void trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p_launch, const double y0_launch[5],
+                         ko_result* out, ko_report* rep) {
+    /* a launch in Boyer-Lindquist phi (Bardeen and pinhole views) is traced in the outgoing
+       Kerr-Schild phi, phi_out = phi_BL - psi(r), which equals phi_BL at infinity */
+    ko_photon pp = *p_launch;
+    double y0[5];
+    for (int i = 0; i < 5; ++i) y0[i] = y0_launch[i];
+    if (pp.ks == 0) {
+        pp.ks = 2;
+        y0[2] = y0_launch[2] - ko_ks_shift(pp.a, y0_launch[0]);
+    }
+    const ko_photon* p = &pp;
     const double a = v->spin;
     const double r_cap = v->capture_factor * ko_horizon_radius(a);
     const double r_esc = amax(v->escape_radius, 1.5 * v->r_cam);
@@ -1029,7 +1069,7 @@ static void trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p,
     /* inside the horizon every ray, followed back, rises to it; one arriving with
        P(r+) = 2 r+ E - a L < 0 came from the far side: dark (trajectory.cpp:234-241) */
     const double rh = ko_horizon_radius(a);
-    if (p->ks && v->r_cam < rh && 2.0 * rh * p->E - a * p->L < 0.0) {
+    if (p->ks == 1 && v->r_cam < rh && 2.0 * rh * p->E - a * p->L < 0.0) {
         out->cls = KO_CAPTURED;
         for (int i = 0; i < 5; ++i) out->y_end[i] = y[i];
         out->steps = 0;
@@ -1096,7 +1136,7 @@ static void trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p,
             }
         }
         /* observer cameras may start below r_cap: capture only heading inward (trajectory.cpp:377) */
-        if (yn[0] < r_cap && (!p->ks || yn[3] < 0.0)) {
+        if (yn[0] < r_cap && (p->ks != 1 || yn[3] < 0.0)) {
             out->cls = KO_CAPTURED;
             for (int i = 0; i < 5; ++i) out->y_end[i] = yn[i];
             out->steps = steps;
@@ -1107,7 +1147,7 @@ static void trace_photon(const ko_view* v, const ko_opts* o, const ko_photon* p,
         if (yn[0] > r_esc && yn[3] > 0.0) {
             if (rep) report_accumulate(p, y, dd, h, dense_root(&dd[0], r_esc, 1), flip, &acc);
             KO_REPORT_OUT();
-            if (p->ks) yn[2] -= ko_ks_shift(a, yn[0]); /* the tail runs in Boyer-Lindquist phi */
+            if (p->ks == 1) yn[2] -= ko_ks_shift(a, yn[0]); /* its tail runs in Boyer-Lindquist phi */
             finish_escape(p, o, yn, &steps, out);
             out->steps = steps;
             return;

@@ -84,7 +84,8 @@ void ko_camera_basis(double yaw, double pitch, double roll, double basis[9]);
 typedef struct {
     double a, a2, E, L, Q;
     double aE, aL, K, C1, A2E2, twoE, twoA2E2;
-    int ks; /* 1: phi is the ingoing Kerr-Schild phi (observer cameras) */
+    int ks; /* phi carried: 0 Boyer-Lindquist, 1 ingoing Kerr-Schild (observer cameras),
+               2 outgoing Kerr-Schild (how ko_trace_photon traces a Boyer-Lindquist launch) */
 } ko_photon;
 
 void ko_photon_init(ko_photon* p, double a, double E, double L, double Q);
