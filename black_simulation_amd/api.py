@@ -95,7 +95,7 @@ class TraceOpts:
     waves_per_cu: int = 0
     steps_per_dequeue: int = 0
     gate_prob: float = 0.999
-    gate_ring: float = 0.015
+    gate_ring: float = 0.01
     waves_per_simd: int = 0
     # EHT render / calibrate: a ctypes.c_int another thread may set to cancel (eht::render's
     # cancel flag); None: not cancellable

@@ -656,7 +656,7 @@ void kt_trace_opts_default(kt_trace_opts* o) {
     o->atol32 = 1e-7;
     o->restep_drift = 1e-6;
     o->gate_prob = 0.999;
-    o->gate_ring = 0.015;
+    o->gate_ring = 0.01;
 }
 
 void kt_view_default(kt_view* v) {

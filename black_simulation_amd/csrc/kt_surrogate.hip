@@ -721,15 +721,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
 // ---- the fast network: 5 -> 128 -> 128 -> 128 -> 128 -> 8, hard SiLU --------------------
 // The same transposed product and in-lane operand layout as k_mlp_fused, over four 32-feature
-// output blocks and eight k-blocks per layer.  Its bf16 weights, 108 KB, fit in LDS whole:
-// a block of eight waves stages them once and then runs over tiles of 256 rays until the
-// batch is done (one block per CU), so no weight traffic or barrier remains in the loop.
+// output blocks and eight k-blocks per layer.  Its 16-bit weights, 108 KB, fit in LDS whole:
+// a block of eight waves stages them once and then runs over tiles of ray tiles until the
+// batch is done (one block per CU), so no weight traffic remains in the loop.  The input
+// layer multiplies split-bf16 features by split-bf16 weights; the hidden and output layers
+// multiply FP16 activations by FP16 weights (the FP32 masters rounded once; FP16 holds
+// three more mantissa bits than bf16, and these values stay far inside its range), which
+// lets the activation run on packed halves.
 //
-// Hard SiLU, a = z clamp(z / 6 + 1/2, 0, 1), is two full-rate vector instructions.  The
-// kernel keeps every activation divided by 6: with z' = z / 6 it is z' clamp(z' + 1/2, 0, 1),
-// an add with the clamp modifier and a multiply.  The hidden layers' weights are unchanged
-// by this (their input carries the 1/6 and so does their output), the input layer's weights
-// and the biases of layers 0-3 are divided by 6, and the output layer multiplies by 6.
+// Hard SiLU, a = z clamp(z / 6 + 1/2, 0, 1).  The kernel keeps every activation divided by
+// 6: with z' = z / 6 it is z' clamp(z' + 1/2, 0, 1), an add with the clamp modifier and a
+// multiply.  The hidden layers' weights are unchanged by this (their input carries the 1/6
+// and so does their output), the input layer's weights and the biases of layers 0-3 are
+// divided by 6, and the output layer multiplies by 6.
 constexpr int kFastH = 128;
 constexpr int kFastBlocks = kFastH / 32, kFastK = kFastH / 16;   // output blocks, k-blocks
 constexpr int kFastLayerFrags = kFastBlocks * kFastK * 64;       // uint4 per hidden layer
@@ -739,10 +743,14 @@ constexpr int kFastFrags = 3 * kFastLayerFrags + kFastOutFrags + kFastInFrags;  
 constexpr int kFastBias = 4 * kFastH + 8;                        // b0..b3 / 6, then b4
 constexpr int kFastThreads = 512, kFastTile = 256;               // 8 waves x 32 rays per network pass
 
+__device__ __forceinline__ uint16_t f2h(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+
 // packed fragments from the FP32 masters: hidden layer l (1-3) block fb, k-block kb, lane
-// (row n = 32 fb + lane % 32, k-slots F(kb, lane / 32, 0..7)); then the output layer
-// [kb][lane] (rows n >= 8 zero); then the input layer's split fragments of W0 / 6 [fb][lane]
-// (k-slot kk holds Wh[kk] for kk < 5, Wh[kk - 5] for kk < 10, Wl[kk - 10] for kk < 15, 0)
+// (row n = 32 fb + lane % 32, k-slots F(kb, lane / 32, 0..7)) in FP16; then the output layer
+// [kb][lane] (rows n >= 8 zero) in FP16; then the input layer's split-bf16 fragments of
+// W0 / 6 [fb][lane] (k-slot kk holds Wh[kk] for kk < 5, Wh[kk - 5] for kk < 10, Wl[kk - 10]
+// for kk < 15, 0)
+
 __global__ void k_pack_fast(const float* P, uint16_t* w, float* bias) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     const int H = kFastH;
@@ -763,12 +771,12 @@ __global__ void k_pack_fast(const float* P, uint16_t* w, float* bias) {
         const int l = 1 + frag / kFastLayerFrags, r = frag % kFastLayerFrags;
         const int fb = r / (kFastK * 64), kb = (r / 64) % kFastK, lane = r % 64;
         const int f = packed_feature(kb, lane >> 5, s), n = 32 * fb + (lane & 31);
-        out = f2bf(P[ktn::param_offset(H, l) + (long long)f * H + n]);
+        out = f2h(P[ktn::param_offset(H, l) + (long long)f * H + n]);
     } else if (frag < 3 * kFastLayerFrags + kFastOutFrags) {
         const int r = frag - 3 * kFastLayerFrags;
         const int kb = r / 64, lane = r % 64, n = lane & 31;
         const int f = packed_feature(kb, lane >> 5, s);
-        out = n < ktn::kOutputs ? f2bf(P[ktn::param_offset(H, 4) + (long long)f * ktn::kOutputs + n]) : 0;
+        out = n < ktn::kOutputs ? f2h(P[ktn::param_offset(H, 4) + (long long)f * ktn::kOutputs + n]) : 0;
     } else {
         const int r = frag - 3 * kFastLayerFrags - kFastOutFrags;
         const int fb = r / 64, lane = r % 64, kk = 8 * (lane >> 5) + s, n = 32 * fb + (lane & 31);
@@ -787,11 +795,38 @@ __device__ __forceinline__ float hsilu_scaled(float zs) {
     return zs * __builtin_amdgcn_fmed3f(zs + 0.5f, 0.0f, 1.0f);
 }
 
-__device__ __forceinline__ void hsilu_pair(const f32x16& z, bf16x8& p0, bf16x8& p1) {
-#pragma unroll
-    for (int i = 0; i < 8; ++i) p0[i] = (__bf16)hsilu_scaled(z[i]);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) p1[i] = (__bf16)hsilu_scaled(z[8 + i]);
+// The fast network's activations are FP16 (its hidden and output layers multiply on
+// v_mfma_f32_32x32x16_f16): a pair of scaled pre-activations goes to packed halves in one
+// convert (round to nearest even), and the hard SiLU x clamp(x + 1/2, 0, 1) is then one
+// packed add with the clamp modifier and one packed multiply: three instructions for two
+// values, where the FP32 form takes five.  (Written as an instruction because the compiler
+// keeps a min/max pair on packed halves instead of the clamp bit.)
+typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 f16x8;
+typedef __attribute__((__vector_size__(2 * sizeof(_Float16)))) _Float16 f16x2;
+
+__device__ __forceinline__ unsigned hsilu2_h(float z0, float z1) {
+    const f16x2 x = {(_Float16)z0, (_Float16)z1};
+    f16x2 t;
+    asm("v_pk_add_f16 %0, %1, 0.5 op_sel_hi:[1,0] clamp" : "=v"(t) : "v"(x));
+    return __builtin_bit_cast(unsigned, x * t);
+}
+
+__device__ __forceinline__ void hsilu_pair(const f32x16& z, f16x8& p0, f16x8& p1) {
+    uint4 lo, hi;
+    lo.x = hsilu2_h(z[0], z[1]);
+    lo.y = hsilu2_h(z[2], z[3]);
+    lo.z = hsilu2_h(z[4], z[5]);
+    lo.w = hsilu2_h(z[6], z[7]);
+    hi.x = hsilu2_h(z[8], z[9]);
+    hi.y = hsilu2_h(z[10], z[11]);
+    hi.z = hsilu2_h(z[12], z[13]);
+    hi.w = hsilu2_h(z[14], z[15]);
+    p0 = __builtin_bit_cast(f16x8, lo);
+    p1 = __builtin_bit_cast(f16x8, hi);
+}
+
+__device__ __forceinline__ f32x16 mfma32h(const uint4& a, const f16x8& b, const f32x16& c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), b, c, 0, 0, 0);
 }
 
 // the input layer's B operand of the 32 rays of a group (lane m and m + 32 hold ray m):
@@ -821,7 +856,7 @@ __device__ __forceinline__ void fast_network2(const uint4* W, const float* Bs, i
                                               const float f1[5], float y0[4], float y1[4]) {
     const uint4* Win = W + 3 * kFastLayerFrags + kFastOutFrags;
     const uint4* Wout = W + 3 * kFastLayerFrags;
-    bf16x8 B0[kFastK], B1[kFastK];
+    f16x8 B0[kFastK], B1[kFastK];
     {
         const bf16x8 xb0 = input_split(f0, h), xb1 = input_split(f1, h);
 #pragma unroll
@@ -838,24 +873,28 @@ __device__ __forceinline__ void fast_network2(const uint4* W, const float* Bs, i
     for (int l = 1; l <= 3; ++l) {
         const uint4* Wl = W + (l - 1) * kFastLayerFrags;
         const float* bl = Bs + l * kFastH;
-        bf16x8 C0[kFastK], C1[kFastK];
+        f16x8 C0[kFastK], C1[kFastK];
+        f32x16 prev0, prev1;
+        // the previous output block's activation runs beside this block's MFMA chains
 #pragma unroll
         for (int fb = 0; fb < kFastBlocks; ++fb) {
             f32x16 acc0 = block_bias(bl, fb, h);
             f32x16 acc1 = acc0;
-            uint4 af[kFastK];
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int i = 0; i < kFastK; ++i) af[i] = Wl[(fb * kFastK + i) * 64 + lane];
-            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int i = 0; i < kFastK; ++i) {
-                acc0 = mfma32(af[i], B0[i], acc0);
-                acc1 = mfma32(af[i], B1[i], acc1);
+                const uint4 w = Wl[(fb * kFastK + i) * 64 + lane];
+                acc0 = mfma32h(w, B0[i], acc0);
+                acc1 = mfma32h(w, B1[i], acc1);
             }
-            hsilu_pair(acc0, C0[2 * fb], C0[2 * fb + 1]);
-            hsilu_pair(acc1, C1[2 * fb], C1[2 * fb + 1]);
+            if (fb > 0) {
+                hsilu_pair(prev0, C0[2 * fb - 2], C0[2 * fb - 1]);
+                hsilu_pair(prev1, C1[2 * fb - 2], C1[2 * fb - 1]);
+            }
+            prev0 = acc0;
+            prev1 = acc1;
         }
+        hsilu_pair(prev0, C0[kFastK - 2], C0[kFastK - 1]);
+        hsilu_pair(prev1, C1[kFastK - 2], C1[kFastK - 1]);
 #pragma unroll
         for (int k = 0; k < kFastK; ++k) {
             B0[k] = C0[k];
@@ -870,8 +909,8 @@ __device__ __forceinline__ void fast_network2(const uint4* W, const float* Bs, i
 #pragma unroll
     for (int kb = 0; kb < kFastK; ++kb) {
         const uint4 w = Wout[kb * 64 + lane];
-        o0[kb & 1] = mfma32(w, B0[kb], o0[kb & 1]);
-        o1[kb & 1] = mfma32(w, B1[kb], o1[kb & 1]);
+        o0[kb & 1] = mfma32h(w, B0[kb], o0[kb & 1]);
+        o1[kb & 1] = mfma32h(w, B1[kb], o1[kb & 1]);
     }
     const float* b4 = Bs + 4 * kFastH;
 #pragma unroll
@@ -881,70 +920,6 @@ __device__ __forceinline__ void fast_network2(const uint4* W, const float* Bs, i
     }
 }
 
-// the network for the 32 rays of a wave's group on features f (lane m and m + 32 hold ray
-// m): returns the four outputs 4 h .. 4 h + 3 of ray m in lane (m, h)
-__device__ __forceinline__ void fast_network(const uint4* W, const float* Bs, int lane, int h, const float f[5],
-                                             float y4[4]) {
-    const uint4* Win = W + 3 * kFastLayerFrags + kFastOutFrags;
-    const uint4* Wout = W + 3 * kFastLayerFrags;
-    // input layer: split-bf16 features as the B operand, one MFMA per output block
-    bf16x8 B[kFastK];
-    {
-        float fh[5], fl[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            fh[k] = (float)(__bf16)f[k];
-            fl[k] = f[k] - fh[k];
-        }
-        const float x0[8] = {fh[0], fh[1], fh[2], fh[3], fh[4], fl[0], fl[1], fl[2]};
-        const float x1[8] = {fl[3], fl[4], fh[0], fh[1], fh[2], fh[3], fh[4], 0.f};
-        bf16x8 xb;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) xb[k] = (__bf16)(h ? x1[k] : x0[k]);
-#pragma unroll
-        for (int fb = 0; fb < kFastBlocks; ++fb) {
-            const f32x16 z = mfma32(Win[fb * 64 + lane], xb, block_bias(Bs, fb, h));
-            hsilu_pair(z, B[2 * fb], B[2 * fb + 1]);
-        }
-    }
-    // hidden layers 1-3 in output-block order, the previous block's activation beside the
-    // current block's MFMA chain
-#pragma unroll
-    for (int l = 1; l <= 3; ++l) {
-        const uint4* Wl = W + (l - 1) * kFastLayerFrags;
-        const float* bl = Bs + l * kFastH;
-        bf16x8 C[kFastK];
-        f32x16 prev;
-#pragma unroll
-        for (int fb = 0; fb < kFastBlocks; ++fb) {
-            f32x16 acc = block_bias(bl, fb, h);
-            uint4 af[kFastK];
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int i = 0; i < kFastK; ++i) af[i] = Wl[(fb * kFastK + i) * 64 + lane];
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int i = 0; i < kFastK; ++i) acc = mfma32(af[i], B[i], acc);
-            if (fb > 0) hsilu_pair(prev, C[2 * fb - 2], C[2 * fb - 1]);
-            prev = acc;
-        }
-        hsilu_pair(prev, C[kFastK - 2], C[kFastK - 1]);
-#pragma unroll
-        for (int k = 0; k < kFastK; ++k) B[k] = C[k];
-    }
-    // output layer: two partial sums over the eight k-blocks, times 6 (the scaled
-    // activations), plus the bias
-    f32x16 o[2];
-#pragma unroll
-    for (int p = 0; p < 2; ++p)
-#pragma unroll
-        for (int i = 0; i < 16; ++i) o[p][i] = 0.f;
-#pragma unroll
-    for (int kb = 0; kb < kFastK; ++kb) o[kb & 1] = mfma32(Wout[kb * 64 + lane], B[kb], o[kb & 1]);
-    const float* b4 = Bs + 4 * kFastH;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) y4[r] = (o[0][r] + o[1][r]) * 6.0f + b4[4 * h + r];
-}
 
 // A wave takes 64 rays at a time, one per lane for everything but the network: lane l forms
 // the inputs of ray l and, after the network has run on rays 0-31 and then on rays 32-63
@@ -1706,7 +1681,7 @@ static hipError_t evaluate(Surrogate* s, Dataset& d, const TrainOpts& o, TrainRe
         const int c = e0 >= e1 && e0 >= e2 ? 0 : (e1 >= e2 ? 1 : 2);
         const double pm = std::max(e0, std::max(e1, e2)) / sum;
         agree += c == cls[q];
-        if (!(pm > 0.999) || !(ring[q] > 0.015)) continue;
+        if (!(pm > 0.999) || !(ring[q] > 0.01)) continue;
         ++gated;
         gated_agree += c == cls[q];
         if (c != cls[q]) continue;

@@ -123,7 +123,7 @@ typedef struct kt_trace_opts {
     int waves_per_cu;       /* persistent grid size (0 = tuned)                     */
     int steps_per_dequeue;  /* persistent refill period (0 = tuned)                 */
     double gate_prob;       /* surrogate: min class probability (0.999)             */
-    double gate_ring;       /* surrogate: min |rho/rho_ring - 1| (0.015)            */
+    double gate_ring;       /* surrogate: min |rho/rho_ring - 1| (0.01)             */
     int waves_per_simd;     /* register budget of the persistent kernel: 1-4 (0 = tuned) */
     /* kt_eht_render, kt_eht_trace_rays and kt_eht_calibrate (as eht::render's and
      * eht::calibrate's cancel, src/eht.h:140-153): a host flag read between slices of the

@@ -65,7 +65,7 @@ def test_defaults():
     o = K.KtTraceOpts()
     lib.kt_trace_opts_default(C.byref(o))
     assert (o.rtol, o.atol, o.max_steps, o.precision, o.mode) == (1e-10, 1e-12, 200000, K.PREC_FP64, K.MODE_AUTO)
-    assert (o.gate_prob, o.gate_ring, o.restep_drift) == (0.999, 0.015, 1e-6)
+    assert (o.gate_prob, o.gate_ring, o.restep_drift) == (0.999, 0.01, 1e-6)
     assert o.cancel is None  # not cancellable
     # the C struct is exactly the ctypes one: the default fills it and nothing past it
     n = C.sizeof(K.KtTraceOpts)

@@ -153,10 +153,15 @@ def test_adam_step_matches_numpy(tracer):
     np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-7)
 
 
+def f16(a):
+    return np.asarray(a, np.float64).astype(np.float16).astype(np.float64)
+
+
 def fast_forward(params, x):
-    """the fast kernel's arithmetic: activations kept divided by 6 (bf16(z' clamp(z' + 1/2,
-    0, 1)) with z' = z / 6), the input layer as split-bf16 products of W0 / 6, bf16 hidden
-    and output weights, the output times 6 plus b4"""
+    """the fast kernel's arithmetic: activations kept divided by 6 and computed on packed
+    halves (x = f16(z'), f16(x f16clamp(x + 1/2)) with z' = z / 6 in FP32), the input layer
+    as split-bf16 products of W0 / 6, FP16 hidden and output weights, the output times 6
+    plus b4"""
     (W0, b0), hidden, (W4, b4) = params[0], params[1:-1], params[-1]
     sixth = lambda v: (np.asarray(v, np.float32) * np.float32(1.0 / 6.0)).astype(np.float64)  # noqa: E731
     W0s = sixth(W0)
@@ -165,17 +170,22 @@ def fast_forward(params, x):
     xf = f32(x)
     fh = bf16(xf)
     fl = bf16(xf - fh)
-    act = lambda zs: bf16(zs * np.clip(zs + 0.5, 0.0, 1.0))  # noqa: E731
+
+    def act(zs):
+        xh = f16(f32(zs))
+        return f16(xh * np.clip(f16(xh + 0.5), 0.0, 1.0))
+
     h = act(fh @ Wh + fl @ Wh + fh @ Wl + sixth(b0))
     for W, b in hidden:
-        h = act(h @ bf16(W) + sixth(b))
-    return 6.0 * (h @ bf16(W4)) + f32(b4)
+        h = act(h @ f16(W) + sixth(b))
+    return 6.0 * (h @ f16(W4)) + f32(b4)
 
 
 def test_fast_network_forward_matches_numpy(tracer):
     """The 128-wide hard-SiLU network in the fast kernel: against the network itself (FP32
-    input layer, bf16 weights and activations) to the bf16 noise, and against the kernel's
-    own arithmetic (activations divided by 6, split-bf16 input layer) far tighter."""
+    input layer, bf16 weights and activations, as trained) to the bf16 noise, and against the
+    kernel's own arithmetic (activations divided by 6 on packed halves, FP16 hidden and
+    output weights, split-bf16 input layer) far tighter."""
     params, x, _, _ = problem(n=2048, sizes=M.FAST_SIZES)
     tracer.surrogate_load(flat(params))
     assert tracer.surrogate_net() == bsa.NET_FAST
