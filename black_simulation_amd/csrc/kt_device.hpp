@@ -51,6 +51,9 @@ struct ViewDev {
     long long offset;  // first ray of this view in the batch
     int kind, width, height, disk, prograde, n_params;
     int sur;  // 1: the view lies in the surrogate's domain (set only for surrogate calls)
+    // with sur: the view's network inputs 0-2, 1 / (1.5 R_sh) for inputs 3-4, and its
+    // inclination (float, from the host)
+    float sur_c[6];
     double a, r_cam, phi_cam;
     double phi0;  // phi at the camera as the integrator carries it: phi_cam - psi(r_cam) (outgoing
                   // Kerr-Schild), or phi_cam itself for observer cameras (ingoing)

@@ -411,7 +411,10 @@ int surrogate_covers(const kts::Domain& d, const ViewDev& v) {
 
 void mark_coverage(const kt_ctx* ctx, std::vector<ViewDev>& vd) {
     const kts::Domain d = kts::get_domain(ctx->surrogate);
-    for (ViewDev& v : vd) v.sur = surrogate_covers(d, v);
+    for (ViewDev& v : vd) {
+        v.sur = surrogate_covers(d, v);
+        if (v.sur) kts::view_constants(ctx->surrogate, v.a, v.sinI, v.cosI, v.r_cam, v.sur_c);
+    }
 }
 
 // the launch-shape and precision fields a caller may set; null when valid

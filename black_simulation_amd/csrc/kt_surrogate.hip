@@ -921,6 +921,54 @@ __device__ __forceinline__ void fast_network2(const uint4* W, const float* Bs, i
 }
 
 
+// The fast kernel's inputs of ray q, false when its view is not covered: the view's
+// constants come from the host (kts::view_constants); the critical-curve radius at the ray's
+// angle from `slice`, the table's psi row interpolated to the view's spin and inclination
+// (staged in LDS for a one-view launch), else from the table itself.
+__device__ __forceinline__ bool fast_features(const FusedArgs& a, const float* slice, long long q, float f[5],
+                                              float* rg) {
+    const ViewDev& v = a.views[find_view(a.views, a.n_views, q)];
+    if (v.kind != 0 || !v.sur) return false;
+    const long long local = q - v.offset;
+    float alpha, beta;
+    if (v.params) {
+        alpha = (float)v.params[2 * local];
+        beta = (float)v.params[2 * local + 1];
+    } else {
+        const unsigned lu = (unsigned)local, w = (unsigned)v.width;
+        const int row = (int)(lu / w), col = (int)(lu - (unsigned)row * w);
+        alpha = (float)(-v.half_width + ((double)col + 0.5) * v.px);
+        beta = (float)(v.half_height - ((double)row + 0.5) * v.py);
+    }
+    f[0] = v.sur_c[0];
+    f[1] = v.sur_c[1];
+    f[2] = v.sur_c[2];
+    f[3] = alpha * v.sur_c[3];
+    f[4] = beta * v.sur_c[3];
+    const float psi = atan2f(fabsf(beta), alpha);
+    float rc;
+    if (slice) {
+        int ip;
+        float fp;
+        cell_f(psi, 0.f, (float)kPi, kTabPsi, ip, fp);
+        rc = (1 - fp) * slice[ip] + fp * slice[ip + 1];
+    } else {
+        rc = table_rho_f(a.t.rho, (float)v.a, v.sur_c[4], psi);
+    }
+    *rg = fabsf(sqrtf(alpha * alpha + beta * beta) * __builtin_amdgcn_rcpf(rc) - 1.0f);
+    return true;
+}
+
+// the psi row of the critical-curve table at spin a and inclination inc (bilinear in them)
+__device__ __forceinline__ float rho_row(const float* rho, float a, float inc, int k) {
+    int ia, ii;
+    float fa, fi;
+    cell_f(a, (float)kTabA0, (float)kTabA1, kTabA, ia, fa);
+    cell_f(inc, (float)kTabI0, (float)kTabI1, kTabI, ii, fi);
+    auto at = [&](int x, int y) { return rho[(x * kTabI + y) * kTabPsi + k]; };
+    return (1 - fa) * ((1 - fi) * at(ia, ii) + fi * at(ia, ii + 1)) + fa * ((1 - fi) * at(ia + 1, ii) + fi * at(ia + 1, ii + 1));
+}
+
 // A wave takes 64 rays at a time, one per lane for everything but the network: lane l forms
 // the inputs of ray l and, after the network has run on rays 0-31 and then on rays 32-63
 // (each with the 32x32 layout, in which lanes m and m + 32 share ray m), gates and writes
@@ -931,10 +979,15 @@ template <bool VIEWS>
 __global__ __launch_bounds__(kFastThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mlp_fast(FusedArgs a) {
     __shared__ uint4 W[kFastFrags];
     __shared__ __attribute__((aligned(16))) float Bs[kFastBias];
+    __shared__ float slice[kTabPsi];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m = lane & 31, h = lane >> 5;
     for (int i = tid; i < kFastFrags; i += kFastThreads) W[i] = a.wfast[i];
     for (int i = tid; i < kFastBias; i += kFastThreads) Bs[i] = a.bfast[i];
+    const bool one_view = VIEWS && a.n_views == 1 && a.views[0].sur;
+    if (one_view)
+        for (int k = tid; k < kTabPsi; k += kFastThreads)
+            slice[k] = rho_row(a.t.rho, (float)a.views[0].a, a.views[0].sur_c[4], k);
     __syncthreads();
     constexpr int kPair = 2 * kFastTile;  // rays per block and iteration: 8 waves x 64
     const long long tiles = (a.n + kPair - 1) / kPair;
@@ -946,11 +999,8 @@ __global__ __launch_bounds__(kFastThreads) __attribute__((amdgpu_waves_per_eu(2,
         float f[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
         float rg = -1.f;
         if (VIEWS) {
-            RayF p;
-            if (ray_params_f<false>(a.views, a.n_views, qc, p)) {
-                features_f(a.t.rho, a.t.rsh, p, f, &rg);
+            if (fast_features(a, one_view ? slice : nullptr, qc, f, &rg))
                 if (!(fabsf(f[3]) <= a.extent) || !(fabsf(f[4]) <= a.extent)) rg = -1.f;
-            }
         } else {
 #pragma unroll
             for (int k = 0; k < 5; ++k) f[k] = a.xf[qc * 5 + k];
@@ -1201,6 +1251,17 @@ void destroy(Surrogate* s) {
 }
 
 bool ready(const Surrogate* s) { return s && s->have_params; }
+
+void view_constants(const Surrogate* s, double a, double sin_i, double cos_i, double r_cam, float c[6]) {
+    const float af = (float)a, ci = (float)cos_i, inc = atan2f((float)sin_i, ci);
+    const float R = table_rsh(s->h_rsh.data(), (double)af, (double)inc);
+    c[0] = log10f((float)r_cam) - 2.0f;
+    c[1] = (ci - 0.5415f) * (1.0f / 0.4545f);
+    c[2] = (af - 0.499f) * (1.0f / 0.499f);
+    c[3] = 1.0f / (1.5f * R);
+    c[4] = inc;
+    c[5] = 0.0f;
+}
 
 Domain get_domain(const Surrogate* s) { return s ? s->dom : Domain(); }
 

@@ -90,6 +90,9 @@ hipError_t train(Surrogate* s, const TrainOpts& o, TrainReport& rep);
 
 // the network's domain: set by train() from its options, else the default training's
 Domain get_domain(const Surrogate* s);
+// a covered view's per-view network inputs (the float forms the kernels use): c[0..2] the
+// inputs from r_cam, inclination and spin, c[3] = 1 / (1.5 R_sh(a, inc)), c[4] = inc
+void view_constants(const Surrogate* s, double a, double sin_i, double cos_i, double r_cam, float c[6]);
 void set_domain(Surrogate* s, const Domain& d);
 
 // The fallback queue the gate fills: ray indices and their count (device).  Calls that
