@@ -31,12 +31,16 @@ struct TraceLaunch {
     int mode;                   // KT_MODE_*
     int block, grid, steps_per_dequeue, waves_per_simd;
     int ks;                     // 1: observer cameras (FP64, Kerr-Schild phi)
+    int no_reset;               // 1: the caller has zeroed the queue head and counters (reset_counters)
 };
 
 constexpr int kQueueBuckets = 3;
 
-// launches the counter reset and the trace kernel on `stream`; returns a hipError_t
+// launches the counter reset (unless L.no_reset) and the trace kernel on `stream`; returns a
+// hipError_t
 hipError_t launch_trace(const TraceLaunch& L, hipStream_t stream);
+// the counter reset alone (for a caller that launches other work between it and the trace)
+hipError_t reset_counters(const TraceLaunch& L, hipStream_t stream);
 
 // The fixed device scratch of the no-allocation entry point, on one GPU: the counters
 // (queue head + kStatCount) and a view table of max_views views.

@@ -1903,8 +1903,11 @@ hipError_t trace_with_surrogate(Surrogate* s, const kti::TraceLaunch& L, double 
         KTS_TRY(queue_reserve(Q, n));
     }
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, Q.count, 4);
+    KTS_TRY(kti::reset_counters(L, st));
     // the network and the gate in one kernel: answered rays get their outputs, the
-    // others join the fallback queue
+    // others join the fallback queue, whose length the integrator reads on the device:
+    // the two kernels run back to back, bracketed by the events the stats read
+    if (ev) KTS_TRY(hipEventRecord(ev[0], st));
     FusedArgs a = fused_args(s, n);
     a.views = L.views;
     a.n_views = L.n_views;
@@ -1917,19 +1920,19 @@ hipError_t trace_with_surrogate(Surrogate* s, const kti::TraceLaunch& L, double 
     a.fallback_stride = Q.cap;
     a.next_tile = Q.count + kti::kQueueBuckets;  // zeroed above with the fallback counts
     launch_network(s, a, true, st);
-    hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(64), 0, st, Q.count, Q.total, (unsigned long long*)nullptr);
     KTS_TRY(hipGetLastError());
-    // the integrator on the queue the gate left (count read on the device).  The one-shot
-    // form's grid covers the whole batch and the blocks past the queue's end leave at once;
-    // the rays queued are the slowest of the frame (photon ring, class boundaries), so the
-    // launch is bound by their latency either way, and the persistent form measured slower
-    // on the 1080p orbit (0.42 against 0.35 ms per frame: its periodic refill costs and
-    // the few rays leave nothing to refill)
+    // the integrator on the queue the gate left, its regions read in order and its length
+    // read on the device (the counters were zeroed before the network).  The rays queued
+    // are the slowest of the frame (photon ring, class boundaries), so the launch is bound
+    // by their latency: the one-shot form's grid is capped near the usual queue length and
+    // strides over a longer one; the persistent form measured slower on the 1080p orbit
+    // (its periodic refill costs, and the few rays leave little to refill)
     kti::TraceLaunch F = L;
     F.ray_list = Q.list;
-    F.total_dev = Q.total;
+    F.total_dev = nullptr;
     F.bucket_count = Q.count;
     F.bucket_stride = Q.cap;
+    F.no_reset = 1;
     if (ev) KTS_TRY(hipEventRecord(ev[1], st));
     KTS_TRY(kti::launch_trace(F, st));
     if (ev) KTS_TRY(hipEventRecord(ev[2], st));

@@ -55,15 +55,22 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
 
 // the queue slot of a launch's q-th ray: regions of a bucketed queue are read in order
 struct QueueMap {
-    long long c0, c1, stride;
+    long long c0, c1, c2, stride;
     bool bucketed;
-    __device__ __forceinline__ explicit QueueMap(const kti::TraceLaunch& L) : c0(0), c1(0), stride(0), bucketed(false) {
+    __device__ __forceinline__ explicit QueueMap(const kti::TraceLaunch& L)
+        : c0(0), c1(0), c2(0), stride(0), bucketed(false) {
         if (L.bucket_count) {
             bucketed = true;
             c0 = (long long)L.bucket_count[0];
             c1 = (long long)L.bucket_count[1];
+            c2 = (long long)L.bucket_count[2];
             stride = L.bucket_stride;
         }
+    }
+    // the launch's ray count: the queue's length, else the device count, else the host's
+    __device__ __forceinline__ long long total(const kti::TraceLaunch& L) const {
+        if (bucketed) return c0 + c1 + c2;
+        return L.total_dev ? *L.total_dev : L.total;
     }
     __device__ __forceinline__ long long slot(long long q) const {
         if (!bucketed || q < c0) return q;
@@ -147,8 +154,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
     const int K = L.steps_per_dequeue;
 
     // a device-side count (the surrogate's fallback queue) overrides the launch size
-    const long long total = L.total_dev ? *L.total_dev : L.total;
     const QueueMap qm(L);
+    const long long total = qm.total(L);
     typename Pol::LaneT s;
     bool active = false;
     bool drained = false;
@@ -204,10 +211,10 @@ __global__ __launch_bounds__(256) void k_trace_oneshot(kti::TraceLaunch L) {
     // issue priority over other kernels' waves on the same SIMD: a fallback queue's time is
     // the latency of its longest rays, while the surrogate's network beside it (another
     // frame, another stream) is throughput work that fills the issue slots left over
-    if (L.total_dev) __builtin_amdgcn_s_setprio(3);
+    if (L.total_dev || L.bucket_count) __builtin_amdgcn_s_setprio(3);
     const ViewDev* views = stage_views(L, view_lds);
-    const long long total = L.total_dev ? *L.total_dev : L.total;
     const QueueMap qm(L);
+    const long long total = qm.total(L);
     const long long stride = (long long)gridDim.x * blockDim.x;
     unsigned long long st = 0, s64 = 0, nre = 0;
     for (long long base = (long long)blockIdx.x * blockDim.x; base < total; base += stride) {  // block-uniform
@@ -329,13 +336,18 @@ void launch_oneshot(const TraceLaunch& L, hipStream_t stream) {
     long long blocks = (L.total + L.block - 1) / L.block;
     // a queue counted on the device: twice the persistent grid's lanes cover the usual
     // share of rejected rays in one pass (the kernel strides over a longer queue)
-    if (L.total_dev) blocks = std::min<long long>(blocks, 2LL * std::max(1, L.grid));
+    if (L.total_dev || L.bucket_count) blocks = std::min<long long>(blocks, 2LL * std::max(1, L.grid));
     const dim3 grid((unsigned)std::max(1LL, blocks)), block(L.block);
     k_trace_oneshot<Pol><<<grid, block, view_lds_bytes(L.n_views), stream>>>(L);
 }
 
-hipError_t launch_trace(const TraceLaunch& L, hipStream_t stream) {
+hipError_t reset_counters(const TraceLaunch& L, hipStream_t stream) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, L.counter, L.stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace(const TraceLaunch& L, hipStream_t stream) {
+    if (!L.no_reset) hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, L.counter, L.stats);
     if (L.total <= 0) return hipGetLastError();
     if (L.out.turns || L.out.r_min || L.out.delay || L.out.b || L.out.q) {
         const dim3 grid((unsigned)((L.total + L.block - 1) / L.block)), block(L.block);
