@@ -48,7 +48,9 @@ OPS_PER_PARTICLE_ATTEMPT = 656
 # (the factors a 1 GiB device copy reads) from rocprofv3 --pmc runs of the same command;
 # the summaries (commands included) are the named files under profiles/
 PROFILED_TRAFFIC = {
-    ("orbit", 1080, "k_mlp_fused"): (3.983e6 + 73.47e6, "profiles/k_mlp_fused_orbit_1080p.md"),
+    ("orbit/surrogate", 1080, "k_mlp_fast"): (1.220e6 + 75.30e6, "profiles/k_mlp_fast_orbit_1080p.md"),
+    ("orbit/surrogate", 1080, "k_trace_oneshot"): (1.203e6 + 3.885e6,
+                                                   "profiles/k_trace_oneshot_fallback_orbit_1080p.md"),
     ("infall", 1080, "k_trace_persistent"): (0.374e6 + 149.1e6, "profiles/k_trace_persistent_plainks_infall_1080p.md"),
     ("eht", 1024, "k_eht"): (2.468e6 + 12.13e6, "profiles/k_eht_m87_1024.md"),
     ("debris", 1 << 22, "k_advance"): (339.2e6 + 899.9e6, "profiles/k_advance_debris_4m.md"),
@@ -452,63 +454,6 @@ def main():
         wall_p, lat_p = wall, lat
     value_p = rays * wall_frames / wall_p / 1e6
 
-    # the same frames through host buffers (the library copies the outputs back over PCIe)
-    # through page-locked buffers allocated once (host_outputs), and through fresh pageable
-    # numpy arrays per frame
-    host_value = host_value_pageable = host_value_f32 = None
-    if args.host_outputs:
-        k0 = frames[len(frames) // 2]
-        hb = tr.host_outputs(rays, pinned=True)
-        tr.trace_views_host(frame_views[k0], opts, out=hb)
-        t0 = time.perf_counter()
-        for k in frames:
-            tr.trace_views_host(frame_views[k], opts, out=hb)
-        host_value = rays * len(frames) / (time.perf_counter() - t0) / 1e6
-        hb = tr.host_outputs(rays, pinned=True, f32=True)  # 21 B/ray: float angles, radius and g
-        tr.trace_views_host(frame_views[k0], opts, out=hb)
-        t0 = time.perf_counter()
-        for k in frames:
-            tr.trace_views_host(frame_views[k], opts, out=hb)
-        host_value_f32 = rays * len(frames) / (time.perf_counter() - t0) / 1e6
-        tr.trace_views_host(frame_views[k0], opts)
-        t0 = time.perf_counter()
-        for k in frames[: max(1, len(frames) // 4)]:
-            tr.trace_views_host(frame_views[k], opts)
-        host_value_pageable = rays * max(1, len(frames) // 4) / (time.perf_counter() - t0) / 1e6
-
-    # per-kernel attribution (untimed pass without graphs: events around each kernel group)
-    integ_ms, sur_ms, attempts, fp64_att, fallback = [], [], [], [], []
-    for k in frames[:: max(1, len(frames) // 24)]:
-        tr.trace_views(frame_views[k], opts, out=orb.out)
-        st = tr.stats()
-        integ_ms.append(st["integrator_ms"])
-        sur_ms.append(st["surrogate_ms"])
-        attempts.append(st["steps"])
-        fp64_att.append(st["fp64_steps"] if opts.precision != bsa.PREC_FP64 else st["steps"])
-        fallback.append(st["fallback_rays"])
-    n_attr = len(integ_ms)
-
-    # HDR colour of the last attribution frame (an HBM-bound pass: per ray it reads the
-    # class byte and four doubles and writes four halves)
-    k_last = frames[:: max(1, len(frames) // 24)][-1]
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    tr.shade(frame_views[k_last], orb.out)
-    ev0.record()
-    for _ in range(10):
-        tr.shade(frame_views[k_last], orb.out)
-    ev1.record()
-    torch.cuda.synchronize()
-    shade_ms = ev0.elapsed_time(ev1) / 10
-    # algorithmic bytes: every ray reads its class byte and writes 8 B; escaped rays read
-    # (theta, phi) and disk rays (r_hit, g), 16 B each
-    cls = orb.out["cls"] & 7
-    n_esc, n_disk = int((cls == bsa.ESCAPED).sum().item()), int((cls == bsa.DISK).sum().item())
-    shade_bytes = rays * (1 + 8) + 16 * (n_esc + n_disk)
-    shade = {"kernel": "k_shade", "kernel_ms": round(shade_ms, 4), "bound": "hbm",
-             "achieved": round(shade_bytes / (shade_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-             "frac": round(shade_bytes / (shade_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-             "bytes_per_ray": round(shade_bytes / rays, 2)}
-
     # accuracy of the chosen path against the FP64 render of every timed frame
     acc = None
     if ref is not orb:
@@ -557,6 +502,64 @@ def main():
             acc["target_met"] = bool(value >= 20 * value_p and d50 <= 1e-4 and d999 <= 1e-3
                                      and acc["class_agreement"] >= 0.999)
 
+    # the same frames through host buffers (the library copies the outputs back over PCIe)
+    # through page-locked buffers allocated once (host_outputs), and through fresh pageable
+    # numpy arrays per frame
+    host_value = host_value_pageable = host_value_f32 = None
+    if args.host_outputs:
+        k0 = frames[len(frames) // 2]
+        hb = tr.host_outputs(rays, pinned=True)
+        tr.trace_views_host(frame_views[k0], opts, out=hb)
+        t0 = time.perf_counter()
+        for k in frames:
+            tr.trace_views_host(frame_views[k], opts, out=hb)
+        host_value = rays * len(frames) / (time.perf_counter() - t0) / 1e6
+        hb = tr.host_outputs(rays, pinned=True, f32=True)  # 21 B/ray: float angles, radius and g
+        tr.trace_views_host(frame_views[k0], opts, out=hb)
+        t0 = time.perf_counter()
+        for k in frames:
+            tr.trace_views_host(frame_views[k], opts, out=hb)
+        host_value_f32 = rays * len(frames) / (time.perf_counter() - t0) / 1e6
+        tr.trace_views_host(frame_views[k0], opts)
+        t0 = time.perf_counter()
+        for k in frames[: max(1, len(frames) // 4)]:
+            tr.trace_views_host(frame_views[k], opts)
+        host_value_pageable = rays * max(1, len(frames) // 4) / (time.perf_counter() - t0) / 1e6
+
+    # per-kernel attribution (untimed pass without graphs: events around each kernel group;
+    # the last GPU work of the path, so a kernel trace of this run ends with its dispatches)
+    integ_ms, sur_ms, attempts, fp64_att, fallback = [], [], [], [], []
+    for k in frames[:: max(1, len(frames) // 24)]:
+        tr.trace_views(frame_views[k], opts, out=orb.out)
+        st = tr.stats()
+        integ_ms.append(st["integrator_ms"])
+        sur_ms.append(st["surrogate_ms"])
+        attempts.append(st["steps"])
+        fp64_att.append(st["fp64_steps"] if opts.precision != bsa.PREC_FP64 else st["steps"])
+        fallback.append(st["fallback_rays"])
+    n_attr = len(integ_ms)
+
+    # HDR colour of the last attribution frame (an HBM-bound pass: per ray it reads the
+    # class byte and four doubles and writes four halves)
+    k_last = frames[:: max(1, len(frames) // 24)][-1]
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    tr.shade(frame_views[k_last], orb.out)
+    ev0.record()
+    for _ in range(10):
+        tr.shade(frame_views[k_last], orb.out)
+    ev1.record()
+    torch.cuda.synchronize()
+    shade_ms = ev0.elapsed_time(ev1) / 10
+    # algorithmic bytes: every ray reads its class byte and writes 8 B; escaped rays read
+    # (theta, phi) and disk rays (r_hit, g), 16 B each
+    cls = orb.out["cls"] & 7
+    n_esc, n_disk = int((cls == bsa.ESCAPED).sum().item()), int((cls == bsa.DISK).sum().item())
+    shade_bytes = rays * (1 + 8) + 16 * (n_esc + n_disk)
+    shade = {"kernel": "k_shade", "kernel_ms": round(shade_ms, 4), "bound": "hbm",
+             "achieved": round(shade_bytes / (shade_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(shade_bytes / (shade_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+             "bytes_per_ray": round(shade_bytes / rays, 2)}
+
     # roofline of the dominant kernel group (HIP-event times of the attribution pass)
     avg_integ = sum(integ_ms) / n_attr
     avg_sur = sum(sur_ms) / n_attr
@@ -568,7 +571,8 @@ def main():
     integ_peak = ops / ideal_s / 1e12 if ideal_s > 0 else FP64_VECTOR_PEAK_TFLOPS
     integ_ach = ops / (avg_integ * 1e-3) / 1e12 if avg_integ > 0 else 0.0
     kname = ["k_trace_persistent", "k_trace_oneshot"][st["mode"]]
-    tr_integ, src_integ = profiled_traffic(args.workload, args.height, kname)
+    wkey = f"{args.workload}/surrogate" if args.path == "surrogate" else args.workload
+    tr_integ, src_integ = profiled_traffic(wkey, args.height, kname)
     roof_integ = {"bound": "valu", "achieved": round(integ_ach, 3), "peak": round(integ_peak, 1), "unit": "TFLOP/s",
                   "frac": round(integ_ach / integ_peak, 4), "traffic": tr_integ, "traffic_source": src_integ,
                   "kernel": kname,
@@ -584,7 +588,7 @@ def main():
         mlp_ach = rays * flop_ray / (avg_sur * 1e-3) / 1e12
         kmlp = "k_mlp_fast" if hidden == 128 else "k_mlp_fused"
         net_desc = f"5-{hidden}x4-8 " + ("hard SiLU" if hidden == 128 else "SiLU")
-        tr_mlp, src_mlp = profiled_traffic(args.workload, args.height, kmlp)
+        tr_mlp, src_mlp = profiled_traffic(wkey, args.height, kmlp)
         roof_mlp = {"bound": "mfma", "achieved": round(mlp_ach, 2), "peak": BF16_MFMA_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(mlp_ach / BF16_MFMA_PEAK_TFLOPS, 4), "traffic": tr_mlp,
                     "traffic_source": src_mlp,
