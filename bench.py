@@ -63,6 +63,10 @@ def profiled_traffic(workload, size, kernel):
         if w == workload and n == size and kernel.startswith(k):
             return v
     return None, None
+# FP64 operations per ray of the closed-form solution (kt_closed.hpp) over the rays of the
+# 1080p orbit frames, counted by the closed form's Python restatement (tests/closed_form.py,
+# `ops`: + - * / sqrt once, FMA twice, each iteration of the special functions as run)
+CLOSED_FORM_OPS_PER_RAY = 281
 # The network 5-256-256-256-256-8: 2 (5*256 + 3*256*256 + 256*8) multiply-adds per ray
 def mlp_flop_per_ray(hidden: int) -> int:
     """multiply-adds x 2 of the network 5 -> H -> H -> H -> H -> 8: 399,872 at H = 256 (the
@@ -332,7 +336,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=120)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--path", choices=["surrogate", "plain", "mixed"], default="surrogate")
+    ap.add_argument("--path", choices=["closed", "surrogate", "plain", "mixed"], default="closed")
+    ap.add_argument("--closed-form", type=int, choices=[0, 1], default=1,
+                    help="--path surrogate: 1 = the closed form verifies the network's escaped and disk answers "
+                         "and gives their values; 0 = the network's own values")
+    ap.add_argument("--no-alt", action="store_true",
+                    help="--path closed: do not time the learned surrogate's paths beside it")
     ap.add_argument("--workload", choices=["orbit", "c2", "c3", "eht", "infall", "debris"], default="orbit")
     ap.add_argument("--particles", type=int, default=1 << 22, help="--workload debris: particles")
     ap.add_argument("--size", type=int, default=1024, help="--workload eht: pixels across")
@@ -420,25 +429,44 @@ def main():
     tune = (lambda o: o) if args.lean else (lambda o: tr.autotune(frame_views[frames[len(frames) // 2]], o))
     mode = {"auto": bsa.MODE_AUTO, "oneshot": bsa.MODE_ONESHOT, "persistent": bsa.MODE_PERSISTENT}[args.mode]
     plain = tune(bsa.TraceOpts(precision=bsa.PREC_FP64, mode=mode))
-    if args.path == "plain":
+    def surrogate_opts(closed_form):
+        rtol = args.rtol or 1e-6
+        if args.fallback == "fp64":
+            o = tune(bsa.TraceOpts(precision=bsa.PREC_FP64, mode=mode, rtol=rtol, atol=rtol / 10))
+        else:
+            o = tune(bsa.TraceOpts(precision=bsa.PREC_MIXED, mode=mode,
+                                   **({"rtol32": args.rtol, "atol32": args.rtol / 10} if args.rtol else {})))
+        o.use_surrogate = 1
+        o.closed_form = closed_form
+        if args.gate_ring:
+            o.gate_ring = args.gate_ring
+        if args.gate_prob:
+            o.gate_prob = args.gate_prob
+        return o
+
+    alt = []  # (name, opts, answered flag): the learned surrogate's paths beside --path closed
+    if args.path == "closed":
+        # the closed form answers every ray it covers; the FP64 integrator (rtol 1e-10, or
+        # --rtol) the rays it declines
+        opts = tune(bsa.TraceOpts(precision=bsa.PREC_FP64, mode=mode,
+                                  **({"rtol": args.rtol, "atol": args.rtol / 10} if args.rtol else {})))
+        opts.closed_form = 1
+        if args.workload == "orbit" and not args.no_alt and not args.lean:
+            load_into(tr, args.weights or FAST_WEIGHTS)
+            alt = [("learned_surrogate_closed_form_values", surrogate_opts(1), bsa.FLAG_SURROGATE),
+                   ("learned_surrogate_network_values", surrogate_opts(0), bsa.FLAG_SURROGATE)]
+    elif args.path == "plain":
         opts = plain
         if args.rtol:  # FP64 at another tolerance, judged against FP64 at rtol 1e-10
             opts = tune(bsa.TraceOpts(precision=bsa.PREC_FP64, mode=mode, rtol=args.rtol, atol=args.rtol / 10))
+    elif args.path == "surrogate":
+        load_into(tr, args.weights or FAST_WEIGHTS)
+        opts = surrogate_opts(args.closed_form)
     else:
-        if args.path == "surrogate" and args.fallback == "fp64":
-            rtol = args.rtol or 1e-6
-            opts = tune(bsa.TraceOpts(precision=bsa.PREC_FP64, mode=mode, rtol=rtol, atol=rtol / 10))
-        else:
-            opts = tune(bsa.TraceOpts(precision=bsa.PREC_MIXED, mode=mode,
-                                      **({"rtol32": args.rtol, "atol32": args.rtol / 10} if args.rtol else {})))
-        if args.path == "surrogate":
-            load_into(tr, args.weights or FAST_WEIGHTS)
-            opts.use_surrogate = 1
-            if args.gate_ring:
-                opts.gate_ring = args.gate_ring
-            if args.gate_prob:
-                opts.gate_prob = args.gate_prob
-    tr_net = tr.surrogate_net() if args.path == "surrogate" else None
+        opts = tune(bsa.TraceOpts(precision=bsa.PREC_MIXED, mode=mode,
+                                  **({"rtol32": args.rtol, "atol32": args.rtol / 10} if args.rtol else {})))
+    tr_net = tr.surrogate_net() if (args.path == "surrogate" or alt) else None
+    answered_flag = bsa.FLAG_CLOSED_FORM if args.path == "closed" else bsa.FLAG_SURROGATE
 
     # the chosen path, timed
     orb = O.Orbit(tr, views, opts, streams=args.streams)
@@ -453,54 +481,75 @@ def main():
     else:
         wall_p, lat_p = wall, lat
     value_p = rays * wall_frames / wall_p / 1e6
+    # the learned surrogate's paths beside the closed form, timed the same way
+    alt_runs = []
+    for name, aopts, aflag in alt:
+        aorb = O.Orbit(tr, views, aopts, streams=args.streams)
+        awall, alat = timed_frames(aorb, warm, frames, torch)
+        alt_runs.append((name, aopts, aflag, aorb, rays * wall_frames / awall / 1e6, alat))
 
-    # accuracy of the chosen path against the FP64 render of every timed frame
+    # accuracy of each path against the FP64 render of every timed frame
     acc = None
+    alt_acc = {}
     if ref is not orb:
-        agree, n_rays, answered = 0, 0, 0
-        counts = {}
-        defl, defl_a, g_rel, g_rel_a, r_rel, r_rel_a = [], [], [], [], [], []
+        paths = [("main", orb, answered_flag)] + [(r[0], r[3], r[2]) for r in alt_runs]
+        tally = {name: {"agree": 0, "answered": 0, "n": 0, "counts": {}, "defl": [], "defl_a": [], "g": [],
+                        "g_a": [], "r": [], "r_a": []} for name, _, _ in paths}
         for k in sorted(set(frames)):
-            mine = orb.render(k)
             theirs = ref.render(k)
-            orb.synchronize()
             ref.synchronize()
-            e = O.errors(mine, theirs)
-            agree += int(e["agree"].sum())
-            answered += int(e["answered"].sum())
-            n_rays += rays
-            for c, d in e["counts"].items():
-                t = counts.setdefault(c, {"rays": 0, "answered": 0, "answered_agree": 0})
-                for f in t:
-                    t[f] += d[f]
-            defl.append(e["defl"])
-            defl_a.append(e["defl_answered"])
-            g_rel.append(e["g_rel"])
-            g_rel_a.append(e["g_rel_answered"])
-            r_rel.append(e["r_rel"])
-            r_rel_a.append(e["r_rel_answered"])
+            for name, porb, pflag in paths:
+                mine = porb.render(k)
+                porb.synchronize()
+                e = O.errors(mine, theirs, pflag)
+                t = tally[name]
+                t["agree"] += int(e["agree"].sum())
+                t["answered"] += int(e["answered"].sum())
+                t["n"] += rays
+                for c, d in e["counts"].items():
+                    tc = t["counts"].setdefault(c, {"rays": 0, "answered": 0, "answered_agree": 0})
+                    for f in tc:
+                        tc[f] += d[f]
+                for key, src in (("defl", "defl"), ("defl_a", "defl_answered"), ("g", "g_rel"),
+                                 ("g_a", "g_rel_answered"), ("r", "r_rel"), ("r_a", "r_rel_answered")):
+                    t[key].append(e[src])
         q = lambda xs: [float("%.3g" % v) for v in O.quantiles(torch.cat(xs))]  # noqa: E731
-        n_answered_esc = counts["escaped"]["answered"]
-        acc = {"frames": len(set(frames)), "class_agreement": round(agree / n_rays, 6),
-               "answered_fraction": round(answered / n_rays, 4),
-               # the frames' class mix (FP64 classes) and, per class, how many rays the network
-               # answered and how many of those it gave the right class
-               "rays_by_class": counts,
-               "answered_class_agreement": round(sum(c["answered_agree"] for c in counts.values())
-                                                 / max(1, answered), 6),
-               "deflection_rad_p50_p99_p999": q(defl), "deflection_rad_answered_p50_p99_p999": q(defl_a),
-               "g_rel_p50_p99_p999": q(g_rel), "g_rel_answered_p50_p99_p999": q(g_rel_a),
-               "r_hit_rel_p50_p99_p999": q(r_rel), "r_hit_rel_answered_p50_p99_p999": q(r_rel_a)}
-        # the target judges what the surrogate itself answered: its escaped rays' deflection
-        # (none answered: not met), all rays' classes, and the speed against plain FP64
-        if args.path == "surrogate":
-            d50, _, d999 = acc["deflection_rad_answered_p50_p99_p999"]
-            acc["target_met"] = bool(n_answered_esc > 0 and value >= 20 * value_p and d50 <= 1e-4
-                                     and d999 <= 1e-3 and acc["class_agreement"] >= 0.999)
-        else:
-            d50, _, d999 = acc["deflection_rad_p50_p99_p999"]
-            acc["target_met"] = bool(value >= 20 * value_p and d50 <= 1e-4 and d999 <= 1e-3
-                                     and acc["class_agreement"] >= 0.999)
+
+        def summary(name, v):
+            t = tally[name]
+            counts = t["counts"]
+            a = {"frames": len(set(frames)), "class_agreement": round(t["agree"] / t["n"], 6),
+                 "answered_fraction": round(t["answered"] / t["n"], 4),
+                 # the frames' class mix (FP64 classes) and, per class, how many rays the path
+                 # answered without the integrator and how many of those got the right class
+                 "rays_by_class": counts,
+                 "answered_class_agreement": round(sum(c["answered_agree"] for c in counts.values())
+                                                   / max(1, t["answered"]), 6),
+                 "deflection_rad_p50_p99_p999": q(t["defl"]), "deflection_rad_answered_p50_p99_p999": q(t["defl_a"]),
+                 "g_rel_p50_p99_p999": q(t["g"]), "g_rel_answered_p50_p99_p999": q(t["g_a"]),
+                 "r_hit_rel_p50_p99_p999": q(t["r"]), "r_hit_rel_answered_p50_p99_p999": q(t["r_a"])}
+            # the target judges what the path answered without the integrator: its escaped
+            # rays' deflection (none answered: not met), every ray's class, and the speed
+            # against plain FP64; the integrator paths are judged on all their rays
+            if name != "main" or args.path in ("closed", "surrogate"):
+                d50, _, d999 = a["deflection_rad_answered_p50_p99_p999"]
+                a["target_met"] = bool(counts["escaped"]["answered"] > 0 and v >= 20 * value_p and d50 <= 1e-4
+                                       and d999 <= 1e-3 and a["class_agreement"] >= 0.999)
+            else:
+                d50, _, d999 = a["deflection_rad_p50_p99_p999"]
+                a["target_met"] = bool(v >= 20 * value_p and d50 <= 1e-4 and d999 <= 1e-3
+                                       and a["class_agreement"] >= 0.999)
+            return a
+
+        acc = summary("main", value)
+        for name, aopts, aflag, aorb, avalue, alat in alt_runs:
+            alt_acc[name] = {"value": round(avalue, 3), "speedup_vs_plain_fp64": round(avalue / value_p, 3),
+                             "frame_ms_p50": round(percentile(alat, 0.5), 3),
+                             "gate": {"prob": aopts.gate_prob, "ring": aopts.gate_ring,
+                                      "escaped_and_disk_verified_by_closed_form": bool(aopts.closed_form)},
+                             "accuracy_vs_fp64": summary(name, avalue)}
+    for r in alt_runs:
+        r[3].close()
 
     # the same frames through host buffers (the library copies the outputs back over PCIe)
     # through page-locked buffers allocated once (host_outputs), and through fresh pageable
@@ -570,8 +619,10 @@ def main():
                / FP32_VECTOR_PEAK_TFLOPS) / 1e12
     integ_peak = ops / ideal_s / 1e12 if ideal_s > 0 else FP64_VECTOR_PEAK_TFLOPS
     integ_ach = ops / (avg_integ * 1e-3) / 1e12 if avg_integ > 0 else 0.0
-    kname = ["k_trace_persistent", "k_trace_oneshot"][st["mode"]]
-    wkey = f"{args.workload}/surrogate" if args.path == "surrogate" else args.workload
+    # the fallback queues of the closed form and the surrogate run the one-shot kernel
+    kname = "k_trace_oneshot" if args.path in ("closed", "surrogate") else ["k_trace_persistent",
+                                                                            "k_trace_oneshot"][st["mode"]]
+    wkey = f"{args.workload}/{args.path}" if args.path in ("surrogate", "closed") else args.workload
     tr_integ, src_integ = profiled_traffic(wkey, args.height, kname)
     roof_integ = {"bound": "valu", "achieved": round(integ_ach, 3), "peak": round(integ_peak, 1), "unit": "TFLOP/s",
                   "frac": round(integ_ach / integ_peak, 4), "traffic": tr_integ, "traffic_source": src_integ,
@@ -597,6 +648,17 @@ def main():
                     "network": net_desc}
         if avg_sur > avg_integ:
             roof, roof_mlp = roof_mlp, roof_integ
+    if args.path == "closed":
+        # the closed-form kernel: FP64 operations per ray of its algorithm (CLOSED_FORM_OPS_PER_RAY)
+        # against the FP64 vector peak; the integrator over the rays it declined beside it
+        cf_ach = rays * CLOSED_FORM_OPS_PER_RAY / (avg_sur * 1e-3) / 1e12 if avg_sur > 0 else 0.0
+        tr_cf, src_cf = profiled_traffic(wkey, args.height, "k_closed_form")
+        roof = {"bound": "valu", "achieved": round(cf_ach, 3), "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(cf_ach / FP64_VECTOR_PEAK_TFLOPS, 4), "traffic": tr_cf, "traffic_source": src_cf,
+                "kernel": "k_closed_form", "kernel_ms": round(avg_sur, 3), "ops_per_ray": CLOSED_FORM_OPS_PER_RAY,
+                "ops_per_ray_is": "mean FP64 operations of the closed form per ray of the orbit frames (+ - * / "
+                                  "sqrt once, FMA twice), counted by tests/closed_form.py"}
+        roof_mlp = roof_integ
 
     desc = {"orbit": f"120-frame orbit r=40 M, i 20->85 deg, a=0.998, {width}x{args.height} Bardeen plane "
                      f"+-1.5 mean shadow radii wide, disk r_ISCO..20 M",
@@ -618,15 +680,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"plain": "f64", "mixed": "f32+f64", "surrogate": "bf16+f32+f64"}[args.path],
+        "dtype": {"plain": "f64", "mixed": "f32+f64", "surrogate": "bf16+f32+f64", "closed": "f64"}[args.path],
         "data": "synthetic: analytic Bardeen image planes, no input files; surrogate weights trained by "
                 "black_simulation_amd.train_surrogate on integrator-labelled rays",
         "config": {"workload": f"{args.path}: {desc}", "path": args.path, "rays_per_step": rays,
                    "network": net_desc,
                    "fallback": (f"{'FP64' if opts.precision == bsa.PREC_FP64 else 'mixed FP32/FP64'} DP5 rtol "
                                 f"{opts.rtol if opts.precision == bsa.PREC_FP64 else opts.rtol32:g}")
+                   if args.path in ("surrogate", "closed") else None,
+                   "gate": {"prob": opts.gate_prob, "ring": opts.gate_ring,
+                            "escaped_and_disk_verified_by_closed_form": bool(opts.closed_form)}
                    if args.path == "surrogate" else None,
-                   "gate": {"prob": opts.gate_prob, "ring": opts.gate_ring} if args.path == "surrogate" else None,
                    "streams": args.streams, "lean": args.lean,
                    "frame_ms_p50": round(percentile(lat, 0.5), 3), "frame_ms_p99": round(percentile(lat, 0.99), 3),
                    "plain_fp64": {"value": round(value_p, 3), "frame_ms_p50": round(percentile(lat_p, 0.5), 3),
@@ -635,8 +699,10 @@ def main():
                    "host_outputs_value": round(host_value, 3) if host_value else None,
                    "host_outputs_pageable_value": round(host_value_pageable, 3) if host_value_pageable else None,
                    "host_outputs_f32_value": round(host_value_f32, 3) if host_value_f32 else None,
-                   "fallback_fraction": round(sum(fallback) / n_attr / rays, 4) if args.path == "surrogate" else None,
+                   "fallback_fraction": round(sum(fallback) / n_attr / rays, 4)
+                   if args.path in ("surrogate", "closed") else None,
                    "accuracy_vs_fp64": acc,
+                   "learned_surrogate": alt_acc or None,
                    "shade": shade,
                    "launch": {"mode": ["persistent", "one-shot"][st["mode"]], "block": st["block"],
                               "grid": st["grid"], "waves_per_cu": st["waves_per_cu"],

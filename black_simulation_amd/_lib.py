@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libkerr_trace.so")
 
 KT_OK, KT_EINVAL, KT_EHIP, KT_ECANCELLED, KT_ENOMEM, KT_EUNSUPPORTED, KT_ENODEVICE = range(7)
 ESCAPED, CAPTURED, DISK, UNFINISHED = 0, 1, 2, 3
-FLAG_FP64_RESTEP, FLAG_SURROGATE = 0x10, 0x20
+FLAG_CLOSED_FORM, FLAG_FP64_RESTEP, FLAG_SURROGATE = 0x08, 0x10, 0x20
 CAMERA_BARDEEN, CAMERA_PINHOLE, CAMERA_OBSERVER = 0, 1, 3
 NET_WIDE, NET_FAST = 0, 1
 PARTICLE_MOVING, PARTICLE_PLUNGED, PARTICLE_ESCAPED, PARTICLE_UNFINISHED = 0, 1, 2, 3
@@ -66,6 +66,7 @@ class KtTraceOpts(C.Structure):
         ("gate_ring", C.c_double),
         ("waves_per_simd", C.c_int),
         ("cancel", C.c_void_p),
+        ("closed_form", C.c_int),
     ]
 
 

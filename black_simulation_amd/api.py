@@ -100,6 +100,9 @@ class TraceOpts:
     # EHT render / calibrate: a ctypes.c_int another thread may set to cancel (eht::render's
     # cancel flag); None: not cancellable
     cancel: object = None
+    # 1: the closed-form solution answers the rays it covers (KT_FLAG_CLOSED_FORM), the
+    # integrator the rest; with use_surrogate, it replaces the values the network answered
+    closed_form: int = 0
 
     def to_c(self) -> K.KtTraceOpts:
         o = K.KtTraceOpts()

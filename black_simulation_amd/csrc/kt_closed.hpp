@@ -1,0 +1,560 @@
+// kt_closed.hpp -- the photon's outcome from the closed-form solution of Carter's equations.
+//
+// Kerr null geodesics separate in Mino time s: with lambda = L/E and eta = Q/E^2 the radial
+// and polar motions are independent quadratures,
+//     ds = dr / sqrt(R(r)),   R(r) = r^4 + A r^2 + B r + C,
+//          A = a^2 - eta - lambda^2,  B = 2 (eta + (lambda - a)^2),  C = -a^2 eta
+//     ds = dmu / sqrt(M(mu)), M(mu) = a^2 (u+ - mu^2)(mu^2 - u-),  u+ u- = -eta / a^2,
+// (the integrator's right-hand side, kt_device.hpp, divided by E^2), and both reduce to
+// Legendre's elliptic integrals once the roots of R and of M are known (Gralla & Lupsasca,
+// Phys. Rev. D 101, 044032 (2020), sections IV-V; Dexter & Agol, ApJ 696, 1616 (2009)).
+// So the answers the integrator reaches after hundreds of DP5 steps follow from a few
+// integrals:
+//   * polar: with mu = sqrt(u+) sin(psi), F(psi | m) (m = u+/u- <= 0) grows by
+//     omega = sqrt(-a^2 u-) per unit of s, so the equator (psi = j pi) is crossed at the
+//     Mino times s_1, s_1 + 2K/omega, ... (K = K(m));
+//   * radial: four real roots r1 <= r2 <= r3 <= r4 (with r4 > r_cap the ray turns at r4
+//     and escapes) or two real and a complex pair (it falls in).  The Mino time from the
+//     largest real root up to r is c F(x(r) | k) in closed form, and r at a given Mino time
+//     is a rational function of sn or cn of it;
+//   * the disk: the first crossing whose radius lies in [r_in, r_out] is the hit, with the
+//     redshift of the Keplerian emitter there (src/trajectory.cpp:311-321);
+//   * an escaping ray's sky: theta_inf from sn at the total Mino time, and phi_inf from the
+//     integrals of dphi/ds = -(a (2r - a lambda) / Delta + lambda / (1 - mu^2)): elliptic
+//     integrals of the third kind, one per horizon radius, and one in psi.
+// Special functions: Carlson's symmetric integrals R_F, R_J, R_C by duplication (B. C.
+// Carlson, Numer. Algorithms 10, 13 (1995)), K(m) by the arithmetic-geometric mean, and
+// Jacobi's sn, cn, dn by descending Landen transformations (Abramowitz & Stegun 16.12).
+// All in FP64; answers agree with the FP64 DP5 integrator at rtol 1e-10 to ~1e-9 (its own
+// error).  Rays the forms here do not cover (polar vortical orbits that escape, L = 0
+// escapes whose phi jumps at the pole, a turning point at the capture radius, a crossing
+// within 1e-9 of the disk's edges, outgoing rays with no turning point) are declined and
+// left to the integrator.
+#pragma once
+
+#include <math.h>
+
+#ifdef __HIPCC__
+#define KTC_FN __host__ __device__ __forceinline__
+// the iterated special functions are called, not inlined at each of their call sites: one
+// copy each keeps the kernel's code and register count down
+#define KTC_CALL __host__ __device__ __noinline__
+#else
+#define KTC_FN inline
+#define KTC_CALL inline
+#endif
+
+namespace ktc {
+
+constexpr double kPiC = 3.14159265358979323846;
+
+// ---- arithmetic ------------------------------------------------------------------------
+// On the device: square root, reciprocal square root and reciprocal from the hardware
+// estimates (relative error ~2^-23) refined by Newton-Raphson steps to about one ulp --
+// a third of the instructions of the correctly rounded forms, which nothing here needs.
+// The host uses the correctly rounded ones.
+KTC_FN double qrsqrt(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-x * y, y, 1.0);  // 1 - x y^2
+    y = fma(0.5 * y, e, y);
+    e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+#else
+    return 1.0 / sqrt(x);
+#endif
+}
+// x >= 0 (0 gives 1e-150, which no use here tells from 0)
+KTC_FN double qsqrt(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    x = fmax(x, 1e-300);
+    const double y = __builtin_amdgcn_rsq(x);
+    double s = x * y, h = 0.5 * y;
+    const double r = fma(-s, h, 0.5);
+    s = fma(s, r, s);
+    h = fma(h, r, h);
+    const double d = fma(-s, s, x);
+    return fma(d, h, s);
+#else
+    return sqrt(x);
+#endif
+}
+KTC_FN double qrcp(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+#else
+    return 1.0 / x;
+#endif
+}
+
+// sin and cos of |u| < 1e6: u - j pi/2 with pi/2 in three parts (Cody-Waite; the product
+// of j and each part is exact for j < 2^20), then the Taylor series of both on
+// [-pi/4, pi/4] to the 19th and 20th order (truncation below 1e-18)
+KTC_FN void qsincos(double u, double& s, double& c) {
+    constexpr double kTwoOverPi = 0.63661977236758134308;
+    constexpr double kP1 = 1.5707963267341256;      // pi/2 to 33 bits
+    constexpr double kP2 = 6.0771005065061922e-11;  // the next 33 bits
+    constexpr double kP3 = 2.0222662487959506e-21;  // the rest
+    const double j = rint(u * kTwoOverPi);
+    const double x = ((u - j * kP1) - j * kP2) - j * kP3;
+    const double x2 = x * x;
+    // sin x = x (1 - x^2/3! + x^4/5! - ...), cos x = 1 - x^2/2! + x^4/4! - ...
+    double ps = -1.0 / 121645100408832000.0;  // -1/19!
+    ps = fma(ps, x2, 1.0 / 355687428096000.0);
+    ps = fma(ps, x2, -1.0 / 1307674368000.0);
+    ps = fma(ps, x2, 1.0 / 6227020800.0);
+    ps = fma(ps, x2, -1.0 / 39916800.0);
+    ps = fma(ps, x2, 1.0 / 362880.0);
+    ps = fma(ps, x2, -1.0 / 5040.0);
+    ps = fma(ps, x2, 1.0 / 120.0);
+    ps = fma(ps, x2, -1.0 / 6.0);
+    const double sx = fma(ps * x2, x, x);
+    double pc = 1.0 / 2432902008176640000.0;  // 1/20!
+    pc = fma(pc, x2, -1.0 / 6402373705728000.0);
+    pc = fma(pc, x2, 1.0 / 20922789888000.0);
+    pc = fma(pc, x2, -1.0 / 87178291200.0);
+    pc = fma(pc, x2, 1.0 / 479001600.0);
+    pc = fma(pc, x2, -1.0 / 3628800.0);
+    pc = fma(pc, x2, 1.0 / 40320.0);
+    pc = fma(pc, x2, -1.0 / 720.0);
+    pc = fma(pc, x2, 1.0 / 24.0);
+    pc = fma(pc, x2, -0.5);
+    const double cx = fma(pc, x2, 1.0);
+    // quadrant j mod 4: (s, c) = (sx, cx), (cx, -sx), (-sx, -cx), (-cx, sx)
+    const long long q = (long long)j & 3;
+    const double a = (q & 1) ? cx : sx, b = (q & 1) ? sx : cx;
+    s = (q & 2) ? -a : a;
+    c = ((q + 1) & 2) ? -b : b;
+}
+
+// ---- special functions -------------------------------------------------------------------
+
+// R_F(x, y, z), x, y, z >= 0, at most one zero: duplication until the arguments agree to
+// 6e-3 of their mean, then the fifth-order series (truncation ~5e-15 relative)
+constexpr double kDupTol = 6e-3;
+
+KTC_CALL double rf(double x, double y, double z) {
+    double A = (x + y + z) * (1.0 / 3.0);
+    for (int it = 0; it < 24; ++it) {
+        if (fmax(fabs(A - x), fmax(fabs(A - y), fabs(A - z))) < kDupTol * A) break;
+        const double sx = qsqrt(x), sy = qsqrt(y), sz = qsqrt(z);
+        const double lam = sx * (sy + sz) + sy * sz;
+        x = 0.25 * (x + lam);
+        y = 0.25 * (y + lam);
+        z = 0.25 * (z + lam);
+        A = (x + y + z) * (1.0 / 3.0);
+    }
+    const double inv = qrcp(A);
+    const double X = (A - x) * inv, Y = (A - y) * inv, Z = -X - Y;
+    const double E2 = X * Y - Z * Z, E3 = X * Y * Z;
+    return (1.0 - E2 * (1.0 / 10.0) + E3 * (1.0 / 14.0) + E2 * E2 * (1.0 / 24.0) - E2 * E3 * (3.0 / 44.0)) * qrsqrt(A);
+}
+
+// R_C(x, y), y > 0: the elementary closed forms, written so that x ~ y loses nothing
+KTC_FN double rc(double x, double y) {
+    if (x < y) {
+        const double d = y - x;
+        return x > 0.0 ? atan(sqrt(d / x)) / sqrt(d) : 0.5 * kPiC / sqrt(y);
+    }
+    if (x > y) {  // atanh(t) / sqrt(d), t = sqrt(d / x): near t = 1 as the logarithm
+        const double d = x - y, t = sqrt(d / x);
+        if (t < 0.5) return atanh(t) / sqrt(d);
+        const double sd = sqrt(d);
+        return log((sqrt(x) + sd) / sqrt(y)) / sd;
+    }
+    return 1.0 / sqrt(x);
+}
+
+// R_J(x, y, z, p), p > 0: duplication with the R_C terms, then the series
+KTC_CALL double rj(double x, double y, double z, double p) {
+    double s = 0.0, fac = 1.0;
+    double A = (x + y + z + 2.0 * p) * 0.2;
+    for (int it = 0; it < 24; ++it) {
+        if (fmax(fmax(fabs(A - x), fabs(A - y)), fmax(fabs(A - z), fabs(A - p))) < kDupTol * A) break;
+        const double sx = qsqrt(x), sy = qsqrt(y), sz = qsqrt(z);
+        const double lam = sx * (sy + sz) + sy * sz;
+        double al = p * (sx + sy + sz) + sx * sy * sz;
+        al *= al;
+        const double pl = p + lam;
+        const double be = p * pl * pl;
+        s += fac * rc(al, be);
+        fac *= 0.25;
+        x = 0.25 * (x + lam);
+        y = 0.25 * (y + lam);
+        z = 0.25 * (z + lam);
+        p = 0.25 * (p + lam);
+        A = (x + y + z + 2.0 * p) * 0.2;
+    }
+    const double inv = qrcp(A);
+    const double X = (A - x) * inv, Y = (A - y) * inv, Z = (A - z) * inv;
+    const double P = -0.5 * (X + Y + Z);
+    const double P2 = P * P, XYZ = X * Y * Z;
+    const double E2 = X * Y + X * Z + Y * Z - 3.0 * P2;
+    const double E3 = XYZ + 2.0 * E2 * P + 4.0 * P2 * P;
+    const double E4 = (2.0 * XYZ + E2 * P + 3.0 * P2 * P) * P;
+    const double E5 = XYZ * P2;
+    const double ser = 1.0 - E2 * (3.0 / 14.0) + E3 * (1.0 / 6.0) + E2 * E2 * (9.0 / 88.0) - E4 * (3.0 / 22.0) -
+                       E2 * E3 * (9.0 / 52.0) + E5 * (3.0 / 26.0);
+    return 3.0 * s + fac * ser * inv * qrsqrt(A);
+}
+
+// Legendre's F(asin s | m) and Pi(n; asin s | m), |s| <= 1, m < 1, n s^2 < 1
+KTC_FN double ell_f(double s, double m) { return s * rf((1.0 - s) * (1.0 + s), 1.0 - m * s * s, 1.0); }
+KTC_FN double ell_pi(double n, double s, double m) {
+    const double c2 = (1.0 - s) * (1.0 + s), d2 = 1.0 - m * s * s, s2 = s * s;
+    return s * rf(c2, d2, 1.0) + n * (1.0 / 3.0) * s * s2 * rj(c2, d2, 1.0, 1.0 - n * s2);
+}
+KTC_FN double ell_pi_complete(double n, double m) {
+    return rf(0.0, 1.0 - m, 1.0) + n * (1.0 / 3.0) * rj(0.0, 1.0 - m, 1.0, 1.0 - n);
+}
+
+// K(m), m < 1: pi / (2 AGM(1, sqrt(1 - m))), the mean converging quadratically
+KTC_CALL double ell_k(double m) {
+    double a = 1.0, b = qsqrt(1.0 - m);
+    for (int it = 0; it < 12; ++it) {
+        if (fabs(a - b) <= 1e-15 * a) break;
+        const double an = 0.5 * (a + b);
+        b = qsqrt(a * b);
+        a = an;
+    }
+    return kPiC * qrcp(a + b);
+}
+
+// Jacobi sn, cn, dn (u | m), 0 <= m < 1: descending Landen transformations
+//   k_{j+1} = (1 - k'_j) / (1 + k'_j),  u_{j+1} = u_j / (1 + k_{j+1}) = u_j (1 + k'_j) / 2,
+//   k'_{j+1} = 2 sqrt(k'_j) / (1 + k'_j),
+// until 1 - k' < 1e-9 (at most seven levels from m = 1 - 1e-12), the first-order
+// small-modulus form there, then back up
+//   sn_j = (1 + k) sn / (1 + k sn^2), cn_j = cn dn / (1 + k sn^2), dn_j = (1 - k sn^2) / (1 + k sn^2)
+KTC_CALL void sncndn(double u, double m, double& sn, double& cn, double& dn) {
+    constexpr int kLevels = 7;
+    double ks[kLevels];
+    double kp = qsqrt(1.0 - m);
+    int n = 0;
+#pragma unroll
+    for (int j = 0; j < kLevels; ++j) {
+        ks[j] = 0.0;
+        if (kp < 1.0 - 1e-9) {
+            const double r = qrcp(1.0 + kp);
+            ks[j] = (1.0 - kp) * r;
+            u = u * (0.5 * (1.0 + kp));
+            kp = 2.0 * qsqrt(kp) * r;
+            n = j + 1;
+        }
+    }
+    const double mn = (1.0 - kp) * (1.0 + kp);
+    double s, c;
+    qsincos(u, s, c);
+    const double d = 0.25 * mn * (u - s * c);  // sn(u|m) = sin u - (m/4)(u - sin u cos u) cos u + O(m^2)
+    const double s1 = s - d * c, c1 = c + d * s;
+    s = s1;
+    c = c1;
+    double dd = qsqrt(1.0 - mn * s * s);
+#pragma unroll
+    for (int j = kLevels - 1; j >= 0; --j) {
+        if (j < n) {
+            const double k1 = ks[j];
+            const double ks2 = k1 * s * s;
+            const double inv = qrcp(1.0 + ks2);
+            const double ns = (1.0 + k1) * s * inv, nc = c * dd * inv, nd = (1.0 - ks2) * inv;
+            s = ns;
+            c = nc;
+            dd = nd;
+        }
+    }
+    sn = s;
+    cn = c;
+    dn = dd;
+}
+
+// the largest real root of the resolvent cubic xi^3 + A xi^2 + (A^2/4 - C) xi - B^2/8 of
+// the depressed quartic r^4 + A r^2 + B r + C (positive when B != 0), Cardano or the
+// trigonometric form, then two Newton steps
+KTC_CALL double resolvent_root(double A, double B, double C) {
+    const double c2 = A, c1 = 0.25 * A * A - C, c0 = -0.125 * B * B;
+    const double p = c1 - c2 * c2 * (1.0 / 3.0);
+    const double q = 2.0 * c2 * c2 * c2 * (1.0 / 27.0) - c2 * c1 * (1.0 / 3.0) + c0;
+    const double hq = 0.5 * q, tp = p * (1.0 / 3.0);
+    const double disc = hq * hq + tp * tp * tp;
+    double t;
+    if (disc > 0.0) {
+        const double sd = sqrt(disc);
+        const double u = cbrt(hq < 0.0 ? -hq + sd : -hq - sd);
+        t = u != 0.0 ? u - tp / u : 0.0;
+    } else {
+        const double r = sqrt(-tp);
+        double ca = -hq / (r * r * r);
+        ca = fmin(1.0, fmax(-1.0, ca));
+        t = 2.0 * r * cos(acos(ca) * (1.0 / 3.0));
+    }
+    double xi = t - c2 * (1.0 / 3.0);
+    for (int it = 0; it < 2; ++it) {
+        const double f = ((xi + c2) * xi + c1) * xi + c0;
+        const double df = (3.0 * xi + 2.0 * c2) * xi + c1;
+        if (df != 0.0) xi -= f / df;
+    }
+    return xi;
+}
+
+// ---- one ray --------------------------------------------------------------------------------
+
+// one ray in E = 1 units: spin a, camera (r_o, mu_o, BL phi_o), lambda = L/E, eta = Q/E^2,
+// the signs of dr/ds and dmu/ds at the camera (s the backward Mino time), the disk and the
+// capture radius; E only scales g
+struct Ray {
+    double a, r_o, mu_o, phi_o, lam, eta, E;
+    int s_r, s_mu;
+    double r_in, r_out, r_cap, r_esc;
+    int disk, prograde;
+};
+
+enum : int { kDeclined = -1 };
+
+struct Outcome {
+    int cls;  // ktd::kEscaped / kCaptured / kDisk, or kDeclined
+    double theta, phi, r_hit, g;
+};
+
+// the Keplerian emitter's redshift at r (src/trajectory.cpp:311-321); false: no timelike orbit
+KTC_FN bool disk_g(double a, double r, int prograde, double E, double L, double& g) {
+    const double sg = prograde ? 1.0 : -1.0;
+    const double om = sg / (r * sqrt(r) + sg * a);
+    const double gtt = -(1.0 - 2.0 / r), gtp = -2.0 * a / r;
+    const double gpp = r * r + a * a + 2.0 * a * a / r;
+    const double norm = -(gtt + 2.0 * gtp * om + gpp * om * om);
+    if (!(norm > 0.0)) return false;
+    g = sqrt(norm) / (E - L * om);
+    return true;
+}
+
+KTC_FN double wrap2pi(double x) {
+    const double t = 2.0 * kPiC;
+    double y = fmod(x, t);
+    if (y < 0.0) y += t;
+    return y;
+}
+
+// a radius within this relative distance of a disk edge is left to the integrator, whose
+// crossing radius is good to ~1e-10
+constexpr double kEdgeBand = 1e-9;
+constexpr int kMaxCrossings = 24;
+
+// The polar motion: omega, u+, m, x = mu_o / sqrt(u+), F(asin x | m), K(m), and the Mino
+// time of the first equatorial crossing (eta > 0).
+struct Polar {
+    double om, up, m, x, Fo, K, s1;
+};
+
+KTC_FN Polar polar_motion(const Ray& ry, double A) {
+    Polar P;
+    const double a2 = ry.a * ry.a, eta = ry.eta;
+    const double disc = sqrt(A * A + 4.0 * a2 * eta);
+    const double om2 = A <= 0.0 ? 0.5 * (disc - A) : 2.0 * a2 * eta / (A + disc);  // a^2 (-u-)
+    P.om = sqrt(om2);
+    P.up = eta / om2;
+    P.m = -a2 * eta / (om2 * om2);
+    P.x = fmin(1.0, fmax(-1.0, ry.mu_o / sqrt(P.up)));
+    P.Fo = ell_f(P.x, P.m);
+    P.K = ell_k(P.m);
+    // F at the next psi = j pi in the direction of motion; starting on the plane is no crossing
+    const double x = P.x;
+    double F1;
+    if (ry.s_mu > 0) F1 = (x > 0.0 ? 2.0 * P.K : 0.0) - P.Fo;
+    else F1 = P.Fo - (x > 0.0 ? 0.0 : -2.0 * P.K);
+    if (x == 0.0) F1 = 2.0 * P.K;
+    P.s1 = F1 / P.om;
+    return P;
+}
+
+// An escaping ray (four real roots, turning at r4) that the polar forms cover (eta > 0,
+// lambda != 0): the sky angles at infinity from the total Mino time S_tot.  x_o is x(r_o)
+// of the radial substitution, x_inf its limit; returns false if a value is not finite.
+KTC_CALL bool escape_sky(const Ray& ry, const Polar& P, double S_tot, double r1, double r3, double r4, double k,
+                         double pref, double x_o, double x_inf, double& theta, double& phi) {
+    const double a = ry.a, a2 = a * a, lam = ry.lam;
+    const double r31 = r3 - r1, r41 = r4 - r1;
+    // theta_inf from sn at the total Mino time (m <= 0:
+    // sn(u | -M) = sd(u sqrt(1 + M) | M / (1 + M)) / sqrt(1 + M))
+    const double Phi = P.Fo + (double)ry.s_mu * P.om * S_tot;  // F(psi_inf | m), psi unwrapped
+    const double M = -P.m, sq = sqrt(1.0 + M);
+    double sn, cn, dn;
+    sncndn(Phi * sq, M / (1.0 + M), sn, cn, dn);
+    const double sn_m = sn / (dn * sq);  // sn(Phi | m)
+    const double mu_inf = sqrt(P.up) * sn_m;
+    theta = acos(fmin(1.0, fmax(-1.0, mu_inf)));
+    // the unwrapped amplitude: psi_inf in [j pi - pi/2, j pi + pi/2]
+    const double jj = floor((Phi + P.K) / (2.0 * P.K));
+    const double rem = fmod(jj, 2.0) == 0.0 ? sn_m : -sn_m;
+    const double P_inf = 2.0 * jj * ell_pi_complete(P.up, P.m) + ell_pi(P.up, rem, P.m);
+    const double P_o = ell_pi(P.up, P.x, P.m);
+    const double G_phi = (double)ry.s_mu * (P_inf - P_o) / P.om;  // integral of ds / (1 - mu^2)
+    // the radial part, integral of a (2r - a lambda) / Delta ds, by partial fractions over
+    // the horizons: a (2 rp - a lambda) / (rp - rq) / (r - rp) for rp = r+, r-; over the
+    // path, Mino time from r4 to r of 1 / (r - p) is
+    //   pref [F / (r3 - p) + (1 / (r4 - p) - 1 / (r3 - p)) Pi(n_p; x | k)],
+    //   n_p = (r3 - p) r41 / ((r4 - p) r31)
+    double Ir = 0.0;
+    if (a != 0.0) {
+        const double sq1 = sqrt(fmax(0.0, 1.0 - a2));
+        if (sq1 == 0.0) return false;  // |a| = 1: the partial fractions degenerate
+        const double Fo = ell_f(x_o, k), Fi = ell_f(x_inf, k);
+        for (int h = 0; h < 2; ++h) {
+            const double p = h == 0 ? 1.0 + sq1 : 1.0 - sq1, q = h == 0 ? 1.0 - sq1 : 1.0 + sq1;
+            const double Ap = a * (2.0 * p - a * lam) / (p - q);
+            const double np = (r3 - p) * r41 / ((r4 - p) * r31);
+            const double ca = 1.0 / (r3 - p), cb = 1.0 / (r4 - p) - ca;
+            const double Jo = ca * Fo + cb * ell_pi(np, x_o, k);
+            const double Ji = ca * Fi + cb * ell_pi(np, x_inf, k);
+            // inward start: camera to r4 and r4 to infinity; outward: camera to infinity
+            Ir += Ap * pref * (ry.s_r < 0 ? Jo + Ji : Ji - Jo);
+        }
+    }
+    phi = wrap2pi(ry.phi_o - (Ir + lam * G_phi));
+    return theta == theta && phi == phi;
+}
+
+// The disk test at the crossings s, s + ds, ... before s_end: the radius there from
+// r_at(s), the first one inside [r_in, r_out] a hit.  Returns 1 (hit: rh, g), 0 (none before
+// s_end, or the ray is below r_cap by then) or -1 (declined).
+template <class RAt>
+KTC_FN int crossings(const Ray& ry, double s, double ds, double s_end, RAt r_at, double& rh, double& g) {
+    for (int j = 0; j < kMaxCrossings; ++j) {
+        if (!(s < s_end)) return 0;
+        const double rc = r_at(s);
+        if (!(rc == rc)) return -1;
+        if (rc < ry.r_cap) return 0;  // inward past the capture radius: later crossings lie deeper
+        if (fabs(rc - ry.r_in) < kEdgeBand * ry.r_in || fabs(rc - ry.r_out) < kEdgeBand * ry.r_out) return -1;
+        if (rc >= ry.r_in && rc <= ry.r_out && disk_g(ry.a, rc, ry.prograde, ry.E, ry.lam * ry.E, g)) {
+            rh = rc;
+            return 1;
+        }
+        s += ds;
+    }
+    return -1;
+}
+
+KTC_FN Outcome solve(const Ray& ry) {
+    Outcome out{kDeclined, 0.0, 0.0, 0.0, 0.0};
+    const double a = ry.a, a2 = a * a, lam = ry.lam, eta = ry.eta;
+    const double A = a2 - eta - lam * lam;
+    const double lma = lam - a;
+    const double B = 2.0 * (eta + lma * lma);
+    const double C = -a2 * eta;
+    if (!(B > 1e-12) || !(ry.E > 0.0) || !(ry.r_out < ry.r_esc)) return out;
+    const double xi = resolvent_root(A, B, C);
+    if (!(xi > 0.0)) return out;
+    const double z = sqrt(0.5 * xi);
+    const double D12 = -0.5 * A - z * z + B / (4.0 * z);
+    const double D34 = -0.5 * A - z * z - B / (4.0 * z);
+    const bool cross = eta > 0.0 && ry.disk;  // eta <= 0: the ray never reaches the equator
+    Polar P{};
+    if (eta > 0.0) P = polar_motion(ry, A);
+    const double ds = eta > 0.0 ? 2.0 * P.K / P.om : 0.0;
+
+    if (D34 >= 0.0) {
+        // four real roots; from r4 up to r the Mino time is pref F(x(r) | k),
+        // x(r)^2 = r31 (r - r4) / (r41 (r - r3)), and r at Mino time J from r4 is
+        // (r31 r4 - x^2 r41 r3) / (r31 - x^2 r41) with x = sn(J / pref | k)
+        const double sD12 = sqrt(fmax(D12, 0.0)), sD34 = sqrt(D34);
+        const double r1 = -z - sD12, r2 = -z + sD12, r3 = z - sD34, r4 = z + sD34;
+        const double r31 = r3 - r1, r41 = r4 - r1, r42 = r4 - r2, r32 = r3 - r2;
+        const double k = r32 * r41 / (r31 * r42);
+        const double pref = 2.0 / sqrt(r31 * r42);
+        const bool turning = r4 > ry.r_cap;
+        if (turning && (r4 < ry.r_cap * (1.0 + 1e-7) || !(r4 < ry.r_o))) return out;
+        if (r4 == ry.r_cap) return out;
+        if (ry.s_r > 0 && !turning) return out;  // outward from below a turning point: not covered
+        const double x_o = sqrt(r31 * (ry.r_o - r4) / (r41 * (ry.r_o - r3)));
+        const double I_o = pref * ell_f(x_o, k);
+        const double x_inf = sqrt(r31 / r41);
+        // the turning point lies S_t ahead of the camera (behind it, S_t < 0, outward start);
+        // a ray that does not turn falls from r_o through r_cap before reaching r4
+        const double S_t = ry.s_r < 0 ? I_o : -I_o;
+        auto r_at = [&](double s) {
+            const double J = s < S_t ? S_t - s : s - S_t;  // Mino time from r4
+            double sn, cn, dn;
+            sncndn(J / pref, k, sn, cn, dn);
+            const double x2 = sn * sn;
+            return (r31 * r4 - x2 * r41 * r3) / (r31 - x2 * r41);
+        };
+        // the crossings on the inward leg first; the escape time is needed only once a
+        // crossing comes after the turning point, or for the sky
+        double I_inf = -1.0;
+        int hit = 0;
+        double rh = 0.0, g = 0.0;
+        if (cross) {
+            hit = crossings(ry, P.s1, ds, S_t, r_at, rh, g);
+            if (hit == 0 && turning) {
+                I_inf = pref * ell_f(x_inf, k);
+                double s = P.s1;
+                if (s < S_t) s += ds * ceil((S_t - s) / ds);  // the first crossing after the turn
+                hit = crossings(ry, s, ds, S_t + I_inf, r_at, rh, g);
+            }
+        }
+        if (hit < 0) return out;
+        if (hit > 0) {
+            out.cls = 2;
+            out.r_hit = rh;
+            out.g = g;
+            return out;
+        }
+        if (!turning) {
+            out.cls = 1;
+            return out;
+        }
+        if (!(eta > 0.0) || lam == 0.0) return out;  // vortical or through the pole: the integrator
+        if (I_inf < 0.0) I_inf = pref * ell_f(x_inf, k);
+        double th, ph;
+        if (!escape_sky(ry, P, S_t + I_inf, r1, r3, r4, k, pref, x_o, x_inf, th, ph)) return out;
+        out.cls = 0;
+        out.theta = th;
+        out.phi = ph;
+        return out;
+    }
+    if (ry.s_r > 0) return out;  // no turning point: an outgoing ray escapes by forms not given here
+    if (D12 >= 0.0) {
+        // r1 <= r2 real, r3 = conj(r4): no turning point outside the horizon.  From r2 up to r
+        // the Mino time is pref F(acos c(r) | k3), c(r) = (Aq (r - r1) - Bq (r - r2)) /
+        // (Aq (r - r1) + Bq (r - r2)), and r at Mino time J from r2 follows from c = cn(J / pref | k3)
+        const double sD12 = sqrt(D12);
+        const double r1 = -z - sD12, r2 = -z + sD12;
+        if (!(r2 < ry.r_cap)) return out;
+        if (cross) {
+            const double im3 = sqrt(-D34);
+            const double Aq = hypot(z - r2, im3), Bq = hypot(z - r1, im3);
+            const double r21 = r2 - r1;
+            const double k3 = ((Aq + Bq) * (Aq + Bq) - r21 * r21) / (4.0 * Aq * Bq);
+            const double pref = 1.0 / sqrt(Aq * Bq);
+            const double ro = ry.r_o;
+            const double c = (Aq * (ro - r1) - Bq * (ro - r2)) / (Aq * (ro - r1) + Bq * (ro - r2));
+            const double f = ell_f(sqrt(fmax(0.0, (1.0 - c) * (1.0 + c))), k3);
+            const double S_t = pref * (c >= 0.0 ? f : 2.0 * ell_k(k3) - f);  // Mino time down to r2
+            auto r_at = [&](double s) {
+                double sn, cn, dn;
+                sncndn((S_t - s) / pref, k3, sn, cn, dn);
+                return (Aq * r1 * (1.0 - cn) - Bq * r2 * (1.0 + cn)) / (Aq * (1.0 - cn) - Bq * (1.0 + cn));
+            };
+            double rh = 0.0, g = 0.0;
+            const int hit = crossings(ry, P.s1, ds, S_t, r_at, rh, g);
+            if (hit < 0) return out;
+            if (hit > 0) {
+                out.cls = 2;
+                out.r_hit = rh;
+                out.g = g;
+                return out;
+            }
+        }
+        out.cls = 1;
+        return out;
+    }
+    // all four roots complex: no turning point, and (eta < 0) no equatorial crossing
+    if (cross) return out;
+    out.cls = 1;
+    return out;
+}
+
+}  // namespace ktc

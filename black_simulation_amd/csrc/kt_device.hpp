@@ -44,7 +44,7 @@ constexpr double kSafety = 0.9, kFacMin = 0.2, kFacMax = 5.0;
 constexpr double kAxisSnap = 3e-4;  // src/kerr.h:115
 
 enum : int { kEscaped = 0, kCaptured = 1, kDisk = 2, kUnfinished = 3 };
-enum : int { kFlagRestep = 0x10, kFlagSurrogate = 0x20, kFlagNonfinite = 0x40 };
+enum : int { kFlagClosedForm = 0x08, kFlagRestep = 0x10, kFlagSurrogate = 0x20, kFlagNonfinite = 0x40 };
 
 // Per-view constants, prepared on the host (trig and frames in FP64 there).
 struct ViewDev {

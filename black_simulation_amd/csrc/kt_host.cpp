@@ -335,6 +335,7 @@ struct kt_ctx {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timed = false;
     kts::Surrogate* surrogate = nullptr;
+    kts::Queue cf_queue;  // the closed form's queue of declined rays (kt_trace_views)
     kti::Scratch scratch{};  // the no-allocation entry point's fixed device arrays (per device)
     float* d_blackbody = nullptr;  // shading: the blackbody colour table
     std::vector<float> h_blackbody;
@@ -426,6 +427,7 @@ const char* check_opts(const kt_trace_opts& o) {
         return "block must be 0 (tuned) or 64, 128, 192 or 256";
     if (o.waves_per_cu < 0 || o.steps_per_dequeue < 0) return "negative launch shape";
     if (o.waves_per_simd < 0 || o.waves_per_simd > 4) return "waves_per_simd must be 0 (tuned) to 4";
+    if (o.closed_form != 0 && o.closed_form != 1) return "closed_form must be 0 or 1";
     return nullptr;
 }
 
@@ -488,6 +490,7 @@ int observer_batch(kt_ctx* ctx, const std::vector<ViewDev>& vd, const kt_trace_o
     if (n != (int)vd.size()) return fail(ctx, KT_EINVAL, "observer cameras cannot share a call with other kinds");
     if (o.precision != KT_PREC_FP64) return fail(ctx, KT_EINVAL, "observer cameras trace in FP64 (KT_PREC_FP64)");
     if (o.use_surrogate) return fail(ctx, KT_EINVAL, "the surrogate does not cover observer cameras");
+    if (o.closed_form) return fail(ctx, KT_EINVAL, "the closed form does not cover observer cameras");
     return KT_OK;
 }
 
@@ -527,6 +530,7 @@ int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const k
     // outputs: device pointers as given, or staging for host pointers
     const bool report = out->turns || out->r_min || out->delay || out->b || out->q;
     if (report && o.use_surrogate) return fail(ctx, KT_EINVAL, "the ray report is not given with the surrogate");
+    if (report && o.closed_form) return fail(ctx, KT_EINVAL, "the ray report is not given with the closed form");
     if (report && o.precision != KT_PREC_FP64) return fail(ctx, KT_EINVAL, "the ray report is traced in FP64 (KT_PREC_FP64)");
     ktd::Outputs dout{};
     size_t off[15] = {};
@@ -605,8 +609,16 @@ int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const k
     if (o.use_surrogate) {
         if (!ctx->surrogate) return fail(ctx, KT_EUNSUPPORTED, "use_surrogate set but no surrogate is loaded");
         KT_HIP(hipEventRecord(ctx->ev[0], stream));
-        hipError_t e = kts::trace_with_surrogate(ctx->surrogate, L, o.gate_prob, o.gate_ring, nullptr, stream, ctx->ev);
+        hipError_t e = kts::trace_with_surrogate(ctx->surrogate, L, o.gate_prob, o.gate_ring, o.closed_form, nullptr,
+                                                 stream, ctx->ev);
         if (e != hipSuccess) return hip_fail(ctx, e, "surrogate trace");
+        KT_HIP(hipEventRecord(ctx->ev[3], stream));
+    } else if (o.closed_form) {
+        KT_HIP(kts::queue_reserve(ctx->cf_queue, total));
+        const kti::RayQueue rq{ctx->cf_queue.list, ctx->cf_queue.count, ctx->cf_queue.total, ctx->cf_queue.cap};
+        KT_HIP(hipEventRecord(ctx->ev[0], stream));
+        hipError_t e = kti::trace_closed_form(L, rq, stream, ctx->ev);
+        if (e != hipSuccess) return hip_fail(ctx, e, "closed-form trace");
         KT_HIP(hipEventRecord(ctx->ev[3], stream));
     } else {
         KT_HIP(hipEventRecord(ctx->ev[0], stream));
@@ -726,6 +738,7 @@ int kt_destroy(kt_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
     if (ctx->surrogate) kts::destroy(ctx->surrogate);
+    kts::queue_release(ctx->cf_queue);
     if (ctx->d_blackbody) (void)hipFree(ctx->d_blackbody);
     if (ctx->d_eht) (void)hipFree(ctx->d_eht);
     if (ctx->d_counter) (void)hipFree(ctx->d_counter);
@@ -799,6 +812,8 @@ int kt_trace_views_noalloc(kt_ctx* ctx, const kt_view* views, int n_views, const
         if (const char* why = check_opts(o)) return fail(ctx, KT_EINVAL, why);
         if (o.use_surrogate)
             return fail(ctx, KT_EINVAL, "kt_trace_views_noalloc does not run the surrogate (its queue is allocated)");
+        if (o.closed_form)
+            return fail(ctx, KT_EINVAL, "kt_trace_views_noalloc does not run the closed form (its queue is allocated)");
         // the view table on the stack: no allocation on the way to the kernels
         ViewDev vd[kti::kMaxLdsViews];
         long long total = 0;
@@ -893,6 +908,7 @@ int kt_autotune(kt_ctx* ctx, const kt_view* views, int n_views, kt_trace_opts* o
         if (const char* why = check_opts(o)) return fail(ctx, KT_EINVAL, why);
         o.mode = KT_MODE_PERSISTENT;
         o.use_surrogate = 0;
+        o.closed_form = 0;
         std::vector<ViewDev> vd;
         long long total = 0;
         if (int st = prepare_views(ctx, views, n_views, vd, total)) return st;
@@ -999,7 +1015,7 @@ int kt_plan_create(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
         KT_HIP(hipSetDevice(ctx->device));
         kt_plan* p = new kt_plan();
         p->ctx = ctx;
-        p->surrogate = o.use_surrogate;
+        p->surrogate = o.use_surrogate || o.closed_form;
         auto cleanup = [&](int code, const char* where, hipError_t e) {
             hip_fail(ctx, e, where);
             kt_plan_destroy(p);
@@ -1014,7 +1030,7 @@ int kt_plan_create(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
             return cleanup(KT_EHIP, "plan upload", e);
         for (auto& ev : p->ev)
             if ((e = hipEventCreate(&ev)) != hipSuccess) return cleanup(KT_EHIP, "plan events", e);
-        if (o.use_surrogate && (e = kts::queue_reserve(p->queue, total)) != hipSuccess)
+        if ((o.use_surrogate || o.closed_form) && (e = kts::queue_reserve(p->queue, total)) != hipSuccess)
             return cleanup(KT_ENOMEM, "plan fallback queue", e);
         ktd::Outputs dout{out->cls, out->theta_inf, out->phi_inf, out->r_hit, out->g, out->steps,
                           out->theta_inf_f32, out->phi_inf_f32, out->r_hit_f32, out->g_f32};
@@ -1027,8 +1043,10 @@ int kt_plan_create(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
             return cleanup(KT_EHIP, "plan stream", e);
         if ((e = hipStreamBeginCapture(p->capture, hipStreamCaptureModeThreadLocal)) != hipSuccess)
             return cleanup(KT_EHIP, "begin capture", e);
+        const kti::RayQueue rq{p->queue.list, p->queue.count, p->queue.total, p->queue.cap};
         hipError_t le = o.use_surrogate ? kts::trace_with_surrogate(ctx->surrogate, p->L, o.gate_prob, o.gate_ring,
-                                                                    &p->queue, p->capture, nullptr)
+                                                                    o.closed_form, &p->queue, p->capture, nullptr)
+                        : o.closed_form ? kti::trace_closed_form(p->L, rq, p->capture, nullptr)
                                         : kti::launch_trace(p->L, p->capture);
         if ((e = hipStreamEndCapture(p->capture, &p->graph)) != hipSuccess) return cleanup(KT_EHIP, "end capture", e);
         if (le != hipSuccess) return cleanup(KT_EHIP, "captured launch", le);

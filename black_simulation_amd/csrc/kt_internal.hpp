@@ -36,6 +36,26 @@ struct TraceLaunch {
 
 constexpr int kQueueBuckets = 3;
 
+// A fallback queue in device memory: kQueueBuckets regions `stride` entries apart, region b
+// holding count[b] ray indices; count[kQueueBuckets] is a spare counter (the surrogate's
+// tile counter) and *total the sum of the regions as the integrator reads it.
+struct RayQueue {
+    long long* list;
+    unsigned long long* count;
+    long long* total;
+    long long stride;
+};
+
+// The closed-form solution (kt_closed.hpp) over the launch: polish = 0 answers every ray it
+// covers and queues the others (region 0); polish = 1 runs after the surrogate's network on
+// the rays it answered as escaped or disk hits, replaces their values and re-queues (in the
+// region of the network's class) those it declines or classes otherwise.
+hipError_t launch_closed_form(const TraceLaunch& L, const RayQueue& q, int polish, hipStream_t stream);
+// Every ray by the closed form, then the integrator (L's precision) over the queue it leaves:
+// zeroes the queue and counters, and writes the queue's length to stats[kStatFallback].
+// ev[0..2] (optional) bracket the closed form and the integrator.
+hipError_t trace_closed_form(const TraceLaunch& L, const RayQueue& q, hipStream_t stream, hipEvent_t* ev);
+
 // launches the counter reset (unless L.no_reset) and the trace kernel on `stream`; returns a
 // hipError_t
 hipError_t launch_trace(const TraceLaunch& L, hipStream_t stream);

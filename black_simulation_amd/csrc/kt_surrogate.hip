@@ -486,6 +486,9 @@ struct FusedArgs {
     // outputs: raw network outputs (n x 8), or the gate's decisions
     float* y;
     float gate_prob, gate_ring;
+    // 1: the closed form verifies and replaces what the network answers as escaped or disk
+    // hits, so those pass the gate whatever their probability and ring distance
+    int verify;
     float extent;          // answered rays have |alpha|, |beta| <= extent x 1.5 R_sh
     ktd::Outputs out;
     long long* fallback;             // kti::kQueueBuckets regions fallback_stride apart
@@ -531,8 +534,9 @@ __device__ __forceinline__ f32x16 mfma32(const uint4& a, const bf16x8& b, const 
 // The gate and the answered rays' outputs, from the network outputs y of ray q (its owner
 // lane decides and writes; lanes that hold a ray twice pass owner only on one): a ray is
 // answered when its largest class probability exceeds gate_prob and it lies outside
-// gate_ring of the critical curve (rg < 0: not coverable); the others join the fallback
-// queue, one atomic per wave.  Every lane of the wave calls it.
+// gate_ring of the critical curve (rg < 0: not coverable), or (verify) when its class is
+// escaped or disk; the others join the fallback queue, one atomic per wave.  Every lane of
+// the wave calls it.
 __device__ __forceinline__ void gate_and_write(const FusedArgs& a, long long q, bool live, bool owner, int lane,
                                                const float y[8], float rg) {
     const float mx = fmaxf(y[0], fmaxf(y[1], y[2]));
@@ -541,7 +545,8 @@ __device__ __forceinline__ void gate_and_write(const FusedArgs& a, long long q, 
     const float p0 = e0 * inv, p1 = e1 * inv, p2 = e2 * inv;
     const int c = p0 >= p1 && p0 >= p2 ? 0 : (p1 >= p2 ? 1 : 2);
     const float pm = c == 0 ? p0 : c == 1 ? p1 : p2;
-    const bool to_fallback = live && owner && (!(pm > a.gate_prob) || !(rg > a.gate_ring));
+    const bool verified = a.verify && (c == 0 || c == 2) && rg >= 0.0f;
+    const bool to_fallback = live && owner && !verified && (!(pm > a.gate_prob) || !(rg > a.gate_ring));
     // the fallback queue in one region per predicted class, so that a wave of the integrator
     // takes rays of like length (escaped rays near the ring run longest, disk rays shortest):
     // one atomic per wave and region
@@ -1894,7 +1899,7 @@ void queue_release(Queue& q) {
 }
 
 hipError_t trace_with_surrogate(Surrogate* s, const kti::TraceLaunch& L, double gate_prob, double gate_ring,
-                                Queue* qu, hipStream_t st, hipEvent_t* ev) {
+                                int closed_form, Queue* qu, hipStream_t st, hipEvent_t* ev) {
     if (!s->have_params) return hipErrorNotReady;
     const long long n = L.total;
     Queue& Q = qu ? *qu : s->queue;
@@ -1913,6 +1918,7 @@ hipError_t trace_with_surrogate(Surrogate* s, const kti::TraceLaunch& L, double 
     a.n_views = L.n_views;
     a.gate_prob = (float)gate_prob;
     a.gate_ring = (float)gate_ring;
+    a.verify = closed_form;
     a.extent = (float)(std::min(s->dom.extent, 1.5) / 1.5);
     a.out = L.out;
     a.fallback = Q.list;
@@ -1921,6 +1927,10 @@ hipError_t trace_with_surrogate(Surrogate* s, const kti::TraceLaunch& L, double 
     a.next_tile = Q.count + kti::kQueueBuckets;  // zeroed above with the fallback counts
     launch_network(s, a, true, st);
     KTS_TRY(hipGetLastError());
+    // the closed form's values for the network's escaped and disk answers: the network
+    // picks the class and the rays the gate trusts, the elliptic integrals give the numbers
+    if (closed_form)
+        KTS_TRY(kti::launch_closed_form(L, kti::RayQueue{Q.list, Q.count, Q.total, Q.cap}, 1, st));
     // the integrator on the queue the gate left, its regions read in order and its length
     // read on the device (the counters were zeroed before the network).  The rays queued
     // are the slowest of the frame (photon ring, class boundaries), so the launch is bound
@@ -1933,6 +1943,7 @@ hipError_t trace_with_surrogate(Surrogate* s, const kti::TraceLaunch& L, double 
     F.bucket_count = Q.count;
     F.bucket_stride = Q.cap;
     F.no_reset = 1;
+    F.mode = 1;
     if (ev) KTS_TRY(hipEventRecord(ev[1], st));
     KTS_TRY(kti::launch_trace(F, st));
     if (ev) KTS_TRY(hipEventRecord(ev[2], st));

@@ -106,12 +106,13 @@ struct Queue {
 hipError_t queue_reserve(Queue& q, long long n);
 void queue_release(Queue& q);
 
-// Surrogate inference over every ray of the batch, the gate, then the integrator over the
-// rays the gate rejects (re-queued through L.ray_list).  q = nullptr uses the surrogate's
+// Surrogate inference over every ray of the batch, the gate, (closed_form) the closed-form
+// values of the rays it answered as escaped or disk hits, then the integrator over the rays
+// the gate rejects (re-queued through L.ray_list).  q = nullptr uses the surrogate's
 // own queue.  ev[1], ev[2] bracket the integrator kernel; ev = nullptr records no events
 // (stream capture), and then q must already hold L.total rays.
 hipError_t trace_with_surrogate(Surrogate* s, const kti::TraceLaunch& L, double gate_prob, double gate_ring,
-                                Queue* q, hipStream_t stream, hipEvent_t* ev);
+                                int closed_form, Queue* q, hipStream_t stream, hipEvent_t* ev);
 
 // critical radius / shadow radius from the table (host), as the kernels see them
 double table_critical_radius(const Surrogate* s, double a, double inc, double psi);

@@ -9,6 +9,7 @@
 //   one-shot:   one lane per ray, no refill (the simple form, kept for comparison).
 // The per-view table is staged in LDS when it fits (<= 64 views).
 #include "kt_internal.hpp"
+#include "kt_closed.hpp"
 
 #include <mutex>
 
@@ -273,6 +274,103 @@ __global__ __launch_bounds__(256) void k_trace_report(kti::TraceLaunch L) {
     if ((threadIdx.x & 63u) == 0u && st) atomicAdd(&L.stats[kti::kStatSteps], st);
 }
 
+// ---- the closed-form solution ---------------------------------------------------------------
+
+// the closed form's inputs for launched ray `la` of view v: lambda, eta and the signs of the
+// photon's motion at the camera; false for what it does not take (observer cameras)
+__device__ __forceinline__ bool closed_ray(const ViewDev& v, const Launch& la, ktc::Ray& ry) {
+    if (v.kind == 3 || la.dark) return false;
+    const Photon<double>& p = la.p;
+    if (!(p.E > 0.0) || la.y0[3] == 0.0 || la.y0[4] == 0.0) return false;
+    const double invE = 1.0 / p.E;
+    ry.a = p.a;
+    ry.E = p.E;
+    ry.lam = p.L * invE;
+    ry.eta = p.Q * invE * invE;
+    ry.r_o = la.y0[0];
+    ry.mu_o = la.y0[1];
+    ry.phi_o = v.kind == 2 ? 0.0 : v.phi_cam;  // Boyer-Lindquist phi at the camera
+    ry.s_r = la.y0[3] < 0.0 ? -1 : 1;
+    ry.s_mu = la.y0[4] < 0.0 ? -1 : 1;
+    ry.r_in = la.r_in;
+    ry.r_out = la.r_out;
+    ry.r_cap = la.r_cap;
+    ry.r_esc = la.r_esc;
+    ry.disk = la.disk;
+    ry.prograde = la.prograde;
+    return true;
+}
+
+// One lane per ray, the grid striding over the launch.  Answered rays get the closed form's
+// outputs (class | kFlagClosedForm, 0 steps); the others join region `region` of the queue,
+// one atomic per wave and region.
+__global__ __launch_bounds__(256) void k_closed_form(kti::TraceLaunch L, kti::RayQueue rq, int polish) {
+    const ViewDev* views = stage_views(L, view_lds);
+    const int lane = (int)(threadIdx.x & 63u);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < L.total; base += stride) {  // block-uniform
+        const long long q = base + threadIdx.x;
+        bool queue = false;
+        int region = 0;
+        if (q < L.total) {
+            int prior = -1;  // polish: the network's class, or skip
+            bool take = true;
+            if (polish) {
+                const int c = L.out.cls[q];
+                prior = c & 7;
+                take = (c & kFlagSurrogate) && (prior == kEscaped || prior == kDisk);
+            }
+            if (take) {
+                const ViewDev& v = views[find_view(views, L.n_views, q)];
+                Launch la;
+                launch_photon(v, q - v.offset, la);
+                ktc::Ray ry;
+                ktc::Outcome o{ktc::kDeclined, 0.0, 0.0, 0.0, 0.0};
+                if (closed_ray(v, la, ry)) o = ktc::solve(ry);
+                if (o.cls < 0 || (polish && o.cls != prior)) {
+                    queue = true;
+                    region = polish ? prior : 0;
+                } else {
+                    Result r;
+                    r.cls = o.cls;
+                    r.flags = 0;
+                    r.theta = o.theta;
+                    r.phi = o.phi;
+                    r.r_hit = o.r_hit;
+                    r.g = o.g;
+                    write_result(L.out, q, r, 0, kFlagClosedForm | (polish ? kFlagSurrogate : 0));
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < kti::kQueueBuckets; ++b) {
+            const unsigned long long mask = __ballot(queue && region == b);
+            if (mask) {
+                const int first = __ffsll((long long)mask) - 1;
+                unsigned long long at = 0;
+                if (lane == first) at = atomicAdd(rq.count + b, (unsigned long long)__popcll(mask));
+                at = __shfl(at, first);
+                if (queue && region == b)
+                    rq.list[b * rq.stride + (long long)(at + __popcll(mask & ((1ull << lane) - 1ull)))] = q;
+            }
+        }
+    }
+}
+
+__global__ void k_zero_queue(unsigned long long* count) {
+    if ((int)threadIdx.x <= kti::kQueueBuckets) count[threadIdx.x] = 0ull;
+}
+
+// the queue's length (the sum of its regions) for the integrator and the stats
+__global__ void k_queue_total(const unsigned long long* count, long long* total, unsigned long long* stat) {
+    if (threadIdx.x == 0) {
+        unsigned long long n = 0;
+        for (int b = 0; b < kti::kQueueBuckets; ++b) n += count[b];
+        *total = (long long)n;
+        if (stat) *stat = n;
+    }
+}
+
 }  // namespace
 
 // ---- the scratch of the entry point that allocates nothing (kt_trace_views_noalloc) --------
@@ -343,6 +441,34 @@ void launch_oneshot(const TraceLaunch& L, hipStream_t stream) {
 
 hipError_t reset_counters(const TraceLaunch& L, hipStream_t stream) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, L.counter, L.stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_closed_form(const TraceLaunch& L, const RayQueue& q, int polish, hipStream_t stream) {
+    if (L.total <= 0) return hipSuccess;
+    // one lane per ray up to 64 waves per CU's worth of blocks, striding beyond
+    const long long blocks = std::min<long long>((L.total + 255) / 256, 2048);
+    k_closed_form<<<dim3((unsigned)blocks), dim3(256), view_lds_bytes(L.n_views), stream>>>(L, q, polish);
+    return hipGetLastError();
+}
+
+hipError_t trace_closed_form(const TraceLaunch& L, const RayQueue& q, hipStream_t stream, hipEvent_t* ev) {
+    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, stream, q.count);
+    hipError_t e = reset_counters(L, stream);
+    if (e != hipSuccess) return e;
+    if (ev && (e = hipEventRecord(ev[0], stream)) != hipSuccess) return e;
+    if ((e = launch_closed_form(L, q, 0, stream)) != hipSuccess) return e;
+    if (ev && (e = hipEventRecord(ev[1], stream)) != hipSuccess) return e;
+    TraceLaunch F = L;
+    F.ray_list = q.list;
+    F.total_dev = nullptr;
+    F.bucket_count = q.count;
+    F.bucket_stride = q.stride;
+    F.no_reset = 1;
+    F.mode = 1;  // the one-shot form over the queue: its grid is capped near the usual queue length
+    if ((e = launch_trace(F, stream)) != hipSuccess) return e;
+    if (ev && (e = hipEventRecord(ev[2], stream)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_queue_total, dim3(1), dim3(64), 0, stream, q.count, q.total, L.stats + kStatFallback);
     return hipGetLastError();
 }
 

@@ -61,18 +61,18 @@ def _sky(theta, phi):
     return st * phi.cos(), st * phi.sin(), theta.cos()
 
 
-def errors(out: dict, ref: dict) -> dict:
+def errors(out: dict, ref: dict, flags: int = K.FLAG_SURROGATE) -> dict:
     """Per-ray comparison of `out` with `ref` (device tensors of one frame).
 
-    Returns device tensors: `agree` (class match, bool per ray), `answered` (ray carries the
-    surrogate flag), and for rays whose class matches: `defl` (angle between the two sky
-    directions, rad) and `defl_answered` for escaped rays, `g_rel` / `r_rel` (relative
-    errors of g and r_hit) and `*_answered` for disk rays.  `counts` (host ints) holds, per
-    class of the reference, its rays, how many of them the surrogate answered and how many
-    of those it gave the reference's class."""
+    Returns device tensors: `agree` (class match, bool per ray), `answered` (ray carries one
+    of `flags`: by default the surrogate's; the closed form's for its path), and for rays
+    whose class matches: `defl` (angle between the two sky directions, rad) and
+    `defl_answered` for escaped rays, `g_rel` / `r_rel` (relative errors of g and r_hit) and
+    `*_answered` for disk rays.  `counts` (host ints) holds, per class of the reference, its
+    rays, how many of them were answered and how many of those got the reference's class."""
     c = out["cls"].to(dtype=ref["cls"].dtype) & 7
     cr = ref["cls"] & 7
-    ans = (out["cls"].to(dtype=ref["cls"].dtype) & K.FLAG_SURROGATE) != 0
+    ans = (out["cls"].to(dtype=ref["cls"].dtype) & flags) != 0
     agree = c == cr
     esc = agree & (cr == K.ESCAPED)
     x1, y1, z1 = _sky(out["theta_inf"][esc], out["phi_inf"][esc])

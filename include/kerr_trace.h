@@ -66,6 +66,7 @@ enum {
 };
 /* flag bits OR-ed into cls */
 enum {
+    KT_FLAG_CLOSED_FORM = 0x08,  /* answered by the closed-form solution (0 steps) */
     KT_FLAG_FP64_RESTEP = 0x10,  /* mixed path: at least one FP64 re-step          */
     KT_FLAG_SURROGATE = 0x20,    /* answered by the learned surrogate              */
     KT_FLAG_NONFINITE = 0x40     /* the state went non-finite: counted as captured */
@@ -133,6 +134,17 @@ typedef struct kt_trace_opts {
      * render runs slice by slice, synchronising the stream after each.  NULL: not
      * cancellable.  The other calls ignore it. */
     const volatile int* cancel;
+    /* 1: the closed-form solution of Carter's equations (Legendre elliptic integrals of the
+     * Mino-time quadratures, FP64) answers the rays of Bardeen planes, pinhole (static ZAMO)
+     * cameras and explicit rays that it covers -- class, disk radius and g, escaping rays'
+     * sky angles -- with 0 steps and KT_FLAG_CLOSED_FORM; the rays it declines (an escape
+     * through a polar vortical orbit or through the pole, a turning point at the capture
+     * radius, a crossing within 1e-9 of a disk edge, outgoing rays with no turning point) go
+     * to the integrator at `precision`.  With use_surrogate, the network's gate runs first
+     * and the closed form replaces the values of the rays it answered as escaped or disk
+     * hits (re-queueing those it declines or classes otherwise).  Not for observer cameras
+     * or the ray report.  0: off (the default). */
+    int closed_form;
 } kt_trace_opts;
 
 /* SoA outputs; any pointer may be NULL.  on_device: 1 = device pointers
@@ -176,10 +188,10 @@ typedef struct kt_device_opts {
 typedef struct kt_stats {
     double kernel_ms;          /* device time of the last trace call's kernels (HIP events) */
     double integrator_ms;      /* ... of its integrator kernel alone                        */
-    double surrogate_ms;       /* ... of its surrogate inference kernels                    */
+    double surrogate_ms;       /* ... of its surrogate inference or closed-form kernels     */
     long long rays;            /* rays traced                                               */
     long long steps;           /* DP5 attempts, summed over rays                            */
-    long long fallback_rays;   /* surrogate path: rays re-queued to the integrator          */
+    long long fallback_rays;   /* surrogate / closed form: rays re-queued to the integrator */
     long long restep_rays;     /* mixed path: rays with at least one FP64 step              */
     long long fp64_steps;      /* mixed path: DP5 attempts taken in FP64                     */
     int block, waves_per_cu, steps_per_dequeue, grid, waves_per_simd, mode;
