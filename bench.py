@@ -66,7 +66,7 @@ def profiled_traffic(workload, size, kernel):
 # FP64 operations per ray of the closed-form solution (kt_closed.hpp) over the rays of the
 # 1080p orbit frames, counted by the closed form's Python restatement (tests/closed_form.py,
 # `ops`: + - * / sqrt once, FMA twice, each iteration of the special functions as run)
-CLOSED_FORM_OPS_PER_RAY = 281
+CLOSED_FORM_OPS_PER_RAY = 291
 # The network 5-256-256-256-256-8: 2 (5*256 + 3*256*256 + 256*8) multiply-adds per ray
 def mlp_flop_per_ray(hidden: int) -> int:
     """multiply-adds x 2 of the network 5 -> H -> H -> H -> H -> 8: 399,872 at H = 256 (the

@@ -134,8 +134,8 @@ KTC_FN void qsincos(double u, double& s, double& c) {
 // ---- special functions -------------------------------------------------------------------
 
 // R_F(x, y, z), x, y, z >= 0, at most one zero: duplication until the arguments agree to
-// 6e-3 of their mean, then the fifth-order series (truncation ~5e-15 relative)
-constexpr double kDupTol = 6e-3;
+// 3e-2 of their mean, then the seventh-order series (DLMF 19.36.1; truncation ~1e-14)
+constexpr double kDupTol = 3e-2;
 
 KTC_CALL double rf(double x, double y, double z) {
     double A = (x + y + z) * (1.0 / 3.0);
@@ -151,7 +151,9 @@ KTC_CALL double rf(double x, double y, double z) {
     const double inv = qrcp(A);
     const double X = (A - x) * inv, Y = (A - y) * inv, Z = -X - Y;
     const double E2 = X * Y - Z * Z, E3 = X * Y * Z;
-    return (1.0 - E2 * (1.0 / 10.0) + E3 * (1.0 / 14.0) + E2 * E2 * (1.0 / 24.0) - E2 * E3 * (3.0 / 44.0)) * qrsqrt(A);
+    return (1.0 - E2 * (1.0 / 10.0) + E3 * (1.0 / 14.0) + E2 * E2 * (1.0 / 24.0) - E2 * E3 * (3.0 / 44.0) -
+            E2 * E2 * E2 * (5.0 / 208.0) + E3 * E3 * (3.0 / 104.0) + E2 * E2 * E3 * (1.0 / 16.0)) *
+           qrsqrt(A);
 }
 
 // R_C(x, y), y > 0: the elementary closed forms, written so that x ~ y loses nothing
@@ -169,12 +171,14 @@ KTC_FN double rc(double x, double y) {
     return 1.0 / sqrt(x);
 }
 
-// R_J(x, y, z, p), p > 0: duplication with the R_C terms, then the series
+// R_J(x, y, z, p), p > 0: duplication with the R_C terms, then the fifth-order series
+constexpr double kDupTolJ = 6e-3;
+
 KTC_CALL double rj(double x, double y, double z, double p) {
     double s = 0.0, fac = 1.0;
     double A = (x + y + z + 2.0 * p) * 0.2;
     for (int it = 0; it < 24; ++it) {
-        if (fmax(fmax(fabs(A - x), fabs(A - y)), fmax(fabs(A - z), fabs(A - p))) < kDupTol * A) break;
+        if (fmax(fmax(fabs(A - x), fabs(A - y)), fmax(fabs(A - z), fabs(A - p))) < kDupTolJ * A) break;
         const double sx = qsqrt(x), sy = qsqrt(y), sz = qsqrt(z);
         const double lam = sx * (sy + sz) + sy * sz;
         double al = p * (sx + sy + sz) + sx * sy * sz;
@@ -212,40 +216,48 @@ KTC_FN double ell_pi_complete(double n, double m) {
     return rf(0.0, 1.0 - m, 1.0) + n * (1.0 / 3.0) * rj(0.0, 1.0 - m, 1.0, 1.0 - n);
 }
 
-// K(m), m < 1: pi / (2 AGM(1, sqrt(1 - m))), the mean converging quadratically
-KTC_CALL double ell_k(double m) {
-    double a = 1.0, b = qsqrt(1.0 - m);
-    for (int it = 0; it < 12; ++it) {
-        if (fabs(a - b) <= 1e-15 * a) break;
-        const double an = 0.5 * (a + b);
-        b = qsqrt(a * b);
-        a = an;
-    }
-    return kPiC * qrcp(a + b);
-}
+// The arithmetic-geometric mean of (1, k'), k'^2 = 1 - m (m < 1), kept level by level:
+//   a_{j+1} = (a_j + b_j) / 2,  b_{j+1} = sqrt(a_j b_j),  c_{j+1} = (a_j - b_j) / 2,
+// until |a - b| / a < 1e-9 (at most seven levels from m = 1 - 1e-12; m < 0 is K's only).  K(m) = pi / (a + b)
+// at the end.  Jacobi's functions at any argument u follow from it by descending Landen
+// transformations (Abramowitz & Stegun 16.12): with k_{j+1} = c_{j+1} / a_{j+1}, the
+// argument u a_N at the bottom level, whose parameter (1 - b/a)(1 + b/a) is below 2e-9,
+// takes the first-order small-parameter form there, and each level up is
+//   sn_j = (a + c) sn / (a + c sn^2),  cn_j = a cn dn / (a + c sn^2),
+//   dn_j = (a - c sn^2) / (a + c sn^2)          (a, c of level j + 1).
+// One modulus serves all the equatorial crossings of a ray.
+struct Agm {
+    static constexpr int kLevels = 7;
+    double a[kLevels], c[kLevels];
+    int n;
+    double aN, bN;
+};
 
-// Jacobi sn, cn, dn (u | m), 0 <= m < 1: descending Landen transformations
-//   k_{j+1} = (1 - k'_j) / (1 + k'_j),  u_{j+1} = u_j / (1 + k_{j+1}) = u_j (1 + k'_j) / 2,
-//   k'_{j+1} = 2 sqrt(k'_j) / (1 + k'_j),
-// until 1 - k' < 1e-9 (at most seven levels from m = 1 - 1e-12), the first-order
-// small-modulus form there, then back up
-//   sn_j = (1 + k) sn / (1 + k sn^2), cn_j = cn dn / (1 + k sn^2), dn_j = (1 - k sn^2) / (1 + k sn^2)
-KTC_CALL void sncndn(double u, double m, double& sn, double& cn, double& dn) {
-    constexpr int kLevels = 7;
-    double ks[kLevels];
-    double kp = qsqrt(1.0 - m);
-    int n = 0;
+KTC_FN void agm_init(Agm& G, double m) {
+    double a = 1.0, b = qsqrt(1.0 - m);
+    G.n = 0;
 #pragma unroll
-    for (int j = 0; j < kLevels; ++j) {
-        ks[j] = 0.0;
-        if (kp < 1.0 - 1e-9) {
-            const double r = qrcp(1.0 + kp);
-            ks[j] = (1.0 - kp) * r;
-            u = u * (0.5 * (1.0 + kp));
-            kp = 2.0 * qsqrt(kp) * r;
-            n = j + 1;
+    for (int j = 0; j < Agm::kLevels; ++j) {
+        G.a[j] = 1.0;
+        G.c[j] = 0.0;
+        if (fabs(a - b) > 1e-9 * a) {
+            const double an = 0.5 * (a + b), cn = 0.5 * (a - b);
+            b = qsqrt(a * b);
+            a = an;
+            G.a[j] = a;
+            G.c[j] = cn;
+            G.n = j + 1;
         }
     }
+    G.aN = a;
+    G.bN = b;
+}
+
+KTC_FN double agm_k(const Agm& G) { return kPiC * qrcp(G.aN + G.bN); }
+
+KTC_FN void agm_sncndn(const Agm& G, double u, double& sn, double& cn, double& dn) {
+    u *= G.aN;
+    const double kp = G.bN * qrcp(G.aN);
     const double mn = (1.0 - kp) * (1.0 + kp);
     double s, c;
     qsincos(u, s, c);
@@ -255,12 +267,12 @@ KTC_CALL void sncndn(double u, double m, double& sn, double& cn, double& dn) {
     c = c1;
     double dd = qsqrt(1.0 - mn * s * s);
 #pragma unroll
-    for (int j = kLevels - 1; j >= 0; --j) {
-        if (j < n) {
-            const double k1 = ks[j];
-            const double ks2 = k1 * s * s;
-            const double inv = qrcp(1.0 + ks2);
-            const double ns = (1.0 + k1) * s * inv, nc = c * dd * inv, nd = (1.0 - ks2) * inv;
+    for (int j = Agm::kLevels - 1; j >= 0; --j) {
+        if (j < G.n) {
+            const double a = G.a[j], cj = G.c[j];
+            const double cs2 = cj * s * s;
+            const double inv = qrcp(a + cs2);
+            const double ns = (a + cj) * s * inv, nc = a * c * dd * inv, nd = (a - cs2) * inv;
             s = ns;
             c = nc;
             dd = nd;
@@ -271,11 +283,27 @@ KTC_CALL void sncndn(double u, double m, double& sn, double& cn, double& dn) {
     dn = dd;
 }
 
+// K(m), m < 1
+KTC_CALL double ell_k(double m) {
+    Agm G;
+    agm_init(G, m);
+    return agm_k(G);
+}
+
+// Jacobi sn, cn, dn (u | m), 0 <= m < 1, for a single argument
+KTC_CALL void sncndn(double u, double m, double& sn, double& cn, double& dn) {
+    Agm G;
+    agm_init(G, m);
+    agm_sncndn(G, u, sn, cn, dn);
+}
+
 // the largest real root of the resolvent cubic xi^3 + A xi^2 + (A^2/4 - C) xi - B^2/8 of
-// the depressed quartic r^4 + A r^2 + B r + C (positive when B != 0), Cardano or the
-// trigonometric form, then two Newton steps
-KTC_CALL double resolvent_root(double A, double B, double C) {
-    const double c2 = A, c1 = 0.25 * A * A - C, c0 = -0.125 * B * B;
+// the depressed quartic r^4 + A r^2 + B r + C (positive when B != 0): Cardano's or the
+// trigonometric form, then three Newton steps.  Their cube root, arccosine and cosine run
+// in FP32 (an estimate good to ~1e-6, which the Newton steps make exact) unless the
+// result is not positive or leaves a residual, as where the cubic's roots crowd together
+// (B ~ 0); then in FP64.
+KTC_FN double resolvent_from(double c2, double c1, double c0, bool fp64) {
     const double p = c1 - c2 * c2 * (1.0 / 3.0);
     const double q = 2.0 * c2 * c2 * c2 * (1.0 / 27.0) - c2 * c1 * (1.0 / 3.0) + c0;
     const double hq = 0.5 * q, tp = p * (1.0 / 3.0);
@@ -283,20 +311,30 @@ KTC_CALL double resolvent_root(double A, double B, double C) {
     double t;
     if (disc > 0.0) {
         const double sd = sqrt(disc);
-        const double u = cbrt(hq < 0.0 ? -hq + sd : -hq - sd);
+        const double w = hq < 0.0 ? -hq + sd : -hq - sd;
+        const double u = fp64 || !(fabs(w) < 1e30) ? cbrt(w) : (double)cbrtf((float)w);
         t = u != 0.0 ? u - tp / u : 0.0;
     } else {
         const double r = sqrt(-tp);
         double ca = -hq / (r * r * r);
         ca = fmin(1.0, fmax(-1.0, ca));
-        t = 2.0 * r * cos(acos(ca) * (1.0 / 3.0));
+        t = 2.0 * r * (fp64 ? cos(acos(ca) * (1.0 / 3.0)) : (double)cosf(acosf((float)ca) * (1.0f / 3.0f)));
     }
     double xi = t - c2 * (1.0 / 3.0);
-    for (int it = 0; it < 2; ++it) {
+    for (int it = 0; it < 3; ++it) {
         const double f = ((xi + c2) * xi + c1) * xi + c0;
         const double df = (3.0 * xi + 2.0 * c2) * xi + c1;
         if (df != 0.0) xi -= f / df;
     }
+    return xi;
+}
+
+KTC_CALL double resolvent_root(double A, double B, double C) {
+    const double c2 = A, c1 = 0.25 * A * A - C, c0 = -0.125 * B * B;
+    double xi = resolvent_from(c2, c1, c0, false);
+    const double f = ((xi + c2) * xi + c1) * xi + c0;
+    const double scale = fabs(xi * xi * xi) + fabs(c2 * xi * xi) + fabs(c1 * xi) + fabs(c0);
+    if (!(xi > 0.0) || !(fabs(f) <= 1e-12 * scale)) xi = resolvent_from(c2, c1, c0, true);
     return xi;
 }
 
@@ -417,26 +455,6 @@ KTC_CALL bool escape_sky(const Ray& ry, const Polar& P, double S_tot, double r1,
     return theta == theta && phi == phi;
 }
 
-// The disk test at the crossings s, s + ds, ... before s_end: the radius there from
-// r_at(s), the first one inside [r_in, r_out] a hit.  Returns 1 (hit: rh, g), 0 (none before
-// s_end, or the ray is below r_cap by then) or -1 (declined).
-template <class RAt>
-KTC_FN int crossings(const Ray& ry, double s, double ds, double s_end, RAt r_at, double& rh, double& g) {
-    for (int j = 0; j < kMaxCrossings; ++j) {
-        if (!(s < s_end)) return 0;
-        const double rc = r_at(s);
-        if (!(rc == rc)) return -1;
-        if (rc < ry.r_cap) return 0;  // inward past the capture radius: later crossings lie deeper
-        if (fabs(rc - ry.r_in) < kEdgeBand * ry.r_in || fabs(rc - ry.r_out) < kEdgeBand * ry.r_out) return -1;
-        if (rc >= ry.r_in && rc <= ry.r_out && disk_g(ry.a, rc, ry.prograde, ry.E, ry.lam * ry.E, g)) {
-            rh = rc;
-            return 1;
-        }
-        s += ds;
-    }
-    return -1;
-}
-
 KTC_FN Outcome solve(const Ray& ry) {
     Outcome out{kDeclined, 0.0, 0.0, 0.0, 0.0};
     const double a = ry.a, a2 = a * a, lam = ry.lam, eta = ry.eta;
@@ -451,109 +469,119 @@ KTC_FN Outcome solve(const Ray& ry) {
     const double D12 = -0.5 * A - z * z + B / (4.0 * z);
     const double D34 = -0.5 * A - z * z - B / (4.0 * z);
     const bool cross = eta > 0.0 && ry.disk;  // eta <= 0: the ray never reaches the equator
-    Polar P{};
-    if (eta > 0.0) P = polar_motion(ry, A);
-    const double ds = eta > 0.0 ? 2.0 * P.K / P.om : 0.0;
 
+    // ---- radial motion.  J, the Mino time from the largest real root rt of R (r4 of four
+    // real roots, r2 of two real and a complex pair) up to r, is pref F(phi(r) | k), and r at
+    // J is the Moebius form (P0 + P1 w) / (Q0 + Q1 w) in w = sn^2(J / pref | k) (four real
+    // roots: sin^2 phi = r31 (r - r4) / (r41 (r - r3))) or w = cn(J / pref | k) (a complex
+    // pair: cos phi = (Aq (r - r1) - Bq (r - r2)) / (Aq (r - r1) + Bq (r - r2)),
+    // Aq = |r3 - r2|, Bq = |r3 - r1|).  S_t: J at the camera, signed so that the ray's own
+    // Mino time s reaches rt at s = S_t (S_t < 0: rt lies behind an outward start).
+    bool real4, turning = false;
+    double pref, k, P0, P1, Q0, Q1, S_t, r1 = 0.0, r3 = 0.0, r4 = 0.0, x_o = 0.0, x_inf = 0.0;
+    Agm G;  // the radial modulus's mean (for the crossings)
     if (D34 >= 0.0) {
-        // four real roots; from r4 up to r the Mino time is pref F(x(r) | k),
-        // x(r)^2 = r31 (r - r4) / (r41 (r - r3)), and r at Mino time J from r4 is
-        // (r31 r4 - x^2 r41 r3) / (r31 - x^2 r41) with x = sn(J / pref | k)
+        real4 = true;
         const double sD12 = sqrt(fmax(D12, 0.0)), sD34 = sqrt(D34);
-        const double r1 = -z - sD12, r2 = -z + sD12, r3 = z - sD34, r4 = z + sD34;
+        r1 = -z - sD12;
+        const double r2 = -z + sD12;
+        r3 = z - sD34;
+        r4 = z + sD34;
         const double r31 = r3 - r1, r41 = r4 - r1, r42 = r4 - r2, r32 = r3 - r2;
-        const double k = r32 * r41 / (r31 * r42);
-        const double pref = 2.0 / sqrt(r31 * r42);
-        const bool turning = r4 > ry.r_cap;
+        k = r32 * r41 / (r31 * r42);
+        pref = 2.0 / sqrt(r31 * r42);
+        turning = r4 > ry.r_cap;
         if (turning && (r4 < ry.r_cap * (1.0 + 1e-7) || !(r4 < ry.r_o))) return out;
         if (r4 == ry.r_cap) return out;
         if (ry.s_r > 0 && !turning) return out;  // outward from below a turning point: not covered
-        const double x_o = sqrt(r31 * (ry.r_o - r4) / (r41 * (ry.r_o - r3)));
+        P0 = r31 * r4;
+        P1 = -r41 * r3;
+        Q0 = r31;
+        Q1 = -r41;
+        x_o = sqrt(r31 * (ry.r_o - r4) / (r41 * (ry.r_o - r3)));
+        x_inf = sqrt(r31 / r41);
         const double I_o = pref * ell_f(x_o, k);
-        const double x_inf = sqrt(r31 / r41);
-        // the turning point lies S_t ahead of the camera (behind it, S_t < 0, outward start);
-        // a ray that does not turn falls from r_o through r_cap before reaching r4
-        const double S_t = ry.s_r < 0 ? I_o : -I_o;
-        auto r_at = [&](double s) {
-            const double J = s < S_t ? S_t - s : s - S_t;  // Mino time from r4
-            double sn, cn, dn;
-            sncndn(J / pref, k, sn, cn, dn);
-            const double x2 = sn * sn;
-            return (r31 * r4 - x2 * r41 * r3) / (r31 - x2 * r41);
-        };
-        // the crossings on the inward leg first; the escape time is needed only once a
-        // crossing comes after the turning point, or for the sky
-        double I_inf = -1.0;
-        int hit = 0;
-        double rh = 0.0, g = 0.0;
-        if (cross) {
-            hit = crossings(ry, P.s1, ds, S_t, r_at, rh, g);
-            if (hit == 0 && turning) {
-                I_inf = pref * ell_f(x_inf, k);
-                double s = P.s1;
-                if (s < S_t) s += ds * ceil((S_t - s) / ds);  // the first crossing after the turn
-                hit = crossings(ry, s, ds, S_t + I_inf, r_at, rh, g);
-            }
-        }
-        if (hit < 0) return out;
-        if (hit > 0) {
-            out.cls = 2;
-            out.r_hit = rh;
-            out.g = g;
-            return out;
-        }
-        if (!turning) {
+        S_t = ry.s_r < 0 ? I_o : -I_o;
+        if (cross) agm_init(G, k);
+    } else {
+        if (ry.s_r > 0) return out;  // no turning point: an outgoing ray escapes by forms not given here
+        if (!(D12 >= 0.0)) {
+            // all four roots complex: no turning point, and (eta < 0) no equatorial crossing
+            if (cross) return out;
             out.cls = 1;
             return out;
         }
-        if (!(eta > 0.0) || lam == 0.0) return out;  // vortical or through the pole: the integrator
-        if (I_inf < 0.0) I_inf = pref * ell_f(x_inf, k);
-        double th, ph;
-        if (!escape_sky(ry, P, S_t + I_inf, r1, r3, r4, k, pref, x_o, x_inf, th, ph)) return out;
-        out.cls = 0;
-        out.theta = th;
-        out.phi = ph;
-        return out;
-    }
-    if (ry.s_r > 0) return out;  // no turning point: an outgoing ray escapes by forms not given here
-    if (D12 >= 0.0) {
-        // r1 <= r2 real, r3 = conj(r4): no turning point outside the horizon.  From r2 up to r
-        // the Mino time is pref F(acos c(r) | k3), c(r) = (Aq (r - r1) - Bq (r - r2)) /
-        // (Aq (r - r1) + Bq (r - r2)), and r at Mino time J from r2 follows from c = cn(J / pref | k3)
+        real4 = false;
         const double sD12 = sqrt(D12);
-        const double r1 = -z - sD12, r2 = -z + sD12;
+        r1 = -z - sD12;
+        const double r2 = -z + sD12;
         if (!(r2 < ry.r_cap)) return out;
-        if (cross) {
-            const double im3 = sqrt(-D34);
-            const double Aq = hypot(z - r2, im3), Bq = hypot(z - r1, im3);
-            const double r21 = r2 - r1;
-            const double k3 = ((Aq + Bq) * (Aq + Bq) - r21 * r21) / (4.0 * Aq * Bq);
-            const double pref = 1.0 / sqrt(Aq * Bq);
-            const double ro = ry.r_o;
-            const double c = (Aq * (ro - r1) - Bq * (ro - r2)) / (Aq * (ro - r1) + Bq * (ro - r2));
-            const double f = ell_f(sqrt(fmax(0.0, (1.0 - c) * (1.0 + c))), k3);
-            const double S_t = pref * (c >= 0.0 ? f : 2.0 * ell_k(k3) - f);  // Mino time down to r2
-            auto r_at = [&](double s) {
-                double sn, cn, dn;
-                sncndn((S_t - s) / pref, k3, sn, cn, dn);
-                return (Aq * r1 * (1.0 - cn) - Bq * r2 * (1.0 + cn)) / (Aq * (1.0 - cn) - Bq * (1.0 + cn));
-            };
-            double rh = 0.0, g = 0.0;
-            const int hit = crossings(ry, P.s1, ds, S_t, r_at, rh, g);
-            if (hit < 0) return out;
-            if (hit > 0) {
+        if (!cross) {
+            out.cls = 1;
+            return out;
+        }
+        const double im3 = sqrt(-D34);
+        const double Aq = hypot(z - r2, im3), Bq = hypot(z - r1, im3);
+        const double r21 = r2 - r1;
+        k = ((Aq + Bq) * (Aq + Bq) - r21 * r21) / (4.0 * Aq * Bq);
+        pref = 1.0 / sqrt(Aq * Bq);
+        P0 = Aq * r1 - Bq * r2;
+        P1 = -(Aq * r1 + Bq * r2);
+        Q0 = Aq - Bq;
+        Q1 = -(Aq + Bq);
+        const double ro = ry.r_o;
+        const double c = (Aq * (ro - r1) - Bq * (ro - r2)) / (Aq * (ro - r1) + Bq * (ro - r2));
+        const double f = ell_f(sqrt(fmax(0.0, (1.0 - c) * (1.0 + c))), k);
+        agm_init(G, k);
+        S_t = pref * (c >= 0.0 ? f : 2.0 * agm_k(G) - f);  // F(acos c) past pi/2
+    }
+
+    // ---- polar motion, and the disk at its equatorial crossings: inward of rt first; for a
+    // turning ray past rt until the escape, whose time is needed only then or for the sky
+    Polar P{};
+    if (eta > 0.0) P = polar_motion(ry, A);
+    double I_inf = -1.0;
+    if (cross) {
+        const double ds = 2.0 * P.K / P.om;
+        double s = P.s1, s_end = S_t;
+        int j = 0;
+        for (; j < kMaxCrossings; ++j) {
+            if (!(s < s_end)) {
+                if (!turning || I_inf >= 0.0) break;
+                I_inf = pref * ell_f(x_inf, k);
+                s_end = S_t + I_inf;
+                if (!(s < s_end)) break;
+            }
+            const double J = s < S_t ? S_t - s : s - S_t;
+            double sn, cn, dn;
+            agm_sncndn(G, J / pref, sn, cn, dn);
+            const double w = real4 ? sn * sn : cn;
+            const double rc = (P0 + P1 * w) / (Q0 + Q1 * w);
+            if (!(rc == rc)) return out;
+            if (rc < ry.r_cap) break;  // inward past the capture radius: later crossings lie deeper
+            if (fabs(rc - ry.r_in) < kEdgeBand * ry.r_in || fabs(rc - ry.r_out) < kEdgeBand * ry.r_out) return out;
+            double g;
+            if (rc >= ry.r_in && rc <= ry.r_out && disk_g(a, rc, ry.prograde, ry.E, lam * ry.E, g)) {
                 out.cls = 2;
-                out.r_hit = rh;
+                out.r_hit = rc;
                 out.g = g;
                 return out;
             }
+            s += ds;
         }
+        if (j == kMaxCrossings) return out;
+    }
+    if (!turning) {
         out.cls = 1;
         return out;
     }
-    // all four roots complex: no turning point, and (eta < 0) no equatorial crossing
-    if (cross) return out;
-    out.cls = 1;
+    if (!(eta > 0.0) || lam == 0.0) return out;  // vortical or through the pole: the integrator
+    if (I_inf < 0.0) I_inf = pref * ell_f(x_inf, k);
+    double th, ph;
+    if (!escape_sky(ry, P, S_t + I_inf, r1, r3, r4, k, pref, x_o, x_inf, th, ph)) return out;
+    out.cls = 0;
+    out.theta = th;
+    out.phi = ph;
     return out;
 }
 

@@ -304,7 +304,8 @@ __device__ __forceinline__ bool closed_ray(const ViewDev& v, const Launch& la, k
 // One lane per ray, the grid striding over the launch.  Answered rays get the closed form's
 // outputs (class | kFlagClosedForm, 0 steps); the others join region `region` of the queue,
 // one atomic per wave and region.
-__global__ __launch_bounds__(256) void k_closed_form(kti::TraceLaunch L, kti::RayQueue rq, int polish) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_closed_form(kti::TraceLaunch L,
+                                                                                              kti::RayQueue rq, int polish) {
     const ViewDev* views = stage_views(L, view_lds);
     const int lane = (int)(threadIdx.x & 63u);
     const long long stride = (long long)gridDim.x * blockDim.x;

@@ -20,7 +20,8 @@ import math
 ESCAPED, CAPTURED, DISK, DECLINED = 0, 1, 2, -1
 EDGE_BAND = 1e-9
 MAX_CROSSINGS = 24
-DUP_TOL = 6e-3
+DUP_TOL = 3e-2
+DUP_TOL_J = 6e-3
 
 
 class Num(float):
@@ -102,7 +103,8 @@ def rf(x, y, z):
     X, Y = (A - x) * inv, (A - y) * inv
     Z = -X - Y
     E2, E3 = X * Y - Z * Z, X * Y * Z
-    return (1.0 - E2 * (1.0 / 10.0) + E3 * (1.0 / 14.0) + E2 * E2 * (1.0 / 24.0) - E2 * E3 * (3.0 / 44.0)) / sqrt(A)
+    return (1.0 - E2 * (1.0 / 10.0) + E3 * (1.0 / 14.0) + E2 * E2 * (1.0 / 24.0) - E2 * E3 * (3.0 / 44.0)
+            - E2 * E2 * E2 * (5.0 / 208.0) + E3 * E3 * (3.0 / 104.0) + E2 * E2 * E3 * (1.0 / 16.0)) / sqrt(A)
 
 
 def rc(x, y):
@@ -123,7 +125,7 @@ def rj(x, y, z, p):
     s, fac = 0.0, 1.0
     A = (x + y + z + 2.0 * p) * 0.2
     for _ in range(24):
-        if max(abs(A - x), abs(A - y), abs(A - z), abs(A - p)) < DUP_TOL * A:
+        if max(abs(A - x), abs(A - y), abs(A - z), abs(A - p)) < DUP_TOL_J * A:
             break
         sx, sy, sz = sqrt(x), sqrt(y), sqrt(z)
         lam = sx * (sy + sz) + sy * sz
@@ -160,39 +162,48 @@ def ell_pi_complete(n, m):
     return rf(0.0, 1.0 - m, 1.0) + n * (1.0 / 3.0) * rj(0.0, 1.0 - m, 1.0, 1.0 - n)
 
 
-def ell_k(m):
+def agm(m):
+    """levels (a_{j+1}, c_{j+1}) of the AGM of (1, sqrt(1 - m)) until (a - b) / a < 1e-9, and (a_N, b_N)"""
     a, b = 1.0, sqrt(1.0 - m)
-    for _ in range(12):
-        if abs(a - b) <= 1e-15 * a:
-            break
-        a, b = 0.5 * (a + b), sqrt(a * b)
-    return math.pi / (a + b)
-
-
-def sncndn(u, m):
-    ks = []
-    kp = sqrt(1.0 - m)
+    levels = []
     for _ in range(7):
-        if not kp < 1.0 - 1e-9:
+        if not abs(a - b) > 1e-9 * a:
             break
-        r = 1.0 / (1.0 + kp)
-        ks.append((1.0 - kp) * r)
-        u = u * (0.5 * (1.0 + kp))
-        kp = 2.0 * sqrt(kp) * r
+        a, b, c = 0.5 * (a + b), sqrt(a * b), 0.5 * (a - b)
+        levels.append((a, c))
+    return levels, a, b
+
+
+def agm_k(G):
+    return math.pi / (G[1] + G[2])
+
+
+def agm_sncndn(G, u):
+    levels, aN, bN = G
+    u = u * aN
+    kp = bN / aN
     mn = (1.0 - kp) * (1.0 + kp)
     s, c = qsincos(u)
     d = 0.25 * mn * (u - s * c)
     s, c = s - d * c, c + d * s
     dd = sqrt(1.0 - mn * s * s)
-    for k1 in reversed(ks):
-        ks2 = k1 * s * s
-        inv = 1.0 / (1.0 + ks2)
-        s, c, dd = (1.0 + k1) * s * inv, c * dd * inv, (1.0 - ks2) * inv
+    for a, cj in reversed(levels):
+        cs2 = cj * s * s
+        inv = 1.0 / (a + cs2)
+        s, c, dd = (a + cj) * s * inv, a * c * dd * inv, (a - cs2) * inv
     return s, c, dd
 
 
-def resolvent_root(A, B, C):
-    c2, c1, c0 = A, 0.25 * A * A - C, -0.125 * B * B
+def ell_k(m):
+    return agm_k(agm(m))
+
+
+def sncndn(u, m):
+    return agm_sncndn(agm(m), u)
+
+
+def _resolvent_from(c2, c1, c0, fp64):
+    import numpy as np
     p = c1 - c2 * c2 * (1.0 / 3.0)
     q = 2.0 * c2 * c2 * c2 * (1.0 / 27.0) - c2 * c1 * (1.0 / 3.0) + c0
     hq, tp = 0.5 * q, p * (1.0 / 3.0)
@@ -200,14 +211,20 @@ def resolvent_root(A, B, C):
     if disc > 0:
         sd = sqrt(disc)
         w = -hq + sd if hq < 0 else -hq - sd
-        u = fn(lambda v: math.copysign(abs(v) ** (1.0 / 3.0), v), w)
+        u = fn(lambda v: math.copysign(abs(v) ** (1 / 3), v) if fp64 or not abs(v) < 1e30
+               else float(np.cbrt(np.float32(v))), w)
         t = u - tp / u if u != 0 else 0.0
     else:
         r = sqrt(-tp)
         ca = min(1.0, max(-1.0, float(-hq / (r * r * r))))
-        t = 2.0 * r * fn(math.cos, fn(math.acos, Num(ca) if isinstance(r, Num) else ca) * (1.0 / 3.0))
+        ca = Num(ca) if isinstance(r, Num) else ca
+        if fp64:
+            t = 2.0 * r * fn(math.cos, fn(math.acos, ca) * (1.0 / 3.0))
+        else:
+            ang = fn(lambda v: float(np.arccos(np.float32(v))), ca)
+            t = 2.0 * r * fn(lambda v: float(np.cos(np.float32(v) * np.float32(1.0 / 3.0))), ang)
     xi = t - c2 * (1.0 / 3.0)
-    for _ in range(2):
+    for _ in range(3):
         f = ((xi + c2) * xi + c1) * xi + c0
         df = (3.0 * xi + 2.0 * c2) * xi + c1
         if df != 0:
@@ -215,7 +232,17 @@ def resolvent_root(A, B, C):
     return xi
 
 
-# ---- one ray ---------------------------------------------------------------------------------
+def resolvent_root(A, B, C):
+    """the largest root of the resolvent cubic: FP32 cube root / arccosine / cosine and three
+    FP64 Newton steps, or all in FP64 where that leaves a root that is not positive or a residual"""
+    c2, c1, c0 = A, 0.25 * A * A - C, -0.125 * B * B
+    xi = _resolvent_from(c2, c1, c0, False)
+    f = ((xi + c2) * xi + c1) * xi + c0
+    scale = abs(xi * xi * xi) + abs(c2 * xi * xi) + abs(c1 * xi) + abs(c0)
+    if not xi > 0 or not abs(f) <= 1e-12 * scale:
+        xi = _resolvent_from(c2, c1, c0, True)
+    return xi
+
 
 def disk_g(a, r, prograde, E, L):
     sg = 1.0 if prograde else -1.0
@@ -226,26 +253,6 @@ def disk_g(a, r, prograde, E, L):
     if not norm > 0:
         return None
     return sqrt(norm) / (E - L * om)
-
-
-def crossings(ray, s, ds, s_end, r_at):
-    """(1, r, g) at the first crossing inside the disk before s_end, (0,) for none, (-1,) declined"""
-    for _ in range(MAX_CROSSINGS):
-        if not s < s_end:
-            return (0,)
-        rc_ = r_at(s)
-        if rc_ != rc_:
-            return (-1,)
-        if rc_ < ray["r_cap"]:
-            return (0,)
-        if abs(rc_ - ray["r_in"]) < EDGE_BAND * ray["r_in"] or abs(rc_ - ray["r_out"]) < EDGE_BAND * ray["r_out"]:
-            return (-1,)
-        if ray["r_in"] <= rc_ <= ray["r_out"]:
-            g = disk_g(ray["a"], rc_, ray["prograde"], ray["E"], ray["lam"] * ray["E"])
-            if g is not None:
-                return (1, rc_, g)
-        s = s + ds
-    return (-1,)
 
 
 def solve(ray):
@@ -268,7 +275,54 @@ def solve(ray):
     D12 = -0.5 * A - z * z + B / (4.0 * z)
     D34 = -0.5 * A - z * z - B / (4.0 * z)
     cross = eta > 0 and ray["disk"]
-    om = up = m = x = Fo = K = s1 = ds = 0.0
+    # radial: J from the largest real root up to r is pref F(phi | k); r(J) = (P0 + P1 w) / (Q0 + Q1 w)
+    turning = False
+    r1 = r3 = r4 = x_o = x_inf = 0.0
+    if D34 >= 0:
+        real4 = True
+        sD12, sD34 = sqrt(max(D12, 0.0)), sqrt(D34)
+        r1, r2, r3, r4 = -z - sD12, -z + sD12, z - sD34, z + sD34
+        r31, r41, r42, r32 = r3 - r1, r4 - r1, r4 - r2, r3 - r2
+        k = r32 * r41 / (r31 * r42)
+        pref = 2.0 / sqrt(r31 * r42)
+        turning = r4 > ray["r_cap"]
+        if turning and (r4 < ray["r_cap"] * (1.0 + 1e-7) or not r4 < ray["r_o"]):
+            return out
+        if r4 == ray["r_cap"] or (ray["s_r"] > 0 and not turning):
+            return out
+        P0, P1, Q0, Q1 = r31 * r4, -r41 * r3, r31, -r41
+        x_o = sqrt(r31 * (ray["r_o"] - r4) / (r41 * (ray["r_o"] - r3)))
+        x_inf = sqrt(r31 / r41)
+        I_o = pref * ell_f(x_o, k)
+        S_t = I_o if ray["s_r"] < 0 else -I_o
+        G = agm(k) if cross else None
+    else:
+        if ray["s_r"] > 0:
+            return out
+        if not D12 >= 0:
+            return out if cross else (CAPTURED, nan, nan, nan, nan)
+        real4 = False
+        sD12 = sqrt(D12)
+        r1, r2 = -z - sD12, -z + sD12
+        if not r2 < ray["r_cap"]:
+            return out
+        if not cross:
+            return (CAPTURED, nan, nan, nan, nan)
+        im3 = sqrt(-D34)
+        Aq = sqrt((z - r2) * (z - r2) + im3 * im3)
+        Bq = sqrt((z - r1) * (z - r1) + im3 * im3)
+        r21 = r2 - r1
+        k = ((Aq + Bq) * (Aq + Bq) - r21 * r21) / (4.0 * Aq * Bq)
+        pref = 1.0 / sqrt(Aq * Bq)
+        P0, P1, Q0, Q1 = Aq * r1 - Bq * r2, -(Aq * r1 + Bq * r2), Aq - Bq, -(Aq + Bq)
+        ro = ray["r_o"]
+        c = (Aq * (ro - r1) - Bq * (ro - r2)) / (Aq * (ro - r1) + Bq * (ro - r2))
+        s2 = (1.0 - c) * (1.0 + c)
+        f = ell_f(sqrt(s2) if s2 > 0 else 0.0, k)
+        G = agm(k)
+        S_t = pref * (f if c >= 0 else 2.0 * agm_k(G) - f)
+    # polar motion and the equatorial crossings
+    om = up = m = x = Fo = K = s1 = 0.0
     if eta > 0:
         disc = sqrt(A * A + 4.0 * a2 * eta)
         om2 = 0.5 * (disc - A) if A <= 0 else 2.0 * a2 * eta / (A + disc)
@@ -286,85 +340,45 @@ def solve(ray):
         if x == 0:
             F1 = 2.0 * K
         s1 = F1 / om
-        ds = 2.0 * K / om
-    if D34 >= 0:
-        sD12, sD34 = sqrt(max(D12, 0.0)), sqrt(D34)
-        r1, r2, r3, r4 = -z - sD12, -z + sD12, z - sD34, z + sD34
-        r31, r41, r42, r32 = r3 - r1, r4 - r1, r4 - r2, r3 - r2
-        k = r32 * r41 / (r31 * r42)
-        pref = 2.0 / sqrt(r31 * r42)
-        turning = r4 > ray["r_cap"]
-        if turning and (r4 < ray["r_cap"] * (1.0 + 1e-7) or not r4 < ray["r_o"]):
-            return out
-        if r4 == ray["r_cap"] or (ray["s_r"] > 0 and not turning):
-            return out
-        x_o = sqrt(r31 * (ray["r_o"] - r4) / (r41 * (ray["r_o"] - r3)))
-        I_o = pref * ell_f(x_o, k)
-        x_inf = sqrt(r31 / r41)
-        S_t = I_o if ray["s_r"] < 0 else -I_o
-
-        def r_at(s):
-            J = S_t - s if s < S_t else s - S_t
-            sn, _, _ = sncndn(J / pref, k)
-            x2 = sn * sn
-            return (r31 * r4 - x2 * r41 * r3) / (r31 - x2 * r41)
-
-        I_inf = None
-        hit = (0,)
-        if cross:
-            hit = crossings(ray, s1, ds, S_t, r_at)
-            if hit[0] == 0 and turning:
-                I_inf = pref * ell_f(x_inf, k)
-                s = s1
-                if s < S_t:
-                    s = s + ds * math.ceil(float((S_t - s) / ds))
-                hit = crossings(ray, s, ds, S_t + I_inf, r_at)
-        if hit[0] < 0:
-            return out
-        if hit[0] > 0:
-            return (DISK, nan, nan, hit[1], hit[2])
-        if not turning:
-            return (CAPTURED, nan, nan, nan, nan)
-        if not eta > 0 or lam == 0:
-            return out
-        if I_inf is None:
-            I_inf = pref * ell_f(x_inf, k)
-        th, ph = escape_sky(ray, om, up, m, x, Fo, K, S_t + I_inf, r1, r3, r4, k, pref, x_o, x_inf)
-        if th != th or ph != ph:
-            return out
-        return (ESCAPED, th, ph, nan, nan)
-    if ray["s_r"] > 0:
-        return out
-    if D12 >= 0:
-        sD12 = sqrt(D12)
-        r1, r2 = -z - sD12, -z + sD12
-        if not r2 < ray["r_cap"]:
-            return out
-        if cross:
-            im3 = sqrt(-D34)
-            Aq = sqrt((z - r2) * (z - r2) + im3 * im3)
-            Bq = sqrt((z - r1) * (z - r1) + im3 * im3)
-            r21 = r2 - r1
-            k3 = ((Aq + Bq) * (Aq + Bq) - r21 * r21) / (4.0 * Aq * Bq)
-            pref = 1.0 / sqrt(Aq * Bq)
-            ro = ray["r_o"]
-            c = (Aq * (ro - r1) - Bq * (ro - r2)) / (Aq * (ro - r1) + Bq * (ro - r2))
-            f = ell_f(sqrt(max((1.0 - c) * (1.0 + c), 0.0)), k3)
-            S_t = pref * (f if c >= 0 else 2.0 * ell_k(k3) - f)
-
-            def r_at(s):
-                _, cn, _ = sncndn((S_t - s) / pref, k3)
-                return (Aq * r1 * (1.0 - cn) - Bq * r2 * (1.0 + cn)) / (Aq * (1.0 - cn) - Bq * (1.0 + cn))
-
-            hit = crossings(ray, s1, ds, S_t, r_at)
-            if hit[0] < 0:
-                return out
-            if hit[0] > 0:
-                return (DISK, nan, nan, hit[1], hit[2])
-        return (CAPTURED, nan, nan, nan, nan)
+    I_inf = None
     if cross:
+        ds = 2.0 * K / om
+        s, s_end = s1, S_t
+        for j in range(MAX_CROSSINGS):
+            if not s < s_end:
+                if not turning or I_inf is not None:
+                    break
+                I_inf = pref * ell_f(x_inf, k)
+                s_end = S_t + I_inf
+                if not s < s_end:
+                    break
+            J = S_t - s if s < S_t else s - S_t
+            sn, cn, _ = agm_sncndn(G, J / pref)
+            w = sn * sn if real4 else cn
+            rc_ = (P0 + P1 * w) / (Q0 + Q1 * w)
+            if rc_ != rc_:
+                return out
+            if rc_ < ray["r_cap"]:
+                break
+            if abs(rc_ - ray["r_in"]) < EDGE_BAND * ray["r_in"] or abs(rc_ - ray["r_out"]) < EDGE_BAND * ray["r_out"]:
+                return out
+            if ray["r_in"] <= rc_ <= ray["r_out"]:
+                g = disk_g(ray["a"], rc_, ray["prograde"], ray["E"], lam * ray["E"])
+                if g is not None:
+                    return (DISK, nan, nan, rc_, g)
+            s = s + ds
+        else:
+            return out
+    if not turning:
+        return (CAPTURED, nan, nan, nan, nan)
+    if not eta > 0 or lam == 0:
         return out
-    return (CAPTURED, nan, nan, nan, nan)
+    if I_inf is None:
+        I_inf = pref * ell_f(x_inf, k)
+    th, ph = escape_sky(ray, om, up, m, x, Fo, K, S_t + I_inf, r1, r3, r4, k, pref, x_o, x_inf)
+    if th != th or ph != ph:
+        return out
+    return (ESCAPED, th, ph, nan, nan)
 
 
 def escape_sky(ray, om, up, m, x, Fo, K, S_tot, r1, r3, r4, k, pref, x_o, x_inf):
