@@ -63,10 +63,11 @@ def profiled_traffic(workload, size, kernel):
         if w == workload and n == size and kernel.startswith(k):
             return v
     return None, None
-# FP64 operations per ray of the closed-form solution (kt_closed.hpp) over the rays of the
-# 1080p orbit frames, counted by the closed form's Python restatement (tests/closed_form.py,
-# `ops`: + - * / sqrt once, FMA twice, each iteration of the special functions as run)
-CLOSED_FORM_OPS_PER_RAY = 291
+# Operations per ray of the closed-form solution (kt_closed.hpp) over the rays of the 1080p
+# orbit frames, counted by its Python restatement (tests/closed_form.py `count_ops`: + - * /
+# sqrt once, FMA twice, each iteration of the special functions as run): the FP64 form, and
+# the FP32 pass with the FP64 one for the rays it leaves unsure
+CLOSED_FORM_OPS_PER_RAY = {"fp64": 305, "fp32": 288}
 # The network 5-256-256-256-256-8: 2 (5*256 + 3*256*256 + 256*8) multiply-adds per ray
 def mlp_flop_per_ray(hidden: int) -> int:
     """multiply-adds x 2 of the network 5 -> H -> H -> H -> H -> 8: 399,872 at H = 256 (the
@@ -340,6 +341,9 @@ def main():
     ap.add_argument("--closed-form", type=int, choices=[0, 1], default=1,
                     help="--path surrogate: 1 = the closed form verifies the network's escaped and disk answers "
                          "and gives their values; 0 = the network's own values")
+    ap.add_argument("--closed-precision", choices=["fp32", "fp64"], default="fp32",
+                    help="--path closed: the elliptic integrals in FP32 (FP64 for the rays FP32 leaves unsure; "
+                         "the default) or in FP64 throughout")
     ap.add_argument("--no-alt", action="store_true",
                     help="--path closed: do not time the learned surrogate's paths beside it")
     ap.add_argument("--workload", choices=["orbit", "c2", "c3", "eht", "infall", "debris"], default="orbit")
@@ -450,11 +454,16 @@ def main():
         # --rtol) the rays it declines
         opts = tune(bsa.TraceOpts(precision=bsa.PREC_FP64, mode=mode,
                                   **({"rtol": args.rtol, "atol": args.rtol / 10} if args.rtol else {})))
-        opts.closed_form = 1
+        opts.closed_form = 2 if args.closed_precision == "fp32" else 1
         if args.workload == "orbit" and not args.no_alt and not args.lean:
+            other = bsa.TraceOpts(**{f: getattr(opts, f) for f in ("precision", "mode", "rtol", "atol", "block",
+                                                                    "waves_per_cu", "steps_per_dequeue",
+                                                                    "waves_per_simd")})
+            other.closed_form = 1 if opts.closed_form == 2 else 2
+            alt = [(f"closed_form_{'fp64' if other.closed_form == 1 else 'fp32'}", other, bsa.FLAG_CLOSED_FORM)]
             load_into(tr, args.weights or FAST_WEIGHTS)
-            alt = [("learned_surrogate_closed_form_values", surrogate_opts(1), bsa.FLAG_SURROGATE),
-                   ("learned_surrogate_network_values", surrogate_opts(0), bsa.FLAG_SURROGATE)]
+            alt += [("learned_surrogate_closed_form_values", surrogate_opts(1), bsa.FLAG_SURROGATE),
+                    ("learned_surrogate_network_values", surrogate_opts(0), bsa.FLAG_SURROGATE)]
     elif args.path == "plain":
         opts = plain
         if args.rtol:  # FP64 at another tolerance, judged against FP64 at rtol 1e-10
@@ -546,7 +555,8 @@ def main():
             alt_acc[name] = {"value": round(avalue, 3), "speedup_vs_plain_fp64": round(avalue / value_p, 3),
                              "frame_ms_p50": round(percentile(alat, 0.5), 3),
                              "gate": {"prob": aopts.gate_prob, "ring": aopts.gate_ring,
-                                      "escaped_and_disk_verified_by_closed_form": bool(aopts.closed_form)},
+                                      "escaped_and_disk_verified_by_closed_form": bool(aopts.closed_form)}
+                             if aopts.use_surrogate else None,
                              "accuracy_vs_fp64": summary(name, avalue)}
     for r in alt_runs:
         r[3].close()
@@ -649,15 +659,20 @@ def main():
         if avg_sur > avg_integ:
             roof, roof_mlp = roof_mlp, roof_integ
     if args.path == "closed":
-        # the closed-form kernel: FP64 operations per ray of its algorithm (CLOSED_FORM_OPS_PER_RAY)
-        # against the FP64 vector peak; the integrator over the rays it declined beside it
-        cf_ach = rays * CLOSED_FORM_OPS_PER_RAY / (avg_sur * 1e-3) / 1e12 if avg_sur > 0 else 0.0
+        # the closed-form kernels: operations per ray of the algorithm (CLOSED_FORM_OPS_PER_RAY)
+        # against the vector peak of the precision they run in (FP32 157.3 TFLOP/s for the FP32
+        # pass, FP64 78.6); the integrator over the rays they declined beside them
+        cprec = args.closed_precision
+        cops = CLOSED_FORM_OPS_PER_RAY[cprec]
+        cpeak = FP32_VECTOR_PEAK_TFLOPS if cprec == "fp32" else FP64_VECTOR_PEAK_TFLOPS
+        cf_ach = rays * cops / (avg_sur * 1e-3) / 1e12 if avg_sur > 0 else 0.0
+        kcf = "k_closed_form<kFast> + k_closed_form<kRedo>" if cprec == "fp32" else "k_closed_form<kAll>"
         tr_cf, src_cf = profiled_traffic(wkey, args.height, "k_closed_form")
-        roof = {"bound": "valu", "achieved": round(cf_ach, 3), "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(cf_ach / FP64_VECTOR_PEAK_TFLOPS, 4), "traffic": tr_cf, "traffic_source": src_cf,
-                "kernel": "k_closed_form", "kernel_ms": round(avg_sur, 3), "ops_per_ray": CLOSED_FORM_OPS_PER_RAY,
-                "ops_per_ray_is": "mean FP64 operations of the closed form per ray of the orbit frames (+ - * / "
-                                  "sqrt once, FMA twice), counted by tests/closed_form.py"}
+        roof = {"bound": "valu", "achieved": round(cf_ach, 3), "peak": cpeak, "unit": "TFLOP/s",
+                "frac": round(cf_ach / cpeak, 4), "traffic": tr_cf, "traffic_source": src_cf,
+                "kernel": kcf, "kernel_ms": round(avg_sur, 3), "ops_per_ray": cops,
+                "ops_per_ray_is": f"mean operations of the closed form ({cprec}) per ray of the orbit frames "
+                                  "(+ - * / sqrt once, FMA twice), counted by tests/closed_form.py"}
         roof_mlp = roof_integ
 
     desc = {"orbit": f"120-frame orbit r=40 M, i 20->85 deg, a=0.998, {width}x{args.height} Bardeen plane "
@@ -680,7 +695,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"plain": "f64", "mixed": "f32+f64", "surrogate": "bf16+f32+f64", "closed": "f64"}[args.path],
+        "dtype": {"plain": "f64", "mixed": "f32+f64", "surrogate": "bf16+f32+f64",
+                  "closed": "f32+f64" if args.closed_precision == "fp32" else "f64"}[args.path],
         "data": "synthetic: analytic Bardeen image planes, no input files; surrogate weights trained by "
                 "black_simulation_amd.train_surrogate on integrator-labelled rays",
         "config": {"workload": f"{args.path}: {desc}", "path": args.path, "rays_per_step": rays,
@@ -702,7 +718,10 @@ def main():
                    "fallback_fraction": round(sum(fallback) / n_attr / rays, 4)
                    if args.path in ("surrogate", "closed") else None,
                    "accuracy_vs_fp64": acc,
-                   "learned_surrogate": alt_acc or None,
+                   "closed_form": ({"fp32": "FP32 elliptic integrals, FP64 roots and the rays FP32 leaves unsure",
+                                    "fp64": "FP64 throughout"}[args.closed_precision]
+                                   if args.path == "closed" else None),
+                   "other_paths": alt_acc or None,
                    "shade": shade,
                    "launch": {"mode": ["persistent", "one-shot"][st["mode"]], "block": st["block"],
                               "grid": st["grid"], "waves_per_cu": st["waves_per_cu"],

@@ -101,7 +101,8 @@ class TraceOpts:
     # cancel flag); None: not cancellable
     cancel: object = None
     # 1: the closed-form solution answers the rays it covers (KT_FLAG_CLOSED_FORM), the
-    # integrator the rest; with use_surrogate, it replaces the values the network answered
+    # integrator the rest; with use_surrogate, it replaces the values the network answered.
+    # 2: the same with FP32 elliptic integrals (FP64 for the rays FP32 leaves unsure)
     closed_form: int = 0
 
     def to_c(self) -> K.KtTraceOpts:

@@ -91,6 +91,54 @@ KTC_FN double qrcp(double x) {
 #endif
 }
 
+// FP32: the hardware's square root, reciprocal square root and reciprocal (about one ulp)
+KTC_FN float qrsqrt(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
+KTC_FN float qsqrt(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+}
+KTC_FN float qrcp(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+// FP32 sin and cos: the quadrant and remainder in FP64, the Taylor series to the 11th and
+// 12th order in FP32 (truncation below 1e-11)
+KTC_FN void qsincos(float uf, float& s, float& c) {
+    const double u = (double)uf;
+    const double j = rint(u * 0.63661977236758134308);
+    const float x = (float)(u - j * 1.57079632679489661923);
+    const float x2 = x * x;
+    float ps = -1.0f / 39916800.0f;
+    ps = fmaf(ps, x2, 1.0f / 362880.0f);
+    ps = fmaf(ps, x2, -1.0f / 5040.0f);
+    ps = fmaf(ps, x2, 1.0f / 120.0f);
+    ps = fmaf(ps, x2, -1.0f / 6.0f);
+    const float sx = fmaf(ps * x2, x, x);
+    float pc = -1.0f / 479001600.0f;
+    pc = fmaf(pc, x2, 1.0f / 3628800.0f);
+    pc = fmaf(pc, x2, -1.0f / 40320.0f);
+    pc = fmaf(pc, x2, 1.0f / 720.0f);
+    pc = fmaf(pc, x2, -1.0f / 24.0f);
+    pc = fmaf(pc, x2, 0.5f);
+    const float cx = fmaf(-pc, x2, 1.0f);
+    const long long q = (long long)j & 3;
+    const float a = (q & 1) ? cx : sx, b = (q & 1) ? sx : cx;
+    s = (q & 2) ? -a : a;
+    c = ((q + 1) & 2) ? -b : b;
+}
+
 // sin and cos of |u| < 1e6: u - j pi/2 with pi/2 in three parts (Cody-Waite; the product
 // of j and each part is exact for j < 2^20), then the Taylor series of both on
 // [-pi/4, pi/4] to the 19th and 20th order (truncation below 1e-18)
@@ -134,25 +182,36 @@ KTC_FN void qsincos(double u, double& s, double& c) {
 // ---- special functions -------------------------------------------------------------------
 
 // R_F(x, y, z), x, y, z >= 0, at most one zero: duplication until the arguments agree to
-// 3e-2 of their mean, then the seventh-order series (DLMF 19.36.1; truncation ~1e-14)
-constexpr double kDupTol = 3e-2;
+// 3e-2 of their mean in FP64 (1e-1 in FP32), then the seventh-order series (DLMF 19.36.1;
+// truncation ~1e-14, ~1e-8 in FP32)
+template <class R>
+struct Tol;
+template <>
+struct Tol<double> {
+    static constexpr double dup = 3e-2, agm = 1e-9;
+};
+template <>
+struct Tol<float> {
+    static constexpr float dup = 1e-1f, agm = 1e-4f;
+};
 
-KTC_CALL double rf(double x, double y, double z) {
-    double A = (x + y + z) * (1.0 / 3.0);
+template <class R>
+KTC_CALL R rf(R x, R y, R z) {
+    R A = (x + y + z) * R(1.0 / 3.0);
     for (int it = 0; it < 24; ++it) {
-        if (fmax(fabs(A - x), fmax(fabs(A - y), fabs(A - z))) < kDupTol * A) break;
-        const double sx = qsqrt(x), sy = qsqrt(y), sz = qsqrt(z);
-        const double lam = sx * (sy + sz) + sy * sz;
-        x = 0.25 * (x + lam);
-        y = 0.25 * (y + lam);
-        z = 0.25 * (z + lam);
-        A = (x + y + z) * (1.0 / 3.0);
+        if (fmax(fabs(A - x), fmax(fabs(A - y), fabs(A - z))) < Tol<R>::dup * A) break;
+        const R sx = qsqrt(x), sy = qsqrt(y), sz = qsqrt(z);
+        const R lam = sx * (sy + sz) + sy * sz;
+        x = R(0.25) * (x + lam);
+        y = R(0.25) * (y + lam);
+        z = R(0.25) * (z + lam);
+        A = (x + y + z) * R(1.0 / 3.0);
     }
-    const double inv = qrcp(A);
-    const double X = (A - x) * inv, Y = (A - y) * inv, Z = -X - Y;
-    const double E2 = X * Y - Z * Z, E3 = X * Y * Z;
-    return (1.0 - E2 * (1.0 / 10.0) + E3 * (1.0 / 14.0) + E2 * E2 * (1.0 / 24.0) - E2 * E3 * (3.0 / 44.0) -
-            E2 * E2 * E2 * (5.0 / 208.0) + E3 * E3 * (3.0 / 104.0) + E2 * E2 * E3 * (1.0 / 16.0)) *
+    const R inv = qrcp(A);
+    const R X = (A - x) * inv, Y = (A - y) * inv, Z = -X - Y;
+    const R E2 = X * Y - Z * Z, E3 = X * Y * Z;
+    return (R(1) - E2 * R(1.0 / 10.0) + E3 * R(1.0 / 14.0) + E2 * E2 * R(1.0 / 24.0) - E2 * E3 * R(3.0 / 44.0) -
+            E2 * E2 * E2 * R(5.0 / 208.0) + E3 * E3 * R(3.0 / 104.0) + E2 * E2 * E3 * R(1.0 / 16.0)) *
            qrsqrt(A);
 }
 
@@ -207,13 +266,22 @@ KTC_CALL double rj(double x, double y, double z, double p) {
 }
 
 // Legendre's F(asin s | m) and Pi(n; asin s | m), |s| <= 1, m < 1, n s^2 < 1
-KTC_FN double ell_f(double s, double m) { return s * rf((1.0 - s) * (1.0 + s), 1.0 - m * s * s, 1.0); }
+template <class R>
+KTC_FN R ell_f(R s, R m) {
+    return s * rf<R>((R(1) - s) * (R(1) + s), R(1) - m * s * s, R(1));
+}
+// the same from s and its cofactors c2 = 1 - s^2, d2 = 1 - m s^2 formed in FP64 by the
+// caller: near s = 1 (a turning point) the amplitude is ill-conditioned in s, not in c2
+template <class R>
+KTC_FN R ell_f2(double s, double c2, double d2) {
+    return R(s) * rf<R>(R(c2), R(d2), R(1));
+}
 KTC_FN double ell_pi(double n, double s, double m) {
     const double c2 = (1.0 - s) * (1.0 + s), d2 = 1.0 - m * s * s, s2 = s * s;
-    return s * rf(c2, d2, 1.0) + n * (1.0 / 3.0) * s * s2 * rj(c2, d2, 1.0, 1.0 - n * s2);
+    return s * rf<double>(c2, d2, 1.0) + n * (1.0 / 3.0) * s * s2 * rj(c2, d2, 1.0, 1.0 - n * s2);
 }
 KTC_FN double ell_pi_complete(double n, double m) {
-    return rf(0.0, 1.0 - m, 1.0) + n * (1.0 / 3.0) * rj(0.0, 1.0 - m, 1.0, 1.0 - n);
+    return rf<double>(0.0, 1.0 - m, 1.0) + n * (1.0 / 3.0) * rj(0.0, 1.0 - m, 1.0, 1.0 - n);
 }
 
 // The arithmetic-geometric mean of (1, k'), k'^2 = 1 - m (m < 1), kept level by level:
@@ -226,22 +294,25 @@ KTC_FN double ell_pi_complete(double n, double m) {
 //   sn_j = (a + c) sn / (a + c sn^2),  cn_j = a cn dn / (a + c sn^2),
 //   dn_j = (a - c sn^2) / (a + c sn^2)          (a, c of level j + 1).
 // One modulus serves all the equatorial crossings of a ray.
+template <class R>
 struct Agm {
     static constexpr int kLevels = 7;
-    double a[kLevels], c[kLevels];
+    R a[kLevels], c[kLevels];
     int n;
-    double aN, bN;
+    R aN, bN;
 };
 
-KTC_FN void agm_init(Agm& G, double m) {
-    double a = 1.0, b = qsqrt(1.0 - m);
+// kp = sqrt(1 - m), given so that m near 1 keeps its digits
+template <class R>
+KTC_FN void agm_init(Agm<R>& G, R kp) {
+    R a = R(1), b = kp;
     G.n = 0;
 #pragma unroll
-    for (int j = 0; j < Agm::kLevels; ++j) {
-        G.a[j] = 1.0;
-        G.c[j] = 0.0;
-        if (fabs(a - b) > 1e-9 * a) {
-            const double an = 0.5 * (a + b), cn = 0.5 * (a - b);
+    for (int j = 0; j < Agm<R>::kLevels; ++j) {
+        G.a[j] = R(1);
+        G.c[j] = R(0);
+        if (fabs(a - b) > Tol<R>::agm * a) {
+            const R an = R(0.5) * (a + b), cn = R(0.5) * (a - b);
             b = qsqrt(a * b);
             a = an;
             G.a[j] = a;
@@ -253,26 +324,30 @@ KTC_FN void agm_init(Agm& G, double m) {
     G.bN = b;
 }
 
-KTC_FN double agm_k(const Agm& G) { return kPiC * qrcp(G.aN + G.bN); }
+template <class R>
+KTC_FN R agm_k(const Agm<R>& G) {
+    return R(kPiC) * qrcp(G.aN + G.bN);
+}
 
-KTC_FN void agm_sncndn(const Agm& G, double u, double& sn, double& cn, double& dn) {
+template <class R>
+KTC_FN void agm_sncndn(const Agm<R>& G, R u, R& sn, R& cn, R& dn) {
     u *= G.aN;
-    const double kp = G.bN * qrcp(G.aN);
-    const double mn = (1.0 - kp) * (1.0 + kp);
-    double s, c;
+    const R kp = G.bN * qrcp(G.aN);
+    const R mn = (R(1) - kp) * (R(1) + kp);
+    R s, c;
     qsincos(u, s, c);
-    const double d = 0.25 * mn * (u - s * c);  // sn(u|m) = sin u - (m/4)(u - sin u cos u) cos u + O(m^2)
-    const double s1 = s - d * c, c1 = c + d * s;
+    const R d = R(0.25) * mn * (u - s * c);  // sn(u|m) = sin u - (m/4)(u - sin u cos u) cos u + O(m^2)
+    const R s1 = s - d * c, c1 = c + d * s;
     s = s1;
     c = c1;
-    double dd = qsqrt(1.0 - mn * s * s);
+    R dd = qsqrt(R(1) - mn * s * s);
 #pragma unroll
-    for (int j = Agm::kLevels - 1; j >= 0; --j) {
+    for (int j = Agm<R>::kLevels - 1; j >= 0; --j) {
         if (j < G.n) {
-            const double a = G.a[j], cj = G.c[j];
-            const double cs2 = cj * s * s;
-            const double inv = qrcp(a + cs2);
-            const double ns = (a + cj) * s * inv, nc = a * c * dd * inv, nd = (a - cs2) * inv;
+            const R a = G.a[j], cj = G.c[j];
+            const R cs2 = cj * s * s;
+            const R inv = qrcp(a + cs2);
+            const R ns = (a + cj) * s * inv, nc = a * c * dd * inv, nd = (a - cs2) * inv;
             s = ns;
             c = nc;
             dd = nd;
@@ -284,16 +359,17 @@ KTC_FN void agm_sncndn(const Agm& G, double u, double& sn, double& cn, double& d
 }
 
 // K(m), m < 1
-KTC_CALL double ell_k(double m) {
-    Agm G;
-    agm_init(G, m);
+template <class R>
+KTC_CALL R ell_k(R m) {
+    Agm<R> G;
+    agm_init<R>(G, qsqrt(R(1) - m));
     return agm_k(G);
 }
 
 // Jacobi sn, cn, dn (u | m), 0 <= m < 1, for a single argument
 KTC_CALL void sncndn(double u, double m, double& sn, double& cn, double& dn) {
-    Agm G;
-    agm_init(G, m);
+    Agm<double> G;
+    agm_init<double>(G, qsqrt(1.0 - m));
     agm_sncndn(G, u, sn, cn, dn);
 }
 
@@ -376,19 +452,24 @@ KTC_FN double wrap2pi(double x) {
     return y;
 }
 
-// a radius within this relative distance of a disk edge is left to the integrator, whose
-// crossing radius is good to ~1e-10
-constexpr double kEdgeBand = 1e-9;
+// a crossing radius within this relative distance of a disk edge is left to the
+// integrator (FP64, whose crossing radius is good to ~1e-10) or, from the FP32 pass, to the
+// FP64 one
+constexpr double kEdgeBand = 1e-9, kEdgeBand32 = 1e-5;
 constexpr int kMaxCrossings = 24;
+enum : int { kUncertain = -2 };  // the FP32 pass leaves the ray to the FP64 one
 
-// The polar motion: omega, u+, m, x = mu_o / sqrt(u+), F(asin x | m), K(m), and the Mino
-// time of the first equatorial crossing (eta > 0).
+// The polar motion: omega, u+, m, x = mu_o / sqrt(u+) (FP64), F(asin x | m), K(m), and the
+// Mino time of the first equatorial crossing (eta > 0), the last three in precision R.
+template <class R>
 struct Polar {
-    double om, up, m, x, Fo, K, s1;
+    double om, up, m, x;
+    R Fo, K, s1;
 };
 
-KTC_FN Polar polar_motion(const Ray& ry, double A) {
-    Polar P;
+template <class R>
+KTC_FN Polar<R> polar_motion(const Ray& ry, double A) {
+    Polar<R> P;
     const double a2 = ry.a * ry.a, eta = ry.eta;
     const double disc = sqrt(A * A + 4.0 * a2 * eta);
     const double om2 = A <= 0.0 ? 0.5 * (disc - A) : 2.0 * a2 * eta / (A + disc);  // a^2 (-u-)
@@ -396,23 +477,24 @@ KTC_FN Polar polar_motion(const Ray& ry, double A) {
     P.up = eta / om2;
     P.m = -a2 * eta / (om2 * om2);
     P.x = fmin(1.0, fmax(-1.0, ry.mu_o / sqrt(P.up)));
-    P.Fo = ell_f(P.x, P.m);
-    P.K = ell_k(P.m);
+    const double c2 = fmax(0.0, (P.up - ry.mu_o * ry.mu_o) / P.up);  // 1 - x^2
+    P.Fo = ell_f2<R>(P.x, c2, 1.0 - P.m * P.x * P.x);
+    P.K = ell_k<R>(R(P.m));
     // F at the next psi = j pi in the direction of motion; starting on the plane is no crossing
     const double x = P.x;
-    double F1;
-    if (ry.s_mu > 0) F1 = (x > 0.0 ? 2.0 * P.K : 0.0) - P.Fo;
-    else F1 = P.Fo - (x > 0.0 ? 0.0 : -2.0 * P.K);
-    if (x == 0.0) F1 = 2.0 * P.K;
-    P.s1 = F1 / P.om;
+    R F1;
+    if (ry.s_mu > 0) F1 = (x > 0.0 ? R(2) * P.K : R(0)) - P.Fo;
+    else F1 = P.Fo - (x > 0.0 ? R(0) : R(-2) * P.K);
+    if (x == 0.0) F1 = R(2) * P.K;
+    P.s1 = F1 * R(1.0 / P.om);
     return P;
 }
 
 // An escaping ray (four real roots, turning at r4) that the polar forms cover (eta > 0,
 // lambda != 0): the sky angles at infinity from the total Mino time S_tot.  x_o is x(r_o)
 // of the radial substitution, x_inf its limit; returns false if a value is not finite.
-KTC_CALL bool escape_sky(const Ray& ry, const Polar& P, double S_tot, double r1, double r3, double r4, double k,
-                         double pref, double x_o, double x_inf, double& theta, double& phi) {
+KTC_CALL bool escape_sky(const Ray& ry, const Polar<double>& P, double S_tot, double r1, double r3, double r4,
+                         double k, double pref, double x_o, double x_inf, double& theta, double& phi) {
     const double a = ry.a, a2 = a * a, lam = ry.lam;
     const double r31 = r3 - r1, r41 = r4 - r1;
     // theta_inf from sn at the total Mino time (m <= 0:
@@ -439,7 +521,7 @@ KTC_CALL bool escape_sky(const Ray& ry, const Polar& P, double S_tot, double r1,
     if (a != 0.0) {
         const double sq1 = sqrt(fmax(0.0, 1.0 - a2));
         if (sq1 == 0.0) return false;  // |a| = 1: the partial fractions degenerate
-        const double Fo = ell_f(x_o, k), Fi = ell_f(x_inf, k);
+        const double Fo = ell_f<double>(x_o, k), Fi = ell_f<double>(x_inf, k);
         for (int h = 0; h < 2; ++h) {
             const double p = h == 0 ? 1.0 + sq1 : 1.0 - sq1, q = h == 0 ? 1.0 - sq1 : 1.0 + sq1;
             const double Ap = a * (2.0 * p - a * lam) / (p - q);
@@ -455,7 +537,16 @@ KTC_CALL bool escape_sky(const Ray& ry, const Polar& P, double S_tot, double r1,
     return theta == theta && phi == phi;
 }
 
-KTC_FN Outcome solve(const Ray& ry) {
+// The outcome of one ray with its elliptic integrals and Jacobi functions in precision R
+// (the roots of R(r) and the class decisions they make are FP64 either way).  R = float
+// answers disk hits and captures to ~1e-6 and returns kUncertain where FP32 could place a
+// crossing on the wrong side of a disk edge or where an escape's sky is needed; those rays
+// take R = double.
+template <class R>
+KTC_FN Outcome solve_t(const Ray& ry) {
+    constexpr bool fp32 = sizeof(R) == 4;
+    const int unsure = fp32 ? (int)kUncertain : (int)kDeclined;
+    const double band = fp32 ? kEdgeBand32 : kEdgeBand;
     Outcome out{kDeclined, 0.0, 0.0, 0.0, 0.0};
     const double a = ry.a, a2 = a * a, lam = ry.lam, eta = ry.eta;
     const double A = a2 - eta - lam * lam;
@@ -478,8 +569,9 @@ KTC_FN Outcome solve(const Ray& ry) {
     // Aq = |r3 - r2|, Bq = |r3 - r1|).  S_t: J at the camera, signed so that the ray's own
     // Mino time s reaches rt at s = S_t (S_t < 0: rt lies behind an outward start).
     bool real4, turning = false;
-    double pref, k, P0, P1, Q0, Q1, S_t, r1 = 0.0, r3 = 0.0, r4 = 0.0, x_o = 0.0, x_inf = 0.0;
-    Agm G;  // the radial modulus's mean (for the crossings)
+    double pref, k, kp, P0, P1, Q0, Q1, r1 = 0.0, r3 = 0.0, r4 = 0.0, x_o = 0.0, x_inf = 0.0, c2_o = 1.0, c2_inf = 1.0;
+    R S_t;
+    Agm<R> G;  // the radial modulus's mean (for the crossings)
     if (D34 >= 0.0) {
         real4 = true;
         const double sD12 = sqrt(fmax(D12, 0.0)), sD34 = sqrt(D34);
@@ -489,6 +581,7 @@ KTC_FN Outcome solve(const Ray& ry) {
         r4 = z + sD34;
         const double r31 = r3 - r1, r41 = r4 - r1, r42 = r4 - r2, r32 = r3 - r2;
         k = r32 * r41 / (r31 * r42);
+        kp = sqrt((r2 - r1) * (r4 - r3) / (r31 * r42));  // sqrt(1 - k)
         pref = 2.0 / sqrt(r31 * r42);
         turning = r4 > ry.r_cap;
         if (turning && (r4 < ry.r_cap * (1.0 + 1e-7) || !(r4 < ry.r_o))) return out;
@@ -500,9 +593,12 @@ KTC_FN Outcome solve(const Ray& ry) {
         Q1 = -r41;
         x_o = sqrt(r31 * (ry.r_o - r4) / (r41 * (ry.r_o - r3)));
         x_inf = sqrt(r31 / r41);
-        const double I_o = pref * ell_f(x_o, k);
+        const double r43 = r4 - r3;
+        c2_o = r43 * (ry.r_o - r1) / (r41 * (ry.r_o - r3));  // 1 - x_o^2
+        c2_inf = r43 / r41;                                  // 1 - x_inf^2
+        const R I_o = R(pref) * ell_f2<R>(x_o, c2_o, kp * kp + k * c2_o);
         S_t = ry.s_r < 0 ? I_o : -I_o;
-        if (cross) agm_init(G, k);
+        if (cross) agm_init<R>(G, R(kp));
     } else {
         if (ry.s_r > 0) return out;  // no turning point: an outgoing ray escapes by forms not given here
         if (!(D12 >= 0.0)) {
@@ -524,6 +620,7 @@ KTC_FN Outcome solve(const Ray& ry) {
         const double Aq = hypot(z - r2, im3), Bq = hypot(z - r1, im3);
         const double r21 = r2 - r1;
         k = ((Aq + Bq) * (Aq + Bq) - r21 * r21) / (4.0 * Aq * Bq);
+        kp = sqrt(fmax(0.0, (r21 * r21 - (Aq - Bq) * (Aq - Bq)) / (4.0 * Aq * Bq)));  // sqrt(1 - k)
         pref = 1.0 / sqrt(Aq * Bq);
         P0 = Aq * r1 - Bq * r2;
         P1 = -(Aq * r1 + Bq * r2);
@@ -531,35 +628,39 @@ KTC_FN Outcome solve(const Ray& ry) {
         Q1 = -(Aq + Bq);
         const double ro = ry.r_o;
         const double c = (Aq * (ro - r1) - Bq * (ro - r2)) / (Aq * (ro - r1) + Bq * (ro - r2));
-        const double f = ell_f(sqrt(fmax(0.0, (1.0 - c) * (1.0 + c))), k);
-        agm_init(G, k);
-        S_t = pref * (c >= 0.0 ? f : 2.0 * agm_k(G) - f);  // F(acos c) past pi/2
+        const R f = ell_f2<R>(sqrt(fmax(0.0, (1.0 - c) * (1.0 + c))), c * c, kp * kp + k * c * c);
+        agm_init<R>(G, R(kp));
+        S_t = R(pref) * (c >= 0.0 ? f : R(2) * agm_k(G) - f);  // F(acos c) past pi/2
     }
 
     // ---- polar motion, and the disk at its equatorial crossings: inward of rt first; for a
     // turning ray past rt until the escape, whose time is needed only then or for the sky
-    Polar P{};
-    if (eta > 0.0) P = polar_motion(ry, A);
-    double I_inf = -1.0;
+    Polar<R> P{};
+    if (eta > 0.0) P = polar_motion<R>(ry, A);
+    R I_inf = R(-1);
     if (cross) {
-        const double ds = 2.0 * P.K / P.om;
-        double s = P.s1, s_end = S_t;
+        const R ds = R(2) * P.K * R(1.0 / P.om);
+        const R ipref = R(1.0 / pref);
+        R s = P.s1, s_end = S_t;
         int j = 0;
         for (; j < kMaxCrossings; ++j) {
             if (!(s < s_end)) {
-                if (!turning || I_inf >= 0.0) break;
-                I_inf = pref * ell_f(x_inf, k);
+                if (!turning || I_inf >= R(0)) break;
+                I_inf = R(pref) * ell_f2<R>(x_inf, c2_inf, kp * kp + k * c2_inf);
                 s_end = S_t + I_inf;
                 if (!(s < s_end)) break;
             }
-            const double J = s < S_t ? S_t - s : s - S_t;
-            double sn, cn, dn;
-            agm_sncndn(G, J / pref, sn, cn, dn);
-            const double w = real4 ? sn * sn : cn;
+            const R J = s < S_t ? S_t - s : s - S_t;
+            R sn, cn, dn;
+            agm_sncndn(G, J * ipref, sn, cn, dn);
+            const double w = real4 ? (double)(sn * sn) : (double)cn;
             const double rc = (P0 + P1 * w) / (Q0 + Q1 * w);
             if (!(rc == rc)) return out;
             if (rc < ry.r_cap) break;  // inward past the capture radius: later crossings lie deeper
-            if (fabs(rc - ry.r_in) < kEdgeBand * ry.r_in || fabs(rc - ry.r_out) < kEdgeBand * ry.r_out) return out;
+            if (fabs(rc - ry.r_in) < band * ry.r_in || fabs(rc - ry.r_out) < band * ry.r_out) {
+                out.cls = unsure;
+                return out;
+            }
             double g;
             if (rc >= ry.r_in && rc <= ry.r_out && disk_g(a, rc, ry.prograde, ry.E, lam * ry.E, g)) {
                 out.cls = 2;
@@ -576,13 +677,20 @@ KTC_FN Outcome solve(const Ray& ry) {
         return out;
     }
     if (!(eta > 0.0) || lam == 0.0) return out;  // vortical or through the pole: the integrator
-    if (I_inf < 0.0) I_inf = pref * ell_f(x_inf, k);
-    double th, ph;
-    if (!escape_sky(ry, P, S_t + I_inf, r1, r3, r4, k, pref, x_o, x_inf, th, ph)) return out;
-    out.cls = 0;
-    out.theta = th;
-    out.phi = ph;
-    return out;
+    if constexpr (fp32) {
+        out.cls = kUncertain;  // the sky angles in FP64
+        return out;
+    } else {
+        double th, ph;
+        if (I_inf < 0.0) I_inf = pref * ell_f2<double>(x_inf, c2_inf, kp * kp + k * c2_inf);
+        if (!escape_sky(ry, P, S_t + I_inf, r1, r3, r4, k, pref, x_o, x_inf, th, ph)) return out;
+        out.cls = 0;
+        out.theta = th;
+        out.phi = ph;
+        return out;
+    }
 }
+
+KTC_FN Outcome solve(const Ray& ry) { return solve_t<double>(ry); }
 
 }  // namespace ktc

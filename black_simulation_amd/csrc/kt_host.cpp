@@ -427,7 +427,7 @@ const char* check_opts(const kt_trace_opts& o) {
         return "block must be 0 (tuned) or 64, 128, 192 or 256";
     if (o.waves_per_cu < 0 || o.steps_per_dequeue < 0) return "negative launch shape";
     if (o.waves_per_simd < 0 || o.waves_per_simd > 4) return "waves_per_simd must be 0 (tuned) to 4";
-    if (o.closed_form != 0 && o.closed_form != 1) return "closed_form must be 0 or 1";
+    if (o.closed_form < 0 || o.closed_form > 2) return "closed_form must be 0, 1 or 2";
     return nullptr;
 }
 
@@ -617,7 +617,7 @@ int trace_common(kt_ctx* ctx, std::vector<ViewDev>& vd, long long total, const k
         KT_HIP(kts::queue_reserve(ctx->cf_queue, total));
         const kti::RayQueue rq{ctx->cf_queue.list, ctx->cf_queue.count, ctx->cf_queue.total, ctx->cf_queue.cap};
         KT_HIP(hipEventRecord(ctx->ev[0], stream));
-        hipError_t e = kti::trace_closed_form(L, rq, stream, ctx->ev);
+        hipError_t e = kti::trace_closed_form(L, rq, o.closed_form == 2, stream, ctx->ev);
         if (e != hipSuccess) return hip_fail(ctx, e, "closed-form trace");
         KT_HIP(hipEventRecord(ctx->ev[3], stream));
     } else {
@@ -1046,7 +1046,7 @@ int kt_plan_create(kt_ctx* ctx, const kt_view* views, int n_views, const kt_trac
         const kti::RayQueue rq{p->queue.list, p->queue.count, p->queue.total, p->queue.cap};
         hipError_t le = o.use_surrogate ? kts::trace_with_surrogate(ctx->surrogate, p->L, o.gate_prob, o.gate_ring,
                                                                     o.closed_form, &p->queue, p->capture, nullptr)
-                        : o.closed_form ? kti::trace_closed_form(p->L, rq, p->capture, nullptr)
+                        : o.closed_form ? kti::trace_closed_form(p->L, rq, o.closed_form == 2, p->capture, nullptr)
                                         : kti::launch_trace(p->L, p->capture);
         if ((e = hipStreamEndCapture(p->capture, &p->graph)) != hipSuccess) return cleanup(KT_EHIP, "end capture", e);
         if (le != hipSuccess) return cleanup(KT_EHIP, "captured launch", le);

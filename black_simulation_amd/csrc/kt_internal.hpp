@@ -51,10 +51,11 @@ struct RayQueue {
 // the rays it answered as escaped or disk hits, replaces their values and re-queues (in the
 // region of the network's class) those it declines or classes otherwise.
 hipError_t launch_closed_form(const TraceLaunch& L, const RayQueue& q, int polish, hipStream_t stream);
-// Every ray by the closed form, then the integrator (L's precision) over the queue it leaves:
-// zeroes the queue and counters, and writes the queue's length to stats[kStatFallback].
-// ev[0..2] (optional) bracket the closed form and the integrator.
-hipError_t trace_closed_form(const TraceLaunch& L, const RayQueue& q, hipStream_t stream, hipEvent_t* ev);
+// Every ray by the closed form (fp32: FP32 elliptic integrals, FP64 for the rays that
+// precision leaves unsure; else FP64 throughout), then the integrator (L's precision) over
+// the rays it declines: zeroes the queue and counters, and writes the queue's length to
+// stats[kStatFallback].  ev[0..2] (optional) bracket the closed form and the integrator.
+hipError_t trace_closed_form(const TraceLaunch& L, const RayQueue& q, int fp32, hipStream_t stream, hipEvent_t* ev);
 
 // launches the counter reset (unless L.no_reset) and the trace kernel on `stream`; returns a
 // hipError_t

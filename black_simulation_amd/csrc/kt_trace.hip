@@ -301,22 +301,34 @@ __device__ __forceinline__ bool closed_ray(const ViewDev& v, const Launch& la, k
     return true;
 }
 
-// One lane per ray, the grid striding over the launch.  Answered rays get the closed form's
-// outputs (class | kFlagClosedForm, 0 steps); the others join region `region` of the queue,
-// one atomic per wave and region.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_closed_form(kti::TraceLaunch L,
-                                                                                              kti::RayQueue rq, int polish) {
+// The closed-form passes, one lane per ray, the grid striding over the rays:
+//   kAll:    every ray of the launch in FP64; declined rays join queue region 0 (the integrator's)
+//   kFast:   every ray with FP32 elliptic integrals; rays it is unsure of join region 1,
+//            declined ones region 0
+//   kRedo:   the rays of region 1 (its length read on the device) in FP64
+//   kPolish: the rays the surrogate answered as escaped or disk hits, in FP64; rays it
+//            declines or classes otherwise join the region of the network's class
+// Answered rays get the closed form's outputs (class | kFlagClosedForm, 0 steps); joining
+// a region takes one atomic per wave and region.
+enum : int { kAll = 0, kFast = 1, kRedo = 2, kPolish = 3 };
+
+template <int MODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == kFast ? 4 : 2))) void k_closed_form(
+    kti::TraceLaunch L, kti::RayQueue rq) {
     const ViewDev* views = stage_views(L, view_lds);
     const int lane = (int)(threadIdx.x & 63u);
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long base = (long long)blockIdx.x * blockDim.x; base < L.total; base += stride) {  // block-uniform
-        const long long q = base + threadIdx.x;
+    const long long total = MODE == kRedo ? (long long)rq.count[1] : L.total;
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < total; base += stride) {  // block-uniform
+        const long long i = base + threadIdx.x;
+        long long q = 0;
         bool queue = false;
         int region = 0;
-        if (q < L.total) {
-            int prior = -1;  // polish: the network's class, or skip
+        if (i < total) {
+            q = MODE == kRedo ? rq.list[rq.stride + i] : i;
+            int prior = -1;  // polish: the network's class
             bool take = true;
-            if (polish) {
+            if (MODE == kPolish) {
                 const int c = L.out.cls[q];
                 prior = c & 7;
                 take = (c & kFlagSurrogate) && (prior == kEscaped || prior == kDisk);
@@ -327,10 +339,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 launch_photon(v, q - v.offset, la);
                 ktc::Ray ry;
                 ktc::Outcome o{ktc::kDeclined, 0.0, 0.0, 0.0, 0.0};
-                if (closed_ray(v, la, ry)) o = ktc::solve(ry);
-                if (o.cls < 0 || (polish && o.cls != prior)) {
+                if (closed_ray(v, la, ry)) {
+                    if constexpr (MODE == kFast) o = ktc::solve_t<float>(ry);
+                    else o = ktc::solve_t<double>(ry);
+                }
+                if (o.cls < 0 || (MODE == kPolish && o.cls != prior)) {
                     queue = true;
-                    region = polish ? prior : 0;
+                    region = MODE == kPolish ? prior : (o.cls == ktc::kUncertain ? 1 : 0);
                 } else {
                     Result r;
                     r.cls = o.cls;
@@ -339,7 +354,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     r.phi = o.phi;
                     r.r_hit = o.r_hit;
                     r.g = o.g;
-                    write_result(L.out, q, r, 0, kFlagClosedForm | (polish ? kFlagSurrogate : 0));
+                    write_result(L.out, q, r, 0, kFlagClosedForm | (MODE == kPolish ? kFlagSurrogate : 0));
                 }
             }
         }
@@ -360,6 +375,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
 __global__ void k_zero_queue(unsigned long long* count) {
     if ((int)threadIdx.x <= kti::kQueueBuckets) count[threadIdx.x] = 0ull;
+}
+
+// region 1 (the FP32 pass's unsure rays) once the FP64 pass has taken it
+__global__ void k_zero_region1(unsigned long long* count) {
+    if (threadIdx.x == 0) count[1] = 0ull;
 }
 
 // the queue's length (the sum of its regions) for the integrator and the stats
@@ -447,18 +467,35 @@ hipError_t reset_counters(const TraceLaunch& L, hipStream_t stream) {
 
 hipError_t launch_closed_form(const TraceLaunch& L, const RayQueue& q, int polish, hipStream_t stream) {
     if (L.total <= 0) return hipSuccess;
-    // one lane per ray up to 64 waves per CU's worth of blocks, striding beyond
-    const long long blocks = std::min<long long>((L.total + 255) / 256, 2048);
-    k_closed_form<<<dim3((unsigned)blocks), dim3(256), view_lds_bytes(L.n_views), stream>>>(L, q, polish);
+    // one lane per ray up to 2048 blocks of 256, striding beyond
+    const dim3 grid((unsigned)std::min<long long>((L.total + 255) / 256, 2048)), block(256);
+    const size_t lds = view_lds_bytes(L.n_views);
+    if (polish) k_closed_form<kPolish><<<grid, block, lds, stream>>>(L, q);
+    else k_closed_form<kAll><<<grid, block, lds, stream>>>(L, q);
     return hipGetLastError();
 }
 
-hipError_t trace_closed_form(const TraceLaunch& L, const RayQueue& q, hipStream_t stream, hipEvent_t* ev) {
+hipError_t trace_closed_form(const TraceLaunch& L, const RayQueue& q, int fp32, hipStream_t stream, hipEvent_t* ev) {
     hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, stream, q.count);
     hipError_t e = reset_counters(L, stream);
     if (e != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev[0], stream)) != hipSuccess) return e;
-    if ((e = launch_closed_form(L, q, 0, stream)) != hipSuccess) return e;
+    if (L.total > 0) {
+        const size_t lds = view_lds_bytes(L.n_views);
+        const dim3 block(256);
+        if (fp32) {
+            // the FP32 pass over every ray, then the FP64 pass over the rays it was unsure of
+            // (a small grid striding over region 1, whose length it reads on the device)
+            k_closed_form<kFast><<<dim3((unsigned)std::min<long long>((L.total + 255) / 256, 4096)), block, lds,
+                                   stream>>>(L, q);
+            k_closed_form<kRedo><<<dim3(256), block, lds, stream>>>(L, q);
+            hipLaunchKernelGGL(k_zero_region1, dim3(1), dim3(64), 0, stream, q.count);
+        } else {
+            k_closed_form<kAll><<<dim3((unsigned)std::min<long long>((L.total + 255) / 256, 2048)), block, lds,
+                                  stream>>>(L, q);
+        }
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     if (ev && (e = hipEventRecord(ev[1], stream)) != hipSuccess) return e;
     TraceLaunch F = L;
     F.ray_list = q.list;

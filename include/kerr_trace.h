@@ -143,7 +143,11 @@ typedef struct kt_trace_opts {
      * to the integrator at `precision`.  With use_surrogate, the network's gate runs first
      * and the closed form replaces the values of the rays it answered as escaped or disk
      * hits (re-queueing those it declines or classes otherwise).  Not for observer cameras
-     * or the ray report.  0: off (the default). */
+     * or the ray report.  0: off (the default).
+     * 2: the same with the elliptic integrals and Jacobi functions in FP32 (the roots of the
+     * radial potential and the class decisions they make stay FP64): disk radius and g to
+     * ~1e-6; a ray whose crossing lies within 1e-5 of a disk edge, or that escapes, is
+     * solved again in FP64 (with use_surrogate, 2 acts as 1). */
     int closed_form;
 } kt_trace_opts;
 

@@ -17,11 +17,13 @@ from __future__ import annotations
 
 import math
 
-ESCAPED, CAPTURED, DISK, DECLINED = 0, 1, 2, -1
-EDGE_BAND = 1e-9
+ESCAPED, CAPTURED, DISK, DECLINED, UNCERTAIN = 0, 1, 2, -1, -2
+EDGE_BAND, EDGE_BAND32 = 1e-9, 1e-5
 MAX_CROSSINGS = 24
-DUP_TOL = 3e-2
+DUP_TOL, DUP_TOL32 = 3e-2, 1e-1  # R_F's duplication stops (FP64, the FP32 pass)
 DUP_TOL_J = 6e-3
+AGM_TOL, AGM_TOL32 = 1e-9, 1e-4
+FP32 = False  # the FP32 pass's tolerances and its unsure rays (its arithmetic is FP64 here)
 
 
 class Num(float):
@@ -93,7 +95,7 @@ def qsincos(u):
 def rf(x, y, z):
     A = (x + y + z) * (1.0 / 3.0)
     for _ in range(24):
-        if max(abs(A - x), abs(A - y), abs(A - z)) < DUP_TOL * A:
+        if max(abs(A - x), abs(A - y), abs(A - z)) < (DUP_TOL32 if FP32 else DUP_TOL) * A:
             break
         sx, sy, sz = sqrt(x), sqrt(y), sqrt(z)
         lam = sx * (sy + sz) + sy * sz
@@ -167,7 +169,7 @@ def agm(m):
     a, b = 1.0, sqrt(1.0 - m)
     levels = []
     for _ in range(7):
-        if not abs(a - b) > 1e-9 * a:
+        if not abs(a - b) > (AGM_TOL32 if FP32 else AGM_TOL) * a:
             break
         a, b, c = 0.5 * (a + b), sqrt(a * b), 0.5 * (a - b)
         levels.append((a, c))
@@ -255,10 +257,20 @@ def disk_g(a, r, prograde, E, L):
     return sqrt(norm) / (E - L * om)
 
 
-def solve(ray):
+def solve(ray, fp32=False):
     """(class, theta_inf, phi_inf, r_hit, g) of one ray, class DECLINED where kt_closed.hpp
-    leaves it to the integrator.  ray: a, r_o, mu_o, phi_o, lam, eta, E, s_r, s_mu, r_in,
-    r_out, r_cap, r_esc, disk, prograde (E = 1 units for lam and eta)."""
+    leaves it to the integrator, UNCERTAIN (fp32: the FP32 pass's tolerances) where it leaves
+    it to the FP64 pass.  ray: a, r_o, mu_o, phi_o, lam, eta, E, s_r, s_mu, r_in, r_out,
+    r_cap, r_esc, disk, prograde (E = 1 units for lam and eta)."""
+    global FP32
+    FP32 = fp32
+    try:
+        return _solve(ray, fp32)
+    finally:
+        FP32 = False
+
+
+def _solve(ray, fp32):
     nan = math.nan
     a, lam, eta = ray["a"], ray["lam"], ray["eta"]
     a2 = a * a
@@ -360,8 +372,9 @@ def solve(ray):
                 return out
             if rc_ < ray["r_cap"]:
                 break
-            if abs(rc_ - ray["r_in"]) < EDGE_BAND * ray["r_in"] or abs(rc_ - ray["r_out"]) < EDGE_BAND * ray["r_out"]:
-                return out
+            band = EDGE_BAND32 if fp32 else EDGE_BAND
+            if abs(rc_ - ray["r_in"]) < band * ray["r_in"] or abs(rc_ - ray["r_out"]) < band * ray["r_out"]:
+                return (UNCERTAIN, nan, nan, nan, nan) if fp32 else out
             if ray["r_in"] <= rc_ <= ray["r_out"]:
                 g = disk_g(ray["a"], rc_, ray["prograde"], ray["E"], lam * ray["E"])
                 if g is not None:
@@ -375,6 +388,8 @@ def solve(ray):
         return out
     if I_inf is None:
         I_inf = pref * ell_f(x_inf, k)
+    if fp32:
+        return (UNCERTAIN, nan, nan, nan, nan)
     th, ph = escape_sky(ray, om, up, m, x, Fo, K, S_t + I_inf, r1, r3, r4, k, pref, x_o, x_inf)
     if th != th or ph != ph:
         return out
@@ -430,8 +445,10 @@ def bardeen_ray(view, alpha, beta, count=False):
             "disk": 1, "prograde": 1}
 
 
-def count_ops(ray) -> int:
-    """operations of solve(ray) (inputs made with bardeen_ray(..., count=True))"""
+def count_ops(ray, fp32=False) -> int:
+    """operations of solve(ray) (inputs made with bardeen_ray(..., count=True)); fp32: of the
+    FP32 pass, and of the FP64 pass too where it leaves the ray unsure"""
     Num.ops = 0
-    solve(ray)
+    if solve(ray, fp32)[0] == UNCERTAIN:
+        solve(ray)
     return Num.ops

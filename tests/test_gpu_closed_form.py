@@ -4,7 +4,8 @@ against the FP64 oracle (the DP5 integrator at rtol 1e-10 restating the referenc
 Bars: every ray's class equal to the oracle's; the rays the closed form answers (flag
 KT_FLAG_CLOSED_FORM, 0 steps) within 1e-6 relative in disk radius and g and 1e-6 rad in sky
 direction, with medians below 1e-9 (the oracle is itself good to ~1e-9); the rays it
-declines traced by the FP64 integrator, bit-equal to the oracle."""
+declines traced by the FP64 integrator, bit-equal to the oracle.  The FP32 pass
+(closed_form = 2): within 1e-4 in disk radius and g, medians below 1e-6."""
 import math
 
 import numpy as np
@@ -127,6 +128,46 @@ def test_pinhole_cameras(tracer):
         off += n
 
 
+def check32(got, want, tag):
+    """the FP32 pass: classes equal to the oracle's; disk radius and g within 1e-4 (median
+    1e-6); escapes solved in FP64 (1e-6 rad); declined rays bit-equal to the oracle"""
+    cls = got["cls"] & 7
+    closed = (got["cls"] & bsa.FLAG_CLOSED_FORM) != 0
+    assert np.array_equal(cls, want["cls"]), f"{tag}: {int((cls != want['cls']).sum())} classes differ"
+    fb = ~closed
+    for k in ("steps", "r_hit", "g"):
+        assert np.array_equal(got[k][fb], want[k][fb], equal_nan=True), f"{tag}: declined rays' {k}"
+    disk = closed & (cls == O.DISK)
+    r_err = np.abs(got["r_hit"][disk] / want["r_hit"][disk] - 1)
+    g_err = np.abs(got["g"][disk] / want["g"][disk] - 1)
+    esc = closed & (cls == O.ESCAPED)
+    d = sky_angle(got["theta_inf"][esc], got["phi_inf"][esc], want["theta"][esc], want["phi"][esc])
+    q = lambda x: f"{np.median(x):.1e}/{np.quantile(x, 0.99):.1e}/{x.max():.1e}" if len(x) else "-"  # noqa: E731
+    print(f"{tag} (FP32 pass): answered {closed.mean():.5f}; r_hit rel p50/p99/max {q(r_err)}, g rel {q(g_err)}, "
+          f"deflection rad {q(d)}", flush=True)
+    for x in (r_err, g_err):
+        if len(x):
+            assert np.median(x) < 1e-6 and x.max() < 1e-4, tag
+    if len(d):
+        assert d.max() < 1e-6, tag
+
+
+def test_fp32_pass(tracer):
+    """closed_form = 2: the c1 plane, three orbit frames and the 64 batch views"""
+    FAST = bsa.TraceOpts(closed_form=2)
+    v = c1_view()
+    check32(bsa.to_numpy(tracer.trace_views([v], FAST)), O.render(oracle_view(v), O.make_opts()), "c1")
+    for views, tag in (([c5_view(k, 192, 108) for k in (0, 60, C5_FRAMES - 1)], "orbit"),
+                       (c3_views(sizes=(8, 16, 32, 64, 128)), "batch")):
+        got = bsa.to_numpy(tracer.trace_views(views, FAST))
+        off = 0
+        for i, vv in enumerate(views):
+            n = vv.rays
+            check32({k: a[off:off + n] for k, a in got.items()}, O.render(oracle_view(vv), O.make_opts()),
+                    f"{tag} view {i}")
+            off += n
+
+
 def test_explicit_rays(tracer):
     """kt_trace_rays with explicit (alpha, beta), as eht::traceRay"""
     v = c1_view()
@@ -198,7 +239,7 @@ def test_closed_form_refusals(tracer):
                    u_obs=bsa.geodesic_four_velocity(0.9, 1.0, 0.0, 0.0, 10.0, 1.0, -1, 1))
     with pytest.raises(RuntimeError, match="closed form"):
         tracer.trace_views([obs], CLOSED)
-    with pytest.raises(RuntimeError, match="closed_form must be 0 or 1"):
-        tracer.trace_views([c1_view()], bsa.TraceOpts(closed_form=2))
+    with pytest.raises(RuntimeError, match="closed_form must be 0, 1 or 2"):
+        tracer.trace_views([c1_view()], bsa.TraceOpts(closed_form=3))
     with pytest.raises(RuntimeError, match="report"):
         tracer.trace_params(c1_view(), [[1.0, 2.0]], CLOSED, report=True)
