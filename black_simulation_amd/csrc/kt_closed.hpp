@@ -386,13 +386,13 @@ KTC_FN double resolvent_from(double c2, double c1, double c0, bool fp64) {
     const double disc = hq * hq + tp * tp * tp;
     double t;
     if (disc > 0.0) {
-        const double sd = sqrt(disc);
+        const double sd = qsqrt(disc);
         const double w = hq < 0.0 ? -hq + sd : -hq - sd;
         const double u = fp64 || !(fabs(w) < 1e30) ? cbrt(w) : (double)cbrtf((float)w);
-        t = u != 0.0 ? u - tp / u : 0.0;
+        t = u != 0.0 ? u - tp * qrcp(u) : 0.0;
     } else {
-        const double r = sqrt(-tp);
-        double ca = -hq / (r * r * r);
+        const double r = qsqrt(-tp);
+        double ca = -hq * qrcp(r * r * r);
         ca = fmin(1.0, fmax(-1.0, ca));
         t = 2.0 * r * (fp64 ? cos(acos(ca) * (1.0 / 3.0)) : (double)cosf(acosf((float)ca) * (1.0f / 3.0f)));
     }
@@ -400,7 +400,7 @@ KTC_FN double resolvent_from(double c2, double c1, double c0, bool fp64) {
     for (int it = 0; it < 3; ++it) {
         const double f = ((xi + c2) * xi + c1) * xi + c0;
         const double df = (3.0 * xi + 2.0 * c2) * xi + c1;
-        if (df != 0.0) xi -= f / df;
+        if (df != 0.0) xi -= f * qrcp(df);
     }
     return xi;
 }
@@ -433,15 +433,19 @@ struct Outcome {
     double theta, phi, r_hit, g;
 };
 
-// the Keplerian emitter's redshift at r (src/trajectory.cpp:311-321); false: no timelike orbit
-KTC_FN bool disk_g(double a, double r, int prograde, double E, double L, double& g) {
-    const double sg = prograde ? 1.0 : -1.0;
-    const double om = sg / (r * sqrt(r) + sg * a);
-    const double gtt = -(1.0 - 2.0 / r), gtp = -2.0 * a / r;
-    const double gpp = r * r + a * a + 2.0 * a * a / r;
-    const double norm = -(gtt + 2.0 * gtp * om + gpp * om * om);
-    if (!(norm > 0.0)) return false;
-    g = sqrt(norm) / (E - L * om);
+// the Keplerian emitter's redshift at r (src/trajectory.cpp:311-321), in precision R; false:
+// no timelike orbit
+template <class R>
+KTC_FN bool disk_g(double a_, double r_, int prograde, double E, double L, double& g) {
+    const R a = R(a_), r = R(r_);
+    const R sg = prograde ? R(1) : R(-1);
+    const R om = sg * qrcp(r * qsqrt(r) + sg * a);
+    const R ir = qrcp(r);
+    const R gtt = -(R(1) - R(2) * ir), gtp = R(-2) * a * ir;
+    const R gpp = r * r + a * a + R(2) * a * a * ir;
+    const R norm = -(gtt + R(2) * gtp * om + gpp * om * om);
+    if (!(norm > R(0))) return false;
+    g = (double)(qsqrt(norm) * qrcp(R(E) - R(L) * om));
     return true;
 }
 
@@ -471,13 +475,15 @@ template <class R>
 KTC_FN Polar<R> polar_motion(const Ray& ry, double A) {
     Polar<R> P;
     const double a2 = ry.a * ry.a, eta = ry.eta;
-    const double disc = sqrt(A * A + 4.0 * a2 * eta);
-    const double om2 = A <= 0.0 ? 0.5 * (disc - A) : 2.0 * a2 * eta / (A + disc);  // a^2 (-u-)
-    P.om = sqrt(om2);
-    P.up = eta / om2;
-    P.m = -a2 * eta / (om2 * om2);
-    P.x = fmin(1.0, fmax(-1.0, ry.mu_o / sqrt(P.up)));
-    const double c2 = fmax(0.0, (P.up - ry.mu_o * ry.mu_o) / P.up);  // 1 - x^2
+    const double disc = qsqrt(A * A + 4.0 * a2 * eta);
+    const double om2 = A <= 0.0 ? 0.5 * (disc - A) : 2.0 * a2 * eta * qrcp(A + disc);  // a^2 (-u-)
+    const double iom2 = qrcp(om2);
+    P.om = qsqrt(om2);
+    P.up = eta * iom2;
+    P.m = -a2 * eta * iom2 * iom2;
+    const double iup = qrcp(P.up);
+    P.x = fmin(1.0, fmax(-1.0, ry.mu_o * qrsqrt(P.up)));
+    const double c2 = fmax(0.0, (P.up - ry.mu_o * ry.mu_o) * iup);  // 1 - x^2
     P.Fo = ell_f2<R>(P.x, c2, 1.0 - P.m * P.x * P.x);
     P.K = ell_k<R>(R(P.m));
     // F at the next psi = j pi in the direction of motion; starting on the plane is no crossing
@@ -486,7 +492,7 @@ KTC_FN Polar<R> polar_motion(const Ray& ry, double A) {
     if (ry.s_mu > 0) F1 = (x > 0.0 ? R(2) * P.K : R(0)) - P.Fo;
     else F1 = P.Fo - (x > 0.0 ? R(0) : R(-2) * P.K);
     if (x == 0.0) F1 = R(2) * P.K;
-    P.s1 = F1 * R(1.0 / P.om);
+    P.s1 = F1 * R(qrcp(P.om));
     return P;
 }
 
@@ -556,9 +562,10 @@ KTC_FN Outcome solve_t(const Ray& ry) {
     if (!(B > 1e-12) || !(ry.E > 0.0) || !(ry.r_out < ry.r_esc)) return out;
     const double xi = resolvent_root(A, B, C);
     if (!(xi > 0.0)) return out;
-    const double z = sqrt(0.5 * xi);
-    const double D12 = -0.5 * A - z * z + B / (4.0 * z);
-    const double D34 = -0.5 * A - z * z - B / (4.0 * z);
+    const double z = qsqrt(0.5 * xi);
+    const double Bz = B * qrcp(4.0 * z);
+    const double D12 = -0.5 * A - z * z + Bz;
+    const double D34 = -0.5 * A - z * z - Bz;
     const bool cross = eta > 0.0 && ry.disk;  // eta <= 0: the ray never reaches the equator
 
     // ---- radial motion.  J, the Mino time from the largest real root rt of R (r4 of four
@@ -574,15 +581,16 @@ KTC_FN Outcome solve_t(const Ray& ry) {
     Agm<R> G;  // the radial modulus's mean (for the crossings)
     if (D34 >= 0.0) {
         real4 = true;
-        const double sD12 = sqrt(fmax(D12, 0.0)), sD34 = sqrt(D34);
+        const double sD12 = qsqrt(fmax(D12, 0.0)), sD34 = qsqrt(D34);
         r1 = -z - sD12;
         const double r2 = -z + sD12;
         r3 = z - sD34;
         r4 = z + sD34;
         const double r31 = r3 - r1, r41 = r4 - r1, r42 = r4 - r2, r32 = r3 - r2;
-        k = r32 * r41 / (r31 * r42);
-        kp = sqrt((r2 - r1) * (r4 - r3) / (r31 * r42));  // sqrt(1 - k)
-        pref = 2.0 / sqrt(r31 * r42);
+        const double i3142 = qrcp(r31 * r42);
+        k = r32 * r41 * i3142;
+        kp = qsqrt((r2 - r1) * (r4 - r3) * i3142);  // sqrt(1 - k)
+        pref = 2.0 * qrsqrt(r31 * r42);
         turning = r4 > ry.r_cap;
         if (turning && (r4 < ry.r_cap * (1.0 + 1e-7) || !(r4 < ry.r_o))) return out;
         if (r4 == ry.r_cap) return out;
@@ -591,11 +599,12 @@ KTC_FN Outcome solve_t(const Ray& ry) {
         P1 = -r41 * r3;
         Q0 = r31;
         Q1 = -r41;
-        x_o = sqrt(r31 * (ry.r_o - r4) / (r41 * (ry.r_o - r3)));
-        x_inf = sqrt(r31 / r41);
+        const double i41 = qrcp(r41), io3 = qrcp(r41 * (ry.r_o - r3));
+        x_o = qsqrt(r31 * (ry.r_o - r4) * io3);
+        x_inf = qsqrt(r31 * i41);
         const double r43 = r4 - r3;
-        c2_o = r43 * (ry.r_o - r1) / (r41 * (ry.r_o - r3));  // 1 - x_o^2
-        c2_inf = r43 / r41;                                  // 1 - x_inf^2
+        c2_o = r43 * (ry.r_o - r1) * io3;  // 1 - x_o^2
+        c2_inf = r43 * i41;                // 1 - x_inf^2
         const R I_o = R(pref) * ell_f2<R>(x_o, c2_o, kp * kp + k * c2_o);
         S_t = ry.s_r < 0 ? I_o : -I_o;
         if (cross) agm_init<R>(G, R(kp));
@@ -608,7 +617,7 @@ KTC_FN Outcome solve_t(const Ray& ry) {
             return out;
         }
         real4 = false;
-        const double sD12 = sqrt(D12);
+        const double sD12 = qsqrt(D12);
         r1 = -z - sD12;
         const double r2 = -z + sD12;
         if (!(r2 < ry.r_cap)) return out;
@@ -616,19 +625,20 @@ KTC_FN Outcome solve_t(const Ray& ry) {
             out.cls = 1;
             return out;
         }
-        const double im3 = sqrt(-D34);
+        const double im3 = qsqrt(-D34);
         const double Aq = hypot(z - r2, im3), Bq = hypot(z - r1, im3);
         const double r21 = r2 - r1;
-        k = ((Aq + Bq) * (Aq + Bq) - r21 * r21) / (4.0 * Aq * Bq);
-        kp = sqrt(fmax(0.0, (r21 * r21 - (Aq - Bq) * (Aq - Bq)) / (4.0 * Aq * Bq)));  // sqrt(1 - k)
-        pref = 1.0 / sqrt(Aq * Bq);
+        const double i4AB = qrcp(4.0 * Aq * Bq);
+        k = ((Aq + Bq) * (Aq + Bq) - r21 * r21) * i4AB;
+        kp = qsqrt(fmax(0.0, (r21 * r21 - (Aq - Bq) * (Aq - Bq)) * i4AB));  // sqrt(1 - k)
+        pref = qrsqrt(Aq * Bq);
         P0 = Aq * r1 - Bq * r2;
         P1 = -(Aq * r1 + Bq * r2);
         Q0 = Aq - Bq;
         Q1 = -(Aq + Bq);
         const double ro = ry.r_o;
-        const double c = (Aq * (ro - r1) - Bq * (ro - r2)) / (Aq * (ro - r1) + Bq * (ro - r2));
-        const R f = ell_f2<R>(sqrt(fmax(0.0, (1.0 - c) * (1.0 + c))), c * c, kp * kp + k * c * c);
+        const double c = (Aq * (ro - r1) - Bq * (ro - r2)) * qrcp(Aq * (ro - r1) + Bq * (ro - r2));
+        const R f = ell_f2<R>(qsqrt(fmax(0.0, (1.0 - c) * (1.0 + c))), c * c, kp * kp + k * c * c);
         agm_init<R>(G, R(kp));
         S_t = R(pref) * (c >= 0.0 ? f : R(2) * agm_k(G) - f);  // F(acos c) past pi/2
     }
@@ -639,8 +649,8 @@ KTC_FN Outcome solve_t(const Ray& ry) {
     if (eta > 0.0) P = polar_motion<R>(ry, A);
     R I_inf = R(-1);
     if (cross) {
-        const R ds = R(2) * P.K * R(1.0 / P.om);
-        const R ipref = R(1.0 / pref);
+        const R ds = R(2) * P.K * R(qrcp(P.om));
+        const R ipref = R(qrcp(pref));
         R s = P.s1, s_end = S_t;
         int j = 0;
         for (; j < kMaxCrossings; ++j) {
@@ -654,7 +664,7 @@ KTC_FN Outcome solve_t(const Ray& ry) {
             R sn, cn, dn;
             agm_sncndn(G, J * ipref, sn, cn, dn);
             const double w = real4 ? (double)(sn * sn) : (double)cn;
-            const double rc = (P0 + P1 * w) / (Q0 + Q1 * w);
+            const double rc = (P0 + P1 * w) * qrcp(Q0 + Q1 * w);
             if (!(rc == rc)) return out;
             if (rc < ry.r_cap) break;  // inward past the capture radius: later crossings lie deeper
             if (fabs(rc - ry.r_in) < band * ry.r_in || fabs(rc - ry.r_out) < band * ry.r_out) {
@@ -662,7 +672,7 @@ KTC_FN Outcome solve_t(const Ray& ry) {
                 return out;
             }
             double g;
-            if (rc >= ry.r_in && rc <= ry.r_out && disk_g(a, rc, ry.prograde, ry.E, lam * ry.E, g)) {
+            if (rc >= ry.r_in && rc <= ry.r_out && disk_g<R>(a, rc, ry.prograde, ry.E, lam * ry.E, g)) {
                 out.cls = 2;
                 out.r_hit = rc;
                 out.g = g;

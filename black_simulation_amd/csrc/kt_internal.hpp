@@ -25,6 +25,7 @@ struct TraceLaunch {
     // b holding bucket_count[b] entries; ray q of the launch is taken from them in order
     const unsigned long long* bucket_count;
     long long bucket_stride;
+    int n_buckets;              // regions of that queue to read: kQueueBuckets (0 means the same), or 1
     double rtol, atol, rtol32, atol32, restep_drift;
     int max_steps;
     int precision;              // KT_PREC_*

@@ -63,8 +63,10 @@ struct QueueMap {
         if (L.bucket_count) {
             bucketed = true;
             c0 = (long long)L.bucket_count[0];
-            c1 = (long long)L.bucket_count[1];
-            c2 = (long long)L.bucket_count[2];
+            if (L.n_buckets != 1) {
+                c1 = (long long)L.bucket_count[1];
+                c2 = (long long)L.bucket_count[2];
+            }
             stride = L.bucket_stride;
         }
     }
@@ -373,22 +375,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == kFa
     }
 }
 
-__global__ void k_zero_queue(unsigned long long* count) {
+// the queue's region counts, and the launch's queue head and counters
+__global__ void k_zero_queue(unsigned long long* count, unsigned long long* head, unsigned long long* stats) {
     if ((int)threadIdx.x <= kti::kQueueBuckets) count[threadIdx.x] = 0ull;
-}
-
-// region 1 (the FP32 pass's unsure rays) once the FP64 pass has taken it
-__global__ void k_zero_region1(unsigned long long* count) {
-    if (threadIdx.x == 0) count[1] = 0ull;
+    if (threadIdx.x == 0) *head = 0ull;
+    if (threadIdx.x < kti::kStatCount) stats[threadIdx.x] = 0ull;
 }
 
 // the queue's length (the sum of its regions) for the integrator and the stats
 __global__ void k_queue_total(const unsigned long long* count, long long* total, unsigned long long* stat) {
     if (threadIdx.x == 0) {
-        unsigned long long n = 0;
-        for (int b = 0; b < kti::kQueueBuckets; ++b) n += count[b];
-        *total = (long long)n;
-        if (stat) *stat = n;
+        *total = (long long)count[0];  // the integrator's region
+        if (stat) *stat = count[0];
     }
 }
 
@@ -476,8 +474,8 @@ hipError_t launch_closed_form(const TraceLaunch& L, const RayQueue& q, int polis
 }
 
 hipError_t trace_closed_form(const TraceLaunch& L, const RayQueue& q, int fp32, hipStream_t stream, hipEvent_t* ev) {
-    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, stream, q.count);
-    hipError_t e = reset_counters(L, stream);
+    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, stream, q.count, L.counter, L.stats);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev[0], stream)) != hipSuccess) return e;
     if (L.total > 0) {
@@ -488,8 +486,7 @@ hipError_t trace_closed_form(const TraceLaunch& L, const RayQueue& q, int fp32, 
             // (a small grid striding over region 1, whose length it reads on the device)
             k_closed_form<kFast><<<dim3((unsigned)std::min<long long>((L.total + 255) / 256, 4096)), block, lds,
                                    stream>>>(L, q);
-            k_closed_form<kRedo><<<dim3(256), block, lds, stream>>>(L, q);
-            hipLaunchKernelGGL(k_zero_region1, dim3(1), dim3(64), 0, stream, q.count);
+            k_closed_form<kRedo><<<dim3(32), block, lds, stream>>>(L, q);
         } else {
             k_closed_form<kAll><<<dim3((unsigned)std::min<long long>((L.total + 255) / 256, 2048)), block, lds,
                                   stream>>>(L, q);
@@ -502,6 +499,7 @@ hipError_t trace_closed_form(const TraceLaunch& L, const RayQueue& q, int fp32, 
     F.total_dev = nullptr;
     F.bucket_count = q.count;
     F.bucket_stride = q.stride;
+    F.n_buckets = 1;  // region 0 only: region 1 held the FP32 pass's unsure rays
     F.no_reset = 1;
     F.mode = 1;  // the one-shot form over the queue: its grid is capped near the usual queue length
     if ((e = launch_trace(F, stream)) != hipSuccess) return e;
