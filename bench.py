@@ -462,7 +462,7 @@ def main():
             other.closed_form = 1 if opts.closed_form == 2 else 2
             alt = [(f"closed_form_{'fp64' if other.closed_form == 1 else 'fp32'}", other, bsa.FLAG_CLOSED_FORM)]
             load_into(tr, args.weights or FAST_WEIGHTS)
-            alt += [("learned_surrogate_closed_form_values", surrogate_opts(1), bsa.FLAG_SURROGATE),
+            alt += [("learned_surrogate_closed_form_values", surrogate_opts(opts.closed_form), bsa.FLAG_SURROGATE),
                     ("learned_surrogate_network_values", surrogate_opts(0), bsa.FLAG_SURROGATE)]
     elif args.path == "plain":
         opts = plain

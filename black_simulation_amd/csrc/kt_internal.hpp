@@ -38,24 +38,23 @@ struct TraceLaunch {
 constexpr int kQueueBuckets = 3;
 
 // A fallback queue in device memory: kQueueBuckets regions `stride` entries apart, region b
-// holding count[b] ray indices; count[kQueueBuckets] is a spare counter (the surrogate's
-// tile counter) and *total the sum of the regions as the integrator reads it.
+// holding count[b] ray indices, which the integrator reads; count[kQueueBuckets] is a spare
+// counter (the surrogate's tile counter); region kQueueBuckets with count[kUnsureCount]
+// holds the rays the closed form's FP32 pass leaves to its FP64 one; *total the sum of the
+// integrator's regions as it reads it.
+constexpr int kUnsureRegion = kQueueBuckets, kUnsureCount = kQueueBuckets + 1, kQueueCounts = kQueueBuckets + 2;
 struct RayQueue {
-    long long* list;
-    unsigned long long* count;
+    long long* list;  // (kQueueBuckets + 1) x stride
+    unsigned long long* count;  // kQueueCounts
     long long* total;
     long long stride;
 };
 
-// The closed-form solution (kt_closed.hpp) over the launch: polish = 0 answers every ray it
-// covers and queues the others (region 0); polish = 1 runs after the surrogate's network on
-// the rays it answered as escaped or disk hits, replaces their values and re-queues (in the
-// region of the network's class) those it declines or classes otherwise.
-hipError_t launch_closed_form(const TraceLaunch& L, const RayQueue& q, int polish, hipStream_t stream);
-// Every ray by the closed form (fp32: FP32 elliptic integrals, FP64 for the rays that
-// precision leaves unsure; else FP64 throughout), then the integrator (L's precision) over
-// the rays it declines: zeroes the queue and counters, and writes the queue's length to
-// stats[kStatFallback].  ev[0..2] (optional) bracket the closed form and the integrator.
+// The closed-form solution (kt_closed.hpp) after the surrogate's network: its values for the
+// rays the network answered as escaped or disk hits (fp32: an FP32 pass, then FP64 for the
+// rays it leaves unsure); a ray it declines or classes otherwise joins the region of the
+// network's class.
+hipError_t launch_closed_form_polish(const TraceLaunch& L, const RayQueue& q, int fp32, hipStream_t stream);
 hipError_t trace_closed_form(const TraceLaunch& L, const RayQueue& q, int fp32, hipStream_t stream, hipEvent_t* ev);
 
 // launches the counter reset (unless L.no_reset) and the trace kernel on `stream`; returns a

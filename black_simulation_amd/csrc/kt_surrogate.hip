@@ -1880,14 +1880,14 @@ hipError_t train(Surrogate* s, const TrainOpts& o, TrainReport& rep) {
 
 hipError_t queue_reserve(Queue& q, long long n) {
     if (!q.count) {
-        KTS_TRY(dalloc(&q.count, 4));
+        KTS_TRY(dalloc(&q.count, kti::kQueueCounts));
         KTS_TRY(dalloc(&q.total, 1));
     }
     if (n <= q.cap) return hipSuccess;
     if (q.list) (void)hipFree(q.list);
     q.list = nullptr;
     q.cap = 0;
-    KTS_TRY(dalloc(&q.list, (size_t)n * kti::kQueueBuckets));
+    KTS_TRY(dalloc(&q.list, (size_t)n * (kti::kQueueBuckets + 1)));
     q.cap = n;
     return hipSuccess;
 }
@@ -1907,7 +1907,7 @@ hipError_t trace_with_surrogate(Surrogate* s, const kti::TraceLaunch& L, double 
         if (!ev) return hipErrorInvalidValue;  // under capture: the queue must exist already
         KTS_TRY(queue_reserve(Q, n));
     }
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, Q.count, 4);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, Q.count, kti::kQueueCounts);
     KTS_TRY(kti::reset_counters(L, st));
     // the network and the gate in one kernel: answered rays get their outputs, the
     // others join the fallback queue, whose length the integrator reads on the device:
@@ -1930,7 +1930,8 @@ hipError_t trace_with_surrogate(Surrogate* s, const kti::TraceLaunch& L, double 
     // the closed form's values for the network's escaped and disk answers: the network
     // picks the class and the rays the gate trusts, the elliptic integrals give the numbers
     if (closed_form)
-        KTS_TRY(kti::launch_closed_form(L, kti::RayQueue{Q.list, Q.count, Q.total, Q.cap}, 1, st));
+        KTS_TRY(kti::launch_closed_form_polish(L, kti::RayQueue{Q.list, Q.count, Q.total, Q.cap}, closed_form == 2,
+                                               st));
     // the integrator on the queue the gate left, its regions read in order and its length
     // read on the device (the counters were zeroed before the network).  The rays queued
     // are the slowest of the frame (photon ring, class boundaries), so the launch is bound

@@ -303,34 +303,33 @@ __device__ __forceinline__ bool closed_ray(const ViewDev& v, const Launch& la, k
     return true;
 }
 
-// The closed-form passes, one lane per ray, the grid striding over the rays:
-//   kAll:    every ray of the launch in FP64; declined rays join queue region 0 (the integrator's)
-//   kFast:   every ray with FP32 elliptic integrals; rays it is unsure of join region 1,
-//            declined ones region 0
-//   kRedo:   the rays of region 1 (its length read on the device) in FP64
-//   kPolish: the rays the surrogate answered as escaped or disk hits, in FP64; rays it
-//            declines or classes otherwise join the region of the network's class
-// Answered rays get the closed form's outputs (class | kFlagClosedForm, 0 steps); joining
-// a region takes one atomic per wave and region.
-enum : int { kAll = 0, kFast = 1, kRedo = 2, kPolish = 3 };
+// The closed-form passes, one lane per ray, the grid striding over the rays: over every ray
+// of the launch (SRC kEvery), the rays the surrogate answered as escaped or disk hits (SRC
+// kAnswered), or the unsure region (SRC kUnsure, its length read on the device), with the
+// elliptic integrals in FP32 (FP32) or FP64.  Answered rays get the closed form's outputs
+// (class | kFlagClosedForm, 0 steps, and kFlagSurrogate after the network); rays the FP32
+// pass is unsure of join the unsure region, rays it declines (or, after the network, classes
+// otherwise) the integrator's region 0 (after the network, that of the network's class).
+// Joining a region takes one atomic per wave and region.
+enum : int { kEvery = 0, kAnswered = 1, kUnsure = 2 };
 
-template <int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == kFast ? 4 : 2))) void k_closed_form(
+template <int SRC, bool POLISH, bool FP32>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FP32 ? 4 : 2))) void k_closed_form(
     kti::TraceLaunch L, kti::RayQueue rq) {
     const ViewDev* views = stage_views(L, view_lds);
     const int lane = (int)(threadIdx.x & 63u);
     const long long stride = (long long)gridDim.x * blockDim.x;
-    const long long total = MODE == kRedo ? (long long)rq.count[1] : L.total;
+    const long long total = SRC == kUnsure ? (long long)rq.count[kti::kUnsureCount] : L.total;
     for (long long base = (long long)blockIdx.x * blockDim.x; base < total; base += stride) {  // block-uniform
         const long long i = base + threadIdx.x;
         long long q = 0;
         bool queue = false;
         int region = 0;
         if (i < total) {
-            q = MODE == kRedo ? rq.list[rq.stride + i] : i;
-            int prior = -1;  // polish: the network's class
+            q = SRC == kUnsure ? rq.list[kti::kUnsureRegion * rq.stride + i] : i;
+            int prior = -1;  // after the network: its class
             bool take = true;
-            if (MODE == kPolish) {
+            if (POLISH) {
                 const int c = L.out.cls[q];
                 prior = c & 7;
                 take = (c & kFlagSurrogate) && (prior == kEscaped || prior == kDisk);
@@ -342,12 +341,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == kFa
                 ktc::Ray ry;
                 ktc::Outcome o{ktc::kDeclined, 0.0, 0.0, 0.0, 0.0};
                 if (closed_ray(v, la, ry)) {
-                    if constexpr (MODE == kFast) o = ktc::solve_t<float>(ry);
+                    if constexpr (FP32) o = ktc::solve_t<float>(ry);
                     else o = ktc::solve_t<double>(ry);
                 }
-                if (o.cls < 0 || (MODE == kPolish && o.cls != prior)) {
+                if (o.cls == ktc::kUncertain) {
                     queue = true;
-                    region = MODE == kPolish ? prior : (o.cls == ktc::kUncertain ? 1 : 0);
+                    region = kti::kUnsureRegion;
+                } else if (o.cls < 0 || (POLISH && o.cls != prior)) {
+                    queue = true;
+                    region = POLISH ? prior : 0;
                 } else {
                     Result r;
                     r.cls = o.cls;
@@ -356,17 +358,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == kFa
                     r.phi = o.phi;
                     r.r_hit = o.r_hit;
                     r.g = o.g;
-                    write_result(L.out, q, r, 0, kFlagClosedForm | (MODE == kPolish ? kFlagSurrogate : 0));
+                    write_result(L.out, q, r, 0, kFlagClosedForm | (POLISH ? kFlagSurrogate : 0));
                 }
             }
         }
 #pragma unroll
-        for (int b = 0; b < kti::kQueueBuckets; ++b) {
+        for (int b = 0; b <= kti::kQueueBuckets; ++b) {
             const unsigned long long mask = __ballot(queue && region == b);
             if (mask) {
                 const int first = __ffsll((long long)mask) - 1;
                 unsigned long long at = 0;
-                if (lane == first) at = atomicAdd(rq.count + b, (unsigned long long)__popcll(mask));
+                const int ci = b == kti::kUnsureRegion ? kti::kUnsureCount : b;
+                if (lane == first) at = atomicAdd(rq.count + ci, (unsigned long long)__popcll(mask));
                 at = __shfl(at, first);
                 if (queue && region == b)
                     rq.list[b * rq.stride + (long long)(at + __popcll(mask & ((1ull << lane) - 1ull)))] = q;
@@ -377,7 +380,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == kFa
 
 // the queue's region counts, and the launch's queue head and counters
 __global__ void k_zero_queue(unsigned long long* count, unsigned long long* head, unsigned long long* stats) {
-    if ((int)threadIdx.x <= kti::kQueueBuckets) count[threadIdx.x] = 0ull;
+    if ((int)threadIdx.x < kti::kQueueCounts) count[threadIdx.x] = 0ull;
     if (threadIdx.x == 0) *head = 0ull;
     if (threadIdx.x < kti::kStatCount) stats[threadIdx.x] = 0ull;
 }
@@ -463,13 +466,19 @@ hipError_t reset_counters(const TraceLaunch& L, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t launch_closed_form(const TraceLaunch& L, const RayQueue& q, int polish, hipStream_t stream) {
+// one lane per ray up to `cap` blocks of 256, striding beyond
+static dim3 closed_grid(long long n, long long cap) { return dim3((unsigned)std::max(1LL, std::min((n + 255) / 256, cap))); }
+
+hipError_t launch_closed_form_polish(const TraceLaunch& L, const RayQueue& q, int fp32, hipStream_t stream) {
     if (L.total <= 0) return hipSuccess;
-    // one lane per ray up to 2048 blocks of 256, striding beyond
-    const dim3 grid((unsigned)std::min<long long>((L.total + 255) / 256, 2048)), block(256);
     const size_t lds = view_lds_bytes(L.n_views);
-    if (polish) k_closed_form<kPolish><<<grid, block, lds, stream>>>(L, q);
-    else k_closed_form<kAll><<<grid, block, lds, stream>>>(L, q);
+    const dim3 block(256);
+    if (fp32) {
+        k_closed_form<kEvery, true, true><<<closed_grid(L.total, 4096), block, lds, stream>>>(L, q);
+        k_closed_form<kUnsure, true, false><<<dim3(32), block, lds, stream>>>(L, q);
+    } else {
+        k_closed_form<kEvery, true, false><<<closed_grid(L.total, 2048), block, lds, stream>>>(L, q);
+    }
     return hipGetLastError();
 }
 
@@ -483,13 +492,11 @@ hipError_t trace_closed_form(const TraceLaunch& L, const RayQueue& q, int fp32, 
         const dim3 block(256);
         if (fp32) {
             // the FP32 pass over every ray, then the FP64 pass over the rays it was unsure of
-            // (a small grid striding over region 1, whose length it reads on the device)
-            k_closed_form<kFast><<<dim3((unsigned)std::min<long long>((L.total + 255) / 256, 4096)), block, lds,
-                                   stream>>>(L, q);
-            k_closed_form<kRedo><<<dim3(32), block, lds, stream>>>(L, q);
+            // (a small grid striding over the unsure region, whose length it reads on the device)
+            k_closed_form<kEvery, false, true><<<closed_grid(L.total, 4096), block, lds, stream>>>(L, q);
+            k_closed_form<kUnsure, false, false><<<dim3(32), block, lds, stream>>>(L, q);
         } else {
-            k_closed_form<kAll><<<dim3((unsigned)std::min<long long>((L.total + 255) / 256, 2048)), block, lds,
-                                  stream>>>(L, q);
+            k_closed_form<kEvery, false, false><<<closed_grid(L.total, 2048), block, lds, stream>>>(L, q);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
@@ -499,7 +506,7 @@ hipError_t trace_closed_form(const TraceLaunch& L, const RayQueue& q, int fp32, 
     F.total_dev = nullptr;
     F.bucket_count = q.count;
     F.bucket_stride = q.stride;
-    F.n_buckets = 1;  // region 0 only: region 1 held the FP32 pass's unsure rays
+    F.n_buckets = 1;  // region 0 only
     F.no_reset = 1;
     F.mode = 1;  // the one-shot form over the queue: its grid is capped near the usual queue length
     if ((e = launch_trace(F, stream)) != hipSuccess) return e;

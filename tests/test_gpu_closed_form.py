@@ -201,18 +201,19 @@ def test_plan_replays_equal_the_eager_call(tracer):
         plan.close()
 
 
-def test_surrogate_with_closed_form_values(tracer):
+@pytest.mark.parametrize("closed", [1, 2])
+def test_surrogate_with_closed_form_values(tracer, closed):
     """use_surrogate with closed_form: the network's escaped and disk answers carry the closed
-    form's values, bit-equal to the closed-form path's; every class as the oracle's on >= 99.9 %
-    of rays (the network's captured answers are its own)"""
+    form's values, bit-equal to the closed-form path's at the same precision; every class as
+    the oracle's on >= 99.9 % of rays (the network's captured answers are its own)"""
     from black_simulation_amd.train_surrogate import FAST_WEIGHTS, load_into
 
     load_into(tracer, FAST_WEIGHTS)
     views = [c5_view(k, 192, 108) for k in (5, 70, 115)]
     print("surrogate + closed-form values: 3 orbit frames", flush=True)
-    sur = bsa.to_numpy(tracer.trace_views(views, bsa.TraceOpts(use_surrogate=1, closed_form=1, rtol=1e-6,
+    sur = bsa.to_numpy(tracer.trace_views(views, bsa.TraceOpts(use_surrogate=1, closed_form=closed, rtol=1e-6,
                                                                 atol=1e-7)))
-    cf = bsa.to_numpy(tracer.trace_views(views, CLOSED))
+    cf = bsa.to_numpy(tracer.trace_views(views, bsa.TraceOpts(closed_form=closed)))
     cls = sur["cls"] & 7
     polished = (sur["cls"] & bsa.FLAG_CLOSED_FORM) != 0
     answered = (sur["cls"] & bsa.FLAG_SURROGATE) != 0
