@@ -303,6 +303,44 @@ __device__ __forceinline__ bool closed_ray(const ViewDev& v, const Launch& la, k
     return true;
 }
 
+// The same for ray `local` of view v: a pixel of a Bardeen plane straight from its (alpha, beta)
+// (src/eht.cpp:262-271: E = 1, L = -alpha sin i, Q = beta^2 + (alpha^2 - a^2) cos^2 i,
+// inward, toward smaller theta when beta > 0; declined where R or M vanishes at the camera,
+// as a launch with w_r or w_mu zero), other kinds through their launch
+__device__ __forceinline__ bool closed_ray_of(const ViewDev& v, long long local, ktc::Ray& ry) {
+    if (v.kind == 0 && !v.params) {
+        const int row = (int)(local / v.width), col = (int)(local - (long long)row * v.width);
+        const double alpha = -v.half_width + ((double)col + 0.5) * v.px;
+        const double beta = v.half_height - ((double)row + 0.5) * v.py;
+        const double a = v.a, lam = -alpha * v.sinI;
+        const double eta = beta * beta + (alpha * alpha - a * a) * v.cosI * v.cosI;
+        const double m2 = v.m0 * v.m0;
+        const double M = eta - (eta + lam * lam - a * a) * m2 - a * a * m2 * m2;  // sin^2 theta Theta at the camera
+        const double r = v.r_cam, P = r * r + a * a - a * lam, lma = lam - a;
+        const double R = P * P - (r * r - 2.0 * r + a * a) * (eta + lma * lma);  // R(r) at the camera
+        if (!(M > 0.0) || !(R > 0.0)) return false;
+        ry.a = a;
+        ry.E = 1.0;
+        ry.lam = lam;
+        ry.eta = eta;
+        ry.r_o = v.r_cam;
+        ry.mu_o = v.m0;
+        ry.phi_o = v.phi_cam;
+        ry.s_r = -1;
+        ry.s_mu = beta > 0.0 ? 1 : -1;
+        ry.r_in = v.r_in;
+        ry.r_out = v.r_out;
+        ry.r_cap = v.r_cap;
+        ry.r_esc = v.r_esc;
+        ry.disk = v.disk;
+        ry.prograde = v.prograde;
+        return true;
+    }
+    Launch la;
+    launch_photon(v, local, la);
+    return closed_ray(v, la, ry);
+}
+
 // The closed-form passes, one lane per ray, the grid striding over the rays: over every ray
 // of the launch (SRC kEvery), the rays the surrogate answered as escaped or disk hits (SRC
 // kAnswered), or the unsure region (SRC kUnsure, its length read on the device), with the
@@ -336,11 +374,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FP32 ? 4 : 
             }
             if (take) {
                 const ViewDev& v = views[find_view(views, L.n_views, q)];
-                Launch la;
-                launch_photon(v, q - v.offset, la);
                 ktc::Ray ry;
                 ktc::Outcome o{ktc::kDeclined, 0.0, 0.0, 0.0, 0.0};
-                if (closed_ray(v, la, ry)) {
+                if (closed_ray_of(v, q - v.offset, ry)) {
                     if constexpr (FP32) o = ktc::solve_t<float>(ry);
                     else o = ktc::solve_t<double>(ry);
                 }
