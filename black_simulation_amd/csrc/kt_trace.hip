@@ -544,7 +544,8 @@ hipError_t trace_closed_form(const TraceLaunch& L, const RayQueue& q, int fp32, 
     F.bucket_stride = q.stride;
     F.n_buckets = 1;  // region 0 only
     F.no_reset = 1;
-    F.mode = 1;  // the one-shot form over the queue: its grid is capped near the usual queue length
+    F.mode = 1;   // the one-shot form over the queue: its grid is capped near the usual queue length,
+    F.grid = 128;  // a few hundred rays at most here (an empty launch of a frame-sized grid costs 8 us)
     if ((e = launch_trace(F, stream)) != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev[2], stream)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_queue_total, dim3(1), dim3(64), 0, stream, q.count, q.total, L.stats + kStatFallback);
