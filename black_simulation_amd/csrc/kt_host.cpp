@@ -761,7 +761,8 @@ int kt_last_stats(const kt_ctx* ctx_c, kt_stats* s) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]) == hipSuccess) ctx->stats.kernel_ms = ms;
         if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) == hipSuccess) ctx->stats.integrator_ms = ms;
-        ctx->stats.surrogate_ms = ctx->stats.kernel_ms - ctx->stats.integrator_ms;
+        // ev[0] .. ev[1] bracket the network or closed-form kernels (nothing on the integrator path)
+        if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->stats.surrogate_ms = ms;
         unsigned long long h[1 + kti::kStatCount];
         KT_HIP(hipMemcpy(h, ctx->stats_src ? ctx->stats_src : ctx->d_counter, sizeof h, hipMemcpyDeviceToHost));
         ctx->stats.steps = (long long)h[1 + kti::kStatSteps];
