@@ -48,6 +48,7 @@ OPS_PER_PARTICLE_ATTEMPT = 656
 # (the factors a 1 GiB device copy reads) from rocprofv3 --pmc runs of the same command;
 # the summaries (commands included) are the named files under profiles/
 PROFILED_TRAFFIC = {
+    ("orbit/closed", 1080, "k_closed_form"): (0.402e6 + 76.73e6, "profiles/k_closed_form_orbit_1080p.md"),
     ("orbit/surrogate", 1080, "k_mlp_fast"): (1.220e6 + 75.30e6, "profiles/k_mlp_fast_orbit_1080p.md"),
     ("orbit/surrogate", 1080, "k_trace_oneshot"): (1.203e6 + 3.885e6,
                                                    "profiles/k_trace_oneshot_fallback_orbit_1080p.md"),
@@ -63,16 +64,21 @@ def profiled_traffic(workload, size, kernel):
         if w == workload and n == size and kernel.startswith(k):
             return v
     return None, None
+
+
 # Operations per ray of the closed-form solution (kt_closed.hpp) over the rays of the 1080p
 # orbit frames, counted by its Python restatement (tests/closed_form.py `count_ops`: + - * /
 # sqrt once, FMA twice, each iteration of the special functions as run): the FP64 form, and
 # the FP32 pass with the FP64 one for the rays it leaves unsure
 CLOSED_FORM_OPS_PER_RAY = {"fp64": 305, "fp32": 288}
-# The network 5-256-256-256-256-8: 2 (5*256 + 3*256*256 + 256*8) multiply-adds per ray
+
+
 def mlp_flop_per_ray(hidden: int) -> int:
     """multiply-adds x 2 of the network 5 -> H -> H -> H -> H -> 8: 399,872 at H = 256 (the
     wide network), 101,632 at H = 128 (the fast one)"""
     return 2 * (5 * hidden + 3 * hidden * hidden + hidden * 8)
+
+
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 BF16_MFMA_PEAK_TFLOPS = 2500.0
@@ -666,8 +672,10 @@ def main():
         cops = CLOSED_FORM_OPS_PER_RAY[cprec]
         cpeak = FP32_VECTOR_PEAK_TFLOPS if cprec == "fp32" else FP64_VECTOR_PEAK_TFLOPS
         cf_ach = rays * cops / (avg_sur * 1e-3) / 1e12 if avg_sur > 0 else 0.0
-        kcf = "k_closed_form<kFast> + k_closed_form<kRedo>" if cprec == "fp32" else "k_closed_form<kAll>"
-        tr_cf, src_cf = profiled_traffic(wkey, args.height, "k_closed_form")
+        kcf = ("k_closed_form<kEvery> FP32 pass + k_closed_form<kUnsure> FP64 re-solve" if cprec == "fp32"
+               else "k_closed_form<kEvery> FP64")
+        # (the profile covers the FP32 pass and its FP64 re-solve)
+        tr_cf, src_cf = profiled_traffic(wkey, args.height, "k_closed_form") if cprec == "fp32" else (None, None)
         roof = {"bound": "valu", "achieved": round(cf_ach, 3), "peak": cpeak, "unit": "TFLOP/s",
                 "frac": round(cf_ach / cpeak, 4), "traffic": tr_cf, "traffic_source": src_cf,
                 "kernel": kcf, "kernel_ms": round(avg_sur, 3), "ops_per_ray": cops,
