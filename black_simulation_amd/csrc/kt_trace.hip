@@ -506,10 +506,12 @@ hipError_t reset_counters(const TraceLaunch& L, hipStream_t stream) {
 
 // one lane per ray up to `cap` blocks of 256, striding beyond
 static dim3 closed_grid(long long n, long long cap) { return dim3((unsigned)std::max(1LL, std::min((n + 255) / 256, cap))); }
-// blocks of the FP64 pass over the rays the FP32 pass left unsure (about one in a thousand;
-// one block per CU, most of which find no work: a frame with many escapes takes 0.13 ms
-// instead of 0.15 with 32)
-constexpr unsigned kRedoBlocks = 256;
+// blocks of the FP64 pass over the rays the FP32 pass left unsure: the escapes and the
+// crossings near a disk edge, one ray in a thousand on the orbit but one in five where the
+// sky fills the frame's edges.  1024 blocks of 256 fill the chip at the kernel's four waves
+// per SIMD; blocks past the region's end return at once (on the orbit 1024 blocks cost what
+// 32 do; on a frame with 20% escapes they take 0.18 ms where 256 take 0.20 and 32 take 0.64)
+constexpr unsigned kRedoBlocks = 1024;
 
 hipError_t launch_closed_form_polish(const TraceLaunch& L, const RayQueue& q, int fp32, hipStream_t stream) {
     if (L.total <= 0) return hipSuccess;
