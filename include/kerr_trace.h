@@ -261,8 +261,9 @@ int kt_plan_stats(kt_plan* plan, kt_stats* stats);
 int kt_plan_destroy(kt_plan* plan);
 
 /* ---- the learned surrogate (no reference counterpart) ----------------------------------
- * An MLP with BF16 weights on the matrix cores, FP32 accumulation and FP32 master weights,
- * one of two networks: KT_NET_WIDE, 5-256-256-256-256-8 with SiLU, and KT_NET_FAST,
+ * An MLP on the matrix cores with FP32 accumulation and FP32 master weights (BF16 operands
+ * in training; FP16 after the input layer in the fast network's inference), one of two
+ * networks: KT_NET_WIDE, 5-256-256-256-256-8 with SiLU, and KT_NET_FAST,
  * 5-128-128-128-128-8 with hard SiLU (x clamp(x/6 + 1/2, 0, 1)), a quarter of the matrix
  * work and no transcendentals.  It answers a ray when its largest class probability
  * exceeds opts.gate_prob and the ray lies outside opts.gate_ring of the critical curve;
