@@ -270,11 +270,12 @@ template <class R>
 KTC_FN R ell_f(R s, R m) {
     return s * rf<R>((R(1) - s) * (R(1) + s), R(1) - m * s * s, R(1));
 }
-// the same from s and its cofactors c2 = 1 - s^2, d2 = 1 - m s^2 formed in FP64 by the
-// caller: near s = 1 (a turning point) the amplitude is ill-conditioned in s, not in c2
+// the same from s and its cofactors c2 = 1 - s^2, d2 = 1 - m s^2, which the caller forms
+// from FP64 differences: near s = 1 (a turning point) the amplitude is ill-conditioned in s,
+// not in c2
 template <class R>
-KTC_FN R ell_f2(double s, double c2, double d2) {
-    return R(s) * rf<R>(R(c2), R(d2), R(1));
+KTC_FN R ell_f2(R s, R c2, R d2) {
+    return s * rf<R>(c2, d2, R(1));
 }
 KTC_FN double ell_pi(double n, double s, double m) {
     const double c2 = (1.0 - s) * (1.0 + s), d2 = 1.0 - m * s * s, s2 = s * s;
@@ -397,10 +398,19 @@ KTC_FN double resolvent_from(double c2, double c1, double c0, bool fp64) {
         t = 2.0 * r * (fp64 ? cos(acos(ca) * (1.0 / 3.0)) : (double)cosf(acosf((float)ca) * (1.0f / 3.0f)));
     }
     double xi = t - c2 * (1.0 / 3.0);
+    // The step's correction needs only a few digits: an FP32 reciprocal of the derivative
+    // (relative error ~1e-7) keeps the convergence quadratic down to 1e-7 of the error, so
+    // three steps from the FP32 estimate still end at FP64 rounding.  The FP64 attempt
+    // divides in FP64.
     for (int it = 0; it < 3; ++it) {
         const double f = ((xi + c2) * xi + c1) * xi + c0;
         const double df = (3.0 * xi + 2.0 * c2) * xi + c1;
-        if (df != 0.0) xi -= f * qrcp(df);
+        if (fp64) {
+            if (df != 0.0) xi -= f * qrcp(df);
+        } else {
+            const float d = (float)df;
+            if (d != 0.0f) xi -= f * (double)qrcp(d);
+        }
     }
     return xi;
 }
@@ -471,6 +481,8 @@ struct Polar {
     R Fo, K, s1;
 };
 
+// u+ and the cofactor 1 - x^2 = (u+ - mu_o^2) / u+ need FP64 (mu_o^2 may lie close to u+);
+// omega, m and x feed only precision-R integrals and are formed in R.
 template <class R>
 KTC_FN Polar<R> polar_motion(const Ray& ry, double A) {
     Polar<R> P;
@@ -478,21 +490,22 @@ KTC_FN Polar<R> polar_motion(const Ray& ry, double A) {
     const double disc = qsqrt(A * A + 4.0 * a2 * eta);
     const double om2 = A <= 0.0 ? 0.5 * (disc - A) : 2.0 * a2 * eta * qrcp(A + disc);  // a^2 (-u-)
     const double iom2 = qrcp(om2);
-    P.om = qsqrt(om2);
     P.up = eta * iom2;
     P.m = -a2 * eta * iom2 * iom2;
-    const double iup = qrcp(P.up);
-    P.x = fmin(1.0, fmax(-1.0, ry.mu_o * qrsqrt(P.up)));
-    const double c2 = fmax(0.0, (P.up - ry.mu_o * ry.mu_o) * iup);  // 1 - x^2
-    P.Fo = ell_f2<R>(P.x, c2, 1.0 - P.m * P.x * P.x);
-    P.K = ell_k<R>(R(P.m));
+    const R up = R(P.up), iup = qrcp(up);
+    P.om = double(qsqrt(R(om2)));
+    P.x = fmin(1.0, fmax(-1.0, double(R(ry.mu_o) * qrsqrt(up))));
+    const R c2 = R(fmax(0.0, P.up - ry.mu_o * ry.mu_o)) * iup;  // 1 - x^2
+    const R xr = R(P.x), mr = R(P.m);
+    P.Fo = ell_f2<R>(xr, c2, R(1) - mr * xr * xr);
+    P.K = ell_k<R>(mr);
     // F at the next psi = j pi in the direction of motion; starting on the plane is no crossing
     const double x = P.x;
     R F1;
     if (ry.s_mu > 0) F1 = (x > 0.0 ? R(2) * P.K : R(0)) - P.Fo;
     else F1 = P.Fo - (x > 0.0 ? R(0) : R(-2) * P.K);
     if (x == 0.0) F1 = R(2) * P.K;
-    P.s1 = F1 * R(qrcp(P.om));
+    P.s1 = F1 * qrcp(R(P.om));
     return P;
 }
 
@@ -575,8 +588,12 @@ KTC_FN Outcome solve_t(const Ray& ry) {
     // pair: cos phi = (Aq (r - r1) - Bq (r - r2)) / (Aq (r - r1) + Bq (r - r2)),
     // Aq = |r3 - r2|, Bq = |r3 - r1|).  S_t: J at the camera, signed so that the ray's own
     // Mino time s reaches rt at s = S_t (S_t < 0: rt lies behind an outward start).
+    // The roots, the Moebius coefficients and every comparison with a radius are FP64; the
+    // modulus k (k' = sqrt(1 - k)), the prefactor and the amplitudes with their cofactors
+    // feed only precision-R integrals and are formed in R from FP64 root differences.
     bool real4, turning = false;
-    double pref, k, kp, P0, P1, Q0, Q1, r1 = 0.0, r3 = 0.0, r4 = 0.0, x_o = 0.0, x_inf = 0.0, c2_o = 1.0, c2_inf = 1.0;
+    double P0, P1, Q0, Q1, r1 = 0.0, r3 = 0.0, r4 = 0.0;
+    R pref, k, kp, x_o = R(0), x_inf = R(0), c2_o = R(1), c2_inf = R(1);
     R S_t;
     Agm<R> G;  // the radial modulus's mean (for the crossings)
     if (D34 >= 0.0) {
@@ -586,11 +603,11 @@ KTC_FN Outcome solve_t(const Ray& ry) {
         const double r2 = -z + sD12;
         r3 = z - sD34;
         r4 = z + sD34;
-        const double r31 = r3 - r1, r41 = r4 - r1, r42 = r4 - r2, r32 = r3 - r2;
-        const double i3142 = qrcp(r31 * r42);
-        k = r32 * r41 * i3142;
-        kp = qsqrt((r2 - r1) * (r4 - r3) * i3142);  // sqrt(1 - k)
-        pref = 2.0 * qrsqrt(r31 * r42);
+        const double r31 = r3 - r1, r41 = r4 - r1, r42 = r4 - r2, r32 = r3 - r2, r43 = r4 - r3;
+        const R i3142 = qrcp(R(r31 * r42));
+        k = R(r32 * r41) * i3142;
+        kp = qsqrt(R((r2 - r1) * r43) * i3142);  // sqrt(1 - k)
+        pref = R(2) * qrsqrt(R(r31 * r42));
         turning = r4 > ry.r_cap;
         if (turning && (r4 < ry.r_cap * (1.0 + 1e-7) || !(r4 < ry.r_o))) return out;
         if (r4 == ry.r_cap) return out;
@@ -599,15 +616,14 @@ KTC_FN Outcome solve_t(const Ray& ry) {
         P1 = -r41 * r3;
         Q0 = r31;
         Q1 = -r41;
-        const double i41 = qrcp(r41), io3 = qrcp(r41 * (ry.r_o - r3));
-        x_o = qsqrt(r31 * (ry.r_o - r4) * io3);
-        x_inf = qsqrt(r31 * i41);
-        const double r43 = r4 - r3;
-        c2_o = r43 * (ry.r_o - r1) * io3;  // 1 - x_o^2
-        c2_inf = r43 * i41;                // 1 - x_inf^2
-        const R I_o = R(pref) * ell_f2<R>(x_o, c2_o, kp * kp + k * c2_o);
+        const R i41 = qrcp(R(r41)), io3 = qrcp(R(r41 * (ry.r_o - r3)));
+        x_o = qsqrt(R(r31 * (ry.r_o - r4)) * io3);
+        x_inf = qsqrt(R(r31) * i41);
+        c2_o = R(r43 * (ry.r_o - r1)) * io3;  // 1 - x_o^2
+        c2_inf = R(r43) * i41;                // 1 - x_inf^2
+        const R I_o = pref * ell_f2<R>(x_o, c2_o, kp * kp + k * c2_o);
         S_t = ry.s_r < 0 ? I_o : -I_o;
-        if (cross) agm_init<R>(G, R(kp));
+        if (cross) agm_init<R>(G, kp);
     } else {
         if (ry.s_r > 0) return out;  // no turning point: an outgoing ray escapes by forms not given here
         if (!(D12 >= 0.0)) {
@@ -625,22 +641,23 @@ KTC_FN Outcome solve_t(const Ray& ry) {
             out.cls = 1;
             return out;
         }
-        const double im3 = qsqrt(-D34);
-        const double Aq = hypot(z - r2, im3), Bq = hypot(z - r1, im3);
+        const double d2 = z - r2, d1 = z - r1;
+        const double Aq = qsqrt(d2 * d2 - D34), Bq = qsqrt(d1 * d1 - D34);  // |r3 - r2|, |r3 - r1|
         const double r21 = r2 - r1;
         const double i4AB = qrcp(4.0 * Aq * Bq);
-        k = ((Aq + Bq) * (Aq + Bq) - r21 * r21) * i4AB;
-        kp = qsqrt(fmax(0.0, (r21 * r21 - (Aq - Bq) * (Aq - Bq)) * i4AB));  // sqrt(1 - k)
-        pref = qrsqrt(Aq * Bq);
+        k = R(((Aq + Bq) * (Aq + Bq) - r21 * r21) * i4AB);
+        kp = R(qsqrt(fmax(0.0, (r21 * r21 - (Aq - Bq) * (Aq - Bq)) * i4AB)));  // sqrt(1 - k)
+        pref = qrsqrt(R(Aq * Bq));
         P0 = Aq * r1 - Bq * r2;
         P1 = -(Aq * r1 + Bq * r2);
         Q0 = Aq - Bq;
         Q1 = -(Aq + Bq);
         const double ro = ry.r_o;
         const double c = (Aq * (ro - r1) - Bq * (ro - r2)) * qrcp(Aq * (ro - r1) + Bq * (ro - r2));
-        const R f = ell_f2<R>(qsqrt(fmax(0.0, (1.0 - c) * (1.0 + c))), c * c, kp * kp + k * c * c);
-        agm_init<R>(G, R(kp));
-        S_t = R(pref) * (c >= 0.0 ? f : R(2) * agm_k(G) - f);  // F(acos c) past pi/2
+        const R cc = R(c * c);
+        const R f = ell_f2<R>(qsqrt(R(fmax(0.0, (1.0 - c) * (1.0 + c)))), cc, kp * kp + k * cc);
+        agm_init<R>(G, kp);
+        S_t = pref * (c >= 0.0 ? f : R(2) * agm_k(G) - f);  // F(acos c) past pi/2
     }
 
     // ---- polar motion, and the disk at its equatorial crossings: inward of rt first; for a
@@ -649,14 +666,14 @@ KTC_FN Outcome solve_t(const Ray& ry) {
     if (eta > 0.0) P = polar_motion<R>(ry, A);
     R I_inf = R(-1);
     if (cross) {
-        const R ds = R(2) * P.K * R(qrcp(P.om));
-        const R ipref = R(qrcp(pref));
+        const R ds = R(2) * P.K * qrcp(R(P.om));
+        const R ipref = qrcp(pref);
         R s = P.s1, s_end = S_t;
         int j = 0;
         for (; j < kMaxCrossings; ++j) {
             if (!(s < s_end)) {
                 if (!turning || I_inf >= R(0)) break;
-                I_inf = R(pref) * ell_f2<R>(x_inf, c2_inf, kp * kp + k * c2_inf);
+                I_inf = pref * ell_f2<R>(x_inf, c2_inf, kp * kp + k * c2_inf);
                 s_end = S_t + I_inf;
                 if (!(s < s_end)) break;
             }

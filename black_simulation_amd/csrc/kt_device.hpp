@@ -455,6 +455,19 @@ __device__ __forceinline__ double isco_dev(double a, int prograde) {
     return prograde ? 3.0 + z2 - root : 3.0 + z2 + root;
 }
 
+// Row and column of pixel `local` of a view `width` pixels wide, row 0 at the top: a 32-bit
+// division where the index fits (a few instructions; a 64-bit one is a long sequence).
+__device__ __forceinline__ void pixel_rc(long long local, int width, int& row, int& col) {
+    if (local <= 0xffffffffLL) {
+        const unsigned l = (unsigned)local, w = (unsigned)width, r = l / w;
+        row = (int)r;
+        col = (int)(l - r * w);
+    } else {
+        row = (int)(local / width);
+        col = (int)(local - (long long)row * width);
+    }
+}
+
 // Launch: the photon of ray `local` of a view -- a pixel of a Bardeen plane (kind 0) or of
 // the pinhole camera (kind 1), an explicit (alpha, beta) or look, or (kind 2) a ray with
 // its own camera: params hold (a, r_cam, inclination, alpha, beta) per ray and the
@@ -493,7 +506,8 @@ __device__ __forceinline__ void launch_photon(const ViewDev& v, long long local,
             alpha = v.params[2 * local];
             beta = v.params[2 * local + 1];
         } else {
-            const int row = (int)(local / v.width), col = (int)(local - (long long)row * v.width);
+            int row, col;
+            pixel_rc(local, v.width, row, col);
             alpha = -v.half_width + ((double)col + 0.5) * v.px;
             beta = v.half_height - ((double)row + 0.5) * v.py;
         }
@@ -508,7 +522,8 @@ __device__ __forceinline__ void launch_photon(const ViewDev& v, long long local,
         n1 = v.params[3 * local + 1];
         n2 = v.params[3 * local + 2];
     } else {
-        const int row = (int)(local / v.width), col = (int)(local - (long long)row * v.width);
+        int row, col;
+        pixel_rc(local, v.width, row, col);
         const double W = (double)v.width, H = (double)v.height;
         const double nx = (((double)col + 0.5) - 0.5 * W) / (0.5 * H);
         const double ny = (((double)(v.height - row) - 0.5) - 0.5 * H) / (0.5 * H);

@@ -311,7 +311,8 @@ __device__ __forceinline__ bool closed_ray(const ViewDev& v, const Launch& la, k
 // as a launch with w_r or w_mu zero), other kinds through their launch
 __device__ __forceinline__ bool closed_ray_of(const ViewDev& v, long long local, ktc::Ray& ry) {
     if (v.kind == 0 && !v.params) {
-        const int row = (int)(local / v.width), col = (int)(local - (long long)row * v.width);
+        int row, col;
+        pixel_rc(local, v.width, row, col);
         const double alpha = -v.half_width + ((double)col + 0.5) * v.px;
         const double beta = v.half_height - ((double)row + 0.5) * v.py;
         const double a = v.a, lam = -alpha * v.sinI;
