@@ -44,6 +44,12 @@
 #define KTC_CALL inline
 #endif
 
+// Inside this solver a * b + c may fuse into one FMA (one rounding instead of two): its
+// answers are held to tolerances, not to bit patterns, and the fused form is the more
+// accurate one.  The library's own setting, no fusing (the integrators' bit-exact parity
+// with the oracle), is set again at the end of the file.
+#pragma clang fp contract(fast)
+
 namespace ktc {
 
 constexpr double kPiC = 3.14159265358979323846;
@@ -721,3 +727,5 @@ KTC_FN Outcome solve_t(const Ray& ry) {
 KTC_FN Outcome solve(const Ray& ry) { return solve_t<double>(ry); }
 
 }  // namespace ktc
+
+#pragma clang fp contract(off)
