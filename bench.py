@@ -70,7 +70,7 @@ def profiled_traffic(workload, size, kernel):
 # orbit frames, counted by its Python restatement (tests/closed_form.py `count_ops`: + - * /
 # sqrt once, FMA twice, each iteration of the special functions as run): the FP64 form, and
 # the FP32 pass with the FP64 one for the rays it leaves unsure
-CLOSED_FORM_OPS_PER_RAY = {"fp64": 305, "fp32": 288}
+CLOSED_FORM_OPS_PER_RAY = {"fp64": 293, "fp32": 276}
 
 
 def mlp_flop_per_ray(hidden: int) -> int:

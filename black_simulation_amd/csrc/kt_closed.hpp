@@ -46,8 +46,8 @@
 
 // Inside this solver a * b + c may fuse into one FMA (one rounding instead of two): its
 // answers are held to tolerances, not to bit patterns, and the fused form is the more
-// accurate one.  The library's own setting, no fusing (the integrators' bit-exact parity
-// with the oracle), is set again at the end of the file.
+// accurate one.  The library's own setting, no fusing (the integrators' FP64 results are
+// reproduced bit for bit on the CPU only without it), is set again at the end of the file.
 #pragma clang fp contract(fast)
 
 namespace ktc {
@@ -119,12 +119,13 @@ KTC_FN float qrcp(float x) {
     return 1.0f / x;
 #endif
 }
-// FP32 sin and cos: the quadrant and remainder in FP64, the Taylor series to the 11th and
-// 12th order in FP32 (truncation below 1e-11)
-KTC_FN void qsincos(float uf, float& s, float& c) {
-    const double u = (double)uf;
-    const double j = rint(u * 0.63661977236758134308);
-    const float x = (float)(u - j * 1.57079632679489661923);
+// FP32 sin and cos: u - j pi/2 by two fused steps, pi/2 in two parts (its FP32 rounding and
+// the remainder; each step rounds once, so x is good to 6e-8 for |j| < 2^20), then the
+// Taylor series to the 11th and 12th order (truncation below 1e-11)
+KTC_FN void qsincos(float u, float& s, float& c) {
+    const float j = rintf(u * 0.636619772f);
+    float x = fmaf(-j, 1.57079637f, u);
+    x = fmaf(-j, -4.37113883e-8f, x);
     const float x2 = x * x;
     float ps = -1.0f / 39916800.0f;
     ps = fmaf(ps, x2, 1.0f / 362880.0f);
@@ -139,7 +140,7 @@ KTC_FN void qsincos(float uf, float& s, float& c) {
     pc = fmaf(pc, x2, -1.0f / 24.0f);
     pc = fmaf(pc, x2, 0.5f);
     const float cx = fmaf(-pc, x2, 1.0f);
-    const long long q = (long long)j & 3;
+    const int q = (int)j & 3;
     const float a = (q & 1) ? cx : sx, b = (q & 1) ? sx : cx;
     s = (q & 2) ? -a : a;
     c = ((q + 1) & 2) ? -b : b;
@@ -382,7 +383,7 @@ KTC_CALL void sncndn(double u, double m, double& sn, double& cn, double& dn) {
 
 // the largest real root of the resolvent cubic xi^3 + A xi^2 + (A^2/4 - C) xi - B^2/8 of
 // the depressed quartic r^4 + A r^2 + B r + C (positive when B != 0): Cardano's or the
-// trigonometric form, then three Newton steps.  Their cube root, arccosine and cosine run
+// trigonometric form, then Newton steps.  Their cube root, arccosine and cosine run
 // in FP32 (an estimate good to ~1e-6, which the Newton steps make exact) unless the
 // result is not positive or leaves a residual, as where the cubic's roots crowd together
 // (B ~ 0); then in FP64.
@@ -405,10 +406,12 @@ KTC_FN double resolvent_from(double c2, double c1, double c0, bool fp64) {
     }
     double xi = t - c2 * (1.0 / 3.0);
     // The step's correction needs only a few digits: an FP32 reciprocal of the derivative
-    // (relative error ~1e-7) keeps the convergence quadratic down to 1e-7 of the error, so
-    // three steps from the FP32 estimate still end at FP64 rounding.  The FP64 attempt
-    // divides in FP64.
-    for (int it = 0; it < 3; ++it) {
+    // (relative error ~1e-7) keeps the convergence quadratic down to 1e-7 of the error.  The
+    // FP32 estimate is good to ~1e-7 (1e-6 at p99.9), so two steps end at FP64 rounding; a
+    // worse one (where the roots crowd) leaves the residual that sends the root to the FP64
+    // attempt, which divides in FP64 and takes three steps.  On the orbit frames, the a =
+    // 0.998 single frame and the 64 batch views, 5 rays in 433,000 take it.
+    for (int it = 0; it < (fp64 ? 3 : 2); ++it) {
         const double f = ((xi + c2) * xi + c1) * xi + c0;
         const double df = (3.0 * xi + 2.0 * c2) * xi + c1;
         if (fp64) {

@@ -226,7 +226,7 @@ def _resolvent_from(c2, c1, c0, fp64):
             ang = fn(lambda v: float(np.arccos(np.float32(v))), ca)
             t = 2.0 * r * fn(lambda v: float(np.cos(np.float32(v) * np.float32(1.0 / 3.0))), ang)
     xi = t - c2 * (1.0 / 3.0)
-    for _ in range(3):
+    for _ in range(3 if fp64 else 2):
         f = ((xi + c2) * xi + c1) * xi + c0
         df = (3.0 * xi + 2.0 * c2) * xi + c1
         if df != 0:
@@ -235,8 +235,9 @@ def _resolvent_from(c2, c1, c0, fp64):
 
 
 def resolvent_root(A, B, C):
-    """the largest root of the resolvent cubic: FP32 cube root / arccosine / cosine and three
-    FP64 Newton steps, or all in FP64 where that leaves a root that is not positive or a residual"""
+    """the largest root of the resolvent cubic: FP32 cube root / arccosine / cosine and two
+    FP64 Newton steps, or all in FP64 (three steps) where that leaves a root that is not
+    positive or a residual"""
     c2, c1, c0 = A, 0.25 * A * A - C, -0.125 * B * B
     xi = _resolvent_from(c2, c1, c0, False)
     f = ((xi + c2) * xi + c1) * xi + c0
