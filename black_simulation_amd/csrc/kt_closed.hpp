@@ -393,16 +393,27 @@ KTC_FN double resolvent_from(double c2, double c1, double c0, bool fp64) {
     const double hq = 0.5 * q, tp = p * (1.0 / 3.0);
     const double disc = hq * hq + tp * tp * tp;
     double t;
-    if (disc > 0.0) {
-        const double sd = qsqrt(disc);
-        const double w = hq < 0.0 ? -hq + sd : -hq - sd;
-        const double u = fp64 || !(fabs(w) < 1e30) ? cbrt(w) : (double)cbrtf((float)w);
-        t = u != 0.0 ? u - tp * qrcp(u) : 0.0;
-    } else {
-        const double r = qsqrt(-tp);
-        double ca = -hq * qrcp(r * r * r);
-        ca = fmin(1.0, fmax(-1.0, ca));
-        t = 2.0 * r * (fp64 ? cos(acos(ca) * (1.0 / 3.0)) : (double)cosf(acosf((float)ca) * (1.0f / 3.0f)));
+    if (fp64 || !(fabs(disc) < 1e30 && fabs(tp) < 1e12)) {
+        if (disc > 0.0) {
+            const double sd = qsqrt(disc);
+            const double w = hq < 0.0 ? -hq + sd : -hq - sd;
+            const double u = cbrt(w);
+            t = u != 0.0 ? u - tp * qrcp(u) : 0.0;
+        } else {
+            const double r = qsqrt(-tp);
+            const double ca = fmin(1.0, fmax(-1.0, -hq * qrcp(r * r * r)));
+            t = 2.0 * r * cos(acos(ca) * (1.0 / 3.0));
+        }
+    } else {  // the estimate in FP32 once the branch is chosen (w is formed without cancellation)
+        if (disc > 0.0) {
+            const double sd = (double)qsqrt((float)disc);
+            const float u = cbrtf((float)(hq < 0.0 ? -hq + sd : -hq - sd));
+            t = u != 0.0f ? (double)(u - (float)tp * qrcp(u)) : 0.0;
+        } else {
+            const float r = qsqrt((float)-tp);
+            const float ca = fminf(1.0f, fmaxf(-1.0f, (float)-hq * qrcp(r * r * r)));
+            t = (double)(2.0f * r * cosf(acosf(ca) * (1.0f / 3.0f)));
+        }
     }
     double xi = t - c2 * (1.0 / 3.0);
     // The step's correction needs only a few digits: an FP32 reciprocal of the derivative
@@ -584,8 +595,9 @@ KTC_FN Outcome solve_t(const Ray& ry) {
     if (!(B > 1e-12) || !(ry.E > 0.0) || !(ry.r_out < ry.r_esc)) return out;
     const double xi = resolvent_root(A, B, C);
     if (!(xi > 0.0)) return out;
-    const double z = qsqrt(0.5 * xi);
-    const double Bz = B * qrcp(4.0 * z);
+    const double h = 0.5 * xi, rs = qrsqrt(h);  // z = sqrt(xi / 2) and 1 / (4 z) from one estimate
+    const double z = h * rs;
+    const double Bz = 0.25 * B * rs;
     const double D12 = -0.5 * A - z * z + Bz;
     const double D34 = -0.5 * A - z * z - Bz;
     const bool cross = eta > 0.0 && ry.disk;  // eta <= 0: the ray never reaches the equator
