@@ -48,7 +48,7 @@ OPS_PER_PARTICLE_ATTEMPT = 656
 # (the factors a 1 GiB device copy reads) from rocprofv3 --pmc runs of the same command;
 # the summaries (commands included) are the named files under profiles/
 PROFILED_TRAFFIC = {
-    ("orbit/closed", 1080, "k_closed_form"): (0.402e6 + 76.73e6, "profiles/k_closed_form_orbit_1080p.md"),
+    ("orbit/closed", 1080, "k_closed_form"): (0.377e6 + 76.73e6, "profiles/k_closed_form_orbit_1080p.md"),
     ("orbit/surrogate", 1080, "k_mlp_fast"): (1.220e6 + 75.30e6, "profiles/k_mlp_fast_orbit_1080p.md"),
     ("orbit/surrogate", 1080, "k_trace_oneshot"): (1.203e6 + 3.885e6,
                                                    "profiles/k_trace_oneshot_fallback_orbit_1080p.md"),
