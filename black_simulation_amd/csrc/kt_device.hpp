@@ -423,10 +423,15 @@ struct Launch {
 };
 
 // phi_KS - phi_BL = psi(r) = integral from infinity to r of a / Delta, so that the two
-// agree at infinity: (a / (r+ - r-)) log((r - r+) / (r - r-)) outside the horizon, and the
-// limit -a / (r - 1) at |a| = 1.
+// agree at infinity: (a / (r+ - r-)) log((r - r+) / (r - r-)) outside the horizon, the
+// limit -a / (r - 1) at |a| = 1, and past it (no horizon, Delta = (r - 1)^2 + b^2,
+// b^2 = a^2 - 1) (a / b)(atan((r - 1) / b) - pi / 2).
 __device__ __forceinline__ double ks_shift(double a, double r) {
     if (a == 0.0) return 0.0;
+    if (fabs(a) > 1.0) {
+        const double b = sqrt(a * a - 1.0);
+        return a / b * (atan((r - 1.0) / b) - 0.5 * kPi);
+    }
     const double d = 2.0 * sqrt(fmax(0.0, 1.0 - a * a));
     const double rm = 1.0 - 0.5 * d;
     if (d == 0.0) return -a / (r - 1.0);

@@ -34,9 +34,16 @@ __attribute__((noinline)) double hsin(double x) { return std::sin(x); }
 __attribute__((noinline)) double hcos(double x) { return std::cos(x); }
 
 // phi_in - phi_BL = psi(r) = integral from infinity to r of a / Delta (zero at infinity);
-// phi_out - phi_BL = -psi(r).  The same operations as the device's ks_shift.
+// phi_out - phi_BL = -psi(r).  The same operations as the device's ks_shift.  Past |a| = 1
+// Delta = (r - 1)^2 + b^2 (b^2 = a^2 - 1) has no zero and psi = (a / b)(atan((r - 1) / b) -
+// pi / 2), finite for every r (src/kerr.cpp:284-291 writes the same function without the
+// constant, which only moves the Kerr-Schild phi's origin).
 double ks_shift_host(double a, double r) {
     if (a == 0.0) return 0.0;
+    if (std::fabs(a) > 1.0) {
+        const double b = std::sqrt(a * a - 1.0);
+        return a / b * (std::atan((r - 1.0) / b) - 0.5 * kPi);
+    }
     const double d = 2.0 * std::sqrt(std::fmax(0.0, 1.0 - a * a));
     const double rm = 1.0 - 0.5 * d;
     if (d == 0.0) return -a / (r - 1.0);
@@ -50,7 +57,46 @@ double horizon(double a) {
     return 1.0 + std::sqrt(std::fmax(0.0, 1.0 - a * a));
 }
 
+// ---- naked singularities, 1 < a <= 1.5 (src/kerr.cpp:117-153, 157-206): no horizon ----
+// The ISCO is the root of Bardeen, Press and Teukolsky's marginal-stability condition
+// r^2 - 6 r + 8 s a sqrt(r) - 3 a^2 = 0 (dE/dr = 0 of the circular orbits; s = 1 prograde),
+// which holds past a = 1 too: in x = sqrt(r), f(x) = x^4 - 6 x^2 + 8 s a x - 3 a^2.  Prograde,
+// f increases from f(0) < 0 to f(1) > 0 (a < 5/3): the root lies inside r = 1.  Retrograde,
+// f falls from f(sqrt 3) < 0 to its least value and then rises through zero once before
+// x = 6.  The reference finds the same radius as the least circular-orbit energy by a scan
+// and a golden-section search, good to ~1e-8; this bisection ends at adjacent doubles.
+// Circular photon orbits: r (r - 3)^2 = 4 a^2, only the retrograde one past a = 1 (the
+// prograde radius is reported as 0, as the reference does).
+constexpr double kMaxNakedSpin = 1.5;
+
+bool naked(double a) { return std::fabs(a) > 1.0; }
+
+template <class F>
+double bisect_root(F f, double lo, double hi) {  // f(lo) < 0 <= f(hi), one sign change
+    for (int k = 0; k < 200; ++k) {
+        const double mid = 0.5 * (lo + hi);
+        if (!(mid > lo && mid < hi)) break;
+        if (f(mid) < 0.0) lo = mid;
+        else hi = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+double isco_naked(double a, int prograde) {
+    const double sa = prograde ? a : -a, a2 = a * a;
+    const auto f = [=](double x) { return ((x * x - 6.0) * x + 8.0 * sa) * x - 3.0 * a2; };
+    const double x = prograde ? bisect_root(f, 0.0, 1.0) : bisect_root(f, std::sqrt(3.0), 6.0);
+    return x * x;
+}
+
+double photon_orbit_naked(double a, int prograde) {
+    if (prograde) return 0.0;
+    const double a2 = a * a;
+    return bisect_root([=](double r) { return r * (r - 3.0) * (r - 3.0) - 4.0 * a2; }, 3.0, 3.0 + 2.0 * std::fabs(a));
+}
+
 double isco(double a, int prograde) {
+    if (naked(a)) return isco_naked(a, prograde);
     double z1 = 1.0 + std::cbrt(1.0 - a * a) * (std::cbrt(1.0 + a) + std::cbrt(1.0 - a));
     double z2 = std::sqrt(3.0 * a * a + z1 * z1);
     double root = std::sqrt(std::fmax(0.0, (3.0 - z1) * (3.0 + z1 + 2.0 * z2)));
@@ -58,6 +104,7 @@ double isco(double a, int prograde) {
 }
 
 double photon_orbit(double a, int prograde) {
+    if (naked(a)) return photon_orbit_naked(a, prograde);
     double s = prograde ? -1.0 : 1.0;
     return 2.0 * (1.0 + hcos(2.0 / 3.0 * std::acos(s * a)));
 }
@@ -221,8 +268,10 @@ const char* check_view(const kt_view& v) {
         if (!std::isfinite(x)) return "frame must be finite";
     if (v.width <= 0 || v.height <= 0) return "image size must be positive";
     if ((long long)v.width * v.height > (1ll << 31)) return "view larger than 2^31 pixels";
-    if (!(std::fabs(v.spin) <= 1.0)) return "spin must lie in [-1, 1] (naked singularities are not traced)";
+    if (!(v.spin >= -1.0 && v.spin <= kMaxNakedSpin))
+        return "spin must lie in [-1, 1.5] (past 1: a naked singularity)";
     if (v.camera_kind == KT_CAMERA_OBSERVER) {
+        if (naked(v.spin)) return "observer cameras need a horizon: naked singularities take Bardeen and pinhole views";
         if (!(v.fov_deg > 0.0 && v.fov_deg < 180.0)) return "fov out of range";
         if (!(v.theta_cam > 0.0 && v.theta_cam < kPi)) return "an observer camera must be off the axis";
         if (has_frame(v)) {
@@ -1887,7 +1936,7 @@ double kt_circular_omega(double a, double r, int prograde) {
 }
 
 int kt_critical_curve(double a, double inclination, int samples, double* alpha, double* beta, int cap) {
-    if (!alpha || !beta || cap <= 0) return 0;
+    if (!alpha || !beta || cap <= 0 || naked(a)) return 0;  // a naked singularity casts no such shadow edge
     const double sinI = hsin(inclination), cosI = hcos(inclination);
     if (std::fabs(a) < 1e-6) {
         int n = 0;

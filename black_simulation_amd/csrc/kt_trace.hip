@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void k_trace_report(kti::TraceLaunch L) {
 // the closed form's inputs for launched ray `la` of view v: lambda, eta and the signs of the
 // photon's motion at the camera; false for what it does not take (observer cameras)
 __device__ __forceinline__ bool closed_ray(const ViewDev& v, const Launch& la, ktc::Ray& ry) {
-    if (v.kind == 3 || la.dark) return false;
+    if (v.kind == 3 || la.dark || !(fabs(la.p.a) <= 1.0)) return false;  // past a = 1: the integrator
     const Photon<double>& p = la.p;
     if (!(p.E > 0.0) || la.y0[3] == 0.0 || la.y0[4] == 0.0) return false;
     const double invE = 1.0 / p.E;
@@ -310,7 +310,7 @@ __device__ __forceinline__ bool closed_ray(const ViewDev& v, const Launch& la, k
 // inward, toward smaller theta when beta > 0; declined where R or M vanishes at the camera,
 // as a launch with w_r or w_mu zero), other kinds through their launch
 __device__ __forceinline__ bool closed_ray_of(const ViewDev& v, long long local, ktc::Ray& ry) {
-    if (v.kind == 0 && !v.params) {
+    if (v.kind == 0 && !v.params && fabs(v.a) <= 1.0) {
         int row, col;
         pixel_rc(local, v.width, row, col);
         const double alpha = -v.half_width + ((double)col + 0.5) * v.px;

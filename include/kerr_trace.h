@@ -90,7 +90,10 @@ enum { KT_MODE_PERSISTENT = 0, KT_MODE_ONESHOT = 1, KT_MODE_AUTO = 2 /* the tune
  * the others are followed out through it.  For observer views phi_cam is the Kerr-Schild
  * phi, as in the simulator; they trace in FP64 only and cannot share a call with other
  * kinds.  Disk fields follow trajectory::RayScene (src/trajectory.h:74-83).  Every real
- * field must be finite: a NaN or infinity anywhere is KT_EINVAL. */
+ * field must be finite: a NaN or infinity anywhere is KT_EINVAL.  spin lies in [-1, 1.5]:
+ * past 1 a naked singularity (src/kerr.cpp:117), traced for Bardeen and pinhole views by the
+ * integrator; it has no horizon, so capture_factor x r+ = 0 and a ray is captured where it
+ * passes through the ring's disk to r < 0 (as eht::traceRay does, src/eht.cpp:285). */
 typedef struct kt_view {
     int camera_kind;
     int width, height;
@@ -467,14 +470,17 @@ int kt_observer_frame(double a, double r, double theta, const double u[4], doubl
 int kt_geodesic_four_velocity(double a, double E, double L, double Q, double r, double theta, int radial_sign,
                               int polar_sign, double u[4]);
 
-/* Host-side closed forms (src/kerr.cpp:157-206, 296-299). */
+/* Host-side closed forms (src/kerr.cpp:157-206, 296-299).  Past a = 1 (a naked
+ * singularity, up to a = 1.5) the horizon radius is 0, the ISCO is the root of the
+ * marginal-stability condition (inside r = 1 prograde), and the prograde photon orbit is 0. */
 double kt_horizon_radius(double a);
 double kt_isco_radius(double a, int prograde);
 double kt_photon_orbit_radius(double a, int prograde);
 double kt_circular_omega(double a, double r, int prograde);
 
 /* Critical curve of spin a at inclination i: writes up to cap points, returns the
- * count (src/eht.cpp:52-80); radius of a closed curve along direction psi (:82-97). */
+ * count (src/eht.cpp:52-80; none past a = 1); radius of a closed curve along direction psi
+ * (:82-97). */
 int kt_critical_curve(double a, double inclination, int samples, double* alpha, double* beta, int cap);
 double kt_critical_radius(const double* alpha, const double* beta, int n, double psi);
 

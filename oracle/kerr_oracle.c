@@ -33,16 +33,41 @@ double ko_inner_horizon_radius(double a) {
     return 1.0 - sqrt(fmax(0.0, 1.0 - a * a));
 }
 
-/* Bardeen, Press & Teukolsky (1972) closed form, kerr.cpp:179-190 (|a| <= 1 only) */
+/* Past a = 1 (naked, kerr.cpp:179-206): roots by bisection to adjacent doubles.  The ISCO is
+   the root of the marginal-stability condition x^4 - 6 x^2 + 8 s a x - 3 a^2 = 0, x = sqrt(r)
+   (dE/dr = 0 of the circular orbits, which the reference finds by a golden-section search
+   on E): in (0, 1) prograde, where f rises through zero once, and in (sqrt 3, 6) retrograde,
+   where f falls to its least value and then rises through zero once.  The retrograde
+   photon orbit solves r (r - 3)^2 = 4 a^2 in (3, 3 + 2|a|); the prograde one is 0. */
+static double isco_f(double x, double sa, double a2) { return ((x * x - 6.0) * x + 8.0 * sa) * x - 3.0 * a2; }
+static double orbit_f(double r, double sa, double a2) { (void)sa; return r * (r - 3.0) * (r - 3.0) - 4.0 * a2; }
+
+static double bisect_root(double (*f)(double, double, double), double sa, double a2, double lo, double hi) {
+    for (int k = 0; k < 200; ++k) {
+        const double mid = 0.5 * (lo + hi);
+        if (!(mid > lo && mid < hi)) break;
+        if (f(mid, sa, a2) < 0.0) lo = mid;
+        else hi = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+/* Bardeen, Press & Teukolsky (1972) closed form, kerr.cpp:179-190 */
 double ko_isco_radius(double a, int prograde) {
+    if (naked(a)) {
+        const double x = prograde ? bisect_root(isco_f, a, a * a, 0.0, 1.0)
+                                  : bisect_root(isco_f, -a, a * a, sqrt(3.0), 6.0);
+        return x * x;
+    }
     double z1 = 1.0 + cbrt(1.0 - a * a) * (cbrt(1.0 + a) + cbrt(1.0 - a));
     double z2 = sqrt(3.0 * a * a + z1 * z1);
     double root = sqrt(fmax(0.0, (3.0 - z1) * (3.0 + z1 + 2.0 * z2)));
     return prograde ? 3.0 + z2 - root : 3.0 + z2 + root;
 }
 
-/* kerr.cpp:192-206 (|a| <= 1) */
+/* kerr.cpp:192-206 */
 double ko_photon_orbit_radius(double a, int prograde) {
+    if (naked(a)) return prograde ? 0.0 : bisect_root(orbit_f, 0.0, a * a, 3.0, 3.0 + 2.0 * fabs(a));
     double s = prograde ? -1.0 : 1.0;
     return 2.0 * (1.0 + ko_cos(2.0 / 3.0 * acos(s * a)));
 }
@@ -842,9 +867,15 @@ void ko_launch_pixel(const ko_view* v, int col, int row, ko_photon* p, double y0
 /* the tracer                                                                 */
 /* ------------------------------------------------------------------------- */
 
-/* phi_KS - phi_BL = psi(r), the integral of a / Delta from infinity: zero at infinity */
+/* phi_KS - phi_BL = psi(r), the integral of a / Delta from infinity: zero at infinity.  Past
+   a = 1, Delta = (r - 1)^2 + b^2 and psi = (a / b)(atan((r - 1) / b) - pi / 2); kerr.cpp:284-291
+   (azimuthShift) is the same function without the constant. */
 double ko_ks_shift(double a, double r) {
     if (a == 0.0) return 0.0;
+    if (naked(a)) {
+        const double b = sqrt(a * a - 1.0);
+        return a / b * (atan((r - 1.0) / b) - 0.5 * 3.14159265358979323846);
+    }
     const double d = 2.0 * sqrt(fmax(0.0, 1.0 - a * a));
     const double rm = 1.0 - 0.5 * d;
     if (d == 0.0) return -a / (r - 1.0);

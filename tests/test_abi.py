@@ -39,13 +39,17 @@ def test_library_exports_every_declared_symbol():
 
 def test_host_closed_forms_equal_oracle():
     lib, ol = K.lib(), O.lib()
-    for a in (0.0, 0.3, 0.9, 0.998, 1.0):
+    for a in (0.0, 0.3, 0.9, 0.998, 1.0, 1.0001, 1.2, 1.5):  # past 1: naked singularities
         assert lib.kt_horizon_radius(a) == ol.ko_horizon_radius(a)
         for pro in (0, 1):
             assert lib.kt_isco_radius(a, pro) == ol.ko_isco_radius(a, pro)
             assert lib.kt_photon_orbit_radius(a, pro) == ol.ko_photon_orbit_radius(a, pro)
             for r in (3.0, 7.5, 40.0):
                 assert lib.kt_circular_omega(a, r, pro) == ol.ko_circular_omega(a, r, pro)
+
+
+def test_no_critical_curve_past_a_equals_1():
+    assert api.critical_curve(1.2, 1.0, 64)[0].size == 0
 
 
 def test_critical_curve_equals_oracle():
