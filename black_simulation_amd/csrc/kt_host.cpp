@@ -539,7 +539,6 @@ int observer_batch(kt_ctx* ctx, const std::vector<ViewDev>& vd, const kt_trace_o
     if (n != (int)vd.size()) return fail(ctx, KT_EINVAL, "observer cameras cannot share a call with other kinds");
     if (o.precision != KT_PREC_FP64) return fail(ctx, KT_EINVAL, "observer cameras trace in FP64 (KT_PREC_FP64)");
     if (o.use_surrogate) return fail(ctx, KT_EINVAL, "the surrogate does not cover observer cameras");
-    if (o.closed_form) return fail(ctx, KT_EINVAL, "the closed form does not cover observer cameras");
     return KT_OK;
 }
 

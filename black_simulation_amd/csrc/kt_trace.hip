@@ -281,9 +281,10 @@ __global__ __launch_bounds__(256) void k_trace_report(kti::TraceLaunch L) {
 // ---- the closed-form solution ---------------------------------------------------------------
 
 // the closed form's inputs for launched ray `la` of view v: lambda, eta and the signs of the
-// photon's motion at the camera; false for what it does not take (observer cameras)
+// photon's motion at the camera; false for what it does not take: observer cameras inside
+// the horizon (their rays start on the far side of r+), spins past 1
 __device__ __forceinline__ bool closed_ray(const ViewDev& v, const Launch& la, ktc::Ray& ry) {
-    if (v.kind == 3 || la.dark || !(fabs(la.p.a) <= 1.0)) return false;  // past a = 1: the integrator
+    if (la.dark || !(fabs(la.p.a) <= 1.0) || (v.kind == 3 && !(v.r_cam > v.r_plus))) return false;
     const Photon<double>& p = la.p;
     if (!(p.E > 0.0) || la.y0[3] == 0.0 || la.y0[4] == 0.0) return false;
     const double invE = 1.0 / p.E;
@@ -293,7 +294,9 @@ __device__ __forceinline__ bool closed_ray(const ViewDev& v, const Launch& la, k
     ry.eta = p.Q * invE * invE;
     ry.r_o = la.y0[0];
     ry.mu_o = la.y0[1];
-    ry.phi_o = v.kind == 2 ? 0.0 : v.phi_cam;  // Boyer-Lindquist phi at the camera
+    // Boyer-Lindquist phi at the camera: an observer view's phi_cam is the ingoing Kerr-Schild
+    // one, phi_BL + psi(r)
+    ry.phi_o = v.kind == 2 ? 0.0 : v.kind == 3 ? v.phi_cam - ks_shift(p.a, la.y0[0]) : v.phi_cam;
     ry.s_r = la.y0[3] < 0.0 ? -1 : 1;
     ry.s_mu = la.y0[4] < 0.0 ? -1 : 1;
     ry.r_in = la.r_in;

@@ -138,19 +138,20 @@ typedef struct kt_trace_opts {
      * cancellable.  The other calls ignore it. */
     const volatile int* cancel;
     /* 1: the closed-form solution of Carter's equations (Legendre elliptic integrals of the
-     * Mino-time quadratures, FP64) answers the rays of Bardeen planes, pinhole (static ZAMO)
-     * cameras and explicit rays that it covers -- class, disk radius and g, escaping rays'
-     * sky angles -- with 0 steps and KT_FLAG_CLOSED_FORM; the rays it declines (an escape
-     * through a polar vortical orbit or through the pole, a turning point at the capture
-     * radius, a crossing within 1e-9 of a disk edge, outgoing rays with no turning point) go
-     * to the integrator at `precision`.  With use_surrogate, the network's gate runs first
-     * and the closed form replaces the values of the rays it answered as escaped or disk
-     * hits (re-queueing those it declines or classes otherwise).  Not for observer cameras
-     * or the ray report.  0: off (the default).
+     * Mino-time quadratures, FP64) answers the rays of Bardeen planes, pinhole cameras (the
+     * static ZAMO's, or any observer's outside the horizon) and explicit rays that it covers
+     * -- class, disk radius and g, escaping rays' sky angles -- with 0 steps and
+     * KT_FLAG_CLOSED_FORM; the rays it declines (an escape through a polar vortical orbit or
+     * through the pole, a turning point at the capture radius, a crossing within 1e-9 of a
+     * disk edge, outgoing rays with no turning point, observer cameras inside the horizon,
+     * spins past 1) go to the integrator at `precision`.  With use_surrogate, the network's
+     * gate runs first and the closed form replaces the values of the rays it answered as
+     * escaped or disk hits (re-queueing those it declines or classes otherwise).  Not with
+     * the ray report.  0: off (the default).
      * 2: the same with the elliptic integrals and Jacobi functions in FP32 (the roots of the
      * radial potential and the class decisions they make stay FP64): disk radius and g to
      * ~1e-6; a ray whose crossing lies within 1e-5 of a disk edge, or that escapes, is
-     * solved again in FP64 (with use_surrogate, 2 acts as 1). */
+     * solved again in FP64. */
     int closed_form;
 } kt_trace_opts;
 
