@@ -131,9 +131,9 @@ def cpu_baseline(view, seconds_target: float = 15.0, more=()) -> dict:
 
 def c1_timing(tr, torch) -> dict:
     """The CPU-runnable configuration in full (a = 0.9, r = 30 M, i = 80 deg, 256^2 Bardeen
-    pixels, FP64 DP5 rtol 1e-10): the GPU's FP64 path (HIP events, best of 5) and the oracle
-    on every pixel over the host's cores; the classes agree bit for bit (the parity tests
-    check every output)."""
+    pixels, FP64 DP5 rtol 1e-10): the GPU's FP64 path and its closed form (HIP events around
+    the whole call, best of 5) and the oracle on every pixel over the host's cores; the
+    classes agree (the parity tests check every output)."""
     import numpy as np
 
     import black_simulation_amd as bsa
@@ -141,18 +141,22 @@ def c1_timing(tr, torch) -> dict:
     from oracle import oracle as O
 
     v = c1_view()
-    opts = bsa.TraceOpts(precision=bsa.PREC_FP64)
-    out = tr.trace_views([v], opts)
     stream = torch.cuda.current_stream()
-    best = 1e30
-    for _ in range(5):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        tr.trace_views([v], opts, out=out)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        best = min(best, e0.elapsed_time(e1))
-    got = out["cls"].cpu().numpy()
+
+    def best_of_5(opts):
+        out = tr.trace_views([v], opts)
+        best = 1e30
+        for _ in range(5):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            tr.trace_views([v], opts, out=out)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        return best, out["cls"].cpu().numpy()
+
+    best, got = best_of_5(bsa.TraceOpts(precision=bsa.PREC_FP64))
+    best_cf, got_cf = best_of_5(bsa.TraceOpts(closed_form=2))
     cores = os.cpu_count() or 1
     ov = O.make_view(**{k: getattr(v, k) for k in (
         "camera_kind", "width", "height", "spin", "r_cam", "theta_cam", "phi_cam", "half_width", "half_height",
@@ -161,8 +165,10 @@ def c1_timing(tr, torch) -> dict:
     want = O.render(ov, O.make_opts(), threads=cores)
     cpu_s = time.perf_counter() - t0
     return {"rays": v.rays, "gpu_fp64_ms": round(best, 3), "gpu_fp64_mrays_s": round(v.rays / best / 1e3, 2),
+            "gpu_closed_form_ms": round(best_cf, 4), "gpu_closed_form_mrays_s": round(v.rays / best_cf / 1e3, 1),
             "cpu_port_s": round(cpu_s, 3), "cpu_port_mrays_s": round(v.rays / cpu_s / 1e6, 4), "cpu_threads": cores,
-            "classes_equal": bool(np.array_equal(got & 7, want["cls"]))}
+            "classes_equal": bool(np.array_equal(got & 7, want["cls"])),
+            "closed_form_classes_equal": bool(np.array_equal(got_cf & 7, want["cls"]))}
 
 
 def _ref_lib():
