@@ -25,11 +25,14 @@
 // Special functions: Carlson's symmetric integrals R_F, R_J, R_C by duplication (B. C.
 // Carlson, Numer. Algorithms 10, 13 (1995)), K(m) by the arithmetic-geometric mean, and
 // Jacobi's sn, cn, dn by descending Landen transformations (Abramowitz & Stegun 16.12).
-// All in FP64; answers agree with the FP64 DP5 integrator at rtol 1e-10 to ~1e-9 (its own
-// error).  Rays the forms here do not cover (polar vortical orbits that escape, L = 0
-// escapes whose phi jumps at the pole, a turning point at the capture radius, a crossing
-// within 1e-9 of the disk's edges, outgoing rays with no turning point) are declined and
-// left to the integrator.
+// In FP64 (solve) the answers agree with the FP64 DP5 integrator at rtol 1e-10 to ~1e-9
+// (its own error).  solve_t<float> runs the integrals and Jacobi's functions in FP32 and
+// keeps the roots and every class decision in FP64: disk radius and g to ~1e-6, and the
+// rays whose answer FP32 cannot settle (an escape's sky, a crossing near a disk edge) are
+// marked for the FP64 form.  Rays the forms here do not cover (polar vortical orbits that
+// escape, L = 0 escapes whose phi jumps at the pole, a turning point at the capture
+// radius, a crossing within 1e-9 of the disk's edges, outgoing rays with no turning point)
+// are declined and left to the integrator.
 #pragma once
 
 #include <math.h>
