@@ -466,20 +466,45 @@ struct Outcome {
     double theta, phi, r_hit, g;
 };
 
-// the Keplerian emitter's redshift at r (src/trajectory.cpp:311-321), in precision R; false:
-// no timelike orbit
+// the Keplerian emitter's redshift at r (src/trajectory.cpp:311-321): 1 with g, 0 where no
+// timelike circular orbit exists, -1 (FP32 only) where the FP32 pass cannot settle g (the ray
+// is then solved again in FP64).  FP64 evaluates the metric form as the integrator does.  Its
+// sum cancels near the ISCO of a fast spin (a hundredfold at a = 0.998) and at the photon
+// orbit, so FP32 uses the same quantity in Bardeen, Press and Teukolsky's form,
+// -g(u/u^t, u/u^t) = x^3 d / (x^3 + s a)^2, x = sqrt(r), with its one cancelling factor
+// d = x^3 - 3 x + 2 s a (zero at the photon orbit) formed in FP64.  Near that orbit g
+// varies steeply with r: a relative error e in the crossing radius becomes kappa e in g,
+// kappa = r |d'(r)| / (2 d).  The FP32 radius is good to ~1e-5, so where kappa > 10 -- or
+// where the sign of d, whether the orbit exists at all, could flip -- the ray goes to FP64.
+// (kappa stays below 6 down to the ISCO of a = 0.998; it grows without bound where an inner
+// edge set below the ISCO brings the disk to the photon orbit.)
 template <class R>
-KTC_FN bool disk_g(double a_, double r_, int prograde, double E, double L, double& g) {
-    const R a = R(a_), r = R(r_);
-    const R sg = prograde ? R(1) : R(-1);
-    const R om = sg * qrcp(r * qsqrt(r) + sg * a);
-    const R ir = qrcp(r);
-    const R gtt = -(R(1) - R(2) * ir), gtp = R(-2) * a * ir;
-    const R gpp = r * r + a * a + R(2) * a * a * ir;
-    const R norm = -(gtt + R(2) * gtp * om + gpp * om * om);
-    if (!(norm > R(0))) return false;
-    g = (double)(qsqrt(norm) * qrcp(R(E) - R(L) * om));
-    return true;
+KTC_FN int disk_g(double a_, double r_, int prograde, double E, double L, double& g) {
+    if constexpr (sizeof(R) == 4) {
+        const double sa = prograde ? a_ : -a_, x64 = qsqrt(r_);
+        const double d = x64 * (r_ - 3.0) + 2.0 * sa;
+        if (!(fabs(d) > 0.075 * x64 * fabs(r_ - 1.0))) return -1;  // kappa > 10: r d'(r) = 1.5 x (r - 1)
+        if (!(d > 0.0)) return 0;
+        const float x = (float)x64, x3 = x * x * x, q = x3 + (float)sa;
+        const float iq = qrcp(q);
+        const float norm = (float)d * x3 * iq * iq;
+        const float om = (prograde ? 1.0f : -1.0f) * iq;
+        const float Lom = (float)L * om, den = (float)E - Lom;
+        if (!(fabsf(den) > 1e-2f * (fabsf((float)E) + fabsf(Lom)))) return -1;
+        g = (double)(qsqrt(norm) * qrcp(den));
+        return 1;
+    } else {
+        const R a = R(a_), r = R(r_);
+        const R sg = prograde ? R(1) : R(-1);
+        const R om = sg * qrcp(r * qsqrt(r) + sg * a);
+        const R ir = qrcp(r);
+        const R gtt = -(R(1) - R(2) * ir), gtp = R(-2) * a * ir;
+        const R gpp = r * r + a * a + R(2) * a * a * ir;
+        const R norm = -(gtt + R(2) * gtp * om + gpp * om * om);
+        if (!(norm > R(0))) return 0;
+        g = (double)(qsqrt(norm) * qrcp(R(E) - R(L) * om));
+        return 1;
+    }
 }
 
 KTC_FN double wrap2pi(double x) {
@@ -713,7 +738,12 @@ KTC_FN Outcome solve_t(const Ray& ry) {
                 return out;
             }
             double g;
-            if (rc >= ry.r_in && rc <= ry.r_out && disk_g<R>(a, rc, ry.prograde, ry.E, lam * ry.E, g)) {
+            const int dg = rc >= ry.r_in && rc <= ry.r_out ? disk_g<R>(a, rc, ry.prograde, ry.E, lam * ry.E, g) : 0;
+            if (dg < 0) {
+                out.cls = unsure;
+                return out;
+            }
+            if (dg > 0) {
                 out.cls = 2;
                 out.r_hit = rc;
                 out.g = g;

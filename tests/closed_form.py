@@ -247,6 +247,15 @@ def resolvent_root(A, B, C):
     return xi
 
 
+def g_settled(a, r, prograde):
+    """the FP32 pass's test (kt_closed.hpp disk_g<float>): g varies with r no more than ten
+    times as steeply as r (kappa = r |d'| / (2 d) <= 10, d = r^1.5 - 3 r^0.5 + 2 s a), and the
+    sign of d, whether a circular orbit exists, cannot flip"""
+    x = math.sqrt(float(r))
+    d = x * (float(r) - 3.0) + 2.0 * (a if prograde else -a)
+    return abs(d) > 0.075 * x * abs(float(r) - 1.0)  # r d'(r) = 1.5 x (r - 1)
+
+
 def disk_g(a, r, prograde, E, L):
     sg = 1.0 if prograde else -1.0
     om = sg / (r * sqrt(r) + sg * a)
@@ -377,6 +386,8 @@ def _solve(ray, fp32):
             if abs(rc_ - ray["r_in"]) < band * ray["r_in"] or abs(rc_ - ray["r_out"]) < band * ray["r_out"]:
                 return (UNCERTAIN, nan, nan, nan, nan) if fp32 else out
             if ray["r_in"] <= rc_ <= ray["r_out"]:
+                if fp32 and not g_settled(ray["a"], rc_, ray["prograde"]):
+                    return (UNCERTAIN, nan, nan, nan, nan)
                 g = disk_g(ray["a"], rc_, ray["prograde"], ray["E"], lam * ray["E"])
                 if g is not None:
                     return (DISK, nan, nan, rc_, g)

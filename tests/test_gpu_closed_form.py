@@ -234,6 +234,26 @@ def test_surrogate_with_closed_form_values(tracer, closed):
     assert polished.mean() > 0.5 and agree / n >= 0.999
 
 
+def test_disk_near_the_photon_orbit(tracer):
+    """a counter-rotating disk whose inner edge lies below its ISCO (a negative spin with the
+    prograde disk and an inner edge at 2.92 M, or the ISCO formula's prograde radius): its
+    emitters reach the photon orbit, where g falls to zero as steeply as sqrt(d), d =
+    r^1.5 - 3 r^0.5 + 2 s a, and whether a circular orbit exists flips with the last digits of
+    the crossing radius.  The FP32 pass hands those rays to FP64: every class equals the
+    oracle's, and g stays within the FP32 bar"""
+    views = [bsa.View(width=48, height=48, spin=-0.3224500706761697, r_cam=397.591112698606,
+                      theta_cam=1.1732759034435825, phi_cam=0.4978852283607192, half_width=8.349956684111284,
+                      half_height=8.349956684111284, disk_outer=31.167540346017052, disk_inner=2.9200540150299337),
+             bsa.View(width=48, height=48, spin=-0.7967212696420716, r_cam=667.9740974565108,
+                      theta_cam=0.2299825628108736, phi_cam=1.2866744128204795, half_width=12.009798480309898,
+                      half_height=12.009798480309898, disk_outer=15.840551924666542)]
+    for i, v in enumerate(views):
+        print(f"disk near the photon orbit {i}", flush=True)
+        want = O.render(oracle_view(v), O.make_opts())
+        check32(bsa.to_numpy(tracer.trace_views([v], bsa.TraceOpts(closed_form=2))), want, f"near photon orbit {i}")
+        check(bsa.to_numpy(tracer.trace_views([v], CLOSED)), want, f"near photon orbit {i} FP64", min_answered=0.9)
+
+
 @pytest.mark.parametrize("closed", [1, 2])
 def test_observer_cameras(tracer, closed):
     """the renderer's camera carried by any observer (KT_CAMERA_OBSERVER, phi_cam the ingoing
