@@ -636,7 +636,8 @@ KTC_FN Outcome solve_t(const Ray& ry) {
     // roots: sin^2 phi = r31 (r - r4) / (r41 (r - r3))) or w = cn(J / pref | k) (a complex
     // pair: cos phi = (Aq (r - r1) - Bq (r - r2)) / (Aq (r - r1) + Bq (r - r2)),
     // Aq = |r3 - r2|, Bq = |r3 - r1|).  S_t: J at the camera, signed so that the ray's own
-    // Mino time s reaches rt at s = S_t (S_t < 0: rt lies behind an outward start).
+    // Mino time s reaches rt at s = S_t (S_t < 0: rt lies behind an outward start).  A camera
+    // between r2 and r3 measures J from r2 instead (below).
     // The roots, the Moebius coefficients and every comparison with a radius are FP64; the
     // modulus k (k' = sqrt(1 - k)), the prefactor and the amplitudes with their cofactors
     // feed only precision-R integrals and are formed in R from FP64 root differences.
@@ -657,22 +658,48 @@ KTC_FN Outcome solve_t(const Ray& ry) {
         k = R(r32 * r41) * i3142;
         kp = qsqrt(R((r2 - r1) * r43) * i3142);  // sqrt(1 - k)
         pref = R(2) * qrsqrt(R(r31 * r42));
-        turning = r4 > ry.r_cap;
-        if (turning && (r4 < ry.r_cap * (1.0 + 1e-7) || !(r4 < ry.r_o))) return out;
-        if (r4 == ry.r_cap) return out;
-        if (ry.s_r > 0 && !turning) return out;  // outward from below a turning point: not covered
-        P0 = r31 * r4;
-        P1 = -r41 * r3;
-        Q0 = r31;
-        Q1 = -r41;
-        const R i41 = qrcp(R(r41)), io3 = qrcp(R(r41 * (ry.r_o - r3)));
-        x_o = qsqrt(R(r31 * (ry.r_o - r4)) * io3);
-        x_inf = qsqrt(R(r31) * i41);
-        c2_o = R(r43 * (ry.r_o - r1)) * io3;  // 1 - x_o^2
-        c2_inf = R(r43) * i41;                // 1 - x_inf^2
-        const R I_o = pref * ell_f2<R>(x_o, c2_o, kp * kp + k * c2_o);
-        S_t = ry.s_r < 0 ? I_o : -I_o;
-        if (cross) agm_init<R>(G, kp);
+        if (!(ry.r_o > r4)) {
+            // The camera between r2 and r3 (a close camera's wide rays; R < 0 between r3 and
+            // r4, where no ray starts): with the same k and prefactor, the Mino time from r2
+            // up to r is pref F(phi | k), sin^2 phi = r31 (r - r2) / (r32 (r - r1)), and r at
+            // it is (r2 r31 - r1 r32 w) / (r31 - r32 w), w = sn^2.  The ray falls to r2, below
+            // the capture radius (no photon is bound between r2 and r3), after turning at r3
+            // when it starts outward: it is captured unless a crossing lands on the disk.
+            const double r21 = r2 - r1;
+            if (!(ry.r_o < r3) || !(r2 < ry.r_cap)) return out;
+            if (!cross) {
+                out.cls = 1;
+                return out;
+            }
+            P0 = r2 * r31;
+            P1 = -r1 * r32;
+            Q0 = r31;
+            Q1 = -r32;
+            const R io = qrcp(R(r32 * (ry.r_o - r1)));
+            x_o = qsqrt(R(r31 * (ry.r_o - r2)) * io);
+            c2_o = R(r21 * (r3 - ry.r_o)) * io;  // 1 - x_o^2
+            const R I_o = pref * ell_f2<R>(x_o, c2_o, kp * kp + k * c2_o);
+            agm_init<R>(G, kp);
+            S_t = ry.s_r < 0 ? I_o : R(2) * pref * agm_k(G) - I_o;  // the Mino time at which r = r2
+        } else {
+            // beyond every root: inward, the ray turns at r4 and escapes if r4 lies above the
+            // capture radius and falls in otherwise; outward, it escapes
+            const bool outward = ry.s_r > 0;
+            turning = outward || r4 > ry.r_cap;
+            if (!outward && (r4 == ry.r_cap || (r4 > ry.r_cap && r4 < ry.r_cap * (1.0 + 1e-7)))) return out;
+            P0 = r31 * r4;
+            P1 = -r41 * r3;
+            Q0 = r31;
+            Q1 = -r41;
+            const R i41 = qrcp(R(r41)), io3 = qrcp(R(r41 * (ry.r_o - r3)));
+            x_o = qsqrt(R(r31 * (ry.r_o - r4)) * io3);
+            x_inf = qsqrt(R(r31) * i41);
+            c2_o = R(r43 * (ry.r_o - r1)) * io3;  // 1 - x_o^2
+            c2_inf = R(r43) * i41;                // 1 - x_inf^2
+            const R I_o = pref * ell_f2<R>(x_o, c2_o, kp * kp + k * c2_o);
+            S_t = outward ? -I_o : I_o;
+            if (cross) agm_init<R>(G, kp);
+        }
     } else {
         if (ry.s_r > 0) return out;  // no turning point: an outgoing ray escapes by forms not given here
         if (!(D12 >= 0.0)) {

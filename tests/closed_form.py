@@ -307,17 +307,31 @@ def _solve(ray, fp32):
         r31, r41, r42, r32 = r3 - r1, r4 - r1, r4 - r2, r3 - r2
         k = r32 * r41 / (r31 * r42)
         pref = 2.0 / sqrt(r31 * r42)
-        turning = r4 > ray["r_cap"]
-        if turning and (r4 < ray["r_cap"] * (1.0 + 1e-7) or not r4 < ray["r_o"]):
-            return out
-        if r4 == ray["r_cap"] or (ray["s_r"] > 0 and not turning):
-            return out
-        P0, P1, Q0, Q1 = r31 * r4, -r41 * r3, r31, -r41
-        x_o = sqrt(r31 * (ray["r_o"] - r4) / (r41 * (ray["r_o"] - r3)))
-        x_inf = sqrt(r31 / r41)
-        I_o = pref * ell_f(x_o, k)
-        S_t = I_o if ray["s_r"] < 0 else -I_o
-        G = agm(k) if cross else None
+        if not ray["r_o"] > r4:
+            # the camera between r2 and r3: J from r2, sin^2 phi = r31 (r - r2) / (r32 (r - r1)),
+            # the ray falls to r2 (below the capture radius), after turning at r3 if outward
+            if not (ray["r_o"] < r3 and r2 < ray["r_cap"]):
+                return out
+            if not cross:
+                return (CAPTURED, nan, nan, nan, nan)
+            P0, P1, Q0, Q1 = r2 * r31, -r1 * r32, r31, -r32
+            x_o = sqrt(r31 * (ray["r_o"] - r2) / (r32 * (ray["r_o"] - r1)))
+            I_o = pref * ell_f(x_o, k)
+            G = agm(k)
+            S_t = I_o if ray["s_r"] < 0 else 2.0 * pref * agm_k(G) - I_o
+        else:
+            # beyond every root: inward it turns at r4 (escaping if r4 is above the capture
+            # radius), outward it escapes
+            outward = ray["s_r"] > 0
+            turning = outward or r4 > ray["r_cap"]
+            if not outward and (r4 == ray["r_cap"] or ray["r_cap"] < r4 < ray["r_cap"] * (1.0 + 1e-7)):
+                return out
+            P0, P1, Q0, Q1 = r31 * r4, -r41 * r3, r31, -r41
+            x_o = sqrt(r31 * (ray["r_o"] - r4) / (r41 * (ray["r_o"] - r3)))
+            x_inf = sqrt(r31 / r41)
+            I_o = pref * ell_f(x_o, k)
+            S_t = -I_o if outward else I_o
+            G = agm(k) if cross else None
     else:
         if ray["s_r"] > 0:
             return out

@@ -113,7 +113,9 @@ def test_edge_views(tracer, case):
 
 def test_pinhole_cameras(tracer):
     """the simulator's camera (static ZAMO frame): looking at the hole from 32 M, and from 8 M
-    looking past it so that part of the field heads outward"""
+    looking past it so that part of the field heads outward (those rays escape without a
+    turning point; the few left to the integrator escape with R's second pair of roots
+    complex)"""
     views = [bsa.View(camera_kind=bsa.CAMERA_PINHOLE, width=48, height=32, spin=0.94, r_cam=32.0, theta_cam=1.48,
                       yaw=0.05, pitch=-0.02, roll=0.1, fov_deg=60.0, disk_outer=20.0),
              bsa.View(camera_kind=bsa.CAMERA_PINHOLE, width=48, height=32, spin=0.7, r_cam=8.0, theta_cam=1.2,
@@ -124,7 +126,7 @@ def test_pinhole_cameras(tracer):
     for i, v in enumerate(views):
         n = v.rays
         check({k: a[off:off + n] for k, a in got.items()}, O.render(oracle_view(v), O.make_opts()), f"pinhole {i}",
-              min_answered=0.5)
+              min_answered=0.99)
         off += n
 
 
@@ -232,6 +234,23 @@ def test_surrogate_with_closed_form_values(tracer, closed):
     print(f"surrogate + closed form: answered {answered.mean():.4f}, closed-form values on {polished.mean():.4f}, "
           f"class agreement {agree / n:.6f}", flush=True)
     assert polished.mean() > 0.5 and agree / n >= 0.999
+
+
+def test_close_cameras(tracer):
+    """Bardeen planes seen from inside the photon region (r = 1.8-3.2 M): some rays start
+    between the roots r2 and r3 of R (the ray falls to r2 below the capture radius, after
+    turning at r3 if it starts outward), others beyond r4; pixels whose launch R(r_cam) < 0
+    names no photon are left to the integrator"""
+    views = [bsa.View(width=40, height=40, spin=0.9, r_cam=2.6, theta_cam=1.2, half_width=7.0, half_height=7.0),
+             bsa.View(width=40, height=40, spin=0.5, r_cam=3.2, theta_cam=0.7, half_width=8.0, half_height=8.0,
+                      disk_inner=2.0),
+             bsa.View(width=40, height=40, spin=0.998, r_cam=1.8, theta_cam=1.4, half_width=6.0, half_height=6.0),
+             bsa.View(width=40, height=40, spin=0.9, r_cam=2.2, theta_cam=1.5, half_width=5.0, half_height=5.0)]
+    for i, v in enumerate(views):
+        print(f"close camera {i}", flush=True)
+        want = O.render(oracle_view(v), O.make_opts())
+        check(bsa.to_numpy(tracer.trace_views([v], CLOSED)), want, f"close camera {i}", min_answered=0.3)
+        check32(bsa.to_numpy(tracer.trace_views([v], bsa.TraceOpts(closed_form=2))), want, f"close camera {i}")
 
 
 def test_disk_near_the_photon_orbit(tracer):
