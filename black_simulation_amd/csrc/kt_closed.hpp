@@ -31,7 +31,8 @@
 // rays whose answer FP32 cannot settle (an escape's sky, a crossing near a disk edge) are
 // marked for the FP64 form.  Rays the forms here do not cover (polar vortical orbits that
 // escape, L = 0 escapes whose phi jumps at the pole, a turning point at the capture
-// radius, a crossing within 1e-9 of the disk's edges, outgoing rays with no turning point)
+// radius, a crossing within 1e-9 of the disk's edges, outgoing rays whose R has a complex
+// pair of roots)
 // are declined and left to the integrator.
 #pragma once
 
@@ -701,7 +702,7 @@ KTC_FN Outcome solve_t(const Ray& ry) {
             if (cross) agm_init<R>(G, kp);
         }
     } else {
-        if (ry.s_r > 0) return out;  // no turning point: an outgoing ray escapes by forms not given here
+        if (ry.s_r > 0) return out;  // outward past a complex pair: it escapes by forms not given here
         if (!(D12 >= 0.0)) {
             // all four roots complex: no turning point, and (eta < 0) no equatorial crossing
             if (cross) return out;
