@@ -32,8 +32,7 @@
 // marked for the FP64 form.  Rays the forms here do not cover (polar vortical orbits that
 // escape, L = 0 escapes whose phi jumps at the pole, a turning point at the capture
 // radius, a crossing within 1e-9 of the disk's edges, outgoing rays whose R has a complex
-// pair of roots)
-// are declined and left to the integrator.
+// pair of roots) are declined and left to the integrator.
 #pragma once
 
 #include <math.h>
