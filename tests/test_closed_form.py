@@ -120,6 +120,35 @@ def test_c3_views_match_the_oracle(j):
     assert r_err.max(initial=0) < 1e-8 and g_err.max(initial=0) < 1e-8 and defl.max(initial=0) < 1e-8
 
 
+def test_close_cameras_match_the_oracle():
+    """cameras inside the photon region (r = 1.8-3.2 M), where rays start between the roots r2
+    and r3 of R as well as beyond r4: every ray with a valid launch (R, M > 0 at the camera)
+    answered, classes equal, disk radius and g within 1e-8"""
+    n_region2 = 0
+    for a, r, inc, hw in ((0.9, 2.6, 1.2, 7.0), (0.998, 1.8, 1.4, 6.0), (0.9, 2.2, 1.5, 5.0)):
+        v = O.make_view(width=24, height=24, spin=a, r_cam=r, theta_cam=inc, half_width=hw, half_height=hw)
+        want = O.render(v, O.make_opts())
+        vd = _view_dict(v)
+        for q in range(24 * 24):
+            row, col = divmod(q, 24)
+            al, be = -hw + (col + 0.5) * 2 * hw / 24, hw - (row + 0.5) * 2 * hw / 24
+            ray = CF.bardeen_ray(vd, al, be)
+            lam, eta = ray["lam"], ray["eta"]
+            P = r * r + a * a - a * lam
+            if P * P - (r * r - 2 * r + a * a) * (eta + (lam - a) ** 2) <= 0:
+                continue  # no photon leaves the camera with this (alpha, beta)
+            A, B = a * a - eta - lam * lam, 2 * (eta + (lam - a) ** 2)
+            xi = CF.resolvent_root(A, B, -a * a * eta)
+            z = math.sqrt(xi / 2)
+            D34 = -0.5 * A - z * z - B / (4 * z)
+            n_region2 += D34 >= 0 and r < z + math.sqrt(D34)
+            cls, _, _, r_hit, g = CF.solve(ray)
+            assert cls == want["cls"][q], (a, r, q)
+            if cls == CF.DISK:
+                assert abs(r_hit / want["r_hit"][q] - 1) < 1e-8 and abs(g / want["g"][q] - 1) < 1e-8
+    assert n_region2 > 30
+
+
 def test_exact_quadratures():
     """answers fixed without either method: the equatorial sky angle of Kerr (a = 0.9, prograde
     and retrograde edge-on rays of a camera at r = 40 M) and a disk radius by equal Mino times"""
