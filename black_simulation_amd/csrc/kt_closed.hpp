@@ -658,7 +658,10 @@ KTC_FN Outcome solve_t(const Ray& ry) {
         k = R(r32 * r41) * i3142;
         kp = qsqrt(R((r2 - r1) * r43) * i3142);  // sqrt(1 - k)
         pref = R(2) * qrsqrt(R(r31 * r42));
-        if (!(ry.r_o > r4)) {
+        // a camera at the turning point r4 itself (a tangential ray, R(r_cam) ~ 0) may find r4
+        // a rounding above r_cam: it starts at r4
+        const double ro = ry.r_o < r4 && ry.r_o > r4 * (1.0 - 1e-12) ? r4 : ry.r_o;
+        if (!(ro >= r4)) {
             // The camera between r2 and r3 (a close camera's wide rays; R < 0 between r3 and
             // r4, where no ray starts): with the same k and prefactor, the Mino time from r2
             // up to r is pref F(phi | k), sin^2 phi = r31 (r - r2) / (r32 (r - r1)), and r at
@@ -691,10 +694,10 @@ KTC_FN Outcome solve_t(const Ray& ry) {
             P1 = -r41 * r3;
             Q0 = r31;
             Q1 = -r41;
-            const R i41 = qrcp(R(r41)), io3 = qrcp(R(r41 * (ry.r_o - r3)));
-            x_o = qsqrt(R(r31 * (ry.r_o - r4)) * io3);
+            const R i41 = qrcp(R(r41)), io3 = qrcp(R(r41 * (ro - r3)));
+            x_o = qsqrt(R(r31 * (ro - r4)) * io3);
             x_inf = qsqrt(R(r31) * i41);
-            c2_o = R(r43 * (ry.r_o - r1)) * io3;  // 1 - x_o^2
+            c2_o = R(r43 * (ro - r1)) * io3;  // 1 - x_o^2
             c2_inf = R(r43) * i41;                // 1 - x_inf^2
             const R I_o = pref * ell_f2<R>(x_o, c2_o, kp * kp + k * c2_o);
             S_t = outward ? -I_o : I_o;

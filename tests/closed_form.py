@@ -307,7 +307,9 @@ def _solve(ray, fp32):
         r31, r41, r42, r32 = r3 - r1, r4 - r1, r4 - r2, r3 - r2
         k = r32 * r41 / (r31 * r42)
         pref = 2.0 / sqrt(r31 * r42)
-        if not ray["r_o"] > r4:
+        # a camera at the turning point itself may find r4 a rounding above r_o: it starts at r4
+        ro = r4 if r4 * (1.0 - 1e-12) < ray["r_o"] < r4 else ray["r_o"]
+        if not ro >= r4:
             # the camera between r2 and r3: J from r2, sin^2 phi = r31 (r - r2) / (r32 (r - r1)),
             # the ray falls to r2 (below the capture radius), after turning at r3 if outward
             if not (ray["r_o"] < r3 and r2 < ray["r_cap"]):
@@ -327,7 +329,7 @@ def _solve(ray, fp32):
             if not outward and (r4 == ray["r_cap"] or ray["r_cap"] < r4 < ray["r_cap"] * (1.0 + 1e-7)):
                 return out
             P0, P1, Q0, Q1 = r31 * r4, -r41 * r3, r31, -r41
-            x_o = sqrt(r31 * (ray["r_o"] - r4) / (r41 * (ray["r_o"] - r3)))
+            x_o = sqrt(r31 * (ro - r4) / (r41 * (ro - r3)))
             x_inf = sqrt(r31 / r41)
             I_o = pref * ell_f(x_o, k)
             S_t = -I_o if outward else I_o
