@@ -31,8 +31,8 @@
 // rays whose answer FP32 cannot settle (an escape's sky, a crossing near a disk edge) are
 // marked for the FP64 form.  Rays the forms here do not cover (polar vortical orbits that
 // escape, L = 0 escapes whose phi jumps at the pole, a turning point at the capture
-// radius, a crossing within 1e-9 of the disk's edges, outgoing rays whose R has a complex
-// pair of roots) are declined and left to the integrator.
+// radius, a crossing within 1e-9 of the disk's edges, outgoing rays whose R has no real
+// root, or a complex pair that the escape's quadrature cannot resolve) are declined and left to the integrator.
 #pragma once
 
 #include <math.h>
@@ -557,11 +557,54 @@ KTC_FN Polar<R> polar_motion(const Ray& ry, double A) {
     return P;
 }
 
-// An escaping ray (four real roots, turning at r4) that the polar forms cover (eta > 0,
-// lambda != 0): the sky angles at infinity from the total Mino time S_tot.  x_o is x(r_o)
-// of the radial substitution, x_inf its limit; returns false if a value is not finite.
-KTC_CALL bool escape_sky(const Ray& ry, const Polar<double>& P, double S_tot, double r1, double r3, double r4,
-                         double k, double pref, double x_o, double x_inf, double& theta, double& phi) {
+// Bernstein parameter rho of the point t = re + i im for quadrature over t in [0, 1]: the
+// ellipse with foci 0 and 1 through t has semi-axis sum rho / 2 (after [0, 1] -> [-1, 1]).
+KTC_CALL double bernstein_rho(double re, double im) {
+    const double x = 2.0 * re - 1.0, y = 2.0 * im;
+    const double s = 0.5 * (sqrt((x - 1.0) * (x - 1.0) + y * y) + sqrt((x + 1.0) * (x + 1.0) + y * y));
+    return s + sqrt(fmax(0.0, s * s - 1.0));
+}
+
+// An outward start past a complex pair of roots (r1 < r2 below the camera, z +- i b): the
+// radial part of the escape's phi, the integral of a (2r - a lambda) / (Delta sqrt R) dr from
+// the camera to infinity, by 32-point Gauss-Legendre quadrature in t = r_o / r.  With u = 1 / r
+// the integrand is a u (2 - a lambda u) / (r_o D S), D = 1 - 2u + a^2 u^2 and
+// S = sqrt((1 - r1 u)(1 - r2 u)((1 - z u)^2 + b^2 u^2)): analytic on [0, 1] but for t = r_o / r
+// at r = r+, r-, r1, r2 and z +- i b.  Its error falls as rho^-64 with rho that of the nearest
+// of these; below rho = 1.6 it returns false and the ray goes to the integrator.
+KTC_CALL bool radial_phi_quad(const Ray& ry, double r1, double r2, double z, double b, double& Ir) {
+    constexpr double kX[16] = {0.04830766568773831, 0.1444719615827965, 0.23928736225213706, 0.33186860228212767, 0.42135127613063533, 0.5068999089322294, 0.5877157572407623, 0.6630442669302152, 0.7321821187402897, 0.7944837959679424, 0.84936761373257, 0.8963211557660522, 0.9349060759377397, 0.9647622555875064, 0.9856115115452684, 0.9972638618494816};
+    constexpr double kW[16] = {0.09654008851472781, 0.09563872007927483, 0.09384439908080457, 0.09117387869576386, 0.08765209300440391, 0.08331192422694685, 0.07819389578707031, 0.07234579410884845, 0.06582222277636175, 0.058684093478535704, 0.050998059262376244, 0.042835898022226426, 0.034273862913021626, 0.025392065309262427, 0.016274394730905965, 0.007018610009469298};
+    const double a = ry.a, lam = ry.lam, ro = ry.r_o;
+    const double sq1 = sqrt(fmax(0.0, 1.0 - a * a));
+    const double pts[4] = {1.0 + sq1, 1.0 - sq1, r1, r2};
+    double rho = bernstein_rho(ro * z / (z * z + b * b), ro * b / (z * z + b * b));
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (pts[i] != 0.0) rho = fmin(rho, bernstein_rho(ro / pts[i], 0.0));
+    if (!(rho >= 1.6)) return false;
+    const double iro = 1.0 / ro;
+    double sum = 0.0;
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+        const double t = 0.5 + 0.5 * (i < 16 ? kX[i] : -kX[i - 16]);
+        const double u = t * iro;
+        const double D = 1.0 - 2.0 * u + a * a * u * u;
+        const double zu = 1.0 - z * u;
+        const double S = sqrt((1.0 - r1 * u) * (1.0 - r2 * u) * (zu * zu + b * b * u * u));
+        sum += kW[i & 15] * a * u * (2.0 - a * lam * u) / (D * S);
+    }
+    Ir = 0.5 * iro * sum;
+    return Ir == Ir;
+}
+
+// An escaping ray that the polar forms cover (eta > 0, lambda != 0): the sky angles at
+// infinity from the total Mino time S_tot.  With four real roots (turning at r4, or outward
+// from beyond it) x_o is x(r_o) of the radial substitution and x_inf its limit; otherwise
+// (real4 false: outward past the complex pair zc +- i bc, r1 < r3 the real roots) the radial
+// part is the quadrature above.  Returns false if a value is not finite or not resolved.
+KTC_CALL bool escape_sky(const Ray& ry, const Polar<double>& P, double S_tot, bool real4, double r1, double r3,
+                         double r4, double zc, double bc, double k, double pref, double x_o, double x_inf, double& theta, double& phi) {
     const double a = ry.a, a2 = a * a, lam = ry.lam;
     const double r31 = r3 - r1, r41 = r4 - r1;
     // theta_inf from sn at the total Mino time (m <= 0:
@@ -585,7 +628,9 @@ KTC_CALL bool escape_sky(const Ray& ry, const Polar<double>& P, double S_tot, do
     //   pref [F / (r3 - p) + (1 / (r4 - p) - 1 / (r3 - p)) Pi(n_p; x | k)],
     //   n_p = (r3 - p) r41 / ((r4 - p) r31)
     double Ir = 0.0;
-    if (a != 0.0) {
+    if (a != 0.0 && !real4) {
+        if (!radial_phi_quad(ry, r1, r3, zc, bc, Ir)) return false;
+    } else if (a != 0.0) {
         const double sq1 = sqrt(fmax(0.0, 1.0 - a2));
         if (sq1 == 0.0) return false;  // |a| = 1: the partial fractions degenerate
         const double Fo = ell_f<double>(x_o, k), Fi = ell_f<double>(x_inf, k);
@@ -642,9 +687,10 @@ KTC_FN Outcome solve_t(const Ray& ry) {
     // modulus k (k' = sqrt(1 - k)), the prefactor and the amplitudes with their cofactors
     // feed only precision-R integrals and are formed in R from FP64 root differences.
     bool real4, turning = false;
-    double P0, P1, Q0, Q1, r1 = 0.0, r3 = 0.0, r4 = 0.0;
+    double P0, P1, Q0, Q1, r1 = 0.0, r3 = 0.0, r4 = 0.0, zc = 0.0, bc = 0.0;
     R pref, k, kp, x_o = R(0), x_inf = R(0), c2_o = R(1), c2_inf = R(1);
     R S_t;
+    R I_inf = R(-1);  // Mino time from the radial origin to infinity, once needed
     Agm<R> G;  // the radial modulus's mean (for the crossings)
     if (D34 >= 0.0) {
         real4 = true;
@@ -704,10 +750,10 @@ KTC_FN Outcome solve_t(const Ray& ry) {
             if (cross) agm_init<R>(G, kp);
         }
     } else {
-        if (ry.s_r > 0) return out;  // outward past a complex pair: it escapes by forms not given here
+        const bool outward = ry.s_r > 0;
         if (!(D12 >= 0.0)) {
             // all four roots complex: no turning point, and (eta < 0) no equatorial crossing
-            if (cross) return out;
+            if (cross || outward) return out;
             out.cls = 1;
             return out;
         }
@@ -715,11 +761,16 @@ KTC_FN Outcome solve_t(const Ray& ry) {
         const double sD12 = qsqrt(D12);
         r1 = -z - sD12;
         const double r2 = -z + sD12;
-        if (!(r2 < ry.r_cap)) return out;
-        if (!cross) {
+        // inward the ray falls to r2, which must lie below the capture radius; outward (the
+        // camera above r2, where R > 0) it runs out to infinity without turning
+        if (!outward && !(r2 < ry.r_cap)) return out;
+        if (!cross && !outward) {
             out.cls = 1;
             return out;
         }
+        r3 = r2;
+        zc = z;
+        bc = qsqrt(-D34);
         const double d2 = z - r2, d1 = z - r1;
         const double Aq = qsqrt(d2 * d2 - D34), Bq = qsqrt(d1 * d1 - D34);  // |r3 - r2|, |r3 - r1|
         const double r21 = r2 - r1;
@@ -737,17 +788,25 @@ KTC_FN Outcome solve_t(const Ray& ry) {
         const R f = ell_f2<R>(qsqrt(R(fmax(0.0, (1.0 - c) * (1.0 + c)))), cc, kp * kp + k * cc);
         agm_init<R>(G, kp);
         S_t = pref * (c >= 0.0 ? f : R(2) * agm_k(G) - f);  // F(acos c) past pi/2
+        if (outward) {
+            // J runs from I_o at the camera to I_inf, where c reaches (Aq - Bq) / (Aq + Bq)
+            const double ci = (Aq - Bq) * qrcp(Aq + Bq);
+            const R ci2 = R(ci * ci);
+            const R fi = ell_f2<R>(qsqrt(R(fmax(0.0, (1.0 - ci) * (1.0 + ci)))), ci2, kp * kp + k * ci2);
+            I_inf = pref * (ci >= 0.0 ? fi : R(2) * agm_k(G) - fi);
+            S_t = -S_t;
+            turning = true;
+        }
     }
 
     // ---- polar motion, and the disk at its equatorial crossings: inward of rt first; for a
     // turning ray past rt until the escape, whose time is needed only then or for the sky
     Polar<R> P{};
     if (eta > 0.0) P = polar_motion<R>(ry, A);
-    R I_inf = R(-1);
     if (cross) {
         const R ds = R(2) * P.K * qrcp(R(P.om));
         const R ipref = qrcp(pref);
-        R s = P.s1, s_end = S_t;
+        R s = P.s1, s_end = I_inf >= R(0) ? S_t + I_inf : S_t;
         int j = 0;
         for (; j < kMaxCrossings; ++j) {
             if (!(s < s_end)) {
@@ -794,7 +853,7 @@ KTC_FN Outcome solve_t(const Ray& ry) {
     } else {
         double th, ph;
         if (I_inf < 0.0) I_inf = pref * ell_f2<double>(x_inf, c2_inf, kp * kp + k * c2_inf);
-        if (!escape_sky(ry, P, S_t + I_inf, r1, r3, r4, k, pref, x_o, x_inf, th, ph)) return out;
+        if (!escape_sky(ry, P, S_t + I_inf, real4, r1, r3, r4, zc, bc, k, pref, x_o, x_inf, th, ph)) return out;
         out.cls = 0;
         out.theta = th;
         out.phi = ph;
