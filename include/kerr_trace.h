@@ -143,8 +143,9 @@ typedef struct kt_trace_opts {
      * -- class, disk radius and g, escaping rays' sky angles -- with 0 steps and
      * KT_FLAG_CLOSED_FORM; the rays it declines (an escape through a polar vortical orbit or
      * through the pole, a turning point at the capture radius, a crossing within 1e-9 of a
-     * disk edge, outgoing rays whose radial potential has a complex pair of roots, launches
-     * with R or Theta negative at the camera, observer cameras inside the horizon, spins
+     * disk edge, outgoing rays whose radial potential has no real root or a complex pair
+     * too near the path for the escape's quadrature, launches with R or Theta negative at
+     * the camera, observer cameras inside the horizon, spins
      * past 1) go to the integrator at `precision`.  With use_surrogate, the network's
      * gate runs first and the closed form replaces the values of the rays it answered as
      * escaped or disk hits (re-queueing those it declines or classes otherwise).  Not with
